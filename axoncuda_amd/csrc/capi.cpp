@@ -1,0 +1,548 @@
+// capi.cpp - extern "C" boundary of libhgp (declared in include/hgp.h).
+//
+// Each wrapper validates its arguments, runs the stage on the context stream and turns
+// every hgp::Error (or std::exception) into a status code with the message kept in the
+// context; nothing here exits or aborts the process.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <unordered_map>
+
+#include "common.hpp"
+#include "host/driver.hpp"
+#include "host/model.hpp"
+#include "prim.hpp"
+#include "stages.hpp"
+
+struct hgp_ctx {
+  hgp::Ctx c;
+};
+
+namespace hgp {
+
+void* dmalloc(Ctx& c, uint64_t bytes) {
+  if (bytes == 0) return nullptr;
+  if (c.budget && c.in_use + bytes > c.budget)
+    throw Error(HGP_E_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget");
+  void* p = nullptr;
+  hipError_t e = hipMallocAsync(&p, bytes, c.stream);
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    throw Error(HGP_E_NOMEM, "hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  }
+  c.in_use += bytes;
+  c.peak = std::max(c.peak, c.in_use);
+  c.live.push_back({p, bytes});
+  return p;
+}
+
+void dfree(Ctx& c, void* p) {
+  if (!p) return;
+  for (size_t i = c.live.size(); i-- > 0;) {
+    if (c.live[i].first == p) {
+      c.in_use -= c.live[i].second;
+      c.live[i] = c.live.back();
+      c.live.pop_back();
+      break;
+    }
+  }
+  (void)hipFreeAsync(p, c.stream);
+}
+
+}  // namespace hgp
+
+using hgp::Ctx;
+
+namespace {
+
+template <typename F>
+hgp_status guard(hgp_ctx* ctx, F&& f) {
+  if (!ctx) return HGP_E_INVALID;
+  try {
+    f(ctx->c);
+    return HGP_OK;
+  } catch (const hgp::Error& e) {
+    ctx->c.last_error = e.what();
+    if (ctx->c.verbose) fprintf(stderr, "hgp error: %s\n", e.what());
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    ctx->c.last_error = "host allocation failed";
+    return HGP_E_NOMEM;
+  } catch (const std::exception& e) {
+    ctx->c.last_error = e.what();
+    return HGP_E_INTERNAL;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void hgp_default_params(hgp_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->mode = HGP_MODE_INCC;
+  p->max_nodes_per_part = 1024;  // loihi64 (sources/constr.cpp:195-202)
+  p->max_inbound_per_part = 4096;
+  p->max_parts = 4096;
+  p->kway = 0;
+  p->epsilon = 0.0f;
+  p->candidates_count = 16;          // MAX_CANDIDATES (headers/defines.cuh:49)
+  p->refine_repeats = 16;            // REFINE_REPEATS (:54)
+  p->save_memory_up_to_level = 2;    // SAVE_MEMORY_UP_TO_LEVEL (:31)
+  p->oversized_multiplier = 1.5f;    // OVERSIZED_SIZE_MULTIPLIER (:26)
+  p->device_touching_construction = 0;
+  p->initial_partitions_merge = 0;
+  p->verbose = 0;
+}
+
+hgp_status hgp_ctx_create(int device, uint64_t hbm_budget, hgp_ctx** out) {
+  if (!out) return HGP_E_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return HGP_E_NODEVICE;
+  }
+  if (device < 0 || device >= count) return HGP_E_NODEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HGP_E_NODEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return HGP_E_NODEVICE;
+  hgp_ctx* ctx = new hgp_ctx();
+  ctx->c.device = device;
+  ctx->c.budget = hbm_budget;
+  ctx->c.num_cus = (uint32_t)prop.multiProcessorCount;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return HGP_E_HIP;
+  }
+  ctx->c.own_stream = true;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+    uint64_t thr = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  }
+  *out = ctx;
+  return HGP_OK;
+}
+
+void hgp_ctx_destroy(hgp_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipStreamSynchronize(ctx->c.stream);
+  for (auto& kv : ctx->c.live) (void)hipFreeAsync(kv.first, ctx->c.stream);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+const char* hgp_ctx_last_error(const hgp_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+hgp_status hgp_ctx_set_stream(hgp_ctx* ctx, void* stream) {
+  return guard(ctx, [&](Ctx& c) {
+    HGP_HIP(hipStreamSynchronize(c.stream));
+    if (c.own_stream) HGP_HIP(hipStreamDestroy(c.stream));
+    c.stream = (hipStream_t)stream;
+    c.own_stream = false;
+  });
+}
+
+hgp_status hgp_synchronize(hgp_ctx* ctx) { return guard(ctx, [&](Ctx& c) { hgp::sync(c); }); }
+
+hgp_status hgp_malloc(hgp_ctx* ctx, uint64_t bytes, void** d_ptr) {
+  if (!d_ptr) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    *d_ptr = hgp::dmalloc(c, bytes);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_free(hgp_ctx* ctx, void* d_ptr) {
+  return guard(ctx, [&](Ctx& c) {
+    hgp::dfree(c, d_ptr);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_memcpy_htod(hgp_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes) {
+  return guard(ctx, [&](Ctx& c) {
+    hgp::copy_h2d(c, d_dst, h_src, bytes);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_memcpy_dtoh(hgp_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes) {
+  return guard(ctx, [&](Ctx& c) { hgp::copy_d2h(c, h_dst, d_src, bytes); });
+}
+
+// ---- host model ---------------------------------------------------------------------------
+hgp_status hgp_hgraph_load(const char* path, int dedup, hgp_hgraph** out) {
+  if (!path || !out) return HGP_E_INVALID;
+  *out = nullptr;
+  try {
+    hgp_hgraph* hg = new hgp_hgraph();
+    std::string err = hgp::load_hgraph(path, *hg);
+    if (!err.empty()) {
+      fprintf(stderr, "Error loading file: %s\n", err.c_str());
+      delete hg;
+      return HGP_E_IO;
+    }
+    if (dedup) hgp::dedup_rule2(*hg);
+    *out = hg;
+    return HGP_OK;
+  } catch (...) {
+    return HGP_E_NOMEM;
+  }
+}
+
+hgp_status hgp_hgraph_from_csr(uint32_t num_nodes, uint32_t num_hedges, const uint64_t* h_offsets, const uint32_t* h_pins,
+                               const uint32_t* h_srcs_count, const float* h_weights, int dedup, hgp_hgraph** out) {
+  if (!out || !h_offsets || (num_hedges && (!h_pins && h_offsets[num_hedges] > 0))) return HGP_E_INVALID;
+  *out = nullptr;
+  try {
+    hgp_hgraph* hg = new hgp_hgraph();
+    hg->n = num_nodes;
+    hg->off.assign(h_offsets, h_offsets + num_hedges + 1);
+    const uint64_t p = h_offsets[num_hedges] - h_offsets[0];
+    hg->pins.assign(h_pins + h_offsets[0], h_pins + h_offsets[0] + p);
+    for (auto& o : hg->off) o -= h_offsets[0];
+    if (h_srcs_count) hg->srcc.assign(h_srcs_count, h_srcs_count + num_hedges);
+    else hg->srcc.assign(num_hedges, 0);
+    if (h_weights) hg->w.assign(h_weights, h_weights + num_hedges);
+    else hg->w.assign(num_hedges, 1.0f);
+    for (uint32_t v : hg->pins)
+      if (v >= num_nodes) {
+        delete hg;
+        return HGP_E_INVALID;
+      }
+    for (uint32_t i = 0; i < num_hedges; ++i)
+      if (hg->off[i + 1] < hg->off[i] || hg->srcc[i] > hg->off[i + 1] - hg->off[i]) {
+        delete hg;
+        return HGP_E_INVALID;
+      }
+    if (dedup) hgp::dedup_rule2(*hg);
+    *out = hg;
+    return HGP_OK;
+  } catch (...) {
+    return HGP_E_NOMEM;
+  }
+}
+
+void hgp_hgraph_free(hgp_hgraph* hg) { delete hg; }
+
+void hgp_hgraph_sizes(const hgp_hgraph* hg, uint32_t* n, uint32_t* e, uint64_t* p) {
+  if (!hg) return;
+  if (n) *n = hg->n;
+  if (e) *e = hg->e();
+  if (p) *p = hg->pins.size();
+}
+
+hgp_status hgp_hgraph_copy_out(const hgp_hgraph* hg, uint64_t* off, uint32_t* pins, uint32_t* srcc, float* w) {
+  if (!hg) return HGP_E_INVALID;
+  if (off) std::memcpy(off, hg->off.data(), hg->off.size() * 8);
+  if (pins) std::memcpy(pins, hg->pins.data(), hg->pins.size() * 4);
+  if (srcc) std::memcpy(srcc, hg->srcc.data(), hg->srcc.size() * 4);
+  if (w) std::memcpy(w, hg->w.data(), hg->w.size() * 4);
+  return HGP_OK;
+}
+
+hgp_status hgp_save_partitioning(const char* path, const uint32_t* part, uint32_t n) {
+  if (!path || (!part && n)) return HGP_E_INVALID;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return HGP_E_IO;
+  std::string buf;
+  buf.reserve((size_t)n * 4);
+  char tmp[16];
+  for (uint32_t v = 0; v < n; ++v) {
+    int l = std::snprintf(tmp, sizeof tmp, "%u\n", part[v]);
+    buf.append(tmp, l);
+  }
+  std::fwrite(buf.data(), 1, buf.size(), f);
+  std::fclose(f);
+  return HGP_OK;
+}
+
+hgp_status hgp_save_partitioned(const hgp_hgraph* hg, const uint32_t* part, const char* path) {
+  if (!hg || !part || !path) return HGP_E_INVALID;
+  std::vector<uint32_t> p(part, part + hg->n);
+  hgp_hgraph q = hgp::partitions_hypergraph(*hg, p);
+  return hgp::save_hgraph(q, path).empty() ? HGP_OK : HGP_E_IO;
+}
+
+hgp_status hgp_hgraph_save(const hgp_hgraph* hg, const char* path) {
+  if (!hg || !path) return HGP_E_INVALID;
+  return hgp::save_hgraph(*hg, path).empty() ? HGP_OK : HGP_E_IO;
+}
+
+hgp_status hgp_hgraph_metrics(const hgp_hgraph* hg, const uint32_t* part, double* km1, double* cutnet, double* soed) {
+  if (!hg || !part) return HGP_E_INVALID;
+  std::vector<uint32_t> p(part, part + hg->n);
+  hgp::Metrics m = hgp::metrics(*hg, p);
+  if (km1) *km1 = m.connectivity;
+  if (cutnet) *cutnet = m.cutnet;
+  if (soed) *soed = m.soed;
+  return HGP_OK;
+}
+
+hgp_status hgp_check_validity(const hgp_hgraph* hg, const uint32_t* part, uint32_t omega, uint32_t delta,
+                              uint32_t max_parts, int32_t* valid) {
+  if (!hg || !part || !valid) return HGP_E_INVALID;
+  std::vector<uint32_t> p(part, part + hg->n);
+  *valid = hgp::validity(*hg, p, omega, delta, max_parts, nullptr);
+  return HGP_OK;
+}
+
+hgp_status hgp_gen_random(uint32_t n, uint32_t e, uint32_t max_size, uint64_t seed, hgp_hgraph** out) {
+  if (!out || n < 2 || max_size < 2) return HGP_E_INVALID;
+  hgp_hgraph* hg = new hgp_hgraph();
+  hgp::gen_random(n, e, max_size, seed, *hg);
+  *out = hg;
+  return HGP_OK;
+}
+
+hgp_status hgp_gen_netlist(uint32_t n, uint32_t e, double alpha, uint32_t max_size, uint32_t radius, double frac_local,
+                           uint64_t seed, hgp_hgraph** out) {
+  if (!out || n < 2 || max_size < 2) return HGP_E_INVALID;
+  hgp_hgraph* hg = new hgp_hgraph();
+  hgp::gen_netlist(n, e, alpha, max_size, radius, frac_local, seed, *hg);
+  *out = hg;
+  return HGP_OK;
+}
+
+hgp_status hgp_gen_snn(uint32_t layers, uint32_t per_layer, uint32_t grid, uint32_t fanout, double sigma, uint64_t seed,
+                       hgp_hgraph** out) {
+  if (!out || layers < 2 || per_layer == 0 || (uint64_t)grid * grid < per_layer || fanout >= per_layer) return HGP_E_INVALID;
+  hgp_hgraph* hg = new hgp_hgraph();
+  hgp::gen_snn(layers, per_layer, grid, fanout, sigma, seed, *hg);
+  *out = hg;
+  return HGP_OK;
+}
+
+// ---- stages -------------------------------------------------------------------------------
+hgp_status hgp_build_touching(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                              const uint32_t* d_srcs_count, uint32_t num_nodes, uint32_t num_hedges,
+                              uint64_t* touching_size, uint32_t** d_touching, uint64_t** d_touching_offsets,
+                              uint32_t** d_inbound_count) {
+  if (!touching_size || !d_touching || !d_touching_offsets || !d_inbound_count) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::build_touching(c, d_hedges, d_hedges_offsets, d_srcs_count, num_nodes, num_hedges, *touching_size, *d_touching,
+                        *d_touching_offsets, *d_inbound_count);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_build_neighbors(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                               const uint32_t* d_touching, const uint64_t* d_touching_offsets, uint32_t num_nodes,
+                               uint64_t* neighbors_size, uint32_t** d_neighbors, uint64_t** d_neighbors_offsets) {
+  if (!neighbors_size || !d_neighbors || !d_neighbors_offsets) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::build_neighbors(c, d_hedges, d_hedges_offsets, d_touching, d_touching_offsets, num_nodes, *neighbors_size,
+                         *d_neighbors, *d_neighbors_offsets);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_candidates(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                          const uint32_t* d_srcs_count, const uint32_t* d_touching, const uint64_t* d_touching_offsets,
+                          const uint32_t* d_inbound_count, const float* d_hedge_weights, const uint32_t* d_nodes_sizes,
+                          uint32_t num_nodes, uint32_t num_hedges, uint32_t candidates_count, uint32_t max_nodes_per_part,
+                          uint32_t max_inbound_per_part, uint32_t* d_pairs, uint32_t* d_scores) {
+  if (!d_pairs || !d_scores || candidates_count < 1 || candidates_count > 16) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::candidates(c, d_hedges, d_hedges_offsets, d_srcs_count, d_touching, d_touching_offsets, d_inbound_count,
+                    d_hedge_weights, d_nodes_sizes, num_nodes, num_hedges, candidates_count, max_nodes_per_part,
+                    max_inbound_per_part, d_pairs, d_scores);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_group_nodes(hgp_ctx* ctx, const uint32_t* d_inbound_count, const uint32_t* d_pairs,
+                           const uint32_t* d_scores, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                           uint32_t candidates_count, uint32_t max_nodes_per_part, uint32_t max_inbound_per_part,
+                           uint32_t* new_num_nodes, uint32_t** d_groups, uint32_t** d_groups_sizes, uint32_t** d_ungroups,
+                           uint64_t** d_ungroups_offsets) {
+  if (!new_num_nodes || !d_groups || !d_groups_sizes || candidates_count < 1 || candidates_count > 16)
+    return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::group_nodes(c, d_inbound_count, d_pairs, d_scores, d_nodes_sizes, num_nodes, candidates_count,
+                     max_nodes_per_part, max_inbound_per_part, *new_num_nodes, *d_groups, *d_groups_sizes, d_ungroups,
+                     d_ungroups_offsets);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_coarsen_neighbors(hgp_ctx* ctx, const uint32_t* d_neighbors, const uint64_t* d_neighbors_offsets,
+                                 const uint32_t* d_groups, const uint32_t* d_ungroups, const uint64_t* d_ungroups_offsets,
+                                 uint32_t num_nodes, uint32_t new_num_nodes, uint64_t* new_neighbors_size,
+                                 uint32_t** d_coarse_neighbors, uint64_t** d_coarse_neighbors_offsets) {
+  (void)d_ungroups;
+  (void)d_ungroups_offsets;
+  if (!new_neighbors_size || !d_coarse_neighbors || !d_coarse_neighbors_offsets) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::coarsen_neighbors(c, d_neighbors, d_neighbors_offsets, d_groups, num_nodes, new_num_nodes, *new_neighbors_size,
+                           *d_coarse_neighbors, *d_coarse_neighbors_offsets);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_coarsen_hedges(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                              const uint32_t* d_srcs_count, const uint32_t* d_groups, uint32_t num_hedges,
+                              uint64_t hedges_size, uint64_t* new_hedges_size, uint32_t** d_coarse_hedges,
+                              uint64_t** d_coarse_hedges_offsets, uint32_t** d_coarse_srcs_count) {
+  if (!new_hedges_size || !d_coarse_hedges || !d_coarse_hedges_offsets || !d_coarse_srcs_count) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    // new_n = 0: the coarse node count is derived on the device from the groups of the pins
+    hgp::coarsen_hedges(c, d_hedges, d_hedges_offsets, d_srcs_count, d_groups, num_hedges, hedges_size, 0,
+                        *new_hedges_size, *d_coarse_hedges, *d_coarse_hedges_offsets, *d_coarse_srcs_count);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_coarsen_touching(hgp_ctx* ctx, const uint32_t* d_coarse_hedges, const uint64_t* d_coarse_hedges_offsets,
+                                const uint32_t* d_coarse_srcs_count, uint32_t new_num_nodes, uint32_t num_hedges,
+                                uint64_t* new_touching_size, uint32_t** d_coarse_touching,
+                                uint64_t** d_coarse_touching_offsets, uint32_t** d_coarse_inbound_count) {
+  if (!new_touching_size || !d_coarse_touching || !d_coarse_touching_offsets || !d_coarse_inbound_count)
+    return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::build_touching(c, d_coarse_hedges, d_coarse_hedges_offsets, d_coarse_srcs_count, new_num_nodes, num_hedges,
+                        *new_touching_size, *d_coarse_touching, *d_coarse_touching_offsets, *d_coarse_inbound_count);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_uncoarsen(hgp_ctx* ctx, const uint32_t* d_groups, const uint32_t* d_coarse_partitions, uint32_t num_nodes,
+                         uint32_t* d_partitions) {
+  return guard(ctx, [&](Ctx& c) {
+    hgp::uncoarsen(c, d_groups, d_coarse_partitions, num_nodes, d_partitions);
+    hgp::sync(c);
+  });
+}
+
+static hgp::RefineLevel make_level(uint32_t n, uint32_t e, uint32_t k, const uint32_t* hedges, const uint64_t* hoff,
+                                   const uint32_t* srcc, const uint32_t* tch, const uint64_t* toff, const uint32_t* inb,
+                                   const float* w, const uint32_t* sizes) {
+  hgp::RefineLevel L{n, e, k, hedges, hoff, srcc, tch, toff, inb, w, sizes};
+  return L;
+}
+
+hgp_status hgp_refine(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                      const uint32_t* d_srcs_count, const uint32_t* d_touching, const uint64_t* d_touching_offsets,
+                      const uint32_t* d_inbound_count, const float* d_hedge_weights, const uint32_t* d_nodes_sizes,
+                      uint32_t level_idx, uint32_t num_nodes, uint32_t num_hedges, uint32_t num_partitions,
+                      int32_t update_final_inbound_counts, uint32_t* d_partitions, uint32_t* d_partitions_sizes,
+                      uint32_t* d_partitions_inbound_sizes) {
+  (void)level_idx;
+  if (!params || !d_partitions || !d_partitions_sizes || num_partitions == 0) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::RefineLevel L = make_level(num_nodes, num_hedges, num_partitions, d_hedges, d_hedges_offsets, d_srcs_count,
+                                    d_touching, d_touching_offsets, d_inbound_count, d_hedge_weights, d_nodes_sizes);
+    hgp::refine(c, L, params->mode, params->max_nodes_per_part, params->max_inbound_per_part, params->refine_repeats,
+                update_final_inbound_counts != 0, d_partitions, d_partitions_sizes, d_partitions_inbound_sizes);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_refine_analyze(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_hedges,
+                              const uint64_t* d_hedges_offsets, const uint32_t* d_srcs_count, const uint32_t* d_touching,
+                              const uint64_t* d_touching_offsets, const uint32_t* d_inbound_count,
+                              const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                              uint32_t num_hedges, uint32_t num_partitions, uint32_t fm_repeat, int32_t chainup,
+                              const uint32_t* d_partitions, const uint32_t* d_partitions_sizes, uint32_t* h_moves,
+                              int64_t* h_gains, uint32_t* h_ranks, int64_t* h_seq_gains, int32_t* h_size_viol,
+                              int32_t* h_inbound_viol, uint32_t* h_best_rank) {
+  if (!params || num_partitions == 0) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::RefineLevel L = make_level(num_nodes, num_hedges, num_partitions, d_hedges, d_hedges_offsets, d_srcs_count,
+                                    d_touching, d_touching_offsets, d_inbound_count, d_hedge_weights, d_nodes_sizes);
+    hgp::AnalyzeOut out;
+    out.moves = h_moves;
+    out.gains = h_gains;
+    out.ranks = h_ranks;
+    out.seq = h_seq_gains;
+    out.sviol = h_size_viol;
+    out.iviol = h_inbound_viol;
+    const uint32_t b = hgp::refine_analyze(c, L, params->mode, params->max_nodes_per_part, params->max_inbound_per_part,
+                                           params->refine_repeats, fm_repeat, chainup != 0, d_partitions,
+                                           d_partitions_sizes, out);
+    if (h_best_rank) *h_best_rank = b;
+  });
+}
+
+hgp_status hgp_greedy_merge_groups(hgp_ctx* ctx, const uint32_t* d_nodes_sizes, const uint32_t* d_inbound_count,
+                                   uint32_t num_nodes, uint32_t num_groups, uint32_t max_nodes_per_part,
+                                   uint32_t max_inbound_per_part, uint32_t* d_groups, uint32_t* d_groups_sizes,
+                                   uint32_t* new_num_groups) {
+  if (!new_num_groups) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::greedy_merge_groups(c, d_nodes_sizes, d_inbound_count, num_nodes, num_groups, max_nodes_per_part,
+                             max_inbound_per_part, d_groups, d_groups_sizes, *new_num_groups);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_merge_small_partitions(hgp_ctx* ctx, const uint32_t* d_partitions_sizes,
+                                      const uint32_t* d_partitions_inbound_sizes, uint32_t num_nodes,
+                                      uint32_t num_partitions, uint32_t max_nodes_per_part, uint32_t max_inbound_per_part,
+                                      uint32_t* d_partitions) {
+  return guard(ctx, [&](Ctx& c) {
+    hgp::merge_small_partitions(c, d_partitions_sizes, d_partitions_inbound_sizes, num_nodes, num_partitions,
+                                max_nodes_per_part, max_inbound_per_part, d_partitions);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_relabel_partitions(hgp_ctx* ctx, uint32_t num_nodes, uint32_t* d_partitions, uint32_t* new_num_partitions) {
+  if (!new_num_partitions) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::relabel(c, num_nodes, d_partitions, *new_num_partitions);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_inbound_sizes(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                             const uint32_t* d_srcs_count, const uint32_t* d_partitions, uint32_t num_hedges,
+                             uint32_t num_partitions, uint32_t* d_out) {
+  return guard(ctx, [&](Ctx& c) {
+    hgp::inbound_sizes(c, d_hedges, d_hedges_offsets, d_srcs_count, d_partitions, num_hedges, num_partitions, d_out);
+    hgp::sync(c);
+  });
+}
+
+hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                            const float* d_hedge_weights, const uint32_t* d_partitions, uint32_t num_hedges,
+                            int64_t* km1_fixed) {
+  if (!km1_fixed) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    *km1_fixed = hgp::connectivity_fixed(c, d_hedges, d_hedges_offsets, d_hedge_weights, d_partitions, num_hedges);
+  });
+}
+
+hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                                 const float* d_hedge_weights, uint32_t num_nodes, uint32_t num_hedges,
+                                 uint64_t hedges_size, const uint32_t* d_nodes_sizes, uint32_t k, float epsilon,
+                                 uint32_t max_nodes_per_part, uint32_t** d_partitions, uint32_t** d_partitions_sizes) {
+  (void)hedges_size;
+  if (!d_partitions || !d_partitions_sizes || k == 0) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::initial_partition(c, d_hedges, d_hedges_offsets, d_hedge_weights, num_nodes, num_hedges, d_nodes_sizes, k,
+                           epsilon, max_nodes_per_part, *d_partitions, *d_partitions_sizes, nullptr);
+  });
+}
+
+hgp_status hgp_partition(hgp_ctx* ctx, const hgp_params* params, const hgp_hgraph* hg, uint32_t* h_partitions_out,
+                         hgp_run_stats* stats) {
+  if (!params || !hg || !h_partitions_out) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    const int saved = c.verbose;
+    c.verbose = params->verbose;
+    try {
+      hgp::partition(c, *params, *hg, h_partitions_out, stats);
+    } catch (...) {
+      c.verbose = saved;
+      throw;
+    }
+    c.verbose = saved;
+  });
+}
+
+}  // extern "C"

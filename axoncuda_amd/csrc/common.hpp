@@ -1,0 +1,230 @@
+// common.hpp - runtime plumbing shared by every HIP translation unit of libhgp.
+//
+// * Errors: device/runtime failures throw hgp::Error(status, msg); the C ABI layer
+//   (capi.cpp) catches them and returns the status code, so no path exits the process.
+// * Memory: stream-ordered allocations (hipMallocAsync on the context stream, pool kept
+//   warm) with an HBM budget that every allocation is charged against.
+// * Wave64 helpers for gfx950 (CDNA4): lane ids, ballots, shuffles, reductions and scans.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hgp.h"
+
+namespace hgp {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t WAVE = 64;
+constexpr uint32_t FIXED_POINT_SCALE = 262144u;  // 2^18 (reference headers/defines.cuh:24)
+
+struct Error : std::runtime_error {
+  hgp_status status;
+  Error(hgp_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define HGP_HIP(expr)                                                                             \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess)                                                                         \
+      throw ::hgp::Error(HGP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e) + " at " + \
+                                        __FILE__ + ":" + std::to_string(__LINE__));               \
+  } while (0)
+
+#define HGP_REQUIRE(cond, status, msg)                      \
+  do {                                                      \
+    if (!(cond)) throw ::hgp::Error((status), (msg));       \
+  } while (0)
+
+// Execution context: device, stream, HBM budget accounting, last error text.
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint64_t budget = 0;  // 0 = whole device
+  uint64_t in_use = 0;
+  uint64_t peak = 0;
+  std::vector<std::pair<void*, uint64_t>> live;  // allocation sizes for accounting
+  std::string last_error;
+  int verbose = 0;
+  uint32_t num_cus = 256;
+};
+
+void* dmalloc(Ctx& c, uint64_t bytes);
+void dfree(Ctx& c, void* p);
+
+// RAII device buffer tied to a context.
+template <typename T>
+struct DBuf {
+  Ctx* ctx = nullptr;
+  T* ptr = nullptr;
+  uint64_t n = 0;
+  DBuf() = default;
+  DBuf(Ctx& c, uint64_t count) : ctx(&c), n(count) {
+    ptr = count ? (T*)dmalloc(c, count * sizeof(T)) : nullptr;
+  }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : ctx(o.ctx), ptr(o.ptr), n(o.n) {
+    o.ptr = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    reset();
+    ctx = o.ctx;
+    ptr = o.ptr;
+    n = o.n;
+    o.ptr = nullptr;
+    o.n = 0;
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  void reset() {
+    if (ptr && ctx) dfree(*ctx, ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  T* release() {
+    T* p = ptr;
+    ptr = nullptr;
+    n = 0;
+    return p;
+  }
+  T* get() const { return ptr; }
+  operator T*() const { return ptr; }
+};
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+inline void memset_async(Ctx& c, void* p, int v, uint64_t bytes) {
+  if (bytes) HGP_HIP(hipMemsetAsync(p, v, bytes, c.stream));
+}
+inline void copy_d2d(Ctx& c, void* dst, const void* src, uint64_t bytes) {
+  if (bytes) HGP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+}
+inline void copy_h2d(Ctx& c, void* dst, const void* src, uint64_t bytes) {
+  if (bytes) HGP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+}
+inline void copy_d2h(Ctx& c, void* dst, const void* src, uint64_t bytes) {
+  if (bytes) {
+    HGP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    HGP_HIP(hipStreamSynchronize(c.stream));
+  }
+}
+template <typename T>
+inline T read_scalar(Ctx& c, const T* d) {
+  T v;
+  copy_d2h(c, &v, d, sizeof(T));
+  return v;
+}
+inline void sync(Ctx& c) { HGP_HIP(hipStreamSynchronize(c.stream)); }
+
+#define HGP_LAUNCH(kernel, grid, block, shmem, ctx, ...)                                    \
+  do {                                                                                      \
+    if ((grid) > 0) {                                                                       \
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, (ctx).stream, __VA_ARGS__); \
+      HGP_HIP(hipGetLastError());                                                           \
+    }                                                                                       \
+  } while (0)
+
+// ---------------------------------------------------------------------------------------
+// device helpers (wave64)
+#if defined(__HIPCC__)
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// number of set bits of `mask` strictly below this lane
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T x = __shfl_xor(v, o, 64);
+    v = x > v ? x : v;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T x = __shfl_xor(v, o, 64);
+    v = x < v ? x : v;
+  }
+  return v;
+}
+
+// inclusive prefix sum across the 64 lanes
+template <typename T>
+__device__ __forceinline__ T wave_scan_incl(T v) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T x = __shfl_up(v, o, 64);
+    if (l >= (uint32_t)o) v += x;
+  }
+  return v;
+}
+
+// 2^18 fixed point of a hedge weight, saturated to [0, 2^32 - 1]
+__device__ __forceinline__ uint32_t fixed_weight(float w) {
+  float v = w * 262144.0f;
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)v;
+}
+
+// Symmetric pairing noise in [0, 64): the mixing steps of the reference's
+// deterministic_noise<64> (headers/utils.cuh:159-171).
+__device__ __forceinline__ uint32_t noise64(uint32_t a, uint32_t b) {
+  uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  uint32_t x = lo * 0x9E3779B1u;
+  x ^= hi + 0x85EBCA6Bu + (x << 6) + (x >> 2);
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 13;
+  x *= 0x9E3779B1u;
+  x ^= x >> 16;
+  return x & 63u;
+}
+
+// Integer mixer used for gain tie-breaks: the steps of the reference's hash_uint32
+// (headers/utils.cuh:252-261).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 17;
+  x *= 0xED5AD4BBu;
+  x ^= x >> 11;
+  x *= 0xAC4C1B51u;
+  x ^= x >> 15;
+  x *= 0x31848BABu;
+  x ^= x >> 14;
+  return x;
+}
+
+__device__ __forceinline__ uint32_t slot_hash(uint32_t x) { return x * 0x9E3779B1u; }
+
+template <typename T>
+__device__ __forceinline__ T atomic_load_relaxed(T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+#endif
+
+}  // namespace hgp

@@ -1,0 +1,467 @@
+// ip_core.hpp - built-in k-way initial partitioner for the coarsest level (host, deterministic).
+//
+// The reference runs Mt-KaHyPar as an external process on the coarse hypergraph
+// (sources/init_part.cu:733-953: drop hedges with fewer than two pins, merge identical
+// hedges summing their 2^18 fixed-point weights, vertex weights = node sizes, -k k -e eps).
+// When no `mtkahypar` binary is available the pipeline uses this partitioner instead:
+// recursive bisection, each bisection = greedy hypergraph growing from several seeds
+// followed by boundary FM passes with rollback, keeping the best cut.  Everything is
+// integer and tie-broken by node id, so the result is a pure function of the input.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <unordered_map>
+#include <vector>
+
+namespace hgp_ip {
+
+struct Hypergraph {
+  uint32_t n = 0;
+  std::vector<uint64_t> off;   // hedge offsets
+  std::vector<uint32_t> pins;  // sorted distinct pins per hedge
+  std::vector<uint64_t> w;     // merged fixed-point weights
+  std::vector<uint64_t> vw;    // node weights
+  std::vector<uint64_t> ioff;  // incidence offsets
+  std::vector<uint32_t> inc;   // incident hedges
+};
+
+inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+inline void build_incidence(Hypergraph& h) {
+  h.ioff.assign(h.n + 1, 0);
+  for (uint32_t p : h.pins) h.ioff[p + 1]++;
+  for (uint32_t v = 0; v < h.n; ++v) h.ioff[v + 1] += h.ioff[v];
+  h.inc.resize(h.pins.size());
+  std::vector<uint64_t> pos(h.ioff.begin(), h.ioff.end() - 1);
+  for (uint32_t e = 0; e + 1 < h.off.size(); ++e)
+    for (uint64_t i = h.off[e]; i < h.off[e + 1]; ++i) h.inc[pos[h.pins[i]]++] = e;
+}
+
+// Coarse level -> IP input: hedges with >= 2 distinct pins, identical pin sets merged
+// (weights summed), in order of first appearance.
+inline Hypergraph from_level(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins,
+                             const uint32_t* wfx, const uint32_t* sizes) {
+  Hypergraph h;
+  h.n = n;
+  h.vw.assign(sizes, sizes + n);
+  h.off.push_back(0);
+  struct Key {
+    uint64_t hash;
+    uint64_t start;
+    uint32_t len;
+  };
+  std::unordered_map<uint64_t, std::vector<uint32_t>> buckets;  // hash -> hedge ids in h
+  std::vector<uint32_t> tmp;
+  for (uint32_t i = 0; i < e; ++i) {
+    tmp.assign(pins + hoff[i], pins + hoff[i + 1]);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    if (tmp.size() < 2) continue;
+    uint64_t hs = 1469598103934665603ull;
+    for (uint32_t p : tmp) hs = splitmix64(hs ^ p);
+    auto& b = buckets[hs];
+    bool merged = false;
+    for (uint32_t cand : b) {
+      uint64_t s = h.off[cand], l = h.off[cand + 1] - s;
+      if (l == tmp.size() && std::equal(tmp.begin(), tmp.end(), h.pins.begin() + s)) {
+        h.w[cand] += wfx[i];
+        merged = true;
+        break;
+      }
+    }
+    if (merged) continue;
+    b.push_back((uint32_t)h.w.size());
+    h.pins.insert(h.pins.end(), tmp.begin(), tmp.end());
+    h.off.push_back(h.pins.size());
+    h.w.push_back(wfx[i]);
+  }
+  build_incidence(h);
+  return h;
+}
+
+// Indexed binary max-heap keyed by (gain, -id).
+class Heap {
+ public:
+  void reset(uint32_t n) {
+    pos_.assign(n, UINT32_MAX);
+    heap_.clear();
+    key_.assign(n, 0);
+  }
+  bool contains(uint32_t v) const { return pos_[v] != UINT32_MAX; }
+  bool empty() const { return heap_.empty(); }
+  uint32_t top() const { return heap_[0]; }
+  int64_t key(uint32_t v) const { return key_[v]; }
+  void push(uint32_t v, int64_t k) {
+    key_[v] = k;
+    pos_[v] = (uint32_t)heap_.size();
+    heap_.push_back(v);
+    up(pos_[v]);
+  }
+  void update(uint32_t v, int64_t k) {
+    int64_t old = key_[v];
+    key_[v] = k;
+    if (better(k, v, old, v)) up(pos_[v]); else down(pos_[v]);
+  }
+  void remove(uint32_t v) {
+    uint32_t i = pos_[v];
+    uint32_t last = heap_.back();
+    heap_.pop_back();
+    pos_[v] = UINT32_MAX;
+    if (i < heap_.size()) {
+      heap_[i] = last;
+      pos_[last] = i;
+      up(i);
+      down(pos_[last]);
+    }
+  }
+
+ private:
+  static bool better(int64_t ka, uint32_t a, int64_t kb, uint32_t b) { return ka > kb || (ka == kb && a < b); }
+  void up(uint32_t i) {
+    while (i > 0) {
+      uint32_t p = (i - 1) / 2;
+      uint32_t a = heap_[i], b = heap_[p];
+      if (!better(key_[a], a, key_[b], b)) break;
+      heap_[i] = b; heap_[p] = a; pos_[b] = i; pos_[a] = p; i = p;
+    }
+  }
+  void down(uint32_t i) {
+    uint32_t n = (uint32_t)heap_.size();
+    for (;;) {
+      uint32_t l = 2 * i + 1, r = l + 1, m = i;
+      if (l < n && better(key_[heap_[l]], heap_[l], key_[heap_[m]], heap_[m])) m = l;
+      if (r < n && better(key_[heap_[r]], heap_[r], key_[heap_[m]], heap_[m])) m = r;
+      if (m == i) break;
+      uint32_t a = heap_[i], b = heap_[m];
+      heap_[i] = b; heap_[m] = a; pos_[b] = i; pos_[a] = m; i = m;
+    }
+  }
+  std::vector<uint32_t> heap_;
+  std::vector<uint32_t> pos_;
+  std::vector<int64_t> key_;
+};
+
+// Two-way partitioning of a sub-hypergraph (local ids 0..n-1).
+struct Bisector {
+  const Hypergraph& h;
+  std::vector<uint8_t> side;
+  std::vector<uint32_t> cnt[2];
+  std::vector<int64_t> gain;
+  Heap heap[2];
+  uint64_t wside[2] = {0, 0};
+
+  explicit Bisector(const Hypergraph& hg) : h(hg) {}
+
+  int64_t cut() const {
+    int64_t c = 0;
+    for (uint32_t e = 0; e + 1 < h.off.size(); ++e)
+      if (cnt[0][e] && cnt[1][e]) c += (int64_t)h.w[e];
+    return c;
+  }
+  void init_counts() {
+    uint32_t m = (uint32_t)h.off.size() - 1;
+    cnt[0].assign(m, 0);
+    cnt[1].assign(m, 0);
+    wside[0] = wside[1] = 0;
+    for (uint32_t v = 0; v < h.n; ++v) wside[side[v]] += h.vw[v];
+    for (uint32_t e = 0; e < m; ++e)
+      for (uint64_t i = h.off[e]; i < h.off[e + 1]; ++i) cnt[side[h.pins[i]]][e]++;
+  }
+  int64_t node_gain(uint32_t v) const {
+    uint8_t a = side[v], b = 1 - a;
+    int64_t g = 0;
+    for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
+      uint32_t e = h.inc[i];
+      if (cnt[a][e] == 1) g += (int64_t)h.w[e];
+      if (cnt[b][e] == 0) g -= (int64_t)h.w[e];
+    }
+    return g;
+  }
+  // Move v to the other side, updating the gains of free pins (classic FM delta rules).
+  void move(uint32_t v, const std::vector<uint8_t>& locked) {
+    uint8_t a = side[v], b = 1 - a;
+    for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
+      uint32_t e = h.inc[i];
+      int64_t we = (int64_t)h.w[e];
+      if (cnt[b][e] == 0) {
+        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+          uint32_t u = h.pins[j];
+          if (u != v && !locked[u]) bump(u, we);
+        }
+      } else if (cnt[b][e] == 1) {
+        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+          uint32_t u = h.pins[j];
+          if (u != v && side[u] == b && !locked[u]) bump(u, -we);
+        }
+      }
+      cnt[a][e]--;
+      cnt[b][e]++;
+      if (cnt[a][e] == 0) {
+        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+          uint32_t u = h.pins[j];
+          if (u != v && !locked[u]) bump(u, -we);
+        }
+      } else if (cnt[a][e] == 1) {
+        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+          uint32_t u = h.pins[j];
+          if (u != v && side[u] == a && !locked[u]) bump(u, we);
+        }
+      }
+    }
+    side[v] = b;
+    wside[a] -= h.vw[v];
+    wside[b] += h.vw[v];
+  }
+  void bump(uint32_t u, int64_t d) {
+    gain[u] += d;
+    uint8_t s = side[u];
+    if (heap[s].contains(u)) heap[s].update(u, gain[u]);
+  }
+
+  // Greedy growing: side 1 holds everything, side 0 grows from `seed` until it reaches target0.
+  void grow(uint32_t seed, uint64_t target0, uint64_t max0) {
+    side.assign(h.n, 1);
+    init_counts();
+    gain.assign(h.n, 0);
+    for (uint32_t v = 0; v < h.n; ++v) gain[v] = node_gain(v);
+    std::vector<uint8_t> locked(h.n, 0);
+    heap[0].reset(h.n);
+    heap[1].reset(h.n);
+    std::vector<uint8_t> inq(h.n, 0);
+    auto touch_nbrs = [&](uint32_t v) {
+      for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
+        uint32_t e = h.inc[i];
+        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+          uint32_t u = h.pins[j];
+          if (side[u] == 1 && !inq[u]) { inq[u] = 1; heap[1].push(u, gain[u]); }
+        }
+      }
+    };
+    uint32_t next_unvisited = 0;
+    uint32_t cur = seed;
+    while (wside[0] < target0) {
+      if (cur == UINT32_MAX) {
+        // pick the best frontier node that fits; fall back to the next unvisited node
+        while (!heap[1].empty()) {
+          uint32_t t = heap[1].top();
+          if (wside[0] + h.vw[t] <= max0) { cur = t; break; }
+          heap[1].remove(t);
+        }
+        if (cur == UINT32_MAX) {
+          while (next_unvisited < h.n && (side[next_unvisited] == 0 || inq[next_unvisited] ||
+                                          wside[0] + h.vw[next_unvisited] > max0))
+            next_unvisited++;
+          if (next_unvisited >= h.n) break;
+          cur = next_unvisited;
+        }
+      }
+      if (heap[1].contains(cur)) heap[1].remove(cur);
+      inq[cur] = 1;
+      move(cur, locked);
+      touch_nbrs(cur);
+      cur = UINT32_MAX;
+    }
+    heap[1].reset(h.n);
+  }
+
+  // One FM pass with rollback to the best prefix.  Returns the cut improvement.
+  int64_t fm_pass(const uint64_t maxw[2]) {
+    std::vector<uint8_t> locked(h.n, 0);
+    heap[0].reset(h.n);
+    heap[1].reset(h.n);
+    for (uint32_t v = 0; v < h.n; ++v) {
+      gain[v] = node_gain(v);
+      // boundary nodes only
+      bool boundary = false;
+      for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1] && !boundary; ++i) {
+        uint32_t e = h.inc[i];
+        boundary = cnt[0][e] && cnt[1][e];
+      }
+      if (boundary) heap[side[v]].push(v, gain[v]);
+    }
+    std::vector<uint32_t> moved;
+    int64_t cum = 0, best = 0;
+    size_t best_len = 0;
+    uint64_t best_imb = std::max(wside[0] > maxw[0] ? wside[0] - maxw[0] : 0,
+                                 wside[1] > maxw[1] ? wside[1] - maxw[1] : 0);
+    uint32_t stall = 0;
+    const uint32_t stall_limit = std::max<uint32_t>(50, h.n / 8);
+    while (!heap[0].empty() || !heap[1].empty()) {
+      // candidate from each side that respects the target's capacity
+      int pick = -1;
+      for (int s = 0; s < 2; ++s) {
+        while (!heap[s].empty()) {
+          uint32_t t = heap[s].top();
+          if (wside[1 - s] + h.vw[t] <= maxw[1 - s] || wside[s] > maxw[s]) break;
+          heap[s].remove(t);
+        }
+      }
+      int64_t g0 = heap[0].empty() ? INT64_MIN : gain[heap[0].top()];
+      int64_t g1 = heap[1].empty() ? INT64_MIN : gain[heap[1].top()];
+      if (g0 == INT64_MIN && g1 == INT64_MIN) break;
+      if (g0 > g1) pick = 0;
+      else if (g1 > g0) pick = 1;
+      else pick = wside[0] >= wside[1] ? 0 : 1;
+      uint32_t v = heap[pick].top();
+      heap[pick].remove(v);
+      locked[v] = 1;
+      cum += gain[v];
+      move(v, locked);
+      moved.push_back(v);
+      // newly boundary neighbours enter the queues
+      for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
+        uint32_t e = h.inc[i];
+        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+          uint32_t u = h.pins[j];
+          if (!locked[u] && !heap[side[u]].contains(u)) heap[side[u]].push(u, gain[u]);
+        }
+      }
+      uint64_t imb = std::max(wside[0] > maxw[0] ? wside[0] - maxw[0] : 0,
+                              wside[1] > maxw[1] ? wside[1] - maxw[1] : 0);
+      if (imb < best_imb || (imb == best_imb && cum > best)) {
+        best = cum;
+        best_len = moved.size();
+        best_imb = imb;
+        stall = 0;
+      } else if (++stall > stall_limit) {
+        break;
+      }
+    }
+    std::vector<uint8_t> none(h.n, 1);  // no gain maintenance during rollback
+    for (size_t i = moved.size(); i > best_len; --i) move(moved[i - 1], none);
+    return best;
+  }
+};
+
+// Restrict h to `nodes` (sorted global ids); hedges keep their pins inside the subset.
+inline Hypergraph subgraph(const Hypergraph& h, const std::vector<uint32_t>& nodes) {
+  Hypergraph s;
+  s.n = (uint32_t)nodes.size();
+  std::vector<uint32_t> local(h.n, UINT32_MAX);
+  for (uint32_t i = 0; i < s.n; ++i) local[nodes[i]] = i;
+  s.vw.resize(s.n);
+  for (uint32_t i = 0; i < s.n; ++i) s.vw[i] = h.vw[nodes[i]];
+  s.off.push_back(0);
+  std::vector<uint8_t> seen(h.off.size() - 1, 0);
+  for (uint32_t gi : nodes) {
+    for (uint64_t k = h.ioff[gi]; k < h.ioff[gi + 1]; ++k) {
+      uint32_t e = h.inc[k];
+      if (seen[e]) continue;
+      seen[e] = 1;
+      uint64_t start = s.pins.size();
+      for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j)
+        if (local[h.pins[j]] != UINT32_MAX) s.pins.push_back(local[h.pins[j]]);
+      if (s.pins.size() - start < 2) { s.pins.resize(start); continue; }
+      std::sort(s.pins.begin() + start, s.pins.end());
+      s.off.push_back(s.pins.size());
+      s.w.push_back(h.w[e]);
+    }
+  }
+  build_incidence(s);
+  return s;
+}
+
+struct Options {
+  uint32_t tries = 4;
+  uint32_t fm_passes = 3;
+  uint64_t seed = 86;
+};
+
+inline void bisect_recursive(const Hypergraph& h, const std::vector<uint32_t>& gids, uint32_t k, uint32_t first,
+                             double eps_b, uint64_t omega, const Options& opt, uint32_t* part, uint32_t depth) {
+  if (k == 1 || h.n == 0) {
+    for (uint32_t g : gids) part[g] = first;
+    return;
+  }
+  uint32_t k0 = k / 2, k1 = k - k0;
+  uint64_t total = 0;
+  for (uint64_t x : h.vw) total += x;
+  uint64_t target0 = (uint64_t)((long double)total * k0 / k);
+  uint64_t maxw[2];
+  maxw[0] = std::min<uint64_t>((uint64_t)omega * k0, (uint64_t)((long double)total * k0 / k * (1.0 + eps_b)));
+  maxw[1] = std::min<uint64_t>((uint64_t)omega * k1, (uint64_t)((long double)total * k1 / k * (1.0 + eps_b)));
+  if (maxw[0] + maxw[1] < total) {  // keep the two capacities jointly feasible
+    maxw[0] = std::min<uint64_t>((uint64_t)omega * k0, total);
+    maxw[1] = std::min<uint64_t>((uint64_t)omega * k1, total);
+  }
+  std::vector<uint8_t> best_side;
+  int64_t best_cut = INT64_MAX;
+  uint64_t best_imb = UINT64_MAX;
+  Bisector b(h);
+  for (uint32_t t = 0; t < opt.tries; ++t) {
+    uint32_t seed = (uint32_t)(splitmix64(opt.seed * 1315423911ull + depth * 2654435761ull + first * 97ull + t) % h.n);
+    b.grow(seed, target0, maxw[0]);
+    for (uint32_t p = 0; p < opt.fm_passes; ++p)
+      if (b.fm_pass(maxw) <= 0 && p > 0) break;
+    int64_t c = b.cut();
+    uint64_t imb = std::max(b.wside[0] > maxw[0] ? b.wside[0] - maxw[0] : 0,
+                            b.wside[1] > maxw[1] ? b.wside[1] - maxw[1] : 0);
+    if (imb < best_imb || (imb == best_imb && c < best_cut)) {
+      best_cut = c;
+      best_imb = imb;
+      best_side = b.side;
+    }
+  }
+  std::vector<uint32_t> l[2], g[2];
+  for (uint32_t v = 0; v < h.n; ++v) {
+    l[best_side[v]].push_back(v);
+    g[best_side[v]].push_back(gids[v]);
+  }
+  for (int s = 0; s < 2; ++s) {
+    Hypergraph sub = subgraph(h, l[s]);
+    bisect_recursive(sub, g[s], s == 0 ? k0 : k1, s == 0 ? first : first + k0, eps_b, omega, opt, part, depth + 1);
+  }
+}
+
+// Moves nodes out of overweight parts into the lightest parts that can take them
+// (largest-first, ties by id) so that every part respects omega when possible.
+inline void repair_balance(const Hypergraph& h, uint32_t k, uint64_t omega, uint32_t* part) {
+  std::vector<uint64_t> pw(k, 0);
+  for (uint32_t v = 0; v < h.n; ++v) pw[part[v]] += h.vw[v];
+  for (uint32_t p = 0; p < k; ++p) {
+    if (pw[p] <= omega) continue;
+    std::vector<uint32_t> members;
+    for (uint32_t v = 0; v < h.n; ++v)
+      if (part[v] == p) members.push_back(v);
+    std::sort(members.begin(), members.end(), [&](uint32_t a, uint32_t b) {
+      return h.vw[a] != h.vw[b] ? h.vw[a] < h.vw[b] : a < b;
+    });
+    for (uint32_t v : members) {
+      if (pw[p] <= omega) break;
+      uint32_t dest = UINT32_MAX;
+      for (uint32_t q = 0; q < k; ++q)
+        if (q != p && pw[q] + h.vw[v] <= omega && (dest == UINT32_MAX || pw[q] < pw[dest])) dest = q;
+      if (dest == UINT32_MAX) continue;
+      pw[p] -= h.vw[v];
+      pw[dest] += h.vw[v];
+      part[v] = dest;
+    }
+  }
+}
+
+// Entry point: part[n] in [0,k), psizes[k] = summed node weights.
+inline void partition(const Hypergraph& h, uint32_t k, float eps, uint64_t omega, uint32_t* part, uint32_t* psizes,
+                      const Options& opt = Options()) {
+  std::vector<uint32_t> gids(h.n);
+  for (uint32_t v = 0; v < h.n; ++v) gids[v] = v;
+  uint32_t levels = 0;
+  while ((1u << levels) < k) levels++;
+  double eps_b = levels ? std::pow(1.0 + (double)eps, 1.0 / levels) - 1.0 : (double)eps;
+  if (k <= 1) {
+    for (uint32_t v = 0; v < h.n; ++v) part[v] = 0;
+  } else {
+    bisect_recursive(h, gids, k, 0, eps_b, omega, opt, part, 0);
+    repair_balance(h, k, omega, part);
+  }
+  for (uint32_t p = 0; p < k; ++p) psizes[p] = 0;
+  for (uint32_t v = 0; v < h.n; ++v) psizes[part[v]] += (uint32_t)h.vw[v];
+}
+
+}  // namespace hgp_ip

@@ -1,0 +1,559 @@
+// model.cpp - host loaders, savers, metrics, validity and seeded generators.
+//
+// Loaders follow the reference formats and rules (includes/hgraph.hpp): .hgr (:936-1013;
+// first non-comment line "E V [fmt]", 1-based pins, fmt 1/11 weights, vertex weights read
+// but not used, single-pin hedges skipped), .snn (:707-743), .axh (:805-854).  Files are
+// read whole and parsed without iostreams so that multi-GB inputs load at disk speed.
+#include "model.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <unordered_map>
+#include <unordered_set>
+
+namespace hgp {
+
+namespace {
+
+bool read_file(const std::string& path, std::vector<char>& buf) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  std::fseek(f, 0, SEEK_END);
+  long sz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  buf.resize((size_t)std::max<long>(sz, 0) + 1);
+  size_t got = sz > 0 ? std::fread(buf.data(), 1, (size_t)sz, f) : 0;
+  std::fclose(f);
+  buf.resize(got + 1);
+  buf[got] = '\0';
+  return true;
+}
+
+bool ends_with(const std::string& s, const char* suf) {
+  size_t l = std::strlen(suf);
+  return s.size() >= l && s.compare(s.size() - l, l, suf) == 0;
+}
+
+struct LineCursor {
+  const char* p;
+  const char* end;
+  // next data line (comments start with '%' after leading blanks); returns [b, e)
+  bool next(const char*& b, const char*& e) {
+    while (p < end) {
+      const char* ls = p;
+      const char* le = (const char*)std::memchr(p, '\n', (size_t)(end - p));
+      if (!le) le = end;
+      p = le < end ? le + 1 : end;
+      const char* q = ls;
+      while (q < le && (*q == ' ' || *q == '\t')) ++q;
+      const char* qe = le;
+      if (qe > q && qe[-1] == '\r') --qe;
+      if (q >= qe) continue;
+      if (*q == '%') continue;
+      b = q;
+      e = qe;
+      return true;
+    }
+    return false;
+  }
+};
+
+inline bool is_blank(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+// parses an unsigned integer token; returns false when the next token is not a number
+inline bool parse_uint(const char*& q, const char* e, uint64_t& out, bool& negative) {
+  while (q < e && is_blank(*q)) ++q;
+  negative = false;
+  if (q >= e) return false;
+  if (*q == '-' || *q == '+') {
+    negative = *q == '-';
+    ++q;
+  }
+  if (q >= e || *q < '0' || *q > '9') return false;
+  uint64_t v = 0;
+  while (q < e && *q >= '0' && *q <= '9') {
+    v = v * 10 + (uint64_t)(*q - '0');
+    if (v > 0xFFFFFFFFFFull) v = 0xFFFFFFFFFFull;
+    ++q;
+  }
+  out = v;
+  return true;
+}
+
+std::string load_hgr(const std::string& path, hgp_hgraph& hg) {
+  std::vector<char> buf;
+  if (!read_file(path, buf)) return "Cannot open .hgr file";
+  LineCursor lc{buf.data(), buf.data() + buf.size() - 1};
+  const char *b, *e;
+  if (!lc.next(b, e)) return "Empty .hgr file";
+  uint64_t E = 0, V = 0, fmt = 0;
+  bool neg;
+  const char* q = b;
+  if (!parse_uint(q, e, E, neg) || !parse_uint(q, e, V, neg)) return "Invalid .hgr header";
+  if (E > 0xFFFFFFFFull) return "Hyperedge count exceeds uint32 range.";
+  if (V > 0xFFFFFFFFull) return "Node count exceeds uint32 range.";
+  if (!parse_uint(q, e, fmt, neg)) fmt = 0;
+  const bool ew = fmt == 1 || fmt == 11, vw = fmt == 10 || fmt == 11;
+  hg = hgp_hgraph();
+  hg.n = (uint32_t)V;
+  hg.pins.reserve(buf.size() / 4);
+  hg.srcc.reserve(E);
+  hg.w.reserve(E);
+  hg.off.reserve(E + 1);
+  for (uint64_t i = 0; i < E; ++i) {
+    if (!lc.next(b, e)) return "Unexpected end of file while reading hyperedges";
+    q = b;
+    float w = 1.0f;
+    if (ew) {
+      while (q < e && is_blank(*q)) ++q;
+      char* endp = nullptr;
+      w = std::strtof(q, &endp);
+      if (endp == q) w = 1.0f;
+      q = endp > e ? e : endp;
+    }
+    const size_t start = hg.pins.size();
+    uint64_t v;
+    while (parse_uint(q, e, v, neg)) {
+      if (neg || v == 0 || v > V) return "Invalid node encountered";
+      hg.pins.push_back((uint32_t)(v - 1));
+      if (q < e && !is_blank(*q)) break;  // token continues with non-digits: extraction stops
+    }
+    const size_t cnt = hg.pins.size() - start;
+    if (cnt < 1) return "Hyperedge must contain at least one node";
+    if (cnt == 1) {
+      hg.pins.resize(start);  // singleton hedges are skipped
+      continue;
+    }
+    hg.off.push_back(hg.pins.size());
+    hg.srcc.push_back(0);
+    hg.w.push_back(w);
+  }
+  if (vw) {
+    hg.vweights.assign(hg.n, 1);
+    for (uint32_t i = 0; i < hg.n; ++i) {
+      if (!lc.next(b, e)) break;
+      q = b;
+      uint64_t x;
+      if (parse_uint(q, e, x, neg)) hg.vweights[i] = (uint32_t)x;
+    }
+  }
+  return "";
+}
+
+std::string load_bin(const std::string& path, bool axh, hgp_hgraph& hg) {
+  std::vector<char> buf;
+  if (!read_file(path, buf)) return axh ? "Cannot open .axh file" : "Cannot open file";
+  const uint8_t* p = (const uint8_t*)buf.data();
+  const uint8_t* end = p + buf.size() - 1;
+  auto r32 = [&](uint32_t& v) -> bool {
+    if (end - p < 4) return false;
+    std::memcpy(&v, p, 4);
+    p += 4;
+    return true;
+  };
+  uint32_t N, E;
+  if (!r32(N) || !r32(E)) return "Truncated header";
+  hg = hgp_hgraph();
+  hg.n = N;
+  hg.srcc.reserve(E);
+  hg.w.reserve(E);
+  hg.off.reserve((size_t)E + 1);
+  for (uint32_t i = 0; i < E; ++i) {
+    uint32_t a, b;
+    if (!r32(a) || !r32(b)) return "Truncated hyperedge";
+    uint32_t npins, nsrc;
+    if (axh) {
+      npins = a;
+      nsrc = b;
+      if (nsrc == 0 || npins < nsrc) return "Invalid AXH hyperedge header";
+      for (uint32_t k = 0; k < npins; ++k) {
+        uint32_t v;
+        if (!r32(v)) return "Truncated hyperedge";
+        if (v >= N) return "Invalid node id in AXH file";
+        hg.pins.push_back(v);
+      }
+    } else {
+      npins = a + 1;
+      nsrc = 1;
+      if (b >= N) return "Invalid source node";
+      hg.pins.push_back(b);
+      for (uint32_t k = 0; k < a; ++k) {
+        uint32_t v;
+        if (!r32(v)) return "Truncated hyperedge";
+        if (v >= N) return "Invalid destination node";
+        hg.pins.push_back(v);
+      }
+    }
+    if (npins == nsrc) return "Hyperedge must have at least 1 destination";
+    float w;
+    if (end - p < 4) return "Truncated weight";
+    std::memcpy(&w, p, 4);
+    p += 4;
+    hg.off.push_back(hg.pins.size());
+    hg.srcc.push_back(nsrc);
+    hg.w.push_back(w);
+  }
+  return "";
+}
+
+void put_u32(std::vector<char>& out, uint32_t v) {
+  char b[4];
+  std::memcpy(b, &v, 4);
+  out.insert(out.end(), b, b + 4);
+}
+
+}  // namespace
+
+std::string load_hgraph(const std::string& path, hgp_hgraph& hg) {
+  if (ends_with(path, ".hgr")) return load_hgr(path, hg);
+  if (ends_with(path, ".snn")) return load_bin(path, false, hg);
+  if (ends_with(path, ".axh")) return load_bin(path, true, hg);
+  return "Failed to load hypergraph, unsupported file format (supported: '.hgr', '.snn', '.axh').";
+}
+
+// deduplicateHyperedges(2,false) (includes/hgraph.hpp:518-581): sources first, first
+// occurrence kept, a destination equal to a source dropped.  Stamped seen-array: O(pins).
+void dedup_rule2(hgp_hgraph& hg) {
+  std::vector<uint32_t> stamp(hg.n, 0);
+  uint32_t gen = 0;
+  uint64_t wpos = 0;
+  uint64_t start = hg.off[0];
+  for (uint32_t i = 0; i < hg.e(); ++i) {
+    const uint64_t s = start, t = hg.off[i + 1];
+    start = t;
+    if (++gen == 0) {
+      std::fill(stamp.begin(), stamp.end(), 0);
+      gen = 1;
+    }
+    const uint64_t ws = wpos;
+    const uint32_t sc = hg.srcc[i];
+    for (uint64_t k = s; k < s + sc; ++k) {
+      const uint32_t v = hg.pins[k];
+      if (stamp[v] != gen) {
+        stamp[v] = gen;
+        hg.pins[wpos++] = v;
+      }
+    }
+    const uint32_t nsrc = (uint32_t)(wpos - ws);
+    for (uint64_t k = s + sc; k < t; ++k) {
+      const uint32_t v = hg.pins[k];
+      if (stamp[v] != gen) {
+        stamp[v] = gen;
+        hg.pins[wpos++] = v;
+      }
+    }
+    hg.off[i] = ws;
+    hg.srcc[i] = nsrc;
+  }
+  hg.off[hg.e()] = wpos;
+  hg.pins.resize(wpos);
+}
+
+std::string save_hgraph(const hgp_hgraph& hg, const std::string& path) {
+  std::vector<char> out;
+  if (ends_with(path, ".hgr")) {
+    // savehMETIS (:1016-1051): weights as u32(w * 1024) when any differs from 1
+    bool emit = false;
+    for (float x : hg.w)
+      if (x != 1.0f) {
+        emit = true;
+        break;
+      }
+    std::string s = std::to_string(hg.e()) + " " + std::to_string(hg.n) + (emit ? " 01\n" : "\n");
+    out.insert(out.end(), s.begin(), s.end());
+    char tmp[32];
+    for (uint32_t i = 0; i < hg.e(); ++i) {
+      if (emit) {
+        int l = std::snprintf(tmp, sizeof tmp, "%u ", (uint32_t)(hg.w[i] * 1024.0f));
+        out.insert(out.end(), tmp, tmp + l);
+      }
+      for (uint64_t k = hg.off[i]; k < hg.off[i + 1]; ++k) {
+        int l = std::snprintf(tmp, sizeof tmp, k + 1 < hg.off[i + 1] ? "%u " : "%u", hg.pins[k] + 1);
+        out.insert(out.end(), tmp, tmp + l);
+      }
+      out.push_back('\n');
+    }
+  } else if (ends_with(path, ".snn") || ends_with(path, ".axh")) {
+    const bool axh = ends_with(path, ".axh");
+    put_u32(out, hg.n);
+    put_u32(out, hg.e());
+    for (uint32_t i = 0; i < hg.e(); ++i) {
+      const uint32_t len = (uint32_t)(hg.off[i + 1] - hg.off[i]);
+      if (axh) {
+        if (hg.srcc[i] == 0 || len < hg.srcc[i]) return "Invalid hyperedge encountered while saving AXH";
+        put_u32(out, len);
+        put_u32(out, hg.srcc[i]);
+      } else {
+        // saveSNN (:746-774): first pin is the source, the rest destinations
+        put_u32(out, len - 1);
+      }
+      for (uint64_t k = hg.off[i]; k < hg.off[i + 1]; ++k) put_u32(out, hg.pins[k]);
+      char b[4];
+      std::memcpy(b, &hg.w[i], 4);
+      out.insert(out.end(), b, b + 4);
+    }
+  } else {
+    return "Failed to save partitioned hypergraph, unsupported file format (supported: '.hgr', '.snn', '.axh').";
+  }
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return "Cannot open output file";
+  std::fwrite(out.data(), 1, out.size(), f);
+  std::fclose(f);
+  return "";
+}
+
+hgp_hgraph partitions_hypergraph(const hgp_hgraph& hg, const std::vector<uint32_t>& part) {
+  uint32_t k = 0;
+  for (uint32_t p : part) k = std::max(k, p + 1);
+  struct Acc {
+    std::vector<uint32_t> s, d;
+    float w = 0;
+  };
+  std::unordered_map<std::string, size_t> index;
+  std::vector<Acc> accs;
+  std::vector<uint32_t> sp, dp;
+  std::string key;
+  for (uint32_t i = 0; i < hg.e(); ++i) {
+    sp.clear();
+    dp.clear();
+    for (uint64_t j = hg.off[i]; j < hg.off[i + 1]; ++j) {
+      if (j - hg.off[i] < hg.srcc[i]) sp.push_back(part[hg.pins[j]]);
+      else dp.push_back(part[hg.pins[j]]);
+    }
+    std::sort(sp.begin(), sp.end());
+    sp.erase(std::unique(sp.begin(), sp.end()), sp.end());
+    std::sort(dp.begin(), dp.end());
+    dp.erase(std::unique(dp.begin(), dp.end()), dp.end());
+    std::vector<uint32_t> d2;
+    for (uint32_t x : dp)
+      if (!std::binary_search(sp.begin(), sp.end(), x)) d2.push_back(x);
+    if (sp.size() <= 1 && d2.empty()) continue;
+    key.assign((const char*)sp.data(), sp.size() * 4);
+    key.push_back('|');
+    key.append((const char*)d2.data(), d2.size() * 4);
+    auto it = index.find(key);
+    if (it == index.end()) {
+      index.emplace(key, accs.size());
+      accs.push_back(Acc{sp, d2, hg.w[i]});
+    } else {
+      accs[it->second].w += hg.w[i];
+    }
+  }
+  hgp_hgraph q;
+  q.n = k;
+  for (auto& a : accs) {
+    q.pins.insert(q.pins.end(), a.s.begin(), a.s.end());
+    q.pins.insert(q.pins.end(), a.d.begin(), a.d.end());
+    q.off.push_back(q.pins.size());
+    q.srcc.push_back((uint32_t)a.s.size());
+    q.w.push_back(a.w);
+  }
+  return q;
+}
+
+// lambda-1 and cut-net of the quotient hypergraph, SOED on the input (grading/grade.cpp:300-308)
+Metrics metrics(const hgp_hgraph& hg, const std::vector<uint32_t>& part) {
+  Metrics m;
+  std::vector<uint32_t> sp, dp, all;
+  for (uint32_t i = 0; i < hg.e(); ++i) {
+    sp.clear();
+    dp.clear();
+    all.clear();
+    for (uint64_t j = hg.off[i]; j < hg.off[i + 1]; ++j) {
+      const uint32_t q = part[hg.pins[j]];
+      all.push_back(q);
+      if (j - hg.off[i] < hg.srcc[i]) sp.push_back(q);
+      else dp.push_back(q);
+    }
+    std::sort(all.begin(), all.end());
+    const size_t lam = std::unique(all.begin(), all.end()) - all.begin();
+    std::sort(sp.begin(), sp.end());
+    sp.erase(std::unique(sp.begin(), sp.end()), sp.end());
+    std::sort(dp.begin(), dp.end());
+    dp.erase(std::unique(dp.begin(), dp.end()), dp.end());
+    size_t nd = 0;
+    for (uint32_t x : dp)
+      if (!std::binary_search(sp.begin(), sp.end(), x)) ++nd;
+    const size_t len = sp.size() + nd;
+    if (!(sp.size() <= 1 && nd == 0)) {
+      m.connectivity += (double)hg.w[i] * (double)(len - 1);
+      if (len > 1) m.cutnet += hg.w[i];
+      m.quotient_pins += len;
+    }
+    m.soed += (double)hg.w[i] * (double)(lam - 1) * (double)(hg.off[i + 1] - hg.off[i]);
+  }
+  return m;
+}
+
+int validity(const hgp_hgraph& hg, const std::vector<uint32_t>& part, uint32_t omega, uint32_t delta, uint32_t max_parts,
+             std::string* why) {
+  std::unordered_map<uint32_t, uint32_t> counter;
+  for (uint32_t p : part) counter[p]++;
+  const uint32_t count = (uint32_t)counter.size();
+  char buf[256];
+  if (count > max_parts) {
+    if (why) {
+      std::snprintf(buf, sizeof buf, "more partitions (%u) than allowed (%u)", count, max_parts);
+      *why = buf;
+    }
+    return 0;
+  }
+  for (auto& kv : counter)
+    if (kv.second > omega) {
+      if (why) {
+        std::snprintf(buf, sizeof buf, "more nodes per partition (%u) than allowed (%u)", kv.second, omega);
+        *why = buf;
+      }
+      return 0;
+    }
+  for (uint32_t i = 0; i < count; ++i)
+    if (!counter.count(i)) {
+      if (why) *why = "Partitions must be incrementally indexed from 0 onward.";
+      return -1;
+    }
+  std::vector<uint32_t> syn(count, 0);
+  std::vector<uint32_t> stamp(count, UINT32_MAX);
+  for (uint32_t i = 0; i < hg.e(); ++i)
+    for (uint64_t j = hg.off[i] + hg.srcc[i]; j < hg.off[i + 1]; ++j) {
+      const uint32_t q = part[hg.pins[j]];
+      if (stamp[q] != i) {
+        stamp[q] = i;
+        syn[q]++;
+      }
+    }
+  for (uint32_t i = 0; i < count; ++i)
+    if (syn[i] > delta) {
+      if (why) {
+        std::snprintf(buf, sizeof buf, "more inbound hedges per partition (%u) than allowed (%u)", syn[i], delta);
+        *why = buf;
+      }
+      return 0;
+    }
+  return 1;
+}
+
+// ---- generators -----------------------------------------------------------------------------
+
+void gen_random(uint32_t n, uint32_t e, uint32_t max_size, uint64_t seed, hgp_hgraph& hg) {
+  std::mt19937_64 rng(seed);
+  std::geometric_distribution<uint32_t> geo(0.5);  // failures before success: X-1 for X on {1,2,...}
+  std::uniform_int_distribution<uint32_t> node(0, n - 1);
+  hg = hgp_hgraph();
+  hg.n = n;
+  std::vector<uint32_t> cur;
+  for (uint32_t i = 0; i < e; ++i) {
+    uint32_t sz = 2 + 1 + geo(rng);
+    if (sz > max_size) sz = max_size;
+    if (sz > n) sz = n;
+    cur.clear();
+    while (cur.size() < sz) {
+      uint32_t v = node(rng);
+      if (std::find(cur.begin(), cur.end(), v) == cur.end()) cur.push_back(v);
+    }
+    hg.pins.insert(hg.pins.end(), cur.begin(), cur.end());
+    hg.off.push_back(hg.pins.size());
+    hg.srcc.push_back(0);
+    hg.w.push_back(1.0f);
+  }
+}
+
+void gen_netlist(uint32_t n, uint32_t e, double alpha, uint32_t max_size, uint32_t radius, double frac_local,
+                 uint64_t seed, hgp_hgraph& hg) {
+  std::mt19937_64 rng(seed);
+  // discrete power law on [2, max_size]: inverse CDF table
+  std::vector<double> cdf;
+  double tot = 0;
+  for (uint32_t s = 2; s <= max_size; ++s) {
+    tot += std::pow((double)s, -alpha);
+    cdf.push_back(tot);
+  }
+  for (double& x : cdf) x /= tot;
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  const uint32_t side = (uint32_t)std::ceil(std::sqrt((double)n));
+  std::uniform_int_distribution<uint32_t> node(0, n - 1);
+  std::uniform_int_distribution<int> off(-(int)radius, (int)radius);
+  hg = hgp_hgraph();
+  hg.n = n;
+  hg.vweights.resize(n);
+  std::uniform_int_distribution<uint32_t> vw(1, 4);
+  for (uint32_t v = 0; v < n; ++v) hg.vweights[v] = vw(rng);
+  std::vector<uint32_t> cur;
+  for (uint32_t i = 0; i < e; ++i) {
+    uint32_t sz = 2 + (uint32_t)(std::lower_bound(cdf.begin(), cdf.end(), U(rng)) - cdf.begin());
+    if (sz > n) sz = n;
+    cur.clear();
+    const uint32_t first = node(rng);
+    cur.push_back(first);
+    const int fx = (int)(first % side), fy = (int)(first / side);
+    uint32_t guard = 0;
+    while (cur.size() < sz && guard++ < 64 * sz + 1024) {
+      uint32_t v;
+      if (U(rng) < frac_local) {
+        int dx, dy;
+        do {
+          dx = off(rng);
+          dy = off(rng);
+        } while (dx * dx + dy * dy > (int)(radius * radius));
+        const int x = fx + dx, y = fy + dy;
+        if (x < 0 || y < 0 || x >= (int)side || y >= (int)side) continue;
+        const uint64_t id = (uint64_t)y * side + (uint64_t)x;
+        if (id >= n) continue;
+        v = (uint32_t)id;
+      } else {
+        v = node(rng);
+      }
+      if (std::find(cur.begin(), cur.end(), v) == cur.end()) cur.push_back(v);
+    }
+    while (cur.size() < sz) {  // dense neighbourhood exhausted: complete uniformly
+      uint32_t v = node(rng);
+      if (std::find(cur.begin(), cur.end(), v) == cur.end()) cur.push_back(v);
+    }
+    hg.pins.insert(hg.pins.end(), cur.begin(), cur.end());
+    hg.off.push_back(hg.pins.size());
+    hg.srcc.push_back(0);
+    hg.w.push_back(1.0f);
+  }
+}
+
+void gen_snn(uint32_t layers, uint32_t per_layer, uint32_t grid, uint32_t fanout, double sigma, uint64_t seed,
+             hgp_hgraph& hg) {
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<double> N01(0.0, 1.0);
+  hg = hgp_hgraph();
+  hg.n = layers * per_layer;
+  std::vector<uint32_t> cur;
+  std::unordered_set<uint32_t> seen;
+  for (uint32_t l = 0; l + 1 < layers; ++l) {
+    for (uint32_t i = 0; i < per_layer; ++i) {
+      const uint32_t src = l * per_layer + i;
+      const double cx = (double)(i % grid), cy = (double)(i / grid);
+      cur.clear();
+      seen.clear();
+      cur.push_back(src);
+      uint64_t guard = 0;
+      while (cur.size() < (size_t)fanout + 1) {
+        int x = (int)std::lround(cx + sigma * N01(rng));
+        int y = (int)std::lround(cy + sigma * N01(rng));
+        if (++guard > 1000ull * fanout) {  // pathological: fall back to uniform cells
+          x = (int)(rng() % grid);
+          y = (int)(rng() % grid);
+        }
+        x = std::min(std::max(x, 0), (int)grid - 1);
+        y = std::min(std::max(y, 0), (int)grid - 1);
+        const uint64_t cell = (uint64_t)y * grid + (uint64_t)x;
+        if (cell >= per_layer) continue;
+        const uint32_t dst = (l + 1) * per_layer + (uint32_t)cell;
+        if (seen.insert(dst).second) cur.push_back(dst);
+      }
+      hg.pins.insert(hg.pins.end(), cur.begin(), cur.end());
+      hg.off.push_back(hg.pins.size());
+      hg.srcc.push_back(1);
+      hg.w.push_back(1.0f);
+    }
+  }
+}
+
+}  // namespace hgp

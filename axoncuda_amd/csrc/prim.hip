@@ -1,0 +1,296 @@
+// prim.hip - scans, stable radix sort and small helpers for gfx950 (wave64).
+//
+// Scans: three passes (tile reduce, scan of tile sums in one workgroup, tile scan).
+// Radix sort: reduce-then-scan LSD with 8-bit digits; the scatter ranks keys within a tile
+// in 256-key rounds, using eight ballots per key to find same-digit lanes of a wave (a
+// stable in-wave rank) and LDS prefix sums across the four waves of the workgroup.
+#include "prim.hpp"
+
+namespace hgp {
+
+namespace {
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* lds_waves, T* total) {
+  // lds_waves: SCAN_BLOCK/64 entries
+  const uint32_t w = threadIdx.x / 64, l = lane_id();
+  T incl = wave_scan_incl(v);
+  if (l == 63) lds_waves[w] = incl;
+  __syncthreads();
+  T base = 0, tot = 0;
+  for (uint32_t i = 0; i < blockDim.x / 64; ++i) {
+    T x = lds_waves[i];
+    if (i < w) base += x;
+    tot += x;
+  }
+  *total = tot;
+  __syncthreads();
+  return base + incl - v;
+}
+
+template <typename TI, typename TO>
+__global__ void k_tile_reduce(const TI* __restrict__ in, uint64_t n, TO* __restrict__ partial) {
+  __shared__ TO sw[SCAN_BLOCK / 64];
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  TO s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + (uint64_t)k * SCAN_BLOCK + threadIdx.x;
+    if (i < n) s += (TO)in[i];
+  }
+  s = wave_sum(s);
+  if (lane_id() == 0) sw[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    TO t = 0;
+    for (int i = 0; i < SCAN_BLOCK / 64; ++i) t += sw[i];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// exclusive scan of partial[0..m) in place by one workgroup; partial[m] = total
+template <typename TO>
+__global__ void k_scan_partials(TO* partial, uint32_t m) {
+  __shared__ TO sw[1024 / 64];
+  __shared__ TO carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < m; base += blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    TO v = i < m ? partial[i] : 0;
+    TO tot;
+    TO ex = block_excl_scan(v, sw, &tot);
+    TO carry = carry_s;
+    if (i < m) partial[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s = carry + tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[m] = carry_s;
+}
+
+template <typename TI, typename TO>
+__global__ void k_tile_scan_excl(const TI* in, TO* out, uint64_t n, const TO* __restrict__ partial) {
+  __shared__ TO sw[SCAN_BLOCK / 64];
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  TO v[SCAN_ITEMS];
+  TO s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? (TO)in[i] : (TO)0;
+    s += v[k];
+  }
+  TO tot;
+  TO ex = block_excl_scan(s, sw, &tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    if (i < n) out[i] = ex;
+    ex += v[k];
+  }
+}
+
+template <typename TI, typename TO>
+__global__ void k_tile_scan_incl(const TI* in, TO* out, uint64_t n, const TO* __restrict__ partial) {
+  __shared__ TO sw[SCAN_BLOCK / 64];
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  TO v[SCAN_ITEMS];
+  TO s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? (TO)in[i] : (TO)0;
+    s += v[k];
+  }
+  TO tot;
+  TO ex = block_excl_scan(s, sw, &tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    uint64_t i = base + k;
+    ex += v[k];
+    if (i < n) out[i] = ex;
+  }
+}
+
+template <typename TO>
+__global__ void k_store_total(TO* out, uint64_t n, const TO* partial, uint32_t m) {
+  out[n] = partial[m];
+}
+
+// Reads all tile sums before the tile scan writes, so in == out is safe.
+template <typename TI, typename TO>
+void scan_impl(Ctx& c, const TI* in, TO* out, uint64_t n, bool exclusive) {
+  uint32_t tiles = ceil_div(n, SCAN_TILE);
+  DBuf<TO> partial(c, (uint64_t)tiles + 1);
+  if (n == 0) {
+    if (exclusive) memset_async(c, out, 0, sizeof(TO));
+    return;
+  }
+  HGP_LAUNCH((k_tile_reduce<TI, TO>), tiles, SCAN_BLOCK, 0, c, in, n, partial.get());
+  HGP_LAUNCH((k_scan_partials<TO>), 1, 1024, 0, c, partial.get(), tiles);
+  if (exclusive) {
+    HGP_LAUNCH((k_tile_scan_excl<TI, TO>), tiles, SCAN_BLOCK, 0, c, in, out, n, partial.get());
+    HGP_LAUNCH((k_store_total<TO>), 1, 1, 0, c, out, n, partial.get(), tiles);
+  } else {
+    HGP_LAUNCH((k_tile_scan_incl<TI, TO>), tiles, SCAN_BLOCK, 0, c, in, out, n, partial.get());
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// radix sort
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_ROUNDS;
+
+template <typename K>
+__global__ void k_rs_hist(const K* __restrict__ keys, uint64_t n, int shift, uint32_t nblocks,
+                          uint32_t* __restrict__ counts) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    uint64_t i = base + (uint64_t)r * RS_BLOCK + threadIdx.x;
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+  }
+  __syncthreads();
+  counts[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename K, bool HAS_VALS>
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                         K* __restrict__ okeys, uint32_t* __restrict__ ovals,
+                                                         uint64_t n, int shift, uint32_t nblocks,
+                                                         const uint64_t* __restrict__ offsets) {
+  __shared__ uint64_t run[256];
+  __shared__ uint32_t wcnt[RS_BLOCK / 64][256];
+  const uint32_t t = threadIdx.x, w = t / 64;
+  run[t] = offsets[(uint64_t)t * nblocks + blockIdx.x];
+  uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    uint64_t i = base + (uint64_t)r * RS_BLOCK + t;
+    if (base + (uint64_t)r * RS_BLOCK >= n) break;  // uniform across the block
+#pragma unroll
+    for (int q = 0; q < RS_BLOCK / 64; ++q) wcnt[q][t] = 0;
+    __syncthreads();
+    const bool valid = i < n;
+    K key = valid ? keys[i] : (K)0;
+    uint32_t val = 0;
+    if (HAS_VALS && valid) val = vals[i];
+    const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = lanes_below(peers);
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    // per digit: exclusive prefix over waves (thread t handles digit t)
+    uint32_t acc = 0;
+#pragma unroll
+    for (int q = 0; q < RS_BLOCK / 64; ++q) {
+      uint32_t c = wcnt[q][t];
+      wcnt[q][t] = acc;
+      acc += c;
+    }
+    __syncthreads();
+    if (valid) {
+      uint64_t pos = run[d] + wcnt[w][d] + rank;
+      okeys[pos] = key;
+      if (HAS_VALS) ovals[pos] = val;
+    }
+    __syncthreads();
+    run[t] += acc;
+  }
+}
+
+template <typename K>
+void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits) {
+  if (n <= 1 || bits <= 0) return;
+  const uint32_t nblocks = ceil_div(n, RS_TILE);
+  DBuf<K> k2(c, n);
+  DBuf<uint32_t> v2(c, vals ? n : 0);
+  DBuf<uint32_t> counts(c, (uint64_t)256 * nblocks);
+  DBuf<uint64_t> offsets(c, (uint64_t)256 * nblocks + 1);
+  K* ka = keys;
+  K* kb = k2.get();
+  uint32_t* va = vals;
+  uint32_t* vb = v2.get();
+  for (int shift = 0; shift < bits; shift += 8) {
+    HGP_LAUNCH((k_rs_hist<K>), nblocks, RS_BLOCK, 0, c, ka, n, shift, nblocks, counts.get());
+    exclusive_scan(c, counts.get(), offsets.get(), (uint64_t)256 * nblocks);
+    if (vals)
+      HGP_LAUNCH((k_rs_scatter<K, true>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks, offsets.get());
+    else
+      HGP_LAUNCH((k_rs_scatter<K, false>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks,
+                 offsets.get());
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    copy_d2d(c, keys, ka, n * sizeof(K));
+    if (vals) copy_d2d(c, vals, va, n * sizeof(uint32_t));
+  }
+}
+
+template <typename K>
+__global__ void k_segment_offsets(const K* __restrict__ keys, int shift, uint64_t n, uint32_t nseg,
+                                  uint64_t* __restrict__ off) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  int64_t a = i == 0 ? -1 : (int64_t)(uint32_t)(keys[i - 1] >> shift);
+  int64_t b = i == n ? (int64_t)nseg : (int64_t)(uint32_t)(keys[i] >> shift);
+  if (b > (int64_t)nseg) b = nseg;
+  for (int64_t s = a + 1; s <= b; ++s) off[s] = i;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_iota_u32(uint32_t* p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+__global__ void k_max_u64(const uint64_t* __restrict__ p, uint64_t n, unsigned long long* out) {
+  uint64_t m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    m = p[i] > m ? p[i] : m;
+  m = wave_max(m);
+  if (lane_id() == 0) atomicMax(out, (unsigned long long)m);
+}
+
+}  // namespace
+
+void exclusive_scan(Ctx& c, const uint32_t* in, uint64_t* out, uint64_t n) { scan_impl<uint32_t, uint64_t>(c, in, out, n, true); }
+void exclusive_scan(Ctx& c, const uint64_t* in, uint64_t* out, uint64_t n) { scan_impl<uint64_t, uint64_t>(c, in, out, n, true); }
+void inclusive_scan(Ctx& c, const int64_t* in, int64_t* out, uint64_t n) { scan_impl<int64_t, int64_t>(c, in, out, n, false); }
+void inclusive_scan(Ctx& c, const int32_t* in, int32_t* out, uint64_t n) { scan_impl<int32_t, int32_t>(c, in, out, n, false); }
+
+void radix_sort(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int bits) { radix_impl<uint32_t>(c, keys, vals, n, bits); }
+void radix_sort(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int bits) { radix_impl<uint64_t>(c, keys, vals, n, bits); }
+
+void segment_offsets(Ctx& c, const uint32_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off) {
+  HGP_LAUNCH((k_segment_offsets<uint32_t>), ceil_div(n + 1, 256), 256, 0, c, keys, shift, n, nseg, off);
+}
+void segment_offsets(Ctx& c, const uint64_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off) {
+  HGP_LAUNCH((k_segment_offsets<uint64_t>), ceil_div(n + 1, 256), 256, 0, c, keys, shift, n, nseg, off);
+}
+
+void fill_u32(Ctx& c, uint32_t* p, uint32_t v, uint64_t n) { HGP_LAUNCH(k_fill_u32, ceil_div(n, 256), 256, 0, c, p, v, n); }
+void iota_u32(Ctx& c, uint32_t* p, uint64_t n) { HGP_LAUNCH(k_iota_u32, ceil_div(n, 256), 256, 0, c, p, n); }
+uint64_t reduce_max_u64(Ctx& c, const uint64_t* p, uint64_t n) {
+  DBuf<unsigned long long> out(c, 1);
+  memset_async(c, out.get(), 0, sizeof(unsigned long long));
+  if (n) HGP_LAUNCH(k_max_u64, std::min<uint32_t>(ceil_div(n, 256), 2048), 256, 0, c, p, n, out.get());
+  return (uint64_t)read_scalar(c, out.get());
+}
+
+}  // namespace hgp

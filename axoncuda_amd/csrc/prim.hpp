@@ -1,0 +1,38 @@
+// prim.hpp - in-house device-wide primitives (replacing the reference's thrust/CUB calls:
+// inclusive/exclusive scans, stable radix sorts, segment offsets, reductions).
+#pragma once
+
+#include "common.hpp"
+
+namespace hgp {
+
+// out[0..n) = exclusive prefix of in[0..n); out[n] = total.  In-place (out == in) is
+// allowed when the element types match.
+void exclusive_scan(Ctx& c, const uint32_t* in, uint64_t* out, uint64_t n);
+void exclusive_scan(Ctx& c, const uint64_t* in, uint64_t* out, uint64_t n);
+// out[i] = sum in[0..i]; in-place allowed.
+void inclusive_scan(Ctx& c, const int64_t* in, int64_t* out, uint64_t n);
+void inclusive_scan(Ctx& c, const int32_t* in, int32_t* out, uint64_t n);
+
+// Stable LSD radix sorts over key bits [0, bits); values ride along (nullable).
+void radix_sort(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int bits);
+void radix_sort(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int bits);
+
+// off[s] = first index i with (keys[i] >> shift) >= s, for s in [0, nseg]; keys sorted.
+void segment_offsets(Ctx& c, const uint32_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off);
+void segment_offsets(Ctx& c, const uint64_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off);
+
+void fill_u32(Ctx& c, uint32_t* p, uint32_t v, uint64_t n);
+void iota_u32(Ctx& c, uint32_t* p, uint64_t n);
+uint64_t reduce_max_u64(Ctx& c, const uint64_t* p, uint64_t n);
+
+inline int bit_width(uint64_t x) {
+  int b = 0;
+  while (x) {
+    b++;
+    x >>= 1;
+  }
+  return b;
+}
+
+}  // namespace hgp

@@ -1,0 +1,1319 @@
+// refine.hip - parallel FM refinement with size and inbound-set event checks
+// (reference refinementRepeats, sources/refinement.cu:18-582; kernels/refinement.cu).
+//
+// One repeat:
+//   1. per-hedge partition lists: for every hedge the distinct partitions of its pins with
+//      the count of all pins and of destination pins (replaces the dense E x K
+//      pins-per-partition matrix and its sparse bitmap variant);
+//   2. gains (fm_refinement_gains_kernel :82-209), exact integers in 2^18 fixed point:
+//      gain(q) = saving - W_v + conn(v,q), conn accumulated in an LDS table keyed by
+//      partition; non-adjacent targets all share conn = 0 and are ranked by the tie hash;
+//   3. ranks: chaining (sources/chaining.cu:19-254) or gain sort (:172-189);
+//   4. cascade (:216-322) computed hedge-centrically: each hedge is visited once and every
+//      pin's contribution at its own rank is added to seq[rank] (integer atomics), instead
+//      of every node re-reading all pins of all its hedges;
+//   5. size and inbound-set validity prefixes via radix-sorted events (:239-491);
+//   6. best prefix (best_move_functor, headers/data_types.cuh:54-77) and apply (:328-370).
+#include <algorithm>
+#include <numeric>
+
+#include "prim.hpp"
+#include "stages.hpp"
+#include "wave_iter.hpp"
+
+namespace hgp {
+
+namespace {
+
+constexpr uint32_t PP_SMALL = 16;
+constexpr uint32_t PP_LDS_CAP = 4096;  // 3 x 16 KiB of dynamic LDS per wave
+constexpr uint32_t GAIN_CAP = 1024;
+constexpr uint32_t GAIN_LIGHT = 512;
+constexpr uint32_t CAS_SMALL = 16;
+constexpr uint32_t CAS_LDS = 2048;
+constexpr uint64_t HEAVY_BATCH_SLOTS = 32ull << 20;
+
+struct Best {
+  int32_t ok;
+  int32_t sv;
+  int64_t seq;
+  uint32_t r;
+  int32_t iv;
+};
+
+__device__ __forceinline__ uint64_t ordered_i64(int64_t g) { return (uint64_t)g ^ 0x8000000000000000ull; }
+
+// ---- 1. per-hedge partition lists ------------------------------------------------------
+__global__ void k_pp_small(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
+                           const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part, uint32_t E,
+                           uint32_t* __restrict__ pp_part, uint32_t* __restrict__ pp_cnt, uint32_t* __restrict__ pp_dcnt,
+                           uint32_t* __restrict__ lambda, uint32_t* __restrict__ large_flag, uint32_t* inbsz) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const uint64_t s = hoff[e], t = hoff[e + 1];
+  if (t - s > PP_SMALL) {
+    large_flag[e] = 1u;
+    return;
+  }
+  large_flag[e] = 0u;
+  uint32_t qs[PP_SMALL], cs[PP_SMALL], ds[PP_SMALL];
+  uint32_t m = 0;
+  const uint32_t sc = srcc[e];
+  for (uint64_t i = s; i < t; ++i) {
+    const uint32_t q = part[hedges[i]];
+    const uint32_t d = (i - s) >= sc ? 1u : 0u;
+    uint32_t k = 0;
+    while (k < m && qs[k] != q) ++k;
+    if (k == m) {
+      qs[m] = q;
+      cs[m] = 0;
+      ds[m] = 0;
+      ++m;
+    }
+    cs[k] += 1;
+    ds[k] += d;
+  }
+  for (uint32_t k = 0; k < m; ++k) {
+    pp_part[s + k] = qs[k];
+    pp_cnt[s + k] = cs[k];
+    pp_dcnt[s + k] = ds[k];
+    if (ds[k]) atomicAdd(&inbsz[qs[k]], 1u);
+  }
+  lambda[e] = m;
+}
+
+__global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
+                                                 const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part,
+                                                 const uint32_t* __restrict__ list, uint32_t cap,
+                                                 uint32_t* __restrict__ pp_part, uint32_t* __restrict__ pp_cnt,
+                                                 uint32_t* __restrict__ pp_dcnt, uint32_t* __restrict__ lambda,
+                                                 uint32_t* inbsz) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* keys = lds;
+  uint32_t* cnt = lds + cap;
+  uint32_t* dcnt = lds + 2 * cap;
+  const uint32_t e = list[blockIdx.x];
+  const uint32_t l = lane_id();
+  for (uint32_t i = l; i < cap; i += 64) {
+    keys[i] = NONE;
+    cnt[i] = 0;
+    dcnt[i] = 0;
+  }
+  __syncthreads();
+  const uint64_t s = hoff[e], t = hoff[e + 1];
+  const uint32_t sc = srcc[e];
+  for (uint64_t i = s + l; i < t; i += 64) {
+    const uint32_t q = part[hedges[i]];
+    uint32_t h = slot_hash(q) & (cap - 1);
+    for (;;) {
+      const uint32_t old = atomicCAS(&keys[h], NONE, q);
+      if (old == NONE || old == q) break;
+      h = (h + 1) & (cap - 1);
+    }
+    atomicAdd(&cnt[h], 1u);
+    if ((i - s) >= sc) atomicAdd(&dcnt[h], 1u);
+  }
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t b = 0; b < cap; b += 64) {
+    const uint32_t i = b + l;
+    const bool has = keys[i] != NONE;
+    const uint64_t m = __ballot(has);
+    if (has) {
+      const uint32_t o = base + lanes_below(m);
+      pp_part[s + o] = keys[i];
+      pp_cnt[s + o] = cnt[i];
+      pp_dcnt[s + o] = dcnt[i];
+      if (dcnt[i]) atomicAdd(&inbsz[keys[i]], 1u);
+    }
+    base += (uint32_t)__popcll(m);
+  }
+  if (l == 0) lambda[e] = base;
+}
+
+// ---- 2. gains ---------------------------------------------------------------------------
+struct GainArgs {
+  const uint32_t* tch;
+  const uint64_t* toff;
+  const uint64_t* hoff;
+  const float* w;
+  const uint32_t* part;
+  const uint32_t* sizes;
+  const uint32_t* psizes;
+  const uint32_t* pp_part;
+  const uint32_t* pp_cnt;
+  const uint32_t* lambda;
+  const uint32_t* order;  // partitions by mix32(q + repeat) descending
+  uint32_t k;
+  uint32_t omega;
+  uint32_t discount;
+  uint32_t repeat;
+  int incc;
+  uint32_t* moves;
+  int64_t* gains;
+};
+
+__device__ __forceinline__ bool target_ok(const GainArgs& a, uint32_t q, uint32_t p, uint32_t s) {
+  return q != p && (uint32_t)(a.psizes[q] + s - s / a.discount) <= a.omega;
+}
+
+__device__ __forceinline__ uint32_t table_find(const uint32_t* keys, uint32_t mask, uint32_t q) {
+  uint32_t h = slot_hash(q) & mask;
+  for (uint32_t probes = 0; probes <= mask; ++probes) {
+    const uint32_t k = keys[h];
+    if (k == q) return h;
+    if (k == NONE) return NONE;
+    h = (h + 1) & mask;
+  }
+  return NONE;
+}
+
+// Shared tail of the gain kernels: pick the best target from the (partition -> conn) table.
+// Wave-collective; `stride`/`first` give this wave's share of the table slots when several
+// waves share one table (heavy path); results combined by the caller.
+__device__ __forceinline__ void best_adjacent(const GainArgs& a, const uint32_t* keys, const int64_t* conn,
+                                              uint32_t cap, uint32_t first, uint32_t stride, uint32_t p, uint32_t s,
+                                              int64_t base, int64_t& bg, uint32_t& bh, uint32_t& bq) {
+  bg = INT64_MIN;
+  bh = 0;
+  bq = NONE;
+  for (uint32_t i = first; i < cap; i += stride) {
+    const uint32_t q = keys[i];
+    if (q == NONE) continue;
+    const int64_t c = conn[i];
+    if (c <= 0 || !target_ok(a, q, p, s)) continue;
+    const int64_t g = base + c;
+    const uint32_t h = mix32(q + a.repeat);
+    if (bq == NONE || g > bg || (g == bg && h > bh)) {
+      bg = g;
+      bh = h;
+      bq = q;
+    }
+  }
+}
+
+__device__ __forceinline__ void wave_best(int64_t& bg, uint32_t& bh, uint32_t& bq) {
+  // reduce (gain, hash) lexicographically across the wave; lanes without a target carry bq = NONE
+  int64_t g = bq == NONE ? INT64_MIN : bg;
+  const int64_t gm = wave_max(g);
+  uint32_t h = (bq != NONE && g == gm) ? bh : 0u;
+  const bool eligible = bq != NONE && g == gm;
+  const uint64_t hk = eligible ? (((uint64_t)h << 32) | 1u) : 0ull;  // (hash, present)
+  const uint64_t hm = wave_max(hk);
+  const bool mine = eligible && hk == hm;
+  const uint64_t who = __ballot(mine);
+  if (who == 0) {
+    bq = NONE;
+    return;
+  }
+  const int src = __ffsll((long long)who) - 1;
+  bq = __shfl(bq, src, 64);
+  bh = __shfl(bh, src, 64);
+  bg = gm;
+}
+
+// non-adjacent fallback: first partition in hash order that is a valid target and has no
+// positive connection to v (all such targets share the gain `base`)
+__device__ __forceinline__ uint32_t first_nonadjacent(const GainArgs& a, const uint32_t* keys, const int64_t* conn,
+                                                      uint32_t mask, uint32_t p, uint32_t s) {
+  const uint32_t l = lane_id();
+  for (uint32_t b = 0; b < a.k; b += 64) {
+    const uint32_t i = b + l;
+    bool ok = false;
+    uint32_t q = NONE;
+    if (i < a.k) {
+      q = a.order[i];
+      if (target_ok(a, q, p, s)) {
+        const uint32_t slot = table_find(keys, mask, q);
+        ok = slot == NONE || conn[slot] <= 0;
+      }
+    }
+    const uint64_t m = __ballot(ok);
+    if (m) return __shfl(q, __ffsll((long long)m) - 1, 64);
+  }
+  return NONE;
+}
+
+__global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t* __restrict__ list, uint32_t nlist) {
+  __shared__ uint32_t s_keys[4][GAIN_CAP];
+  __shared__ int64_t s_conn[4][GAIN_CAP];
+  const uint32_t wv = threadIdx.x / 64, l = lane_id();
+  const uint32_t idx = blockIdx.x * 4 + wv;
+  if (idx >= nlist) return;
+  const uint32_t v = list[idx];
+  uint32_t* keys = s_keys[wv];
+  int64_t* conn = s_conn[wv];
+  for (uint32_t i = l; i < GAIN_CAP; i += 64) {
+    keys[i] = NONE;
+    conn[i] = 0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const uint32_t p = a.part[v], s = a.sizes[v];
+  const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
+  int64_t saving = 0, Wv = 0;
+  for (uint64_t j = t0 + l; j < t1; j += 64) Wv += (int64_t)fixed_weight(a.w[a.tch[j]]);
+  wave_visit_entries(
+      a.tch, t0, t1, [&](uint32_t e) { return a.hoff[e]; }, [&](uint32_t e) { return a.lambda[e]; },
+      [&](const EntryVisit& ev) {
+        if (!ev.act) return;
+        const uint32_t q = a.pp_part[ev.abs];
+        const int64_t We = (int64_t)fixed_weight(a.w[ev.e]);
+        if (q == p) {
+          const uint32_t c = a.pp_cnt[ev.abs];
+          if (a.incc) saving += (int64_t)((uint64_t)We / ((uint64_t)c * c));
+          else if (c == 1) saving += We;
+          return;
+        }
+        uint32_t h = slot_hash(q) & (GAIN_CAP - 1);
+        for (;;) {
+          const uint32_t old = atomicCAS(&keys[h], NONE, q);
+          if (old == NONE || old == q) break;
+          h = (h + 1) & (GAIN_CAP - 1);
+        }
+        atomicAdd((unsigned long long*)&conn[h], (unsigned long long)We);
+      });
+  saving = wave_sum(saving);
+  Wv = wave_sum(Wv);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const int64_t base = saving - Wv;
+  int64_t bg;
+  uint32_t bh, bq;
+  best_adjacent(a, keys, conn, GAIN_CAP, l, 64, p, s, base, bg, bh, bq);
+  wave_best(bg, bh, bq);
+  if (bq == NONE) {
+    bq = first_nonadjacent(a, keys, conn, GAIN_CAP - 1, p, s);
+    bg = base;
+  }
+  if (l == 0) {
+    a.moves[v] = bq;
+    a.gains[v] = bq == NONE ? 0 : bg;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gains_heavy(GainArgs a, const uint32_t* __restrict__ list,
+                                                     const uint64_t* __restrict__ tab_off, uint32_t* __restrict__ gkeys,
+                                                     int64_t* __restrict__ gconn) {
+  __shared__ int64_t s_sum[2][4];
+  __shared__ int64_t s_bg[4];
+  __shared__ uint32_t s_bh[4], s_bq[4];
+  const uint32_t v = list[blockIdx.x];
+  const uint64_t tb = tab_off[blockIdx.x];
+  const uint32_t cap = (uint32_t)(tab_off[blockIdx.x + 1] - tb);
+  uint32_t* keys = gkeys + tb;
+  int64_t* conn = gconn + tb;
+  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
+    keys[i] = NONE;
+    conn[i] = 0;
+  }
+  __threadfence_block();
+  __syncthreads();
+  const uint32_t wv = threadIdx.x / 64, l = lane_id();
+  const uint32_t p = a.part[v], s = a.sizes[v];
+  const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
+  const uint64_t cnt = t1 - t0;
+  const uint64_t lo = t0 + cnt * wv / 4, hi = t0 + cnt * (wv + 1) / 4;
+  int64_t saving = 0, Wv = 0;
+  for (uint64_t j = lo + l; j < hi; j += 64) Wv += (int64_t)fixed_weight(a.w[a.tch[j]]);
+  wave_visit_entries(
+      a.tch, lo, hi, [&](uint32_t e) { return a.hoff[e]; }, [&](uint32_t e) { return a.lambda[e]; },
+      [&](const EntryVisit& ev) {
+        if (!ev.act) return;
+        const uint32_t q = a.pp_part[ev.abs];
+        const int64_t We = (int64_t)fixed_weight(a.w[ev.e]);
+        if (q == p) {
+          const uint32_t c = a.pp_cnt[ev.abs];
+          if (a.incc) saving += (int64_t)((uint64_t)We / ((uint64_t)c * c));
+          else if (c == 1) saving += We;
+          return;
+        }
+        uint32_t h = slot_hash(q) & (cap - 1);
+        for (;;) {
+          const uint32_t old = atomicCAS(&keys[h], NONE, q);
+          if (old == NONE || old == q) break;
+          h = (h + 1) & (cap - 1);
+        }
+        atomicAdd((unsigned long long*)&conn[h], (unsigned long long)We);
+      });
+  saving = wave_sum(saving);
+  Wv = wave_sum(Wv);
+  if (l == 0) {
+    s_sum[0][wv] = saving;
+    s_sum[1][wv] = Wv;
+  }
+  __threadfence();
+  __syncthreads();
+  const int64_t base = (s_sum[0][0] + s_sum[0][1] + s_sum[0][2] + s_sum[0][3]) -
+                       (s_sum[1][0] + s_sum[1][1] + s_sum[1][2] + s_sum[1][3]);
+  int64_t bg;
+  uint32_t bh, bq;
+  best_adjacent(a, keys, conn, cap, threadIdx.x, blockDim.x, p, s, base, bg, bh, bq);
+  wave_best(bg, bh, bq);
+  if (l == 0) {
+    s_bg[wv] = bg;
+    s_bh[wv] = bh;
+    s_bq[wv] = bq;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    int64_t g = INT64_MIN;
+    uint32_t h = 0, q = NONE;
+    for (int i = 0; i < 4; ++i) {
+      if (s_bq[i] == NONE) continue;
+      if (q == NONE || s_bg[i] > g || (s_bg[i] == g && s_bh[i] > h)) {
+        g = s_bg[i];
+        h = s_bh[i];
+        q = s_bq[i];
+      }
+    }
+    if (q == NONE) {
+      q = first_nonadjacent(a, keys, conn, cap - 1, p, s);
+      g = base;
+    }
+    if (l == 0) {
+      a.moves[v] = q;
+      a.gains[v] = q == NONE ? 0 : g;
+    }
+  }
+}
+
+__global__ void k_gain_bound(const uint32_t* __restrict__ tch, const uint64_t* __restrict__ toff,
+                             const uint32_t* __restrict__ lambda, uint32_t n, uint32_t k, uint64_t* __restrict__ bound,
+                             uint32_t* __restrict__ light) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  uint64_t b = 0;
+  for (uint64_t j = toff[v]; j < toff[v + 1]; ++j) b += lambda[tch[j]];
+  bound[v] = b;
+  light[v] = (b <= GAIN_LIGHT || k <= GAIN_LIGHT) ? 1u : 0u;
+}
+
+__global__ void k_split(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint32_t n,
+                        uint32_t* __restrict__ a, uint32_t* __restrict__ b) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  if (flag[v]) a[pos[v]] = v;
+  else b[v - pos[v]] = v;
+}
+
+__global__ void k_heavy_tabcap(const uint32_t* __restrict__ list, const uint64_t* __restrict__ bound, uint32_t m,
+                               uint32_t k, uint64_t* __restrict__ cap) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint64_t b = bound[list[i]];
+  if (b > k) b = k;
+  b = 2 * b + 64;
+  uint64_t c = 1;
+  while (c < b) c <<= 1;
+  cap[i] = c;
+}
+
+// ---- 3. ranking -------------------------------------------------------------------------
+__global__ void k_rank_keys_gain(const uint32_t* __restrict__ moves, const int64_t* __restrict__ gains, uint32_t n,
+                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  keys[v] = moves[v] == NONE ? 0ull : ~ordered_i64(gains[v]);
+  vals[v] = v;
+}
+__global__ void k_invalid_keys(const uint32_t* __restrict__ moves, const uint32_t* __restrict__ vals, uint32_t n,
+                               uint32_t* __restrict__ keys) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = moves[vals[i]] == NONE ? 1u : 0u;
+}
+__global__ void k_scatter_ranks(const uint32_t* __restrict__ vals, uint32_t n, uint32_t* __restrict__ ranks) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) ranks[vals[r]] = r;
+}
+__global__ void k_count_valid(const uint32_t* __restrict__ moves, uint32_t n, uint32_t* cnt) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t ok = (v < n && moves[v] != NONE) ? 1u : 0u;
+  uint64_t m = __ballot(ok);
+  if (lane_id() == 0 && m) atomicAdd(cnt, (uint32_t)__popcll(m));
+}
+
+// chaining: edges in sorted space j (by source partition asc, valid first, gain desc)
+struct ChainArrays {
+  const uint32_t* orig;  // j -> node
+  uint32_t* src;
+  uint32_t* dst;
+  uint32_t* sz;
+  int64_t* w;
+  uint32_t* ok;
+  uint32_t* ob;
+  uint32_t* oe;
+  uint32_t* next;
+  uint32_t* prev;
+  uint32_t* choice;
+  unsigned long long* claim;
+};
+
+__global__ void k_chain_sortkeys(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
+                                 const int64_t* __restrict__ gains, uint32_t n, uint64_t* __restrict__ k1,
+                                 uint32_t* __restrict__ vals) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  k1[v] = moves[v] == NONE ? 0ull : ~ordered_i64(gains[v]);
+  vals[v] = v;
+}
+// second (more significant) pass key: (source partition << 1) | invalid
+__global__ void k_chain_sortkeys2(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
+                                  const uint32_t* __restrict__ vals, uint32_t n, uint32_t* __restrict__ k2) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t v = vals[i];
+  k2[i] = (part[v] << 1) | (moves[v] == NONE ? 1u : 0u);
+}
+__global__ void k_chain_gather(ChainArrays c, const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
+                               const int64_t* __restrict__ gains, const uint32_t* __restrict__ sizes, uint32_t n) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t v = c.orig[j];
+  c.src[j] = part[v];
+  c.dst[j] = moves[v];
+  c.sz[j] = sizes[v];
+  c.w[j] = gains[v];
+  c.ok[j] = moves[v] != NONE;
+  c.next[j] = NONE;
+  c.prev[j] = NONE;
+}
+__global__ void k_chain_ranges(ChainArrays c, const uint64_t* __restrict__ poff, uint32_t n) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  if (!c.ok[j]) {
+    c.ob[j] = 0;
+    c.oe[j] = 0;
+    return;
+  }
+  const uint32_t q = c.dst[j];
+  c.ob[j] = (uint32_t)poff[q];
+  c.oe[j] = (uint32_t)poff[q + 1];
+}
+__device__ __forceinline__ int size_bucket(uint32_t s) { return s == 0 ? 0 : 31 - __clz(s); }
+
+__global__ void k_chain_propose(ChainArrays c, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t choice = NONE;
+  c.claim[i] = 0ull;
+  if (c.next[i] == NONE && c.ob[i] < c.oe[i]) {
+    const uint32_t si = c.sz[i];
+    const int bi = size_bucket(si);
+    const uint32_t lim = min(c.oe[i], c.ob[i] + 256u);
+    int64_t bw = 0;
+    uint32_t bd = 0;
+    for (uint32_t j = c.ob[i]; j < lim; ++j) {
+      if (j == i || c.prev[j] != NONE || !c.ok[j]) continue;
+      const uint32_t sj = c.sz[j];
+      if (abs(bi - size_bucket(sj)) > 1) continue;
+      const uint32_t d = si > sj ? si - sj : sj - si;
+      const int64_t wj = c.w[j];
+      if (choice == NONE || wj > bw || (wj == bw && d < bd)) {
+        choice = j;
+        bw = wj;
+        bd = d;
+      }
+    }
+  }
+  c.choice[i] = choice;
+}
+__global__ void k_chain_resolve(ChainArrays c, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = c.choice[i];
+  if (s == NONE || c.prev[s] != NONE) return;
+  const uint32_t d = c.sz[i] > c.sz[s] ? c.sz[i] - c.sz[s] : c.sz[s] - c.sz[i];
+  const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - d) << 32) | (0xFFFFFFFFu - i);
+  atomicMax(&c.claim[s], key);
+}
+__global__ void k_chain_commit(ChainArrays c, uint32_t n) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const unsigned long long k = c.claim[s];
+  if (k == 0ull || c.prev[s] != NONE) return;
+  const uint32_t p = 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFull);
+  c.prev[s] = p;
+  if (c.next[p] == NONE) c.next[p] = s;
+}
+__global__ void k_chain_comp(ChainArrays c, uint32_t n, uint32_t* __restrict__ comp, uint32_t* __restrict__ pos,
+                             uint32_t* __restrict__ cflag, long long* __restrict__ cw,
+                             uint32_t* __restrict__ cinv) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t cur = i, mn = i, cc = NONE, ps = 0;
+  for (uint32_t d = 0; d < 256; ++d) {
+    const uint32_t p = c.prev[cur];
+    if (p == NONE) {
+      cc = cur;
+      ps = d;
+      break;
+    }
+    cur = p;
+    mn = cur < mn ? cur : mn;
+  }
+  if (cc == NONE) {
+    cc = mn;
+    ps = 0;
+  }
+  comp[i] = cc;
+  pos[i] = ps;
+  cflag[cc] = 1u;
+  if (c.ok[i]) atomicAdd((unsigned long long*)&cw[cc], (unsigned long long)c.w[i]);  // two's complement sum
+  else atomicOr(&cinv[cc], 1u);
+}
+__global__ void k_comp_list(const uint32_t* __restrict__ cflag, const uint64_t* __restrict__ cpos, uint32_t n,
+                            const long long* __restrict__ cw, const uint32_t* __restrict__ cinv,
+                            uint32_t* __restrict__ clist, uint64_t* __restrict__ ckey, uint32_t* __restrict__ cinvk) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || !cflag[j]) return;
+  const uint64_t o = cpos[j];
+  clist[o] = j;
+  ckey[o] = ~ordered_i64((int64_t)cw[j]);
+  cinvk[o] = cinv[j];
+}
+__global__ void k_comp_rank(const uint32_t* __restrict__ clist, uint32_t m, uint32_t* __restrict__ crank) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < m) crank[clist[r]] = r;
+}
+__global__ void k_edge_keys(const uint32_t* __restrict__ comp, const uint32_t* __restrict__ pos,
+                            const uint32_t* __restrict__ crank, uint32_t n, int jb, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  keys[j] = ((uint64_t)crank[comp[j]] << (8 + jb)) | ((uint64_t)pos[j] << jb) | j;
+  vals[j] = j;
+}
+__global__ void k_edge_ranks(const uint32_t* __restrict__ sorted_j, const uint32_t* __restrict__ orig, uint32_t n,
+                             uint32_t* __restrict__ ranks) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) ranks[orig[sorted_j[r]]] = r;
+}
+
+// ---- 4. cascade (hedge-centric) ------------------------------------------------------------
+struct CascadeArgs {
+  const uint32_t* hedges;
+  const uint64_t* hoff;
+  const float* w;
+  const uint32_t* part;
+  const uint32_t* moves;
+  const uint32_t* ranks;
+  int incc;
+  unsigned long long* seq;
+};
+
+__device__ __forceinline__ int64_t cascade_term(int incc, int64_t W, int64_t cp, int64_t cq) {
+  int64_t t = 0;
+  if (incc) {
+    if (cp >= 1) t += (int64_t)((uint64_t)W / ((uint64_t)cp * (uint64_t)cp));
+  } else if (cp == 1) {
+    t += W;
+  }
+  if (cq == 0) t -= W;
+  return t;
+}
+
+__global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t* __restrict__ large_flag) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const uint64_t s = a.hoff[e], t = a.hoff[e + 1];
+  if (t - s > CAS_SMALL) {
+    large_flag[e] = 1u;
+    return;
+  }
+  large_flag[e] = 0u;
+  uint32_t pt[CAS_SMALL], mv[CAS_SMALL], rk[CAS_SMALL];
+  const uint32_t len = (uint32_t)(t - s);
+  bool any = false;
+  for (uint32_t i = 0; i < len; ++i) {
+    const uint32_t x = a.hedges[s + i];
+    pt[i] = a.part[x];
+    mv[i] = a.moves[x];
+    rk[i] = a.ranks[x];
+    any |= mv[i] != NONE;
+  }
+  if (!any) return;
+  const int64_t W = (int64_t)fixed_weight(a.w[e]);
+  for (uint32_t i = 0; i < len; ++i) {
+    if (mv[i] == NONE) continue;
+    const uint32_t p = pt[i], q = mv[i], r = rk[i];
+    int64_t cp = 0, cq = 0;
+    for (uint32_t j = 0; j < len; ++j) {
+      cp += pt[j] == p;
+      cq += pt[j] == q;
+      if (mv[j] != NONE && rk[j] < r) {
+        if (mv[j] == p) cp++;
+        else if (mv[j] == q) cq++;
+        if (pt[j] == p) cp--;
+        else if (pt[j] == q) cq--;
+      }
+    }
+    atomicAdd(&a.seq[r], (unsigned long long)cascade_term(a.incc, W, cp, cq));
+  }
+}
+
+__global__ void __launch_bounds__(64) k_cascade_large(CascadeArgs a, const uint32_t* __restrict__ list) {
+  __shared__ uint32_t s_pt[CAS_LDS], s_mv[CAS_LDS], s_rk[CAS_LDS];
+  const uint32_t e = list[blockIdx.x];
+  const uint32_t l = lane_id();
+  const uint64_t s = a.hoff[e];
+  const uint32_t len = (uint32_t)(a.hoff[e + 1] - s);
+  const bool in_lds = len <= CAS_LDS;
+  bool any = false;
+  if (in_lds) {
+    for (uint32_t i = l; i < len; i += 64) {
+      const uint32_t x = a.hedges[s + i];
+      s_pt[i] = a.part[x];
+      s_mv[i] = a.moves[x];
+      s_rk[i] = a.ranks[x];
+      any |= s_mv[i] != NONE;
+    }
+  } else {
+    for (uint32_t i = l; i < len; i += 64) any |= a.moves[a.hedges[s + i]] != NONE;
+  }
+  __syncthreads();
+  if (!__ballot(any)) return;
+  const int64_t W = (int64_t)fixed_weight(a.w[e]);
+  auto PT = [&](uint32_t j) { return in_lds ? s_pt[j] : a.part[a.hedges[s + j]]; };
+  auto MV = [&](uint32_t j) { return in_lds ? s_mv[j] : a.moves[a.hedges[s + j]]; };
+  auto RK = [&](uint32_t j) { return in_lds ? s_rk[j] : a.ranks[a.hedges[s + j]]; };
+  for (uint32_t i = l; i < len; i += 64) {
+    const uint32_t q = MV(i);
+    if (q == NONE) continue;
+    const uint32_t p = PT(i), r = RK(i);
+    int64_t cp = 0, cq = 0;
+    for (uint32_t j = 0; j < len; ++j) {
+      const uint32_t pj = PT(j), mj = MV(j);
+      cp += pj == p;
+      cq += pj == q;
+      if (mj != NONE && RK(j) < r) {
+        if (mj == p) cp++;
+        else if (mj == q) cq++;
+        if (pj == p) cp--;
+        else if (pj == q) cq--;
+      }
+    }
+    atomicAdd(&a.seq[r], (unsigned long long)cascade_term(a.incc, W, cp, cq));
+  }
+}
+
+// ---- 5. events ----------------------------------------------------------------------------
+__global__ void k_size_events(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
+                              const uint32_t* __restrict__ ranks, const uint32_t* __restrict__ sizes, uint32_t n, int rb,
+                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t q = moves[v];
+  if (q == NONE) return;
+  const uint32_t r = ranks[v];
+  keys[2 * (uint64_t)r] = ((uint64_t)part[v] << rb) | r;
+  vals[2 * (uint64_t)r] = (uint32_t)(-(int32_t)sizes[v]);
+  keys[2 * (uint64_t)r + 1] = ((uint64_t)q << rb) | r;
+  vals[2 * (uint64_t)r + 1] = sizes[v];
+}
+__global__ void k_to_i64(const uint32_t* __restrict__ vals, uint64_t m, int64_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = (int64_t)(int32_t)vals[i];
+}
+__global__ void k_size_flags(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                             const int64_t* __restrict__ cum, const uint64_t* __restrict__ poff, uint64_t m, int rb,
+                             const uint32_t* __restrict__ psizes, uint32_t omega, int32_t* sv) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t q = (uint32_t)(keys[i] >> rb);
+  const uint32_t r = (uint32_t)(keys[i] & (((uint64_t)1 << rb) - 1));
+  const uint64_t st = poff[q];
+  const int64_t before_seg = st == 0 ? 0 : cum[st - 1];
+  const int64_t c = cum[i] - before_seg;
+  const int64_t d = (int64_t)(int32_t)vals[i];
+  const int64_t cur = (int64_t)psizes[q] + c, prev = cur - d, om = (int64_t)omega;
+  const int64_t ex = (cur > om ? cur - om : 0) - (prev > om ? prev - om : 0);
+  atomicAdd(&sv[r], (int32_t)ex);
+}
+
+// inbound events: (partition, hedge, rank) +-1 for every inbound hedge of every move
+__global__ void k_inbound_event_counts(const uint32_t* __restrict__ moves, const uint32_t* __restrict__ inb, uint32_t n,
+                                       uint32_t* __restrict__ cnt) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) cnt[v] = moves[v] == NONE ? 0u : 2u * inb[v];
+}
+__global__ void k_inbound_events(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
+                                 const uint32_t* __restrict__ ranks, const uint32_t* __restrict__ tch,
+                                 const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
+                                 const uint64_t* __restrict__ eoff, uint32_t n, int eb, int rb,
+                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t q = moves[v];
+  if (q == NONE) return;
+  const uint32_t p = part[v], r = ranks[v];
+  uint64_t o = eoff[v];
+  for (uint64_t j = toff[v]; j < toff[v] + inb[v]; ++j) {
+    const uint64_t e = tch[j];
+    keys[o] = ((uint64_t)p << (eb + rb)) | (e << rb) | r;
+    vals[o] = 0xFFFFFFFFu;  // -1
+    keys[o + 1] = ((uint64_t)q << (eb + rb)) | (e << rb) | r;
+    vals[o + 1] = 1u;
+    o += 2;
+  }
+}
+// count of destination pins of hedge e in partition q (0 when absent)
+__device__ __forceinline__ int64_t dst_count(const uint64_t* hoff, const uint32_t* lambda, const uint32_t* pp_part,
+                                             const uint32_t* pp_dcnt, uint32_t e, uint32_t q) {
+  const uint64_t s = hoff[e];
+  const uint32_t m = lambda[e];
+  for (uint32_t i = 0; i < m; ++i)
+    if (pp_part[s + i] == q) return pp_dcnt[s + i];
+  return 0;
+}
+__global__ void k_inbound_transitions(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                      const int64_t* __restrict__ cum, uint64_t m, int eb, int rb,
+                                      const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ lambda,
+                                      const uint32_t* __restrict__ pp_part, const uint32_t* __restrict__ pp_dcnt,
+                                      uint32_t* __restrict__ tflag, int32_t* __restrict__ tdelta) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t key = keys[i];
+  const uint64_t grp = key >> rb;  // (partition, hedge)
+  // start of this (partition, hedge) run: binary search backwards
+  uint64_t lo = 0, hi = i;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if ((keys[mid] >> rb) < grp) lo = mid + 1;
+    else hi = mid;
+  }
+  const int64_t before = lo == 0 ? 0 : cum[lo - 1];
+  const uint32_t q = (uint32_t)(key >> (eb + rb));
+  const uint32_t e = (uint32_t)((key >> rb) & (((uint64_t)1 << eb) - 1));
+  const int64_t init = dst_count(hoff, lambda, pp_part, pp_dcnt, e, q);
+  const int64_t cur = init + (cum[i] - before);
+  const int64_t prev = cur - (int64_t)(int32_t)vals[i];
+  int32_t d = 0;
+  if (prev == 0 && cur > 0) d = 1;
+  else if (prev > 0 && cur == 0) d = -1;
+  tflag[i] = d != 0;
+  tdelta[i] = d;
+}
+__global__ void k_inbound_size_events(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ tflag,
+                                      const int32_t* __restrict__ tdelta, const uint64_t* __restrict__ tpos, uint64_t m,
+                                      int eb, int rb, uint64_t* __restrict__ okeys, uint32_t* __restrict__ ovals) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !tflag[i]) return;
+  const uint64_t key = keys[i];
+  const uint64_t q = key >> (eb + rb);
+  const uint64_t r = key & (((uint64_t)1 << rb) - 1);
+  okeys[tpos[i]] = (q << rb) | r;
+  ovals[tpos[i]] = (uint32_t)tdelta[i];
+}
+__global__ void k_inbound_flags(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                const int64_t* __restrict__ cum, const uint64_t* __restrict__ poff, uint64_t m, int rb,
+                                const uint32_t* __restrict__ isz, uint32_t delta, int32_t* iv) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t q = (uint32_t)(keys[i] >> rb);
+  const uint32_t r = (uint32_t)(keys[i] & (((uint64_t)1 << rb) - 1));
+  const uint64_t st = poff[q];
+  const int64_t before_seg = st == 0 ? 0 : cum[st - 1];
+  const int64_t cur = (int64_t)isz[q] + (cum[i] - before_seg);
+  const int64_t prev = cur - (int64_t)(int32_t)vals[i];
+  const bool was_valid = prev <= (int64_t)delta, is_valid = cur <= (int64_t)delta;
+  if (was_valid && !is_valid) atomicAdd(&iv[r], 1);
+  if (!was_valid && is_valid) atomicSub(&iv[r], 1);
+}
+
+// ---- 6. best prefix and apply ---------------------------------------------------------------
+__device__ __forceinline__ bool better(const Best& a, const Best& b) {  // a strictly better than b
+  if (b.r == NONE) return a.r != NONE;
+  if (a.r == NONE) return false;
+  if (a.ok != b.ok) return a.ok > b.ok;
+  if (!a.ok) return a.r < b.r;
+  if (a.sv != b.sv) return a.sv < b.sv;
+  if (a.seq != b.seq) return a.seq > b.seq;
+  return a.r < b.r;
+}
+__device__ Best block_best(Best x) {
+  __shared__ Best sb[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    Best y;
+    y.ok = __shfl_xor(x.ok, o, 64);
+    y.sv = __shfl_xor(x.sv, o, 64);
+    y.seq = __shfl_xor(x.seq, o, 64);
+    y.r = __shfl_xor(x.r, o, 64);
+    y.iv = __shfl_xor(x.iv, o, 64);
+    if (better(y, x)) x = y;
+  }
+  if (lane_id() == 0) sb[threadIdx.x / 64] = x;
+  __syncthreads();
+  Best b = sb[0];
+  for (int i = 1; i < (int)(blockDim.x / 64); ++i)
+    if (better(sb[i], b)) b = sb[i];
+  return b;
+}
+__global__ void __launch_bounds__(256) k_best_partial(const int64_t* __restrict__ seq, const int32_t* __restrict__ sv,
+                                                      const int32_t* __restrict__ iv, uint32_t V, Best* __restrict__ out) {
+  Best x;
+  x.r = NONE;
+  x.ok = 0;
+  x.sv = 0;
+  x.seq = 0;
+  x.iv = 0;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < V; r += gridDim.x * blockDim.x) {
+    Best y;
+    y.r = r;
+    y.ok = iv[r] == 0;
+    y.sv = sv[r];
+    y.seq = seq[r];
+    y.iv = iv[r];
+    if (better(y, x)) x = y;
+  }
+  Best b = block_best(x);
+  if (threadIdx.x == 0) out[blockIdx.x] = b;
+}
+__global__ void __launch_bounds__(256) k_best_final(const Best* __restrict__ part, uint32_t m, Best* __restrict__ out) {
+  Best x;
+  x.r = NONE;
+  x.ok = 0;
+  x.sv = 0;
+  x.seq = 0;
+  x.iv = 0;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x)
+    if (better(part[i], x)) x = part[i];
+  Best b = block_best(x);
+  if (threadIdx.x == 0) *out = b;
+}
+__global__ void k_apply(const uint32_t* __restrict__ moves, const uint32_t* __restrict__ ranks,
+                        const uint32_t* __restrict__ sizes, uint32_t n, uint32_t best, uint32_t* __restrict__ part,
+                        uint32_t* psizes) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t q = moves[v];
+  if (q == NONE || ranks[v] > best) return;
+  const uint32_t s = sizes[v];
+  atomicSub(&psizes[part[v]], s);
+  atomicAdd(&psizes[q], s);
+  part[v] = q;
+}
+
+__global__ void k_connectivity(const uint32_t* __restrict__ lambda, const float* __restrict__ w, uint32_t E,
+                               unsigned long long* out) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t c = 0;
+  if (e < E && lambda[e] > 0) c = (int64_t)fixed_weight(w[e]) * (int64_t)(lambda[e] - 1);
+  c = wave_sum(c);
+  if (lane_id() == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t m,
+                             uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = src[idx[i]];
+}
+
+__global__ void k_max_u32(const uint32_t* __restrict__ p, uint32_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = p[i] > m ? p[i] : m;
+  m = wave_max(m);
+  if (lane_id() == 0) atomicMax(out, m);
+}
+
+}  // namespace
+
+// ==========================================================================================
+// host side
+
+namespace {
+
+struct PartitionLists {
+  DBuf<uint32_t> part, cnt, dcnt, lambda, inbsz;
+};
+
+void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, PartitionLists& pl) {
+  const uint64_t P = L.e ? read_scalar(c, L.hoff + L.e) : 0;
+  pl.part = DBuf<uint32_t>(c, P);
+  pl.cnt = DBuf<uint32_t>(c, P);
+  pl.dcnt = DBuf<uint32_t>(c, P);
+  pl.lambda = DBuf<uint32_t>(c, L.e);
+  pl.inbsz = DBuf<uint32_t>(c, std::max<uint32_t>(L.k, 1));
+  memset_async(c, pl.inbsz.get(), 0, (uint64_t)L.k * sizeof(uint32_t));
+  DBuf<uint32_t> lflag(c, L.e);
+  HGP_LAUNCH(k_pp_small, ceil_div(L.e, 256), 256, 0, c, L.hedges, L.hoff, L.srcc, part, L.e, pl.part.get(), pl.cnt.get(),
+             pl.dcnt.get(), pl.lambda.get(), lflag.get(), pl.inbsz.get());
+  DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
+  exclusive_scan(c, lflag.get(), pos.get(), L.e);
+  const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.e);
+  if (nl == 0) return;
+  DBuf<uint32_t> large(c, nl), small(c, L.e - nl);
+  // k_split puts flagged (large) entries first list
+  HGP_LAUNCH(k_split, ceil_div(L.e, 256), 256, 0, c, lflag.get(), pos.get(), L.e, large.get(), small.get());
+  // distinct partitions per hedge <= k: a table of 2k slots (power of two) never fills
+  HGP_REQUIRE(L.k <= PP_LDS_CAP / 2, HGP_E_CAPACITY, "partition lists: more than 2048 partitions are not supported");
+  uint32_t cap = 128;
+  while (cap < 2ull * L.k + 64 && cap < PP_LDS_CAP) cap <<= 1;
+  HGP_LAUNCH(k_pp_large, nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, large.get(), cap,
+             pl.part.get(), pl.cnt.get(), pl.dcnt.get(), pl.lambda.get(), pl.inbsz.get());
+}
+
+uint32_t discount_of(uint32_t rep, uint32_t R) { return rep < R / 3 ? 1u : (rep < 2 * R / 3 ? 2u : NONE); }
+
+uint32_t mix32_host(uint32_t x) {
+  x ^= x >> 17;
+  x *= 0xED5AD4BBu;
+  x ^= x >> 11;
+  x *= 0xAC4C1B51u;
+  x ^= x >> 15;
+  x *= 0x31848BABu;
+  x ^= x >> 14;
+  return x;
+}
+
+void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int mode, uint32_t omega, uint32_t repeat,
+                   uint32_t discount, const uint32_t* part, const uint32_t* psizes, uint32_t* moves, int64_t* gains) {
+  // hash order of partitions for the non-adjacent fallback
+  std::vector<uint32_t> order(L.k);
+  std::iota(order.begin(), order.end(), 0u);
+  std::sort(order.begin(), order.end(),
+            [&](uint32_t a, uint32_t b) { return mix32_host(a + repeat) > mix32_host(b + repeat); });
+  DBuf<uint32_t> dorder(c, std::max<uint32_t>(L.k, 1));
+  copy_h2d(c, dorder.get(), order.data(), (uint64_t)L.k * sizeof(uint32_t));
+  GainArgs a;
+  a.tch = L.tch;
+  a.toff = L.toff;
+  a.hoff = L.hoff;
+  a.w = L.w;
+  a.part = part;
+  a.sizes = L.sizes;
+  a.psizes = psizes;
+  a.pp_part = pl.part.get();
+  a.pp_cnt = pl.cnt.get();
+  a.lambda = pl.lambda.get();
+  a.order = dorder.get();
+  a.k = L.k;
+  a.omega = omega;
+  a.discount = discount;
+  a.repeat = repeat;
+  a.incc = mode == HGP_MODE_INCC;
+  a.moves = moves;
+  a.gains = gains;
+  DBuf<uint64_t> bound(c, L.n);
+  DBuf<uint32_t> light(c, L.n);
+  HGP_LAUNCH(k_gain_bound, ceil_div(L.n, 256), 256, 0, c, L.tch, L.toff, pl.lambda.get(), L.n, L.k, bound.get(),
+             light.get());
+  DBuf<uint64_t> pos(c, (uint64_t)L.n + 1);
+  exclusive_scan(c, light.get(), pos.get(), L.n);
+  const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.n);
+  const uint32_t nh = L.n - nl;
+  DBuf<uint32_t> ll(c, nl), hl(c, nh);
+  HGP_LAUNCH(k_split, ceil_div(L.n, 256), 256, 0, c, light.get(), pos.get(), L.n, ll.get(), hl.get());
+  HGP_LAUNCH(k_gains_light, ceil_div(nl, 4), 256, 0, c, a, ll.get(), nl);
+  sync(c);  // `order` host buffer
+  if (nh == 0) return;
+  DBuf<uint64_t> caps(c, nh);
+  HGP_LAUNCH(k_heavy_tabcap, ceil_div(nh, 256), 256, 0, c, hl.get(), bound.get(), nh, L.k, caps.get());
+  std::vector<uint64_t> hc(nh);
+  copy_d2h(c, hc.data(), caps.get(), (uint64_t)nh * sizeof(uint64_t));
+  uint32_t b0 = 0;
+  while (b0 < nh) {
+    uint64_t tot = 0;
+    uint32_t b1 = b0;
+    while (b1 < nh && (b1 == b0 || tot + hc[b1] <= HEAVY_BATCH_SLOTS)) tot += hc[b1++];
+    std::vector<uint64_t> off(b1 - b0 + 1, 0);
+    for (uint32_t i = b0; i < b1; ++i) off[i - b0 + 1] = off[i - b0] + hc[i];
+    DBuf<uint64_t> doff(c, off.size());
+    copy_h2d(c, doff.get(), off.data(), off.size() * sizeof(uint64_t));
+    DBuf<uint32_t> gk(c, tot);
+    DBuf<int64_t> gc(c, tot);
+    HGP_LAUNCH(k_gains_heavy, b1 - b0, 256, 0, c, a, hl.get() + b0, doff.get(), gk.get(), gc.get());
+    sync(c);
+    b0 = b1;
+  }
+}
+
+void rank_by_gain(Ctx& c, uint32_t n, const uint32_t* moves, const int64_t* gains, uint32_t* ranks) {
+  DBuf<uint64_t> keys(c, n);
+  DBuf<uint32_t> vals(c, n), k2(c, n);
+  HGP_LAUNCH(k_rank_keys_gain, ceil_div(n, 256), 256, 0, c, moves, gains, n, keys.get(), vals.get());
+  radix_sort(c, keys.get(), vals.get(), n, 64);
+  HGP_LAUNCH(k_invalid_keys, ceil_div(n, 256), 256, 0, c, moves, vals.get(), n, k2.get());
+  radix_sort(c, k2.get(), vals.get(), n, 1);
+  HGP_LAUNCH(k_scatter_ranks, ceil_div(n, 256), 256, 0, c, vals.get(), n, ranks);
+}
+
+void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, const uint32_t* moves, const int64_t* gains,
+                      const uint32_t* sizes, uint32_t* ranks) {
+  const uint32_t g = ceil_div(n, 256);
+  // sorted space: stable by (src partition asc, valid first, gain desc), ties by node id
+  DBuf<uint64_t> k1(c, n);
+  DBuf<uint32_t> orig(c, n), k2(c, n);
+  HGP_LAUNCH(k_chain_sortkeys, g, 256, 0, c, part, moves, gains, n, k1.get(), orig.get());
+  radix_sort(c, k1.get(), orig.get(), n, 64);
+  HGP_LAUNCH(k_chain_sortkeys2, g, 256, 0, c, part, moves, orig.get(), n, k2.get());
+  radix_sort(c, k2.get(), orig.get(), n, bit_width(2ull * k + 1));
+  k1.reset();
+  DBuf<uint32_t> src(c, n), dst(c, n), sz(c, n), ok(c, n), ob(c, n), oe(c, n), next(c, n), prev(c, n), choice(c, n);
+  DBuf<int64_t> w(c, n);
+  DBuf<unsigned long long> claim(c, n);
+  ChainArrays ca{orig.get(), src.get(), dst.get(), sz.get(), w.get(), ok.get(), ob.get(), oe.get(),
+                 next.get(), prev.get(), choice.get(), claim.get()};
+  HGP_LAUNCH(k_chain_gather, g, 256, 0, c, ca, part, moves, gains, sizes, n);
+  DBuf<uint64_t> poff(c, (uint64_t)k + 1);
+  // src partitions of sorted space are the k2 keys >> 1
+  segment_offsets(c, k2.get(), 1, n, k, poff.get());
+  HGP_LAUNCH(k_chain_ranges, g, 256, 0, c, ca, poff.get(), n);
+  for (int it = 0; it < 4; ++it) {
+    HGP_LAUNCH(k_chain_propose, g, 256, 0, c, ca, n);
+    HGP_LAUNCH(k_chain_resolve, g, 256, 0, c, ca, n);
+    HGP_LAUNCH(k_chain_commit, g, 256, 0, c, ca, n);
+  }
+  DBuf<uint32_t> comp(c, n), pos(c, n), cflag(c, n), cinv(c, n);
+  DBuf<long long> cw(c, n);
+  memset_async(c, cflag.get(), 0, (uint64_t)n * 4);
+  memset_async(c, cinv.get(), 0, (uint64_t)n * 4);
+  memset_async(c, cw.get(), 0, (uint64_t)n * 8);
+  HGP_LAUNCH(k_chain_comp, g, 256, 0, c, ca, n, comp.get(), pos.get(), cflag.get(), cw.get(), cinv.get());
+  DBuf<uint64_t> cpos(c, (uint64_t)n + 1);
+  exclusive_scan(c, cflag.get(), cpos.get(), n);
+  const uint32_t m = (uint32_t)read_scalar(c, cpos.get() + n);
+  DBuf<uint32_t> clist(c, m), cinvk(c, m), crank(c, n);
+  DBuf<uint64_t> ckey(c, m);
+  HGP_LAUNCH(k_comp_list, g, 256, 0, c, cflag.get(), cpos.get(), n, cw.get(), cinv.get(), clist.get(), ckey.get(),
+             cinvk.get());
+  // components by (valid first, summed gain desc, head id asc): the list starts in ascending
+  // head id; a stable sort on the summed gain, then a stable 1-bit pass that moves the
+  // components made of an invalid move (summed gain 0) behind every valid one
+  radix_sort(c, ckey.get(), clist.get(), m, 64);
+  DBuf<uint32_t> inv2(c, m);
+  HGP_LAUNCH(k_gather_u32, ceil_div(m, 256), 256, 0, c, cinv.get(), clist.get(), m, inv2.get());
+  radix_sort(c, inv2.get(), clist.get(), m, 1);
+  HGP_LAUNCH(k_comp_rank, ceil_div(m, 256), 256, 0, c, clist.get(), m, crank.get());
+  // final order: key (component rank, position, sorted index)
+  const int jb = std::max(1, bit_width(n));
+  HGP_REQUIRE(bit_width(m) + 8 + jb <= 64, HGP_E_CAPACITY, "chaining: order key exceeds 64 bits");
+  DBuf<uint64_t> ek(c, n);
+  DBuf<uint32_t> ev(c, n);
+  HGP_LAUNCH(k_edge_keys, g, 256, 0, c, comp.get(), pos.get(), crank.get(), n, jb, ek.get(), ev.get());
+  radix_sort(c, ek.get(), ev.get(), n, bit_width(m) + 8 + jb);
+  HGP_LAUNCH(k_edge_ranks, g, 256, 0, c, ev.get(), orig.get(), n, ranks);
+}
+
+struct RoundState {
+  DBuf<uint32_t> moves, ranks;
+  DBuf<int64_t> gains;
+  DBuf<unsigned long long> seq;  // int64 values, indexed by rank
+  DBuf<int32_t> sv, iv;
+  uint32_t V = 0;
+  Best best;
+};
+
+void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t delta, uint32_t R, uint32_t rep,
+                   bool chainup, const uint32_t* part, const uint32_t* psizes, RoundState& st) {
+  const uint32_t n = L.n;
+  const uint32_t g = ceil_div(n, 256);
+  const bool incc = mode == HGP_MODE_INCC;
+  PartitionLists pl;
+  build_partition_lists(c, L, part, pl);
+  st.moves = DBuf<uint32_t>(c, n);
+  st.gains = DBuf<int64_t>(c, n);
+  st.ranks = DBuf<uint32_t>(c, n);
+  compute_gains(c, L, pl, mode, omega, rep, discount_of(rep, R), part, psizes, st.moves.get(), st.gains.get());
+  if (chainup) rank_by_gain(c, n, st.moves.get(), st.gains.get(), st.ranks.get());
+  else rank_by_chaining(c, n, L.k, part, st.moves.get(), st.gains.get(), L.sizes, st.ranks.get());
+  {
+    DBuf<uint32_t> cnt(c, 1);
+    memset_async(c, cnt.get(), 0, 4);
+    HGP_LAUNCH(k_count_valid, g, 256, 0, c, st.moves.get(), n, cnt.get());
+    st.V = read_scalar(c, cnt.get());
+  }
+  const uint32_t V = st.V;
+  st.seq = DBuf<unsigned long long>(c, std::max<uint32_t>(V, 1));
+  st.sv = DBuf<int32_t>(c, std::max<uint32_t>(V, 1));
+  st.iv = DBuf<int32_t>(c, std::max<uint32_t>(V, 1));
+  st.best.r = NONE;
+  if (V == 0) return;
+  memset_async(c, st.seq.get(), 0, (uint64_t)V * 8);
+  memset_async(c, st.sv.get(), 0, (uint64_t)V * 4);
+  memset_async(c, st.iv.get(), 0, (uint64_t)V * 4);
+  // cascade
+  {
+    CascadeArgs a{L.hedges, L.hoff, L.w, part, st.moves.get(), st.ranks.get(), incc ? 1 : 0, st.seq.get()};
+    DBuf<uint32_t> lflag(c, L.e);
+    HGP_LAUNCH(k_cascade_small, ceil_div(L.e, 256), 256, 0, c, a, L.e, lflag.get());
+    DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
+    exclusive_scan(c, lflag.get(), pos.get(), L.e);
+    const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.e);
+    if (nl) {
+      DBuf<uint32_t> large(c, nl), small(c, L.e - nl);
+      HGP_LAUNCH(k_split, ceil_div(L.e, 256), 256, 0, c, lflag.get(), pos.get(), L.e, large.get(), small.get());
+      HGP_LAUNCH(k_cascade_large, nl, 64, 0, c, a, large.get());
+    }
+    inclusive_scan(c, (const int64_t*)st.seq.get(), (int64_t*)st.seq.get(), V);
+  }
+  const int rb = std::max(1, bit_width(V - 1));
+  const int kb = std::max(1, bit_width(L.k));
+  // size events
+  {
+    const uint64_t m = 2ull * V;
+    DBuf<uint64_t> keys(c, m), poff(c, (uint64_t)L.k + 1);
+    DBuf<uint32_t> vals(c, m);
+    DBuf<int64_t> cum(c, m);
+    HGP_LAUNCH(k_size_events, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), L.sizes, n, rb, keys.get(), vals.get());
+    radix_sort(c, keys.get(), vals.get(), m, rb + kb);
+    HGP_LAUNCH(k_to_i64, ceil_div(m, 256), 256, 0, c, vals.get(), m, cum.get());
+    inclusive_scan(c, cum.get(), cum.get(), m);
+    segment_offsets(c, keys.get(), rb, m, L.k, poff.get());
+    HGP_LAUNCH(k_size_flags, ceil_div(m, 256), 256, 0, c, keys.get(), vals.get(), cum.get(), poff.get(), m, rb, psizes,
+               omega, st.sv.get());
+    inclusive_scan(c, st.sv.get(), st.sv.get(), V);
+  }
+  // inbound-set events (INCC only: with k-way's Delta = INT32_MAX every prefix is valid)
+  if (incc) {
+    const int eb = std::max(1, bit_width(L.e));
+    HGP_REQUIRE(kb + eb + rb <= 64, HGP_E_CAPACITY, "inbound events: key exceeds 64 bits");
+    DBuf<uint32_t> ecnt(c, n);
+    DBuf<uint64_t> eoff(c, (uint64_t)n + 1);
+    HGP_LAUNCH(k_inbound_event_counts, g, 256, 0, c, st.moves.get(), L.inb, n, ecnt.get());
+    exclusive_scan(c, ecnt.get(), eoff.get(), n);
+    const uint64_t m = read_scalar(c, eoff.get() + n);
+    if (m) {
+      DBuf<uint64_t> keys(c, m);
+      DBuf<uint32_t> vals(c, m);
+      HGP_LAUNCH(k_inbound_events, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), L.tch, L.toff, L.inb, eoff.get(), n,
+                 eb, rb, keys.get(), vals.get());
+      radix_sort(c, keys.get(), vals.get(), m, kb + eb + rb);
+      DBuf<int64_t> cum(c, m);
+      HGP_LAUNCH(k_to_i64, ceil_div(m, 256), 256, 0, c, vals.get(), m, cum.get());
+      inclusive_scan(c, cum.get(), cum.get(), m);
+      DBuf<uint32_t> tflag(c, m);
+      DBuf<int32_t> tdelta(c, m);
+      HGP_LAUNCH(k_inbound_transitions, ceil_div(m, 256), 256, 0, c, keys.get(), vals.get(), cum.get(), m, eb, rb, L.hoff,
+                 pl.lambda.get(), pl.part.get(), pl.dcnt.get(), tflag.get(), tdelta.get());
+      DBuf<uint64_t> tpos(c, m + 1);
+      exclusive_scan(c, tflag.get(), tpos.get(), m);
+      const uint64_t T = read_scalar(c, tpos.get() + m);
+      if (T) {
+        DBuf<uint64_t> ok(c, T), poff(c, (uint64_t)L.k + 1);
+        DBuf<uint32_t> ov(c, T);
+        HGP_LAUNCH(k_inbound_size_events, ceil_div(m, 256), 256, 0, c, keys.get(), tflag.get(), tdelta.get(), tpos.get(),
+                   m, eb, rb, ok.get(), ov.get());
+        keys.reset();
+        radix_sort(c, ok.get(), ov.get(), T, rb + kb);
+        DBuf<int64_t> cum2(c, T);
+        HGP_LAUNCH(k_to_i64, ceil_div(T, 256), 256, 0, c, ov.get(), T, cum2.get());
+        inclusive_scan(c, cum2.get(), cum2.get(), T);
+        segment_offsets(c, ok.get(), rb, T, L.k, poff.get());
+        HGP_LAUNCH(k_inbound_flags, ceil_div(T, 256), 256, 0, c, ok.get(), ov.get(), cum2.get(), poff.get(), T, rb,
+                   pl.inbsz.get(), delta, st.iv.get());
+      }
+    }
+    inclusive_scan(c, st.iv.get(), st.iv.get(), V);
+  }
+  // best prefix
+  {
+    const uint32_t blocks = std::min<uint32_t>(ceil_div(V, 256), 1024);
+    DBuf<Best> partial(c, blocks), out(c, 1);
+    HGP_LAUNCH(k_best_partial, blocks, 256, 0, c, (const int64_t*)st.seq.get(), st.sv.get(), st.iv.get(), V,
+               partial.get());
+    HGP_LAUNCH(k_best_final, 1, 256, 0, c, partial.get(), blocks, out.get());
+    st.best = read_scalar(c, out.get());
+  }
+}
+
+}  // namespace
+
+void refine(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t delta, uint32_t R, bool final_counts,
+            uint32_t* part, uint32_t* psizes, uint32_t* pinbound) {
+  bool chainup = false;
+  for (uint32_t rep = 0; rep < R; ++rep) {
+    RoundState st;
+    analyze_round(c, L, mode, omega, delta, R, rep, chainup, part, psizes, st);
+    int32_t sv = 1, iv = 1;
+    int64_t gain = 0;
+    bool apply = false;
+    if (st.best.r != NONE) {
+      sv = st.best.sv;
+      iv = st.best.iv;
+      gain = st.best.seq;
+      apply = sv <= 0 && iv <= 0 && gain >= 0;
+    }
+    if (c.verbose >= 2)
+      fprintf(stderr, "  refine n=%u rep=%u V=%u best=%d sv=%d iv=%d gain=%lld %s\n", L.n, rep, st.V, (int)st.best.r, sv,
+              iv, (long long)gain, apply ? "apply" : "skip");
+    if (apply) {
+      HGP_LAUNCH(k_apply, ceil_div(L.n, 256), 256, 0, c, st.moves.get(), st.ranks.get(), L.sizes, L.n, st.best.r, part,
+                 psizes);
+    } else {
+      // schedule of sources/refinement.cu:539-542
+      if (sv > 0 && !chainup) chainup = true;
+      else if (rep < R / 3) rep = R / 2;
+      else if (rep < 2 * R / 3) rep = 2 * R / 3;
+      else rep = R;
+    }
+  }
+  if (final_counts && pinbound) inbound_sizes(c, L.hedges, L.hoff, L.srcc, part, L.e, L.k, pinbound);
+}
+
+uint32_t refine_analyze(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t delta, uint32_t R,
+                        uint32_t fm_repeat, bool chainup, const uint32_t* part, const uint32_t* psizes,
+                        const AnalyzeOut& out) {
+  RoundState st;
+  analyze_round(c, L, mode, omega, delta, R, fm_repeat, chainup, part, psizes, st);
+  const uint32_t n = L.n, V = st.V;
+  if (out.moves) copy_d2h(c, out.moves, st.moves.get(), (uint64_t)n * 4);
+  if (out.gains) copy_d2h(c, out.gains, st.gains.get(), (uint64_t)n * 8);
+  if (out.ranks) copy_d2h(c, out.ranks, st.ranks.get(), (uint64_t)n * 4);
+  if (out.seq) {
+    std::fill(out.seq, out.seq + n, 0);
+    copy_d2h(c, out.seq, st.seq.get(), (uint64_t)V * 8);
+  }
+  if (out.sviol) {
+    std::fill(out.sviol, out.sviol + n, 0);
+    copy_d2h(c, out.sviol, st.sv.get(), (uint64_t)V * 4);
+  }
+  if (out.iviol) {
+    std::fill(out.iviol, out.iviol + n, 0);
+    copy_d2h(c, out.iviol, st.iv.get(), (uint64_t)V * 4);
+  }
+  return st.best.r;
+}
+
+void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* part,
+                   uint32_t e, uint32_t k, uint32_t* out) {
+  RefineLevel L{};
+  L.e = e;
+  L.k = k;
+  L.hedges = hedges;
+  L.hoff = hoff;
+  L.srcc = srcc;
+  PartitionLists pl;
+  build_partition_lists(c, L, part, pl);
+  copy_d2d(c, out, pl.inbsz.get(), (uint64_t)k * 4);
+}
+
+int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* part,
+                           uint32_t e) {
+  if (e == 0) return 0;
+  // partitions in use: max id + 1 (needed to size the per-partition inbound counters)
+  const uint64_t P = read_scalar(c, hoff + e);
+  DBuf<uint32_t> mx(c, 1);
+  memset_async(c, mx.get(), 0, 4);
+  // max over the partition ids of the pins
+  DBuf<uint32_t> pinpart(c, P);
+  HGP_LAUNCH(k_gather_u32, ceil_div(P, 256), 256, 0, c, part, hedges, (uint32_t)P, pinpart.get());
+  HGP_LAUNCH(k_max_u32, std::min<uint32_t>(ceil_div(P, 256), 2048), 256, 0, c, pinpart.get(), (uint32_t)P, mx.get());
+  const uint32_t k = read_scalar(c, mx.get()) + 1;
+  DBuf<uint32_t> srcc0(c, e);
+  memset_async(c, srcc0.get(), 0, (uint64_t)e * 4);
+  RefineLevel L{};
+  L.e = e;
+  L.k = k;
+  L.hedges = hedges;
+  L.hoff = hoff;
+  L.srcc = srcc0.get();
+  PartitionLists pl;
+  build_partition_lists(c, L, part, pl);
+  DBuf<unsigned long long> acc(c, 1);
+  memset_async(c, acc.get(), 0, 8);
+  HGP_LAUNCH(k_connectivity, ceil_div(e, 256), 256, 0, c, pl.lambda.get(), w, e, acc.get());
+  return (int64_t)read_scalar(c, acc.get());
+}
+
+}  // namespace hgp

@@ -1,0 +1,67 @@
+// stages.hpp - host-side entry points of the pipeline stages (device pointers in/out).
+// Outputs returned through `T*&` are allocated with dmalloc on the context and owned by
+// the caller (released with dfree), matching the reference's step-function contract.
+#pragma once
+
+#include "common.hpp"
+
+namespace hgp {
+
+// ---- construction (construct.hip) -----------------------------------------------------
+void build_touching(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, uint32_t n,
+                    uint32_t e, uint64_t& tsize, uint32_t*& tch, uint64_t*& toff, uint32_t*& inb);
+void build_neighbors(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* tch, const uint64_t* toff,
+                     uint32_t n, uint64_t& nsize, uint32_t*& nbr, uint64_t*& noff);
+void coarsen_neighbors(Ctx& c, const uint32_t* nbr, const uint64_t* noff, const uint32_t* groups, uint32_t n,
+                       uint32_t new_n, uint64_t& nsize, uint32_t*& cnbr, uint64_t*& cnoff);
+void coarsen_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* groups,
+                    uint32_t e, uint64_t hsize, uint32_t new_n, uint64_t& csize, uint32_t*& chedges, uint64_t*& choff,
+                    uint32_t*& csrcc);
+void uncoarsen(Ctx& c, const uint32_t* groups, const uint32_t* cpart, uint32_t n, uint32_t* part);
+
+// ---- coarsening (coarsen.hip) ---------------------------------------------------------
+void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* tch,
+                const uint64_t* toff, const uint32_t* inb, const float* w, const uint32_t* sizes, uint32_t n,
+                uint32_t e, uint32_t cnc, uint32_t omega, uint32_t delta, uint32_t* pairs, uint32_t* scores);
+void group_nodes(Ctx& c, const uint32_t* inb, const uint32_t* pairs, const uint32_t* scores, const uint32_t* sizes,
+                 uint32_t n, uint32_t cnc, uint32_t omega, uint32_t delta, uint32_t& new_n, uint32_t*& groups,
+                 uint32_t*& gsizes, uint32_t** ungroups, uint64_t** uoff, uint32_t* raw_out = nullptr);
+
+// ---- refinement (refine.hip) ------------------------------------------------------------
+struct RefineLevel {
+  uint32_t n, e, k;
+  const uint32_t* hedges;
+  const uint64_t* hoff;
+  const uint32_t* srcc;
+  const uint32_t* tch;
+  const uint64_t* toff;
+  const uint32_t* inb;
+  const float* w;
+  const uint32_t* sizes;
+};
+struct AnalyzeOut {  // optional host copies of the intermediates
+  uint32_t* moves = nullptr;
+  int64_t* gains = nullptr;
+  uint32_t* ranks = nullptr;
+  int64_t* seq = nullptr;
+  int32_t* sviol = nullptr;
+  int32_t* iviol = nullptr;
+};
+void refine(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t delta, uint32_t repeats,
+            bool final_counts, uint32_t* part, uint32_t* psizes, uint32_t* pinbound);
+uint32_t refine_analyze(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t delta, uint32_t repeats,
+                        uint32_t fm_repeat, bool chainup, const uint32_t* part, const uint32_t* psizes,
+                        const AnalyzeOut& out);
+void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* part,
+                   uint32_t e, uint32_t k, uint32_t* out);
+int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* part,
+                           uint32_t e);
+
+// ---- post-processing (postproc.hip) -----------------------------------------------------
+void greedy_merge_groups(Ctx& c, const uint32_t* sizes, const uint32_t* inb, uint32_t n, uint32_t ngroups,
+                         uint32_t omega, uint32_t delta, uint32_t* groups, uint32_t* gsizes, uint32_t& new_ngroups);
+void merge_small_partitions(Ctx& c, const uint32_t* psizes, const uint32_t* pinbound, uint32_t n, uint32_t k,
+                            uint32_t omega, uint32_t delta, uint32_t* part);
+void relabel(Ctx& c, uint32_t n, uint32_t* part, uint32_t& new_k);
+
+}  // namespace hgp

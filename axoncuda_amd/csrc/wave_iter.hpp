@@ -1,0 +1,119 @@
+// wave_iter.hpp - wave-cooperative traversal of "all pins of all hedges touching a node".
+//
+// The hedges of a touching list have very different sizes (2 .. 1000+ pins); giving one
+// hedge to one lane or one hedge to the whole wave wastes most lanes.  Instead the wave
+// takes 64 touching entries at a time, prefix-sums their hedge sizes across lanes, and
+// then walks the flattened pin range 64 pins per step; each lane finds the hedge that
+// owns its pin with a 6-step shuffle binary search.  Every lane calls the visitor on every
+// step (with `act` false past the end) so visitors may use wave-wide ballots.
+#pragma once
+
+#include "common.hpp"
+
+namespace hgp {
+
+struct PinVisit {
+  bool act;         // lane holds a real pin
+  uint64_t j;       // index into the touching list
+  uint32_t e;       // hedge id
+  uint32_t pos;     // position of the pin inside the hedge
+  uint32_t len;     // hedge size
+  uint32_t pin;     // node id
+};
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, uint32_t src) {
+  uint32_t lo = __shfl((uint32_t)v, (int)src, 64);
+  uint32_t hi = __shfl((uint32_t)(v >> 32), (int)src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Generic form: hedge e owns the entry range [start(e), start(e) + length(e)) of some
+// per-hedge array; the visitor gets the flattened entry index in `pin` replaced by the
+// absolute entry index (field `abs`).
+struct EntryVisit {
+  bool act;
+  uint64_t j;    // index into the touching list
+  uint32_t e;    // hedge id
+  uint32_t pos;  // position inside the hedge's range
+  uint64_t abs;  // absolute entry index
+};
+
+template <typename StartF, typename LenF, typename F>
+__device__ __forceinline__ void wave_visit_entries(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
+                                                   StartF&& start_of, LenF&& len_of, F&& visit) {
+  const uint32_t l = lane_id();
+  for (uint64_t jb = t0; jb < t1; jb += 64) {
+    const uint64_t j = jb + l;
+    uint32_t e = 0, len = 0;
+    uint64_t s = 0;
+    if (j < t1) {
+      e = tch[j];
+      s = start_of(e);
+      len = len_of(e);
+    }
+    const uint32_t incl = wave_scan_incl(len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - len;
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+      const uint32_t fi = f0 + l;
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t step = 32; step > 0; step >>= 1) {
+        const uint32_t cand = k + step;
+        const uint32_t ex = __shfl(excl, (int)cand, 64);
+        if (ex <= fi) k = cand;
+      }
+      EntryVisit ev;
+      ev.act = fi < total;
+      ev.e = __shfl(e, (int)k, 64);
+      const uint64_t os = shfl_u64(s, k);
+      const uint32_t oex = __shfl(excl, (int)k, 64);
+      ev.j = jb + k;
+      ev.pos = fi - oex;
+      ev.abs = os + ev.pos;
+      visit(ev);
+    }
+  }
+}
+
+template <typename F>
+__device__ __forceinline__ void wave_visit_pins(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
+                                                const uint64_t* __restrict__ hoff,
+                                                const uint32_t* __restrict__ hedges, F&& visit) {
+  const uint32_t l = lane_id();
+  for (uint64_t jb = t0; jb < t1; jb += 64) {
+    const uint64_t j = jb + l;
+    uint32_t e = 0, len = 0;
+    uint64_t s = 0;
+    if (j < t1) {
+      e = tch[j];
+      s = hoff[e];
+      len = (uint32_t)(hoff[e + 1] - s);
+    }
+    const uint32_t incl = wave_scan_incl(len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - len;
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+      const uint32_t fi = f0 + l;
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t step = 32; step > 0; step >>= 1) {
+        const uint32_t cand = k + step;
+        const uint32_t ex = __shfl(excl, (int)cand, 64);
+        if (ex <= fi) k = cand;
+      }
+      PinVisit pv;
+      pv.act = fi < total;
+      pv.e = __shfl(e, (int)k, 64);
+      const uint64_t os = shfl_u64(s, k);
+      const uint32_t oex = __shfl(excl, (int)k, 64);
+      pv.len = __shfl(len, (int)k, 64);
+      pv.j = jb + k;
+      pv.pos = fi - oex;
+      pv.pin = pv.act ? hedges[os + pv.pos] : 0u;
+      visit(pv);
+    }
+  }
+}
+
+}  // namespace hgp

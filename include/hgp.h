@@ -1,0 +1,279 @@
+/*
+ * hgp.h - C ABI of the MI355X-native multilevel hypergraph partitioner.
+ *
+ * Every entry point replaces one host step of the reference `hgraph_gpu`
+ * pipeline (EMJzero/AxonCUDA).  The reference steps are C++ functions that take
+ * raw device pointers and return callee-allocated device buffers that the caller
+ * frees; the entry points below keep that contract with plain C types:
+ *   - `d_` pointers are device (HBM) pointers, `h_` pointers are host pointers;
+ *   - buffers returned through `T** d_out` are allocated by the library and are
+ *     released with hgp_free();
+ *   - buffers passed in as `T* d_out` are caller-allocated and sized as stated;
+ *   - every call returns an hgp_status; no call exits or aborts the process.
+ *     hgp_ctx_last_error() gives the message of the last failure.
+ * Constraints (node cap, inbound cap) are per-call arguments: there is no
+ * process-global device state, so concurrent jobs with different constraints
+ * can share one context.  All calls on one context are serialised on the
+ * context's stream (hgp_ctx_set_stream); a context is not thread-safe.
+ *
+ * Canonical orders (what the parity tests compare bit-exactly):
+ *   touching   : per node, inbound hedges ascending, then outbound ascending.
+ *   neighbours : per node, ascending node ids, the node itself excluded.
+ *   coarse hedge: sources ascending, then destinations descending.
+ */
+#ifndef HGP_H
+#define HGP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t hgp_status;
+#define HGP_OK 0
+#define HGP_E_INVALID 1    /* bad argument */
+#define HGP_E_HIP 2        /* HIP runtime error */
+#define HGP_E_NOMEM 3      /* device allocation failed / over the HBM budget */
+#define HGP_E_CAPACITY 4   /* an internal table overflowed (retry with larger -om) */
+#define HGP_E_IO 5         /* file could not be read or written / parse error */
+#define HGP_E_INFEASIBLE 6 /* the constraints cannot be met */
+#define HGP_E_NODEVICE 7   /* no usable gfx950 device */
+#define HGP_E_INTERNAL 8
+
+#define HGP_MODE_INCC 0 /* inbound-constrained minimal partitioning (-c / -m) */
+#define HGP_MODE_KWAY 1 /* k-way epsilon-balanced partitioning (-k) */
+
+typedef struct hgp_ctx hgp_ctx;
+typedef struct hgp_hgraph hgp_hgraph;
+
+/* Run configuration: the reference `runconfig` (includes/runconfig.hpp:32-51)
+ * plus the resolved constraint triple (sources/runconfig.cpp:186-217). */
+typedef struct hgp_params {
+  int32_t mode;                     /* HGP_MODE_* */
+  uint32_t max_nodes_per_part;      /* Omega */
+  uint32_t max_inbound_per_part;    /* Delta */
+  uint32_t max_parts;
+  uint32_t kway;                    /* -k k   (KWAY only) */
+  float epsilon;                    /* -k eps (KWAY only) */
+  uint32_t candidates_count;        /* -cnc, 1..16 (default 16) */
+  uint32_t refine_repeats;          /* -rfr (default 16) */
+  uint32_t save_memory_up_to_level; /* -smh (default 2) */
+  float oversized_multiplier;       /* -om (default 1.5) */
+  int32_t device_touching_construction; /* -dtc */
+  int32_t initial_partitions_merge;     /* -ipm */
+  int32_t verbose;                  /* -v 0..3 */
+} hgp_params;
+
+/* Per-run statistics filled by hgp_partition. */
+typedef struct hgp_run_stats {
+  double core_ms;       /* device-resident input -> partition on host */
+  double device_ms;     /* upload included */
+  double ip_ms;         /* initial partitioning (KWAY) */
+  uint32_t levels;      /* coarsening levels executed */
+  uint32_t num_partitions;
+  uint64_t total_pins;
+  uint64_t bytes_algorithmic; /* compulsory traffic model, summed over levels */
+  uint32_t level_nodes[64];
+  uint64_t level_pins[64];
+} hgp_run_stats;
+
+/* ---- context and device memory ------------------------------------------ */
+void hgp_default_params(hgp_params* p);
+hgp_status hgp_ctx_create(int device, uint64_t hbm_budget, hgp_ctx** out);
+void hgp_ctx_destroy(hgp_ctx* ctx);
+const char* hgp_ctx_last_error(const hgp_ctx* ctx);
+hgp_status hgp_ctx_set_stream(hgp_ctx* ctx, void* hip_stream);
+hgp_status hgp_synchronize(hgp_ctx* ctx);
+hgp_status hgp_malloc(hgp_ctx* ctx, uint64_t bytes, void** d_ptr);
+hgp_status hgp_free(hgp_ctx* ctx, void* d_ptr);
+hgp_status hgp_memcpy_htod(hgp_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
+hgp_status hgp_memcpy_dtoh(hgp_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+
+/* ---- host model: loaders and savers (includes/hgraph.hpp) ---------------- */
+/* Loads .hgr (hMETIS, loadhMETIS :936-1013), .snn (loadSNN :707-743) or .axh
+ * (loadAXH :805-854) by extension, then removes duplicate pins per hedge keeping
+ * sources (deduplicateHyperedges(2,false) :518-581, called at sources/main.cu:125). */
+hgp_status hgp_hgraph_load(const char* path, int dedup, hgp_hgraph** out);
+hgp_status hgp_hgraph_from_csr(uint32_t num_nodes, uint32_t num_hedges, const uint64_t* h_offsets,
+                               const uint32_t* h_pins, const uint32_t* h_srcs_count,
+                               const float* h_weights, int dedup, hgp_hgraph** out);
+void hgp_hgraph_free(hgp_hgraph* hg);
+void hgp_hgraph_sizes(const hgp_hgraph* hg, uint32_t* num_nodes, uint32_t* num_hedges, uint64_t* num_pins);
+hgp_status hgp_hgraph_copy_out(const hgp_hgraph* hg, uint64_t* h_offsets, uint32_t* h_pins,
+                               uint32_t* h_srcs_count, float* h_weights);
+/* Partition file: one id per line (savePartitioning :1054-1059). */
+hgp_status hgp_save_partitioning(const char* path, const uint32_t* h_part, uint32_t num_nodes);
+/* Quotient hypergraph writer for -s (sources/runconfig.cpp:219-247):
+ * getPartitionsHypergraph(part,2,true) (:286-376) saved as .hgr/.snn/.axh. */
+hgp_status hgp_save_partitioned(const hgp_hgraph* hg, const uint32_t* h_part, const char* path);
+/* Metrics on the host model (lambda-1 :214-222 over the quotient hypergraph, cut-net, SOED). */
+hgp_status hgp_hgraph_metrics(const hgp_hgraph* hg, const uint32_t* h_part, double* connectivity,
+                              double* cutnet, double* soed);
+/* Validity: count, node cap, distinct inbound per partition (sources/constr.cpp:90-147).
+ * *valid = 1 when valid. */
+hgp_status hgp_check_validity(const hgp_hgraph* hg, const uint32_t* h_part, uint32_t max_nodes_per_part,
+                              uint32_t max_inbound_per_part, uint32_t max_parts, int32_t* valid);
+
+/* ---- synthetic workloads (seeded, deterministic) ------------------------- */
+/* Random hypergraph: |e| = 2 + Geometric(1/2) capped at max_size, uniform distinct pins. */
+hgp_status hgp_gen_random(uint32_t num_nodes, uint32_t num_hedges, uint32_t max_size, uint64_t seed,
+                          hgp_hgraph** out);
+/* Netlist-shaped: power-law sizes (alpha, [2,max_size]), 2D-grid locality (frac_local of the
+ * pins within `radius` of the first pin), the rest uniform. */
+hgp_status hgp_gen_netlist(uint32_t num_nodes, uint32_t num_hedges, double alpha, uint32_t max_size,
+                           uint32_t radius, double frac_local, uint64_t seed, hgp_hgraph** out);
+/* Layered feed-forward SNN: `layers` x `per_layer` neurons on a grid of side `grid`; every
+ * neuron of layers 0..L-2 is one hedge (src = itself) with `fanout` distinct destinations in
+ * the next layer drawn from a 2D Gaussian of std `sigma`. */
+hgp_status hgp_gen_snn(uint32_t layers, uint32_t per_layer, uint32_t grid, uint32_t fanout, double sigma,
+                       uint64_t seed, hgp_hgraph** out);
+hgp_status hgp_hgraph_save(const hgp_hgraph* hg, const char* path);
+
+/* ---- stages (device pointers) -------------------------------------------- */
+
+/* buildTouching (sources/construction.cu:64-179; kernels/construction.cu:9-84).
+ * Incidence CSR: touching_size == number of pins. */
+hgp_status hgp_build_touching(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                              const uint32_t* d_srcs_count, uint32_t num_nodes, uint32_t num_hedges,
+                              uint64_t* touching_size, uint32_t** d_touching,
+                              uint64_t** d_touching_offsets, uint32_t** d_inbound_count);
+
+/* buildNeighbors (sources/construction.cu:242-340; kernels/construction.cu:157-297).
+ * N(v) = union of the pins of v's touching hedges, minus v. */
+hgp_status hgp_build_neighbors(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                               const uint32_t* d_touching, const uint64_t* d_touching_offsets,
+                               uint32_t num_nodes, uint64_t* neighbors_size, uint32_t** d_neighbors,
+                               uint64_t** d_neighbors_offsets);
+
+/* candidatesProposal (sources/coarsening.cu:17-68; kernels/coarsening.cu:10-213).
+ * Writes d_pairs/d_scores [num_nodes * candidates_count]: the top candidates by
+ * (score desc, id desc); empty slots are UINT32_MAX / 0.  Scores are computed on the
+ * fly from the hypergraph (no neighbour sets needed), so the reference's
+ * d_neighbors arguments have no counterpart here. */
+hgp_status hgp_candidates(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                          const uint32_t* d_srcs_count, const uint32_t* d_touching,
+                          const uint64_t* d_touching_offsets, const uint32_t* d_inbound_count,
+                          const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                          uint32_t num_hedges, uint32_t candidates_count, uint32_t max_nodes_per_part,
+                          uint32_t max_inbound_per_part, uint32_t* d_pairs, uint32_t* d_scores);
+
+/* groupNodes (sources/coarsening.cu:70-210; kernels/coarsening.cu:241-505;
+ * build_orphan_pairs sources/chaining.cu:260-329).  Pairs nodes along their
+ * candidates (groups of at most two), pairs leftover candidate-less nodes, and
+ * relabels groups zero-based in ascending raw-id order. */
+hgp_status hgp_group_nodes(hgp_ctx* ctx, const uint32_t* d_inbound_count, const uint32_t* d_pairs,
+                           const uint32_t* d_scores, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                           uint32_t candidates_count, uint32_t max_nodes_per_part,
+                           uint32_t max_inbound_per_part, uint32_t* new_num_nodes, uint32_t** d_groups,
+                           uint32_t** d_groups_sizes, uint32_t** d_ungroups, uint64_t** d_ungroups_offsets);
+
+/* coarsenNeighbors (sources/construction.cu:342-561):
+ * N'(g) = { groups[x] : x in N(m), m in g } minus g. */
+hgp_status hgp_coarsen_neighbors(hgp_ctx* ctx, const uint32_t* d_neighbors,
+                                 const uint64_t* d_neighbors_offsets, const uint32_t* d_groups,
+                                 const uint32_t* d_ungroups, const uint64_t* d_ungroups_offsets,
+                                 uint32_t num_nodes, uint32_t new_num_nodes, uint64_t* new_neighbors_size,
+                                 uint32_t** d_coarse_neighbors, uint64_t** d_coarse_neighbors_offsets);
+
+/* coarsenHedges (sources/construction.cu:563-676; kernels/construction.cu:304-489).
+ * Coarse hedge = distinct groups of its pins; a source group that is also a
+ * destination group is dropped from the sources; the hedge count never shrinks. */
+hgp_status hgp_coarsen_hedges(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                              const uint32_t* d_srcs_count, const uint32_t* d_groups, uint32_t num_hedges,
+                              uint64_t hedges_size, uint64_t* new_hedges_size, uint32_t** d_coarse_hedges,
+                              uint64_t** d_coarse_hedges_offsets, uint32_t** d_coarse_srcs_count);
+
+/* coarsenTouching (sources/construction.cu:678-789).  Built here as the incidence
+ * of the coarse hedges, which equals the reference's union of the members'
+ * inbound sets followed by the members' outbound sets minus the inbound ones. */
+hgp_status hgp_coarsen_touching(hgp_ctx* ctx, const uint32_t* d_coarse_hedges,
+                                const uint64_t* d_coarse_hedges_offsets, const uint32_t* d_coarse_srcs_count,
+                                uint32_t new_num_nodes, uint32_t num_hedges, uint64_t* new_touching_size,
+                                uint32_t** d_coarse_touching, uint64_t** d_coarse_touching_offsets,
+                                uint32_t** d_coarse_inbound_count);
+
+/* apply_uncoarsening_partitions (kernels/construction.cu:953-968). */
+hgp_status hgp_uncoarsen(hgp_ctx* ctx, const uint32_t* d_groups, const uint32_t* d_coarse_partitions,
+                         uint32_t num_nodes, uint32_t* d_partitions);
+
+/* refinementRepeats (sources/refinement.cu:18-582): `refine_repeats` rounds of
+ * parallel FM with size and inbound-set event checks.  Updates d_partitions and
+ * d_partitions_sizes in place; when update_final_inbound_counts is set the exact
+ * inbound-set size of every partition is written to d_partitions_inbound_sizes. */
+hgp_status hgp_refine(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_hedges,
+                      const uint64_t* d_hedges_offsets, const uint32_t* d_srcs_count,
+                      const uint32_t* d_touching, const uint64_t* d_touching_offsets,
+                      const uint32_t* d_inbound_count, const float* d_hedge_weights,
+                      const uint32_t* d_nodes_sizes, uint32_t level_idx, uint32_t num_nodes,
+                      uint32_t num_hedges, uint32_t num_partitions, int32_t update_final_inbound_counts,
+                      uint32_t* d_partitions, uint32_t* d_partitions_sizes,
+                      uint32_t* d_partitions_inbound_sizes);
+
+/* One analysis round of refinementRepeats without applying it, exposing every
+ * intermediate (host outputs, each num_nodes long):
+ *   moves[v], gains[v]   : fm_refinement_gains_kernel (kernels/refinement.cu:82-209),
+ *                          gains in 2^18 fixed point;
+ *   ranks[v]             : chaining (sources/chaining.cu:19-254) or gain sort (:172-189);
+ *   seq_gains[r]         : fm_refinement_cascade_kernel (:216-322) + inclusive scan;
+ *   size_viol[r], inbound_viol[r]: scanned event flags (:239-305, :307-491);
+ *   *best_rank           : best_move_functor max_element (:493-500). */
+hgp_status hgp_refine_analyze(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_hedges,
+                              const uint64_t* d_hedges_offsets, const uint32_t* d_srcs_count,
+                              const uint32_t* d_touching, const uint64_t* d_touching_offsets,
+                              const uint32_t* d_inbound_count, const float* d_hedge_weights,
+                              const uint32_t* d_nodes_sizes, uint32_t num_nodes, uint32_t num_hedges,
+                              uint32_t num_partitions, uint32_t fm_repeat, int32_t chainup,
+                              const uint32_t* d_partitions, const uint32_t* d_partitions_sizes,
+                              uint32_t* h_moves, int64_t* h_gains, uint32_t* h_ranks, int64_t* h_seq_gains,
+                              int32_t* h_size_viol, int32_t* h_inbound_viol, uint32_t* h_best_rank);
+
+/* greedyMergeGroups (-ipm, sources/postprocess.cu:16-245). */
+hgp_status hgp_greedy_merge_groups(hgp_ctx* ctx, const uint32_t* d_nodes_sizes,
+                                   const uint32_t* d_inbound_count, uint32_t num_nodes, uint32_t num_groups,
+                                   uint32_t max_nodes_per_part, uint32_t max_inbound_per_part,
+                                   uint32_t* d_groups, uint32_t* d_groups_sizes, uint32_t* new_num_groups);
+
+/* mergeSmallPartitions (INCC, sources/postprocess.cu:247-420). */
+hgp_status hgp_merge_small_partitions(hgp_ctx* ctx, const uint32_t* d_partitions_sizes,
+                                      const uint32_t* d_partitions_inbound_sizes, uint32_t num_nodes,
+                                      uint32_t num_partitions, uint32_t max_nodes_per_part,
+                                      uint32_t max_inbound_per_part, uint32_t* d_partitions);
+
+/* Final zero-based relabel in ascending old-id order (sources/main.cu:817-830). */
+hgp_status hgp_relabel_partitions(hgp_ctx* ctx, uint32_t num_nodes, uint32_t* d_partitions,
+                                  uint32_t* new_num_partitions);
+
+/* Exact inbound-set size per partition (inbound_sets_size_kernel,
+ * kernels/refinement.cu:1176-1219): d_out[num_partitions]. */
+hgp_status hgp_inbound_sizes(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                             const uint32_t* d_srcs_count, const uint32_t* d_partitions, uint32_t num_hedges,
+                             uint32_t num_partitions, uint32_t* d_out);
+
+/* lambda-1 on the device: sum_e w_e * (lambda(e) - 1), in 2^18 fixed point
+ * (compute_connectivity_kernel, kernels/init_part.cu:287-314). */
+hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                            const float* d_hedge_weights, const uint32_t* d_partitions, uint32_t num_hedges,
+                            int64_t* km1_fixed);
+
+/* initial_partitioning_kahypar (sources/init_part.cu:733-953): k-way initial
+ * partition of a coarse level.  Runs `mtkahypar` when it is on PATH (same
+ * command line as the reference); otherwise the built-in recursive-bisection
+ * partitioner.  Allocates d_partitions[num_nodes] and d_partitions_sizes[k]. */
+hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                                 const float* d_hedge_weights, uint32_t num_nodes, uint32_t num_hedges,
+                                 uint64_t hedges_size, const uint32_t* d_nodes_sizes, uint32_t k, float epsilon,
+                                 uint32_t max_nodes_per_part, uint32_t** d_partitions,
+                                 uint32_t** d_partitions_sizes);
+
+/* ---- whole pipeline (sources/main.cu:38-940) ------------------------------ */
+/* Partitions the host model; writes h_partitions_out[num_nodes] (zero-based ids). */
+hgp_status hgp_partition(hgp_ctx* ctx, const hgp_params* params, const hgp_hgraph* hg,
+                         uint32_t* h_partitions_out, hgp_run_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HGP_H */

@@ -1,0 +1,310 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and of the reference's own host
+tools built into oracle/_ref/ (test infrastructure only: tests/, smoke() and bench.py's
+cpu_baseline leg use it as the checker; the product never does)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+REF_LIB = os.path.join(_HERE, "_ref", "libref_host.so")
+REF_GRADER = os.path.join(_HERE, "_ref", "hgrade")
+NONE = 0xFFFFFFFF
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+i64p = C.POINTER(C.c_int64)
+i32p = C.POINTER(C.c_int32)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+
+_lib = None
+_ref = None
+
+
+def P(a, t):
+    return a.ctypes.data_as(t)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(LIB)
+        _lib.orc_hg_load.restype = C.c_void_p
+        _lib.orc_hg_load.argtypes = [C.c_char_p, C.c_int]
+        _lib.orc_hg_sizes.argtypes = [C.c_void_p, u32p, u32p, u64p]
+        _lib.orc_hg_copy.argtypes = [C.c_void_p, u64p, u32p, u32p, f32p]
+        _lib.orc_hg_free.argtypes = [C.c_void_p]
+        _lib.orc_dedup.restype = C.c_uint64
+        _lib.orc_fixed_weight.restype = C.c_uint32
+        _lib.orc_fixed_weight.argtypes = [C.c_float]
+        _lib.orc_noise.restype = C.c_uint32
+        _lib.orc_hash_uint32.restype = C.c_uint32
+        _lib.orc_connectivity_fixed.restype = C.c_int64
+        _lib.orc_connectivity_fixed.argtypes = [C.c_uint32, u64p, u32p, f32p, u32p]
+        _lib.orc_initial_partition.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, f32p, u32p, C.c_uint32, C.c_float,
+                                               C.c_uint32, u32p, u32p]
+        _lib.orc_partition.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                                       C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p, u32p, u32p]
+        _lib.orc_metrics.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p, u32p, f64p, f64p, f64p]
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_LIB)
+        _ref.ref_load.restype = C.c_void_p
+        _ref.ref_load.argtypes = [C.c_char_p, C.c_int]
+        _ref.ref_sizes.argtypes = [C.c_void_p, u32p, u32p, u64p]
+        _ref.ref_copy.argtypes = [C.c_void_p, u64p, u32p, u32p, f32p]
+        _ref.ref_free.argtypes = [C.c_void_p]
+        _ref.ref_metrics.argtypes = [C.c_void_p, u32p, f64p, f64p, f64p]
+        _ref.ref_validity.argtypes = [C.c_void_p, u32p, C.c_uint32, C.c_uint32, C.c_uint32]
+    return _ref
+
+
+class HG:
+    """Plain host CSR (n, offsets, pins, srcs_count, weights)."""
+
+    def __init__(self, n, off, pins, srcc, w):
+        self.n = int(n)
+        self.off = np.ascontiguousarray(off, np.uint64)
+        self.pins = np.ascontiguousarray(pins, np.uint32)
+        self.srcc = np.ascontiguousarray(srcc, np.uint32)
+        self.w = np.ascontiguousarray(w, np.float32)
+
+    @property
+    def e(self):
+        return len(self.srcc)
+
+
+def _load_with(lb, load, sizes, copy, free, path, dedup):
+    h = load(path.encode(), 1 if dedup else 0)
+    if not h:
+        return None
+    n, e, p = C.c_uint32(), C.c_uint32(), C.c_uint64()
+    sizes(h, C.byref(n), C.byref(e), C.byref(p))
+    off = np.zeros(e.value + 1, np.uint64)
+    pins = np.zeros(p.value, np.uint32)
+    srcc = np.zeros(e.value, np.uint32)
+    w = np.zeros(e.value, np.float32)
+    copy(h, P(off, u64p), P(pins, u32p), P(srcc, u32p), P(w, f32p))
+    free(h)
+    return HG(n.value, off, pins, srcc, w)
+
+
+def load(path, dedup=True):
+    L = lib()
+    return _load_with(L, L.orc_hg_load, L.orc_hg_sizes, L.orc_hg_copy, L.orc_hg_free, path, dedup)
+
+
+def ref_load(path, dedup=True):
+    R = ref()
+    return _load_with(R, R.ref_load, R.ref_sizes, R.ref_copy, R.ref_free, path, dedup)
+
+
+def ref_metrics(path, part):
+    R = ref()
+    h = R.ref_load(path.encode(), 0)
+    assert h
+    part = np.ascontiguousarray(part, np.uint32)
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    rc = R.ref_metrics(h, P(part, u32p), C.byref(a), C.byref(b), C.byref(c))
+    R.ref_free(h)
+    return (a.value, b.value, c.value) if rc == 0 else None
+
+
+def ref_validity(path, part, omega, delta, max_parts):
+    R = ref()
+    h = R.ref_load(path.encode(), 0)
+    assert h
+    part = np.ascontiguousarray(part, np.uint32)
+    v = R.ref_validity(h, P(part, u32p), omega, delta, max_parts)
+    R.ref_free(h)
+    return v
+
+
+def dedup(hg: HG) -> HG:
+    off = hg.off.copy()
+    pins = hg.pins.copy()
+    srcc = hg.srcc.copy()
+    np_ = lib().orc_dedup(hg.e, P(off, u64p), P(pins, u32p), P(srcc, u32p))
+    return HG(hg.n, off, pins[:np_], srcc, hg.w)
+
+
+def touching(hg: HG):
+    p = len(hg.pins)
+    toff = np.zeros(hg.n + 1, np.uint64)
+    tch = np.zeros(p, np.uint32)
+    inb = np.zeros(hg.n, np.uint32)
+    lib().orc_build_touching(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(toff, u64p),
+                             P(tch, u32p), P(inb, u32p))
+    return toff, tch, inb
+
+
+def neighbors(hg: HG, toff, tch):
+    noff = np.zeros(hg.n + 1, np.uint64)
+    lib().orc_build_neighbors(hg.n, P(hg.off, u64p), P(hg.pins, u32p), P(toff, u64p), P(tch, u32p), P(noff, u64p), None)
+    nbr = np.zeros(int(noff[-1]), np.uint32)
+    lib().orc_build_neighbors(hg.n, P(hg.off, u64p), P(hg.pins, u32p), P(toff, u64p), P(tch, u32p), P(noff, u64p),
+                              P(nbr, u32p))
+    return noff, nbr
+
+
+def candidates(hg: HG, toff, tch, inb, sizes, cnc, omega, delta):
+    pairs = np.zeros(hg.n * cnc, np.uint32)
+    scores = np.zeros(hg.n * cnc, np.uint32)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    lib().orc_candidates(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(toff, u64p), P(tch, u32p),
+                         P(inb, u32p), P(hg.w, f32p), P(sizes, u32p), cnc, omega, delta, P(pairs, u32p),
+                         P(scores, u32p))
+    return pairs, scores
+
+
+def group_nodes(n, inb, pairs, scores, sizes, cnc, omega, delta):
+    new_n = C.c_uint32()
+    groups = np.zeros(n, np.uint32)
+    gs = np.zeros(n, np.uint32)
+    ung = np.zeros(n, np.uint32)
+    uoff = np.zeros(n + 1, np.uint64)
+    raw = np.zeros(n, np.uint32)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    lib().orc_group_nodes(n, P(inb, u32p), P(pairs, u32p), P(scores, u32p), P(sizes, u32p), cnc, omega, delta,
+                          C.byref(new_n), P(groups, u32p), P(gs, u32p), P(ung, u32p), P(uoff, u64p), P(raw, u32p))
+    k = new_n.value
+    return k, groups, gs[:k], ung, uoff[:k + 1], raw
+
+
+def coarsen_hedges(hg: HG, groups):
+    choff = np.zeros(hg.e + 1, np.uint64)
+    csrcc = np.zeros(hg.e, np.uint32)
+    groups = np.ascontiguousarray(groups, np.uint32)
+    lib().orc_coarsen_hedges(hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(groups, u32p),
+                             P(choff, u64p), None, P(csrcc, u32p))
+    cp = np.zeros(int(choff[-1]), np.uint32)
+    lib().orc_coarsen_hedges(hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(groups, u32p),
+                             P(choff, u64p), P(cp, u32p), P(csrcc, u32p))
+    return choff, cp, csrcc
+
+
+def coarsen_neighbors(n, noff, nbr, groups, new_n):
+    cnoff = np.zeros(new_n + 1, np.uint64)
+    groups = np.ascontiguousarray(groups, np.uint32)
+    lib().orc_coarsen_neighbors(n, P(noff, u64p), P(nbr, u32p), P(groups, u32p), new_n, P(cnoff, u64p), None)
+    cn = np.zeros(int(cnoff[-1]), np.uint32)
+    lib().orc_coarsen_neighbors(n, P(noff, u64p), P(nbr, u32p), P(groups, u32p), new_n, P(cnoff, u64p), P(cn, u32p))
+    return cnoff, cn
+
+
+def refine_analyze(mode, omega, delta, repeats, hg: HG, toff, tch, inb, sizes, k, fm_repeat, chainup, part, psizes):
+    n = hg.n
+    moves = np.zeros(n, np.uint32)
+    gains = np.zeros(n, np.int64)
+    ranks = np.zeros(n, np.uint32)
+    seq = np.zeros(n, np.int64)
+    sv = np.zeros(n, np.int32)
+    iv = np.zeros(n, np.int32)
+    best = C.c_uint32()
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    part = np.ascontiguousarray(part, np.uint32)
+    psizes = np.ascontiguousarray(psizes, np.uint32)
+    lib().orc_refine_analyze(mode, omega, delta, repeats, n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p),
+                             P(toff, u64p), P(tch, u32p), P(inb, u32p), P(hg.w, f32p), P(sizes, u32p), k, fm_repeat,
+                             1 if chainup else 0, P(part, u32p), P(psizes, u32p), P(moves, u32p), P(gains, i64p),
+                             P(ranks, u32p), P(seq, i64p), P(sv, i32p), P(iv, i32p), C.byref(best))
+    return dict(moves=moves, gains=gains, ranks=ranks, seq=seq, sv=sv, iv=iv, best=best.value)
+
+
+def refine(mode, omega, delta, repeats, hg: HG, toff, tch, inb, sizes, k, final_counts, part, psizes):
+    part = np.ascontiguousarray(part, np.uint32).copy()
+    psizes = np.ascontiguousarray(psizes, np.uint32).copy()
+    pinb = np.zeros(k, np.uint32)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    lib().orc_refine(mode, omega, delta, repeats, hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p),
+                     P(toff, u64p), P(tch, u32p), P(inb, u32p), P(hg.w, f32p), P(sizes, u32p), k,
+                     1 if final_counts else 0, P(part, u32p), P(psizes, u32p), P(pinb, u32p))
+    return part, psizes, pinb
+
+
+def inbound_sizes(hg: HG, part, k):
+    out = np.zeros(k, np.uint32)
+    part = np.ascontiguousarray(part, np.uint32)
+    lib().orc_inbound_sizes(hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(part, u32p), k, P(out, u32p))
+    return out
+
+
+def connectivity_fixed(hg: HG, part):
+    part = np.ascontiguousarray(part, np.uint32)
+    return lib().orc_connectivity_fixed(hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(part, u32p))
+
+
+def greedy_merge_groups(n, ng, sizes, inb, omega, delta, groups, gsizes):
+    groups = np.ascontiguousarray(groups, np.uint32).copy()
+    gsizes = np.ascontiguousarray(gsizes, np.uint32).copy()
+    gsizes = np.concatenate([gsizes, np.zeros(max(0, ng - len(gsizes)), np.uint32)])
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    new = C.c_uint32()
+    lib().orc_greedy_merge_groups(n, ng, P(sizes, u32p), P(inb, u32p), omega, delta, P(groups, u32p), P(gsizes, u32p),
+                                  C.byref(new))
+    return new.value, groups, gsizes[:new.value]
+
+
+def merge_small_partitions(n, k, psizes, pinb, omega, delta, part):
+    part = np.ascontiguousarray(part, np.uint32).copy()
+    psizes = np.ascontiguousarray(psizes, np.uint32)
+    pinb = np.ascontiguousarray(pinb, np.uint32)
+    lib().orc_merge_small_partitions(n, k, P(psizes, u32p), P(pinb, u32p), omega, delta, P(part, u32p))
+    return part
+
+
+def relabel(part):
+    part = np.ascontiguousarray(part, np.uint32).copy()
+    k = C.c_uint32()
+    lib().orc_relabel(len(part), P(part, u32p), C.byref(k))
+    return k.value, part
+
+
+def initial_partition(hg: HG, sizes, k, eps, omega):
+    part = np.zeros(hg.n, np.uint32)
+    ps = np.zeros(k, np.uint32)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    lib().orc_initial_partition(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(sizes, u32p), k, eps,
+                                omega, P(part, u32p), P(ps, u32p))
+    return part, ps
+
+
+def partition(hg: HG, mode, omega, delta, max_parts, kway=0, eps=0.0, cnc=16, repeats=16, ipm=False):
+    part = np.zeros(hg.n, np.uint32)
+    lv = C.c_uint32()
+    lib().orc_partition(mode, omega, delta, max_parts, kway, eps, cnc, repeats, 1 if ipm else 0, hg.n, hg.e,
+                        P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(hg.w, f32p), P(part, u32p),
+                        C.byref(lv))
+    return part, lv.value
+
+
+def validity(hg: HG, part, omega, delta, max_parts):
+    part = np.ascontiguousarray(part, np.uint32)
+    return lib().orc_check_validity(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(part, u32p),
+                                    omega, delta, max_parts)
+
+
+def metrics(hg: HG, part):
+    part = np.ascontiguousarray(part, np.uint32)
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    lib().orc_metrics(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(hg.w, f32p), P(part, u32p),
+                      C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+def kway_omega(n, k, eps):
+    """(uint32)ceil((1 + eps) * (float)n / k) in float32, as sources/runconfig.cpp:191."""
+    v = (np.float32(1) + np.float32(eps)) * np.float32(n) / np.float32(k)
+    return int(np.ceil(np.float32(v)))

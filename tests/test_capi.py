@@ -1,0 +1,98 @@
+"""CPU checks of the C ABI boundary: the library loads, exports every symbol declared in
+include/hgp.h, and its host-side model (loaders, dedup, metrics, validity) agrees with the
+oracle.  No device compute is called here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import axoncuda_amd as H
+import gen
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "hgp.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hgp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = H.load_library()
+    syms = declared_symbols()
+    assert len(syms) >= 40
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    bound = {name for name, _, _ in H.SIGNATURES}
+    assert set(syms) <= bound, set(syms) - bound
+
+
+def test_no_device_is_reported(monkeypatch):
+    lib = H.load_library()
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    ctx = C.c_void_p()
+    assert lib.hgp_ctx_create(0, 0, C.byref(ctx)) == 7  # HGP_E_NODEVICE
+
+
+@pytest.mark.parametrize("fmt", ["hgr", "snn", "axh"])
+def test_product_loader_matches_oracle(tmp_path, fmt, rng):
+    if fmt == "hgr":
+        n, off, pins, srcc, w = gen.random_hypergraph(60, 90, rng, max_size=7, weights="int")
+        p = str(tmp_path / "x.hgr")
+        gen.write_hgr(p, n, off, pins, w, singletons=2)
+    elif fmt == "snn":
+        n, off, pins, srcc, w = gen.random_hypergraph(60, 90, rng, max_size=7, src_prob=1, max_src=1, dup_prob=0.5,
+                                                      weights="float")
+        p = str(tmp_path / "x.snn")
+        gen.write_snn(p, n, off, pins, w)
+    else:
+        n, off, pins, srcc, w = gen.random_hypergraph(60, 90, rng, max_size=7, src_prob=1, max_src=3, dup_prob=0.6,
+                                                      weights="float")
+        p = str(tmp_path / "x.axh")
+        gen.write_axh(p, n, off, pins, srcc, w)
+    for dedup in (False, True):
+        got = H.Hypergraph.load(p, dedup=dedup).arrays()
+        exp = O.load(p, dedup)
+        assert got[0] == exp.n
+        for a, b in zip(got[1:], (exp.off, exp.pins, exp.srcc, exp.w)):
+            assert np.array_equal(a, b)
+
+
+def test_product_metrics_and_validity_match_oracle(rng):
+    n, off, pins, srcc, w = gen.random_hypergraph(200, 300, rng, max_size=8, src_prob=0.7, max_src=2, weights="int")
+    hg = H.Hypergraph.from_csr(n, off, pins, srcc, w, dedup=True)
+    orc = O.HG(*hg.arrays())
+    for k in (2, 7, 30):
+        part = O.relabel(rng.integers(0, k, size=n).astype(np.uint32))[1]
+        assert np.allclose(hg.metrics(part), O.metrics(orc, part))
+        for cap in ((n, 10**9, k), (n // k + 2, 60, k), (5, 10**9, k)):
+            assert hg.validity(part, *cap) == O.validity(orc, part, *cap)
+
+
+def test_generators_are_deterministic():
+    a = H.Hypergraph.generate("netlist", n=5000, e=5200, seed=7).arrays()
+    b = H.Hypergraph.generate("netlist", n=5000, e=5200, seed=7).arrays()
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
+    s = H.Hypergraph.generate("snn", layers=3, per_layer=400, grid=20, fanout=16, sigma=3.0, seed=2).arrays()
+    n, off, pins, srcc, w = s
+    assert n == 1200 and len(srcc) == 800 and np.all(srcc == 1)
+    assert np.all(np.diff(off) == 17)
+
+
+def test_cli_help_and_errors():
+    exe = os.path.join(H.BIN_DIR, "hgraph_gpu")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0 and "Usage:" in r.stdout
+    r = subprocess.run([exe, "-m", "1", "2"], capture_output=True, text=True)
+    assert r.returncode == 1 and "three constraints" in r.stderr
+    r = subprocess.run([exe, "-cnc", "17"], capture_output=True, text=True)
+    assert r.returncode == 1
+    r = subprocess.run([exe, "-bogus"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Unknown option" in r.stderr

@@ -1,0 +1,77 @@
+"""GPU end-to-end: hgp_partition against the oracle pipeline (bit-exact partitions), plus
+validity, determinism and the CLI binary."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import axoncuda_amd as H
+import gen
+import gpu_stages as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def hg_pair(n, off, pins, srcc, w):
+    """the same CSR as a product model (deduplicated) and as an oracle CSR"""
+    prod = H.Hypergraph.from_csr(n, off, pins, srcc, w, dedup=True)
+    orc = O.HG(*prod.arrays())
+    return prod, orc
+
+
+@pytest.mark.parametrize("k", [2, 4, 16])
+def test_kway_pipeline_matches_oracle(dev, k, rng):
+    n, off, pins, srcc, w = gen.random_hypergraph(20000, 20000, rng, max_size=10)
+    prod, orc = hg_pair(n, off, pins, srcc, w)
+    eps = 0.03
+    omega = O.kway_omega(n, k, eps)
+    prm = G.params(H.HGP_MODE_KWAY, omega, 2**31 - 1, k, kway=k, eps=eps)
+    part, st = G.partition(dev, prod, prm)
+    exp, levels = O.partition(orc, 1, omega, 2**31 - 1, k, kway=k, eps=eps)
+    assert np.array_equal(part, exp)
+    assert O.validity(orc, part, omega, 2**31 - 1, k) == 1
+    assert st.levels == levels
+
+
+@pytest.mark.parametrize("ipm", [False, True])
+def test_incc_pipeline_matches_oracle(dev, ipm, rng):
+    n, off, pins, srcc, w = gen.snn_like(6, 800, 12, rng)
+    prod, orc = hg_pair(n, off, pins, srcc, w)
+    omega, delta, maxp = 64, 600, 4800
+    prm = G.params(H.HGP_MODE_INCC, omega, delta, maxp)
+    prm.initial_partitions_merge = 1 if ipm else 0
+    part, st = G.partition(dev, prod, prm)
+    exp, _ = O.partition(orc, 0, omega, delta, maxp, ipm=ipm)
+    assert np.array_equal(part, exp)
+    assert O.validity(orc, part, omega, delta, maxp) == 1
+
+
+def test_pipeline_deterministic(dev, rng):
+    n, off, pins, srcc, w = gen.random_hypergraph(30000, 32000, rng, max_size=12, weights="int")
+    prod, orc = hg_pair(n, off, pins, srcc, w)
+    omega = O.kway_omega(n, 8, 0.03)
+    prm = G.params(H.HGP_MODE_KWAY, omega, 2**31 - 1, 8, kway=8, eps=0.03)
+    a, _ = G.partition(dev, prod, prm)
+    b, _ = G.partition(dev, prod, prm)
+    assert np.array_equal(a, b)
+    assert O.validity(orc, a, omega, 2**31 - 1, 8) == 1
+
+
+def test_cli_roundtrip(dev, tmp_path, rng):
+    n, off, pins, srcc, w = gen.random_hypergraph(5000, 5000, rng, max_size=8)
+    f = str(tmp_path / "c0.hgr")
+    gen.write_hgr(f, n, off, pins)
+    pfile = str(tmp_path / "c0.part")
+    exe = os.path.join(H.BIN_DIR, "hgraph_gpu")
+    r = subprocess.run([exe, "-r", f, "-k", "2", "0.03", "-p", pfile], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Connectivity:" in r.stdout and "Total pins:" in r.stdout
+    part = np.loadtxt(pfile, dtype=np.uint32)
+    hg = O.load(f, True)
+    omega = O.kway_omega(n, 2, 0.03)
+    assert O.validity(hg, part, omega, 2**31 - 1, 2) == 1
+    grader = os.path.join(H.BIN_DIR, "hgrade")
+    g = subprocess.run([grader, "-r", f, "-prt", pfile, "-k-prt", "2", "0.03"], capture_output=True, text=True, timeout=120)
+    assert g.returncode == 0 and "Connectivity:" in g.stdout
