@@ -117,6 +117,10 @@ SIGNATURES = [
     ("hgp_initial_partition", C.c_int32, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, _vp, C.c_uint32,
                                           C.c_float, C.c_uint32, C.POINTER(_vp), C.POINTER(_vp)]),
     ("hgp_partition", C.c_int32, [_vp, C.POINTER(Params), _vp, _u32p, C.POINTER(RunStats)]),
+    ("hgp_profile_kernel", C.c_int32, [_vp, C.c_char_p]),
+    ("hgp_profile_read", C.c_int32, [_vp, C.POINTER(C.c_double), _u64p, _u64p]),
+    ("hgp_partition_device", C.c_int32, [_vp, C.POINTER(Params), C.c_uint32, C.c_uint32, _vp, _vp, _vp, _vp, _u32p,
+                                         C.POINTER(RunStats)]),
 ]
 
 _lib = None
@@ -284,3 +288,19 @@ class Device:
             self.close()
         except Exception:
             pass
+
+
+class KernelProfile:
+    """Times every launch of one kernel (HIP events on the library's stream) from now on."""
+
+    def __init__(self, dev: Device, kernel: str):
+        self.dev = dev
+        dev.check(dev.lib.hgp_profile_read(dev.ctx, None, None, None), "hgp_profile_read")
+        dev.check(dev.lib.hgp_profile_kernel(dev.ctx, kernel.encode()), "hgp_profile_kernel")
+
+    def read(self) -> dict:
+        ms, n, b = C.c_double(), C.c_uint64(), C.c_uint64()
+        self.dev.check(self.dev.lib.hgp_profile_read(self.dev.ctx, C.byref(ms), C.byref(n), C.byref(b)),
+                       "hgp_profile_read")
+        self.dev.check(self.dev.lib.hgp_profile_kernel(self.dev.ctx, None), "hgp_profile_kernel")
+        return {"ms": ms.value, "launches": n.value, "bytes": b.value}

@@ -129,6 +129,10 @@ hgp_status hgp_ctx_create(int device, uint64_t hbm_budget, hgp_ctx** out) {
 void hgp_ctx_destroy(hgp_ctx* ctx) {
   if (!ctx) return;
   (void)hipStreamSynchronize(ctx->c.stream);
+  for (auto& r : ctx->c.prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
   for (auto& kv : ctx->c.live) (void)hipFreeAsync(kv.first, ctx->c.stream);
   (void)hipStreamSynchronize(ctx->c.stream);
   if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
@@ -536,7 +540,62 @@ hgp_status hgp_partition(hgp_ctx* ctx, const hgp_params* params, const hgp_hgrap
     const int saved = c.verbose;
     c.verbose = params->verbose;
     try {
-      hgp::partition(c, *params, *hg, h_partitions_out, stats);
+      hgp::PipelineInput in;
+      in.n = hg->n;
+      in.e = hg->e();
+      in.p = hg->pins.size();
+      in.host = hg;
+      hgp::partition(c, *params, in, h_partitions_out, stats);
+    } catch (...) {
+      c.verbose = saved;
+      throw;
+    }
+    c.verbose = saved;
+  });
+}
+
+hgp_status hgp_profile_kernel(hgp_ctx* ctx, const char* kernel_name) {
+  return guard(ctx, [&](Ctx& c) { c.prof_kernel = kernel_name ? kernel_name : ""; });
+}
+
+hgp_status hgp_profile_read(hgp_ctx* ctx, double* total_ms, uint64_t* launches, uint64_t* algorithmic_bytes) {
+  return guard(ctx, [&](Ctx& c) {
+    hgp::sync(c);
+    double ms = 0.0;
+    uint64_t bytes = 0;
+    for (auto& r : c.prof) {
+      float t = 0.0f;
+      HGP_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      ms += t;
+      bytes += r.bytes;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = c.prof.size();
+    if (algorithmic_bytes) *algorithmic_bytes = bytes;
+    c.prof.clear();
+  });
+}
+
+hgp_status hgp_partition_device(hgp_ctx* ctx, const hgp_params* params, uint32_t num_nodes, uint32_t num_hedges,
+                                const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                                const uint32_t* d_srcs_count, const float* d_hedge_weights,
+                                uint32_t* h_partitions_out, hgp_run_stats* stats) {
+  if (!params || !h_partitions_out || !d_hedges_offsets) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    const int saved = c.verbose;
+    c.verbose = params->verbose;
+    try {
+      hgp::PipelineInput in;
+      in.n = num_nodes;
+      in.e = num_hedges;
+      in.p = num_hedges ? hgp::read_scalar(c, d_hedges_offsets + num_hedges) : 0;
+      in.d_hedges = d_hedges;
+      in.d_hoff = d_hedges_offsets;
+      in.d_srcc = d_srcs_count;
+      in.d_w = d_hedge_weights;
+      hgp::partition(c, *params, in, h_partitions_out, stats);
     } catch (...) {
       c.verbose = saved;
       throw;

@@ -26,11 +26,12 @@ constexpr uint32_t CAND_CAP = 1024;    // LDS slots per wave
 constexpr uint32_t CAND_LOG = 10;
 constexpr uint32_t CAND_LIGHT = 512;   // pin-visit bound served from LDS
 constexpr uint32_t CAND_SLOTS = CAND_CAP / 64;
-constexpr uint64_t HEAVY_BATCH_SLOTS = 64ull << 20;
+constexpr uint32_t CAND_MED_CAP = 4096;   // LDS table slots of a medium node (48 KB)
+constexpr uint64_t HEAVY_TABLE_SLOTS = 64ull << 20;  // HBM table slots shared by the heavy grid
 
+// Pins visited when scanning a node's touching hedges (itself excluded once per hedge).
 __global__ void k_visit_bound(const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ tch,
-                              const uint64_t* __restrict__ toff, uint32_t n, uint64_t* __restrict__ bound,
-                              uint32_t* __restrict__ light_flag) {
+                              const uint64_t* __restrict__ toff, uint32_t n, uint64_t* __restrict__ bound) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   uint64_t b = 0;
@@ -39,15 +40,6 @@ __global__ void k_visit_bound(const uint64_t* __restrict__ hoff, const uint32_t*
     b += hoff[e + 1] - hoff[e] - 1;
   }
   bound[v] = b;
-  light_flag[v] = b <= CAND_LIGHT ? 1u : 0u;
-}
-
-__global__ void k_split_lists(const uint32_t* __restrict__ light_flag, const uint64_t* __restrict__ lpos, uint32_t n,
-                              uint32_t* __restrict__ light, uint32_t* __restrict__ heavy) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  if (light_flag[v]) light[lpos[v]] = v;
-  else heavy[v - lpos[v]] = v;
 }
 
 // candidate key = (final score << 32) | v, or 0 when v is filtered out
@@ -132,30 +124,112 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
   }
 }
 
-// one workgroup per heavy node, table in global memory
-__global__ void __launch_bounds__(256) k_cand_heavy(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
-                                                    const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ tch,
-                                                    const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
-                                                    const float* __restrict__ w, const uint32_t* __restrict__ sizes,
-                                                    const uint32_t* __restrict__ list, const uint64_t* __restrict__ tab_off,
-                                                    uint32_t* __restrict__ gkeys, uint32_t* __restrict__ gscore,
-                                                    uint32_t* __restrict__ ginter, uint32_t cnc, uint32_t omega,
-                                                    uint32_t delta, uint32_t* __restrict__ pairs,
-                                                    uint32_t* __restrict__ scores) {
-  __shared__ uint64_t s_red[4];
-  __shared__ uint64_t s_best;
-  const uint32_t u = list[blockIdx.x];
-  const uint64_t base = tab_off[blockIdx.x];
-  const uint32_t cap = (uint32_t)(tab_off[blockIdx.x + 1] - base);
-  uint32_t* keys = gkeys + base;
-  uint32_t* score = gscore + base;
-  uint32_t* inter = ginter + base;
+// Sorted (descending) per-thread list of the 16 best candidate keys seen so far.
+__device__ __forceinline__ void top_insert(uint64_t (&top)[16], uint64_t k) {
+  if (k <= top[15]) return;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const bool sw = k > top[i];
+    const uint64_t t = top[i];
+    top[i] = sw ? k : t;
+    k = sw ? t : k;
+  }
+}
+
+// Block-wide top-cnc of the per-thread lists (keys are distinct: they embed the node id).
+// Each wave extracts its cnc best by repeated wave maxima, then wave 0 merges the 4x16.
+__device__ __forceinline__ void block_top_write(uint64_t (&top)[16], uint32_t cnc, uint64_t* s_w, uint32_t u,
+                                                uint32_t* __restrict__ pairs, uint32_t* __restrict__ scores) {
+  const uint32_t wv = threadIdx.x / 64, l = lane_id();
+  for (uint32_t i = 0; i < 16; ++i) {
+    uint64_t m = 0;
+    if (i < cnc) {
+      m = wave_max(top[0]);
+      if (m != 0 && top[0] == m) {
+#pragma unroll
+        for (int j = 0; j < 15; ++j) top[j] = top[j + 1];
+        top[15] = 0;
+      }
+    }
+    if (l == 0) s_w[wv * 16 + i] = m;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    uint64_t x = s_w[l];
+    for (uint32_t i = 0; i < cnc; ++i) {
+      const uint64_t m = wave_max(x);
+      if (m != 0 && x == m) x = 0;
+      if (l == 0) {
+        pairs[(uint64_t)u * cnc + i] = m ? (uint32_t)m : NONE;
+        scores[(uint64_t)u * cnc + i] = (uint32_t)(m >> 32);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Table capacity of a node whose touching hedges hold `bound` other pins: distinct
+// neighbours are at most min(bound, n - 1); a power-of-two open-addressing table of twice
+// that (plus slack) never fills.
+__device__ __forceinline__ uint32_t hash_cap(uint64_t bound, uint32_t n) {
+  const uint64_t d = bound < (uint64_t)n ? bound : (uint64_t)n;
+  uint64_t c = 64;
+  while (c < 2 * d + 64) c <<= 1;
+  return (uint32_t)c;
+}
+
+// tier of each node: 0 = light (one wave, LDS), 1 = medium (one block, LDS table),
+// 2 = heavy (one block, table in HBM); cap[v] = table slots (heavy: n means direct index)
+__global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uint32_t* __restrict__ f_light,
+                             uint32_t* __restrict__ f_med, uint32_t* __restrict__ cap) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint64_t b = bound[v];
+  const uint32_t c = hash_cap(b, n);
+  const bool light = b <= CAND_LIGHT;
+  const bool med = !light && c <= CAND_MED_CAP;
+  f_light[v] = light ? 1u : 0u;
+  f_med[v] = med ? 1u : 0u;
+  cap[v] = (!light && !med && (uint64_t)n <= c) ? n : c;
+}
+
+__global__ void k_split3(const uint32_t* __restrict__ f_light, const uint32_t* __restrict__ f_med,
+                         const uint64_t* __restrict__ lpos, const uint64_t* __restrict__ mpos, uint32_t n,
+                         uint32_t* __restrict__ light, uint32_t* __restrict__ med, uint32_t* __restrict__ heavy) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  if (f_light[v]) light[lpos[v]] = v;
+  else if (f_med[v]) med[mpos[v]] = v;
+  else heavy[v - lpos[v] - mpos[v]] = v;
+}
+
+__global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ cap, uint32_t nh,
+                                uint32_t* out) {
+  uint32_t m = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += gridDim.x * blockDim.x)
+    m = cap[heavy[i]] > m ? cap[heavy[i]] : m;
+  m = wave_max(m);
+  if (lane_id() == 0) atomicMax(out, m);
+}
+
+// Scores one node with a whole workgroup: the four waves split its touching hedges, insert
+// into the table (LDS for medium nodes, the block's HBM region for heavy ones; a direct
+// index when the table would be as large as the node count), then rank the entries.
+template <bool LDS>
+__device__ __forceinline__ void cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
+                                                uint32_t* inter, const uint32_t* __restrict__ hedges,
+                                                const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ srcc,
+                                                const uint32_t* __restrict__ tch, const uint64_t* __restrict__ toff,
+                                                const uint32_t* __restrict__ inb, const float* __restrict__ w,
+                                                const uint32_t* __restrict__ sizes, uint32_t cnc, uint32_t omega,
+                                                uint32_t delta, uint32_t* __restrict__ pairs,
+                                                uint32_t* __restrict__ scores, uint64_t* s_w) {
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
     keys[s] = NONE;
     score[s] = 0;
     inter[s] = 0;
   }
-  __threadfence_block();
+  if (!LDS) __threadfence();
   __syncthreads();
   const uint64_t t0 = toff[u], t1 = toff[u + 1];
   const uint32_t inb_u = inb[u];
@@ -166,44 +240,66 @@ __global__ void __launch_bounds__(256) k_cand_heavy(const uint32_t* __restrict__
     if (!pv.act || pv.pin == u) return;
     const uint32_t contrib = fixed_weight(w[pv.e]) / pv.len;
     const bool shared_in = (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
-    table_insert(keys, score, inter, cap - 1, pv.pin, contrib, shared_in);
+    if (direct) {
+      keys[pv.pin] = pv.pin;
+      atomicAdd(&score[pv.pin], contrib);
+      if (shared_in) atomicAdd(&inter[pv.pin], 1u);
+    } else {
+      table_insert(keys, score, inter, cap - 1, pv.pin, contrib, shared_in);
+    }
   });
-  __threadfence();
+  if (!LDS) __threadfence();
   __syncthreads();
   const uint32_t size_u = sizes[u];
-  uint64_t prev = ~0ull;  // keys strictly below the previous winner remain eligible
-  for (uint32_t i = 0; i < cnc; ++i) {
-    uint64_t m = 0;
-    for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
-      const uint32_t v = keys[s];
-      if (v == NONE) continue;
-      const uint64_t k = cand_key(u, v, score[s], inter[s], size_u, inb_u, sizes, inb, omega, delta);
-      if (k < prev && k > m) m = k;
-    }
-    m = wave_max(m);
-    if (lane_id() == 0) s_red[wv] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint64_t b2 = s_red[0];
-      for (int q = 1; q < 4; ++q) b2 = s_red[q] > b2 ? s_red[q] : b2;
-      s_best = b2;
-      pairs[(uint64_t)u * cnc + i] = b2 ? (uint32_t)b2 : NONE;
-      scores[(uint64_t)u * cnc + i] = (uint32_t)(b2 >> 32);
-    }
-    __syncthreads();
-    prev = s_best;
-    if (prev == 0) prev = 0;  // nothing left: later slots stay empty (k < 0 is never true)
+  uint64_t top[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) top[i] = 0;
+  for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
+    const uint32_t v = keys[s];
+    if (v == NONE) continue;
+    top_insert(top, cand_key(u, v, score[s], inter[s], size_u, inb_u, sizes, inb, omega, delta));
   }
+  block_top_write(top, cnc, s_w, u, pairs, scores);
 }
 
-__global__ void k_heavy_caps(const uint32_t* __restrict__ heavy, const uint64_t* __restrict__ bound, uint32_t nh,
-                             uint64_t* __restrict__ cap) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nh) return;
-  uint64_t b = 2 * bound[heavy[i]] + 64;
-  uint64_t c = 1;
-  while (c < b) c <<= 1;
-  cap[i] = c;
+__global__ void __launch_bounds__(256) k_cand_medium(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
+                                                     const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ tch,
+                                                     const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
+                                                     const float* __restrict__ w, const uint32_t* __restrict__ sizes,
+                                                     const uint32_t* __restrict__ list, const uint32_t* __restrict__ caps,
+                                                     uint32_t cnc, uint32_t omega, uint32_t delta,
+                                                     uint32_t* __restrict__ pairs, uint32_t* __restrict__ scores) {
+  __shared__ uint32_t s_keys[CAND_MED_CAP];
+  __shared__ uint32_t s_score[CAND_MED_CAP];
+  __shared__ uint32_t s_inter[CAND_MED_CAP];
+  __shared__ uint64_t s_w[64];
+  const uint32_t u = list[blockIdx.x];
+  cand_block_node<true>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w, sizes, cnc,
+                        omega, delta, pairs, scores, s_w);
+}
+
+// Heavy nodes: a persistent grid, each workgroup owning `region` slots of HBM tables and
+// taking nodes blockIdx.x, blockIdx.x + gridDim.x, ...
+__global__ void __launch_bounds__(256) k_cand_heavy(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
+                                                    const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ tch,
+                                                    const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
+                                                    const float* __restrict__ w, const uint32_t* __restrict__ sizes,
+                                                    const uint32_t* __restrict__ list, uint32_t nlist,
+                                                    const uint32_t* __restrict__ caps, uint32_t n, uint64_t region,
+                                                    uint32_t* __restrict__ gkeys, uint32_t* __restrict__ gscore,
+                                                    uint32_t* __restrict__ ginter, uint32_t cnc, uint32_t omega,
+                                                    uint32_t delta, uint32_t* __restrict__ pairs,
+                                                    uint32_t* __restrict__ scores) {
+  __shared__ uint64_t s_w[64];
+  uint32_t* keys = gkeys + region * blockIdx.x;
+  uint32_t* score = gscore + region * blockIdx.x;
+  uint32_t* inter = ginter + region * blockIdx.x;
+  for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
+    const uint32_t u = list[i];
+    const uint32_t cap = caps[u];
+    cand_block_node<false>(u, cap, cap == n, keys, score, inter, hedges, hoff, srcc, tch, toff, inb, w, sizes, cnc, omega,
+                           delta, pairs, scores, s_w);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -385,38 +481,34 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   (void)e;
   HGP_REQUIRE(cnc >= 1 && cnc <= 16, HGP_E_INVALID, "candidates: candidates_count must be in 1..16");
   if (n == 0) return;
+  const uint32_t g = ceil_div(n, 256);
   DBuf<uint64_t> bound(c, n);
-  DBuf<uint32_t> lflag(c, n);
-  HGP_LAUNCH(k_visit_bound, ceil_div(n, 256), 256, 0, c, hoff, tch, toff, n, bound.get(), lflag.get());
-  DBuf<uint64_t> lpos(c, (uint64_t)n + 1);
-  exclusive_scan(c, lflag.get(), lpos.get(), n);
+  DBuf<uint32_t> f_light(c, n), f_med(c, n), caps(c, n);
+  HGP_LAUNCH(k_visit_bound, g, 256, 0, c, hoff, tch, toff, n, bound.get());
+  HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, f_light.get(), f_med.get(), caps.get());
+  DBuf<uint64_t> lpos(c, (uint64_t)n + 1), mpos(c, (uint64_t)n + 1);
+  exclusive_scan(c, f_light.get(), lpos.get(), n);
+  exclusive_scan(c, f_med.get(), mpos.get(), n);
   const uint32_t nl = (uint32_t)read_scalar(c, lpos.get() + n);
-  const uint32_t nh = n - nl;
-  DBuf<uint32_t> light(c, nl), heavy(c, nh);
-  HGP_LAUNCH(k_split_lists, ceil_div(n, 256), 256, 0, c, lflag.get(), lpos.get(), n, light.get(), heavy.get());
+  const uint32_t nm = (uint32_t)read_scalar(c, mpos.get() + n);
+  const uint32_t nh = n - nl - nm;
+  DBuf<uint32_t> light(c, nl), med(c, nm), heavy(c, nh);
+  HGP_LAUNCH(k_split3, g, 256, 0, c, f_light.get(), f_med.get(), lpos.get(), mpos.get(), n, light.get(), med.get(),
+             heavy.get());
   HGP_LAUNCH(k_cand_light, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, light.get(), nl, cnc,
              omega, delta, pairs, scores);
+  HGP_LAUNCH(k_cand_medium, nm, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, med.get(), caps.get(), cnc, omega,
+             delta, pairs, scores);
   if (nh == 0) return;
-  DBuf<uint64_t> caps(c, (uint64_t)nh + 1);
-  HGP_LAUNCH(k_heavy_caps, ceil_div(nh, 256), 256, 0, c, heavy.get(), bound.get(), nh, caps.get());
-  std::vector<uint64_t> hcap(nh);
-  copy_d2h(c, hcap.data(), caps.get(), nh * sizeof(uint64_t));
-  // batch heavy nodes so that their tables fit the scratch budget
-  uint32_t b0 = 0;
-  while (b0 < nh) {
-    uint64_t tot = 0;
-    uint32_t b1 = b0;
-    while (b1 < nh && (b1 == b0 || tot + hcap[b1] <= HEAVY_BATCH_SLOTS)) tot += hcap[b1++];
-    std::vector<uint64_t> off(b1 - b0 + 1, 0);
-    for (uint32_t i = b0; i < b1; ++i) off[i - b0 + 1] = off[i - b0] + hcap[i];
-    DBuf<uint64_t> doff(c, off.size());
-    copy_h2d(c, doff.get(), off.data(), off.size() * sizeof(uint64_t));
-    DBuf<uint32_t> gk(c, tot), gs(c, tot), gi(c, tot);
-    HGP_LAUNCH(k_cand_heavy, b1 - b0, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get() + b0,
-               doff.get(), gk.get(), gs.get(), gi.get(), cnc, omega, delta, pairs, scores);
-    sync(c);  // host vector `off` must outlive the copy
-    b0 = b1;
-  }
+  DBuf<uint32_t> mc(c, 1);
+  memset_async(c, mc.get(), 0, 4);
+  HGP_LAUNCH(k_heavy_max_cap, std::min<uint32_t>(ceil_div(nh, 256), 1024), 256, 0, c, heavy.get(), caps.get(), nh,
+             mc.get());
+  const uint64_t region = read_scalar(c, mc.get());
+  const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)nh, HEAVY_TABLE_SLOTS / region, 4096}));
+  DBuf<uint32_t> gk(c, grid * region), gs(c, grid * region), gi(c, grid * region);
+  HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(), nh,
+             caps.get(), n, region, gk.get(), gs.get(), gi.get(), cnc, omega, delta, pairs, scores);
 }
 
 void group_nodes(Ctx& c, const uint32_t* inb, const uint32_t* pairs, const uint32_t* scores, const uint32_t* sizes,

@@ -42,6 +42,13 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::hgp::Error((status), (msg));       \
   } while (0)
 
+// One timed launch of the profiled kernel: events around it on the context stream and the
+// kernel's algorithmic bytes for that launch.
+struct ProfRec {
+  hipEvent_t a = nullptr, b = nullptr;
+  uint64_t bytes = 0;
+};
+
 // Execution context: device, stream, HBM budget accounting, last error text.
 struct Ctx {
   int device = 0;
@@ -54,6 +61,8 @@ struct Ctx {
   std::string last_error;
   int verbose = 0;
   uint32_t num_cus = 256;
+  std::string prof_kernel;     // base name of the kernel whose launches are timed ("" = none)
+  std::vector<ProfRec> prof;   // timed launches not yet read
 };
 
 void* dmalloc(Ctx& c, uint64_t bytes);
@@ -125,13 +134,41 @@ inline T read_scalar(Ctx& c, const T* d) {
 }
 inline void sync(Ctx& c) { HGP_HIP(hipStreamSynchronize(c.stream)); }
 
-#define HGP_LAUNCH(kernel, grid, block, shmem, ctx, ...)                                    \
+// True when `spelled` (the launch site's kernel expression, e.g. "(k_rs_scatter<K, true>)")
+// names the profiled kernel: its identifier before any template arguments.
+inline bool prof_match(const Ctx& c, const char* spelled) {
+  if (c.prof_kernel.empty()) return false;
+  while (*spelled == '(' || *spelled == ' ') ++spelled;
+  size_t len = 0;
+  while (spelled[len] && spelled[len] != '<' && spelled[len] != ')' && spelled[len] != ' ') ++len;
+  return c.prof_kernel.size() == len && c.prof_kernel.compare(0, len, spelled, len) == 0;
+}
+inline hipEvent_t prof_start(Ctx& c) {
+  hipEvent_t e;
+  HGP_HIP(hipEventCreate(&e));
+  HGP_HIP(hipEventRecord(e, c.stream));
+  return e;
+}
+inline void prof_stop(Ctx& c, hipEvent_t a, uint64_t bytes) {
+  hipEvent_t e;
+  HGP_HIP(hipEventCreate(&e));
+  HGP_HIP(hipEventRecord(e, c.stream));
+  c.prof.push_back({a, e, bytes});
+}
+
+// Kernel launch on the context stream.  HGP_LAUNCH_B also states the launch's algorithmic
+// bytes, which the kernel profiler (hgp_profile_kernel) reports next to the timing.
+#define HGP_LAUNCH_B(bytes, kernel, grid, block, shmem, ctx, ...)                             \
   do {                                                                                      \
     if ((grid) > 0) {                                                                       \
+      const bool _prof = ::hgp::prof_match((ctx), #kernel);                                 \
+      hipEvent_t _pa = _prof ? ::hgp::prof_start(ctx) : nullptr;                            \
       hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, (ctx).stream, __VA_ARGS__); \
       HGP_HIP(hipGetLastError());                                                           \
+      if (_prof) ::hgp::prof_stop((ctx), _pa, (uint64_t)(bytes));                           \
     }                                                                                       \
   } while (0)
+#define HGP_LAUNCH(kernel, grid, block, shmem, ctx, ...) HGP_LAUNCH_B(0, kernel, grid, block, shmem, ctx, __VA_ARGS__)
 
 // ---------------------------------------------------------------------------------------
 // device helpers (wave64)

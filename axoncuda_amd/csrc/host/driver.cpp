@@ -152,10 +152,10 @@ void unspill(Ctx& c, DBuf<T>& d, std::vector<T>& h) {
 
 }  // namespace
 
-void partition(Ctx& c, const hgp_params& prm, const hgp_hgraph& hg, uint32_t* part_out, hgp_run_stats* stats) {
+void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t* part_out, hgp_run_stats* stats) {
   const bool kw = prm.mode == HGP_MODE_KWAY;
-  const uint32_t n0 = hg.n, E = hg.e();
-  const uint64_t P0 = hg.pins.size();
+  const uint32_t n0 = in.n, E = in.e;
+  const uint64_t P0 = in.p;
   const uint32_t omega = prm.max_nodes_per_part, delta = prm.max_inbound_per_part, max_parts = prm.max_parts;
   HGP_REQUIRE(omega > 0 && delta > 0 && max_parts > 0, HGP_E_INVALID, "constraints must be positive");
   HGP_REQUIRE(prm.candidates_count >= 1 && prm.candidates_count <= 16, HGP_E_INVALID, "-cnc must be in 1..16");
@@ -181,10 +181,17 @@ void partition(Ctx& c, const hgp_params& prm, const hgp_hgraph& hg, uint32_t* pa
     L.hoff = DBuf<uint64_t>(c, (uint64_t)E + 1);
     L.srcc = DBuf<uint32_t>(c, E);
     L.sizes = DBuf<uint32_t>(c, n0);
-    copy_h2d(c, L.hedges.get(), hg.pins.data(), P0 * 4);
-    copy_h2d(c, L.hoff.get(), hg.off.data(), ((uint64_t)E + 1) * 8);
-    copy_h2d(c, L.srcc.get(), hg.srcc.data(), (uint64_t)E * 4);
-    copy_h2d(c, w.get(), hg.w.data(), (uint64_t)E * 4);
+    if (in.host) {
+      copy_h2d(c, L.hedges.get(), in.host->pins.data(), P0 * 4);
+      copy_h2d(c, L.hoff.get(), in.host->off.data(), ((uint64_t)E + 1) * 8);
+      copy_h2d(c, L.srcc.get(), in.host->srcc.data(), (uint64_t)E * 4);
+      copy_h2d(c, w.get(), in.host->w.data(), (uint64_t)E * 4);
+    } else {
+      copy_d2d(c, L.hedges.get(), in.d_hedges, P0 * 4);
+      copy_d2d(c, L.hoff.get(), in.d_hoff, ((uint64_t)E + 1) * 8);
+      copy_d2d(c, L.srcc.get(), in.d_srcc, (uint64_t)E * 4);
+      copy_d2d(c, w.get(), in.d_w, (uint64_t)E * 4);
+    }
     fill_u32(c, L.sizes.get(), 1u, n0);  // every input node counts 1 (sources/main.cu:217-218)
     HGP_HIP(hipEventRecord(ev_core, c.stream));
     uint64_t ts;

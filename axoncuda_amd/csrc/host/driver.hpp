@@ -10,6 +10,17 @@ void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, con
                        const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, uint32_t*& part_out,
                        uint32_t*& psizes_out, double* ms);
 
-void partition(Ctx& c, const hgp_params& prm, const hgp_hgraph& hg, uint32_t* part_out, hgp_run_stats* stats);
+// Level-0 input: either a host model or device-resident arrays (copied into the level).
+struct PipelineInput {
+  uint32_t n = 0, e = 0;
+  uint64_t p = 0;
+  const hgp_hgraph* host = nullptr;
+  const uint32_t* d_hedges = nullptr;
+  const uint64_t* d_hoff = nullptr;
+  const uint32_t* d_srcc = nullptr;
+  const float* d_w = nullptr;
+};
+
+void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t* part_out, hgp_run_stats* stats);
 
 }  // namespace hgp

@@ -12,7 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <unordered_map>
+#include <thread>
 #include <vector>
 
 namespace hgp_ip {
@@ -45,19 +45,18 @@ inline void build_incidence(Hypergraph& h) {
 }
 
 // Coarse level -> IP input: hedges with >= 2 distinct pins, identical pin sets merged
-// (weights summed), in order of first appearance.
+// (weights summed), in order of first appearance.  Identical sets are found through an
+// open-addressing table keyed by a hash of the sorted pin set.
 inline Hypergraph from_level(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins,
                              const uint32_t* wfx, const uint32_t* sizes) {
   Hypergraph h;
   h.n = n;
   h.vw.assign(sizes, sizes + n);
   h.off.push_back(0);
-  struct Key {
-    uint64_t hash;
-    uint64_t start;
-    uint32_t len;
-  };
-  std::unordered_map<uint64_t, std::vector<uint32_t>> buckets;  // hash -> hedge ids in h
+  uint64_t cap = 16;
+  while (cap < 2ull * e + 2) cap <<= 1;
+  std::vector<uint32_t> table(cap, UINT32_MAX);  // unique hedge ids
+  std::vector<uint64_t> hashes;                  // hash of each unique hedge
   std::vector<uint32_t> tmp;
   for (uint32_t i = 0; i < e; ++i) {
     tmp.assign(pins + hoff[i], pins + hoff[i + 1]);
@@ -66,18 +65,21 @@ inline Hypergraph from_level(uint32_t n, uint32_t e, const uint64_t* hoff, const
     if (tmp.size() < 2) continue;
     uint64_t hs = 1469598103934665603ull;
     for (uint32_t p : tmp) hs = splitmix64(hs ^ p);
-    auto& b = buckets[hs];
+    uint64_t slot = hs & (cap - 1);
     bool merged = false;
-    for (uint32_t cand : b) {
-      uint64_t s = h.off[cand], l = h.off[cand + 1] - s;
-      if (l == tmp.size() && std::equal(tmp.begin(), tmp.end(), h.pins.begin() + s)) {
+    while (table[slot] != UINT32_MAX) {
+      const uint32_t cand = table[slot];
+      const uint64_t s = h.off[cand], l = h.off[cand + 1] - s;
+      if (hashes[cand] == hs && l == tmp.size() && std::equal(tmp.begin(), tmp.end(), h.pins.begin() + s)) {
         h.w[cand] += wfx[i];
         merged = true;
         break;
       }
+      slot = (slot + 1) & (cap - 1);
     }
     if (merged) continue;
-    b.push_back((uint32_t)h.w.size());
+    table[slot] = (uint32_t)h.w.size();
+    hashes.push_back(hs);
     h.pins.insert(h.pins.end(), tmp.begin(), tmp.end());
     h.off.push_back(h.pins.size());
     h.w.push_back(wfx[i]);
@@ -235,9 +237,14 @@ struct Bisector {
     heap[0].reset(h.n);
     heap[1].reset(h.n);
     std::vector<uint8_t> inq(h.n, 0);
+    // A hedge's pins are queued the first time one of its pins moves; afterwards every pin
+    // of it is queued or moved already, so it is never scanned again.
+    std::vector<uint8_t> expanded(h.off.size() - 1, 0);
     auto touch_nbrs = [&](uint32_t v) {
       for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
         uint32_t e = h.inc[i];
+        if (expanded[e]) continue;
+        expanded[e] = 1;
         for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
           uint32_t u = h.pins[j];
           if (side[u] == 1 && !inq[u]) { inq[u] = 1; heap[1].push(u, gain[u]); }
@@ -286,7 +293,7 @@ struct Bisector {
       }
       if (boundary) heap[side[v]].push(v, gain[v]);
     }
-    std::vector<uint32_t> moved;
+    std::vector<uint32_t> moved, fresh, parked[2];
     int64_t cum = 0, best = 0;
     size_t best_len = 0;
     uint64_t best_imb = std::max(wside[0] > maxw[0] ? wside[0] - maxw[0] : 0,
@@ -301,6 +308,7 @@ struct Bisector {
           uint32_t t = heap[s].top();
           if (wside[1 - s] + h.vw[t] <= maxw[1 - s] || wside[s] > maxw[s]) break;
           heap[s].remove(t);
+          parked[s].push_back(t);
         }
       }
       int64_t g0 = heap[0].empty() ? INT64_MIN : gain[heap[0].top()];
@@ -313,16 +321,24 @@ struct Bisector {
       heap[pick].remove(v);
       locked[v] = 1;
       cum += gain[v];
-      move(v, locked);
-      moved.push_back(v);
-      // newly boundary neighbours enter the queues
+      // hedges that this move cuts: their free pins become boundary nodes
+      const uint8_t from = side[v], to = 1 - from;
+      fresh.clear();
       for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
         uint32_t e = h.inc[i];
+        if (cnt[to][e] == 0 && cnt[from][e] >= 2) fresh.push_back(e);
+      }
+      move(v, locked);
+      moved.push_back(v);
+      // side `from` became lighter: nodes parked on side `to` for lack of room may fit now
+      for (uint32_t u : parked[to])
+        if (!locked[u] && side[u] == to && !heap[to].contains(u)) heap[to].push(u, gain[u]);
+      parked[to].clear();
+      for (uint32_t e : fresh)
         for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
           uint32_t u = h.pins[j];
           if (!locked[u] && !heap[side[u]].contains(u)) heap[side[u]].push(u, gain[u]);
         }
-      }
       uint64_t imb = std::max(wside[0] > maxw[0] ? wside[0] - maxw[0] : 0,
                               wside[1] > maxw[1] ? wside[1] - maxw[1] : 0);
       if (imb < best_imb || (imb == best_imb && cum > best)) {
@@ -372,8 +388,31 @@ struct Options {
   uint32_t tries = 4;
   uint32_t fm_passes = 3;
   uint64_t seed = 86;
+  uint32_t parallel_depth = 4;         // threads for tries/halves above this recursion depth
+  uint64_t parallel_min_pins = 20000;  // smaller subproblems stay on the calling thread
 };
 
+struct TryResult {
+  int64_t cut = INT64_MAX;
+  uint64_t imb = UINT64_MAX;
+  std::vector<uint8_t> side;
+};
+
+// One bisection attempt: greedy growing from a seed, then FM passes.
+inline void bisect_try(const Hypergraph& h, uint32_t seed, uint64_t target0, const uint64_t maxw[2],
+                       const Options& opt, TryResult& out) {
+  Bisector b(h);
+  b.grow(seed, target0, maxw[0]);
+  for (uint32_t p = 0; p < opt.fm_passes; ++p)
+    if (b.fm_pass(maxw) <= 0 && p > 0) break;
+  out.cut = b.cut();
+  out.imb = std::max(b.wside[0] > maxw[0] ? b.wside[0] - maxw[0] : 0, b.wside[1] > maxw[1] ? b.wside[1] - maxw[1] : 0);
+  out.side = std::move(b.side);
+}
+
+// Recursive bisection.  The tries of one bisection run on their own threads and the two
+// halves recurse concurrently (down to depth opt.parallel_depth); every thread owns its
+// state and the best try is chosen in try order, so the result does not depend on timing.
 inline void bisect_recursive(const Hypergraph& h, const std::vector<uint32_t>& gids, uint32_t k, uint32_t first,
                              double eps_b, uint64_t omega, const Options& opt, uint32_t* part, uint32_t depth) {
   if (k == 1 || h.n == 0) {
@@ -391,32 +430,41 @@ inline void bisect_recursive(const Hypergraph& h, const std::vector<uint32_t>& g
     maxw[0] = std::min<uint64_t>((uint64_t)omega * k0, total);
     maxw[1] = std::min<uint64_t>((uint64_t)omega * k1, total);
   }
-  std::vector<uint8_t> best_side;
-  int64_t best_cut = INT64_MAX;
-  uint64_t best_imb = UINT64_MAX;
-  Bisector b(h);
-  for (uint32_t t = 0; t < opt.tries; ++t) {
-    uint32_t seed = (uint32_t)(splitmix64(opt.seed * 1315423911ull + depth * 2654435761ull + first * 97ull + t) % h.n);
-    b.grow(seed, target0, maxw[0]);
-    for (uint32_t p = 0; p < opt.fm_passes; ++p)
-      if (b.fm_pass(maxw) <= 0 && p > 0) break;
-    int64_t c = b.cut();
-    uint64_t imb = std::max(b.wside[0] > maxw[0] ? b.wside[0] - maxw[0] : 0,
-                            b.wside[1] > maxw[1] ? b.wside[1] - maxw[1] : 0);
-    if (imb < best_imb || (imb == best_imb && c < best_cut)) {
-      best_cut = c;
-      best_imb = imb;
-      best_side = b.side;
-    }
+  std::vector<TryResult> res(opt.tries);
+  std::vector<uint32_t> seeds(opt.tries);
+  for (uint32_t t = 0; t < opt.tries; ++t)
+    seeds[t] = (uint32_t)(splitmix64(opt.seed * 1315423911ull + depth * 2654435761ull + first * 97ull + t) % h.n);
+  const bool par = depth < opt.parallel_depth && (uint64_t)h.pins.size() >= opt.parallel_min_pins;
+  if (par) {
+    std::vector<std::thread> th;
+    for (uint32_t t = 1; t < opt.tries; ++t)
+      th.emplace_back(bisect_try, std::cref(h), seeds[t], target0, maxw, std::cref(opt), std::ref(res[t]));
+    if (opt.tries) bisect_try(h, seeds[0], target0, maxw, opt, res[0]);
+    for (auto& x : th) x.join();
+  } else {
+    for (uint32_t t = 0; t < opt.tries; ++t) bisect_try(h, seeds[t], target0, maxw, opt, res[t]);
   }
+  uint32_t bi = 0;
+  for (uint32_t t = 1; t < opt.tries; ++t)
+    if (res[t].imb < res[bi].imb || (res[t].imb == res[bi].imb && res[t].cut < res[bi].cut)) bi = t;
+  const std::vector<uint8_t>& best_side = res[bi].side;
   std::vector<uint32_t> l[2], g[2];
   for (uint32_t v = 0; v < h.n; ++v) {
     l[best_side[v]].push_back(v);
     g[best_side[v]].push_back(gids[v]);
   }
-  for (int s = 0; s < 2; ++s) {
+  res.clear();
+  auto half = [&](int s) {
     Hypergraph sub = subgraph(h, l[s]);
     bisect_recursive(sub, g[s], s == 0 ? k0 : k1, s == 0 ? first : first + k0, eps_b, omega, opt, part, depth + 1);
+  };
+  if (par) {
+    std::thread t0(half, 0);
+    half(1);
+    t0.join();
+  } else {
+    half(0);
+    half(1);
   }
 }
 

@@ -224,13 +224,18 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits) {
   uint32_t* va = vals;
   uint32_t* vb = v2.get();
   for (int shift = 0; shift < bits; shift += 8) {
-    HGP_LAUNCH((k_rs_hist<K>), nblocks, RS_BLOCK, 0, c, ka, n, shift, nblocks, counts.get());
+    // bytes: keys read once, one count per (digit, tile) written
+    HGP_LAUNCH_B(n * sizeof(K) + 1024ull * nblocks, (k_rs_hist<K>), nblocks, RS_BLOCK, 0, c, ka, n, shift, nblocks,
+                 counts.get());
     exclusive_scan(c, counts.get(), offsets.get(), (uint64_t)256 * nblocks);
+    // bytes: keys (and values) read once and written once, one offset per (digit, tile) read
+    const uint64_t sb = n * (2 * sizeof(K) + (vals ? 8 : 0)) + 2048ull * nblocks;
     if (vals)
-      HGP_LAUNCH((k_rs_scatter<K, true>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks, offsets.get());
+      HGP_LAUNCH_B(sb, (k_rs_scatter<K, true>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks,
+                   offsets.get());
     else
-      HGP_LAUNCH((k_rs_scatter<K, false>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks,
-                 offsets.get());
+      HGP_LAUNCH_B(sb, (k_rs_scatter<K, false>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks,
+                   offsets.get());
     std::swap(ka, kb);
     std::swap(va, vb);
   }

@@ -26,6 +26,7 @@ namespace hgp {
 namespace {
 
 constexpr uint32_t PP_SMALL = 16;
+constexpr uint32_t PP_HIST_CAP = 8192;  // partitions whose inbound counts aggregate in LDS
 constexpr uint32_t PP_LDS_CAP = 4096;  // 3 x 16 KiB of dynamic LDS per wave
 constexpr uint32_t GAIN_CAP = 1024;
 constexpr uint32_t GAIN_LIGHT = 512;
@@ -44,44 +45,66 @@ struct Best {
 __device__ __forceinline__ uint64_t ordered_i64(int64_t g) { return (uint64_t)g ^ 0x8000000000000000ull; }
 
 // ---- 1. per-hedge partition lists ------------------------------------------------------
-__global__ void k_pp_small(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
-                           const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part, uint32_t E,
-                           uint32_t* __restrict__ pp_part, uint32_t* __restrict__ pp_cnt, uint32_t* __restrict__ pp_dcnt,
-                           uint32_t* __restrict__ lambda, uint32_t* __restrict__ large_flag, uint32_t* inbsz) {
+// Per-hedge partition lists for hedges of at most PP_SMALL pins (one thread per hedge).
+// With INB, also the destination counts and the per-partition inbound sizes; those are
+// aggregated in an LDS histogram when the partition count fits it (k_lds > 0), so a few
+// hot partitions do not serialize millions of global atomics.
+template <bool INB>
+__global__ void __launch_bounds__(256) k_pp_small(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
+                                                  const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part,
+                                                  uint32_t E, uint32_t* __restrict__ pp_part, uint32_t* __restrict__ pp_cnt,
+                                                  uint32_t* __restrict__ pp_dcnt, uint32_t* __restrict__ lambda,
+                                                  uint32_t* __restrict__ large_flag, uint32_t* inbsz, uint32_t k_lds) {
+  extern __shared__ uint32_t hist[];
+  if (INB && k_lds) {
+    for (uint32_t i = threadIdx.x; i < k_lds; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+  }
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  const uint64_t s = hoff[e], t = hoff[e + 1];
-  if (t - s > PP_SMALL) {
-    large_flag[e] = 1u;
-    return;
-  }
-  large_flag[e] = 0u;
-  uint32_t qs[PP_SMALL], cs[PP_SMALL], ds[PP_SMALL];
-  uint32_t m = 0;
-  const uint32_t sc = srcc[e];
-  for (uint64_t i = s; i < t; ++i) {
-    const uint32_t q = part[hedges[i]];
-    const uint32_t d = (i - s) >= sc ? 1u : 0u;
-    uint32_t k = 0;
-    while (k < m && qs[k] != q) ++k;
-    if (k == m) {
-      qs[m] = q;
-      cs[m] = 0;
-      ds[m] = 0;
-      ++m;
+  if (e < E) {
+    const uint64_t s = hoff[e], t = hoff[e + 1];
+    if (t - s > PP_SMALL) {
+      large_flag[e] = 1u;
+    } else {
+      large_flag[e] = 0u;
+      uint32_t qs[PP_SMALL], cs[PP_SMALL], ds[PP_SMALL];
+      uint32_t m = 0;
+      const uint32_t sc = INB ? srcc[e] : 0u;
+      for (uint64_t i = s; i < t; ++i) {
+        const uint32_t q = part[hedges[i]];
+        uint32_t k = 0;
+        while (k < m && qs[k] != q) ++k;
+        if (k == m) {
+          qs[m] = q;
+          cs[m] = 0;
+          ds[m] = 0;
+          ++m;
+        }
+        cs[k] += 1;
+        if (INB) ds[k] += (i - s) >= sc ? 1u : 0u;
+      }
+      for (uint32_t k = 0; k < m; ++k) {
+        pp_part[s + k] = qs[k];
+        pp_cnt[s + k] = cs[k];
+        if (INB) {
+          pp_dcnt[s + k] = ds[k];
+          if (ds[k]) {
+            if (k_lds) atomicAdd(&hist[qs[k]], 1u);
+            else atomicAdd(&inbsz[qs[k]], 1u);
+          }
+        }
+      }
+      lambda[e] = m;
     }
-    cs[k] += 1;
-    ds[k] += d;
   }
-  for (uint32_t k = 0; k < m; ++k) {
-    pp_part[s + k] = qs[k];
-    pp_cnt[s + k] = cs[k];
-    pp_dcnt[s + k] = ds[k];
-    if (ds[k]) atomicAdd(&inbsz[qs[k]], 1u);
+  if (INB && k_lds) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < k_lds; i += blockDim.x)
+      if (hist[i]) atomicAdd(&inbsz[i], hist[i]);
   }
-  lambda[e] = m;
 }
 
+template <bool INB>
 __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
                                                  const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part,
                                                  const uint32_t* __restrict__ list, uint32_t cap,
@@ -111,7 +134,7 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
       h = (h + 1) & (cap - 1);
     }
     atomicAdd(&cnt[h], 1u);
-    if ((i - s) >= sc) atomicAdd(&dcnt[h], 1u);
+    if (INB && (i - s) >= sc) atomicAdd(&dcnt[h], 1u);
   }
   __syncthreads();
   uint32_t base = 0;
@@ -123,8 +146,10 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
       const uint32_t o = base + lanes_below(m);
       pp_part[s + o] = keys[i];
       pp_cnt[s + o] = cnt[i];
-      pp_dcnt[s + o] = dcnt[i];
-      if (dcnt[i]) atomicAdd(&inbsz[keys[i]], 1u);
+      if (INB) {
+        pp_dcnt[s + o] = dcnt[i];
+        if (dcnt[i]) atomicAdd(&inbsz[keys[i]], 1u);
+      }
     }
     base += (uint32_t)__popcll(m);
   }
@@ -492,6 +517,10 @@ __global__ void k_chain_ranges(ChainArrays c, const uint64_t* __restrict__ poff,
 }
 __device__ __forceinline__ int size_bucket(uint32_t s) { return s == 0 ? 0 : 31 - __clz(s); }
 
+// Each open move proposes to the best free move out of its destination partition among the
+// first 256 entries of that partition's list.  The list is sorted by gain (descending, valid
+// moves first), so once a choice exists the scan stops at the first lower gain or invalid
+// entry: nothing later can beat it.
 __global__ void k_chain_propose(ChainArrays c, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -504,11 +533,13 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n) {
     int64_t bw = 0;
     uint32_t bd = 0;
     for (uint32_t j = c.ob[i]; j < lim; ++j) {
-      if (j == i || c.prev[j] != NONE || !c.ok[j]) continue;
+      if (!c.ok[j]) break;
+      const int64_t wj = c.w[j];
+      if (choice != NONE && wj < bw) break;
+      if (j == i || c.prev[j] != NONE) continue;
       const uint32_t sj = c.sz[j];
       if (abs(bi - size_bucket(sj)) > 1) continue;
       const uint32_t d = si > sj ? si - sj : sj - si;
-      const int64_t wj = c.w[j];
       if (choice == NONE || wj > bw || (wj == bw && d < bd)) {
         choice = j;
         bw = wj;
@@ -518,14 +549,34 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n) {
   }
   c.choice[i] = choice;
 }
+// Claims: the proposer with the smallest size difference, then the lowest index, wins.
+// Proposals pile onto few targets (the heads of the partition lists), so lanes of a wave
+// that share a target first agree on their best key and one lane issues the atomic.
 __global__ void k_chain_resolve(ChainArrays c, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t s = c.choice[i];
-  if (s == NONE || c.prev[s] != NONE) return;
-  const uint32_t d = c.sz[i] > c.sz[s] ? c.sz[i] - c.sz[s] : c.sz[s] - c.sz[i];
-  const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - d) << 32) | (0xFFFFFFFFu - i);
-  atomicMax(&c.claim[s], key);
+  uint32_t s = NONE;
+  unsigned long long key = 0ull;
+  if (i < n) {
+    s = c.choice[i];
+    if (s != NONE && c.prev[s] == NONE) {
+      const uint32_t d = c.sz[i] > c.sz[s] ? c.sz[i] - c.sz[s] : c.sz[s] - c.sz[i];
+      key = ((unsigned long long)(0xFFFFFFFFu - d) << 32) | (0xFFFFFFFFu - i);
+    } else {
+      s = NONE;
+    }
+  }
+  bool act = s != NONE;
+  for (int it = 0; it < 8; ++it) {
+    const uint64_t am = __ballot(act);
+    if (!am) break;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)am) - 1u;
+    const uint32_t s0 = __shfl(s, leader);
+    const bool same = act && s == s0;
+    const unsigned long long m = wave_max(same ? key : 0ull);
+    if (lane_id() == leader) atomicMax(&c.claim[s0], m);
+    if (same) act = false;
+  }
+  if (act) atomicMax(&c.claim[s], key);
 }
 __global__ void k_chain_commit(ChainArrays c, uint32_t n) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -927,17 +978,27 @@ struct PartitionLists {
   DBuf<uint32_t> part, cnt, dcnt, lambda, inbsz;
 };
 
-void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, PartitionLists& pl) {
+// Partition lists of every hedge.  want_inbound adds the destination counts and the
+// per-partition inbound sizes (the inbound-set constraint needs them; k-way does not).
+void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, PartitionLists& pl, bool want_inbound) {
   const uint64_t P = L.e ? read_scalar(c, L.hoff + L.e) : 0;
   pl.part = DBuf<uint32_t>(c, P);
   pl.cnt = DBuf<uint32_t>(c, P);
-  pl.dcnt = DBuf<uint32_t>(c, P);
   pl.lambda = DBuf<uint32_t>(c, L.e);
-  pl.inbsz = DBuf<uint32_t>(c, std::max<uint32_t>(L.k, 1));
-  memset_async(c, pl.inbsz.get(), 0, (uint64_t)L.k * sizeof(uint32_t));
+  if (want_inbound) {
+    pl.dcnt = DBuf<uint32_t>(c, P);
+    pl.inbsz = DBuf<uint32_t>(c, std::max<uint32_t>(L.k, 1));
+    memset_async(c, pl.inbsz.get(), 0, (uint64_t)L.k * sizeof(uint32_t));
+  }
   DBuf<uint32_t> lflag(c, L.e);
-  HGP_LAUNCH(k_pp_small, ceil_div(L.e, 256), 256, 0, c, L.hedges, L.hoff, L.srcc, part, L.e, pl.part.get(), pl.cnt.get(),
-             pl.dcnt.get(), pl.lambda.get(), lflag.get(), pl.inbsz.get());
+  if (want_inbound) {
+    const uint32_t k_lds = L.k <= PP_HIST_CAP ? L.k : 0u;
+    HGP_LAUNCH((k_pp_small<true>), ceil_div(L.e, 256), 256, k_lds * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part,
+               L.e, pl.part.get(), pl.cnt.get(), pl.dcnt.get(), pl.lambda.get(), lflag.get(), pl.inbsz.get(), k_lds);
+  } else {
+    HGP_LAUNCH((k_pp_small<false>), ceil_div(L.e, 256), 256, 0, c, L.hedges, L.hoff, L.srcc, part, L.e, pl.part.get(),
+               pl.cnt.get(), (uint32_t*)nullptr, pl.lambda.get(), lflag.get(), (uint32_t*)nullptr, 0u);
+  }
   DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
   exclusive_scan(c, lflag.get(), pos.get(), L.e);
   const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.e);
@@ -949,8 +1010,12 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
   HGP_REQUIRE(L.k <= PP_LDS_CAP / 2, HGP_E_CAPACITY, "partition lists: more than 2048 partitions are not supported");
   uint32_t cap = 128;
   while (cap < 2ull * L.k + 64 && cap < PP_LDS_CAP) cap <<= 1;
-  HGP_LAUNCH(k_pp_large, nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, large.get(), cap,
-             pl.part.get(), pl.cnt.get(), pl.dcnt.get(), pl.lambda.get(), pl.inbsz.get());
+  if (want_inbound)
+    HGP_LAUNCH((k_pp_large<true>), nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, large.get(),
+               cap, pl.part.get(), pl.cnt.get(), pl.dcnt.get(), pl.lambda.get(), pl.inbsz.get());
+  else
+    HGP_LAUNCH((k_pp_large<false>), nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, large.get(),
+               cap, pl.part.get(), pl.cnt.get(), (uint32_t*)nullptr, pl.lambda.get(), (uint32_t*)nullptr);
 }
 
 uint32_t discount_of(uint32_t rep, uint32_t R) { return rep < R / 3 ? 1u : (rep < 2 * R / 3 ? 2u : NONE); }
@@ -1110,7 +1175,7 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
   const uint32_t g = ceil_div(n, 256);
   const bool incc = mode == HGP_MODE_INCC;
   PartitionLists pl;
-  build_partition_lists(c, L, part, pl);
+  build_partition_lists(c, L, part, pl, incc);
   st.moves = DBuf<uint32_t>(c, n);
   st.gains = DBuf<int64_t>(c, n);
   st.ranks = DBuf<uint32_t>(c, n);
@@ -1284,7 +1349,7 @@ void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const u
   L.hoff = hoff;
   L.srcc = srcc;
   PartitionLists pl;
-  build_partition_lists(c, L, part, pl);
+  build_partition_lists(c, L, part, pl, true);
   copy_d2d(c, out, pl.inbsz.get(), (uint64_t)k * 4);
 }
 
@@ -1309,7 +1374,7 @@ int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff,
   L.hoff = hoff;
   L.srcc = srcc0.get();
   PartitionLists pl;
-  build_partition_lists(c, L, part, pl);
+  build_partition_lists(c, L, part, pl, false);
   DBuf<unsigned long long> acc(c, 1);
   memset_async(c, acc.get(), 0, 8);
   HGP_LAUNCH(k_connectivity, ceil_div(e, 256), 256, 0, c, pl.lambda.get(), w, e, acc.get());

@@ -272,6 +272,21 @@ hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const u
 hgp_status hgp_partition(hgp_ctx* ctx, const hgp_params* params, const hgp_hgraph* hg,
                          uint32_t* h_partitions_out, hgp_run_stats* stats);
 
+/* ---- kernel profiler (no counterpart in the reference) --------------------
+ * Times every launch of one kernel (by name, e.g. "k_cand_light"; NULL or "" stops) with
+ * HIP events on the context stream; hgp_profile_read synchronizes, returns the summed
+ * device time, the launch count and the launches' algorithmic bytes (0 for kernels without
+ * a byte model), and clears the record. */
+hgp_status hgp_profile_kernel(hgp_ctx* ctx, const char* kernel_name);
+hgp_status hgp_profile_read(hgp_ctx* ctx, double* total_ms, uint64_t* launches, uint64_t* algorithmic_bytes);
+
+/* Same pipeline on a hypergraph already resident in HBM (sources first, rule-2
+ * deduplicated as hgp_hgraph_load leaves it).  The inputs are only read. */
+hgp_status hgp_partition_device(hgp_ctx* ctx, const hgp_params* params, uint32_t num_nodes, uint32_t num_hedges,
+                                const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                                const uint32_t* d_srcs_count, const float* d_hedge_weights,
+                                uint32_t* h_partitions_out, hgp_run_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
