@@ -30,8 +30,12 @@ constexpr uint32_t PP_HIST_CAP = 8192;  // partitions whose inbound counts aggre
 constexpr uint32_t PP_LDS_CAP = 4096;  // 3 x 16 KiB of dynamic LDS per wave
 constexpr uint32_t GAIN_CAP = 1024;
 constexpr uint32_t GAIN_LIGHT = 512;
+constexpr uint32_t GAIN_TINY = 16;  // partition entries handled by one thread in registers
 constexpr uint32_t CAS_SMALL = 16;
 constexpr uint32_t CAS_LDS = 2048;
+constexpr uint32_t CAS_SORT = 1024;               // pins of a hedge cascaded through sorted events
+constexpr uint32_t CAS_SORT_S = 128;              // ... by the small-LDS instance (4x event slots)
+constexpr uint32_t CAS_EV = 4096;                 // event slots: next power of two >= 3 * CAS_SORT
 constexpr uint64_t HEAVY_BATCH_SLOTS = 32ull << 20;
 
 struct Best {
@@ -203,7 +207,7 @@ __device__ __forceinline__ void best_adjacent(const GainArgs& a, const uint32_t*
   bh = 0;
   bq = NONE;
   for (uint32_t i = first; i < cap; i += stride) {
-    const uint32_t q = keys[i];
+    const uint32_t q = keys ? keys[i] : i;  // keys == nullptr: table indexed by partition
     if (q == NONE) continue;
     const int64_t c = conn[i];
     if (c <= 0 || !target_ok(a, q, p, s)) continue;
@@ -249,7 +253,7 @@ __device__ __forceinline__ uint32_t first_nonadjacent(const GainArgs& a, const u
     if (i < a.k) {
       q = a.order[i];
       if (target_ok(a, q, p, s)) {
-        const uint32_t slot = table_find(keys, mask, q);
+        const uint32_t slot = keys ? table_find(keys, mask, q) : q;
         ok = slot == NONE || conn[slot] <= 0;
       }
     }
@@ -259,17 +263,109 @@ __device__ __forceinline__ uint32_t first_nonadjacent(const GainArgs& a, const u
   return NONE;
 }
 
-__global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t* __restrict__ list, uint32_t nlist) {
-  __shared__ uint32_t s_keys[4][GAIN_CAP];
-  __shared__ int64_t s_conn[4][GAIN_CAP];
+// Nodes whose touching hedges list at most GAIN_TINY partition entries: one thread per node,
+// the (partition, connection) table held in registers.
+__global__ void __launch_bounds__(256) k_gains_tiny(GainArgs a, const uint32_t* __restrict__ list, uint32_t nlist) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nlist) return;
+  const uint32_t v = list[i];
+  const uint32_t p = a.part[v], s = a.sizes[v];
+  uint32_t qs[GAIN_TINY];
+  int64_t cs[GAIN_TINY];
+#pragma unroll
+  for (uint32_t k = 0; k < GAIN_TINY; ++k) {
+    qs[k] = NONE;
+    cs[k] = 0;
+  }
+  uint32_t m = 0;
+  int64_t saving = 0, Wv = 0;
+  for (uint64_t j = a.toff[v]; j < a.toff[v + 1]; ++j) {
+    const uint32_t e = a.tch[j];
+    const int64_t We = (int64_t)fixed_weight(a.w[e]);
+    Wv += We;
+    const uint64_t b = a.hoff[e];
+    const uint32_t lam = a.lambda[e];
+    for (uint32_t x = 0; x < lam; ++x) {
+      const uint32_t q = a.pp_part[b + x];
+      if (q == p) {
+        const uint32_t c = a.pp_cnt[b + x];
+        if (a.incc) saving += (int64_t)((uint64_t)We / ((uint64_t)c * c));
+        else if (c == 1) saving += We;
+        continue;
+      }
+      bool found = false;
+#pragma unroll
+      for (uint32_t k = 0; k < GAIN_TINY; ++k) {
+        const bool hit = qs[k] == q;
+        cs[k] += hit ? We : 0;
+        found |= hit;
+      }
+      if (!found) {
+#pragma unroll
+        for (uint32_t k = 0; k < GAIN_TINY; ++k) {
+          const bool put = k == m;
+          qs[k] = put ? q : qs[k];
+          cs[k] = put ? We : cs[k];
+        }
+        ++m;
+      }
+    }
+  }
+  const int64_t base = saving - Wv;
+  int64_t bg = INT64_MIN;
+  uint32_t bh = 0, bq = NONE;
+#pragma unroll
+  for (uint32_t k = 0; k < GAIN_TINY; ++k) {
+    const uint32_t q = qs[k];
+    if (q == NONE || cs[k] <= 0 || !target_ok(a, q, p, s)) continue;
+    const int64_t g = base + cs[k];
+    const uint32_t h = mix32(q + a.repeat);
+    if (bq == NONE || g > bg || (g == bg && h > bh)) {
+      bg = g;
+      bh = h;
+      bq = q;
+    }
+  }
+  if (bq == NONE) {  // first valid partition in hash order without a positive connection
+    for (uint32_t x = 0; x < a.k; ++x) {
+      const uint32_t q = a.order[x];
+      if (!target_ok(a, q, p, s)) continue;
+      bool adjacent = false;
+#pragma unroll
+      for (uint32_t k = 0; k < GAIN_TINY; ++k) adjacent |= qs[k] == q && cs[k] > 0;
+      if (!adjacent) {
+        bq = q;
+        break;
+      }
+    }
+    bg = base;
+  }
+  a.moves[v] = bq;
+  a.gains[v] = bq == NONE ? 0 : bg;
+}
+
+// One wave per node.  With at most GAIN_CAP partitions the LDS table is indexed by partition
+// (k slots); otherwise it is an open-addressing table sized to twice the node's possible
+// distinct partitions.
+// Dynamic LDS: 4 waves x capmax slots of (int64 connection, uint32 key).
+__global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t* __restrict__ list, uint32_t nlist,
+                                                     const uint64_t* __restrict__ bound, uint32_t capmax) {
+  extern __shared__ int64_t s_conn_all[];
   const uint32_t wv = threadIdx.x / 64, l = lane_id();
   const uint32_t idx = blockIdx.x * 4 + wv;
   if (idx >= nlist) return;
   const uint32_t v = list[idx];
-  uint32_t* keys = s_keys[wv];
-  int64_t* conn = s_conn[wv];
-  for (uint32_t i = l; i < GAIN_CAP; i += 64) {
-    keys[i] = NONE;
+  const bool direct = a.k <= GAIN_CAP;
+  uint32_t cap = a.k;
+  if (!direct) {
+    const uint64_t d = bound[v] < (uint64_t)a.k ? bound[v] : (uint64_t)a.k;
+    cap = 64;
+    while (cap < 2 * d + 64 && cap < GAIN_CAP) cap <<= 1;
+  }
+  uint32_t* keys = direct ? nullptr : (uint32_t*)(s_conn_all + 4 * capmax) + wv * capmax;
+  int64_t* conn = s_conn_all + wv * capmax;
+  for (uint32_t i = l; i < cap; i += 64) {
+    if (!direct) keys[i] = NONE;
     conn[i] = 0;
   }
   __builtin_amdgcn_wave_barrier();
@@ -290,11 +386,14 @@ __global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t*
           else if (c == 1) saving += We;
           return;
         }
-        uint32_t h = slot_hash(q) & (GAIN_CAP - 1);
-        for (;;) {
-          const uint32_t old = atomicCAS(&keys[h], NONE, q);
-          if (old == NONE || old == q) break;
-          h = (h + 1) & (GAIN_CAP - 1);
+        uint32_t h = q;
+        if (!direct) {
+          h = slot_hash(q) & (cap - 1);
+          for (;;) {
+            const uint32_t old = atomicCAS(&keys[h], NONE, q);
+            if (old == NONE || old == q) break;
+            h = (h + 1) & (cap - 1);
+          }
         }
         atomicAdd((unsigned long long*)&conn[h], (unsigned long long)We);
       });
@@ -305,10 +404,10 @@ __global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t*
   const int64_t base = saving - Wv;
   int64_t bg;
   uint32_t bh, bq;
-  best_adjacent(a, keys, conn, GAIN_CAP, l, 64, p, s, base, bg, bh, bq);
+  best_adjacent(a, keys, conn, cap, l, 64, p, s, base, bg, bh, bq);
   wave_best(bg, bh, bq);
   if (bq == NONE) {
-    bq = first_nonadjacent(a, keys, conn, GAIN_CAP - 1, p, s);
+    bq = first_nonadjacent(a, keys, conn, cap - 1, p, s);
     bg = base;
   }
   if (l == 0) {
@@ -403,15 +502,35 @@ __global__ void __launch_bounds__(256) k_gains_heavy(GainArgs a, const uint32_t*
   }
 }
 
+// bound = partition entries listed by the node's touching hedges; tiers: tiny (thread per
+// node), light (wave per node), heavy (workgroup per node, table in HBM)
 __global__ void k_gain_bound(const uint32_t* __restrict__ tch, const uint64_t* __restrict__ toff,
                              const uint32_t* __restrict__ lambda, uint32_t n, uint32_t k, uint64_t* __restrict__ bound,
-                             uint32_t* __restrict__ light) {
+                             uint32_t* __restrict__ tiny, uint32_t* __restrict__ light) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   uint64_t b = 0;
   for (uint64_t j = toff[v]; j < toff[v + 1]; ++j) b += lambda[tch[j]];
   bound[v] = b;
-  light[v] = (b <= GAIN_LIGHT || k <= GAIN_LIGHT) ? 1u : 0u;
+  const bool t = b <= GAIN_TINY;
+  tiny[v] = t ? 1u : 0u;
+  light[v] = (!t && (b <= GAIN_LIGHT || k <= GAIN_LIGHT)) ? 1u : 0u;
+}
+
+__global__ void k_compact(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint32_t n,
+                          uint32_t* __restrict__ out) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n && flag[v]) out[pos[v]] = v;
+}
+
+__global__ void k_split3(const uint32_t* __restrict__ f_a, const uint32_t* __restrict__ f_b,
+                         const uint64_t* __restrict__ apos, const uint64_t* __restrict__ bpos, uint32_t n,
+                         uint32_t* __restrict__ la, uint32_t* __restrict__ lb, uint32_t* __restrict__ lc) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  if (f_a[v]) la[apos[v]] = v;
+  else if (f_b[v]) lb[bpos[v]] = v;
+  else lc[v - apos[v] - bpos[v]] = v;
 }
 
 __global__ void k_split(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint32_t n,
@@ -664,15 +783,19 @@ __device__ __forceinline__ int64_t cascade_term(int incc, int64_t W, int64_t cp,
   return t;
 }
 
-__global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t* __restrict__ large_flag) {
+// Hedges of at most CAS_SMALL pins, one thread each; larger ones are flagged for the sorted
+// kernels (up to CAS_SORT_S or sort_max pins) or the direct LDS/HBM kernel.
+__global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, uint32_t* __restrict__ f_s,
+                                uint32_t* __restrict__ f_m, uint32_t* __restrict__ f_big) {
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   const uint64_t s = a.hoff[e], t = a.hoff[e + 1];
-  if (t - s > CAS_SMALL) {
-    large_flag[e] = 1u;
-    return;
-  }
-  large_flag[e] = 0u;
+  const uint64_t hl = t - s;
+  const bool sorted = hl > CAS_SMALL && hl <= sort_max;
+  f_s[e] = (sorted && hl <= CAS_SORT_S) ? 1u : 0u;
+  f_m[e] = (sorted && hl > CAS_SORT_S) ? 1u : 0u;
+  f_big[e] = (hl > CAS_SMALL && !sorted) ? 1u : 0u;
+  if (hl > CAS_SMALL) return;
   uint32_t pt[CAS_SMALL], mv[CAS_SMALL], rk[CAS_SMALL];
   const uint32_t len = (uint32_t)(t - s);
   bool any = false;
@@ -745,6 +868,114 @@ __global__ void __launch_bounds__(64) k_cascade_large(CascadeArgs a, const uint3
       }
     }
     atomicAdd(&a.seq[r], (unsigned long long)cascade_term(a.incc, W, cp, cq));
+  }
+}
+
+// Cascade of one hedge through sorted events (hedges of CAS_SMALL < len <= CAS_SORT pins,
+// one wave per hedge).  Every pin contributes an "init" event (its partition, +1) and every
+// moving pin an "out" event (source, -1) and an "in" event (target, +1) at its rank.  Sorted
+// by (partition, rank), the exclusive prefix of a move's out event is the pin count of its
+// source just before the move (the pin itself included) and that of its in event is the
+// count of its target: exactly c_p and c_q of the cascade term, in O(len log^2 len) instead
+// of O(len * moving pins).
+// key = partition << (rb + 11) | rank' << 11 | dir << 10 | local index (rank' = 0 for init,
+// rank + 1 for moves; dir 1 = in event).
+__device__ __forceinline__ int32_t wave_seg_scan(int32_t v, bool& flag) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const int32_t pv = __shfl_up(v, d, 64);
+    const int pf = __shfl_up((int)flag, d, 64);
+    if (l >= d) {
+      if (!flag) v += pv;
+      flag = flag || pf;
+    }
+  }
+  return v;
+}
+
+template <uint32_t PINS, uint32_t EVS>
+__global__ void __launch_bounds__(64) k_cascade_sorted(CascadeArgs a, const uint32_t* __restrict__ list, int rb) {
+  __shared__ uint64_t s_ev[EVS];
+  __shared__ uint32_t s_rk[PINS];
+  __shared__ int32_t s_cp[PINS], s_cq[PINS];
+  const uint32_t e = list[blockIdx.x];
+  const uint32_t l = lane_id();
+  const uint64_t s = a.hoff[e];
+  const uint32_t len = (uint32_t)(a.hoff[e + 1] - s);
+  // init events at [0, len), move events appended after them
+  uint32_t moving = 0;
+  for (uint32_t i = l; i < len; i += 64) {
+    const uint32_t x = a.hedges[s + i];
+    const uint32_t p = a.part[x], q = a.moves[x];
+    s_ev[i] = ((uint64_t)p << (rb + 11)) | i;
+    s_rk[i] = q == NONE ? NONE : a.ranks[x];
+    moving += q != NONE;
+  }
+  moving = wave_sum(moving);
+  if (moving == 0) return;
+  const uint32_t nev = len + 2 * moving;
+  uint32_t ev_n = 64;
+  while (ev_n < nev) ev_n <<= 1;
+  __syncthreads();
+  uint32_t base = len;
+  for (uint32_t b = 0; b < len; b += 64) {
+    const uint32_t i = b + l;
+    const bool mv = i < len && s_rk[i] != NONE;
+    const uint64_t mk = __ballot(mv);
+    if (mv) {
+      const uint32_t o = base + 2 * lanes_below(mk);
+      const uint32_t x = a.hedges[s + i];
+      const uint64_t r1 = (uint64_t)s_rk[i] + 1;
+      s_ev[o] = ((uint64_t)a.part[x] << (rb + 11)) | (r1 << 11) | i;
+      s_ev[o + 1] = ((uint64_t)a.moves[x] << (rb + 11)) | (r1 << 11) | (1u << 10) | i;
+    }
+    base += 2 * (uint32_t)__popcll(mk);
+  }
+  for (uint32_t i = nev + l; i < ev_n; i += 64) s_ev[i] = ~0ull;
+  __syncthreads();
+  // bitonic sort, ascending
+  for (uint32_t k = 2; k <= ev_n; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t t = l; t < ev_n / 2; t += 64) {
+        const uint32_t i = (t / j) * 2 * j + (t % j);
+        const uint32_t ixj = i + j;
+        const uint64_t x = s_ev[i], y = s_ev[ixj];
+        const bool up = (i & k) == 0;
+        if ((x > y) == up) {
+          s_ev[i] = y;
+          s_ev[ixj] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // segmented (by partition) exclusive prefix of the event deltas
+  const int shift = rb + 11;
+  int32_t carry = 0;
+  for (uint32_t b = 0; b < nev; b += 64) {
+    const uint32_t i = b + l;
+    const bool act = i < nev;
+    const uint64_t key = act ? s_ev[i] : ~0ull;
+    const uint32_t rank1 = (uint32_t)((key >> 11) & (((uint64_t)1 << rb) - 1));
+    const bool dir_in = (key >> 10) & 1u;
+    const int32_t d = !act ? 0 : (rank1 == 0 ? 1 : (dir_in ? 1 : -1));
+    bool flag = act && (i == 0 || (s_ev[i - 1] >> shift) != (key >> shift));
+    int32_t v = wave_seg_scan(d, flag);
+    if (!flag) v += carry;  // segment continues from the previous chunk
+    if (act && rank1 != 0) {
+      const uint32_t idx = (uint32_t)(key & 1023u);
+      if (dir_in) s_cq[idx] = v - d;
+      else s_cp[idx] = v - d;
+    }
+    carry = __shfl(v, 63, 64);
+  }
+  __syncthreads();
+  const int64_t W = (int64_t)fixed_weight(a.w[e]);
+  for (uint32_t i = l; i < len; i += 64) {
+    const uint32_t r = s_rk[i];
+    if (r == NONE) continue;
+    atomicAdd(&a.seq[r], (unsigned long long)cascade_term(a.incc, W, s_cp[i], s_cq[i]));
   }
 }
 
@@ -1060,16 +1291,21 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   a.moves = moves;
   a.gains = gains;
   DBuf<uint64_t> bound(c, L.n);
-  DBuf<uint32_t> light(c, L.n);
+  DBuf<uint32_t> tiny(c, L.n), light(c, L.n);
   HGP_LAUNCH(k_gain_bound, ceil_div(L.n, 256), 256, 0, c, L.tch, L.toff, pl.lambda.get(), L.n, L.k, bound.get(),
-             light.get());
-  DBuf<uint64_t> pos(c, (uint64_t)L.n + 1);
+             tiny.get(), light.get());
+  DBuf<uint64_t> tpos(c, (uint64_t)L.n + 1), pos(c, (uint64_t)L.n + 1);
+  exclusive_scan(c, tiny.get(), tpos.get(), L.n);
   exclusive_scan(c, light.get(), pos.get(), L.n);
+  const uint32_t nt = (uint32_t)read_scalar(c, tpos.get() + L.n);
   const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.n);
-  const uint32_t nh = L.n - nl;
-  DBuf<uint32_t> ll(c, nl), hl(c, nh);
-  HGP_LAUNCH(k_split, ceil_div(L.n, 256), 256, 0, c, light.get(), pos.get(), L.n, ll.get(), hl.get());
-  HGP_LAUNCH(k_gains_light, ceil_div(nl, 4), 256, 0, c, a, ll.get(), nl);
+  const uint32_t nh = L.n - nt - nl;
+  DBuf<uint32_t> tl(c, nt), ll(c, nl), hl(c, nh);
+  HGP_LAUNCH(k_split3, ceil_div(L.n, 256), 256, 0, c, tiny.get(), light.get(), tpos.get(), pos.get(), L.n, tl.get(),
+             ll.get(), hl.get());
+  HGP_LAUNCH(k_gains_tiny, ceil_div(nt, 256), 256, 0, c, a, tl.get(), nt);
+  const uint32_t capmax = L.k <= GAIN_CAP ? std::max<uint32_t>(L.k, 1) : GAIN_CAP;
+  HGP_LAUNCH(k_gains_light, ceil_div(nl, 4), 256, 4 * capmax * 12, c, a, ll.get(), nl, bound.get(), capmax);
   sync(c);  // `order` host buffer
   if (nh == 0) return;
   DBuf<uint64_t> caps(c, nh);
@@ -1200,15 +1436,21 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
   // cascade
   {
     CascadeArgs a{L.hedges, L.hoff, L.w, part, st.moves.get(), st.ranks.get(), incc ? 1 : 0, st.seq.get()};
-    DBuf<uint32_t> lflag(c, L.e);
-    HGP_LAUNCH(k_cascade_small, ceil_div(L.e, 256), 256, 0, c, a, L.e, lflag.get());
+    const int crb = std::max(1, bit_width(V));  // rank + 1 <= V
+    const int cpb = std::max(1, bit_width(L.k));
+    const uint32_t sort_max = cpb + crb + 11 < 64 ? CAS_SORT : 0u;
+    DBuf<uint32_t> fl[3] = {DBuf<uint32_t>(c, L.e), DBuf<uint32_t>(c, L.e), DBuf<uint32_t>(c, L.e)};
+    HGP_LAUNCH(k_cascade_small, ceil_div(L.e, 256), 256, 0, c, a, L.e, sort_max, fl[0].get(), fl[1].get(), fl[2].get());
     DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
-    exclusive_scan(c, lflag.get(), pos.get(), L.e);
-    const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.e);
-    if (nl) {
-      DBuf<uint32_t> large(c, nl), small(c, L.e - nl);
-      HGP_LAUNCH(k_split, ceil_div(L.e, 256), 256, 0, c, lflag.get(), pos.get(), L.e, large.get(), small.get());
-      HGP_LAUNCH(k_cascade_large, nl, 64, 0, c, a, large.get());
+    for (int cls = 0; cls < 3; ++cls) {
+      exclusive_scan(c, fl[cls].get(), pos.get(), L.e);
+      const uint32_t m = (uint32_t)read_scalar(c, pos.get() + L.e);
+      if (m == 0) continue;
+      DBuf<uint32_t> lst(c, m);
+      HGP_LAUNCH(k_compact, ceil_div(L.e, 256), 256, 0, c, fl[cls].get(), pos.get(), L.e, lst.get());
+      if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 4 * CAS_SORT_S>), m, 64, 0, c, a, lst.get(), crb);
+      else if (cls == 1) HGP_LAUNCH((k_cascade_sorted<CAS_SORT, CAS_EV>), m, 64, 0, c, a, lst.get(), crb);
+      else HGP_LAUNCH(k_cascade_large, m, 64, 0, c, a, lst.get());
     }
     inclusive_scan(c, (const int64_t*)st.seq.get(), (int64_t*)st.seq.get(), V);
   }
