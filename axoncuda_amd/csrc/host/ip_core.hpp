@@ -45,32 +45,57 @@ inline void build_incidence(Hypergraph& h) {
 }
 
 // Coarse level -> IP input: hedges with >= 2 distinct pins, identical pin sets merged
-// (weights summed), in order of first appearance.  Identical sets are found through an
-// open-addressing table keyed by a hash of the sorted pin set.
+// (weights summed), in order of first appearance.  The per-hedge work (sort, dedup, hash)
+// runs on `threads` threads over contiguous hedge ranges; identical sets are then found in
+// hedge order through an open-addressing table keyed by the hash of the sorted pin set.
 inline Hypergraph from_level(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins,
-                             const uint32_t* wfx, const uint32_t* sizes) {
+                             const uint32_t* wfx, const uint32_t* sizes, uint32_t threads = 8) {
   Hypergraph h;
   h.n = n;
   h.vw.assign(sizes, sizes + n);
-  h.off.push_back(0);
+  // 1. per hedge: sorted distinct pins (in place in a copy), length, hash
+  std::vector<uint32_t> sp(pins, pins + hoff[e]);
+  std::vector<uint32_t> len(e, 0);
+  std::vector<uint64_t> hs(e, 0);
+  auto work = [&](uint32_t lo, uint32_t hi) {
+    for (uint32_t i = lo; i < hi; ++i) {
+      uint32_t* b = sp.data() + hoff[i];
+      uint32_t* t = sp.data() + hoff[i + 1];
+      std::sort(b, t);
+      uint32_t l = (uint32_t)(std::unique(b, t) - b);
+      len[i] = l;
+      if (l < 2) continue;
+      uint64_t x = 1469598103934665603ull;
+      for (uint32_t j = 0; j < l; ++j) x = splitmix64(x ^ b[j]);
+      hs[i] = x;
+    }
+  };
+  threads = std::max<uint32_t>(1, std::min<uint32_t>(threads, e / 65536 + 1));
+  if (threads == 1) {
+    work(0, e);
+  } else {
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < threads; ++t)
+      th.emplace_back(work, (uint32_t)((uint64_t)e * t / threads), (uint32_t)((uint64_t)e * (t + 1) / threads));
+    for (auto& x : th) x.join();
+  }
+  // 2. merge identical sets in hedge order
   uint64_t cap = 16;
   while (cap < 2ull * e + 2) cap <<= 1;
   std::vector<uint32_t> table(cap, UINT32_MAX);  // unique hedge ids
   std::vector<uint64_t> hashes;                  // hash of each unique hedge
-  std::vector<uint32_t> tmp;
+  h.off.push_back(0);
   for (uint32_t i = 0; i < e; ++i) {
-    tmp.assign(pins + hoff[i], pins + hoff[i + 1]);
-    std::sort(tmp.begin(), tmp.end());
-    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-    if (tmp.size() < 2) continue;
-    uint64_t hs = 1469598103934665603ull;
-    for (uint32_t p : tmp) hs = splitmix64(hs ^ p);
-    uint64_t slot = hs & (cap - 1);
+    const uint32_t l = len[i];
+    if (l < 2) continue;
+    const uint32_t* b = sp.data() + hoff[i];
+    const uint64_t x = hs[i];
+    uint64_t slot = x & (cap - 1);
     bool merged = false;
     while (table[slot] != UINT32_MAX) {
       const uint32_t cand = table[slot];
-      const uint64_t s = h.off[cand], l = h.off[cand + 1] - s;
-      if (hashes[cand] == hs && l == tmp.size() && std::equal(tmp.begin(), tmp.end(), h.pins.begin() + s)) {
+      const uint64_t s0 = h.off[cand], l0 = h.off[cand + 1] - s0;
+      if (hashes[cand] == x && l0 == l && std::equal(b, b + l, h.pins.begin() + s0)) {
         h.w[cand] += wfx[i];
         merged = true;
         break;
@@ -79,8 +104,8 @@ inline Hypergraph from_level(uint32_t n, uint32_t e, const uint64_t* hoff, const
     }
     if (merged) continue;
     table[slot] = (uint32_t)h.w.size();
-    hashes.push_back(hs);
-    h.pins.insert(h.pins.end(), tmp.begin(), tmp.end());
+    hashes.push_back(x);
+    h.pins.insert(h.pins.end(), b, b + l);
     h.off.push_back(h.pins.size());
     h.w.push_back(wfx[i]);
   }
@@ -403,8 +428,11 @@ inline void bisect_try(const Hypergraph& h, uint32_t seed, uint64_t target0, con
                        const Options& opt, TryResult& out) {
   Bisector b(h);
   b.grow(seed, target0, maxw[0]);
-  for (uint32_t p = 0; p < opt.fm_passes; ++p)
-    if (b.fm_pass(maxw) <= 0 && p > 0) break;
+  // FM passes until one gains nothing or less than 1/1000 of the cut (after the second)
+  for (uint32_t p = 0; p < opt.fm_passes; ++p) {
+    const int64_t g = b.fm_pass(maxw);
+    if (p > 0 && (g <= 0 || g < b.cut() / 1000)) break;
+  }
   out.cut = b.cut();
   out.imb = std::max(b.wside[0] > maxw[0] ? b.wside[0] - maxw[0] : 0, b.wside[1] > maxw[1] ? b.wside[1] - maxw[1] : 0);
   out.side = std::move(b.side);

@@ -1098,7 +1098,8 @@ __global__ void k_inbound_flags(const uint64_t* __restrict__ keys, const uint32_
   const int64_t before_seg = st == 0 ? 0 : cum[st - 1];
   const int64_t cur = (int64_t)isz[q] + (cum[i] - before_seg);
   const int64_t prev = cur - (int64_t)(int32_t)vals[i];
-  const bool was_valid = prev <= (int64_t)delta, is_valid = cur <= (int64_t)delta;
+  // the first event of a partition assumes it started valid (kernels/refinement.cu:643-644)
+  const bool was_valid = i == st || prev <= (int64_t)delta, is_valid = cur <= (int64_t)delta;
   if (was_valid && !is_valid) atomicAdd(&iv[r], 1);
   if (!was_valid && is_valid) atomicSub(&iv[r], 1);
 }

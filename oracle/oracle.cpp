@@ -660,7 +660,10 @@ void analyze(const Level& L, int mode, uint32_t omega, uint32_t delta, uint32_t 
     std::vector<std::unordered_map<uint32_t, int64_t>> dc(L.E);
     for (uint32_t e = 0; e < L.E; ++e)
       for (auto& kv : dcnt[e]) dc[e][kv.first] = kv.second;
+    // A partition's first inbound-size change in the sequence is taken to start from a
+    // valid state (flag_inbound_events_kernel, kernels/refinement.cu:643-644).
     auto invalid = [&](int64_t x) { return x > (int64_t)delta; };
+    std::vector<uint8_t> touched(k, 0);
     for (uint32_t r = 0; r < A.V; ++r) {
       uint32_t v = by_rank[r];
       uint32_t p = part[v], q = A.moves[v];
@@ -671,7 +674,8 @@ void analyze(const Level& L, int mode, uint32_t omega, uint32_t delta, uint32_t 
         int64_t before = a;
         a -= 1;
         if (before > 0 && a == 0) {
-          bool was = invalid(isz[p]);
+          bool was = touched[p] && invalid(isz[p]);
+          touched[p] = 1;
           isz[p] -= 1;
           d += (int32_t)invalid(isz[p]) - (int32_t)was;
         }
@@ -679,7 +683,8 @@ void analyze(const Level& L, int mode, uint32_t omega, uint32_t delta, uint32_t 
         before = b;
         b += 1;
         if (before == 0 && b > 0) {
-          bool was = invalid(isz[q]);
+          bool was = touched[q] && invalid(isz[q]);
+          touched[q] = 1;
           isz[q] += 1;
           d += (int32_t)invalid(isz[q]) - (int32_t)was;
         }

@@ -1,5 +1,6 @@
 import os
 import sys
+import zlib
 
 import numpy as np
 import pytest
@@ -25,4 +26,5 @@ def dev():
 
 @pytest.fixture
 def rng(request):
-    return np.random.default_rng(abs(hash(request.node.name)) % (2**32))
+    """Seeded by the test's name (crc32: stable across processes, unlike hash())."""
+    return np.random.default_rng(zlib.crc32(request.node.name.encode()))
