@@ -27,6 +27,8 @@ constexpr uint32_t CAND_LOG = 10;
 constexpr uint32_t CAND_LIGHT = 512;   // pin-visit bound served from LDS
 constexpr uint32_t CAND_SLOTS = CAND_CAP / 64;
 constexpr uint32_t CAND_MED_CAP = 4096;   // LDS table slots of a medium node (48 KB)
+constexpr uint32_t CAND_BIG_SLOTS = 16384;  // LDS hash slots of a big node (128 KB; half with inbound counts)
+constexpr uint32_t CAND_DIRECT_N = 32768;   // levels up to this size index LDS by node id (132 KB)
 constexpr uint64_t HEAVY_TABLE_SLOTS = 64ull << 20;  // HBM table slots shared by the heavy grid
 
 // Pins visited when scanning a node's touching hedges (itself excluded once per hedge).
@@ -137,9 +139,12 @@ __device__ __forceinline__ void top_insert(uint64_t (&top)[16], uint64_t k) {
 }
 
 // Block-wide top-cnc of the per-thread lists (keys are distinct: they embed the node id).
-// Each wave extracts its cnc best by repeated wave maxima, then wave 0 merges the 4x16.
+// Each of the NW waves extracts its cnc best by repeated wave maxima; wave 0 then merges the
+// NW x 16 survivors (NW / 4 per lane).
+template <int NW>
 __device__ __forceinline__ void block_top_write(uint64_t (&top)[16], uint32_t cnc, uint64_t* s_w, uint32_t u,
                                                 uint32_t* __restrict__ pairs, uint32_t* __restrict__ scores) {
+  static_assert(NW % 4 == 0, "NW must be a multiple of 4");
   const uint32_t wv = threadIdx.x / 64, l = lane_id();
   for (uint32_t i = 0; i < 16; ++i) {
     uint64_t m = 0;
@@ -155,10 +160,18 @@ __device__ __forceinline__ void block_top_write(uint64_t (&top)[16], uint32_t cn
   }
   __syncthreads();
   if (wv == 0) {
-    uint64_t x = s_w[l];
+    constexpr int PER = NW / 4;
+    uint64_t x[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) x[q] = s_w[l + 64 * q];
     for (uint32_t i = 0; i < cnc; ++i) {
-      const uint64_t m = wave_max(x);
-      if (m != 0 && x == m) x = 0;
+      uint64_t lm = 0;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) lm = x[q] > lm ? x[q] : lm;
+      const uint64_t m = wave_max(lm);
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+        if (m != 0 && x[q] == m) x[q] = 0;
       if (l == 0) {
         pairs[(uint64_t)u * cnc + i] = m ? (uint32_t)m : NONE;
         scores[(uint64_t)u * cnc + i] = (uint32_t)(m >> 32);
@@ -178,29 +191,37 @@ __device__ __forceinline__ uint32_t hash_cap(uint64_t bound, uint32_t n) {
   return (uint32_t)c;
 }
 
-// tier of each node: 0 = light (one wave, LDS), 1 = medium (one block, LDS table),
-// 2 = heavy (one block, table in HBM); cap[v] = table slots (heavy: n means direct index)
-__global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uint32_t* __restrict__ f_light,
-                             uint32_t* __restrict__ f_med, uint32_t* __restrict__ cap) {
+// Node classes: 0 light (one wave, LDS 1024 slots), 1 medium (256 threads, LDS table of
+// <= 4096 slots), 2 big (1024 threads, one LDS table: indexed by node id when the level has
+// at most direct_n nodes, else hashed with <= big_cap slots), 3 heavy (HBM tables).
+// cap[v] = table slots; a heavy node whose table would not be smaller than n indexes directly.
+__global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uint32_t direct_n, uint32_t big_cap,
+                             uint32_t* __restrict__ f_light, uint32_t* __restrict__ f_med, uint32_t* __restrict__ f_big,
+                             uint32_t* __restrict__ cap) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const uint64_t b = bound[v];
   const uint32_t c = hash_cap(b, n);
   const bool light = b <= CAND_LIGHT;
   const bool med = !light && c <= CAND_MED_CAP;
+  const bool big = !light && !med && (n <= direct_n || c <= big_cap);
   f_light[v] = light ? 1u : 0u;
   f_med[v] = med ? 1u : 0u;
-  cap[v] = (!light && !med && (uint64_t)n <= c) ? n : c;
+  f_big[v] = big ? 1u : 0u;
+  cap[v] = (big && n <= direct_n) || (!light && !med && !big && (uint64_t)n <= c) ? n : c;
 }
 
-__global__ void k_split3(const uint32_t* __restrict__ f_light, const uint32_t* __restrict__ f_med,
-                         const uint64_t* __restrict__ lpos, const uint64_t* __restrict__ mpos, uint32_t n,
-                         uint32_t* __restrict__ light, uint32_t* __restrict__ med, uint32_t* __restrict__ heavy) {
+__global__ void k_split4(const uint32_t* __restrict__ fa, const uint32_t* __restrict__ fb,
+                         const uint32_t* __restrict__ fc, const uint64_t* __restrict__ pa,
+                         const uint64_t* __restrict__ pb, const uint64_t* __restrict__ pc, uint32_t n,
+                         uint32_t* __restrict__ la, uint32_t* __restrict__ lb, uint32_t* __restrict__ lc,
+                         uint32_t* __restrict__ ld) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  if (f_light[v]) light[lpos[v]] = v;
-  else if (f_med[v]) med[mpos[v]] = v;
-  else heavy[v - lpos[v] - mpos[v]] = v;
+  if (fa[v]) la[pa[v]] = v;
+  else if (fb[v]) lb[pb[v]] = v;
+  else if (fc[v]) lc[pc[v]] = v;
+  else ld[v - pa[v] - pb[v] - pc[v]] = v;
 }
 
 __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ cap, uint32_t nh,
@@ -212,10 +233,12 @@ __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32
   if (lane_id() == 0) atomicMax(out, m);
 }
 
-// Scores one node with a whole workgroup: the four waves split its touching hedges, insert
-// into the table (LDS for medium nodes, the block's HBM region for heavy ones; a direct
-// index when the table would be as large as the node count), then rank the entries.
-template <bool LDS>
+// Scores one node with a whole workgroup of NW waves: the waves split its touching hedges
+// and insert into the table, then every entry is ranked.  The table is LDS or the block's
+// HBM region (GMEM).  Hashed: keys[] holds node ids.  Direct (cap == level size): slot = node
+// id and keys[] is a presence bitmap.  INTER keeps the shared-inbound counts that the
+// inbound-union filter needs; without it (Delta >= twice the hedge count) they are 0.
+template <bool GMEM, bool INTER, int NW>
 __device__ __forceinline__ void cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
                                                 uint32_t* inter, const uint32_t* __restrict__ hedges,
                                                 const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ srcc,
@@ -224,72 +247,109 @@ __device__ __forceinline__ void cand_block_node(uint32_t u, uint32_t cap, bool d
                                                 const uint32_t* __restrict__ sizes, uint32_t cnc, uint32_t omega,
                                                 uint32_t delta, uint32_t* __restrict__ pairs,
                                                 uint32_t* __restrict__ scores, uint64_t* s_w) {
+  const uint32_t nkeys = direct ? (cap + 31) / 32 : cap;
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
-    keys[s] = NONE;
     score[s] = 0;
-    inter[s] = 0;
+    if (INTER) inter[s] = 0;
   }
-  if (!LDS) __threadfence();
+  for (uint32_t s = threadIdx.x; s < nkeys; s += blockDim.x) keys[s] = direct ? 0u : NONE;
+  if (GMEM) __threadfence();
   __syncthreads();
   const uint64_t t0 = toff[u], t1 = toff[u + 1];
   const uint32_t inb_u = inb[u];
   const uint32_t wv = threadIdx.x / 64;
   const uint64_t cnt = t1 - t0;
-  const uint64_t a = t0 + cnt * wv / 4, b = t0 + cnt * (wv + 1) / 4;
+  const uint64_t a = t0 + cnt * wv / NW, b = t0 + cnt * (wv + 1) / NW;
   wave_visit_pins(tch, a, b, hoff, hedges, [&](const PinVisit& pv) {
     if (!pv.act || pv.pin == u) return;
     const uint32_t contrib = fixed_weight(w[pv.e]) / pv.len;
-    const bool shared_in = (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
+    const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
     if (direct) {
-      keys[pv.pin] = pv.pin;
+      const uint32_t bit = 1u << (pv.pin & 31u);
+      if (!(atomic_load_relaxed(&keys[pv.pin >> 5]) & bit)) atomicOr(&keys[pv.pin >> 5], bit);
       atomicAdd(&score[pv.pin], contrib);
-      if (shared_in) atomicAdd(&inter[pv.pin], 1u);
+      if (INTER && shared_in) atomicAdd(&inter[pv.pin], 1u);
     } else {
-      table_insert(keys, score, inter, cap - 1, pv.pin, contrib, shared_in);
+      uint32_t h = slot_hash(pv.pin) & (cap - 1);
+      for (;;) {
+        const uint32_t old = atomicCAS(&keys[h], NONE, pv.pin);
+        if (old == NONE || old == pv.pin) break;
+        h = (h + 1) & (cap - 1);
+      }
+      atomicAdd(&score[h], contrib);
+      if (INTER && shared_in) atomicAdd(&inter[h], 1u);
     }
   });
-  if (!LDS) __threadfence();
+  if (GMEM) __threadfence();
   __syncthreads();
   const uint32_t size_u = sizes[u];
   uint64_t top[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) top[i] = 0;
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
-    const uint32_t v = keys[s];
-    if (v == NONE) continue;
-    top_insert(top, cand_key(u, v, score[s], inter[s], size_u, inb_u, sizes, inb, omega, delta));
+    uint32_t v;
+    if (direct) {
+      if (!((keys[s >> 5] >> (s & 31u)) & 1u)) continue;
+      v = s;
+    } else {
+      v = keys[s];
+      if (v == NONE) continue;
+    }
+    top_insert(top, cand_key(u, v, score[s], INTER ? inter[s] : 0u, size_u, inb_u, sizes, inb, omega, delta));
   }
-  block_top_write(top, cnc, s_w, u, pairs, scores);
+  block_top_write<NW>(top, cnc, s_w, u, pairs, scores);
 }
 
-__global__ void __launch_bounds__(256) k_cand_medium(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
-                                                     const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ tch,
-                                                     const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
-                                                     const float* __restrict__ w, const uint32_t* __restrict__ sizes,
-                                                     const uint32_t* __restrict__ list, const uint32_t* __restrict__ caps,
-                                                     uint32_t cnc, uint32_t omega, uint32_t delta,
-                                                     uint32_t* __restrict__ pairs, uint32_t* __restrict__ scores) {
+#define CAND_ARGS                                                                                                   \
+  const uint32_t *__restrict__ hedges, const uint64_t *__restrict__ hoff, const uint32_t *__restrict__ srcc,        \
+      const uint32_t *__restrict__ tch, const uint64_t *__restrict__ toff, const uint32_t *__restrict__ inb,        \
+      const float *__restrict__ w, const uint32_t *__restrict__ sizes, const uint32_t *__restrict__ list,           \
+      const uint32_t *__restrict__ caps, uint32_t cnc, uint32_t omega, uint32_t delta, uint32_t *__restrict__ pairs, \
+      uint32_t *__restrict__ scores
+
+template <bool INTER>
+__global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS) {
   __shared__ uint32_t s_keys[CAND_MED_CAP];
   __shared__ uint32_t s_score[CAND_MED_CAP];
-  __shared__ uint32_t s_inter[CAND_MED_CAP];
+  __shared__ uint32_t s_inter[INTER ? CAND_MED_CAP : 1];
   __shared__ uint64_t s_w[64];
   const uint32_t u = list[blockIdx.x];
-  cand_block_node<true>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w, sizes, cnc,
-                        omega, delta, pairs, scores, s_w);
+  cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
+                                   sizes, cnc, omega, delta, pairs, scores, s_w);
+}
+
+// One node per 1024-thread workgroup with a hashed LDS table of up to CAND_BIG_SLOTS.
+template <bool INTER>
+__global__ void __launch_bounds__(1024) k_cand_big(CAND_ARGS) {
+  constexpr uint32_t SLOTS = INTER ? CAND_BIG_SLOTS / 2 : CAND_BIG_SLOTS;
+  __shared__ uint32_t s_keys[SLOTS];
+  __shared__ uint32_t s_score[SLOTS];
+  __shared__ uint32_t s_inter[INTER ? SLOTS : 1];
+  __shared__ uint64_t s_w[256];
+  const uint32_t u = list[blockIdx.x];
+  cand_block_node<false, INTER, 16>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb,
+                                    w, sizes, cnc, omega, delta, pairs, scores, s_w);
+}
+
+// One node per 1024-thread workgroup, LDS score array indexed by node id (levels of at most
+// CAND_DIRECT_N nodes, half that with shared-inbound counts).
+template <bool INTER>
+__global__ void __launch_bounds__(1024) k_cand_direct(CAND_ARGS) {
+  constexpr uint32_t N = INTER ? CAND_DIRECT_N / 2 : CAND_DIRECT_N;
+  __shared__ uint32_t s_bits[N / 32];
+  __shared__ uint32_t s_score[N];
+  __shared__ uint32_t s_inter[INTER ? N : 1];
+  __shared__ uint64_t s_w[256];
+  const uint32_t u = list[blockIdx.x];
+  cand_block_node<false, INTER, 16>(u, caps[u], true, s_bits, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
+                                    sizes, cnc, omega, delta, pairs, scores, s_w);
 }
 
 // Heavy nodes: a persistent grid, each workgroup owning `region` slots of HBM tables and
 // taking nodes blockIdx.x, blockIdx.x + gridDim.x, ...
-__global__ void __launch_bounds__(256) k_cand_heavy(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
-                                                    const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ tch,
-                                                    const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
-                                                    const float* __restrict__ w, const uint32_t* __restrict__ sizes,
-                                                    const uint32_t* __restrict__ list, uint32_t nlist,
-                                                    const uint32_t* __restrict__ caps, uint32_t n, uint64_t region,
+__global__ void __launch_bounds__(256) k_cand_heavy(CAND_ARGS, uint32_t nlist, uint32_t n, uint64_t region,
                                                     uint32_t* __restrict__ gkeys, uint32_t* __restrict__ gscore,
-                                                    uint32_t* __restrict__ ginter, uint32_t cnc, uint32_t omega,
-                                                    uint32_t delta, uint32_t* __restrict__ pairs,
-                                                    uint32_t* __restrict__ scores) {
+                                                    uint32_t* __restrict__ ginter) {
   __shared__ uint64_t s_w[64];
   uint32_t* keys = gkeys + region * blockIdx.x;
   uint32_t* score = gscore + region * blockIdx.x;
@@ -297,8 +357,8 @@ __global__ void __launch_bounds__(256) k_cand_heavy(const uint32_t* __restrict__
   for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
     const uint32_t u = list[i];
     const uint32_t cap = caps[u];
-    cand_block_node<false>(u, cap, cap == n, keys, score, inter, hedges, hoff, srcc, tch, toff, inb, w, sizes, cnc, omega,
-                           delta, pairs, scores, s_w);
+    cand_block_node<true, true, 4>(u, cap, cap == n, keys, score, inter, hedges, hoff, srcc, tch, toff, inb, w, sizes,
+                                   cnc, omega, delta, pairs, scores, s_w);
   }
 }
 
@@ -478,27 +538,43 @@ __global__ void k_apply_ids(const uint32_t* __restrict__ raw, const uint64_t* __
 void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* tch,
                 const uint64_t* toff, const uint32_t* inb, const float* w, const uint32_t* sizes, uint32_t n, uint32_t e,
                 uint32_t cnc, uint32_t omega, uint32_t delta, uint32_t* pairs, uint32_t* scores) {
-  (void)e;
   HGP_REQUIRE(cnc >= 1 && cnc <= 16, HGP_E_INVALID, "candidates: candidates_count must be in 1..16");
   if (n == 0) return;
+  // shared-inbound counts matter only when two inbound counts (each <= e) can exceed Delta
+  const bool inter = (uint64_t)delta < 2ull * e;
+  const uint32_t direct_n = inter ? CAND_DIRECT_N / 2 : CAND_DIRECT_N;
+  const uint32_t big_cap = inter ? CAND_BIG_SLOTS / 2 : CAND_BIG_SLOTS;
   const uint32_t g = ceil_div(n, 256);
   DBuf<uint64_t> bound(c, n);
-  DBuf<uint32_t> f_light(c, n), f_med(c, n), caps(c, n);
+  DBuf<uint32_t> fl(c, n), fm(c, n), fb(c, n), caps(c, n);
   HGP_LAUNCH(k_visit_bound, g, 256, 0, c, hoff, tch, toff, n, bound.get());
-  HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, f_light.get(), f_med.get(), caps.get());
-  DBuf<uint64_t> lpos(c, (uint64_t)n + 1), mpos(c, (uint64_t)n + 1);
-  exclusive_scan(c, f_light.get(), lpos.get(), n);
-  exclusive_scan(c, f_med.get(), mpos.get(), n);
+  HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, direct_n, big_cap, fl.get(), fm.get(), fb.get(), caps.get());
+  DBuf<uint64_t> lpos(c, (uint64_t)n + 1), mpos(c, (uint64_t)n + 1), bpos(c, (uint64_t)n + 1);
+  exclusive_scan(c, fl.get(), lpos.get(), n);
+  exclusive_scan(c, fm.get(), mpos.get(), n);
+  exclusive_scan(c, fb.get(), bpos.get(), n);
   const uint32_t nl = (uint32_t)read_scalar(c, lpos.get() + n);
   const uint32_t nm = (uint32_t)read_scalar(c, mpos.get() + n);
-  const uint32_t nh = n - nl - nm;
-  DBuf<uint32_t> light(c, nl), med(c, nm), heavy(c, nh);
-  HGP_LAUNCH(k_split3, g, 256, 0, c, f_light.get(), f_med.get(), lpos.get(), mpos.get(), n, light.get(), med.get(),
-             heavy.get());
+  const uint32_t nb = (uint32_t)read_scalar(c, bpos.get() + n);
+  const uint32_t nh = n - nl - nm - nb;
+  DBuf<uint32_t> light(c, nl), med(c, nm), big(c, nb), heavy(c, nh);
+  HGP_LAUNCH(k_split4, g, 256, 0, c, fl.get(), fm.get(), fb.get(), lpos.get(), mpos.get(), bpos.get(), n, light.get(),
+             med.get(), big.get(), heavy.get());
   HGP_LAUNCH(k_cand_light, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, light.get(), nl, cnc,
              omega, delta, pairs, scores);
-  HGP_LAUNCH(k_cand_medium, nm, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, med.get(), caps.get(), cnc, omega,
-             delta, pairs, scores);
+#define CAND_LAUNCH(K, list, cnt, threads)                                                                          \
+  HGP_LAUNCH(K, cnt, threads, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, list, caps.get(), cnc, omega, delta, \
+             pairs, scores)
+  if (inter) CAND_LAUNCH(k_cand_medium<true>, med.get(), nm, 256);
+  else CAND_LAUNCH(k_cand_medium<false>, med.get(), nm, 256);
+  if (n <= direct_n) {
+    if (inter) CAND_LAUNCH(k_cand_direct<true>, big.get(), nb, 1024);
+    else CAND_LAUNCH(k_cand_direct<false>, big.get(), nb, 1024);
+  } else {
+    if (inter) CAND_LAUNCH(k_cand_big<true>, big.get(), nb, 1024);
+    else CAND_LAUNCH(k_cand_big<false>, big.get(), nb, 1024);
+  }
+#undef CAND_LAUNCH
   if (nh == 0) return;
   DBuf<uint32_t> mc(c, 1);
   memset_async(c, mc.get(), 0, 4);
@@ -507,8 +583,8 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   const uint64_t region = read_scalar(c, mc.get());
   const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)nh, HEAVY_TABLE_SLOTS / region, 4096}));
   DBuf<uint32_t> gk(c, grid * region), gs(c, grid * region), gi(c, grid * region);
-  HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(), nh,
-             caps.get(), n, region, gk.get(), gs.get(), gi.get(), cnc, omega, delta, pairs, scores);
+  HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(),
+             caps.get(), cnc, omega, delta, pairs, scores, nh, n, region, gk.get(), gs.get(), gi.get());
 }
 
 void group_nodes(Ctx& c, const uint32_t* inb, const uint32_t* pairs, const uint32_t* scores, const uint32_t* sizes,

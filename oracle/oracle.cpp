@@ -990,10 +990,13 @@ int orc_candidates(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t*
                    const uint32_t* sizes, uint32_t cnc, uint32_t omega, uint32_t delta, uint32_t* pairs,
                    uint32_t* scores) {
   (void)e;
-  std::unordered_map<uint32_t, std::pair<uint32_t, uint32_t>> acc;  // v -> (score, shared inbound)
+  // dense accumulators over v plus the list of touched v (order-independent integer sums)
+  std::vector<uint32_t> acc_score(n, 0), acc_inter(n, 0);
+  std::vector<uint8_t> seen(n, 0);
+  std::vector<uint32_t> touched;
   std::vector<std::pair<uint32_t, uint32_t>> cand;
   for (uint32_t u = 0; u < n; ++u) {
-    acc.clear();
+    touched.clear();
     for (uint64_t j = toff[u]; j < toff[u + 1]; ++j) {
       uint32_t h = tch[j];
       bool inbound = (j - toff[u]) < inb[u];
@@ -1002,21 +1005,29 @@ int orc_candidates(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t*
       for (uint64_t i = hoff[h]; i < hoff[h + 1]; ++i) {
         uint32_t v = pins[i];
         if (v == u) continue;
-        auto& a = acc[v];
-        a.first += contrib;
-        if (inbound && (i - hoff[h]) >= srcc[h]) a.second += 1;
+        if (!seen[v]) {
+          seen[v] = 1;
+          touched.push_back(v);
+        }
+        acc_score[v] += contrib;
+        if (inbound && (i - hoff[h]) >= srcc[h]) acc_inter[v] += 1;
       }
     }
     cand.clear();
-    for (auto& kv : acc) {
-      uint32_t v = kv.first;
+    for (uint32_t v : touched) {
+      const uint32_t sc = acc_score[v], in = acc_inter[v];
+      seen[v] = 0;
+      acc_score[v] = 0;
+      acc_inter[v] = 0;
       if ((uint32_t)(sizes[u] + sizes[v]) > omega) continue;
-      if ((uint32_t)((inb[v] - kv.second.second) + inb[u]) > delta) continue;
-      uint32_t s = kv.second.first + noise64(u, v);
+      if ((uint32_t)((inb[v] - in) + inb[u]) > delta) continue;
+      uint32_t s = sc + noise64(u, v);
       if (s == 0) continue;
       cand.push_back({s, v});
     }
-    std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) {
+    // top-cnc by (score desc, id desc); ids are distinct, so the order is total
+    const size_t top = std::min<size_t>(cnc, cand.size());
+    std::partial_sort(cand.begin(), cand.begin() + top, cand.end(), [](const auto& a, const auto& b) {
       return a.first != b.first ? a.first > b.first : a.second > b.second;
     });
     for (uint32_t i = 0; i < cnc; ++i) {

@@ -64,6 +64,43 @@ def test_candidates(dev, case, cnc, rng):
         assert np.array_equal(scores, es)
 
 
+def tier_graph(n, big_sizes, n_small, rng):
+    """Mostly small hedges plus a few very large ones, so that their pins need candidate
+    tables of every size class (LDS hash, LDS indexed by node id, HBM hash, HBM indexed)."""
+    off, pins, srcc = [0], [], []
+    for sz in big_sizes:
+        nodes = rng.choice(n, size=sz, replace=False)
+        pins.extend(int(x) for x in nodes)
+        off.append(len(pins))
+        srcc.append(1)
+    for _ in range(n_small):
+        sz = int(rng.integers(2, 6))
+        nodes = rng.choice(n, size=sz, replace=False)
+        pins.extend(int(x) for x in nodes)
+        off.append(len(pins))
+        srcc.append(int(rng.integers(0, 2)))
+    e = len(srcc)
+    w = (rng.random(e) * 3 + 0.5).astype(np.float32)
+    return O.dedup(O.HG(n, np.array(off, np.uint64), np.array(pins, np.uint32), np.array(srcc, np.uint32), w))
+
+
+@pytest.mark.parametrize("n,big,deltas", [
+    (40000, (2500, 5000, 9000), (2**31 - 1, 30)),  # LDS hash (big), HBM hash (heavy)
+    (20000, (9000,), (2**31 - 1, 30)),             # LDS indexed by id; HBM indexed by id with inbound counts
+])
+def test_candidates_table_tiers(dev, n, big, deltas, rng):
+    hg = tier_graph(n, big, 3000, rng)
+    L = G.Level(dev, hg)
+    L.build_touching()
+    ro, rt, ri = O.touching(hg)
+    sizes = rng.integers(1, 3, size=hg.n).astype(np.uint32)
+    for delta in deltas:
+        pairs, scores, *_ = G.candidates(dev, L, sizes, 16, 6, delta)
+        ep, es = O.candidates(hg, ro, rt, ri, sizes, 16, 6, delta)
+        assert np.array_equal(pairs, ep), f"pairs differ (delta={delta})"
+        assert np.array_equal(scores, es), f"scores differ (delta={delta})"
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("cnc", [1, 3, 16])
 def test_group_nodes(dev, case, cnc, rng):
