@@ -20,33 +20,64 @@ struct hgp_ctx {
 
 namespace hgp {
 
+namespace {
+
+uint64_t size_class(uint64_t bytes) {
+  if (bytes <= 4096) return 4096;
+  int b = 63 - __builtin_clzll(bytes - 1);  // 2^b < bytes <= 2^(b+1)
+  const uint64_t step = 1ull << (b - 1);    // four classes per power of two
+  return (bytes + step - 1) / step * step;
+}
+
+}  // namespace
+
 void* dmalloc(Ctx& c, uint64_t bytes) {
   if (bytes == 0) return nullptr;
-  if (c.budget && c.in_use + bytes > c.budget)
+  const uint64_t cls = size_class(bytes);
+  if (c.budget && c.in_use + cls > c.budget)
     throw Error(HGP_E_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget");
   void* p = nullptr;
-  hipError_t e = hipMallocAsync(&p, bytes, c.stream);
-  if (e != hipSuccess || !p) {
-    (void)hipGetLastError();
-    throw Error(HGP_E_NOMEM, "hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  auto it = c.free_lists.find(cls);
+  if (it != c.free_lists.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+    c.cached -= cls;
+  } else {
+    if (c.budget && c.in_use + c.cached + cls > c.budget) release_cached(c);
+    hipError_t e = hipMallocAsync(&p, cls, c.stream);
+    if ((e != hipSuccess || !p) && c.cached) {  // out of memory: give the parked blocks back and retry
+      (void)hipGetLastError();
+      release_cached(c);
+      e = hipMallocAsync(&p, cls, c.stream);
+    }
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      throw Error(HGP_E_NOMEM, "hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+    }
   }
-  c.in_use += bytes;
+  c.in_use += cls;
   c.peak = std::max(c.peak, c.in_use);
-  c.live.push_back({p, bytes});
+  c.live[p] = cls;
   return p;
 }
 
 void dfree(Ctx& c, void* p) {
   if (!p) return;
-  for (size_t i = c.live.size(); i-- > 0;) {
-    if (c.live[i].first == p) {
-      c.in_use -= c.live[i].second;
-      c.live[i] = c.live.back();
-      c.live.pop_back();
-      break;
-    }
-  }
-  (void)hipFreeAsync(p, c.stream);
+  auto it = c.live.find(p);
+  if (it == c.live.end()) return;  // not ours
+  const uint64_t cls = it->second;
+  c.live.erase(it);
+  c.in_use -= cls;
+  c.free_lists[cls].push_back(p);
+  c.cached += cls;
+}
+
+void release_cached(Ctx& c) {
+  for (auto& kv : c.free_lists)
+    for (void* p : kv.second) (void)hipFreeAsync(p, c.stream);
+  c.free_lists.clear();
+  c.cached = 0;
+  (void)hipStreamSynchronize(c.stream);
 }
 
 }  // namespace hgp
@@ -134,6 +165,7 @@ void hgp_ctx_destroy(hgp_ctx* ctx) {
     (void)hipEventDestroy(r.b);
   }
   for (auto& kv : ctx->c.live) (void)hipFreeAsync(kv.first, ctx->c.stream);
+  hgp::release_cached(ctx->c);
   (void)hipStreamSynchronize(ctx->c.stream);
   if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -55,18 +56,32 @@ struct Ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint64_t budget = 0;  // 0 = whole device
-  uint64_t in_use = 0;
+  uint64_t in_use = 0;   // bytes handed out (size classes)
   uint64_t peak = 0;
-  std::vector<std::pair<void*, uint64_t>> live;  // allocation sizes for accounting
+  uint64_t cached = 0;   // bytes parked in the free lists
+  std::unordered_map<void*, uint64_t> live;                    // pointer -> size class
+  std::unordered_map<uint64_t, std::vector<void*>> free_lists;  // size class -> parked blocks
   std::string last_error;
   int verbose = 0;
   uint32_t num_cus = 256;
+  // status arrays of the single-pass scans (prim.hip)
+  uint32_t* scan_flags = nullptr;
+  void* scan_agg = nullptr;
+  void* scan_incl = nullptr;
+  unsigned long long* scan_ticket = nullptr;
+  uint32_t scan_cap_tiles = 0;
+  uint32_t scan_gen = 0;
+  uint64_t scan_tickets_issued = 0;
   std::string prof_kernel;     // base name of the kernel whose launches are timed ("" = none)
   std::vector<ProfRec> prof;   // timed launches not yet read
 };
 
+// Device memory: a caching allocator on top of hipMallocAsync.  Freed blocks are parked
+// per size class (4 classes per power of two) and handed out again by later requests on
+// the same stream, which is safe in stream order and avoids a HIP call per scratch buffer.
 void* dmalloc(Ctx& c, uint64_t bytes);
 void dfree(Ctx& c, void* p);
+void release_cached(Ctx& c);  // returns every parked block to HIP
 
 // RAII device buffer tied to a context.
 template <typename T>

@@ -1,6 +1,6 @@
 // prim.hip - scans, stable radix sort and small helpers for gfx950 (wave64).
 //
-// Scans: three passes (tile reduce, scan of tile sums in one workgroup, tile scan).
+// Scans: one pass with decoupled look-back over 4096-element tiles.
 // Radix sort: reduce-then-scan LSD with 8-bit digits; the scatter ranks keys within a tile
 // in 256-key rounds, using eight ballots per key to find same-digit lanes of a wave (a
 // stable in-wave rank) and LDS prefix sums across the four waves of the workgroup.
@@ -10,134 +10,132 @@ namespace hgp {
 
 namespace {
 
+// Single-pass scan with decoupled look-back.  A workgroup takes the next tile number from a
+// ticket counter (so every tile it waits for belongs to a workgroup that is already running),
+// scans its 4096 elements, publishes its aggregate, then walks back over the predecessors'
+// published aggregates / inclusive prefixes until it meets an inclusive one.  Flags carry a
+// per-scan generation, so the status arrays are reused without clearing.
 constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 
-template <typename T>
-__device__ __forceinline__ T block_excl_scan(T v, T* lds_waves, T* total) {
-  // lds_waves: SCAN_BLOCK/64 entries
-  const uint32_t w = threadIdx.x / 64, l = lane_id();
-  T incl = wave_scan_incl(v);
-  if (l == 63) lds_waves[w] = incl;
+template <typename TI, typename TO, bool EXCL>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_lookback(const TI* in, TO* out, uint64_t n, uint32_t tiles,
+                                                              uint32_t* flags, TO* agg, TO* incl,
+                                                              unsigned long long* ticket, uint64_t base,
+                                                              uint32_t gen) {
+  __shared__ uint32_t s_tile;
+  __shared__ TO s_wave[SCAN_BLOCK / 64];
+  __shared__ TO s_prefix;
+  if (threadIdx.x == 0) s_tile = (uint32_t)(atomicAdd(ticket, 1ull) - base);
   __syncthreads();
-  T base = 0, tot = 0;
-  for (uint32_t i = 0; i < blockDim.x / 64; ++i) {
-    T x = lds_waves[i];
-    if (i < w) base += x;
+  const uint32_t tile = s_tile;
+  const uint32_t w = threadIdx.x / 64, l = lane_id();
+  const uint64_t start = (uint64_t)tile * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  TO v[SCAN_ITEMS];
+  TO sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint64_t i = start + k;
+    v[k] = i < n ? (TO)in[i] : (TO)0;
+    sum += v[k];
+  }
+  const TO winc = wave_scan_incl(sum);
+  if (l == 63) s_wave[w] = winc;
+  __syncthreads();
+  TO wbase = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_BLOCK / 64; ++i) {
+    const TO x = s_wave[i];
+    if (i < (int)w) wbase += x;
     tot += x;
   }
-  *total = tot;
-  __syncthreads();
-  return base + incl - v;
-}
-
-template <typename TI, typename TO>
-__global__ void k_tile_reduce(const TI* __restrict__ in, uint64_t n, TO* __restrict__ partial) {
-  __shared__ TO sw[SCAN_BLOCK / 64];
-  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
-  TO s = 0;
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    uint64_t i = base + (uint64_t)k * SCAN_BLOCK + threadIdx.x;
-    if (i < n) s += (TO)in[i];
-  }
-  s = wave_sum(s);
-  if (lane_id() == 0) sw[threadIdx.x / 64] = s;
-  __syncthreads();
+  const TO texcl = wbase + winc - sum;
   if (threadIdx.x == 0) {
-    TO t = 0;
-    for (int i = 0; i < SCAN_BLOCK / 64; ++i) t += sw[i];
-    partial[blockIdx.x] = t;
+    const uint32_t g = gen << 2;
+    TO prefix = 0;
+    if (tile == 0) {
+      incl[0] = tot;
+      __hip_atomic_store(&flags[0], g | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      agg[tile] = tot;
+      __hip_atomic_store(&flags[tile], g | 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      int64_t j = (int64_t)tile - 1;
+      for (;;) {
+        const uint32_t f = __hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if ((f & ~3u) != g || (f & 3u) == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        if ((f & 3u) == 2u) {
+          prefix += incl[j];
+          break;
+        }
+        prefix += agg[j];
+        --j;
+      }
+      incl[tile] = prefix + tot;
+      __hip_atomic_store(&flags[tile], g | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_prefix = prefix;
+    if (EXCL && tile == tiles - 1) out[n] = prefix + tot;
   }
-}
-
-// exclusive scan of partial[0..m) in place by one workgroup; partial[m] = total
-template <typename TO>
-__global__ void k_scan_partials(TO* partial, uint32_t m) {
-  __shared__ TO sw[1024 / 64];
-  __shared__ TO carry_s;
-  if (threadIdx.x == 0) carry_s = 0;
   __syncthreads();
-  for (uint32_t base = 0; base < m; base += blockDim.x) {
-    uint32_t i = base + threadIdx.x;
-    TO v = i < m ? partial[i] : 0;
-    TO tot;
-    TO ex = block_excl_scan(v, sw, &tot);
-    TO carry = carry_s;
-    if (i < m) partial[i] = carry + ex;
-    __syncthreads();
-    if (threadIdx.x == 0) carry_s = carry + tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[m] = carry_s;
-}
-
-template <typename TI, typename TO>
-__global__ void k_tile_scan_excl(const TI* in, TO* out, uint64_t n, const TO* __restrict__ partial) {
-  __shared__ TO sw[SCAN_BLOCK / 64];
-  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-  TO v[SCAN_ITEMS];
-  TO s = 0;
+  TO run = s_prefix + texcl;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    uint64_t i = base + k;
-    v[k] = i < n ? (TO)in[i] : (TO)0;
-    s += v[k];
-  }
-  TO tot;
-  TO ex = block_excl_scan(s, sw, &tot) + partial[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    uint64_t i = base + k;
-    if (i < n) out[i] = ex;
-    ex += v[k];
+    const uint64_t i = start + k;
+    if (EXCL) {
+      if (i < n) out[i] = run;
+      run += v[k];
+    } else {
+      run += v[k];
+      if (i < n) out[i] = run;
+    }
   }
 }
 
-template <typename TI, typename TO>
-__global__ void k_tile_scan_incl(const TI* in, TO* out, uint64_t n, const TO* __restrict__ partial) {
-  __shared__ TO sw[SCAN_BLOCK / 64];
-  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-  TO v[SCAN_ITEMS];
-  TO s = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    uint64_t i = base + k;
-    v[k] = i < n ? (TO)in[i] : (TO)0;
-    s += v[k];
+// Status arrays of the context's scans (one stream: scans never overlap).
+void scan_status(Ctx& c, uint32_t tiles) {
+  if (tiles <= c.scan_cap_tiles && c.scan_ticket) return;
+  uint32_t cap = 1024;
+  while (cap < tiles) cap <<= 1;
+  if (c.scan_flags) {
+    dfree(c, c.scan_flags);
+    dfree(c, c.scan_agg);
+    dfree(c, c.scan_incl);
   }
-  TO tot;
-  TO ex = block_excl_scan(s, sw, &tot) + partial[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    uint64_t i = base + k;
-    ex += v[k];
-    if (i < n) out[i] = ex;
+  c.scan_flags = (uint32_t*)dmalloc(c, (uint64_t)cap * 4);
+  c.scan_agg = dmalloc(c, (uint64_t)cap * 8);
+  c.scan_incl = dmalloc(c, (uint64_t)cap * 8);
+  memset_async(c, c.scan_flags, 0, (uint64_t)cap * 4);  // generation 0 is never used
+  if (!c.scan_ticket) {
+    c.scan_ticket = (unsigned long long*)dmalloc(c, 8);
+    memset_async(c, c.scan_ticket, 0, 8);
+    c.scan_tickets_issued = 0;
   }
+  c.scan_cap_tiles = cap;
 }
 
-template <typename TO>
-__global__ void k_store_total(TO* out, uint64_t n, const TO* partial, uint32_t m) {
-  out[n] = partial[m];
-}
-
-// Reads all tile sums before the tile scan writes, so in == out is safe.
+// in == out is safe: each workgroup reads only its own tile before writing it.
 template <typename TI, typename TO>
 void scan_impl(Ctx& c, const TI* in, TO* out, uint64_t n, bool exclusive) {
-  uint32_t tiles = ceil_div(n, SCAN_TILE);
-  DBuf<TO> partial(c, (uint64_t)tiles + 1);
   if (n == 0) {
     if (exclusive) memset_async(c, out, 0, sizeof(TO));
     return;
   }
-  HGP_LAUNCH((k_tile_reduce<TI, TO>), tiles, SCAN_BLOCK, 0, c, in, n, partial.get());
-  HGP_LAUNCH((k_scan_partials<TO>), 1, 1024, 0, c, partial.get(), tiles);
-  if (exclusive) {
-    HGP_LAUNCH((k_tile_scan_excl<TI, TO>), tiles, SCAN_BLOCK, 0, c, in, out, n, partial.get());
-    HGP_LAUNCH((k_store_total<TO>), 1, 1, 0, c, out, n, partial.get(), tiles);
-  } else {
-    HGP_LAUNCH((k_tile_scan_incl<TI, TO>), tiles, SCAN_BLOCK, 0, c, in, out, n, partial.get());
-  }
+  const uint32_t tiles = ceil_div(n, SCAN_TILE);
+  scan_status(c, tiles);
+  c.scan_gen = (c.scan_gen + 1) & 0x3FFFFFFFu;
+  if (c.scan_gen == 0) c.scan_gen = 1;
+  const uint64_t base = c.scan_tickets_issued;
+  c.scan_tickets_issued += tiles;
+  if (exclusive)
+    HGP_LAUNCH((k_scan_lookback<TI, TO, true>), tiles, SCAN_BLOCK, 0, c, in, out, n, tiles, c.scan_flags,
+               (TO*)c.scan_agg, (TO*)c.scan_incl, c.scan_ticket, base, c.scan_gen);
+  else
+    HGP_LAUNCH((k_scan_lookback<TI, TO, false>), tiles, SCAN_BLOCK, 0, c, in, out, n, tiles, c.scan_flags,
+               (TO*)c.scan_agg, (TO*)c.scan_incl, c.scan_ticket, base, c.scan_gen);
 }
 
 // ---------------------------------------------------------------------------------------
