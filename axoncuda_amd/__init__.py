@@ -119,6 +119,11 @@ SIGNATURES = [
     ("hgp_partition", C.c_int32, [_vp, C.POINTER(Params), _vp, _u32p, C.POINTER(RunStats)]),
     ("hgp_profile_kernel", C.c_int32, [_vp, C.c_char_p]),
     ("hgp_profile_read", C.c_int32, [_vp, C.POINTER(C.c_double), _u64p, _u64p]),
+    ("hgp_server_create", C.c_int32, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(_vp)]),
+    ("hgp_server_submit", C.c_int32, [_vp, _vp, C.POINTER(Params), _u64p]),
+    ("hgp_server_wait", C.c_int32, [_vp, C.c_uint64, _u32p, C.POINTER(RunStats), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
+    ("hgp_server_destroy", None, [_vp]),
     ("hgp_partition_device", C.c_int32, [_vp, C.POINTER(Params), C.c_uint32, C.c_uint32, _vp, _vp, _vp, _vp, _u32p,
                                          C.POINTER(RunStats)]),
 ]
@@ -304,3 +309,47 @@ class KernelProfile:
                        "hgp_profile_read")
         self.dev.check(self.dev.lib.hgp_profile_kernel(self.dev.ctx, None), "hgp_profile_kernel")
         return {"ms": ms.value, "launches": n.value, "bytes": b.value}
+
+
+class Server:
+    """Continuous-batching partition service (hgp_server_*): submit jobs, wait for results."""
+
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 0):
+        self.lib = load_library()
+        h = _vp()
+        st = self.lib.hgp_server_create(device, hbm_budget, workers, C.byref(h))
+        if st != HGP_OK:
+            raise HgpError(f"hgp_server_create failed with status {st}")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, hg: Hypergraph, prm: Params) -> int:
+        jid = C.c_uint64()
+        st = self.lib.hgp_server_submit(self.h, hg.h, C.byref(prm), C.byref(jid))
+        if st != HGP_OK:
+            raise HgpError(f"hgp_server_submit failed with status {st}")
+        self._keep[jid.value] = hg  # the hypergraph must outlive the job
+        return jid.value
+
+    def wait(self, job_id: int):
+        hg = self._keep[job_id]
+        n, _, _ = hg.sizes()
+        part = np.zeros(n, np.uint32)
+        st = RunStats()
+        lat, q = C.c_double(), C.c_double()
+        s = self.lib.hgp_server_wait(self.h, job_id, _ptr(part, _u32p), C.byref(st), C.byref(lat), C.byref(q))
+        del self._keep[job_id]
+        if s != HGP_OK:
+            raise HgpError(f"job {job_id} failed with status {s}")
+        return part, st, lat.value, q.value
+
+    def close(self):
+        if self.h:
+            self.lib.hgp_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -28,7 +28,8 @@ constexpr uint32_t CAND_LIGHT = 512;   // pin-visit bound served from LDS
 constexpr uint32_t CAND_SLOTS = CAND_CAP / 64;
 constexpr uint32_t CAND_MED_CAP = 4096;   // LDS table slots of a medium node (48 KB)
 constexpr uint32_t CAND_BIG_SLOTS = 16384;  // LDS hash slots of a big node (128 KB; half with inbound counts)
-constexpr uint32_t CAND_DIRECT_N = 32768;   // levels up to this size index LDS by node id (132 KB)
+constexpr uint32_t CAND_DIRECT_N = 39168;   // levels up to this size index LDS by node id (<= 160 KB;
+                                            // half that with inbound counts)
 constexpr uint64_t HEAVY_TABLE_SLOTS = 64ull << 20;  // HBM table slots shared by the heavy grid
 
 // Pins visited when scanning a node's touching hedges (itself excluded once per hedge).
@@ -238,21 +239,31 @@ __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32
 // HBM region (GMEM).  Hashed: keys[] holds node ids.  Direct (cap == level size): slot = node
 // id and keys[] is a presence bitmap.  INTER keeps the shared-inbound counts that the
 // inbound-union filter needs; without it (Delta >= twice the hedge count) they are 0.
-template <bool GMEM, bool INTER, int NW>
-__device__ __forceinline__ void cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
+//
+// OPT (optimistic): the node's distinct neighbours may exceed the table.  New keys are
+// counted; past `limit` (5/8 of the table) the workgroup stops, appends the node to `ovf`
+// and leaves it for the HBM-table kernel.  At most one insert per thread is in flight when
+// the limit trips, so the table never fills.  Returns false when the node overflowed.
+template <bool GMEM, bool INTER, int NW, bool OPT = false>
+__device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
                                                 uint32_t* inter, const uint32_t* __restrict__ hedges,
                                                 const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ srcc,
                                                 const uint32_t* __restrict__ tch, const uint64_t* __restrict__ toff,
                                                 const uint32_t* __restrict__ inb, const float* __restrict__ w,
                                                 const uint32_t* __restrict__ sizes, uint32_t cnc, uint32_t omega,
                                                 uint32_t delta, uint32_t* __restrict__ pairs,
-                                                uint32_t* __restrict__ scores, uint64_t* s_w) {
+                                                uint32_t* __restrict__ scores, uint64_t* s_w,
+                                                uint32_t* s_state = nullptr, uint32_t limit = 0) {
   const uint32_t nkeys = direct ? (cap + 31) / 32 : cap;
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
     score[s] = 0;
     if (INTER) inter[s] = 0;
   }
   for (uint32_t s = threadIdx.x; s < nkeys; s += blockDim.x) keys[s] = direct ? 0u : NONE;
+  if (OPT && threadIdx.x == 0) {
+    s_state[0] = 0;  // new keys
+    s_state[1] = 0;  // overflow flag
+  }
   if (GMEM) __threadfence();
   __syncthreads();
   const uint64_t t0 = toff[u], t1 = toff[u + 1];
@@ -260,7 +271,10 @@ __device__ __forceinline__ void cand_block_node(uint32_t u, uint32_t cap, bool d
   const uint32_t wv = threadIdx.x / 64;
   const uint64_t cnt = t1 - t0;
   const uint64_t a = t0 + cnt * wv / NW, b = t0 + cnt * (wv + 1) / NW;
-  wave_visit_pins(tch, a, b, hoff, hedges, [&](const PinVisit& pv) {
+  auto overflowed = [&] {
+    return OPT && __hip_atomic_load(&s_state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
+  };
+  wave_visit_pins_until(tch, a, b, hoff, hedges, [&](const PinVisit& pv) {
     if (!pv.act || pv.pin == u) return;
     const uint32_t contrib = fixed_weight(w[pv.e]) / pv.len;
     const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
@@ -273,15 +287,20 @@ __device__ __forceinline__ void cand_block_node(uint32_t u, uint32_t cap, bool d
       uint32_t h = slot_hash(pv.pin) & (cap - 1);
       for (;;) {
         const uint32_t old = atomicCAS(&keys[h], NONE, pv.pin);
-        if (old == NONE || old == pv.pin) break;
+        if (old == NONE) {
+          if (OPT && atomicAdd(&s_state[0], 1u) + 1u > limit) atomicOr(&s_state[1], 1u);
+          break;
+        }
+        if (old == pv.pin) break;
         h = (h + 1) & (cap - 1);
       }
       atomicAdd(&score[h], contrib);
       if (INTER && shared_in) atomicAdd(&inter[h], 1u);
     }
-  });
+  }, overflowed);
   if (GMEM) __threadfence();
   __syncthreads();
+  if (OPT && s_state[1]) return false;
   const uint32_t size_u = sizes[u];
   uint64_t top[16];
 #pragma unroll
@@ -298,6 +317,7 @@ __device__ __forceinline__ void cand_block_node(uint32_t u, uint32_t cap, bool d
     top_insert(top, cand_key(u, v, score[s], INTER ? inter[s] : 0u, size_u, inb_u, sizes, inb, omega, delta));
   }
   block_top_write<NW>(top, cnc, s_w, u, pairs, scores);
+  return true;
 }
 
 #define CAND_ARGS                                                                                                   \
@@ -329,6 +349,25 @@ __global__ void __launch_bounds__(1024) k_cand_big(CAND_ARGS) {
   const uint32_t u = list[blockIdx.x];
   cand_block_node<false, INTER, 16>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb,
                                     w, sizes, cnc, omega, delta, pairs, scores, s_w);
+}
+
+// Nodes whose worst-case table exceeds LDS: try the LDS table anyway (their distinct
+// neighbours are usually far fewer than their visit bound); overflowing nodes are listed in
+// `ovf` for the HBM-table kernel.
+template <bool INTER>
+__global__ void __launch_bounds__(1024) k_cand_opt(CAND_ARGS, uint32_t* __restrict__ ovf,
+                                                   uint32_t* __restrict__ ovf_count) {
+  constexpr uint32_t SLOTS = INTER ? CAND_BIG_SLOTS / 2 : CAND_BIG_SLOTS;
+  __shared__ uint32_t s_keys[SLOTS];
+  __shared__ uint32_t s_score[SLOTS];
+  __shared__ uint32_t s_inter[INTER ? SLOTS : 1];
+  __shared__ uint64_t s_w[256];
+  __shared__ uint32_t s_state[2];
+  const uint32_t u = list[blockIdx.x];
+  const bool done = cand_block_node<false, INTER, 16, true>(u, SLOTS, false, s_keys, s_score, s_inter, hedges, hoff,
+                                                            srcc, tch, toff, inb, w, sizes, cnc, omega, delta, pairs,
+                                                            scores, s_w, s_state, SLOTS / 8 * 5);
+  if (!done && threadIdx.x == 0) ovf[atomicAdd(ovf_count, 1u)] = u;
 }
 
 // One node per 1024-thread workgroup, LDS score array indexed by node id (levels of at most
@@ -576,15 +615,26 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   }
 #undef CAND_LAUNCH
   if (nh == 0) return;
+  // heavy nodes: LDS attempt first, the overflow list goes to the HBM tables
+  DBuf<uint32_t> ovf(c, nh), novf(c, 1);
+  memset_async(c, novf.get(), 0, 4);
+  if (inter)
+    HGP_LAUNCH(k_cand_opt<true>, nh, 1024, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(), caps.get(),
+               cnc, omega, delta, pairs, scores, ovf.get(), novf.get());
+  else
+    HGP_LAUNCH(k_cand_opt<false>, nh, 1024, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(), caps.get(),
+               cnc, omega, delta, pairs, scores, ovf.get(), novf.get());
+  const uint32_t no = read_scalar(c, novf.get());
+  if (no == 0) return;
   DBuf<uint32_t> mc(c, 1);
   memset_async(c, mc.get(), 0, 4);
-  HGP_LAUNCH(k_heavy_max_cap, std::min<uint32_t>(ceil_div(nh, 256), 1024), 256, 0, c, heavy.get(), caps.get(), nh,
+  HGP_LAUNCH(k_heavy_max_cap, std::min<uint32_t>(ceil_div(no, 256), 1024), 256, 0, c, ovf.get(), caps.get(), no,
              mc.get());
   const uint64_t region = read_scalar(c, mc.get());
-  const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)nh, HEAVY_TABLE_SLOTS / region, 4096}));
+  const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)no, HEAVY_TABLE_SLOTS / region, 4096}));
   DBuf<uint32_t> gk(c, grid * region), gs(c, grid * region), gi(c, grid * region);
-  HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(),
-             caps.get(), cnc, omega, delta, pairs, scores, nh, n, region, gk.get(), gs.get(), gi.get());
+  HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, ovf.get(),
+             caps.get(), cnc, omega, delta, pairs, scores, no, n, region, gk.get(), gs.get(), gi.get());
 }
 
 void group_nodes(Ctx& c, const uint32_t* inb, const uint32_t* pairs, const uint32_t* scores, const uint32_t* sizes,

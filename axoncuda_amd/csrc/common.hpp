@@ -177,10 +177,11 @@ inline void prof_stop(Ctx& c, hipEvent_t a, uint64_t bytes) {
   do {                                                                                      \
     if ((grid) > 0) {                                                                       \
       const bool _prof = ::hgp::prof_match((ctx), #kernel);                                 \
+      const uint64_t _bytes = _prof ? (uint64_t)(bytes) : 0; /* before the timed launch */  \
       hipEvent_t _pa = _prof ? ::hgp::prof_start(ctx) : nullptr;                            \
       hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, (ctx).stream, __VA_ARGS__); \
       HGP_HIP(hipGetLastError());                                                           \
-      if (_prof) ::hgp::prof_stop((ctx), _pa, (uint64_t)(bytes));                           \
+      if (_prof) ::hgp::prof_stop((ctx), _pa, _bytes);                                      \
     }                                                                                       \
   } while (0)
 #define HGP_LAUNCH(kernel, grid, block, shmem, ctx, ...) HGP_LAUNCH_B(0, kernel, grid, block, shmem, ctx, __VA_ARGS__)

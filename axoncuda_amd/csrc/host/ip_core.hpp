@@ -409,9 +409,16 @@ inline Hypergraph subgraph(const Hypergraph& h, const std::vector<uint32_t>& nod
   return s;
 }
 
+#ifndef HGP_IP_TRIES
+#define HGP_IP_TRIES 4
+#endif
+#ifndef HGP_IP_FM_PASSES
+#define HGP_IP_FM_PASSES 3
+#endif
+
 struct Options {
-  uint32_t tries = 4;
-  uint32_t fm_passes = 3;
+  uint32_t tries = HGP_IP_TRIES;          // growing seeds per bisection
+  uint32_t fm_passes = HGP_IP_FM_PASSES;  // FM passes after each growing
   uint64_t seed = 86;
   uint32_t parallel_depth = 4;         // threads for tries/halves above this recursion depth
   uint64_t parallel_min_pins = 20000;  // smaller subproblems stay on the calling thread

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 
@@ -520,21 +521,25 @@ void gen_netlist(uint32_t n, uint32_t e, double alpha, uint32_t max_size, uint32
 
 void gen_snn(uint32_t layers, uint32_t per_layer, uint32_t grid, uint32_t fanout, double sigma, uint64_t seed,
              hgp_hgraph& hg) {
-  std::mt19937_64 rng(seed);
-  std::normal_distribution<double> N01(0.0, 1.0);
+  // One generator stream per layer (seeded from `seed` and the layer index), so the layers
+  // are drawn on their own threads and the result does not depend on the thread count.
   hg = hgp_hgraph();
   hg.n = layers * per_layer;
-  std::vector<uint32_t> cur;
-  std::unordered_set<uint32_t> seen;
-  for (uint32_t l = 0; l + 1 < layers; ++l) {
+  const uint32_t L = layers - 1;
+  std::vector<std::vector<uint32_t>> lp(L);
+  auto layer = [&](uint32_t l) {
+    std::mt19937_64 rng(seed * 0x9E3779B97F4A7C15ull + l + 1);
+    std::normal_distribution<double> N01(0.0, 1.0);
+    std::vector<uint32_t>& out = lp[l];
+    out.reserve((size_t)per_layer * (fanout + 1));
+    std::vector<uint32_t> stamp(per_layer, UINT32_MAX);
     for (uint32_t i = 0; i < per_layer; ++i) {
       const uint32_t src = l * per_layer + i;
       const double cx = (double)(i % grid), cy = (double)(i / grid);
-      cur.clear();
-      seen.clear();
-      cur.push_back(src);
+      out.push_back(src);
+      uint32_t got = 0;
       uint64_t guard = 0;
-      while (cur.size() < (size_t)fanout + 1) {
+      while (got < fanout) {
         int x = (int)std::lround(cx + sigma * N01(rng));
         int y = (int)std::lround(cy + sigma * N01(rng));
         if (++guard > 1000ull * fanout) {  // pathological: fall back to uniform cells
@@ -544,12 +549,26 @@ void gen_snn(uint32_t layers, uint32_t per_layer, uint32_t grid, uint32_t fanout
         x = std::min(std::max(x, 0), (int)grid - 1);
         y = std::min(std::max(y, 0), (int)grid - 1);
         const uint64_t cell = (uint64_t)y * grid + (uint64_t)x;
-        if (cell >= per_layer) continue;
-        const uint32_t dst = (l + 1) * per_layer + (uint32_t)cell;
-        if (seen.insert(dst).second) cur.push_back(dst);
+        if (cell >= per_layer || stamp[cell] == i) continue;  // off the layer, or a repeat
+        stamp[cell] = i;
+        out.push_back((l + 1) * per_layer + (uint32_t)cell);
+        ++got;
       }
-      hg.pins.insert(hg.pins.end(), cur.begin(), cur.end());
-      hg.off.push_back(hg.pins.size());
+    }
+  };
+  std::vector<std::thread> th;
+  for (uint32_t l = 0; l < L; ++l) th.emplace_back(layer, l);
+  for (auto& t : th) t.join();
+  uint64_t total = 0;
+  for (auto& v : lp) total += v.size();
+  hg.pins.reserve(total);
+  hg.off.reserve((size_t)L * per_layer + 1);
+  for (uint32_t l = 0; l < L; ++l) {
+    const uint64_t base = hg.pins.size();
+    hg.pins.insert(hg.pins.end(), lp[l].begin(), lp[l].end());
+    std::vector<uint32_t>().swap(lp[l]);
+    for (uint32_t i = 0; i < per_layer; ++i) {
+      hg.off.push_back(base + (uint64_t)(i + 1) * (fanout + 1));
       hg.srcc.push_back(1);
       hg.w.push_back(1.0f);
     }

@@ -51,34 +51,53 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_lookback(const TI* in, TO* 
     tot += x;
   }
   const TO texcl = wbase + winc - sum;
-  if (threadIdx.x == 0) {
+  // look-back by wave 0: each round examines the 64 preceding tiles at once
+  if (w == 0) {
     const uint32_t g = gen << 2;
     TO prefix = 0;
     if (tile == 0) {
-      incl[0] = tot;
-      __hip_atomic_store(&flags[0], g | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (l == 0) {
+        incl[0] = tot;
+        __hip_atomic_store(&flags[0], g | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
     } else {
-      agg[tile] = tot;
-      __hip_atomic_store(&flags[tile], g | 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      int64_t j = (int64_t)tile - 1;
+      if (l == 0) {
+        agg[tile] = tot;
+        __hip_atomic_store(&flags[tile], g | 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      int64_t hi = (int64_t)tile - 1;  // window [hi - 63, hi]; lane l looks at tile hi - l
       for (;;) {
-        const uint32_t f = __hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if ((f & ~3u) != g || (f & 3u) == 0) {
+        const int64_t j = hi - (int64_t)l;
+        uint32_t st = 2u;  // tiles before 0 act as an inclusive zero
+        if (j >= 0) {
+          const uint32_t f = __hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          st = (f & ~3u) == g ? (f & 3u) : 0u;
+        }
+        const uint64_t incl_mask = __ballot(st == 2u);
+        const uint64_t zero_mask = __ballot(st == 0u);
+        // nearest inclusive tile (lowest lane with st == 2); every lane before it must be published
+        const uint32_t first_incl = incl_mask ? (uint32_t)__ffsll((unsigned long long)incl_mask) - 1u : 64u;
+        const uint64_t before = first_incl >= 64 ? ~0ull : ((1ull << first_incl) - 1ull);
+        if (zero_mask & before) {
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        if ((f & 3u) == 2u) {
-          prefix += incl[j];
-          break;
-        }
-        prefix += agg[j];
-        --j;
+        TO part = 0;
+        if (l < first_incl && j >= 0) part = agg[j];
+        if (l == first_incl && j >= 0) part = incl[j];
+        prefix += wave_sum(part);
+        if (first_incl < 64) break;
+        hi -= 64;
       }
-      incl[tile] = prefix + tot;
-      __hip_atomic_store(&flags[tile], g | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (l == 0) {
+        incl[tile] = prefix + tot;
+        __hip_atomic_store(&flags[tile], g | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
-    s_prefix = prefix;
-    if (EXCL && tile == tiles - 1) out[n] = prefix + tot;
+    if (l == 0) {
+      s_prefix = prefix;
+      if (EXCL && tile == tiles - 1) out[n] = prefix + tot;
+    }
   }
   __syncthreads();
   TO run = s_prefix + texcl;
@@ -243,6 +262,25 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits) {
   }
 }
 
+__global__ void k_minmax_u64(const uint64_t* __restrict__ p, uint64_t n, unsigned long long* out) {
+  uint64_t mn = ~0ull, mx = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = p[i];
+    mn = v < mn ? v : mn;
+    mx = v > mx ? v : mx;
+  }
+  mn = wave_min(mn);
+  mx = wave_max(mx);
+  if (lane_id() == 0) {
+    atomicMin(&out[0], (unsigned long long)mn);
+    atomicMax(&out[1], (unsigned long long)mx);
+  }
+}
+__global__ void k_sub_u64(uint64_t* p, uint64_t n, const unsigned long long* mm) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] -= mm[0];
+}
+
 template <typename K>
 __global__ void k_segment_offsets(const K* __restrict__ keys, int shift, uint64_t n, uint32_t nseg,
                                   uint64_t* __restrict__ off) {
@@ -285,6 +323,19 @@ void segment_offsets(Ctx& c, const uint32_t* keys, int shift, uint64_t n, uint32
 }
 void segment_offsets(Ctx& c, const uint64_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off) {
   HGP_LAUNCH((k_segment_offsets<uint64_t>), ceil_div(n + 1, 256), 256, 0, c, keys, shift, n, nseg, off);
+}
+
+void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n) {
+  if (n <= 1) return;
+  DBuf<unsigned long long> mm(c, 2);
+  const unsigned long long init[2] = {~0ull, 0ull};
+  copy_h2d(c, mm.get(), init, sizeof init);
+  HGP_LAUNCH(k_minmax_u64, std::min<uint32_t>(ceil_div(n, 256), 1024), 256, 0, c, keys, n, mm.get());
+  unsigned long long h[2];
+  copy_d2h(c, h, mm.get(), sizeof h);
+  if (h[1] == h[0]) return;  // all keys equal: a stable sort is the identity
+  HGP_LAUNCH(k_sub_u64, ceil_div(n, 256), 256, 0, c, keys, n, mm.get());
+  radix_impl<uint64_t>(c, keys, vals, n, bit_width(h[1] - h[0]));
 }
 
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, uint64_t n) { HGP_LAUNCH(k_fill_u32, ceil_div(n, 256), 256, 0, c, p, v, n); }

@@ -17,6 +17,9 @@ void inclusive_scan(Ctx& c, const int32_t* in, int32_t* out, uint64_t n);
 // Stable LSD radix sorts over key bits [0, bits); values ride along (nullable).
 void radix_sort(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int bits);
 void radix_sort(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int bits);
+// Stable sort over the keys' actual range: subtracts the minimum key (the keys are left
+// rebased) and sorts only the bits that the maximum difference needs.
+void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n);
 
 // off[s] = first index i with (keys[i] >> shift) >= s, for s in [0, nseg]; keys sorted.
 void segment_offsets(Ctx& c, const uint32_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off);

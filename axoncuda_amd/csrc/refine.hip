@@ -669,33 +669,37 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n) {
   c.choice[i] = choice;
 }
 // Claims: the proposer with the smallest size difference, then the lowest index, wins.
-// Proposals pile onto few targets (the heads of the partition lists), so lanes of a wave
-// that share a target first agree on their best key and one lane issues the atomic.
-__global__ void k_chain_resolve(ChainArrays c, uint32_t n) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t s = NONE;
-  unsigned long long key = 0ull;
+// Proposals pile onto few targets (the heads of the partition lists) but the lanes of a wave
+// rarely share one, so a 1024-thread workgroup first merges its claims per target in an LDS
+// table and then issues one global atomic per distinct target.
+constexpr uint32_t RESOLVE_BLOCK = 1024;
+constexpr uint32_t RESOLVE_SLOTS = 2048;  // > 2 x the workgroup's claims: the table never fills
+__global__ void __launch_bounds__(RESOLVE_BLOCK) k_chain_resolve(ChainArrays c, uint32_t n) {
+  __shared__ uint32_t s_tgt[RESOLVE_SLOTS];
+  __shared__ unsigned long long s_key[RESOLVE_SLOTS];
+  for (uint32_t t = threadIdx.x; t < RESOLVE_SLOTS; t += RESOLVE_BLOCK) {
+    s_tgt[t] = NONE;
+    s_key[t] = 0ull;
+  }
+  __syncthreads();
+  const uint32_t i = blockIdx.x * RESOLVE_BLOCK + threadIdx.x;
   if (i < n) {
-    s = c.choice[i];
+    const uint32_t s = c.choice[i];
     if (s != NONE && c.prev[s] == NONE) {
       const uint32_t d = c.sz[i] > c.sz[s] ? c.sz[i] - c.sz[s] : c.sz[s] - c.sz[i];
-      key = ((unsigned long long)(0xFFFFFFFFu - d) << 32) | (0xFFFFFFFFu - i);
-    } else {
-      s = NONE;
+      const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - d) << 32) | (0xFFFFFFFFu - i);
+      uint32_t h = slot_hash(s) & (RESOLVE_SLOTS - 1);
+      for (;;) {
+        const uint32_t old = atomicCAS(&s_tgt[h], NONE, s);
+        if (old == NONE || old == s) break;
+        h = (h + 1) & (RESOLVE_SLOTS - 1);
+      }
+      atomicMax(&s_key[h], key);
     }
   }
-  bool act = s != NONE;
-  for (int it = 0; it < 8; ++it) {
-    const uint64_t am = __ballot(act);
-    if (!am) break;
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)am) - 1u;
-    const uint32_t s0 = __shfl(s, leader);
-    const bool same = act && s == s0;
-    const unsigned long long m = wave_max(same ? key : 0ull);
-    if (lane_id() == leader) atomicMax(&c.claim[s0], m);
-    if (same) act = false;
-  }
-  if (act) atomicMax(&c.claim[s], key);
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < RESOLVE_SLOTS; t += RESOLVE_BLOCK)
+    if (s_tgt[t] != NONE) atomicMax(&c.claim[s_tgt[t]], s_key[t]);
 }
 __global__ void k_chain_commit(ChainArrays c, uint32_t n) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -747,11 +751,11 @@ __global__ void k_comp_rank(const uint32_t* __restrict__ clist, uint32_t m, uint
   if (r < m) crank[clist[r]] = r;
 }
 __global__ void k_edge_keys(const uint32_t* __restrict__ comp, const uint32_t* __restrict__ pos,
-                            const uint32_t* __restrict__ crank, uint32_t n, int jb, uint64_t* __restrict__ keys,
+                            const uint32_t* __restrict__ crank, uint32_t n, uint64_t* __restrict__ keys,
                             uint32_t* __restrict__ vals) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  keys[j] = ((uint64_t)crank[comp[j]] << (8 + jb)) | ((uint64_t)pos[j] << jb) | j;
+  keys[j] = ((uint64_t)crank[comp[j]] << 8) | pos[j];
   vals[j] = j;
 }
 __global__ void k_edge_ranks(const uint32_t* __restrict__ sorted_j, const uint32_t* __restrict__ orig, uint32_t n,
@@ -765,12 +769,16 @@ struct CascadeArgs {
   const uint32_t* hedges;
   const uint64_t* hoff;
   const float* w;
-  const uint32_t* part;
-  const uint32_t* moves;
-  const uint32_t* ranks;
+  const uint4* pm;  // per node: (partition, move target or NONE, rank, 0), one 16-byte gather per pin
   int incc;
   unsigned long long* seq;
 };
+
+__global__ void k_pack_nodes(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
+                             const uint32_t* __restrict__ ranks, uint32_t n, uint4* __restrict__ pm) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) pm[v] = make_uint4(part[v], moves[v], ranks[v], 0u);
+}
 
 __device__ __forceinline__ int64_t cascade_term(int incc, int64_t W, int64_t cp, int64_t cq) {
   int64_t t = 0;
@@ -801,9 +809,10 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
   bool any = false;
   for (uint32_t i = 0; i < len; ++i) {
     const uint32_t x = a.hedges[s + i];
-    pt[i] = a.part[x];
-    mv[i] = a.moves[x];
-    rk[i] = a.ranks[x];
+    const uint4 q4 = a.pm[x];
+    pt[i] = q4.x;
+    mv[i] = q4.y;
+    rk[i] = q4.z;
     any |= mv[i] != NONE;
   }
   if (!any) return;
@@ -837,20 +846,21 @@ __global__ void __launch_bounds__(64) k_cascade_large(CascadeArgs a, const uint3
   if (in_lds) {
     for (uint32_t i = l; i < len; i += 64) {
       const uint32_t x = a.hedges[s + i];
-      s_pt[i] = a.part[x];
-      s_mv[i] = a.moves[x];
-      s_rk[i] = a.ranks[x];
+      const uint4 q4 = a.pm[x];
+      s_pt[i] = q4.x;
+      s_mv[i] = q4.y;
+      s_rk[i] = q4.z;
       any |= s_mv[i] != NONE;
     }
   } else {
-    for (uint32_t i = l; i < len; i += 64) any |= a.moves[a.hedges[s + i]] != NONE;
+    for (uint32_t i = l; i < len; i += 64) any |= a.pm[a.hedges[s + i]].y != NONE;
   }
   __syncthreads();
   if (!__ballot(any)) return;
   const int64_t W = (int64_t)fixed_weight(a.w[e]);
-  auto PT = [&](uint32_t j) { return in_lds ? s_pt[j] : a.part[a.hedges[s + j]]; };
-  auto MV = [&](uint32_t j) { return in_lds ? s_mv[j] : a.moves[a.hedges[s + j]]; };
-  auto RK = [&](uint32_t j) { return in_lds ? s_rk[j] : a.ranks[a.hedges[s + j]]; };
+  auto PT = [&](uint32_t j) { return in_lds ? s_pt[j] : a.pm[a.hedges[s + j]].x; };
+  auto MV = [&](uint32_t j) { return in_lds ? s_mv[j] : a.pm[a.hedges[s + j]].y; };
+  auto RK = [&](uint32_t j) { return in_lds ? s_rk[j] : a.pm[a.hedges[s + j]].z; };
   for (uint32_t i = l; i < len; i += 64) {
     const uint32_t q = MV(i);
     if (q == NONE) continue;
@@ -907,9 +917,10 @@ __global__ void __launch_bounds__(64) k_cascade_sorted(CascadeArgs a, const uint
   uint32_t moving = 0;
   for (uint32_t i = l; i < len; i += 64) {
     const uint32_t x = a.hedges[s + i];
-    const uint32_t p = a.part[x], q = a.moves[x];
+    const uint4 q4 = a.pm[x];
+    const uint32_t p = q4.x, q = q4.y;
     s_ev[i] = ((uint64_t)p << (rb + 11)) | i;
-    s_rk[i] = q == NONE ? NONE : a.ranks[x];
+    s_rk[i] = q == NONE ? NONE : q4.z;
     moving += q != NONE;
   }
   moving = wave_sum(moving);
@@ -927,8 +938,9 @@ __global__ void __launch_bounds__(64) k_cascade_sorted(CascadeArgs a, const uint
       const uint32_t o = base + 2 * lanes_below(mk);
       const uint32_t x = a.hedges[s + i];
       const uint64_t r1 = (uint64_t)s_rk[i] + 1;
-      s_ev[o] = ((uint64_t)a.part[x] << (rb + 11)) | (r1 << 11) | i;
-      s_ev[o + 1] = ((uint64_t)a.moves[x] << (rb + 11)) | (r1 << 11) | (1u << 10) | i;
+      const uint4 q4 = a.pm[x];
+      s_ev[o] = ((uint64_t)q4.x << (rb + 11)) | (r1 << 11) | i;
+      s_ev[o + 1] = ((uint64_t)q4.y << (rb + 11)) | (r1 << 11) | (1u << 10) | i;
     }
     base += 2 * (uint32_t)__popcll(mk);
   }
@@ -1263,6 +1275,29 @@ uint32_t mix32_host(uint32_t x) {
   return x;
 }
 
+// Algorithmic bytes of the gain kernels over `list` (evaluated only while the kernel is
+// profiled): per node 36 (partition, size, two offsets, move and gain out) + 24 per touching
+// hedge (hedge id, weight, offset, list length, own-partition count) + 4 per listed
+// partition entry.
+__global__ void k_gain_bytes(const uint32_t* __restrict__ list, uint32_t m, const uint64_t* __restrict__ toff,
+                             const uint64_t* __restrict__ bound, unsigned long long* out) {
+  uint64_t b = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const uint32_t v = list[i];
+    b += 36 + 24 * (toff[v + 1] - toff[v]) + 4 * bound[v];
+  }
+  b = wave_sum(b);
+  if (lane_id() == 0 && b) atomicAdd(out, (unsigned long long)b);
+}
+
+uint64_t gain_bytes(Ctx& c, const uint32_t* list, uint32_t m, const uint64_t* toff, const uint64_t* bound) {
+  if (m == 0) return 0;
+  DBuf<unsigned long long> out(c, 1);
+  memset_async(c, out.get(), 0, 8);
+  HGP_LAUNCH(k_gain_bytes, std::min<uint32_t>(ceil_div(m, 256), 1024), 256, 0, c, list, m, toff, bound, out.get());
+  return read_scalar(c, out.get());
+}
+
 void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int mode, uint32_t omega, uint32_t repeat,
                    uint32_t discount, const uint32_t* part, const uint32_t* psizes, uint32_t* moves, int64_t* gains) {
   // hash order of partitions for the non-adjacent fallback
@@ -1304,9 +1339,11 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   DBuf<uint32_t> tl(c, nt), ll(c, nl), hl(c, nh);
   HGP_LAUNCH(k_split3, ceil_div(L.n, 256), 256, 0, c, tiny.get(), light.get(), tpos.get(), pos.get(), L.n, tl.get(),
              ll.get(), hl.get());
-  HGP_LAUNCH(k_gains_tiny, ceil_div(nt, 256), 256, 0, c, a, tl.get(), nt);
+  HGP_LAUNCH_B(gain_bytes(c, tl.get(), nt, L.toff, bound.get()), k_gains_tiny, ceil_div(nt, 256), 256, 0, c, a,
+               tl.get(), nt);
   const uint32_t capmax = L.k <= GAIN_CAP ? std::max<uint32_t>(L.k, 1) : GAIN_CAP;
-  HGP_LAUNCH(k_gains_light, ceil_div(nl, 4), 256, 4 * capmax * 12, c, a, ll.get(), nl, bound.get(), capmax);
+  HGP_LAUNCH_B(gain_bytes(c, ll.get(), nl, L.toff, bound.get()), k_gains_light, ceil_div(nl, 4), 256, 4 * capmax * 12,
+               c, a, ll.get(), nl, bound.get(), capmax);
   sync(c);  // `order` host buffer
   if (nh == 0) return;
   DBuf<uint64_t> caps(c, nh);
@@ -1334,7 +1371,7 @@ void rank_by_gain(Ctx& c, uint32_t n, const uint32_t* moves, const int64_t* gain
   DBuf<uint64_t> keys(c, n);
   DBuf<uint32_t> vals(c, n), k2(c, n);
   HGP_LAUNCH(k_rank_keys_gain, ceil_div(n, 256), 256, 0, c, moves, gains, n, keys.get(), vals.get());
-  radix_sort(c, keys.get(), vals.get(), n, 64);
+  radix_sort_range(c, keys.get(), vals.get(), n);
   HGP_LAUNCH(k_invalid_keys, ceil_div(n, 256), 256, 0, c, moves, vals.get(), n, k2.get());
   radix_sort(c, k2.get(), vals.get(), n, 1);
   HGP_LAUNCH(k_scatter_ranks, ceil_div(n, 256), 256, 0, c, vals.get(), n, ranks);
@@ -1347,7 +1384,7 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   DBuf<uint64_t> k1(c, n);
   DBuf<uint32_t> orig(c, n), k2(c, n);
   HGP_LAUNCH(k_chain_sortkeys, g, 256, 0, c, part, moves, gains, n, k1.get(), orig.get());
-  radix_sort(c, k1.get(), orig.get(), n, 64);
+  radix_sort_range(c, k1.get(), orig.get(), n);
   HGP_LAUNCH(k_chain_sortkeys2, g, 256, 0, c, part, moves, orig.get(), n, k2.get());
   radix_sort(c, k2.get(), orig.get(), n, bit_width(2ull * k + 1));
   k1.reset();
@@ -1363,7 +1400,7 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   HGP_LAUNCH(k_chain_ranges, g, 256, 0, c, ca, poff.get(), n);
   for (int it = 0; it < 4; ++it) {
     HGP_LAUNCH(k_chain_propose, g, 256, 0, c, ca, n);
-    HGP_LAUNCH(k_chain_resolve, g, 256, 0, c, ca, n);
+    HGP_LAUNCH(k_chain_resolve, ceil_div(n, RESOLVE_BLOCK), RESOLVE_BLOCK, 0, c, ca, n);
     HGP_LAUNCH(k_chain_commit, g, 256, 0, c, ca, n);
   }
   DBuf<uint32_t> comp(c, n), pos(c, n), cflag(c, n), cinv(c, n);
@@ -1382,18 +1419,17 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   // components by (valid first, summed gain desc, head id asc): the list starts in ascending
   // head id; a stable sort on the summed gain, then a stable 1-bit pass that moves the
   // components made of an invalid move (summed gain 0) behind every valid one
-  radix_sort(c, ckey.get(), clist.get(), m, 64);
+  radix_sort_range(c, ckey.get(), clist.get(), m);
   DBuf<uint32_t> inv2(c, m);
   HGP_LAUNCH(k_gather_u32, ceil_div(m, 256), 256, 0, c, cinv.get(), clist.get(), m, inv2.get());
   radix_sort(c, inv2.get(), clist.get(), m, 1);
   HGP_LAUNCH(k_comp_rank, ceil_div(m, 256), 256, 0, c, clist.get(), m, crank.get());
-  // final order: key (component rank, position, sorted index)
-  const int jb = std::max(1, bit_width(n));
-  HGP_REQUIRE(bit_width(m) + 8 + jb <= 64, HGP_E_CAPACITY, "chaining: order key exceeds 64 bits");
+  // final order: (component rank, position), ties in sorted-space index order: the keys are
+  // laid out in index order and the sort is stable
   DBuf<uint64_t> ek(c, n);
   DBuf<uint32_t> ev(c, n);
-  HGP_LAUNCH(k_edge_keys, g, 256, 0, c, comp.get(), pos.get(), crank.get(), n, jb, ek.get(), ev.get());
-  radix_sort(c, ek.get(), ev.get(), n, bit_width(m) + 8 + jb);
+  HGP_LAUNCH(k_edge_keys, g, 256, 0, c, comp.get(), pos.get(), crank.get(), n, ek.get(), ev.get());
+  radix_sort(c, ek.get(), ev.get(), n, bit_width(m) + 8);
   HGP_LAUNCH(k_edge_ranks, g, 256, 0, c, ev.get(), orig.get(), n, ranks);
 }
 
@@ -1436,7 +1472,9 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
   memset_async(c, st.iv.get(), 0, (uint64_t)V * 4);
   // cascade
   {
-    CascadeArgs a{L.hedges, L.hoff, L.w, part, st.moves.get(), st.ranks.get(), incc ? 1 : 0, st.seq.get()};
+    DBuf<uint4> pm(c, n);
+    HGP_LAUNCH(k_pack_nodes, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), n, pm.get());
+    CascadeArgs a{L.hedges, L.hoff, L.w, pm.get(), incc ? 1 : 0, st.seq.get()};
     const int crb = std::max(1, bit_width(V));  // rank + 1 <= V
     const int cpb = std::max(1, bit_width(L.k));
     const uint32_t sort_max = cpb + crb + 11 < 64 ? CAS_SORT : 0u;

@@ -76,12 +76,14 @@ __device__ __forceinline__ void wave_visit_entries(const uint32_t* __restrict__ 
   }
 }
 
-template <typename F>
-__device__ __forceinline__ void wave_visit_pins(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
-                                                const uint64_t* __restrict__ hoff,
-                                                const uint32_t* __restrict__ hedges, F&& visit) {
+// `stop()` (wave-uniform) is checked before every step; the traversal ends when it is true.
+template <typename F, typename S>
+__device__ __forceinline__ void wave_visit_pins_until(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
+                                                      const uint64_t* __restrict__ hoff,
+                                                      const uint32_t* __restrict__ hedges, F&& visit, S&& stop) {
   const uint32_t l = lane_id();
   for (uint64_t jb = t0; jb < t1; jb += 64) {
+    if (stop()) return;
     const uint64_t j = jb + l;
     uint32_t e = 0, len = 0;
     uint64_t s = 0;
@@ -94,6 +96,7 @@ __device__ __forceinline__ void wave_visit_pins(const uint32_t* __restrict__ tch
     const uint32_t total = __shfl(incl, 63, 64);
     const uint32_t excl = incl - len;
     for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+      if (f0 && stop()) return;
       const uint32_t fi = f0 + l;
       uint32_t k = 0;
 #pragma unroll
@@ -114,6 +117,13 @@ __device__ __forceinline__ void wave_visit_pins(const uint32_t* __restrict__ tch
       visit(pv);
     }
   }
+}
+
+template <typename F>
+__device__ __forceinline__ void wave_visit_pins(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
+                                                const uint64_t* __restrict__ hoff,
+                                                const uint32_t* __restrict__ hedges, F&& visit) {
+  wave_visit_pins_until(tch, t0, t1, hoff, hedges, visit, [] { return false; });
 }
 
 }  // namespace hgp

@@ -35,12 +35,13 @@ import axoncuda_amd as H  # noqa: E402
 N_NODES, N_NETS, ALPHA, MAX_SIZE, RADIUS, FRAC_LOCAL = 3_200_000, 3_400_000, 2.5, 1000, 8, 0.8
 K, EPS = 64, 0.03
 HBM_PEAK_GBS = 8000.0
-# Kernel whose launches the roofline object covers (the largest share of device time on
-# this workload in the kernel-trace summary under profiles/).
-ROOFLINE_KERNEL = os.environ.get("HGP_BENCH_KERNEL", "k_cand_light")
-# Plain copy calibration of FETCH_SIZE / WRITE_SIZE (bytes counted per algorithmic byte),
-# measured with rocprofv3 --pmc on a copy kernel of known size; None until measured.
-TRAFFIC_PER_LAUNCH = None
+# Kernel whose launches the roofline object covers: the largest share of device time on this
+# workload in the kernel-trace summary under profiles/.
+ROOFLINE_KERNEL = "k_gains_light"
+# HBM bytes per launch of that kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# scaled by the same counters on a plain device copy of known size (profiles/traffic_<kernel>.json).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{ROOFLINE_KERNEL}.json")
+COPY_CAL_BYTES = 1 << 30  # the plain copy that calibrates the counters
 
 
 def kway_omega(n: int, k: int, eps: float) -> int:
@@ -76,6 +77,88 @@ def cpu_baseline(budget_s: float = 30.0) -> dict:
                       f"one run of the oracle pipeline ({dt:.1f} s)"}
 
 
+def serve_jobs(n_jobs: int, max_pins: float, seed: int):
+    """The serving stream's jobs: pins log-uniform in [1e4, max_pins]; half random
+    hypergraphs of the 20K-node kind (sizes 2 + Geom(1/2), unit weights), half layered SNNs
+    of the inbound-constrained kind; modes uniform over k = 2, 8, 32 at eps 0.03 and
+    -m 1024 4096 1048576."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_jobs):
+        pins = float(np.exp(rng.uniform(np.log(1e4), np.log(max_pins))))
+        if i % 2 == 0:
+            e = max(64, int(pins / 4))
+            hg = H.Hypergraph.generate("random", n=e, e=e, max_size=64, seed=seed * 1000 + i)
+        else:
+            fan = 64
+            per_layer = max(fan + 2, int(pins / (7 * (fan + 1))))
+            grid = int(np.ceil(np.sqrt(per_layer)))
+            hg = H.Hypergraph.generate("snn", layers=8, per_layer=per_layer, grid=grid, fanout=fan, sigma=4.0,
+                                       seed=seed * 1000 + i)
+        n, _, p = hg.sizes()
+        mode = int(rng.integers(0, 4))
+        prm = H.Params()
+        H.load_library().hgp_default_params(C.byref(prm))
+        prm.save_memory_up_to_level = 0
+        if mode < 3:
+            k = (2, 8, 32)[mode]
+            prm.mode, prm.kway, prm.epsilon = H.HGP_MODE_KWAY, k, EPS
+            prm.max_nodes_per_part, prm.max_inbound_per_part, prm.max_parts = kway_omega(n, k, EPS), 2**31 - 1, k
+        else:
+            prm.mode = H.HGP_MODE_INCC
+            prm.max_nodes_per_part, prm.max_inbound_per_part, prm.max_parts = 1024, 4096, 1 << 20
+        jobs.append((hg, prm, p))
+    return jobs
+
+
+def serve_bench(args) -> dict:
+    """Plain sequential runs of the job stream, then the same jobs through the continuous-
+    batching service with Poisson arrivals at twice the plain service rate."""
+    jobs = serve_jobs(args.jobs, args.max_pins, args.seed)
+    total_pins = sum(p for _, _, p in jobs)
+    print(f"# serving stream: {len(jobs)} jobs, {total_pins} pins", file=sys.stderr, flush=True)
+    lib = H.load_library()
+    dev = H.Device(0)
+    plain_parts = []
+    t0 = time.perf_counter()
+    for hg, prm, _ in jobs:
+        n, _, _ = hg.sizes()
+        part = np.zeros(n, np.uint32)
+        st = H.RunStats()
+        dev.check(lib.hgp_partition(dev.ctx, C.byref(prm), hg.h, part.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                    C.byref(st)), "hgp_partition")
+        plain_parts.append(part)
+    t_plain = time.perf_counter() - t0
+    dev.close()
+    print(f"# plain sequential: {t_plain:.2f} s", file=sys.stderr, flush=True)
+    rate = 2.0 * len(jobs) / t_plain  # offered load: twice what plain runs sustain
+    arrivals = np.cumsum(np.random.default_rng(args.seed + 7).exponential(1.0 / rate, len(jobs)))
+    srv = H.Server(0, workers=args.workers)
+    ids = []
+    t0 = time.perf_counter()
+    for (hg, prm, _), ta in zip(jobs, arrivals):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        ids.append(srv.submit(hg, prm))
+    lat, same = [], 0
+    for i, j in enumerate(ids):
+        part, st, l, q = srv.wait(j)
+        lat.append(l)
+        same += int(np.array_equal(part, plain_parts[i]))
+    t_serve = time.perf_counter() - t0
+    srv.close()
+    lat = np.array(lat)
+    return {"metric": "aggregate pins/s of the serving stream; p50/p99 job latency", "value": total_pins / t_serve,
+            "unit": "pins/s", "plain_value": total_pins / t_plain, "speedup_vs_plain": t_plain / t_serve,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "jobs": len(jobs), "workers": args.workers, "total_pins": total_pins,
+            "arrival_rate_per_s": rate, "results_equal_to_plain": same, "data": "synthetic",
+            "config": {"workload": f"{len(jobs)} jobs, pins log-uniform in [1e4, {args.max_pins:g}], half random "
+                                   "(|e| = 2 + Geom(1/2)), half layered SNN (fan-out 64); modes k=2/8/32 eps 0.03 "
+                                   "or -m 1024 4096 1048576; Poisson arrivals at 2x the plain service rate"}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,11 +166,20 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--serve", action="store_true", help="run the serving-stream benchmark instead")
+    ap.add_argument("--jobs", type=int, default=256)
+    ap.add_argument("--max-pins", type=float, default=2e6)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain device copy of 1 GiB (for rocprofv3 --pmc counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(value is the one GPU's throughput)", file=sys.stderr)
         return 2
+    if args.serve:
+        print(json.dumps(serve_bench(args)))
+        return 0
     lib = H.load_library()
 
     t0 = time.perf_counter()
@@ -133,16 +225,28 @@ def main() -> int:
     ms_per_step = elapsed * 1e3 / args.steps
     value = p * args.steps / elapsed
 
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = None
     if kp["launches"]:
         avg_s = kp["ms"] / kp["launches"] / 1e3
         bytes_per_launch = kp["bytes"] / kp["launches"]
         achieved = bytes_per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_PER_LAUNCH,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                     "launches_per_step": kp["launches"] / args.steps, "avg_launch_us": avg_s * 1e6,
                     "algorithmic_bytes_per_launch": bytes_per_launch,
                     "kernel_share_of_step": kp["ms"] / (ms_per_step * args.steps)}
+
+    if args.copy_calibration:
+        import torch  # device memory plumbing only
+        a = torch.empty(COPY_CAL_BYTES, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: {COPY_CAL_BYTES} bytes copied device to device", file=sys.stderr, flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline()
     out = {

@@ -287,6 +287,20 @@ hgp_status hgp_partition_device(hgp_ctx* ctx, const hgp_params* params, uint32_t
                                 const uint32_t* d_srcs_count, const float* d_hedge_weights,
                                 uint32_t* h_partitions_out, hgp_run_stats* stats);
 
+/* ---- continuous-batching service (no counterpart in the reference) ---------
+ * Runs submitted jobs concurrently on one GPU: `workers` host threads, each with its own
+ * context and HIP stream, take queued jobs whose estimated HBM footprint fits next to the
+ * running ones (hbm_budget 0 = 90% of the free device memory).  The hypergraph passed to
+ * submit must stay alive until hgp_server_wait returns for that job.  Results equal those
+ * of hgp_partition on the same input.  latency_ms = submit -> done, queue_ms = submit ->
+ * start. */
+typedef struct hgp_server hgp_server;
+hgp_status hgp_server_create(int device, uint64_t hbm_budget, uint32_t workers, hgp_server** out);
+hgp_status hgp_server_submit(hgp_server* server, const hgp_hgraph* hg, const hgp_params* params, uint64_t* job_id);
+hgp_status hgp_server_wait(hgp_server* server, uint64_t job_id, uint32_t* h_partitions_out, hgp_run_stats* stats,
+                           double* latency_ms, double* queue_ms);
+void hgp_server_destroy(hgp_server* server);
+
 #ifdef __cplusplus
 }
 #endif
