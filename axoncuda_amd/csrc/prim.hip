@@ -114,6 +114,119 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_lookback(const TI* in, TO* 
   }
 }
 
+// Three-pass scan for large inputs (tile reduce, scan of tile sums, tile scan): at thousands
+// of tiles the decoupled look-back's chain of publications is slower than two extra passes.
+constexpr int RS3_ITEMS = 8;
+constexpr int RS3_TILE = SCAN_BLOCK * RS3_ITEMS;
+constexpr uint32_t LOOKBACK_MAX_TILES = 64;
+
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* lds_waves, T* total) {
+  // lds_waves: SCAN_BLOCK/64 entries
+  const uint32_t w = threadIdx.x / 64, l = lane_id();
+  T incl = wave_scan_incl(v);
+  if (l == 63) lds_waves[w] = incl;
+  __syncthreads();
+  T base = 0, tot = 0;
+  for (uint32_t i = 0; i < blockDim.x / 64; ++i) {
+    T x = lds_waves[i];
+    if (i < w) base += x;
+    tot += x;
+  }
+  *total = tot;
+  __syncthreads();
+  return base + incl - v;
+}
+
+template <typename TI, typename TO>
+__global__ void k_tile_reduce(const TI* __restrict__ in, uint64_t n, TO* __restrict__ partial) {
+  __shared__ TO sw[SCAN_BLOCK / 64];
+  uint64_t base = (uint64_t)blockIdx.x * RS3_TILE;
+  TO s = 0;
+  for (int k = 0; k < RS3_ITEMS; ++k) {
+    uint64_t i = base + (uint64_t)k * SCAN_BLOCK + threadIdx.x;
+    if (i < n) s += (TO)in[i];
+  }
+  s = wave_sum(s);
+  if (lane_id() == 0) sw[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    TO t = 0;
+    for (int i = 0; i < SCAN_BLOCK / 64; ++i) t += sw[i];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// exclusive scan of partial[0..m) in place by one workgroup; partial[m] = total
+template <typename TO>
+__global__ void k_scan_partials(TO* partial, uint32_t m) {
+  __shared__ TO sw[1024 / 64];
+  __shared__ TO carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < m; base += blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    TO v = i < m ? partial[i] : 0;
+    TO tot;
+    TO ex = block_excl_scan(v, sw, &tot);
+    TO carry = carry_s;
+    if (i < m) partial[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s = carry + tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[m] = carry_s;
+}
+
+template <typename TI, typename TO>
+__global__ void k_tile_scan_excl(const TI* in, TO* out, uint64_t n, const TO* __restrict__ partial) {
+  __shared__ TO sw[SCAN_BLOCK / 64];
+  uint64_t base = (uint64_t)blockIdx.x * RS3_TILE + (uint64_t)threadIdx.x * RS3_ITEMS;
+  TO v[RS3_ITEMS];
+  TO s = 0;
+#pragma unroll
+  for (int k = 0; k < RS3_ITEMS; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? (TO)in[i] : (TO)0;
+    s += v[k];
+  }
+  TO tot;
+  TO ex = block_excl_scan(s, sw, &tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < RS3_ITEMS; ++k) {
+    uint64_t i = base + k;
+    if (i < n) out[i] = ex;
+    ex += v[k];
+  }
+}
+
+template <typename TI, typename TO>
+__global__ void k_tile_scan_incl(const TI* in, TO* out, uint64_t n, const TO* __restrict__ partial) {
+  __shared__ TO sw[SCAN_BLOCK / 64];
+  uint64_t base = (uint64_t)blockIdx.x * RS3_TILE + (uint64_t)threadIdx.x * RS3_ITEMS;
+  TO v[RS3_ITEMS];
+  TO s = 0;
+#pragma unroll
+  for (int k = 0; k < RS3_ITEMS; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? (TO)in[i] : (TO)0;
+    s += v[k];
+  }
+  TO tot;
+  TO ex = block_excl_scan(s, sw, &tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < RS3_ITEMS; ++k) {
+    uint64_t i = base + k;
+    ex += v[k];
+    if (i < n) out[i] = ex;
+  }
+}
+
+template <typename TO>
+__global__ void k_store_total(TO* out, uint64_t n, const TO* partial, uint32_t m) {
+  out[n] = partial[m];
+}
+
 // Status arrays of the context's scans (one stream: scans never overlap).
 void scan_status(Ctx& c, uint32_t tiles) {
   if (tiles <= c.scan_cap_tiles && c.scan_ticket) return;
@@ -144,6 +257,19 @@ void scan_impl(Ctx& c, const TI* in, TO* out, uint64_t n, bool exclusive) {
     return;
   }
   const uint32_t tiles = ceil_div(n, SCAN_TILE);
+  if (tiles > LOOKBACK_MAX_TILES) {
+    const uint32_t t3 = ceil_div(n, RS3_TILE);
+    DBuf<TO> partial(c, (uint64_t)t3 + 1);
+    HGP_LAUNCH((k_tile_reduce<TI, TO>), t3, SCAN_BLOCK, 0, c, in, n, partial.get());
+    HGP_LAUNCH((k_scan_partials<TO>), 1, 1024, 0, c, partial.get(), t3);
+    if (exclusive) {
+      HGP_LAUNCH((k_tile_scan_excl<TI, TO>), t3, SCAN_BLOCK, 0, c, in, out, n, partial.get());
+      HGP_LAUNCH((k_store_total<TO>), 1, 1, 0, c, out, n, partial.get(), t3);
+    } else {
+      HGP_LAUNCH((k_tile_scan_incl<TI, TO>), t3, SCAN_BLOCK, 0, c, in, out, n, partial.get());
+    }
+    return;
+  }
   scan_status(c, tiles);
   c.scan_gen = (c.scan_gen + 1) & 0x3FFFFFFFu;
   if (c.scan_gen == 0) c.scan_gen = 1;
@@ -269,9 +395,19 @@ __global__ void k_minmax_u64(const uint64_t* __restrict__ p, uint64_t n, unsigne
     mn = v < mn ? v : mn;
     mx = v > mx ? v : mx;
   }
+  __shared__ uint64_t s_mn[4], s_mx[4];
   mn = wave_min(mn);
   mx = wave_max(mx);
   if (lane_id() == 0) {
+    s_mn[threadIdx.x / 64] = mn;
+    s_mx[threadIdx.x / 64] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 4; ++i) {
+      mn = s_mn[i] < mn ? s_mn[i] : mn;
+      mx = s_mx[i] > mx ? s_mx[i] : mx;
+    }
     atomicMin(&out[0], (unsigned long long)mn);
     atomicMax(&out[1], (unsigned long long)mx);
   }
@@ -330,7 +466,7 @@ void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n) {
   DBuf<unsigned long long> mm(c, 2);
   const unsigned long long init[2] = {~0ull, 0ull};
   copy_h2d(c, mm.get(), init, sizeof init);
-  HGP_LAUNCH(k_minmax_u64, std::min<uint32_t>(ceil_div(n, 256), 1024), 256, 0, c, keys, n, mm.get());
+  HGP_LAUNCH(k_minmax_u64, std::min<uint32_t>(ceil_div(n, 256), 512), 256, 0, c, keys, n, mm.get());
   unsigned long long h[2];
   copy_d2h(c, h, mm.get(), sizeof h);
   if (h[1] == h[0]) return;  // all keys equal: a stable sort is the identity
