@@ -175,13 +175,118 @@ class Heap {
   std::vector<int64_t> key_;
 };
 
+// The same queue as a dense array scanned for its maximum (key desc, id asc, as the heap):
+// an update is O(1) and the maximum is recomputed only when the cached one is removed or
+// lowered.  For the few thousand nodes of a coarsest level this beats the heap's sift per
+// gain update by a wide margin.
+class DenseQueue {
+ public:
+  void reset(uint32_t n) {
+    key_.assign(n, kAbsent);
+    count_ = 0;
+    cached_ = UINT32_MAX;
+  }
+  bool contains(uint32_t v) const { return key_[v] != kAbsent; }
+  bool empty() const { return count_ == 0; }
+  uint32_t top() const {
+    if (cached_ == UINT32_MAX) {
+      int64_t best = kAbsent;
+      uint32_t bi = UINT32_MAX;
+      const int64_t* k = key_.data();
+      for (uint32_t i = 0, n = (uint32_t)key_.size(); i < n; ++i)
+        if (k[i] > best) {
+          best = k[i];
+          bi = i;
+        }
+      cached_ = bi;
+    }
+    return cached_;
+  }
+  // Best entry satisfying `ok` (one scan); UINT32_MAX when there is none.
+  template <typename Ok>
+  uint32_t best_if(Ok&& ok) const {
+    int64_t best = kAbsent;
+    uint32_t bi = UINT32_MAX;
+    const int64_t* k = key_.data();
+    for (uint32_t i = 0, n = (uint32_t)key_.size(); i < n; ++i)
+      if (k[i] > best && ok(i)) {
+        best = k[i];
+        bi = i;
+      }
+    return bi;
+  }
+  void push(uint32_t v, int64_t k) {
+    key_[v] = k;
+    count_++;
+    if (cached_ != UINT32_MAX && better(k, v, key_[cached_], cached_)) cached_ = v;
+  }
+  void update(uint32_t v, int64_t k) {
+    const int64_t old = key_[v];
+    key_[v] = k;
+    if (cached_ == UINT32_MAX) return;
+    if (v == cached_) {
+      if (k < old) cached_ = UINT32_MAX;
+    } else if (better(k, v, key_[cached_], cached_)) {
+      cached_ = v;
+    }
+  }
+  void remove(uint32_t v) {
+    key_[v] = kAbsent;
+    count_--;
+    if (v == cached_) cached_ = UINT32_MAX;
+  }
+
+ private:
+  static constexpr int64_t kAbsent = INT64_MIN;
+  static bool better(int64_t ka, uint32_t a, int64_t kb, uint32_t b) { return ka > kb || (ka == kb && a < b); }
+  std::vector<int64_t> key_;
+  uint32_t count_ = 0;
+  mutable uint32_t cached_ = UINT32_MAX;
+};
+
+// Gain queue of a bisection: dense scans for small sub-hypergraphs, a heap otherwise.
+class GainQueue {
+ public:
+  void reset(uint32_t n) {
+    dense_ = n <= kDenseMax;
+    if (dense_) d_.reset(n);
+    else h_.reset(n);
+  }
+  bool contains(uint32_t v) const { return dense_ ? d_.contains(v) : h_.contains(v); }
+  bool empty() const { return dense_ ? d_.empty() : h_.empty(); }
+  uint32_t top() const { return dense_ ? d_.top() : h_.top(); }
+  void push(uint32_t v, int64_t k) { dense_ ? d_.push(v, k) : h_.push(v, k); }
+  void update(uint32_t v, int64_t k) { dense_ ? d_.update(v, k) : h_.update(v, k); }
+  void remove(uint32_t v) { dense_ ? d_.remove(v) : h_.remove(v); }
+  // Best queued node satisfying `ok`.  The heap pops the failing tops on the way (into
+  // `parked` when given; the caller re-queues them once they may fit again), which selects
+  // the same node as the dense scan.
+  template <typename Ok>
+  uint32_t best_if(Ok&& ok, std::vector<uint32_t>* parked) {
+    if (dense_) return d_.best_if(ok);
+    while (!h_.empty()) {
+      const uint32_t t = h_.top();
+      if (ok(t)) return t;
+      h_.remove(t);
+      if (parked) parked->push_back(t);
+    }
+    return UINT32_MAX;
+  }
+
+ private:
+  static constexpr uint32_t kDenseMax = 32768;
+  bool dense_ = true;
+  DenseQueue d_;
+  Heap h_;
+};
+
 // Two-way partitioning of a sub-hypergraph (local ids 0..n-1).
 struct Bisector {
   const Hypergraph& h;
   std::vector<uint8_t> side;
   std::vector<uint32_t> cnt[2];
   std::vector<int64_t> gain;
-  Heap heap[2];
+  GainQueue heap[2];
   uint64_t wside[2] = {0, 0};
 
   explicit Bisector(const Hypergraph& hg) : h(hg) {}
@@ -280,12 +385,9 @@ struct Bisector {
     uint32_t cur = seed;
     while (wside[0] < target0) {
       if (cur == UINT32_MAX) {
-        // pick the best frontier node that fits; fall back to the next unvisited node
-        while (!heap[1].empty()) {
-          uint32_t t = heap[1].top();
-          if (wside[0] + h.vw[t] <= max0) { cur = t; break; }
-          heap[1].remove(t);
-        }
+        // pick the best frontier node that fits (a node that does not fit never will: side 0
+        // only grows); fall back to the next unvisited node
+        cur = heap[1].best_if([&](uint32_t t) { return wside[0] + h.vw[t] <= max0; }, nullptr);
         if (cur == UINT32_MAX) {
           while (next_unvisited < h.n && (side[next_unvisited] == 0 || inq[next_unvisited] ||
                                           wside[0] + h.vw[next_unvisited] > max0))
@@ -328,21 +430,17 @@ struct Bisector {
     while (!heap[0].empty() || !heap[1].empty()) {
       // candidate from each side that respects the target's capacity
       int pick = -1;
-      for (int s = 0; s < 2; ++s) {
-        while (!heap[s].empty()) {
-          uint32_t t = heap[s].top();
-          if (wside[1 - s] + h.vw[t] <= maxw[1 - s] || wside[s] > maxw[s]) break;
-          heap[s].remove(t);
-          parked[s].push_back(t);
-        }
-      }
-      int64_t g0 = heap[0].empty() ? INT64_MIN : gain[heap[0].top()];
-      int64_t g1 = heap[1].empty() ? INT64_MIN : gain[heap[1].top()];
+      uint32_t cand[2];
+      for (int s = 0; s < 2; ++s)
+        cand[s] = heap[s].best_if(
+            [&](uint32_t t) { return wside[1 - s] + h.vw[t] <= maxw[1 - s] || wside[s] > maxw[s]; }, &parked[s]);
+      int64_t g0 = cand[0] == UINT32_MAX ? INT64_MIN : gain[cand[0]];
+      int64_t g1 = cand[1] == UINT32_MAX ? INT64_MIN : gain[cand[1]];
       if (g0 == INT64_MIN && g1 == INT64_MIN) break;
       if (g0 > g1) pick = 0;
       else if (g1 > g0) pick = 1;
       else pick = wside[0] >= wside[1] ? 0 : 1;
-      uint32_t v = heap[pick].top();
+      uint32_t v = cand[pick];
       heap[pick].remove(v);
       locked[v] = 1;
       cum += gain[v];
@@ -413,7 +511,7 @@ inline Hypergraph subgraph(const Hypergraph& h, const std::vector<uint32_t>& nod
 #define HGP_IP_TRIES 4
 #endif
 #ifndef HGP_IP_FM_PASSES
-#define HGP_IP_FM_PASSES 3
+#define HGP_IP_FM_PASSES 1
 #endif
 
 struct Options {
