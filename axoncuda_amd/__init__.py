@@ -71,6 +71,7 @@ SIGNATURES = [
     ("hgp_free", C.c_int32, [_vp, _vp]),
     ("hgp_memcpy_htod", C.c_int32, [_vp, _vp, _vp, C.c_uint64]),
     ("hgp_memcpy_dtoh", C.c_int32, [_vp, _vp, _vp, C.c_uint64]),
+    ("hgp_memcpy_dtod", C.c_int32, [_vp, _vp, _vp, C.c_uint64]),
     ("hgp_hgraph_load", C.c_int32, [C.c_char_p, C.c_int, C.POINTER(_vp)]),
     ("hgp_hgraph_from_csr", C.c_int32, [C.c_uint32, C.c_uint32, _u64p, _u32p, _u32p, C.POINTER(C.c_float), C.c_int,
                                         C.POINTER(_vp)]),

@@ -206,6 +206,13 @@ hgp_status hgp_memcpy_htod(hgp_ctx* ctx, void* d_dst, const void* h_src, uint64_
   });
 }
 
+hgp_status hgp_memcpy_dtod(hgp_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes) {
+  return guard(ctx, [&](Ctx& c) {
+    hgp::copy_d2d(c, d_dst, d_src, bytes);
+    hgp::sync(c);
+  });
+}
+
 hgp_status hgp_memcpy_dtoh(hgp_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes) {
   return guard(ctx, [&](Ctx& c) { hgp::copy_d2h(c, h_dst, d_src, bytes); });
 }

@@ -12,7 +12,7 @@ sources/main.cu:260-264,891-896), with inputs already in HBM.
 The roofline object covers the dominant kernel: its launches are bracketed by HIP events on
 the library's stream during the timed steps, and its algorithmic bytes per launch come from
 the library's byte model of that kernel.  cpu_baseline is the sequential oracle run on a
-bounded sample of the same workload (50K nodes) on the host.
+bounded sample of the same workload (100K nodes) on the host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -60,11 +60,11 @@ def kway_params(n: int, k: int, eps: float) -> H.Params:
 
 
 def cpu_baseline(budget_s: float = 30.0) -> dict:
-    """Sequential oracle on a 50K-node sample of the same workload (one host core)."""
+    """Sequential oracle on a 100K-node sample of the same workload (one host core)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    n = 50_000
+    n = 100_000
     e = int(round(n * N_NETS / N_NODES))
     hg = H.Hypergraph.generate("netlist", n=n, e=e, alpha=ALPHA, max_size=MAX_SIZE, radius=RADIUS,
                                frac_local=FRAC_LOCAL, seed=1)
@@ -241,11 +241,10 @@ def main() -> int:
                     "kernel_share_of_step": kp["ms"] / (ms_per_step * args.steps)}
 
     if args.copy_calibration:
-        import torch  # device memory plumbing only
-        a = torch.empty(COPY_CAL_BYTES, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
+        a, b = dev.alloc(COPY_CAL_BYTES), dev.alloc(COPY_CAL_BYTES)
+        dev.check(lib.hgp_memcpy_dtod(dev.ctx, C.c_void_p(b), C.c_void_p(a), COPY_CAL_BYTES), "hgp_memcpy_dtod")
+        dev.free(a)
+        dev.free(b)
         print(f"# copy calibration: {COPY_CAL_BYTES} bytes copied device to device", file=sys.stderr, flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline()

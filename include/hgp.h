@@ -90,6 +90,8 @@ hgp_status hgp_malloc(hgp_ctx* ctx, uint64_t bytes, void** d_ptr);
 hgp_status hgp_free(hgp_ctx* ctx, void* d_ptr);
 hgp_status hgp_memcpy_htod(hgp_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
 hgp_status hgp_memcpy_dtoh(hgp_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+/* Device-to-device copy on the context stream (synchronous); no counterpart in the reference. */
+hgp_status hgp_memcpy_dtod(hgp_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
 
 /* ---- host model: loaders and savers (includes/hgraph.hpp) ---------------- */
 /* Loads .hgr (hMETIS, loadhMETIS :936-1013), .snn (loadSNN :707-743) or .axh
