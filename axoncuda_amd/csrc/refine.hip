@@ -56,9 +56,10 @@ __device__ __forceinline__ uint64_t ordered_i64(int64_t g) { return (uint64_t)g 
 template <bool INB>
 __global__ void __launch_bounds__(256) k_pp_small(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
                                                   const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part,
-                                                  uint32_t E, uint32_t* __restrict__ pp_part, uint32_t* __restrict__ pp_cnt,
+                                                  const float* __restrict__ w, uint32_t E, uint2* __restrict__ pp,
                                                   uint32_t* __restrict__ pp_dcnt, uint32_t* __restrict__ lambda,
-                                                  uint32_t* __restrict__ large_flag, uint32_t* inbsz, uint32_t k_lds) {
+                                                  uint4* __restrict__ hinfo, uint32_t* __restrict__ large_flag,
+                                                  uint32_t* inbsz, uint32_t k_lds) {
   extern __shared__ uint32_t hist[];
   if (INB && k_lds) {
     for (uint32_t i = threadIdx.x; i < k_lds; i += blockDim.x) hist[i] = 0;
@@ -88,8 +89,7 @@ __global__ void __launch_bounds__(256) k_pp_small(const uint32_t* __restrict__ h
         if (INB) ds[k] += (i - s) >= sc ? 1u : 0u;
       }
       for (uint32_t k = 0; k < m; ++k) {
-        pp_part[s + k] = qs[k];
-        pp_cnt[s + k] = cs[k];
+        pp[s + k] = make_uint2(qs[k], cs[k]);
         if (INB) {
           pp_dcnt[s + k] = ds[k];
           if (ds[k]) {
@@ -99,6 +99,7 @@ __global__ void __launch_bounds__(256) k_pp_small(const uint32_t* __restrict__ h
         }
       }
       lambda[e] = m;
+      hinfo[e] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), m, w ? fixed_weight(w[e]) : 0u);
     }
   }
   if (INB && k_lds) {
@@ -111,10 +112,10 @@ __global__ void __launch_bounds__(256) k_pp_small(const uint32_t* __restrict__ h
 template <bool INB>
 __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
                                                  const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part,
-                                                 const uint32_t* __restrict__ list, uint32_t cap,
-                                                 uint32_t* __restrict__ pp_part, uint32_t* __restrict__ pp_cnt,
+                                                 const float* __restrict__ w,
+                                                 const uint32_t* __restrict__ list, uint32_t cap, uint2* __restrict__ pp,
                                                  uint32_t* __restrict__ pp_dcnt, uint32_t* __restrict__ lambda,
-                                                 uint32_t* inbsz) {
+                                                 uint4* __restrict__ hinfo, uint32_t* inbsz) {
   extern __shared__ uint32_t lds[];
   uint32_t* keys = lds;
   uint32_t* cnt = lds + cap;
@@ -148,8 +149,7 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
     const uint64_t m = __ballot(has);
     if (has) {
       const uint32_t o = base + lanes_below(m);
-      pp_part[s + o] = keys[i];
-      pp_cnt[s + o] = cnt[i];
+      pp[s + o] = make_uint2(keys[i], cnt[i]);
       if (INB) {
         pp_dcnt[s + o] = dcnt[i];
         if (dcnt[i]) atomicAdd(&inbsz[keys[i]], 1u);
@@ -157,21 +157,21 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
     }
     base += (uint32_t)__popcll(m);
   }
-  if (l == 0) lambda[e] = base;
+  if (l == 0) {
+    lambda[e] = base;
+    hinfo[e] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), base, w ? fixed_weight(w[e]) : 0u);
+  }
 }
 
 // ---- 2. gains ---------------------------------------------------------------------------
 struct GainArgs {
   const uint32_t* tch;
   const uint64_t* toff;
-  const uint64_t* hoff;
-  const float* w;
+  const uint4* hinfo;  // per hedge: list start (lo, hi), list length (lambda), fixed-point weight
+  const uint2* pp;     // per list entry: (partition, pin count)
   const uint32_t* part;
   const uint32_t* sizes;
   const uint32_t* psizes;
-  const uint32_t* pp_part;
-  const uint32_t* pp_cnt;
-  const uint32_t* lambda;
   const uint32_t* order;  // partitions by mix32(q + repeat) descending
   uint32_t k;
   uint32_t omega;
@@ -280,15 +280,16 @@ __global__ void __launch_bounds__(256) k_gains_tiny(GainArgs a, const uint32_t* 
   uint32_t m = 0;
   int64_t saving = 0, Wv = 0;
   for (uint64_t j = a.toff[v]; j < a.toff[v + 1]; ++j) {
-    const uint32_t e = a.tch[j];
-    const int64_t We = (int64_t)fixed_weight(a.w[e]);
+    const uint4 hi = a.hinfo[a.tch[j]];
+    const int64_t We = (int64_t)hi.w;
     Wv += We;
-    const uint64_t b = a.hoff[e];
-    const uint32_t lam = a.lambda[e];
+    const uint64_t b = ((uint64_t)hi.y << 32) | hi.x;
+    const uint32_t lam = hi.z;
     for (uint32_t x = 0; x < lam; ++x) {
-      const uint32_t q = a.pp_part[b + x];
+      const uint2 pe = a.pp[b + x];
+      const uint32_t q = pe.x;
       if (q == p) {
-        const uint32_t c = a.pp_cnt[b + x];
+        const uint32_t c = pe.y;
         if (a.incc) saving += (int64_t)((uint64_t)We / ((uint64_t)c * c));
         else if (c == 1) saving += We;
         continue;
@@ -373,15 +374,13 @@ __global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t*
   const uint32_t p = a.part[v], s = a.sizes[v];
   const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
   int64_t saving = 0, Wv = 0;
-  for (uint64_t j = t0 + l; j < t1; j += 64) Wv += (int64_t)fixed_weight(a.w[a.tch[j]]);
-  wave_visit_entries(
-      a.tch, t0, t1, [&](uint32_t e) { return a.hoff[e]; }, [&](uint32_t e) { return a.lambda[e]; },
-      [&](const EntryVisit& ev) {
+  Wv += wave_visit_hedge_lists(a.tch, t0, t1, a.hinfo, [&](const EntryVisit& ev) {
         if (!ev.act) return;
-        const uint32_t q = a.pp_part[ev.abs];
-        const int64_t We = (int64_t)fixed_weight(a.w[ev.e]);
+        const uint2 pe = a.pp[ev.abs];
+        const uint32_t q = pe.x;
+        const int64_t We = (int64_t)ev.w;
         if (q == p) {
-          const uint32_t c = a.pp_cnt[ev.abs];
+          const uint32_t c = pe.y;
           if (a.incc) saving += (int64_t)((uint64_t)We / ((uint64_t)c * c));
           else if (c == 1) saving += We;
           return;
@@ -439,15 +438,13 @@ __global__ void __launch_bounds__(256) k_gains_heavy(GainArgs a, const uint32_t*
   const uint64_t cnt = t1 - t0;
   const uint64_t lo = t0 + cnt * wv / 4, hi = t0 + cnt * (wv + 1) / 4;
   int64_t saving = 0, Wv = 0;
-  for (uint64_t j = lo + l; j < hi; j += 64) Wv += (int64_t)fixed_weight(a.w[a.tch[j]]);
-  wave_visit_entries(
-      a.tch, lo, hi, [&](uint32_t e) { return a.hoff[e]; }, [&](uint32_t e) { return a.lambda[e]; },
-      [&](const EntryVisit& ev) {
+  Wv += wave_visit_hedge_lists(a.tch, lo, hi, a.hinfo, [&](const EntryVisit& ev) {
         if (!ev.act) return;
-        const uint32_t q = a.pp_part[ev.abs];
-        const int64_t We = (int64_t)fixed_weight(a.w[ev.e]);
+        const uint2 pe = a.pp[ev.abs];
+        const uint32_t q = pe.x;
+        const int64_t We = (int64_t)ev.w;
         if (q == p) {
-          const uint32_t c = a.pp_cnt[ev.abs];
+          const uint32_t c = pe.y;
           if (a.incc) saving += (int64_t)((uint64_t)We / ((uint64_t)c * c));
           else if (c == 1) saving += We;
           return;
@@ -1052,19 +1049,19 @@ __global__ void k_inbound_events(const uint32_t* __restrict__ part, const uint32
   }
 }
 // count of destination pins of hedge e in partition q (0 when absent)
-__device__ __forceinline__ int64_t dst_count(const uint64_t* hoff, const uint32_t* lambda, const uint32_t* pp_part,
-                                             const uint32_t* pp_dcnt, uint32_t e, uint32_t q) {
-  const uint64_t s = hoff[e];
-  const uint32_t m = lambda[e];
-  for (uint32_t i = 0; i < m; ++i)
-    if (pp_part[s + i] == q) return pp_dcnt[s + i];
+__device__ __forceinline__ int64_t dst_count(const uint4* hinfo, const uint2* pp, const uint32_t* pp_dcnt, uint32_t e,
+                                             uint32_t q) {
+  const uint4 hi = hinfo[e];
+  const uint64_t s = ((uint64_t)hi.y << 32) | hi.x;
+  for (uint32_t i = 0; i < hi.z; ++i)
+    if (pp[s + i].x == q) return pp_dcnt[s + i];
   return 0;
 }
 __global__ void k_inbound_transitions(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                       const int64_t* __restrict__ cum, uint64_t m, int eb, int rb,
-                                      const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ lambda,
-                                      const uint32_t* __restrict__ pp_part, const uint32_t* __restrict__ pp_dcnt,
-                                      uint32_t* __restrict__ tflag, int32_t* __restrict__ tdelta) {
+                                      const uint4* __restrict__ hinfo, const uint2* __restrict__ pp,
+                                      const uint32_t* __restrict__ pp_dcnt, uint32_t* __restrict__ tflag,
+                                      int32_t* __restrict__ tdelta) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const uint64_t key = keys[i];
@@ -1079,7 +1076,7 @@ __global__ void k_inbound_transitions(const uint64_t* __restrict__ keys, const u
   const int64_t before = lo == 0 ? 0 : cum[lo - 1];
   const uint32_t q = (uint32_t)(key >> (eb + rb));
   const uint32_t e = (uint32_t)((key >> rb) & (((uint64_t)1 << eb) - 1));
-  const int64_t init = dst_count(hoff, lambda, pp_part, pp_dcnt, e, q);
+  const int64_t init = dst_count(hinfo, pp, pp_dcnt, e, q);
   const int64_t cur = init + (cum[i] - before);
   const int64_t prev = cur - (int64_t)(int32_t)vals[i];
   int32_t d = 0;
@@ -1219,15 +1216,17 @@ __global__ void k_max_u32(const uint32_t* __restrict__ p, uint32_t n, uint32_t* 
 namespace {
 
 struct PartitionLists {
-  DBuf<uint32_t> part, cnt, dcnt, lambda, inbsz;
+  DBuf<uint2> pp;      // (partition, pin count) per list entry, at the hedge's own offsets
+  DBuf<uint4> hinfo;   // per hedge: list start (lo, hi), length, fixed-point weight
+  DBuf<uint32_t> dcnt, lambda, inbsz;
 };
 
 // Partition lists of every hedge.  want_inbound adds the destination counts and the
 // per-partition inbound sizes (the inbound-set constraint needs them; k-way does not).
 void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, PartitionLists& pl, bool want_inbound) {
   const uint64_t P = L.e ? read_scalar(c, L.hoff + L.e) : 0;
-  pl.part = DBuf<uint32_t>(c, P);
-  pl.cnt = DBuf<uint32_t>(c, P);
+  pl.pp = DBuf<uint2>(c, P);
+  pl.hinfo = DBuf<uint4>(c, L.e);
   pl.lambda = DBuf<uint32_t>(c, L.e);
   if (want_inbound) {
     pl.dcnt = DBuf<uint32_t>(c, P);
@@ -1238,10 +1237,11 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
   if (want_inbound) {
     const uint32_t k_lds = L.k <= PP_HIST_CAP ? L.k : 0u;
     HGP_LAUNCH((k_pp_small<true>), ceil_div(L.e, 256), 256, k_lds * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part,
-               L.e, pl.part.get(), pl.cnt.get(), pl.dcnt.get(), pl.lambda.get(), lflag.get(), pl.inbsz.get(), k_lds);
+               L.w, L.e, pl.pp.get(), pl.dcnt.get(), pl.lambda.get(), pl.hinfo.get(), lflag.get(), pl.inbsz.get(),
+               k_lds);
   } else {
-    HGP_LAUNCH((k_pp_small<false>), ceil_div(L.e, 256), 256, 0, c, L.hedges, L.hoff, L.srcc, part, L.e, pl.part.get(),
-               pl.cnt.get(), (uint32_t*)nullptr, pl.lambda.get(), lflag.get(), (uint32_t*)nullptr, 0u);
+    HGP_LAUNCH((k_pp_small<false>), ceil_div(L.e, 256), 256, 0, c, L.hedges, L.hoff, L.srcc, part, L.w, L.e,
+               pl.pp.get(), (uint32_t*)nullptr, pl.lambda.get(), pl.hinfo.get(), lflag.get(), (uint32_t*)nullptr, 0u);
   }
   DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
   exclusive_scan(c, lflag.get(), pos.get(), L.e);
@@ -1255,11 +1255,11 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
   uint32_t cap = 128;
   while (cap < 2ull * L.k + 64 && cap < PP_LDS_CAP) cap <<= 1;
   if (want_inbound)
-    HGP_LAUNCH((k_pp_large<true>), nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, large.get(),
-               cap, pl.part.get(), pl.cnt.get(), pl.dcnt.get(), pl.lambda.get(), pl.inbsz.get());
+    HGP_LAUNCH((k_pp_large<true>), nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, L.w,
+               large.get(), cap, pl.pp.get(), pl.dcnt.get(), pl.lambda.get(), pl.hinfo.get(), pl.inbsz.get());
   else
-    HGP_LAUNCH((k_pp_large<false>), nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, large.get(),
-               cap, pl.part.get(), pl.cnt.get(), (uint32_t*)nullptr, pl.lambda.get(), (uint32_t*)nullptr);
+    HGP_LAUNCH((k_pp_large<false>), nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, L.w,
+               large.get(), cap, pl.pp.get(), (uint32_t*)nullptr, pl.lambda.get(), pl.hinfo.get(), (uint32_t*)nullptr);
 }
 
 uint32_t discount_of(uint32_t rep, uint32_t R) { return rep < R / 3 ? 1u : (rep < 2 * R / 3 ? 2u : NONE); }
@@ -1276,15 +1276,15 @@ uint32_t mix32_host(uint32_t x) {
 }
 
 // Algorithmic bytes of the gain kernels over `list` (evaluated only while the kernel is
-// profiled): per node 36 (partition, size, two offsets, move and gain out) + 24 per touching
-// hedge (hedge id, weight, offset, list length, own-partition count) + 4 per listed
-// partition entry.
+// profiled): per node 36 (partition, size, two offsets, move and gain out) + 20 per touching
+// hedge (hedge id and its 16-byte record: list start, list length, weight) + 8 per listed
+// partition entry (partition and pin count).
 __global__ void k_gain_bytes(const uint32_t* __restrict__ list, uint32_t m, const uint64_t* __restrict__ toff,
                              const uint64_t* __restrict__ bound, unsigned long long* out) {
   uint64_t b = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
     const uint32_t v = list[i];
-    b += 36 + 24 * (toff[v + 1] - toff[v]) + 4 * bound[v];
+    b += 36 + 20 * (toff[v + 1] - toff[v]) + 8 * bound[v];
   }
   b = wave_sum(b);
   if (lane_id() == 0 && b) atomicAdd(out, (unsigned long long)b);
@@ -1310,14 +1310,11 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   GainArgs a;
   a.tch = L.tch;
   a.toff = L.toff;
-  a.hoff = L.hoff;
-  a.w = L.w;
+  a.hinfo = pl.hinfo.get();
+  a.pp = pl.pp.get();
   a.part = part;
   a.sizes = L.sizes;
   a.psizes = psizes;
-  a.pp_part = pl.part.get();
-  a.pp_cnt = pl.cnt.get();
-  a.lambda = pl.lambda.get();
   a.order = dorder.get();
   a.k = L.k;
   a.omega = omega;
@@ -1530,8 +1527,8 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
       inclusive_scan(c, cum.get(), cum.get(), m);
       DBuf<uint32_t> tflag(c, m);
       DBuf<int32_t> tdelta(c, m);
-      HGP_LAUNCH(k_inbound_transitions, ceil_div(m, 256), 256, 0, c, keys.get(), vals.get(), cum.get(), m, eb, rb, L.hoff,
-                 pl.lambda.get(), pl.part.get(), pl.dcnt.get(), tflag.get(), tdelta.get());
+      HGP_LAUNCH(k_inbound_transitions, ceil_div(m, 256), 256, 0, c, keys.get(), vals.get(), cum.get(), m, eb, rb,
+                 pl.hinfo.get(), pl.pp.get(), pl.dcnt.get(), tflag.get(), tdelta.get());
       DBuf<uint64_t> tpos(c, m + 1);
       exclusive_scan(c, tflag.get(), tpos.get(), m);
       const uint64_t T = read_scalar(c, tpos.get() + m);

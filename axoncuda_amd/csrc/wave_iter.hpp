@@ -36,6 +36,7 @@ struct EntryVisit {
   uint32_t e;    // hedge id
   uint32_t pos;  // position inside the hedge's range
   uint64_t abs;  // absolute entry index
+  uint32_t w;    // the hedge's fixed-point weight (wave_visit_hedge_lists only)
 };
 
 template <typename StartF, typename LenF, typename F>
@@ -74,6 +75,53 @@ __device__ __forceinline__ void wave_visit_entries(const uint32_t* __restrict__ 
       visit(ev);
     }
   }
+}
+
+// The same traversal over per-hedge lists described by one 16-byte record per hedge
+// (start lo, start hi, length, fixed-point weight), so each touching hedge costs one load.
+// Returns this lane's share of the summed weights of the touching hedges.
+template <typename F>
+__device__ __forceinline__ int64_t wave_visit_hedge_lists(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
+                                                          const uint4* __restrict__ hinfo, F&& visit) {
+  const uint32_t l = lane_id();
+  int64_t wsum = 0;
+  for (uint64_t jb = t0; jb < t1; jb += 64) {
+    const uint64_t j = jb + l;
+    uint32_t e = 0, len = 0, wf = 0;
+    uint64_t s = 0;
+    if (j < t1) {
+      e = tch[j];
+      const uint4 hi = hinfo[e];
+      s = ((uint64_t)hi.y << 32) | hi.x;
+      len = hi.z;
+      wf = hi.w;
+      wsum += wf;
+    }
+    const uint32_t incl = wave_scan_incl(len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - len;
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+      const uint32_t fi = f0 + l;
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t step = 32; step > 0; step >>= 1) {
+        const uint32_t cand = k + step;
+        const uint32_t ex = __shfl(excl, (int)cand, 64);
+        if (ex <= fi) k = cand;
+      }
+      EntryVisit ev;
+      ev.act = fi < total;
+      ev.e = __shfl(e, (int)k, 64);
+      ev.w = __shfl(wf, (int)k, 64);
+      const uint64_t os = shfl_u64(s, k);
+      const uint32_t oex = __shfl(excl, (int)k, 64);
+      ev.j = jb + k;
+      ev.pos = fi - oex;
+      ev.abs = os + ev.pos;
+      visit(ev);
+    }
+  }
+  return wsum;
 }
 
 // `stop()` (wave-uniform) is checked before every step; the traversal ends when it is true.
