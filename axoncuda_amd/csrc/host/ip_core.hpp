@@ -34,14 +34,49 @@ inline uint64_t splitmix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
-inline void build_incidence(Hypergraph& h) {
+// Parallel for over [0, m) in `threads` contiguous chunks (the calling thread takes one).
+template <typename F>
+inline void parallel_chunks(uint64_t m, uint32_t threads, F&& f) {
+  if (threads <= 1 || m < 2) {
+    f(0u, (uint64_t)0, m);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (uint32_t t = 1; t < threads; ++t) th.emplace_back([&, t] { f(t, m * t / threads, m * (t + 1) / threads); });
+  f(0u, (uint64_t)0, m / threads);
+  for (auto& x : th) x.join();
+}
+
+// Incidence lists (hedges of each node, ascending).  Large inputs are split over threads by
+// hedge range: per-thread node counts give every thread its own write positions, so the
+// lists come out exactly as a sequential pass writes them.
+inline void build_incidence(Hypergraph& h, uint32_t threads = 1) {
+  const uint32_t m = (uint32_t)h.off.size() - 1;
+  if ((uint64_t)h.n * threads > h.pins.size()) threads = 1;  // per-thread counts would dominate
+  threads = std::max<uint32_t>(1, std::min<uint32_t>(threads, (uint32_t)(h.pins.size() >> 20) + 1));
+  std::vector<uint64_t> cnt((uint64_t)threads * (h.n + 1), 0);
+  parallel_chunks(m, threads, [&](uint32_t t, uint64_t lo, uint64_t hi) {
+    uint64_t* c = cnt.data() + (uint64_t)t * (h.n + 1);
+    for (uint64_t i = h.off[lo]; i < h.off[hi]; ++i) c[h.pins[i]]++;
+  });
   h.ioff.assign(h.n + 1, 0);
-  for (uint32_t p : h.pins) h.ioff[p + 1]++;
-  for (uint32_t v = 0; v < h.n; ++v) h.ioff[v + 1] += h.ioff[v];
+  uint64_t run = 0;
+  for (uint32_t v = 0; v < h.n; ++v) {
+    h.ioff[v] = run;
+    for (uint32_t t = 0; t < threads; ++t) {
+      uint64_t& c = cnt[(uint64_t)t * (h.n + 1) + v];
+      const uint64_t x = c;
+      c = run;  // becomes thread t's write position for node v
+      run += x;
+    }
+  }
+  h.ioff[h.n] = run;
   h.inc.resize(h.pins.size());
-  std::vector<uint64_t> pos(h.ioff.begin(), h.ioff.end() - 1);
-  for (uint32_t e = 0; e + 1 < h.off.size(); ++e)
-    for (uint64_t i = h.off[e]; i < h.off[e + 1]; ++i) h.inc[pos[h.pins[i]]++] = e;
+  parallel_chunks(m, threads, [&](uint32_t t, uint64_t lo, uint64_t hi) {
+    uint64_t* pos = cnt.data() + (uint64_t)t * (h.n + 1);
+    for (uint64_t e = lo; e < hi; ++e)
+      for (uint64_t i = h.off[e]; i < h.off[e + 1]; ++i) h.inc[pos[h.pins[i]]++] = (uint32_t)e;
+  });
 }
 
 // Coarse level -> IP input: hedges with >= 2 distinct pins, identical pin sets merged
@@ -79,37 +114,63 @@ inline Hypergraph from_level(uint32_t n, uint32_t e, const uint64_t* hoff, const
       th.emplace_back(work, (uint32_t)((uint64_t)e * t / threads), (uint32_t)((uint64_t)e * (t + 1) / threads));
     for (auto& x : th) x.join();
   }
-  // 2. merge identical sets in hedge order
-  uint64_t cap = 16;
-  while (cap < 2ull * e + 2) cap <<= 1;
-  std::vector<uint32_t> table(cap, UINT32_MAX);  // unique hedge ids
-  std::vector<uint64_t> hashes;                  // hash of each unique hedge
-  h.off.push_back(0);
-  for (uint32_t i = 0; i < e; ++i) {
-    const uint32_t l = len[i];
-    if (l < 2) continue;
-    const uint32_t* b = sp.data() + hoff[i];
-    const uint64_t x = hs[i];
-    uint64_t slot = x & (cap - 1);
-    bool merged = false;
-    while (table[slot] != UINT32_MAX) {
-      const uint32_t cand = table[slot];
-      const uint64_t s0 = h.off[cand], l0 = h.off[cand + 1] - s0;
-      if (hashes[cand] == x && l0 == l && std::equal(b, b + l, h.pins.begin() + s0)) {
-        h.w[cand] += wfx[i];
-        merged = true;
-        break;
+  // 2. merge identical sets: every hedge finds the first hedge with the same pin set.  The
+  // hash space is split over threads; each thread walks its hedges in order with a table of
+  // its own, so rep[] does not depend on the thread count.
+  std::vector<uint32_t> rep(e, UINT32_MAX);
+  {
+    const uint32_t T = threads;
+    std::vector<std::thread> th;
+    auto part = [&](uint32_t t) {
+      uint64_t mine = 0;
+      for (uint32_t i = 0; i < e; ++i) mine += len[i] >= 2 && (hs[i] >> 48) % T == t;
+      uint64_t cap = 16;
+      while (cap < 2 * mine + 2) cap <<= 1;
+      std::vector<uint32_t> table(cap, UINT32_MAX);
+      for (uint32_t i = 0; i < e; ++i) {
+        const uint32_t l = len[i];
+        if (l < 2 || (hs[i] >> 48) % T != t) continue;
+        const uint64_t x = hs[i];
+        const uint32_t* b = sp.data() + hoff[i];
+        uint64_t slot = x & (cap - 1);
+        for (;;) {
+          const uint32_t c = table[slot];
+          if (c == UINT32_MAX) {
+            table[slot] = i;
+            rep[i] = i;
+            break;
+          }
+          if (hs[c] == x && len[c] == l && std::equal(b, b + l, sp.data() + hoff[c])) {
+            rep[i] = c;
+            break;
+          }
+          slot = (slot + 1) & (cap - 1);
+        }
       }
-      slot = (slot + 1) & (cap - 1);
-    }
-    if (merged) continue;
-    table[slot] = (uint32_t)h.w.size();
-    hashes.push_back(x);
-    h.pins.insert(h.pins.end(), b, b + l);
-    h.off.push_back(h.pins.size());
-    h.w.push_back(wfx[i]);
+    };
+    for (uint32_t t = 1; t < T; ++t) th.emplace_back(part, t);
+    part(0);
+    for (auto& x : th) x.join();
   }
-  build_incidence(h);
+  // 3. unique hedges in order of first appearance; weights summed over their copies
+  std::vector<uint32_t> uid(e, UINT32_MAX);
+  h.off.push_back(0);
+  uint64_t np = 0;
+  for (uint32_t i = 0; i < e; ++i)
+    if (rep[i] == i) {
+      uid[i] = (uint32_t)h.w.size();
+      h.w.push_back(0);
+      np += len[i];
+      h.off.push_back(np);
+    }
+  for (uint32_t i = 0; i < e; ++i)
+    if (rep[i] != UINT32_MAX) h.w[uid[rep[i]]] += wfx[i];
+  h.pins.resize(np);
+  parallel_chunks(e, threads, [&](uint32_t, uint64_t lo, uint64_t hi) {
+    for (uint64_t i = lo; i < hi; ++i)
+      if (rep[i] == i) std::copy(sp.data() + hoff[i], sp.data() + hoff[i] + len[i], h.pins.data() + h.off[uid[i]]);
+  });
+  build_incidence(h, threads);
   return h;
 }
 
@@ -281,73 +342,94 @@ class GainQueue {
 };
 
 // Two-way partitioning of a sub-hypergraph (local ids 0..n-1).
-struct Bisector {
+// U = uint32_t packs a hedge into 16 bytes (pin offsets and weights below 2^32), uint64_t
+// into 24 bytes for anything larger.
+template <typename U>
+struct BisectorT {
+  // Everything a move reads or writes about one hedge, in one record (one cache line access
+  // per incident hedge instead of one per array).  The pin count is c[0] + c[1].
+  struct HRec {
+    U off;          // first pin
+    U w;            // weight
+    uint32_t c[2];  // pins on side 0 / side 1
+  };
   const Hypergraph& h;
+  std::vector<HRec> rec;
   std::vector<uint8_t> side;
-  std::vector<uint32_t> cnt[2];
   std::vector<int64_t> gain;
   GainQueue heap[2];
   uint64_t wside[2] = {0, 0};
+  bool gains_exact = false;  // gain[] holds every node's current move gain
 
-  explicit Bisector(const Hypergraph& hg) : h(hg) {}
+  explicit BisectorT(const Hypergraph& hg) : h(hg) {
+    const uint32_t m = (uint32_t)h.off.size() - 1;
+    rec.resize(m);
+    for (uint32_t e = 0; e < m; ++e) {
+      rec[e].off = (U)h.off[e];
+      rec[e].w = (U)h.w[e];
+    }
+  }
+  static uint32_t len(const HRec& r) { return r.c[0] + r.c[1]; }
 
   int64_t cut() const {
     int64_t c = 0;
-    for (uint32_t e = 0; e + 1 < h.off.size(); ++e)
-      if (cnt[0][e] && cnt[1][e]) c += (int64_t)h.w[e];
+    for (const HRec& r : rec)
+      if (r.c[0] && r.c[1]) c += (int64_t)r.w;
     return c;
   }
   void init_counts() {
-    uint32_t m = (uint32_t)h.off.size() - 1;
-    cnt[0].assign(m, 0);
-    cnt[1].assign(m, 0);
     wside[0] = wside[1] = 0;
     for (uint32_t v = 0; v < h.n; ++v) wside[side[v]] += h.vw[v];
-    for (uint32_t e = 0; e < m; ++e)
-      for (uint64_t i = h.off[e]; i < h.off[e + 1]; ++i) cnt[side[h.pins[i]]][e]++;
+    for (uint32_t e = 0; e < (uint32_t)rec.size(); ++e) {
+      HRec& r = rec[e];
+      r.c[0] = r.c[1] = 0;
+      for (uint64_t i = h.off[e]; i < h.off[e + 1]; ++i) r.c[side[h.pins[i]]]++;
+    }
   }
   int64_t node_gain(uint32_t v) const {
     uint8_t a = side[v], b = 1 - a;
     int64_t g = 0;
-    for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
-      uint32_t e = h.inc[i];
-      if (cnt[a][e] == 1) g += (int64_t)h.w[e];
-      if (cnt[b][e] == 0) g -= (int64_t)h.w[e];
+    const uint64_t i0 = h.ioff[v], i1 = h.ioff[v + 1];
+    for (uint64_t i = i0; i < i1; ++i) {
+      if (i + 16 < i1) __builtin_prefetch(&rec[h.inc[i + 16]]);
+      const HRec& r = rec[h.inc[i]];
+      if (r.c[a] == 1) g += (int64_t)r.w;
+      if (r.c[b] == 0) g -= (int64_t)r.w;
     }
     return g;
   }
-  // Move v to the other side, updating the gains of free pins (classic FM delta rules).
+  // Move v to the other side, updating the gains of free pins (classic FM delta rules,
+  // all four applied in one scan of the hedge).
   void move(uint32_t v, const std::vector<uint8_t>& locked) {
-    uint8_t a = side[v], b = 1 - a;
-    for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
-      uint32_t e = h.inc[i];
-      int64_t we = (int64_t)h.w[e];
-      if (cnt[b][e] == 0) {
-        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
-          uint32_t u = h.pins[j];
-          if (u != v && !locked[u]) bump(u, we);
-        }
-      } else if (cnt[b][e] == 1) {
-        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
-          uint32_t u = h.pins[j];
-          if (u != v && side[u] == b && !locked[u]) bump(u, -we);
-        }
-      }
-      cnt[a][e]--;
-      cnt[b][e]++;
-      if (cnt[a][e] == 0) {
-        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
-          uint32_t u = h.pins[j];
-          if (u != v && !locked[u]) bump(u, -we);
-        }
-      } else if (cnt[a][e] == 1) {
-        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
-          uint32_t u = h.pins[j];
-          if (u != v && side[u] == a && !locked[u]) bump(u, we);
-        }
+    const uint8_t a = side[v], b = 1 - a;
+    const uint64_t i0 = h.ioff[v], i1 = h.ioff[v + 1];
+    for (uint64_t i = i0; i < i1; ++i) {
+      // the incident hedges are scattered over the record array: fetch ahead to overlap misses
+      if (i + 16 < i1) __builtin_prefetch(&rec[h.inc[i + 16]], 1);
+      if (i + 8 < i1) __builtin_prefetch(h.pins.data() + rec[h.inc[i + 8]].off);
+      HRec& r = rec[h.inc[i]];
+      const int64_t we = (int64_t)r.w;
+      const uint32_t bprev = r.c[b], anew = r.c[a] - 1;
+      r.c[a] = anew;
+      r.c[b] = bprev + 1;
+      // gain change of a free pin u on side a (da) and on side b (db)
+      int64_t da = 0, db = 0;
+      if (bprev == 0) da += we;    // the hedge stops being internal to side a
+      if (bprev == 1) db -= we;    // the lone b pin no longer frees the hedge by moving
+      if (anew == 0) db -= we;     // the hedge becomes internal to side b
+      if (anew == 1) da += we;     // the lone a pin now frees the hedge by moving
+      if (da == 0 && db == 0) continue;
+      const uint32_t* pp = h.pins.data() + r.off;
+      const uint32_t rl = len(r);
+      for (uint32_t j = 0; j < rl; ++j) {
+        const uint32_t u = pp[j];
+        if (u == v || locked[u]) continue;
+        const int64_t d = side[u] == a ? da : db;
+        if (d) bump(u, d);
       }
     }
     side[v] = b;
+    gain[v] = -gain[v];  // moving back undoes the move
     wside[a] -= h.vw[v];
     wside[b] += h.vw[v];
   }
@@ -371,11 +453,15 @@ struct Bisector {
     // of it is queued or moved already, so it is never scanned again.
     std::vector<uint8_t> expanded(h.off.size() - 1, 0);
     auto touch_nbrs = [&](uint32_t v) {
-      for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
+      const uint64_t i0 = h.ioff[v], i1 = h.ioff[v + 1];
+      for (uint64_t i = i0; i < i1; ++i) {
+        if (i + 16 < i1) __builtin_prefetch(&expanded[h.inc[i + 16]]);
+        if (i + 8 < i1 && !expanded[h.inc[i + 8]]) __builtin_prefetch(h.pins.data() + rec[h.inc[i + 8]].off);
         uint32_t e = h.inc[i];
         if (expanded[e]) continue;
         expanded[e] = 1;
-        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+        const HRec& r = rec[e];
+        for (uint64_t j = r.off; j < (uint64_t)r.off + len(r); ++j) {
           uint32_t u = h.pins[j];
           if (side[u] == 1 && !inq[u]) { inq[u] = 1; heap[1].push(u, gain[u]); }
         }
@@ -403,6 +489,7 @@ struct Bisector {
       cur = UINT32_MAX;
     }
     heap[1].reset(h.n);
+    gains_exact = true;  // nothing was locked, so every gain was maintained
   }
 
   // One FM pass with rollback to the best prefix.  Returns the cut improvement.
@@ -411,15 +498,16 @@ struct Bisector {
     heap[0].reset(h.n);
     heap[1].reset(h.n);
     for (uint32_t v = 0; v < h.n; ++v) {
-      gain[v] = node_gain(v);
+      if (!gains_exact) gain[v] = node_gain(v);
       // boundary nodes only
       bool boundary = false;
       for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1] && !boundary; ++i) {
-        uint32_t e = h.inc[i];
-        boundary = cnt[0][e] && cnt[1][e];
+        const HRec& r = rec[h.inc[i]];
+        boundary = r.c[0] && r.c[1];
       }
       if (boundary) heap[side[v]].push(v, gain[v]);
     }
+    gains_exact = false;  // moves below skip locked nodes and the rollback maintains nothing
     std::vector<uint32_t> moved, fresh, parked[2];
     int64_t cum = 0, best = 0;
     size_t best_len = 0;
@@ -449,7 +537,7 @@ struct Bisector {
       fresh.clear();
       for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
         uint32_t e = h.inc[i];
-        if (cnt[to][e] == 0 && cnt[from][e] >= 2) fresh.push_back(e);
+        if (rec[e].c[to] == 0 && rec[e].c[from] >= 2) fresh.push_back(e);
       }
       move(v, locked);
       moved.push_back(v);
@@ -479,7 +567,9 @@ struct Bisector {
   }
 };
 
-// Restrict h to `nodes` (sorted global ids); hedges keep their pins inside the subset.
+// Restrict h to `nodes` (ascending ids): hedges keep their pins inside the subset, in
+// hedge order, and those left with fewer than two pins are dropped.  Local ids follow the
+// global order, so the pins of every hedge stay sorted.  One sequential sweep over the pins.
 inline Hypergraph subgraph(const Hypergraph& h, const std::vector<uint32_t>& nodes) {
   Hypergraph s;
   s.n = (uint32_t)nodes.size();
@@ -488,21 +578,22 @@ inline Hypergraph subgraph(const Hypergraph& h, const std::vector<uint32_t>& nod
   s.vw.resize(s.n);
   for (uint32_t i = 0; i < s.n; ++i) s.vw[i] = h.vw[nodes[i]];
   s.off.push_back(0);
-  std::vector<uint8_t> seen(h.off.size() - 1, 0);
-  for (uint32_t gi : nodes) {
-    for (uint64_t k = h.ioff[gi]; k < h.ioff[gi + 1]; ++k) {
-      uint32_t e = h.inc[k];
-      if (seen[e]) continue;
-      seen[e] = 1;
-      uint64_t start = s.pins.size();
-      for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j)
-        if (local[h.pins[j]] != UINT32_MAX) s.pins.push_back(local[h.pins[j]]);
-      if (s.pins.size() - start < 2) { s.pins.resize(start); continue; }
-      std::sort(s.pins.begin() + start, s.pins.end());
-      s.off.push_back(s.pins.size());
-      s.w.push_back(h.w[e]);
+  s.pins.reserve(h.pins.size() / 2 + 16);
+  const uint32_t m = (uint32_t)h.off.size() - 1;
+  for (uint32_t e = 0; e < m; ++e) {
+    const uint64_t start = s.pins.size();
+    for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+      const uint32_t x = local[h.pins[j]];
+      if (x != UINT32_MAX) s.pins.push_back(x);
     }
+    if (s.pins.size() - start < 2) {
+      s.pins.resize(start);
+      continue;
+    }
+    s.off.push_back(s.pins.size());
+    s.w.push_back(h.w[e]);
   }
+  s.pins.shrink_to_fit();
   build_incidence(s);
   return s;
 }
@@ -531,16 +622,26 @@ struct TryResult {
 // One bisection attempt: greedy growing from a seed, then FM passes.
 inline void bisect_try(const Hypergraph& h, uint32_t seed, uint64_t target0, const uint64_t maxw[2],
                        const Options& opt, TryResult& out) {
-  Bisector b(h);
-  b.grow(seed, target0, maxw[0]);
-  // FM passes until one gains nothing or less than 1/1000 of the cut (after the second)
-  for (uint32_t p = 0; p < opt.fm_passes; ++p) {
-    const int64_t g = b.fm_pass(maxw);
-    if (p > 0 && (g <= 0 || g < b.cut() / 1000)) break;
+  auto run = [&](auto& b) {
+    b.grow(seed, target0, maxw[0]);
+    // FM passes until one gains nothing or less than 1/1000 of the cut (after the second)
+    for (uint32_t p = 0; p < opt.fm_passes; ++p) {
+      const int64_t g = b.fm_pass(maxw);
+      if (p > 0 && (g <= 0 || g < b.cut() / 1000)) break;
+    }
+    out.cut = b.cut();
+    out.imb = std::max(b.wside[0] > maxw[0] ? b.wside[0] - maxw[0] : 0, b.wside[1] > maxw[1] ? b.wside[1] - maxw[1] : 0);
+    out.side = std::move(b.side);
+  };
+  bool narrow = h.pins.size() < (1ull << 32);
+  for (uint64_t x : h.w) narrow = narrow && x < (1ull << 32);
+  if (narrow) {
+    BisectorT<uint32_t> b(h);
+    run(b);
+  } else {
+    BisectorT<uint64_t> b(h);
+    run(b);
   }
-  out.cut = b.cut();
-  out.imb = std::max(b.wside[0] > maxw[0] ? b.wside[0] - maxw[0] : 0, b.wside[1] > maxw[1] ? b.wside[1] - maxw[1] : 0);
-  out.side = std::move(b.side);
 }
 
 // Recursive bisection.  The tries of one bisection run on their own threads and the two
