@@ -31,6 +31,7 @@ constexpr uint32_t CAND_BIG_SLOTS = 16384;  // LDS hash slots of a big node (128
 constexpr uint32_t CAND_DIRECT_N = 39168;   // levels up to this size index LDS by node id (<= 160 KB;
                                             // half that with inbound counts)
 constexpr uint64_t HEAVY_TABLE_SLOTS = 64ull << 20;  // HBM table slots shared by the heavy grid
+constexpr uint32_t HEAVY_TRY_SLOTS = 65536;          // first-attempt HBM table of a heavy node
 
 // Pins visited when scanning a node's touching hedges (itself excluded once per hedge).
 __global__ void k_visit_bound(const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ tch,
@@ -225,11 +226,16 @@ __global__ void k_split4(const uint32_t* __restrict__ fa, const uint32_t* __rest
   else ld[v - pa[v] - pb[v] - pc[v]] = v;
 }
 
+// Largest table among the listed nodes, each capped at `try_cap` unless it indexes directly
+// by node id (cap == n).
 __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ cap, uint32_t nh,
-                                uint32_t* out) {
+                                uint32_t n, uint32_t try_cap, uint32_t* out) {
   uint32_t m = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += gridDim.x * blockDim.x)
-    m = cap[heavy[i]] > m ? cap[heavy[i]] : m;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += gridDim.x * blockDim.x) {
+    const uint32_t c = cap[heavy[i]];
+    const uint32_t r = c == n ? c : (c < try_cap ? c : try_cap);
+    m = r > m ? r : m;
+  }
   m = wave_max(m);
   if (lane_id() == 0) atomicMax(out, m);
 }
@@ -241,10 +247,9 @@ __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32
 // inbound-union filter needs; without it (Delta >= twice the hedge count) they are 0.
 //
 // OPT (optimistic): the node's distinct neighbours may exceed the table.  New keys are
-// counted; past `limit` (5/8 of the table) the workgroup stops, appends the node to `ovf`
-// and leaves it for the HBM-table kernel.  At most one insert per thread is in flight when
+// counted; past `limit` the workgroup stops and the caller hands the node to a larger table.  At most one insert per thread is in flight when
 // the limit trips, so the table never fills.  Returns false when the node overflowed.
-template <bool GMEM, bool INTER, int NW, bool OPT = false>
+template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true>
 __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
                                                 uint32_t* inter, const uint32_t* __restrict__ hedges,
                                                 const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ srcc,
@@ -284,7 +289,8 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
       atomicAdd(&score[pv.pin], contrib);
       if (INTER && shared_in) atomicAdd(&inter[pv.pin], 1u);
     } else {
-      uint32_t h = slot_hash(pv.pin) & (cap - 1);
+      // P2: power-of-two table, masked index; otherwise any size, multiply-high index
+      uint32_t h = P2 ? slot_hash(pv.pin) & (cap - 1) : __umulhi(slot_hash(pv.pin), cap);
       for (;;) {
         const uint32_t old = atomicCAS(&keys[h], NONE, pv.pin);
         if (old == NONE) {
@@ -292,7 +298,7 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
           break;
         }
         if (old == pv.pin) break;
-        h = (h + 1) & (cap - 1);
+        h = P2 ? (h + 1) & (cap - 1) : (h + 1 == cap ? 0u : h + 1);
       }
       atomicAdd(&score[h], contrib);
       if (INTER && shared_in) atomicAdd(&inter[h], 1u);
@@ -351,22 +357,28 @@ __global__ void __launch_bounds__(1024) k_cand_big(CAND_ARGS) {
                                     w, sizes, cnc, omega, delta, pairs, scores, s_w);
 }
 
-// Nodes whose worst-case table exceeds LDS: try the LDS table anyway (their distinct
-// neighbours are usually far fewer than their visit bound); overflowing nodes are listed in
-// `ovf` for the HBM-table kernel.
+// Nodes whose worst-case table exceeds LDS: try the largest LDS table that fits (their
+// distinct neighbours are usually far fewer than their visit bound); overflowing nodes are
+// listed in `ovf` for the HBM-table kernel.  The table is not a power of two (it fills the
+// 160 KB of LDS), and the attempt stops at 3/4 load: with at most one insert per thread in
+// flight after that, LIMIT + 1024 < SLOTS keeps a free slot for every probe.
+constexpr uint32_t OPT_SLOTS_INTER = 13440;  // 12 bytes per slot
+constexpr uint32_t OPT_SLOTS = 20160;        // 8 bytes per slot
 template <bool INTER>
 __global__ void __launch_bounds__(1024) k_cand_opt(CAND_ARGS, uint32_t* __restrict__ ovf,
                                                    uint32_t* __restrict__ ovf_count) {
-  constexpr uint32_t SLOTS = INTER ? CAND_BIG_SLOTS / 2 : CAND_BIG_SLOTS;
+  constexpr uint32_t SLOTS = INTER ? OPT_SLOTS_INTER : OPT_SLOTS;
+  constexpr uint32_t LIMIT = SLOTS / 4 * 3;
+  static_assert(LIMIT + 1024 < SLOTS, "table must keep a free slot for every in-flight insert");
   __shared__ uint32_t s_keys[SLOTS];
   __shared__ uint32_t s_score[SLOTS];
   __shared__ uint32_t s_inter[INTER ? SLOTS : 1];
   __shared__ uint64_t s_w[256];
   __shared__ uint32_t s_state[2];
   const uint32_t u = list[blockIdx.x];
-  const bool done = cand_block_node<false, INTER, 16, true>(u, SLOTS, false, s_keys, s_score, s_inter, hedges, hoff,
-                                                            srcc, tch, toff, inb, w, sizes, cnc, omega, delta, pairs,
-                                                            scores, s_w, s_state, SLOTS / 8 * 5);
+  const bool done = cand_block_node<false, INTER, 16, true, false>(u, SLOTS, false, s_keys, s_score, s_inter, hedges,
+                                                                   hoff, srcc, tch, toff, inb, w, sizes, cnc, omega,
+                                                                   delta, pairs, scores, s_w, s_state, LIMIT);
   if (!done && threadIdx.x == 0) ovf[atomicAdd(ovf_count, 1u)] = u;
 }
 
@@ -385,19 +397,33 @@ __global__ void __launch_bounds__(1024) k_cand_direct(CAND_ARGS) {
 }
 
 // Heavy nodes: a persistent grid, each workgroup owning `region` slots of HBM tables and
-// taking nodes blockIdx.x, blockIdx.x + gridDim.x, ...
+// taking nodes blockIdx.x, blockIdx.x + gridDim.x, ...  A table costs its size to clear and
+// to rank, and the worst-case size (from the visit bound) can be far above the distinct
+// neighbours, so nodes whose worst case exceeds `try_cap` first try a table of try_cap
+// slots (optimistically, stopping at 3/4 load) and only the ones that overflow it go to a
+// second launch (try_cap = 0) with their worst-case table.
 __global__ void __launch_bounds__(256) k_cand_heavy(CAND_ARGS, uint32_t nlist, uint32_t n, uint64_t region,
                                                     uint32_t* __restrict__ gkeys, uint32_t* __restrict__ gscore,
-                                                    uint32_t* __restrict__ ginter) {
+                                                    uint32_t* __restrict__ ginter, uint32_t try_cap,
+                                                    uint32_t* __restrict__ ovf, uint32_t* __restrict__ ovf_count) {
   __shared__ uint64_t s_w[64];
+  __shared__ uint32_t s_state[2];
   uint32_t* keys = gkeys + region * blockIdx.x;
   uint32_t* score = gscore + region * blockIdx.x;
   uint32_t* inter = ginter + region * blockIdx.x;
   for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
     const uint32_t u = list[i];
     const uint32_t cap = caps[u];
-    cand_block_node<true, true, 4>(u, cap, cap == n, keys, score, inter, hedges, hoff, srcc, tch, toff, inb, w, sizes,
-                                   cnc, omega, delta, pairs, scores, s_w);
+    if (try_cap == 0 || cap == n || cap <= try_cap) {
+      cand_block_node<true, true, 4>(u, cap, cap == n, keys, score, inter, hedges, hoff, srcc, tch, toff, inb, w,
+                                     sizes, cnc, omega, delta, pairs, scores, s_w);
+    } else {
+      const bool done = cand_block_node<true, true, 4, true>(u, try_cap, false, keys, score, inter, hedges, hoff, srcc,
+                                                             tch, toff, inb, w, sizes, cnc, omega, delta, pairs,
+                                                             scores, s_w, s_state, try_cap / 4 * 3);
+      if (!done && threadIdx.x == 0) ovf[atomicAdd(ovf_count, 1u)] = u;
+      __syncthreads();  // s_state is reused by the next node
+    }
   }
 }
 
@@ -626,15 +652,27 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
                cnc, omega, delta, pairs, scores, ovf.get(), novf.get());
   const uint32_t no = read_scalar(c, novf.get());
   if (no == 0) return;
-  DBuf<uint32_t> mc(c, 1);
-  memset_async(c, mc.get(), 0, 4);
-  HGP_LAUNCH(k_heavy_max_cap, std::min<uint32_t>(ceil_div(no, 256), 1024), 256, 0, c, ovf.get(), caps.get(), no,
-             mc.get());
-  const uint64_t region = read_scalar(c, mc.get());
-  const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)no, HEAVY_TABLE_SLOTS / region, 4096}));
-  DBuf<uint32_t> gk(c, grid * region), gs(c, grid * region), gi(c, grid * region);
-  HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, ovf.get(),
-             caps.get(), cnc, omega, delta, pairs, scores, no, n, region, gk.get(), gs.get(), gi.get());
+  // HBM tables: first with tables capped at HEAVY_TRY_SLOTS, then the overflow with full ones
+  DBuf<uint32_t> ovf2(c, no), novf2(c, 1), mc(c, 1);
+  const uint32_t* lst = ovf.get();
+  uint32_t cntl = no;
+  for (int pass = 0; pass < 2 && cntl; ++pass) {
+    const uint32_t try_cap = pass == 0 ? HEAVY_TRY_SLOTS : 0u;
+    memset_async(c, mc.get(), 0, 4);
+    memset_async(c, novf2.get(), 0, 4);
+    HGP_LAUNCH(k_heavy_max_cap, std::min<uint32_t>(ceil_div(cntl, 256), 1024), 256, 0, c, lst, caps.get(), cntl, n,
+               try_cap ? try_cap : 0xFFFFFFFFu, mc.get());
+    const uint64_t region = read_scalar(c, mc.get());
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)cntl, HEAVY_TABLE_SLOTS / region, 4096}));
+    DBuf<uint32_t> gk(c, grid * region), gs(c, grid * region), gi(c, grid * region);
+    HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, lst, caps.get(),
+               cnc, omega, delta, pairs, scores, cntl, n, region, gk.get(), gs.get(), gi.get(), try_cap, ovf2.get(),
+               novf2.get());
+    if (pass == 0) {
+      cntl = read_scalar(c, novf2.get());
+      lst = ovf2.get();
+    }
+  }
 }
 
 void group_nodes(Ctx& c, const uint32_t* inb, const uint32_t* pairs, const uint32_t* scores, const uint32_t* sizes,

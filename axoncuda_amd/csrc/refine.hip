@@ -109,18 +109,36 @@ __global__ void __launch_bounds__(256) k_pp_small(const uint32_t* __restrict__ h
   }
 }
 
-template <bool INB>
+// Table slots for a hedge of `len` pins spread over at most `k` partitions: a power of two
+// of at least twice the distinct partitions (+64), so open addressing never fills.
+__host__ __device__ __forceinline__ uint32_t pp_table_cap(uint64_t len, uint32_t k) {
+  const uint64_t d = len < k ? len : k;
+  uint32_t cap = 128;
+  while (cap < 2 * d + 64) cap <<= 1;
+  return cap;
+}
+
+// One wave per hedge with more than 16 pins.  GMEM = false: the (partition, count, dst
+// count) table lives in LDS with `cap` slots; GMEM = true (hedges whose table would not fit
+// in LDS): in HBM at gtab + 3 * goff[i], sized by pp_table_cap.
+template <bool INB, bool GMEM>
 __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
                                                  const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part,
                                                  const float* __restrict__ w,
-                                                 const uint32_t* __restrict__ list, uint32_t cap, uint2* __restrict__ pp,
-                                                 uint32_t* __restrict__ pp_dcnt, uint32_t* __restrict__ lambda,
-                                                 uint4* __restrict__ hinfo, uint32_t* inbsz) {
+                                                 const uint32_t* __restrict__ list, uint32_t cap, uint32_t k,
+                                                 const uint64_t* __restrict__ goff, uint32_t* __restrict__ gtab,
+                                                 uint2* __restrict__ pp, uint32_t* __restrict__ pp_dcnt,
+                                                 uint32_t* __restrict__ lambda, uint4* __restrict__ hinfo,
+                                                 uint32_t* inbsz) {
   extern __shared__ uint32_t lds[];
-  uint32_t* keys = lds;
-  uint32_t* cnt = lds + cap;
-  uint32_t* dcnt = lds + 2 * cap;
   const uint32_t e = list[blockIdx.x];
+  uint32_t* keys = lds;
+  if (GMEM) {
+    cap = pp_table_cap(hoff[e + 1] - hoff[e], k);
+    keys = gtab + 3 * goff[blockIdx.x];
+  }
+  uint32_t* cnt = keys + cap;
+  uint32_t* dcnt = keys + 2 * cap;
   const uint32_t l = lane_id();
   for (uint32_t i = l; i < cap; i += 64) {
     keys[i] = NONE;
@@ -142,17 +160,21 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
     if (INB && (i - s) >= sc) atomicAdd(&dcnt[h], 1u);
   }
   __syncthreads();
+  // HBM tables were written by atomics at the device cache: read them there too
+  auto ld = [](uint32_t* x) { return GMEM ? atomic_load_relaxed(x) : *x; };
   uint32_t base = 0;
   for (uint32_t b = 0; b < cap; b += 64) {
     const uint32_t i = b + l;
-    const bool has = keys[i] != NONE;
+    const uint32_t q = ld(&keys[i]);
+    const bool has = q != NONE;
     const uint64_t m = __ballot(has);
     if (has) {
       const uint32_t o = base + lanes_below(m);
-      pp[s + o] = make_uint2(keys[i], cnt[i]);
+      pp[s + o] = make_uint2(q, ld(&cnt[i]));
       if (INB) {
-        pp_dcnt[s + o] = dcnt[i];
-        if (dcnt[i]) atomicAdd(&inbsz[keys[i]], 1u);
+        const uint32_t dc = ld(&dcnt[i]);
+        pp_dcnt[s + o] = dc;
+        if (dc) atomicAdd(&inbsz[q], 1u);
       }
     }
     base += (uint32_t)__popcll(m);
@@ -161,6 +183,30 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
     lambda[e] = base;
     hinfo[e] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), base, w ? fixed_weight(w[e]) : 0u);
   }
+}
+
+__global__ void k_flag_longer(const uint32_t* __restrict__ list, uint32_t m, const uint64_t* __restrict__ hoff,
+                              uint64_t len, uint32_t* __restrict__ flag) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t e = list[i];
+  flag[i] = hoff[e + 1] - hoff[e] > len ? 1u : 0u;
+}
+// stable split of list entries: flagged ones to a[], the rest to b[]
+__global__ void k_split_list(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos,
+                             const uint32_t* __restrict__ list, uint32_t m, uint32_t* __restrict__ a,
+                             uint32_t* __restrict__ b) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  if (flag[i]) a[pos[i]] = list[i];
+  else b[i - pos[i]] = list[i];
+}
+__global__ void k_pp_gcap(const uint32_t* __restrict__ list, uint32_t m, const uint64_t* __restrict__ hoff, uint32_t k,
+                          uint64_t* __restrict__ cap) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t e = list[i];
+  cap[i] = pp_table_cap(hoff[e + 1] - hoff[e], k);
 }
 
 // ---- 2. gains ---------------------------------------------------------------------------
@@ -496,6 +542,150 @@ __global__ void __launch_bounds__(256) k_gains_heavy(GainArgs a, const uint32_t*
       a.moves[v] = q;
       a.gains[v] = q == NONE ? 0 : g;
     }
+  }
+}
+
+// Nodes above the light tier when k > GAIN_LIGHT: one 1024-thread workgroup per node with a
+// (partition -> conn) table filling the LDS.  DIRECT (k <= GB_DIRECT): indexed by partition.
+// Hashed: GB_SLOTS slots (not a power of two: multiply-high index).  A new partition is
+// inserted only while fewer than GB_LIMIT are present; a node that meets one more sets the
+// overflow flag and is listed in `ovf` for the HBM-table kernel.  At most one insert per
+// thread can pass the check concurrently, so GB_LIMIT + 1024 < GB_SLOTS keeps a free slot
+// for every probe.
+constexpr uint32_t GB_DIRECT = 20160;  // 8 bytes per slot
+constexpr uint32_t GB_SLOTS = 13440;   // 12 bytes per slot
+constexpr uint32_t GB_LIMIT = GB_SLOTS / 4 * 3;
+static_assert(GB_LIMIT + 1024 < GB_SLOTS, "the table must keep a free slot for every in-flight insert");
+
+__device__ __forceinline__ uint32_t gb_find(const uint32_t* keys, uint32_t q) {
+  uint32_t h = __umulhi(slot_hash(q), GB_SLOTS);
+  for (;;) {
+    const uint32_t k = keys[h];
+    if (k == q) return h;
+    if (k == NONE) return NONE;
+    h = h + 1 == GB_SLOTS ? 0u : h + 1;
+  }
+}
+
+template <bool DIRECT>
+__global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t* __restrict__ list,
+                                                      uint32_t* __restrict__ ovf, uint32_t* __restrict__ novf) {
+  constexpr uint32_t SLOTS = DIRECT ? GB_DIRECT : GB_SLOTS;
+  __shared__ int64_t s_conn[SLOTS];
+  __shared__ uint32_t s_keys[DIRECT ? 1 : SLOTS];
+  __shared__ int64_t s_sum[2][16];
+  __shared__ int64_t s_bg[16];
+  __shared__ uint32_t s_bh[16], s_bq[16];
+  __shared__ uint32_t s_state[2];  // partitions inserted, overflow flag
+  __shared__ uint32_t s_first;
+  const uint32_t v = list[blockIdx.x];
+  const uint32_t cap = DIRECT ? a.k : SLOTS;
+  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
+    s_conn[i] = 0;
+    if (!DIRECT) s_keys[i] = NONE;
+  }
+  if (threadIdx.x == 0) {
+    s_state[0] = 0;
+    s_state[1] = 0;
+    s_first = UINT32_MAX;
+  }
+  __syncthreads();
+  const uint32_t wv = threadIdx.x / 64, l = lane_id();
+  const uint32_t p = a.part[v], s = a.sizes[v];
+  const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
+  const uint64_t cnt = t1 - t0;
+  const uint64_t lo = t0 + cnt * wv / 16, hi = t0 + cnt * (wv + 1) / 16;
+  int64_t saving = 0, Wv = 0;
+  Wv += wave_visit_hedge_lists(a.tch, lo, hi, a.hinfo, [&](const EntryVisit& ev) {
+    if (!ev.act) return;
+    const uint2 pe = a.pp[ev.abs];
+    const uint32_t q = pe.x;
+    const int64_t We = (int64_t)ev.w;
+    if (q == p) {
+      const uint32_t c = pe.y;
+      if (a.incc) saving += (int64_t)((uint64_t)We / ((uint64_t)c * c));
+      else if (c == 1) saving += We;
+      return;
+    }
+    uint32_t h = q;
+    if (!DIRECT) {
+      h = __umulhi(slot_hash(q), GB_SLOTS);
+      for (;;) {
+        const uint32_t k = atomic_load_relaxed(&s_keys[h]);
+        if (k == q) break;
+        if (k == NONE) {
+          if (atomic_load_relaxed(&s_state[0]) >= GB_LIMIT) {
+            atomicOr(&s_state[1], 1u);  // table full enough: the node goes to the HBM kernel
+            return;
+          }
+          const uint32_t old = atomicCAS(&s_keys[h], NONE, q);
+          if (old == NONE) {
+            atomicAdd(&s_state[0], 1u);
+            break;
+          }
+          if (old == q) break;
+        }
+        h = h + 1 == GB_SLOTS ? 0u : h + 1;
+      }
+    }
+    atomicAdd((unsigned long long*)&s_conn[h], (unsigned long long)We);
+  });
+  saving = wave_sum(saving);
+  Wv = wave_sum(Wv);
+  if (l == 0) {
+    s_sum[0][wv] = saving;
+    s_sum[1][wv] = Wv;
+  }
+  __syncthreads();
+  if (!DIRECT && s_state[1]) {
+    if (threadIdx.x == 0) ovf[atomicAdd(novf, 1u)] = v;
+    return;
+  }
+  int64_t base = 0;
+  for (int i = 0; i < 16; ++i) base += s_sum[0][i] - s_sum[1][i];
+  int64_t bg;
+  uint32_t bh, bq;
+  best_adjacent(a, DIRECT ? nullptr : s_keys, s_conn, cap, threadIdx.x, blockDim.x, p, s, base, bg, bh, bq);
+  wave_best(bg, bh, bq);
+  if (l == 0) {
+    s_bg[wv] = bg;
+    s_bh[wv] = bh;
+    s_bq[wv] = bq;
+  }
+  __syncthreads();
+  int64_t g = INT64_MIN;
+  uint32_t h = 0, q = NONE;
+  for (int i = 0; i < 16; ++i) {
+    if (s_bq[i] == NONE) continue;
+    if (q == NONE || s_bg[i] > g || (s_bg[i] == g && s_bh[i] > h)) {
+      g = s_bg[i];
+      h = s_bh[i];
+      q = s_bq[i];
+    }
+  }
+  if (q == NONE) {
+    // non-adjacent fallback: the first valid target in hash order without a positive
+    // connection, searched by the whole workgroup 1024 partitions at a time
+    for (uint32_t b = 0; b < a.k; b += blockDim.x) {
+      const uint32_t i = b + threadIdx.x;
+      if (i < a.k) {
+        const uint32_t t = a.order[i];
+        if (target_ok(a, t, p, s)) {
+          const uint32_t slot = DIRECT ? t : gb_find(s_keys, t);
+          if (slot == NONE || s_conn[slot] <= 0) atomicMin(&s_first, i);
+        }
+      }
+      __syncthreads();
+      const bool found = s_first != UINT32_MAX;
+      __syncthreads();  // every thread has read s_first before the next round may lower it
+      if (found) break;
+    }
+    if (s_first != UINT32_MAX) q = a.order[s_first];
+    g = base;
+  }
+  if (threadIdx.x == 0) {
+    a.moves[v] = q;
+    a.gains[v] = q == NONE ? 0 : g;
   }
 }
 
@@ -1245,21 +1435,52 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
   }
   DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
   exclusive_scan(c, lflag.get(), pos.get(), L.e);
-  const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.e);
+  uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.e);
   if (nl == 0) return;
   DBuf<uint32_t> large(c, nl), small(c, L.e - nl);
   // k_split puts flagged (large) entries first list
   HGP_LAUNCH(k_split, ceil_div(L.e, 256), 256, 0, c, lflag.get(), pos.get(), L.e, large.get(), small.get());
-  // distinct partitions per hedge <= k: a table of 2k slots (power of two) never fills
-  HGP_REQUIRE(L.k <= PP_LDS_CAP / 2, HGP_E_CAPACITY, "partition lists: more than 2048 partitions are not supported");
-  uint32_t cap = 128;
-  while (cap < 2ull * L.k + 64 && cap < PP_LDS_CAP) cap <<= 1;
-  if (want_inbound)
-    HGP_LAUNCH((k_pp_large<true>), nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, L.w,
-               large.get(), cap, pl.pp.get(), pl.dcnt.get(), pl.lambda.get(), pl.hinfo.get(), pl.inbsz.get());
-  else
-    HGP_LAUNCH((k_pp_large<false>), nl, 64, 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part, L.w,
-               large.get(), cap, pl.pp.get(), (uint32_t*)nullptr, pl.lambda.get(), pl.hinfo.get(), (uint32_t*)nullptr);
+  // Distinct partitions per hedge <= min(k, len).  Tables of up to PP_LDS_CAP slots sit in
+  // LDS; hedges needing more (len and k both above ~2K) get HBM tables.
+  const uint32_t lds_max_len = (PP_LDS_CAP - 64) / 2;
+  const uint32_t cap = pp_table_cap(lds_max_len, L.k);  // <= PP_LDS_CAP
+  DBuf<uint32_t> glist;
+  uint32_t ng = 0;
+  if (L.k > lds_max_len) {
+    DBuf<uint32_t> gflag(c, nl);
+    DBuf<uint64_t> gpos(c, (uint64_t)nl + 1);
+    HGP_LAUNCH(k_flag_longer, ceil_div(nl, 256), 256, 0, c, large.get(), nl, L.hoff, (uint64_t)lds_max_len, gflag.get());
+    exclusive_scan(c, gflag.get(), gpos.get(), nl);
+    ng = (uint32_t)read_scalar(c, gpos.get() + nl);
+    if (ng) {
+      DBuf<uint32_t> lds_list(c, nl - ng);
+      glist = DBuf<uint32_t>(c, ng);
+      HGP_LAUNCH(k_split_list, ceil_div(nl, 256), 256, 0, c, gflag.get(), gpos.get(), large.get(), nl, glist.get(),
+                 lds_list.get());
+      large = std::move(lds_list);
+      nl -= ng;
+    }
+  }
+  auto launch = [&](auto inb, auto gmem, const uint32_t* lst, uint32_t m, const uint64_t* goff, uint32_t* gtab) {
+    constexpr bool INB = decltype(inb)::value, GMEM = decltype(gmem)::value;
+    if (m == 0) return;
+    HGP_LAUNCH((k_pp_large<INB, GMEM>), m, 64, GMEM ? 0 : 3 * cap * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part,
+               L.w, lst, cap, L.k, goff, gtab, pl.pp.get(), INB ? pl.dcnt.get() : (uint32_t*)nullptr, pl.lambda.get(),
+               pl.hinfo.get(), INB ? pl.inbsz.get() : (uint32_t*)nullptr);
+  };
+  using T = std::true_type;
+  using F = std::false_type;
+  if (want_inbound) launch(T{}, F{}, large.get(), nl, nullptr, nullptr);
+  else launch(F{}, F{}, large.get(), nl, nullptr, nullptr);
+  if (ng) {
+    DBuf<uint64_t> gcap(c, ng), goff(c, (uint64_t)ng + 1);
+    HGP_LAUNCH(k_pp_gcap, ceil_div(ng, 256), 256, 0, c, glist.get(), ng, L.hoff, L.k, gcap.get());
+    exclusive_scan(c, gcap.get(), goff.get(), ng);
+    const uint64_t slots = read_scalar(c, goff.get() + ng);
+    DBuf<uint32_t> gtab(c, 3 * slots);
+    if (want_inbound) launch(T{}, T{}, glist.get(), ng, goff.get(), gtab.get());
+    else launch(F{}, T{}, glist.get(), ng, goff.get(), gtab.get());
+  }
 }
 
 uint32_t discount_of(uint32_t rep, uint32_t R) { return rep < R / 3 ? 1u : (rep < 2 * R / 3 ? 2u : NONE); }
@@ -1332,7 +1553,7 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   exclusive_scan(c, light.get(), pos.get(), L.n);
   const uint32_t nt = (uint32_t)read_scalar(c, tpos.get() + L.n);
   const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.n);
-  const uint32_t nh = L.n - nt - nl;
+  uint32_t nh = L.n - nt - nl;
   DBuf<uint32_t> tl(c, nt), ll(c, nl), hl(c, nh);
   HGP_LAUNCH(k_split3, ceil_div(L.n, 256), 256, 0, c, tiny.get(), light.get(), tpos.get(), pos.get(), L.n, tl.get(),
              ll.get(), hl.get());
@@ -1341,8 +1562,21 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   const uint32_t capmax = L.k <= GAIN_CAP ? std::max<uint32_t>(L.k, 1) : GAIN_CAP;
   HGP_LAUNCH_B(gain_bytes(c, ll.get(), nl, L.toff, bound.get()), k_gains_light, ceil_div(nl, 4), 256, 4 * capmax * 12,
                c, a, ll.get(), nl, bound.get(), capmax);
-  sync(c);  // `order` host buffer
+  if (nh == 0) {
+    sync(c);  // `order` host buffer
+    return;
+  }
+  // workgroup per node with an LDS table; hashed tables that would overflow go to HBM
+  DBuf<uint32_t> ovf(c, nh), novf(c, 1);
+  memset_async(c, novf.get(), 0, 4);
+  if (L.k <= GB_DIRECT) {
+    HGP_LAUNCH((k_gains_block<true>), nh, 1024, 0, c, a, hl.get(), ovf.get(), novf.get());
+  } else {
+    HGP_LAUNCH((k_gains_block<false>), nh, 1024, 0, c, a, hl.get(), ovf.get(), novf.get());
+  }
+  nh = (uint32_t)read_scalar(c, novf.get());  // also completes the `order` upload
   if (nh == 0) return;
+  hl = std::move(ovf);
   DBuf<uint64_t> caps(c, nh);
   HGP_LAUNCH(k_heavy_tabcap, ceil_div(nh, 256), 256, 0, c, hl.get(), bound.get(), nh, L.k, caps.get());
   std::vector<uint64_t> hc(nh);

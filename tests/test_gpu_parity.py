@@ -101,6 +101,50 @@ def test_candidates_table_tiers(dev, n, big, deltas, rng):
         assert np.array_equal(scores, es), f"scores differ (delta={delta})"
 
 
+def hub_graph(n, hubs, n_small, rng):
+    """Hub nodes, each on `count` hedges of `size` pins drawn from a pool of `pool` nodes, so
+    their visit bounds (and worst-case tables) are far above their distinct neighbours."""
+    off, pins, srcc = [0], [], []
+    for hub, count, size, pool in hubs:
+        for _ in range(count):
+            nodes = rng.choice(pool, size=size, replace=False)
+            nodes = nodes[nodes != hub][: size - 1]
+            pins.append(hub)
+            pins.extend(int(x) for x in nodes)
+            off.append(len(pins))
+            srcc.append(1)
+    for _ in range(n_small):
+        sz = int(rng.integers(2, 6))
+        pins.extend(int(x) for x in rng.choice(n, size=sz, replace=False))
+        off.append(len(pins))
+        srcc.append(int(rng.integers(0, 2)))
+    w = (rng.random(len(srcc)) * 3 + 0.5).astype(np.float32)
+    return O.dedup(O.HG(n, np.array(off, np.uint64), np.array(pins, np.uint32), np.array(srcc, np.uint32), w))
+
+
+def test_candidates_heavy_tables(dev, rng):
+    """Nodes whose LDS attempt overflows: one fits an HBM table sized by its visit bound
+    (<= 65536 slots), one fits the 65536-slot first attempt although its bound asks for
+    more, and one overflows that attempt too and is redone with its worst-case table."""
+    n = 120000
+    hubs = [(7, 75, 400, 30000),      # bound 30K: table 65536, ~19K distinct
+            (11, 100, 400, 30000),    # bound 40K: table 131072, ~22K distinct (< 3/4 of 65536)
+            (13, 200, 400, n)]        # bound 80K: table 262144, ~58K distinct (> 3/4 of 65536)
+    hg = hub_graph(n, hubs, 4000, rng)
+    L = G.Level(dev, hg)
+    L.build_touching()
+    ro, rt, ri = O.touching(hg)
+    nb = [len(np.unique(hg.pins[np.concatenate([np.arange(int(hg.off[e]), int(hg.off[e + 1])) for e in rt[ro[h]:ro[h + 1]]])]))
+          for h in (7, 11, 13)]
+    assert 15120 < nb[0] and 15120 < nb[1] < 49152 < nb[2], nb
+    sizes = rng.integers(1, 3, size=hg.n).astype(np.uint32)
+    for delta in (2**31 - 1, 30):
+        pairs, scores, *_ = G.candidates(dev, L, sizes, 16, 6, delta)
+        ep, es = O.candidates(hg, ro, rt, ri, sizes, 16, 6, delta)
+        assert np.array_equal(pairs, ep), f"pairs differ (delta={delta})"
+        assert np.array_equal(scores, es), f"scores differ (delta={delta})"
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("cnc", [1, 3, 16])
 def test_group_nodes(dev, case, cnc, rng):
@@ -210,6 +254,37 @@ def refine_setup(case_name, k, mode, rng):
 def test_refine_analyze(dev, rc, chainup, fm_repeat, rng):
     name, mode, k, case_name = rc
     hg, sizes, part, psizes, omega, delta = refine_setup(case_name, k, mode, rng)
+    L = G.Level(dev, hg)
+    L.build_touching()
+    ro, rt, ri = O.touching(hg)
+    got = G.refine_analyze(dev, L, sizes, k, mode, omega, delta, 16, fm_repeat, chainup, part, psizes)
+    exp = O.refine_analyze(mode, omega, delta, 16, hg, ro, rt, ri, sizes, k, fm_repeat, chainup, part, psizes)
+    for key in ("moves", "gains", "ranks", "seq", "sv", "iv"):
+        assert np.array_equal(got[key], exp[key]), f"{key} differs"
+    assert got["best"] == exp["best"]
+
+
+@pytest.mark.parametrize("mode,k,fm_repeat,chainup,n,e,max_size", [
+    (1, 4096, 0, False, 4500, 10, 5000),   # gains table in LDS indexed by partition
+    (0, 3000, 6, True, 4500, 10, 5000),
+    (1, 22000, 0, True, 24000, 8, 9000),   # hashed LDS gains tables, some overflowing to HBM
+    (0, 22000, 13, False, 24000, 8, 9000),
+])
+def test_refine_analyze_huge_lists(dev, mode, k, fm_repeat, chainup, n, e, max_size, rng):
+    """Hedges of thousands of pins over thousands of partitions: partition lists longer than
+    the LDS tables take the HBM-table path, and nodes adjacent to many partitions use the
+    workgroup gain kernel (LDS tables indexed or hashed, HBM tables past 3/4 load)."""
+    raw = O.HG(*gen.random_hypergraph(n, e, rng, max_size=max_size, src_prob=0.3, max_src=2, weights="unit"))
+    hg = O.dedup(raw)
+    sizes = rng.integers(1, 3, size=hg.n).astype(np.uint32)
+    part = rng.integers(0, k, size=hg.n).astype(np.uint32)
+    psizes = np.bincount(part, weights=sizes, minlength=k).astype(np.uint32)
+    lam = [len(np.unique(part[hg.pins[hg.off[e]:hg.off[e + 1]]])) for e in range(len(hg.off) - 1)]
+    assert max(lam) > 2016, "the case must reach the HBM-table path"
+    if mode == 1:
+        omega, delta = int(np.ceil(1.08 * sizes.sum() / k)), 2**31 - 1
+    else:
+        omega, delta = int(psizes.max()) + 3, int(O.inbound_sizes(hg, part, k).max()) + 2
     L = G.Level(dev, hg)
     L.build_touching()
     ro, rt, ri = O.touching(hg)
