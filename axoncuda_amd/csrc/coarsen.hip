@@ -14,6 +14,8 @@
 // cooperative tree walk with an exact maximum-weight matching of each round's pointer
 // forest, computed bottom-up by "last child continues" atomics (integer, order
 // independent) and decided top-down along holder chains; see the oracle for the rule.
+#include <cstdio>
+
 #include "prim.hpp"
 #include "stages.hpp"
 #include "wave_iter.hpp"
@@ -226,14 +228,13 @@ __global__ void k_split4(const uint32_t* __restrict__ fa, const uint32_t* __rest
   else ld[v - pa[v] - pb[v] - pc[v]] = v;
 }
 
-// Largest table among the listed nodes, each capped at `try_cap` unless it indexes directly
-// by node id (cap == n).
+// Largest table among the listed nodes, each capped at `try_cap`.
 __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ cap, uint32_t nh,
-                                uint32_t n, uint32_t try_cap, uint32_t* out) {
+                                uint32_t try_cap, uint32_t* out) {
   uint32_t m = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += gridDim.x * blockDim.x) {
     const uint32_t c = cap[heavy[i]];
-    const uint32_t r = c == n ? c : (c < try_cap ? c : try_cap);
+    const uint32_t r = c < try_cap ? c : try_cap;
     m = r > m ? r : m;
   }
   m = wave_max(m);
@@ -249,7 +250,7 @@ __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32
 // OPT (optimistic): the node's distinct neighbours may exceed the table.  New keys are
 // counted; past `limit` the workgroup stops and the caller hands the node to a larger table.  At most one insert per thread is in flight when
 // the limit trips, so the table never fills.  Returns false when the node overflowed.
-template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true>
+template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true, int CH = 1>
 __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
                                                 uint32_t* inter, const uint32_t* __restrict__ hedges,
                                                 const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ srcc,
@@ -260,67 +261,73 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
                                                 uint32_t* __restrict__ scores, uint64_t* s_w,
                                                 uint32_t* s_state = nullptr, uint32_t limit = 0) {
   const uint32_t nkeys = direct ? (cap + 31) / 32 : cap;
-  for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
-    score[s] = 0;
-    if (INTER) inter[s] = 0;
-  }
-  for (uint32_t s = threadIdx.x; s < nkeys; s += blockDim.x) keys[s] = direct ? 0u : NONE;
-  if (OPT && threadIdx.x == 0) {
-    s_state[0] = 0;  // new keys
-    s_state[1] = 0;  // overflow flag
-  }
-  if (GMEM) __threadfence();
-  __syncthreads();
   const uint64_t t0 = toff[u], t1 = toff[u + 1];
   const uint32_t inb_u = inb[u];
+  const uint32_t size_u = sizes[u];
   const uint32_t wv = threadIdx.x / 64;
   const uint64_t cnt = t1 - t0;
   const uint64_t a = t0 + cnt * wv / NW, b = t0 + cnt * (wv + 1) / NW;
   auto overflowed = [&] {
     return OPT && __hip_atomic_load(&s_state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
   };
-  wave_visit_pins_until(tch, a, b, hoff, hedges, [&](const PinVisit& pv) {
-    if (!pv.act || pv.pin == u) return;
-    const uint32_t contrib = fixed_weight(w[pv.e]) / pv.len;
-    const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
-    if (direct) {
-      const uint32_t bit = 1u << (pv.pin & 31u);
-      if (!(atomic_load_relaxed(&keys[pv.pin >> 5]) & bit)) atomicOr(&keys[pv.pin >> 5], bit);
-      atomicAdd(&score[pv.pin], contrib);
-      if (INTER && shared_in) atomicAdd(&inter[pv.pin], 1u);
-    } else {
-      // P2: power-of-two table, masked index; otherwise any size, multiply-high index
-      uint32_t h = P2 ? slot_hash(pv.pin) & (cap - 1) : __umulhi(slot_hash(pv.pin), cap);
-      for (;;) {
-        const uint32_t old = atomicCAS(&keys[h], NONE, pv.pin);
-        if (old == NONE) {
-          if (OPT && atomicAdd(&s_state[0], 1u) + 1u > limit) atomicOr(&s_state[1], 1u);
-          break;
-        }
-        if (old == pv.pin) break;
-        h = P2 ? (h + 1) & (cap - 1) : (h + 1 == cap ? 0u : h + 1);
-      }
-      atomicAdd(&score[h], contrib);
-      if (INTER && shared_in) atomicAdd(&inter[h], 1u);
-    }
-  }, overflowed);
-  if (GMEM) __threadfence();
-  __syncthreads();
-  if (OPT && s_state[1]) return false;
-  const uint32_t size_u = sizes[u];
   uint64_t top[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) top[i] = 0;
-  for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
-    uint32_t v;
-    if (direct) {
-      if (!((keys[s >> 5] >> (s & 31u)) & 1u)) continue;
-      v = s;
-    } else {
-      v = keys[s];
-      if (v == NONE) continue;
+  // CH > 1: the neighbours are split by id modulo CH and scored one class per pass, so the
+  // table holds a CH-th of them; the per-thread top lists carry over between passes.
+  for (int chunk = 0; chunk < CH; ++chunk) {
+    for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
+      score[s] = 0;
+      if (INTER) inter[s] = 0;
     }
-    top_insert(top, cand_key(u, v, score[s], INTER ? inter[s] : 0u, size_u, inb_u, sizes, inb, omega, delta));
+    for (uint32_t s = threadIdx.x; s < nkeys; s += blockDim.x) keys[s] = direct ? 0u : NONE;
+    if (OPT && threadIdx.x == 0) {
+      s_state[0] = 0;  // new keys
+      s_state[1] = 0;  // overflow flag
+    }
+    if (GMEM) __threadfence();
+    __syncthreads();
+    wave_visit_pins_until(tch, a, b, hoff, hedges, [&](const PinVisit& pv) {
+      if (!pv.act || pv.pin == u) return;
+      if (CH > 1 && pv.pin % CH != (uint32_t)chunk) return;
+      const uint32_t contrib = fixed_weight(w[pv.e]) / pv.len;
+      const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
+      if (direct) {
+        const uint32_t bit = 1u << (pv.pin & 31u);
+        if (!(atomic_load_relaxed(&keys[pv.pin >> 5]) & bit)) atomicOr(&keys[pv.pin >> 5], bit);
+        atomicAdd(&score[pv.pin], contrib);
+        if (INTER && shared_in) atomicAdd(&inter[pv.pin], 1u);
+      } else {
+        // P2: power-of-two table, masked index; otherwise any size, multiply-high index
+        uint32_t h = P2 ? slot_hash(pv.pin) & (cap - 1) : __umulhi(slot_hash(pv.pin), cap);
+        for (;;) {
+          const uint32_t old = atomicCAS(&keys[h], NONE, pv.pin);
+          if (old == NONE) {
+            if (OPT && atomicAdd(&s_state[0], 1u) + 1u > limit) atomicOr(&s_state[1], 1u);
+            break;
+          }
+          if (old == pv.pin) break;
+          h = P2 ? (h + 1) & (cap - 1) : (h + 1 == cap ? 0u : h + 1);
+        }
+        atomicAdd(&score[h], contrib);
+        if (INTER && shared_in) atomicAdd(&inter[h], 1u);
+      }
+    }, overflowed);
+    if (GMEM) __threadfence();
+    __syncthreads();
+    if (OPT && s_state[1]) return false;
+    for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
+      uint32_t v;
+      if (direct) {
+        if (!((keys[s >> 5] >> (s & 31u)) & 1u)) continue;
+        v = s;
+      } else {
+        v = keys[s];
+        if (v == NONE) continue;
+      }
+      top_insert(top, cand_key(u, v, score[s], INTER ? inter[s] : 0u, size_u, inb_u, sizes, inb, omega, delta));
+    }
+    if (CH > 1) __syncthreads();  // the next pass clears the table
   }
   block_top_write<NW>(top, cnc, s_w, u, pairs, scores);
   return true;
@@ -364,7 +371,10 @@ __global__ void __launch_bounds__(1024) k_cand_big(CAND_ARGS) {
 // flight after that, LIMIT + 1024 < SLOTS keeps a free slot for every probe.
 constexpr uint32_t OPT_SLOTS_INTER = 13440;  // 12 bytes per slot
 constexpr uint32_t OPT_SLOTS = 20160;        // 8 bytes per slot
-template <bool INTER>
+constexpr int OPT_CHUNKS = 4;                // passes of the second LDS attempt
+// CH > 1 scores the neighbours in CH passes over the touching hedges (one id class each),
+// for nodes that overflowed the single-pass table.
+template <bool INTER, int CH>
 __global__ void __launch_bounds__(1024) k_cand_opt(CAND_ARGS, uint32_t* __restrict__ ovf,
                                                    uint32_t* __restrict__ ovf_count) {
   constexpr uint32_t SLOTS = INTER ? OPT_SLOTS_INTER : OPT_SLOTS;
@@ -376,9 +386,10 @@ __global__ void __launch_bounds__(1024) k_cand_opt(CAND_ARGS, uint32_t* __restri
   __shared__ uint64_t s_w[256];
   __shared__ uint32_t s_state[2];
   const uint32_t u = list[blockIdx.x];
-  const bool done = cand_block_node<false, INTER, 16, true, false>(u, SLOTS, false, s_keys, s_score, s_inter, hedges,
-                                                                   hoff, srcc, tch, toff, inb, w, sizes, cnc, omega,
-                                                                   delta, pairs, scores, s_w, s_state, LIMIT);
+  const bool done = cand_block_node<false, INTER, 16, true, false, CH>(u, SLOTS, false, s_keys, s_score, s_inter,
+                                                                       hedges, hoff, srcc, tch, toff, inb, w, sizes,
+                                                                       cnc, omega, delta, pairs, scores, s_w, s_state,
+                                                                       LIMIT);
   if (!done && threadIdx.x == 0) ovf[atomicAdd(ovf_count, 1u)] = u;
 }
 
@@ -401,7 +412,8 @@ __global__ void __launch_bounds__(1024) k_cand_direct(CAND_ARGS) {
 // to rank, and the worst-case size (from the visit bound) can be far above the distinct
 // neighbours, so nodes whose worst case exceeds `try_cap` first try a table of try_cap
 // slots (optimistically, stopping at 3/4 load) and only the ones that overflow it go to a
-// second launch (try_cap = 0) with their worst-case table.
+// second launch (try_cap = 0) with their worst-case table.  Nodes indexed by node id
+// (cap == n) also wait for the second launch, so that they do not size every region.
 __global__ void __launch_bounds__(256) k_cand_heavy(CAND_ARGS, uint32_t nlist, uint32_t n, uint64_t region,
                                                     uint32_t* __restrict__ gkeys, uint32_t* __restrict__ gscore,
                                                     uint32_t* __restrict__ ginter, uint32_t try_cap,
@@ -414,9 +426,11 @@ __global__ void __launch_bounds__(256) k_cand_heavy(CAND_ARGS, uint32_t nlist, u
   for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
     const uint32_t u = list[i];
     const uint32_t cap = caps[u];
-    if (try_cap == 0 || cap == n || cap <= try_cap) {
+    if (try_cap == 0 || cap <= try_cap) {
       cand_block_node<true, true, 4>(u, cap, cap == n, keys, score, inter, hedges, hoff, srcc, tch, toff, inb, w,
                                      sizes, cnc, omega, delta, pairs, scores, s_w);
+    } else if (cap == n) {
+      if (threadIdx.x == 0) ovf[atomicAdd(ovf_count, 1u)] = u;
     } else {
       const bool done = cand_block_node<true, true, 4, true>(u, try_cap, false, keys, score, inter, hedges, hoff, srcc,
                                                              tch, toff, inb, w, sizes, cnc, omega, delta, pairs,
@@ -645,12 +659,29 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   DBuf<uint32_t> ovf(c, nh), novf(c, 1);
   memset_async(c, novf.get(), 0, 4);
   if (inter)
-    HGP_LAUNCH(k_cand_opt<true>, nh, 1024, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(), caps.get(),
-               cnc, omega, delta, pairs, scores, ovf.get(), novf.get());
+    HGP_LAUNCH((k_cand_opt<true, 1>), nh, 1024, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(),
+               caps.get(), cnc, omega, delta, pairs, scores, ovf.get(), novf.get());
   else
-    HGP_LAUNCH(k_cand_opt<false>, nh, 1024, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(), caps.get(),
-               cnc, omega, delta, pairs, scores, ovf.get(), novf.get());
-  const uint32_t no = read_scalar(c, novf.get());
+    HGP_LAUNCH((k_cand_opt<false, 1>), nh, 1024, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, heavy.get(),
+               caps.get(), cnc, omega, delta, pairs, scores, ovf.get(), novf.get());
+  uint32_t no = read_scalar(c, novf.get());
+  const uint32_t no1 = no;
+  if (no) {
+    // second LDS attempt in OPT_CHUNKS passes; what still overflows goes to HBM tables
+    DBuf<uint32_t> ovf_b(c, no);
+    memset_async(c, novf.get(), 0, 4);
+    if (inter)
+      HGP_LAUNCH((k_cand_opt<true, OPT_CHUNKS>), no, 1024, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, ovf.get(),
+                 caps.get(), cnc, omega, delta, pairs, scores, ovf_b.get(), novf.get());
+    else
+      HGP_LAUNCH((k_cand_opt<false, OPT_CHUNKS>), no, 1024, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes,
+                 ovf.get(), caps.get(), cnc, omega, delta, pairs, scores, ovf_b.get(), novf.get());
+    no = read_scalar(c, novf.get());
+    ovf = std::move(ovf_b);
+  }
+  if (c.verbose >= 2)
+    fprintf(stderr, "  candidates: n=%u light %u medium %u big %u optimistic %u (overflow %u, then %u)\n", n, nl, nm,
+            nb, nh, no1, no);
   if (no == 0) return;
   // HBM tables: first with tables capped at HEAVY_TRY_SLOTS, then the overflow with full ones
   DBuf<uint32_t> ovf2(c, no), novf2(c, 1), mc(c, 1);
@@ -660,7 +691,7 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     const uint32_t try_cap = pass == 0 ? HEAVY_TRY_SLOTS : 0u;
     memset_async(c, mc.get(), 0, 4);
     memset_async(c, novf2.get(), 0, 4);
-    HGP_LAUNCH(k_heavy_max_cap, std::min<uint32_t>(ceil_div(cntl, 256), 1024), 256, 0, c, lst, caps.get(), cntl, n,
+    HGP_LAUNCH(k_heavy_max_cap, std::min<uint32_t>(ceil_div(cntl, 256), 1024), 256, 0, c, lst, caps.get(), cntl,
                try_cap ? try_cap : 0xFFFFFFFFu, mc.get());
     const uint64_t region = read_scalar(c, mc.get());
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)cntl, HEAVY_TABLE_SLOTS / region, 4096}));
@@ -668,6 +699,9 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, lst, caps.get(),
                cnc, omega, delta, pairs, scores, cntl, n, region, gk.get(), gs.get(), gi.get(), try_cap, ovf2.get(),
                novf2.get());
+    if (c.verbose >= 2)
+      fprintf(stderr, "  candidates: %u nodes in HBM tables (pass %d, region %llu, grid %llu)\n", cntl, pass,
+              (unsigned long long)region, (unsigned long long)grid);
     if (pass == 0) {
       cntl = read_scalar(c, novf2.get());
       lst = ovf2.get();

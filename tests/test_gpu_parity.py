@@ -122,21 +122,27 @@ def hub_graph(n, hubs, n_small, rng):
     return O.dedup(O.HG(n, np.array(off, np.uint64), np.array(pins, np.uint32), np.array(srcc, np.uint32), w))
 
 
-def test_candidates_heavy_tables(dev, rng):
-    """Nodes whose LDS attempt overflows: one fits an HBM table sized by its visit bound
-    (<= 65536 slots), one fits the 65536-slot first attempt although its bound asks for
-    more, and one overflows that attempt too and is redone with its worst-case table."""
-    n = 120000
-    hubs = [(7, 75, 400, 30000),      # bound 30K: table 65536, ~19K distinct
-            (11, 100, 400, 30000),    # bound 40K: table 131072, ~22K distinct (< 3/4 of 65536)
-            (13, 200, 400, n)]        # bound 80K: table 262144, ~58K distinct (> 3/4 of 65536)
+@pytest.mark.parametrize("n,hubs,expect", [
+    # (hub, hedges, pins per hedge, pool): distinct neighbours ~20K, ~45K and ~75K
+    (120000, [(7, 75, 400, 30000), (11, 141, 400, 120000), (13, 295, 400, 120000)],
+     [(15000, 25000), (40400, 49000), (70000, 80000)]),
+    # n = 50000: the hub's worst-case table is the whole level, indexed by node id
+    (50000, [(5, 229, 400, 50000)], [(40400, 50000)]),
+])
+def test_candidates_heavy_tables(dev, n, hubs, expect, rng):
+    """Nodes whose visit bound is far above their distinct neighbours, so they skip the
+    guaranteed tiers: they are scored in the one-pass LDS table, the four-pass LDS table
+    (one id class per pass), a 65536-slot HBM table at up to 3/4 load, or their worst-case
+    HBM table (hashed, or indexed by node id), depending on the distinct count and on
+    whether shared-inbound counts are kept (they shrink the LDS tables)."""
     hg = hub_graph(n, hubs, 4000, rng)
     L = G.Level(dev, hg)
     L.build_touching()
     ro, rt, ri = O.touching(hg)
-    nb = [len(np.unique(hg.pins[np.concatenate([np.arange(int(hg.off[e]), int(hg.off[e + 1])) for e in rt[ro[h]:ro[h + 1]]])]))
-          for h in (7, 11, 13)]
-    assert 15120 < nb[0] and 15120 < nb[1] < 49152 < nb[2], nb
+    for (h, *_), (lo, hi) in zip(hubs, expect):
+        idx = np.concatenate([np.arange(int(hg.off[e]), int(hg.off[e + 1])) for e in rt[ro[h]:ro[h + 1]]])
+        nb = len(np.unique(hg.pins[idx]))
+        assert lo < nb < hi, (h, nb)
     sizes = rng.integers(1, 3, size=hg.n).astype(np.uint32)
     for delta in (2**31 - 1, 30):
         pairs, scores, *_ = G.candidates(dev, L, sizes, 16, 6, delta)
