@@ -67,3 +67,21 @@ def test_netlist_kway_scaled(dev):
     assert device_km1(dev, hg, part) == pytest.approx(km1, rel=0, abs=1e-6)
     part2, _ = G.partition(dev, hg, prm)
     assert np.array_equal(part, part2), "two runs differ"
+
+
+def test_snn_incc_full_size(dev):
+    """The full SNN workload: 8 layers x 131,072 neurons, fan-out 256 from a Gaussian
+    receptive field (sigma 16, 363 x 363 grid), 235.8M pins, under -m 1024 4096 with -ipm
+    (about 40 s on one MI355X).  Every constraint holds, the partition count respects the
+    lower bound, and the device lambda-1 equals the host metric."""
+    print("case: snn 8x131072 fanout 256 (full size), ipm", flush=True)
+    hg = H.Hypergraph.generate("snn", layers=8, per_layer=131072, grid=363, fanout=256, sigma=16.0, seed=1)
+    n, e, p = hg.sizes()
+    assert (n, e, p) == (1048576, 917504, 235798528)
+    prm = incc_params(1024, 4096, 1 << 20, True)
+    part, st = G.partition(dev, hg, prm)
+    assert hg.validity(part, 1024, 4096, 1 << 20) == 1
+    k = int(part.max()) + 1
+    assert st.num_partitions == k and k >= -(-n // 1024)
+    km1 = hg.metrics(part)[0]
+    assert device_km1(dev, hg, part) == pytest.approx(km1, rel=0, abs=1e-6)
