@@ -98,7 +98,9 @@ void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, con
     float v = h_w[i] * 262144.0f;
     wfx[i] = !(v > 0.0f) ? 0u : (v >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)v);
   }
+  const auto t1 = std::chrono::high_resolution_clock::now();
   hgp_ip::Hypergraph h = hgp_ip::from_level(n, e, h_off.data(), h_pins.data(), wfx.data(), h_sizes.data());
+  const auto t2 = std::chrono::high_resolution_clock::now();
   std::vector<uint32_t> part(n, 0), psizes(k, 0);
   static const std::string mtk = find_mtkahypar();
   bool done = false;
@@ -115,6 +117,12 @@ void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, con
     }
   }
   if (!done) hgp_ip::partition(h, k, eps, omega, part.data(), psizes.data());
+  const auto t3 = std::chrono::high_resolution_clock::now();
+  if (c.verbose >= 2) {
+    using msd = std::chrono::duration<double, std::milli>;
+    fprintf(stderr, "  initial partition: download %.1f ms, merge %.1f ms (%zu hedges, %zu pins), partition %.1f ms\n",
+            msd(t1 - t0).count(), msd(t2 - t1).count(), h.off.size() - 1, h.pins.size(), msd(t3 - t2).count());
+  }
   DBuf<uint32_t> dp(c, n), ds(c, k);
   copy_h2d(c, dp.get(), part.data(), (uint64_t)n * 4);
   copy_h2d(c, ds.get(), psizes.data(), (uint64_t)k * 4);

@@ -152,24 +152,42 @@ inline Hypergraph from_level(uint32_t n, uint32_t e, const uint64_t* hoff, const
     part(0);
     for (auto& x : th) x.join();
   }
-  // 3. unique hedges in order of first appearance; weights summed over their copies
+  // 3. unique hedges in order of first appearance; weights summed over their copies.  Per
+  // chunk counts of unique hedges and their pins give every chunk its output positions.
   std::vector<uint32_t> uid(e, UINT32_MAX);
-  h.off.push_back(0);
-  uint64_t np = 0;
-  for (uint32_t i = 0; i < e; ++i)
-    if (rep[i] == i) {
-      uid[i] = (uint32_t)h.w.size();
-      h.w.push_back(0);
-      np += len[i];
-      h.off.push_back(np);
-    }
-  for (uint32_t i = 0; i < e; ++i)
-    if (rep[i] != UINT32_MAX) h.w[uid[rep[i]]] += wfx[i];
-  h.pins.resize(np);
-  parallel_chunks(e, threads, [&](uint32_t, uint64_t lo, uint64_t hi) {
+  std::vector<uint64_t> cu(threads + 1, 0), cp(threads + 1, 0);
+  parallel_chunks(e, threads, [&](uint32_t t, uint64_t lo, uint64_t hi) {
+    uint64_t u = 0, q = 0;
     for (uint64_t i = lo; i < hi; ++i)
-      if (rep[i] == i) std::copy(sp.data() + hoff[i], sp.data() + hoff[i] + len[i], h.pins.data() + h.off[uid[i]]);
+      if (rep[i] == i) {
+        u++;
+        q += len[i];
+      }
+    cu[t + 1] = u;
+    cp[t + 1] = q;
   });
+  for (uint32_t t = 0; t < threads; ++t) {
+    cu[t + 1] += cu[t];
+    cp[t + 1] += cp[t];
+  }
+  const uint64_t m = cu[threads], np = cp[threads];
+  h.off.resize(m + 1);
+  h.w.resize(m);
+  h.pins.resize(np);
+  h.off[0] = 0;
+  parallel_chunks(e, threads, [&](uint32_t t, uint64_t lo, uint64_t hi) {
+    uint64_t u = cu[t], q = cp[t];
+    for (uint64_t i = lo; i < hi; ++i)
+      if (rep[i] == i) {
+        uid[i] = (uint32_t)u;
+        h.w[u] = wfx[i];
+        std::copy(sp.data() + hoff[i], sp.data() + hoff[i] + len[i], h.pins.data() + q);
+        q += len[i];
+        h.off[++u] = q;
+      }
+  });
+  for (uint32_t i = 0; i < e; ++i)  // the copies (few) add their weights to their first hedge
+    if (rep[i] != UINT32_MAX && rep[i] != i) h.w[uid[rep[i]]] += wfx[i];
   build_incidence(h, threads);
   return h;
 }
@@ -441,10 +459,19 @@ struct BisectorT {
 
   // Greedy growing: side 1 holds everything, side 0 grows from `seed` until it reaches target0.
   void grow(uint32_t seed, uint64_t target0, uint64_t max0) {
+    // Everything starts on side 1: every hedge has all its pins there, and moving a node
+    // to side 0 would cut every hedge it is on, so gain(v) = -(sum of its hedge weights).
     side.assign(h.n, 1);
-    init_counts();
+    wside[0] = 0;
+    wside[1] = 0;
+    for (uint32_t v = 0; v < h.n; ++v) wside[1] += h.vw[v];
     gain.assign(h.n, 0);
-    for (uint32_t v = 0; v < h.n; ++v) gain[v] = node_gain(v);
+    for (uint32_t e = 0; e < (uint32_t)rec.size(); ++e) {
+      HRec& r = rec[e];
+      r.c[0] = 0;
+      r.c[1] = (uint32_t)(h.off[e + 1] - h.off[e]);
+      for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) gain[h.pins[j]] -= (int64_t)r.w;
+    }
     std::vector<uint8_t> locked(h.n, 0);
     heap[0].reset(h.n);
     heap[1].reset(h.n);
@@ -514,7 +541,7 @@ struct BisectorT {
     uint64_t best_imb = std::max(wside[0] > maxw[0] ? wside[0] - maxw[0] : 0,
                                  wside[1] > maxw[1] ? wside[1] - maxw[1] : 0);
     uint32_t stall = 0;
-    const uint32_t stall_limit = std::max<uint32_t>(50, h.n / 8);
+    const uint32_t stall_limit = std::max<uint32_t>(50, h.n / 32);
     while (!heap[0].empty() || !heap[1].empty()) {
       // candidate from each side that respects the target's capacity
       int pick = -1;
