@@ -418,7 +418,12 @@ struct BisectorT {
   }
   // Move v to the other side, updating the gains of free pins (classic FM delta rules,
   // all four applied in one scan of the hedge).
-  void move(uint32_t v, const std::vector<uint8_t>& locked) {
+  // `frontier` (growing only): a hedge that gets its first side-b pin queues its side-a pins
+  // that are not queued yet, with their final gains (later bumps update the queue).
+  // `boundary` (FM): a hedge that this move cuts makes its free pins boundary nodes, queued
+  // on their side when they are not queued already.
+  void move(uint32_t v, const std::vector<uint8_t>& locked, std::vector<uint8_t>* frontier = nullptr,
+            bool boundary = false) {
     const uint8_t a = side[v], b = 1 - a;
     const uint64_t i0 = h.ioff[v], i1 = h.ioff[v + 1];
     for (uint64_t i = i0; i < i1; ++i) {
@@ -436,7 +441,9 @@ struct BisectorT {
       if (bprev == 1) db -= we;    // the lone b pin no longer frees the hedge by moving
       if (anew == 0) db -= we;     // the hedge becomes internal to side b
       if (anew == 1) da += we;     // the lone a pin now frees the hedge by moving
-      if (da == 0 && db == 0) continue;
+      const bool queue = frontier && bprev == 0;
+      const bool cut_now = boundary && bprev == 0 && anew >= 1;
+      if (da == 0 && db == 0 && !queue && !cut_now) continue;
       const uint32_t* pp = h.pins.data() + r.off;
       const uint32_t rl = len(r);
       for (uint32_t j = 0; j < rl; ++j) {
@@ -444,6 +451,11 @@ struct BisectorT {
         if (u == v || locked[u]) continue;
         const int64_t d = side[u] == a ? da : db;
         if (d) bump(u, d);
+        if (queue && side[u] == a && !(*frontier)[u]) {
+          (*frontier)[u] = 1;
+          heap[a].push(u, gain[u]);
+        }
+        if (cut_now && !heap[side[u]].contains(u)) heap[side[u]].push(u, gain[u]);
       }
     }
     side[v] = b;
@@ -475,25 +487,8 @@ struct BisectorT {
     std::vector<uint8_t> locked(h.n, 0);
     heap[0].reset(h.n);
     heap[1].reset(h.n);
+    // Frontier: a hedge's pins are queued when its first pin moves to side 0 (inside move).
     std::vector<uint8_t> inq(h.n, 0);
-    // A hedge's pins are queued the first time one of its pins moves; afterwards every pin
-    // of it is queued or moved already, so it is never scanned again.
-    std::vector<uint8_t> expanded(h.off.size() - 1, 0);
-    auto touch_nbrs = [&](uint32_t v) {
-      const uint64_t i0 = h.ioff[v], i1 = h.ioff[v + 1];
-      for (uint64_t i = i0; i < i1; ++i) {
-        if (i + 16 < i1) __builtin_prefetch(&expanded[h.inc[i + 16]]);
-        if (i + 8 < i1 && !expanded[h.inc[i + 8]]) __builtin_prefetch(h.pins.data() + rec[h.inc[i + 8]].off);
-        uint32_t e = h.inc[i];
-        if (expanded[e]) continue;
-        expanded[e] = 1;
-        const HRec& r = rec[e];
-        for (uint64_t j = r.off; j < (uint64_t)r.off + len(r); ++j) {
-          uint32_t u = h.pins[j];
-          if (side[u] == 1 && !inq[u]) { inq[u] = 1; heap[1].push(u, gain[u]); }
-        }
-      }
-    };
     uint32_t next_unvisited = 0;
     uint32_t cur = seed;
     while (wside[0] < target0) {
@@ -511,8 +506,7 @@ struct BisectorT {
       }
       if (heap[1].contains(cur)) heap[1].remove(cur);
       inq[cur] = 1;
-      move(cur, locked);
-      touch_nbrs(cur);
+      move(cur, locked, &inq);
       cur = UINT32_MAX;
     }
     heap[1].reset(h.n);
@@ -535,7 +529,7 @@ struct BisectorT {
       if (boundary) heap[side[v]].push(v, gain[v]);
     }
     gains_exact = false;  // moves below skip locked nodes and the rollback maintains nothing
-    std::vector<uint32_t> moved, fresh, parked[2];
+    std::vector<uint32_t> moved, parked[2];
     int64_t cum = 0, best = 0;
     size_t best_len = 0;
     uint64_t best_imb = std::max(wside[0] > maxw[0] ? wside[0] - maxw[0] : 0,
@@ -559,24 +553,14 @@ struct BisectorT {
       heap[pick].remove(v);
       locked[v] = 1;
       cum += gain[v];
-      // hedges that this move cuts: their free pins become boundary nodes
-      const uint8_t from = side[v], to = 1 - from;
-      fresh.clear();
-      for (uint64_t i = h.ioff[v]; i < h.ioff[v + 1]; ++i) {
-        uint32_t e = h.inc[i];
-        if (rec[e].c[to] == 0 && rec[e].c[from] >= 2) fresh.push_back(e);
-      }
-      move(v, locked);
+      // hedges that this move cuts queue their free pins as boundary nodes (inside move)
+      const uint8_t to = 1 - side[v];
+      move(v, locked, nullptr, true);
       moved.push_back(v);
       // side `from` became lighter: nodes parked on side `to` for lack of room may fit now
       for (uint32_t u : parked[to])
         if (!locked[u] && side[u] == to && !heap[to].contains(u)) heap[to].push(u, gain[u]);
       parked[to].clear();
-      for (uint32_t e : fresh)
-        for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
-          uint32_t u = h.pins[j];
-          if (!locked[u] && !heap[side[u]].contains(u)) heap[side[u]].push(u, gain[u]);
-        }
       uint64_t imb = std::max(wside[0] > maxw[0] ? wside[0] - maxw[0] : 0,
                               wside[1] > maxw[1] ? wside[1] - maxw[1] : 0);
       if (imb < best_imb || (imb == best_imb && cum > best)) {
