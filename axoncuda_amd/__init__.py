@@ -117,6 +117,8 @@ SIGNATURES = [
     ("hgp_connectivity", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, _i64p]),
     ("hgp_initial_partition", C.c_int32, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, _vp, C.c_uint32,
                                           C.c_float, C.c_uint32, C.POINTER(_vp), C.POINTER(_vp)]),
+    ("hgp_ip_input_check", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_uint64)]),
     ("hgp_partition", C.c_int32, [_vp, C.POINTER(Params), _vp, _u32p, C.POINTER(RunStats)]),
     ("hgp_profile_kernel", C.c_int32, [_vp, C.c_char_p]),
     ("hgp_profile_read", C.c_int32, [_vp, C.POINTER(C.c_double), _u64p, _u64p]),

@@ -10,6 +10,7 @@
 
 #include "common.hpp"
 #include "host/driver.hpp"
+#include "host/ip_core.hpp"
 #include "host/model.hpp"
 #include "prim.hpp"
 #include "stages.hpp"
@@ -569,6 +570,32 @@ hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const u
   return guard(ctx, [&](Ctx& c) {
     hgp::initial_partition(c, d_hedges, d_hedges_offsets, d_hedge_weights, num_nodes, num_hedges, d_nodes_sizes, k,
                            epsilon, max_nodes_per_part, *d_partitions, *d_partitions_sizes, nullptr);
+  });
+}
+
+hgp_status hgp_ip_input_check(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                              const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                              uint32_t num_hedges, int32_t* equal, uint64_t* merged_hedges) {
+  if (!equal) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp_ip::Hypergraph dh;
+    hgp::ip_input_device(c, d_hedges, d_hedges_offsets, d_hedge_weights, d_nodes_sizes, num_nodes, num_hedges, dh);
+    const uint32_t e = num_hedges;
+    std::vector<uint64_t> off((size_t)e + 1, 0);
+    if (e) hgp::copy_d2h(c, off.data(), d_hedges_offsets, ((uint64_t)e + 1) * 8);
+    std::vector<uint32_t> pins(off[e]), sizes(num_nodes), wfx(e);
+    std::vector<float> w(e);
+    if (off[e]) hgp::copy_d2h(c, pins.data(), d_hedges, off[e] * 4);
+    if (num_nodes) hgp::copy_d2h(c, sizes.data(), d_nodes_sizes, (uint64_t)num_nodes * 4);
+    if (e) hgp::copy_d2h(c, w.data(), d_hedge_weights, (uint64_t)e * 4);
+    for (uint32_t i = 0; i < e; ++i) {
+      const float v = w[i] * 262144.0f;
+      wfx[i] = !(v > 0.0f) ? 0u : (v >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)v);
+    }
+    const hgp_ip::Hypergraph hh = hgp_ip::from_level(num_nodes, e, off.data(), pins.data(), wfx.data(), sizes.data());
+    *equal = dh.n == hh.n && dh.off == hh.off && dh.pins == hh.pins && dh.w == hh.w && dh.vw == hh.vw &&
+             dh.ioff == hh.ioff && dh.inc == hh.inc;
+    if (merged_hedges) *merged_hedges = dh.off.size() - 1;
   });
 }
 

@@ -86,20 +86,9 @@ void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, con
                        const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, uint32_t*& part_out,
                        uint32_t*& psizes_out, double* ms) {
   auto t0 = std::chrono::high_resolution_clock::now();
-  std::vector<uint64_t> h_off((size_t)e + 1);
-  copy_d2h(c, h_off.data(), hoff, ((uint64_t)e + 1) * 8);
-  std::vector<uint32_t> h_pins(h_off[e]), h_sizes(n);
-  std::vector<float> h_w(e);
-  copy_d2h(c, h_pins.data(), hedges, h_off[e] * 4);
-  copy_d2h(c, h_w.data(), w, (uint64_t)e * 4);
-  copy_d2h(c, h_sizes.data(), sizes, (uint64_t)n * 4);
-  std::vector<uint32_t> wfx(e);
-  for (uint32_t i = 0; i < e; ++i) {
-    float v = h_w[i] * 262144.0f;
-    wfx[i] = !(v > 0.0f) ? 0u : (v >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)v);
-  }
-  const auto t1 = std::chrono::high_resolution_clock::now();
-  hgp_ip::Hypergraph h = hgp_ip::from_level(n, e, h_off.data(), h_pins.data(), wfx.data(), h_sizes.data());
+  const auto t1 = t0;
+  hgp_ip::Hypergraph h;
+  ip_input_device(c, hedges, hoff, w, sizes, n, e, h);
   const auto t2 = std::chrono::high_resolution_clock::now();
   std::vector<uint32_t> part(n, 0), psizes(k, 0);
   static const std::string mtk = find_mtkahypar();
@@ -107,7 +96,7 @@ void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, con
   if (!mtk.empty()) {
     done = run_mtkahypar(mtk, h, k, eps, part.data());
     if (done) {
-      for (uint32_t v = 0; v < n; ++v) psizes[part[v]] += h_sizes[v];
+      for (uint32_t v = 0; v < n; ++v) psizes[part[v]] += (uint32_t)h.vw[v];
       for (uint32_t p = 0; p < k; ++p)
         if (psizes[p] > omega) done = false;  // the reference rejects oversized parts (init_part.cu:930-936)
     }
@@ -120,8 +109,8 @@ void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, con
   const auto t3 = std::chrono::high_resolution_clock::now();
   if (c.verbose >= 2) {
     using msd = std::chrono::duration<double, std::milli>;
-    fprintf(stderr, "  initial partition: download %.1f ms, merge %.1f ms (%zu hedges, %zu pins), partition %.1f ms\n",
-            msd(t1 - t0).count(), msd(t2 - t1).count(), h.off.size() - 1, h.pins.size(), msd(t3 - t2).count());
+    fprintf(stderr, "  initial partition: input on the device %.1f ms (%zu hedges, %zu pins), partition %.1f ms\n",
+            msd(t2 - t1).count(), h.off.size() - 1, h.pins.size(), msd(t3 - t2).count());
   }
   DBuf<uint32_t> dp(c, n), ds(c, k);
   copy_h2d(c, dp.get(), part.data(), (uint64_t)n * 4);

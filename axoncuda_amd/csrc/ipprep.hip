@@ -1,0 +1,235 @@
+// ipprep.hip - the initial partitioner's input built on the device.
+//
+// The coarsest level is handed to the initial partitioner the way the reference hands it to
+// Mt-KaHyPar (sources/init_part.cu:733-953): hedges with at least two distinct pins, pins
+// sorted, identical pin sets merged into the first of them with their 2^18 fixed-point
+// weights summed, unique hedges in order of first appearance, node sizes as vertex weights.
+// The host version of this step (hgp_ip::from_level) walks 6.7M pins on the c2 benchmark;
+// here it is a handful of sorts and scans, and the host receives the merged hypergraph and
+// its incidence lists ready to use.  The output equals from_level's exactly.
+#include <vector>
+
+#include "host/ip_core.hpp"
+#include "prim.hpp"
+#include "stages.hpp"
+
+namespace hgp {
+namespace {
+
+// (hedge << pb) | pin for every pin: one sort orders the pins inside every hedge
+__global__ void k_ipp_keys(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff, uint32_t e, int pb,
+                           uint64_t* __restrict__ keys) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= e) return;
+  for (uint64_t i = hoff[x]; i < hoff[x + 1]; ++i) keys[i] = ((uint64_t)x << pb) | hedges[i];
+}
+
+// keep[i] = the pin differs from its predecessor in the same hedge (drops repeated pins)
+__global__ void k_ipp_keep(const uint64_t* __restrict__ keys, uint64_t p, uint32_t* __restrict__ keep) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p) return;
+  keep[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// distinct pins of hedge x = kpos[hoff[x+1]] - kpos[hoff[x]]; compacted pin list
+__global__ void k_ipp_compact(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ keep,
+                              const uint64_t* __restrict__ kpos, uint64_t p, uint64_t pmask,
+                              uint32_t* __restrict__ pins) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p || !keep[i]) return;
+  pins[kpos[i]] = (uint32_t)(keys[i] & pmask);
+}
+
+// per hedge: distinct-pin range, hash of the sorted set, fixed-point weight; hedges of fewer
+// than two distinct pins are flagged out
+__global__ void k_ipp_hash(const uint64_t* __restrict__ hoff, const uint64_t* __restrict__ kpos,
+                           const uint32_t* __restrict__ pins, const float* __restrict__ w, uint32_t e,
+                           uint64_t* __restrict__ hs, uint64_t* __restrict__ start, uint32_t* __restrict__ len,
+                           uint32_t* __restrict__ wfx, uint32_t* __restrict__ live) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= e) return;
+  const uint64_t s = kpos[hoff[x]], t = kpos[hoff[x + 1]];
+  const uint32_t l = (uint32_t)(t - s);
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t i = s; i < t; ++i) {
+    uint64_t z = (h ^ pins[i]) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    h = z ^ (z >> 31);
+  }
+  hs[x] = h;
+  start[x] = s;
+  len[x] = l;
+  const float v = w[x] * 262144.0f;
+  wfx[x] = !(v > 0.0f) ? 0u : (v >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)v);
+  live[x] = l >= 2 ? 1u : 0u;
+}
+
+__global__ void k_ipp_live_keys(const uint32_t* __restrict__ live, const uint64_t* __restrict__ lpos,
+                                const uint64_t* __restrict__ hs, uint32_t e, uint64_t* __restrict__ keys,
+                                uint32_t* __restrict__ ids) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= e || !live[x]) return;
+  keys[lpos[x]] = hs[x];
+  ids[lpos[x]] = x;
+}
+
+__device__ __forceinline__ bool same_set(const uint32_t* __restrict__ pins, const uint64_t* __restrict__ start,
+                                         const uint32_t* __restrict__ len, uint32_t a, uint32_t b) {
+  if (len[a] != len[b]) return false;
+  const uint32_t* pa = pins + start[a];
+  const uint32_t* pb = pins + start[b];
+  for (uint32_t i = 0; i < len[a]; ++i)
+    if (pa[i] != pb[i]) return false;
+  return true;
+}
+
+// Sorted by (hash, hedge id): the first hedge of an identical set is the first identical
+// entry of its hash run.  Runs hold one set unless hashes collide; then every earlier
+// entry of the run is compared.
+__global__ void k_ipp_rep(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids, uint32_t m,
+                          const uint32_t* __restrict__ pins, const uint64_t* __restrict__ start,
+                          const uint32_t* __restrict__ len, uint32_t* __restrict__ rep) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t h = keys[i];
+  const uint32_t x = ids[i];
+  uint32_t s = i;
+  while (s > 0 && keys[s - 1] == h) --s;
+  uint32_t r = x;
+  if (s < i) {
+    if (same_set(pins, start, len, ids[s], x)) {
+      r = ids[s];
+    } else {
+      for (uint32_t j = s + 1; j < i; ++j)
+        if (same_set(pins, start, len, ids[j], x)) {
+          r = ids[j];
+          break;
+        }
+    }
+  }
+  rep[x] = r;
+}
+
+__global__ void k_ipp_first(const uint32_t* __restrict__ rep, uint32_t e, uint32_t* __restrict__ first) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < e) first[x] = rep[x] == x ? 1u : 0u;
+}
+
+// output length and weight per unique hedge (u = its rank among the first hedges)
+__global__ void k_ipp_out_meta(const uint32_t* __restrict__ rep, const uint64_t* __restrict__ uid, uint32_t e,
+                               const uint32_t* __restrict__ len, const uint32_t* __restrict__ wfx,
+                               uint64_t* __restrict__ olen, unsigned long long* __restrict__ ow) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= e || rep[x] == NONE) return;
+  const uint32_t r = rep[x];
+  const uint64_t u = uid[r];
+  if (r == x) olen[u] = len[x];
+  atomicAdd(&ow[u], (unsigned long long)wfx[x]);
+}
+
+// pins of the unique hedges at their output offsets, plus the (pin, hedge) incidence keys
+__global__ void k_ipp_out_pins(const uint32_t* __restrict__ rep, const uint64_t* __restrict__ uid, uint32_t e,
+                               const uint32_t* __restrict__ pins, const uint64_t* __restrict__ start,
+                               const uint32_t* __restrict__ len, const uint64_t* __restrict__ ooff, int hb,
+                               uint32_t* __restrict__ opins, uint64_t* __restrict__ ikeys) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= e || rep[x] != x) return;
+  const uint64_t u = uid[x];
+  const uint64_t o = ooff[u];
+  for (uint32_t i = 0; i < len[x]; ++i) {
+    const uint32_t v = pins[start[x] + i];
+    opins[o + i] = v;
+    ikeys[o + i] = ((uint64_t)v << hb) | u;
+  }
+}
+
+__global__ void k_ipp_inc(const uint64_t* __restrict__ ikeys, uint64_t p, uint64_t hmask, uint32_t* __restrict__ inc) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p) inc[i] = (uint32_t)(ikeys[i] & hmask);
+}
+
+}  // namespace
+
+void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
+                     uint32_t n, uint32_t e, hgp_ip::Hypergraph& h) {
+  h = hgp_ip::Hypergraph();
+  h.n = n;
+  h.vw.resize(n);
+  if (n) {
+    std::vector<uint32_t> s(n);
+    copy_d2h(c, s.data(), sizes, (uint64_t)n * 4);
+    for (uint32_t v = 0; v < n; ++v) h.vw[v] = s[v];
+  }
+  h.off.assign(1, 0);
+  h.ioff.assign((uint64_t)n + 1, 0);
+  if (e == 0 || n == 0) return;
+  const uint64_t p = read_scalar(c, hoff + e);
+  const int pb = std::max(1, bit_width(n - 1));
+  const int eb = std::max(1, bit_width(e - 1));
+  HGP_REQUIRE(pb + eb <= 64, HGP_E_CAPACITY, "initial partition input: hedge and node ids exceed 64 key bits");
+  const uint32_t ge = ceil_div(e, 256);
+  // 1. sorted distinct pins per hedge
+  DBuf<uint64_t> keys(c, std::max<uint64_t>(p, 1));
+  HGP_LAUNCH(k_ipp_keys, ge, 256, 0, c, hedges, hoff, e, pb, keys.get());
+  radix_sort(c, keys.get(), nullptr, p, pb + eb);
+  DBuf<uint32_t> keep(c, std::max<uint64_t>(p, 1));
+  DBuf<uint64_t> kpos(c, p + 1);
+  HGP_LAUNCH(k_ipp_keep, ceil_div(p, 256), 256, 0, c, keys.get(), p, keep.get());
+  exclusive_scan(c, keep.get(), kpos.get(), p);
+  DBuf<uint32_t> pins(c, std::max<uint64_t>(p, 1));
+  HGP_LAUNCH(k_ipp_compact, ceil_div(p, 256), 256, 0, c, keys.get(), keep.get(), kpos.get(), p,
+             ((uint64_t)1 << pb) - 1, pins.get());
+  keep.reset();
+  // 2. per hedge: hash, range, weight; hedges of >= 2 pins sorted by (hash, id)
+  DBuf<uint64_t> hs(c, e), start(c, e);
+  DBuf<uint32_t> len(c, e), wfx(c, e), live(c, e);
+  HGP_LAUNCH(k_ipp_hash, ge, 256, 0, c, hoff, kpos.get(), pins.get(), w, e, hs.get(), start.get(), len.get(),
+             wfx.get(), live.get());
+  DBuf<uint64_t> lpos(c, (uint64_t)e + 1);
+  exclusive_scan(c, live.get(), lpos.get(), e);
+  const uint32_t m = (uint32_t)read_scalar(c, lpos.get() + e);
+  if (m == 0) return;
+  DBuf<uint64_t> hk(c, m);
+  DBuf<uint32_t> ids(c, m);
+  HGP_LAUNCH(k_ipp_live_keys, ge, 256, 0, c, live.get(), lpos.get(), hs.get(), e, hk.get(), ids.get());
+  radix_sort(c, hk.get(), ids.get(), m, 64);
+  // 3. representative of every hedge; the first hedges in id order are the output
+  DBuf<uint32_t> rep(c, e), first(c, e);
+  fill_u32(c, rep.get(), NONE, e);
+  HGP_LAUNCH(k_ipp_rep, ceil_div(m, 256), 256, 0, c, hk.get(), ids.get(), m, pins.get(), start.get(), len.get(),
+             rep.get());
+  HGP_LAUNCH(k_ipp_first, ge, 256, 0, c, rep.get(), e, first.get());
+  DBuf<uint64_t> uid(c, (uint64_t)e + 1);
+  exclusive_scan(c, first.get(), uid.get(), e);
+  const uint32_t u = (uint32_t)read_scalar(c, uid.get() + e);
+  DBuf<uint64_t> olen(c, u), ooff(c, (uint64_t)u + 1);
+  DBuf<unsigned long long> ow(c, u);
+  memset_async(c, ow.get(), 0, (uint64_t)u * 8);
+  HGP_LAUNCH(k_ipp_out_meta, ge, 256, 0, c, rep.get(), uid.get(), e, len.get(), wfx.get(), olen.get(), ow.get());
+  exclusive_scan(c, olen.get(), ooff.get(), u);
+  const uint64_t op = read_scalar(c, ooff.get() + u);
+  DBuf<uint32_t> opins(c, op);
+  DBuf<uint64_t> ikeys(c, op);
+  const int hb = std::max(1, bit_width(u - 1));
+  HGP_REQUIRE(pb + hb <= 64, HGP_E_CAPACITY, "initial partition input: hedge and node ids exceed 64 key bits");
+  HGP_LAUNCH(k_ipp_out_pins, ge, 256, 0, c, rep.get(), uid.get(), e, pins.get(), start.get(), len.get(), ooff.get(),
+             hb, opins.get(), ikeys.get());
+  // 4. incidence: hedges of every node in ascending order
+  radix_sort(c, ikeys.get(), nullptr, op, pb + hb);
+  DBuf<uint32_t> inc(c, op);
+  DBuf<uint64_t> ioff(c, (uint64_t)n + 1);
+  HGP_LAUNCH(k_ipp_inc, ceil_div(op, 256), 256, 0, c, ikeys.get(), op, ((uint64_t)1 << hb) - 1, inc.get());
+  segment_offsets(c, ikeys.get(), hb, op, n, ioff.get());
+  h.off.resize((uint64_t)u + 1);
+  h.w.resize(u);
+  h.pins.resize(op);
+  h.inc.resize(op);
+  copy_d2h(c, h.off.data(), ooff.get(), ((uint64_t)u + 1) * 8);
+  copy_d2h(c, h.w.data(), ow.get(), (uint64_t)u * 8);
+  copy_d2h(c, h.pins.data(), opins.get(), op * 4);
+  copy_d2h(c, h.inc.data(), inc.get(), op * 4);
+  copy_d2h(c, h.ioff.data(), ioff.get(), ((uint64_t)n + 1) * 8);
+}
+
+}  // namespace hgp

@@ -5,6 +5,10 @@
 
 #include "common.hpp"
 
+namespace hgp_ip {
+struct Hypergraph;
+}
+
 namespace hgp {
 
 // ---- construction (construct.hip) -----------------------------------------------------
@@ -56,6 +60,13 @@ void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const u
                    uint32_t e, uint32_t k, uint32_t* out);
 int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* part,
                            uint32_t e);
+
+// ---- initial partitioner input (ipprep.hip) ------------------------------------------------
+// The coarsest level as the initial partitioner takes it (hedges of >= 2 distinct pins,
+// sorted, identical sets merged in order of first appearance, incidence lists), built on the
+// device; equal to hgp_ip::from_level on the same level.
+void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
+                     uint32_t n, uint32_t e, hgp_ip::Hypergraph& h);
 
 // ---- post-processing (postproc.hip) -----------------------------------------------------
 void greedy_merge_groups(Ctx& c, const uint32_t* sizes, const uint32_t* inb, uint32_t n, uint32_t ngroups,

@@ -269,6 +269,16 @@ hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const u
                                  uint32_t max_nodes_per_part, uint32_t** d_partitions,
                                  uint32_t** d_partitions_sizes);
 
+/* The initial partitioner's input (hedges of >= 2 distinct pins, sorted, identical sets
+ * merged with summed 2^18 fixed-point weights, incidence lists) is built on the device.
+ * This entry point builds it both on the device and on the host (the restatement of
+ * sources/init_part.cu:733-953 that the partitioner's own tests use) and sets *equal to 1
+ * when the two agree; *merged_hedges receives the device result's hedge count.  No
+ * counterpart in the reference: it exists for parity tests of that step. */
+hgp_status hgp_ip_input_check(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                              const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                              uint32_t num_hedges, int32_t* equal, uint64_t* merged_hedges);
+
 /* ---- whole pipeline (sources/main.cu:38-940) ------------------------------ */
 /* Partitions the host model; writes h_partitions_out[num_nodes] (zero-based ids). */
 hgp_status hgp_partition(hgp_ctx* ctx, const hgp_params* params, const hgp_hgraph* hg,

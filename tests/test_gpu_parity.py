@@ -367,6 +367,37 @@ def test_inbound_and_connectivity(dev, case, rng):
     assert km1.value == O.connectivity_fixed(hg, part)
 
 
+@pytest.mark.parametrize("n,e,max_size,copies", [(300, 2000, 5, 0.3), (5000, 40000, 12, 0.2), (64, 3000, 4, 0.6)])
+def test_ip_input_device_equals_host(dev, n, e, max_size, copies, rng):
+    """The initial partitioner's input built on the device (sort, merge identical pin sets,
+    sum weights, incidence) equals the host restatement exactly.  The hedges include exact
+    copies, copies with their pins permuted, repeated pins, single-pin hedges and weights
+    that round to zero in fixed point."""
+    hedges = []
+    for _ in range(e):
+        if hedges and rng.random() < copies:
+            h = list(hedges[int(rng.integers(0, len(hedges)))])
+            rng.shuffle(h)
+        else:
+            h = [int(x) for x in rng.choice(n, size=int(rng.integers(1, max_size + 1)), replace=False)]
+            if rng.random() < 0.1:
+                h.append(h[0])  # a repeated pin
+        hedges.append(h)
+    off = np.zeros(e + 1, np.uint64)
+    off[1:] = np.cumsum([len(h) for h in hedges])
+    pins = np.array([x for h in hedges for x in h], np.uint32)
+    w = (rng.random(e) * 3).astype(np.float32)
+    w[rng.random(e) < 0.05] = 1e-7
+    hg = O.HG(n, off, pins, np.zeros(e, np.uint32), w)
+    L = G.Level(dev, hg)
+    d_sizes = dev.upload(rng.integers(1, 5, size=n).astype(np.uint32))
+    eq, m = C.c_int32(), C.c_uint64()
+    dev.check(dev.lib.hgp_ip_input_check(dev.ctx, G.V(L.pins), G.V(L.off), G.V(L.w), G.V(d_sizes), n, e, C.byref(eq),
+                                         C.byref(m)), "ip input check")
+    assert eq.value == 1
+    assert 0 < m.value < e
+
+
 def test_initial_partition(dev, rng):
     hg = make(CASES[1], rng)
     L = G.Level(dev, hg)
