@@ -69,9 +69,9 @@ __global__ void __launch_bounds__(256) k_pp_small(const uint32_t* __restrict__ h
   if (e < E) {
     const uint64_t s = hoff[e], t = hoff[e + 1];
     if (t - s > PP_SMALL) {
-      large_flag[e] = 1u;
+      if (large_flag) large_flag[e] = 1u;
     } else {
-      large_flag[e] = 0u;
+      if (large_flag) large_flag[e] = 0u;
       uint32_t qs[PP_SMALL], cs[PP_SMALL], ds[PP_SMALL];
       uint32_t m = 0;
       const uint32_t sc = INB ? srcc[e] : 0u;
@@ -311,10 +311,7 @@ __device__ __forceinline__ uint32_t first_nonadjacent(const GainArgs& a, const u
 
 // Nodes whose touching hedges list at most GAIN_TINY partition entries: one thread per node,
 // the (partition, connection) table held in registers.
-__global__ void __launch_bounds__(256) k_gains_tiny(GainArgs a, const uint32_t* __restrict__ list, uint32_t nlist) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nlist) return;
-  const uint32_t v = list[i];
+__device__ __forceinline__ void gains_tiny_node(const GainArgs& a, uint32_t v) {
   const uint32_t p = a.part[v], s = a.sizes[v];
   uint32_t qs[GAIN_TINY];
   int64_t cs[GAIN_TINY];
@@ -395,13 +392,8 @@ __global__ void __launch_bounds__(256) k_gains_tiny(GainArgs a, const uint32_t* 
 // (k slots); otherwise it is an open-addressing table sized to twice the node's possible
 // distinct partitions.
 // Dynamic LDS: 4 waves x capmax slots of (int64 connection, uint32 key).
-__global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t* __restrict__ list, uint32_t nlist,
-                                                     const uint64_t* __restrict__ bound, uint32_t capmax) {
-  extern __shared__ int64_t s_conn_all[];
-  const uint32_t wv = threadIdx.x / 64, l = lane_id();
-  const uint32_t idx = blockIdx.x * 4 + wv;
-  if (idx >= nlist) return;
-  const uint32_t v = list[idx];
+__device__ __forceinline__ void gains_light_node(const GainArgs& a, uint32_t v, const uint64_t* __restrict__ bound,
+                                                 uint32_t capmax, int64_t* s_conn_all, uint32_t wv, uint32_t l) {
   const bool direct = a.k <= GAIN_CAP;
   uint32_t cap = a.k;
   if (!direct) {
@@ -458,6 +450,21 @@ __global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t*
   if (l == 0) {
     a.moves[v] = bq;
     a.gains[v] = bq == NONE ? 0 : bg;
+  }
+}
+
+// Persistent: the grid strides over the `*count` listed nodes (the count stays on the device).
+__global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ count,
+                                                     const uint64_t* __restrict__ bound, uint32_t capmax) {
+  extern __shared__ int64_t s_conn_all[];
+  const uint32_t wv = threadIdx.x / 64, l = lane_id();
+  const uint32_t nlist = *count;
+  for (uint32_t idx = blockIdx.x * 4 + wv; idx < nlist; idx += gridDim.x * 4) {
+    gains_light_node(a, list[idx], bound, capmax, s_conn_all, wv, l);
+    // every lane has read the table before the next node clears it
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -552,6 +559,8 @@ __global__ void __launch_bounds__(256) k_gains_heavy(GainArgs a, const uint32_t*
 // overflow flag and is listed in `ovf` for the HBM-table kernel.  At most one insert per
 // thread can pass the check concurrently, so GB_LIMIT + 1024 < GB_SLOTS keeps a free slot
 // for every probe.
+constexpr uint32_t LIGHT_GRID = 16384;  // workgroups of the persistent light kernel (4 nodes each at a time)
+constexpr uint32_t BLOCK_GRID = 2048;   // workgroups of the persistent block kernel
 constexpr uint32_t GB_DIRECT = 20160;  // 8 bytes per slot
 constexpr uint32_t GB_SLOTS = 13440;   // 12 bytes per slot
 constexpr uint32_t GB_LIMIT = GB_SLOTS / 4 * 3;
@@ -569,7 +578,8 @@ __device__ __forceinline__ uint32_t gb_find(const uint32_t* keys, uint32_t q) {
 
 template <bool DIRECT>
 __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t* __restrict__ list,
-                                                      uint32_t* __restrict__ ovf, uint32_t* __restrict__ novf) {
+                                                      const uint32_t* __restrict__ count, uint32_t* __restrict__ ovf,
+                                                      uint32_t* __restrict__ novf) {
   constexpr uint32_t SLOTS = DIRECT ? GB_DIRECT : GB_SLOTS;
   __shared__ int64_t s_conn[SLOTS];
   __shared__ uint32_t s_keys[DIRECT ? 1 : SLOTS];
@@ -578,8 +588,10 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
   __shared__ uint32_t s_bh[16], s_bq[16];
   __shared__ uint32_t s_state[2];  // partitions inserted, overflow flag
   __shared__ uint32_t s_first;
-  const uint32_t v = list[blockIdx.x];
   const uint32_t cap = DIRECT ? a.k : SLOTS;
+  const uint32_t nlist = *count;  // persistent: the grid strides over the listed nodes
+  for (uint32_t bi = blockIdx.x; bi < nlist; bi += gridDim.x) {
+  const uint32_t v = list[bi];
   for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
     s_conn[i] = 0;
     if (!DIRECT) s_keys[i] = NONE;
@@ -637,9 +649,11 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
     s_sum[1][wv] = Wv;
   }
   __syncthreads();
-  if (!DIRECT && s_state[1]) {
+  const bool overflow = !DIRECT && s_state[1];
+  if (overflow) {
     if (threadIdx.x == 0) ovf[atomicAdd(novf, 1u)] = v;
-    return;
+    __syncthreads();  // s_state is read by every thread before the next node resets it
+    continue;
   }
   int64_t base = 0;
   for (int i = 0; i < 16; ++i) base += s_sum[0][i] - s_sum[1][i];
@@ -687,21 +701,47 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
     a.moves[v] = q;
     a.gains[v] = q == NONE ? 0 : g;
   }
+  __syncthreads();  // the shared state is read before the next node resets it
+  }
 }
 
-// bound = partition entries listed by the node's touching hedges; tiers: tiny (thread per
-// node), light (wave per node), heavy (workgroup per node, table in HBM)
-__global__ void k_gain_bound(const uint32_t* __restrict__ tch, const uint64_t* __restrict__ toff,
-                             const uint32_t* __restrict__ lambda, uint32_t n, uint32_t k, uint64_t* __restrict__ bound,
-                             uint32_t* __restrict__ tiny, uint32_t* __restrict__ light) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
+// Routes every node by bound = partition entries listed by its touching hedges: bound <=
+// GAIN_TINY is scored right here (thread per node, registers); the others are appended to
+// the light list (wave per node; all of them when k <= GAIN_LIGHT) or the block list
+// (workgroup per node), with the counts left on the device.  List order varies from run to
+// run; each node's result does not depend on it.
+__global__ void __launch_bounds__(256) k_gains_route(GainArgs a, uint32_t n, uint64_t* __restrict__ bound,
+                                                     uint32_t* __restrict__ light, uint32_t* __restrict__ block,
+                                                     uint32_t* __restrict__ counts) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t b = 0;
-  for (uint64_t j = toff[v]; j < toff[v + 1]; ++j) b += lambda[tch[j]];
-  bound[v] = b;
-  const bool t = b <= GAIN_TINY;
-  tiny[v] = t ? 1u : 0u;
-  light[v] = (!t && (b <= GAIN_LIGHT || k <= GAIN_LIGHT)) ? 1u : 0u;
+  if (v < n)
+    for (uint64_t j = a.toff[v]; j < a.toff[v + 1]; ++j) b += a.hinfo[a.tch[j]].z;
+  const bool tiny = v < n && b <= GAIN_TINY;
+  if (tiny) gains_tiny_node(a, v);
+  const bool rest = v < n && !tiny;
+  if (rest) bound[v] = b;
+  const bool to_light = rest && (b <= GAIN_LIGHT || a.k <= GAIN_LIGHT);
+  const bool to_block = rest && !to_light;
+  const uint32_t l = lane_id();
+  const uint64_t ml = __ballot(to_light), mb = __ballot(to_block);
+  uint32_t base_l = 0, base_b = 0;
+  if (l == 0) {
+    if (ml) base_l = atomicAdd(&counts[0], (uint32_t)__popcll(ml));
+    if (mb) base_b = atomicAdd(&counts[1], (uint32_t)__popcll(mb));
+  }
+  base_l = __shfl(base_l, 0, 64);
+  base_b = __shfl(base_b, 0, 64);
+  if (to_light) light[base_l + lanes_below(ml)] = v;
+  if (to_block) block[base_b + lanes_below(mb)] = v;
+}
+
+// partitions in descending mix32(q + repeat) order (mix32 is a bijection: no ties)
+__global__ void k_order_keys(uint32_t k, uint32_t repeat, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= k) return;
+  keys[q] = ~mix32(q + repeat);
+  vals[q] = q;
 }
 
 __global__ void k_compact(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint32_t n,
@@ -710,15 +750,6 @@ __global__ void k_compact(const uint32_t* __restrict__ flag, const uint64_t* __r
   if (v < n && flag[v]) out[pos[v]] = v;
 }
 
-__global__ void k_split3(const uint32_t* __restrict__ f_a, const uint32_t* __restrict__ f_b,
-                         const uint64_t* __restrict__ apos, const uint64_t* __restrict__ bpos, uint32_t n,
-                         uint32_t* __restrict__ la, uint32_t* __restrict__ lb, uint32_t* __restrict__ lc) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  if (f_a[v]) la[apos[v]] = v;
-  else if (f_b[v]) lb[bpos[v]] = v;
-  else lc[v - apos[v] - bpos[v]] = v;
-}
 
 __global__ void k_split(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint32_t n,
                         uint32_t* __restrict__ a, uint32_t* __restrict__ b) {
@@ -987,9 +1018,11 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
   const uint64_t s = a.hoff[e], t = a.hoff[e + 1];
   const uint64_t hl = t - s;
   const bool sorted = hl > CAS_SMALL && hl <= sort_max;
-  f_s[e] = (sorted && hl <= CAS_SORT_S) ? 1u : 0u;
-  f_m[e] = (sorted && hl > CAS_SORT_S) ? 1u : 0u;
-  f_big[e] = (hl > CAS_SMALL && !sorted) ? 1u : 0u;
+  if (f_s) {  // class flags (null once the level's classes are known)
+    f_s[e] = (sorted && hl <= CAS_SORT_S) ? 1u : 0u;
+    f_m[e] = (sorted && hl > CAS_SORT_S) ? 1u : 0u;
+    f_big[e] = (hl > CAS_SMALL && !sorted) ? 1u : 0u;
+  }
   if (hl > CAS_SMALL) return;
   uint32_t pt[CAS_SMALL], mv[CAS_SMALL], rk[CAS_SMALL];
   const uint32_t len = (uint32_t)(t - s);
@@ -1413,8 +1446,29 @@ struct PartitionLists {
 
 // Partition lists of every hedge.  want_inbound adds the destination counts and the
 // per-partition inbound sizes (the inbound-set constraint needs them; k-way does not).
-void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, PartitionLists& pl, bool want_inbound) {
-  const uint64_t P = L.e ? read_scalar(c, L.hoff + L.e) : 0;
+// The hedge classes of a level depend only on hedge lengths and k, so refine() computes them
+// in its first round and reuses them: for the partition-list kernels the pin count, the
+// hedges of more than PP_SMALL pins with LDS tables, those with HBM tables and their table
+// offsets; for the cascade the three classes of hedges above CAS_SMALL pins (for the
+// sort_max they were built with).
+struct LevelLayout {
+  bool ready = false;
+  uint64_t P = 0;
+  uint32_t nl = 0, ng = 0;
+  DBuf<uint32_t> large, glist;
+  DBuf<uint64_t> goff;
+  uint64_t gslots = 0;
+  uint32_t cas_sort_max = 0xFFFFFFFFu;
+  DBuf<uint32_t> cas_list[3];
+  uint32_t cas_n[3] = {0, 0, 0};
+};
+
+void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, PartitionLists& pl, bool want_inbound,
+                           LevelLayout* layout = nullptr) {
+  LevelLayout local;
+  LevelLayout& Y = layout ? *layout : local;
+  if (!Y.ready) Y.P = L.e ? read_scalar(c, L.hoff + L.e) : 0;
+  const uint64_t P = Y.P;
   pl.pp = DBuf<uint2>(c, P);
   pl.hinfo = DBuf<uint4>(c, L.e);
   pl.lambda = DBuf<uint32_t>(c, L.e);
@@ -1423,7 +1477,8 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
     pl.inbsz = DBuf<uint32_t>(c, std::max<uint32_t>(L.k, 1));
     memset_async(c, pl.inbsz.get(), 0, (uint64_t)L.k * sizeof(uint32_t));
   }
-  DBuf<uint32_t> lflag(c, L.e);
+  DBuf<uint32_t> lflag;
+  if (!Y.ready) lflag = DBuf<uint32_t>(c, L.e);
   if (want_inbound) {
     const uint32_t k_lds = L.k <= PP_HIST_CAP ? L.k : 0u;
     HGP_LAUNCH((k_pp_small<true>), ceil_div(L.e, 256), 256, k_lds * sizeof(uint32_t), c, L.hedges, L.hoff, L.srcc, part,
@@ -1433,32 +1488,40 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
     HGP_LAUNCH((k_pp_small<false>), ceil_div(L.e, 256), 256, 0, c, L.hedges, L.hoff, L.srcc, part, L.w, L.e,
                pl.pp.get(), (uint32_t*)nullptr, pl.lambda.get(), pl.hinfo.get(), lflag.get(), (uint32_t*)nullptr, 0u);
   }
-  DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
-  exclusive_scan(c, lflag.get(), pos.get(), L.e);
-  uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.e);
-  if (nl == 0) return;
-  DBuf<uint32_t> large(c, nl), small(c, L.e - nl);
-  // k_split puts flagged (large) entries first list
-  HGP_LAUNCH(k_split, ceil_div(L.e, 256), 256, 0, c, lflag.get(), pos.get(), L.e, large.get(), small.get());
   // Distinct partitions per hedge <= min(k, len).  Tables of up to PP_LDS_CAP slots sit in
   // LDS; hedges needing more (len and k both above ~2K) get HBM tables.
   const uint32_t lds_max_len = (PP_LDS_CAP - 64) / 2;
   const uint32_t cap = pp_table_cap(lds_max_len, L.k);  // <= PP_LDS_CAP
-  DBuf<uint32_t> glist;
-  uint32_t ng = 0;
-  if (L.k > lds_max_len) {
-    DBuf<uint32_t> gflag(c, nl);
-    DBuf<uint64_t> gpos(c, (uint64_t)nl + 1);
-    HGP_LAUNCH(k_flag_longer, ceil_div(nl, 256), 256, 0, c, large.get(), nl, L.hoff, (uint64_t)lds_max_len, gflag.get());
-    exclusive_scan(c, gflag.get(), gpos.get(), nl);
-    ng = (uint32_t)read_scalar(c, gpos.get() + nl);
-    if (ng) {
-      DBuf<uint32_t> lds_list(c, nl - ng);
-      glist = DBuf<uint32_t>(c, ng);
-      HGP_LAUNCH(k_split_list, ceil_div(nl, 256), 256, 0, c, gflag.get(), gpos.get(), large.get(), nl, glist.get(),
-                 lds_list.get());
-      large = std::move(lds_list);
-      nl -= ng;
+  if (!Y.ready) {
+    Y.ready = true;
+    DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
+    exclusive_scan(c, lflag.get(), pos.get(), L.e);
+    Y.nl = (uint32_t)read_scalar(c, pos.get() + L.e);
+    if (Y.nl == 0) return;
+    Y.large = DBuf<uint32_t>(c, Y.nl);
+    DBuf<uint32_t> small(c, L.e - Y.nl);
+    // k_split puts flagged (large) entries first list
+    HGP_LAUNCH(k_split, ceil_div(L.e, 256), 256, 0, c, lflag.get(), pos.get(), L.e, Y.large.get(), small.get());
+    if (L.k > lds_max_len) {
+      DBuf<uint32_t> gflag(c, Y.nl);
+      DBuf<uint64_t> gpos(c, (uint64_t)Y.nl + 1);
+      HGP_LAUNCH(k_flag_longer, ceil_div(Y.nl, 256), 256, 0, c, Y.large.get(), Y.nl, L.hoff, (uint64_t)lds_max_len,
+                 gflag.get());
+      exclusive_scan(c, gflag.get(), gpos.get(), Y.nl);
+      Y.ng = (uint32_t)read_scalar(c, gpos.get() + Y.nl);
+      if (Y.ng) {
+        DBuf<uint32_t> lds_list(c, Y.nl - Y.ng);
+        Y.glist = DBuf<uint32_t>(c, Y.ng);
+        HGP_LAUNCH(k_split_list, ceil_div(Y.nl, 256), 256, 0, c, gflag.get(), gpos.get(), Y.large.get(), Y.nl,
+                   Y.glist.get(), lds_list.get());
+        Y.large = std::move(lds_list);
+        Y.nl -= Y.ng;
+        DBuf<uint64_t> gcap(c, Y.ng);
+        Y.goff = DBuf<uint64_t>(c, (uint64_t)Y.ng + 1);
+        HGP_LAUNCH(k_pp_gcap, ceil_div(Y.ng, 256), 256, 0, c, Y.glist.get(), Y.ng, L.hoff, L.k, gcap.get());
+        exclusive_scan(c, gcap.get(), Y.goff.get(), Y.ng);
+        Y.gslots = read_scalar(c, Y.goff.get() + Y.ng);
+      }
     }
   }
   auto launch = [&](auto inb, auto gmem, const uint32_t* lst, uint32_t m, const uint64_t* goff, uint32_t* gtab) {
@@ -1470,39 +1533,27 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
   };
   using T = std::true_type;
   using F = std::false_type;
-  if (want_inbound) launch(T{}, F{}, large.get(), nl, nullptr, nullptr);
-  else launch(F{}, F{}, large.get(), nl, nullptr, nullptr);
-  if (ng) {
-    DBuf<uint64_t> gcap(c, ng), goff(c, (uint64_t)ng + 1);
-    HGP_LAUNCH(k_pp_gcap, ceil_div(ng, 256), 256, 0, c, glist.get(), ng, L.hoff, L.k, gcap.get());
-    exclusive_scan(c, gcap.get(), goff.get(), ng);
-    const uint64_t slots = read_scalar(c, goff.get() + ng);
-    DBuf<uint32_t> gtab(c, 3 * slots);
-    if (want_inbound) launch(T{}, T{}, glist.get(), ng, goff.get(), gtab.get());
-    else launch(F{}, T{}, glist.get(), ng, goff.get(), gtab.get());
+  if (want_inbound) launch(T{}, F{}, Y.large.get(), Y.nl, nullptr, nullptr);
+  else launch(F{}, F{}, Y.large.get(), Y.nl, nullptr, nullptr);
+  if (Y.ng) {
+    DBuf<uint32_t> gtab(c, 3 * Y.gslots);
+    if (want_inbound) launch(T{}, T{}, Y.glist.get(), Y.ng, Y.goff.get(), gtab.get());
+    else launch(F{}, T{}, Y.glist.get(), Y.ng, Y.goff.get(), gtab.get());
   }
 }
 
 uint32_t discount_of(uint32_t rep, uint32_t R) { return rep < R / 3 ? 1u : (rep < 2 * R / 3 ? 2u : NONE); }
 
-uint32_t mix32_host(uint32_t x) {
-  x ^= x >> 17;
-  x *= 0xED5AD4BBu;
-  x ^= x >> 11;
-  x *= 0xAC4C1B51u;
-  x ^= x >> 15;
-  x *= 0x31848BABu;
-  x ^= x >> 14;
-  return x;
-}
 
 // Algorithmic bytes of the gain kernels over `list` (evaluated only while the kernel is
 // profiled): per node 36 (partition, size, two offsets, move and gain out) + 20 per touching
 // hedge (hedge id and its 16-byte record: list start, list length, weight) + 8 per listed
 // partition entry (partition and pin count).
-__global__ void k_gain_bytes(const uint32_t* __restrict__ list, uint32_t m, const uint64_t* __restrict__ toff,
-                             const uint64_t* __restrict__ bound, unsigned long long* out) {
+__global__ void k_gain_bytes(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                             const uint64_t* __restrict__ toff, const uint64_t* __restrict__ bound,
+                             unsigned long long* out) {
   uint64_t b = 0;
+  const uint32_t m = *count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
     const uint32_t v = list[i];
     b += 36 + 20 * (toff[v + 1] - toff[v]) + 8 * bound[v];
@@ -1511,23 +1562,22 @@ __global__ void k_gain_bytes(const uint32_t* __restrict__ list, uint32_t m, cons
   if (lane_id() == 0 && b) atomicAdd(out, (unsigned long long)b);
 }
 
-uint64_t gain_bytes(Ctx& c, const uint32_t* list, uint32_t m, const uint64_t* toff, const uint64_t* bound) {
-  if (m == 0) return 0;
+uint64_t gain_bytes(Ctx& c, const uint32_t* list, const uint32_t* count, uint32_t n, const uint64_t* toff,
+                    const uint64_t* bound) {
+  if (n == 0) return 0;
   DBuf<unsigned long long> out(c, 1);
   memset_async(c, out.get(), 0, 8);
-  HGP_LAUNCH(k_gain_bytes, std::min<uint32_t>(ceil_div(m, 256), 1024), 256, 0, c, list, m, toff, bound, out.get());
+  HGP_LAUNCH(k_gain_bytes, std::min<uint32_t>(ceil_div(n, 256), 1024), 256, 0, c, list, count, toff, bound, out.get());
   return read_scalar(c, out.get());
 }
 
 void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int mode, uint32_t omega, uint32_t repeat,
                    uint32_t discount, const uint32_t* part, const uint32_t* psizes, uint32_t* moves, int64_t* gains) {
-  // hash order of partitions for the non-adjacent fallback
-  std::vector<uint32_t> order(L.k);
-  std::iota(order.begin(), order.end(), 0u);
-  std::sort(order.begin(), order.end(),
-            [&](uint32_t a, uint32_t b) { return mix32_host(a + repeat) > mix32_host(b + repeat); });
-  DBuf<uint32_t> dorder(c, std::max<uint32_t>(L.k, 1));
-  copy_h2d(c, dorder.get(), order.data(), (uint64_t)L.k * sizeof(uint32_t));
+  // hash order of partitions for the non-adjacent fallback, sorted on the device
+  const uint32_t kk = std::max<uint32_t>(L.k, 1);
+  DBuf<uint32_t> okeys(c, kk), dorder(c, kk);
+  HGP_LAUNCH(k_order_keys, ceil_div(L.k, 256), 256, 0, c, L.k, repeat, okeys.get(), dorder.get());
+  radix_sort(c, okeys.get(), dorder.get(), L.k, 32);
   GainArgs a;
   a.tch = L.tch;
   a.toff = L.toff;
@@ -1544,37 +1594,30 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   a.incc = mode == HGP_MODE_INCC;
   a.moves = moves;
   a.gains = gains;
+  // tiny nodes are scored by the router; the light and block lists and their counts stay on
+  // the device, and the persistent light and block kernels stride over them
   DBuf<uint64_t> bound(c, L.n);
-  DBuf<uint32_t> tiny(c, L.n), light(c, L.n);
-  HGP_LAUNCH(k_gain_bound, ceil_div(L.n, 256), 256, 0, c, L.tch, L.toff, pl.lambda.get(), L.n, L.k, bound.get(),
-             tiny.get(), light.get());
-  DBuf<uint64_t> tpos(c, (uint64_t)L.n + 1), pos(c, (uint64_t)L.n + 1);
-  exclusive_scan(c, tiny.get(), tpos.get(), L.n);
-  exclusive_scan(c, light.get(), pos.get(), L.n);
-  const uint32_t nt = (uint32_t)read_scalar(c, tpos.get() + L.n);
-  const uint32_t nl = (uint32_t)read_scalar(c, pos.get() + L.n);
-  uint32_t nh = L.n - nt - nl;
-  DBuf<uint32_t> tl(c, nt), ll(c, nl), hl(c, nh);
-  HGP_LAUNCH(k_split3, ceil_div(L.n, 256), 256, 0, c, tiny.get(), light.get(), tpos.get(), pos.get(), L.n, tl.get(),
-             ll.get(), hl.get());
-  HGP_LAUNCH_B(gain_bytes(c, tl.get(), nt, L.toff, bound.get()), k_gains_tiny, ceil_div(nt, 256), 256, 0, c, a,
-               tl.get(), nt);
+  DBuf<uint32_t> ll(c, L.n), bl(c, L.k > GAIN_LIGHT ? L.n : 1), counts(c, 2);
+  memset_async(c, counts.get(), 0, 8);
+  HGP_LAUNCH(k_gains_route, ceil_div(L.n, 256), 256, 0, c, a, L.n, bound.get(), ll.get(), bl.get(), counts.get());
   const uint32_t capmax = L.k <= GAIN_CAP ? std::max<uint32_t>(L.k, 1) : GAIN_CAP;
-  HGP_LAUNCH_B(gain_bytes(c, ll.get(), nl, L.toff, bound.get()), k_gains_light, ceil_div(nl, 4), 256, 4 * capmax * 12,
-               c, a, ll.get(), nl, bound.get(), capmax);
-  if (nh == 0) {
-    sync(c);  // `order` host buffer
+  const uint32_t lgrid = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID));
+  HGP_LAUNCH_B(gain_bytes(c, ll.get(), counts.get(), L.n, L.toff, bound.get()), k_gains_light, lgrid, 256,
+               4 * capmax * 12, c, a, ll.get(), counts.get(), bound.get(), capmax);
+  if (L.k <= GAIN_LIGHT) return;
+  DBuf<uint32_t> hl;
+  uint32_t nh = 0;
+  // workgroup per node with an LDS table; hashed tables that would overflow go to HBM
+  const uint32_t bgrid = std::max<uint32_t>(1, std::min<uint32_t>(L.n, BLOCK_GRID));
+  if (L.k <= GB_DIRECT) {
+    HGP_LAUNCH((k_gains_block<true>), bgrid, 1024, 0, c, a, bl.get(), counts.get() + 1, (uint32_t*)nullptr,
+               (uint32_t*)nullptr);
     return;
   }
-  // workgroup per node with an LDS table; hashed tables that would overflow go to HBM
-  DBuf<uint32_t> ovf(c, nh), novf(c, 1);
+  DBuf<uint32_t> ovf(c, L.n), novf(c, 1);
   memset_async(c, novf.get(), 0, 4);
-  if (L.k <= GB_DIRECT) {
-    HGP_LAUNCH((k_gains_block<true>), nh, 1024, 0, c, a, hl.get(), ovf.get(), novf.get());
-  } else {
-    HGP_LAUNCH((k_gains_block<false>), nh, 1024, 0, c, a, hl.get(), ovf.get(), novf.get());
-  }
-  nh = (uint32_t)read_scalar(c, novf.get());  // also completes the `order` upload
+  HGP_LAUNCH((k_gains_block<false>), bgrid, 1024, 0, c, a, bl.get(), counts.get() + 1, ovf.get(), novf.get());
+  nh = (uint32_t)read_scalar(c, novf.get());
   if (nh == 0) return;
   hl = std::move(ovf);
   DBuf<uint64_t> caps(c, nh);
@@ -1674,12 +1717,13 @@ struct RoundState {
 };
 
 void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t delta, uint32_t R, uint32_t rep,
-                   bool chainup, const uint32_t* part, const uint32_t* psizes, RoundState& st) {
+                   bool chainup, const uint32_t* part, const uint32_t* psizes, RoundState& st,
+                   LevelLayout* layout = nullptr) {
   const uint32_t n = L.n;
   const uint32_t g = ceil_div(n, 256);
   const bool incc = mode == HGP_MODE_INCC;
   PartitionLists pl;
-  build_partition_lists(c, L, part, pl, incc);
+  build_partition_lists(c, L, part, pl, incc, layout);
   st.moves = DBuf<uint32_t>(c, n);
   st.gains = DBuf<int64_t>(c, n);
   st.ranks = DBuf<uint32_t>(c, n);
@@ -1709,18 +1753,31 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
     const int crb = std::max(1, bit_width(V));  // rank + 1 <= V
     const int cpb = std::max(1, bit_width(L.k));
     const uint32_t sort_max = cpb + crb + 11 < 64 ? CAS_SORT : 0u;
-    DBuf<uint32_t> fl[3] = {DBuf<uint32_t>(c, L.e), DBuf<uint32_t>(c, L.e), DBuf<uint32_t>(c, L.e)};
+    LevelLayout local;
+    LevelLayout& Y = layout ? *layout : local;
+    const bool known = Y.cas_sort_max == sort_max;
+    DBuf<uint32_t> fl[3];
+    if (!known)
+      for (int cls = 0; cls < 3; ++cls) fl[cls] = DBuf<uint32_t>(c, L.e);
     HGP_LAUNCH(k_cascade_small, ceil_div(L.e, 256), 256, 0, c, a, L.e, sort_max, fl[0].get(), fl[1].get(), fl[2].get());
-    DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
+    if (!known) {
+      Y.cas_sort_max = sort_max;
+      DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
+      for (int cls = 0; cls < 3; ++cls) {
+        exclusive_scan(c, fl[cls].get(), pos.get(), L.e);
+        Y.cas_n[cls] = (uint32_t)read_scalar(c, pos.get() + L.e);
+        Y.cas_list[cls] = DBuf<uint32_t>(c, Y.cas_n[cls]);
+        if (Y.cas_n[cls])
+          HGP_LAUNCH(k_compact, ceil_div(L.e, 256), 256, 0, c, fl[cls].get(), pos.get(), L.e, Y.cas_list[cls].get());
+      }
+    }
     for (int cls = 0; cls < 3; ++cls) {
-      exclusive_scan(c, fl[cls].get(), pos.get(), L.e);
-      const uint32_t m = (uint32_t)read_scalar(c, pos.get() + L.e);
+      const uint32_t m = Y.cas_n[cls];
       if (m == 0) continue;
-      DBuf<uint32_t> lst(c, m);
-      HGP_LAUNCH(k_compact, ceil_div(L.e, 256), 256, 0, c, fl[cls].get(), pos.get(), L.e, lst.get());
-      if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 4 * CAS_SORT_S>), m, 64, 0, c, a, lst.get(), crb);
-      else if (cls == 1) HGP_LAUNCH((k_cascade_sorted<CAS_SORT, CAS_EV>), m, 64, 0, c, a, lst.get(), crb);
-      else HGP_LAUNCH(k_cascade_large, m, 64, 0, c, a, lst.get());
+      const uint32_t* lst = Y.cas_list[cls].get();
+      if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 4 * CAS_SORT_S>), m, 64, 0, c, a, lst, crb);
+      else if (cls == 1) HGP_LAUNCH((k_cascade_sorted<CAS_SORT, CAS_EV>), m, 64, 0, c, a, lst, crb);
+      else HGP_LAUNCH(k_cascade_large, m, 64, 0, c, a, lst);
     }
     inclusive_scan(c, (const int64_t*)st.seq.get(), (int64_t*)st.seq.get(), V);
   }
@@ -1799,9 +1856,10 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
 void refine(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t delta, uint32_t R, bool final_counts,
             uint32_t* part, uint32_t* psizes, uint32_t* pinbound) {
   bool chainup = false;
+  LevelLayout layout;
   for (uint32_t rep = 0; rep < R; ++rep) {
     RoundState st;
-    analyze_round(c, L, mode, omega, delta, R, rep, chainup, part, psizes, st);
+    analyze_round(c, L, mode, omega, delta, R, rep, chainup, part, psizes, st, &layout);
     int32_t sv = 1, iv = 1;
     int64_t gain = 0;
     bool apply = false;
