@@ -580,32 +580,55 @@ struct BisectorT {
 
 // Restrict h to `nodes` (ascending ids): hedges keep their pins inside the subset, in
 // hedge order, and those left with fewer than two pins are dropped.  Local ids follow the
-// global order, so the pins of every hedge stay sorted.  One sequential sweep over the pins.
-inline Hypergraph subgraph(const Hypergraph& h, const std::vector<uint32_t>& nodes) {
+// global order, so the pins of every hedge stay sorted.  Two sweeps over contiguous hedge
+// ranges on `threads` threads (count, then fill at the range's own offsets), so the result
+// is the same for any thread count.
+inline Hypergraph subgraph(const Hypergraph& h, const std::vector<uint32_t>& nodes, uint32_t threads = 1) {
   Hypergraph s;
   s.n = (uint32_t)nodes.size();
   std::vector<uint32_t> local(h.n, UINT32_MAX);
   for (uint32_t i = 0; i < s.n; ++i) local[nodes[i]] = i;
   s.vw.resize(s.n);
   for (uint32_t i = 0; i < s.n; ++i) s.vw[i] = h.vw[nodes[i]];
-  s.off.push_back(0);
-  s.pins.reserve(h.pins.size() / 2 + 16);
   const uint32_t m = (uint32_t)h.off.size() - 1;
-  for (uint32_t e = 0; e < m; ++e) {
-    const uint64_t start = s.pins.size();
-    for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
-      const uint32_t x = local[h.pins[j]];
-      if (x != UINT32_MAX) s.pins.push_back(x);
+  threads = std::max<uint32_t>(1, std::min<uint32_t>(threads, (uint32_t)(h.pins.size() >> 18) + 1));
+  std::vector<uint64_t> ch(threads + 1, 0), cp(threads + 1, 0);
+  parallel_chunks(m, threads, [&](uint32_t t, uint64_t lo, uint64_t hi) {
+    uint64_t nh = 0, np = 0;
+    for (uint64_t e = lo; e < hi; ++e) {
+      uint32_t c = 0;
+      for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) c += local[h.pins[j]] != UINT32_MAX;
+      if (c >= 2) {
+        nh++;
+        np += c;
+      }
     }
-    if (s.pins.size() - start < 2) {
-      s.pins.resize(start);
-      continue;
-    }
-    s.off.push_back(s.pins.size());
-    s.w.push_back(h.w[e]);
+    ch[t + 1] = nh;
+    cp[t + 1] = np;
+  });
+  for (uint32_t t = 0; t < threads; ++t) {
+    ch[t + 1] += ch[t];
+    cp[t + 1] += cp[t];
   }
-  s.pins.shrink_to_fit();
-  build_incidence(s);
+  s.off.resize(ch[threads] + 1);
+  s.w.resize(ch[threads]);
+  s.pins.resize(cp[threads]);
+  s.off[0] = 0;
+  parallel_chunks(m, threads, [&](uint32_t t, uint64_t lo, uint64_t hi) {
+    uint64_t k = ch[t], q = cp[t];
+    for (uint64_t e = lo; e < hi; ++e) {
+      uint32_t c = 0;
+      for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) c += local[h.pins[j]] != UINT32_MAX;
+      if (c < 2) continue;  // writes stay inside the range counted for this chunk
+      for (uint64_t j = h.off[e]; j < h.off[e + 1]; ++j) {
+        const uint32_t x = local[h.pins[j]];
+        if (x != UINT32_MAX) s.pins[q++] = x;
+      }
+      s.w[k] = h.w[e];
+      s.off[++k] = q;
+    }
+  });
+  build_incidence(s, threads);
   return s;
 }
 
@@ -699,8 +722,11 @@ inline void bisect_recursive(const Hypergraph& h, const std::vector<uint32_t>& g
     g[best_side[v]].push_back(gids[v]);
   }
   res.clear();
+  // threads for building a half: the cores left to each of the 2^(depth+1) concurrent halves
+  const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const uint32_t sub_threads = par ? std::max<uint32_t>(1, hw >> (depth + 1)) : 1u;
   auto half = [&](int s) {
-    Hypergraph sub = subgraph(h, l[s]);
+    Hypergraph sub = subgraph(h, l[s], sub_threads);
     bisect_recursive(sub, g[s], s == 0 ? k0 : k1, s == 0 ? first : first + k0, eps_b, omega, opt, part, depth + 1);
   };
   if (par) {
