@@ -723,15 +723,25 @@ __global__ void __launch_bounds__(256) k_gains_route(GainArgs a, uint32_t n, uin
   if (rest) bound[v] = b;
   const bool to_light = rest && (b <= GAIN_LIGHT || a.k <= GAIN_LIGHT);
   const bool to_block = rest && !to_light;
-  const uint32_t l = lane_id();
+  // list slots: one atomic per workgroup and list, then wave and lane offsets
+  __shared__ uint32_t s_cnt[2][4], s_base[2];
+  const uint32_t l = lane_id(), wv = threadIdx.x / 64;
   const uint64_t ml = __ballot(to_light), mb = __ballot(to_block);
-  uint32_t base_l = 0, base_b = 0;
   if (l == 0) {
-    if (ml) base_l = atomicAdd(&counts[0], (uint32_t)__popcll(ml));
-    if (mb) base_b = atomicAdd(&counts[1], (uint32_t)__popcll(mb));
+    s_cnt[0][wv] = (uint32_t)__popcll(ml);
+    s_cnt[1][wv] = (uint32_t)__popcll(mb);
   }
-  base_l = __shfl(base_l, 0, 64);
-  base_b = __shfl(base_b, 0, 64);
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const uint32_t t = s_cnt[threadIdx.x][0] + s_cnt[threadIdx.x][1] + s_cnt[threadIdx.x][2] + s_cnt[threadIdx.x][3];
+    s_base[threadIdx.x] = t ? atomicAdd(&counts[threadIdx.x], t) : 0u;
+  }
+  __syncthreads();
+  uint32_t base_l = s_base[0], base_b = s_base[1];
+  for (uint32_t w = 0; w < wv; ++w) {
+    base_l += s_cnt[0][w];
+    base_b += s_cnt[1][w];
+  }
   if (to_light) light[base_l + lanes_below(ml)] = v;
   if (to_block) block[base_b + lanes_below(mb)] = v;
 }
@@ -788,11 +798,19 @@ __global__ void k_scatter_ranks(const uint32_t* __restrict__ vals, uint32_t n, u
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < n) ranks[vals[r]] = r;
 }
-__global__ void k_count_valid(const uint32_t* __restrict__ moves, uint32_t n, uint32_t* cnt) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t ok = (v < n && moves[v] != NONE) ? 1u : 0u;
-  uint64_t m = __ballot(ok);
-  if (lane_id() == 0 && m) atomicAdd(cnt, (uint32_t)__popcll(m));
+// grid-stride count, reduced per workgroup: one atomic per workgroup
+__global__ void __launch_bounds__(256) k_count_valid(const uint32_t* __restrict__ moves, uint32_t n, uint32_t* cnt) {
+  __shared__ uint32_t s_part[4];
+  uint32_t c = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    c += moves[v] != NONE ? 1u : 0u;
+  c = wave_sum(c);
+  if (lane_id() == 0) s_part[threadIdx.x / 64] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    if (t) atomicAdd(cnt, t);
+  }
 }
 
 // chaining: edges in sorted space j (by source partition asc, valid first, gain desc)
@@ -1733,7 +1751,7 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
   {
     DBuf<uint32_t> cnt(c, 1);
     memset_async(c, cnt.get(), 0, 4);
-    HGP_LAUNCH(k_count_valid, g, 256, 0, c, st.moves.get(), n, cnt.get());
+    HGP_LAUNCH(k_count_valid, std::min<uint32_t>(g, 1024), 256, 0, c, st.moves.get(), n, cnt.get());
     st.V = read_scalar(c, cnt.get());
   }
   const uint32_t V = st.V;

@@ -85,6 +85,8 @@ def serve_jobs(n_jobs: int, max_pins: float, seed: int):
     rng = np.random.default_rng(seed)
     jobs = []
     for i in range(n_jobs):
+        if i % 32 == 0:
+            print(f"# generating job {i}", file=sys.stderr, flush=True)
         pins = float(np.exp(rng.uniform(np.log(1e4), np.log(max_pins))))
         if i % 2 == 0:
             e = max(64, int(pins / 4))
@@ -113,7 +115,8 @@ def serve_jobs(n_jobs: int, max_pins: float, seed: int):
 
 def serve_bench(args) -> dict:
     """Plain sequential runs of the job stream, then the same jobs through the continuous-
-    batching service with Poisson arrivals at twice the plain service rate."""
+    batching service with Poisson arrivals at `load` times the plain service rate (the
+    speedup over plain runs cannot exceed about `load`)."""
     jobs = serve_jobs(args.jobs, args.max_pins, args.seed)
     total_pins = sum(p for _, _, p in jobs)
     print(f"# serving stream: {len(jobs)} jobs, {total_pins} pins", file=sys.stderr, flush=True)
@@ -121,7 +124,9 @@ def serve_bench(args) -> dict:
     dev = H.Device(0)
     plain_parts = []
     t0 = time.perf_counter()
-    for hg, prm, _ in jobs:
+    for i, (hg, prm, _) in enumerate(jobs):
+        if i % 32 == 0:
+            print(f"# plain: job {i}", file=sys.stderr, flush=True)
         n, _, _ = hg.sizes()
         part = np.zeros(n, np.uint32)
         st = H.RunStats()
@@ -131,7 +136,7 @@ def serve_bench(args) -> dict:
     t_plain = time.perf_counter() - t0
     dev.close()
     print(f"# plain sequential: {t_plain:.2f} s", file=sys.stderr, flush=True)
-    rate = 2.0 * len(jobs) / t_plain  # offered load: twice what plain runs sustain
+    rate = args.load * len(jobs) / t_plain  # offered load: a multiple of what plain runs sustain
     arrivals = np.cumsum(np.random.default_rng(args.seed + 7).exponential(1.0 / rate, len(jobs)))
     srv = H.Server(0, workers=args.workers)
     ids = []
@@ -143,6 +148,8 @@ def serve_bench(args) -> dict:
         ids.append(srv.submit(hg, prm))
     lat, same = [], 0
     for i, j in enumerate(ids):
+        if i % 32 == 0:
+            print(f"# served: waiting for job {i}", file=sys.stderr, flush=True)
         part, st, l, q = srv.wait(j)
         lat.append(l)
         same += int(np.array_equal(part, plain_parts[i]))
@@ -156,7 +163,7 @@ def serve_bench(args) -> dict:
             "arrival_rate_per_s": rate, "results_equal_to_plain": same, "data": "synthetic",
             "config": {"workload": f"{len(jobs)} jobs, pins log-uniform in [1e4, {args.max_pins:g}], half random "
                                    "(|e| = 2 + Geom(1/2)), half layered SNN (fan-out 64); modes k=2/8/32 eps 0.03 "
-                                   "or -m 1024 4096 1048576; Poisson arrivals at 2x the plain service rate"}}
+                                   f"or -m 1024 4096 1048576; Poisson arrivals at {args.load:g}x the plain service rate"}}
 
 
 def main() -> int:
@@ -169,7 +176,8 @@ def main() -> int:
     ap.add_argument("--serve", action="store_true", help="run the serving-stream benchmark instead")
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--max-pins", type=float, default=2e6)
-    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--load", type=float, default=4.0, help="offered load of --serve, in multiples of the plain rate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain device copy of 1 GiB (for rocprofv3 --pmc counter calibration)")
     args = ap.parse_args()
