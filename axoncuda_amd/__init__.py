@@ -27,6 +27,9 @@ _i32p = C.POINTER(C.c_int32)
 _vp = C.c_void_p
 
 
+HGP_IP_HOST, HGP_IP_DEVICE = 0, 1
+
+
 class Params(C.Structure):
     _fields_ = [
         ("mode", C.c_int32),
@@ -42,6 +45,7 @@ class Params(C.Structure):
         ("device_touching_construction", C.c_int32),
         ("initial_partitions_merge", C.c_int32),
         ("verbose", C.c_int32),
+        ("initial_partitioner", C.c_int32),
     ]
 
 

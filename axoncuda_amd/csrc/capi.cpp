@@ -126,6 +126,7 @@ void hgp_default_params(hgp_params* p) {
   p->device_touching_construction = 0;
   p->initial_partitions_merge = 0;
   p->verbose = 0;
+  p->initial_partitioner = HGP_IP_HOST;
 }
 
 hgp_status hgp_ctx_create(int device, uint64_t hbm_budget, hgp_ctx** out) {

@@ -35,6 +35,7 @@ void print_help() {
       "  -smh <lvl>  Recover device memory by migrating to the host the pre-coarsening hypergraph up the provided level\n"
       "  -dtc        When set, construct touching sets on the device, rather than on the host (faster - uses more device memory)\n"
       "  -ipm        When set, initial partitions are greedily merged to reduce their number (and save memory - minor quality loss)\n"
+      "  -ip <where> k-way initial partitioner without mtkahypar: 'host' (recursive bisection, default) or 'device' (label propagation)\n"
       "  -v <lvl>    Set the verbosity level: 0 metrics only, 1 steps and phases, 2 kernel launches, 3  \n"
       "  -h          Show this help message\n");
 }
@@ -140,6 +141,12 @@ int main(int argc, char** argv) {
       prm.device_touching_construction = 1;
     } else if (a == "-ipm") {
       prm.initial_partitions_merge = 1;
+    } else if (a == "-ip") {
+      if (i + 1 >= argc) { std::fprintf(stderr, "Error: -ip requires 'host' or 'device'\n"); return 1; }
+      const std::string w = argv[++i];
+      if (w == "host") prm.initial_partitioner = HGP_IP_HOST;
+      else if (w == "device") prm.initial_partitioner = HGP_IP_DEVICE;
+      else { std::fprintf(stderr, "Error: -ip requires 'host' or 'device'\n"); return 1; }
     } else if (a == "-v") {
       if (i + 1 >= argc) { std::fprintf(stderr, "Error: -v requires a positive value between 0 and 3\n"); return 1; }
       verbosity = (int)stoul_or_die(argv[++i], "-v");

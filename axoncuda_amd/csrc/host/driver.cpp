@@ -84,14 +84,21 @@ bool run_mtkahypar(const std::string& bin, const hgp_ip::Hypergraph& h, uint32_t
 
 void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
                        const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, uint32_t*& part_out,
-                       uint32_t*& psizes_out, double* ms) {
+                       uint32_t*& psizes_out, double* ms, const RefineLevel* device_level) {
   auto t0 = std::chrono::high_resolution_clock::now();
+  static const std::string mtk = find_mtkahypar();
+  if (device_level && mtk.empty()) {  // device label-propagation partitioner (opt-in)
+    RefineLevel RL = *device_level;
+    RL.k = k;
+    initial_partition_device(c, RL, omega, DEVICE_IP_STARTS, DEVICE_IP_ROUNDS, part_out, psizes_out);
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+    return;
+  }
   const auto t1 = t0;
   hgp_ip::Hypergraph h;
   ip_input_device(c, hedges, hoff, w, sizes, n, e, h);
   const auto t2 = std::chrono::high_resolution_clock::now();
   std::vector<uint32_t> part(n, 0), psizes(k, 0);
-  static const std::string mtk = find_mtkahypar();
   bool done = false;
   if (!mtk.empty()) {
     done = run_mtkahypar(mtk, h, k, eps, part.data());
@@ -213,10 +220,13 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
     st.level_nodes[depth] = n;
     st.level_pins[depth] = L.P;
     if (c.verbose) fprintf(stderr, "Coarsening level %u, remaining nodes=%u, pins=%llu\n", depth, n, (unsigned long long)L.P);
+    const RefineLevel ipl{L.n, E, prm.kway, L.hedges.get(), L.hoff.get(), L.srcc.get(), L.tch.get(), L.toff.get(),
+                          L.inb.get(), w.get(), L.sizes.get()};
+    const RefineLevel* ip_dev = prm.initial_partitioner == HGP_IP_DEVICE ? &ipl : nullptr;
     if (kw && n < 8192) {  // KWAY_INIT_UPPER_THREASHOLD (headers/defines.cuh:35)
       uint32_t *pp, *ps;
       initial_partition(c, L.hedges.get(), L.hoff.get(), w.get(), n, E, L.sizes.get(), prm.kway, prm.epsilon, omega, pp, ps,
-                        &st.ip_ms);
+                        &st.ip_ms, ip_dev);
       part = DBuf<uint32_t>();
       part.ctx = &c, part.ptr = pp, part.n = n;
       psizes.ctx = &c, psizes.ptr = ps, psizes.n = prm.kway;
@@ -240,7 +250,7 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
     if (kw && (ratio > 0.95f || nn < 1024)) {  // KWAY_INIT_SHRINK_RATIO_LIMIT / LOWER_THREASHOLD
       uint32_t *pp, *ps;
       initial_partition(c, L.hedges.get(), L.hoff.get(), w.get(), n, E, L.sizes.get(), prm.kway, prm.epsilon, omega, pp, ps,
-                        &st.ip_ms);
+                        &st.ip_ms, ip_dev);
       part.ctx = &c, part.ptr = pp, part.n = n;
       psizes.ctx = &c, psizes.ptr = ps, psizes.n = prm.kway;
       K = max_parts;

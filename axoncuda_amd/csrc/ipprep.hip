@@ -128,20 +128,24 @@ __global__ void k_ipp_out_meta(const uint32_t* __restrict__ rep, const uint64_t*
   atomicAdd(&ow[u], (unsigned long long)wfx[x]);
 }
 
-// pins of the unique hedges at their output offsets, plus the (pin, hedge) incidence keys
+// pins of the unique hedges at their output offsets
 __global__ void k_ipp_out_pins(const uint32_t* __restrict__ rep, const uint64_t* __restrict__ uid, uint32_t e,
                                const uint32_t* __restrict__ pins, const uint64_t* __restrict__ start,
-                               const uint32_t* __restrict__ len, const uint64_t* __restrict__ ooff, int hb,
-                               uint32_t* __restrict__ opins, uint64_t* __restrict__ ikeys) {
+                               const uint32_t* __restrict__ len, const uint64_t* __restrict__ ooff,
+                               uint32_t* __restrict__ opins) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= e || rep[x] != x) return;
   const uint64_t u = uid[x];
   const uint64_t o = ooff[u];
-  for (uint32_t i = 0; i < len[x]; ++i) {
-    const uint32_t v = pins[start[x] + i];
-    opins[o + i] = v;
-    ikeys[o + i] = ((uint64_t)v << hb) | u;
-  }
+  for (uint32_t i = 0; i < len[x]; ++i) opins[o + i] = pins[start[x] + i];
+}
+
+// (pin, hedge) incidence keys of the merged hypergraph
+__global__ void k_ipp_inc_keys(const uint64_t* __restrict__ ooff, const uint32_t* __restrict__ opins, uint32_t u,
+                               int hb, uint64_t* __restrict__ ikeys) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= u) return;
+  for (uint64_t i = ooff[x]; i < ooff[x + 1]; ++i) ikeys[i] = ((uint64_t)opins[i] << hb) | x;
 }
 
 __global__ void k_ipp_inc(const uint64_t* __restrict__ ikeys, uint64_t p, uint64_t hmask, uint32_t* __restrict__ inc) {
@@ -151,23 +155,31 @@ __global__ void k_ipp_inc(const uint64_t* __restrict__ ikeys, uint64_t p, uint64
 
 }  // namespace
 
-void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
-                     uint32_t n, uint32_t e, hgp_ip::Hypergraph& h) {
-  h = hgp_ip::Hypergraph();
-  h.n = n;
-  h.vw.resize(n);
-  if (n) {
-    std::vector<uint32_t> s(n);
-    copy_d2h(c, s.data(), sizes, (uint64_t)n * 4);
-    for (uint32_t v = 0; v < n; ++v) h.vw[v] = s[v];
-  }
-  h.off.assign(1, 0);
-  h.ioff.assign((uint64_t)n + 1, 0);
+namespace {
+
+__global__ void k_ipp_float_w(const unsigned long long* __restrict__ ow, uint32_t m, float* __restrict__ wf) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) wf[i] = (float)((double)ow[i] / 262144.0);
+}
+
+// Steps 1-3: sorted distinct pins, representatives, unique hedges in order of first
+// appearance with their pins (opins, ooff) and summed fixed-point weights (ow); u = count.
+struct Merged {
+  DBuf<uint32_t> opins;
+  DBuf<uint64_t> ooff;
+  DBuf<unsigned long long> ow;
+  uint32_t u = 0;
+  uint64_t op = 0;
+};
+
+void merge_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
+                  Merged& out) {
+  out = Merged();
   if (e == 0 || n == 0) return;
   const uint64_t p = read_scalar(c, hoff + e);
   const int pb = std::max(1, bit_width(n - 1));
   const int eb = std::max(1, bit_width(e - 1));
-  HGP_REQUIRE(pb + eb <= 64, HGP_E_CAPACITY, "initial partition input: hedge and node ids exceed 64 key bits");
+  HGP_REQUIRE(pb + eb <= 64, HGP_E_CAPACITY, "hedge merge: hedge and node ids exceed 64 key bits");
   const uint32_t ge = ceil_div(e, 256);
   // 1. sorted distinct pins per hedge
   DBuf<uint64_t> keys(c, std::max<uint64_t>(p, 1));
@@ -181,6 +193,7 @@ void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const
   HGP_LAUNCH(k_ipp_compact, ceil_div(p, 256), 256, 0, c, keys.get(), keep.get(), kpos.get(), p,
              ((uint64_t)1 << pb) - 1, pins.get());
   keep.reset();
+  keys.reset();
   // 2. per hedge: hash, range, weight; hedges of >= 2 pins sorted by (hash, id)
   DBuf<uint64_t> hs(c, e), start(c, e);
   DBuf<uint32_t> len(c, e), wfx(c, e), live(c, e);
@@ -203,19 +216,61 @@ void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const
   DBuf<uint64_t> uid(c, (uint64_t)e + 1);
   exclusive_scan(c, first.get(), uid.get(), e);
   const uint32_t u = (uint32_t)read_scalar(c, uid.get() + e);
-  DBuf<uint64_t> olen(c, u), ooff(c, (uint64_t)u + 1);
-  DBuf<unsigned long long> ow(c, u);
-  memset_async(c, ow.get(), 0, (uint64_t)u * 8);
-  HGP_LAUNCH(k_ipp_out_meta, ge, 256, 0, c, rep.get(), uid.get(), e, len.get(), wfx.get(), olen.get(), ow.get());
-  exclusive_scan(c, olen.get(), ooff.get(), u);
-  const uint64_t op = read_scalar(c, ooff.get() + u);
-  DBuf<uint32_t> opins(c, op);
-  DBuf<uint64_t> ikeys(c, op);
+  DBuf<uint64_t> olen(c, u);
+  out.ooff = DBuf<uint64_t>(c, (uint64_t)u + 1);
+  out.ow = DBuf<unsigned long long>(c, u);
+  memset_async(c, out.ow.get(), 0, (uint64_t)u * 8);
+  HGP_LAUNCH(k_ipp_out_meta, ge, 256, 0, c, rep.get(), uid.get(), e, len.get(), wfx.get(), olen.get(), out.ow.get());
+  exclusive_scan(c, olen.get(), out.ooff.get(), u);
+  out.op = read_scalar(c, out.ooff.get() + u);
+  out.u = u;
+  out.opins = DBuf<uint32_t>(c, out.op);
+  HGP_LAUNCH(k_ipp_out_pins, ge, 256, 0, c, rep.get(), uid.get(), e, pins.get(), start.get(), len.get(),
+             out.ooff.get(), out.opins.get());
+}
+
+}  // namespace
+
+void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
+                        uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out) {
+  Merged M;
+  merge_hedges(c, hedges, hoff, w, n, e, M);
+  e_out = M.u;
+  if (M.u == 0) {
+    M.ooff = DBuf<uint64_t>(c, 1);
+    memset_async(c, M.ooff.get(), 0, 8);
+  }
+  DBuf<float> wf(c, std::max<uint32_t>(M.u, 1));
+  if (M.u) HGP_LAUNCH(k_ipp_float_w, ceil_div(M.u, 256), 256, 0, c, M.ow.get(), M.u, wf.get());
+  if (!M.opins.get()) M.opins = DBuf<uint32_t>(c, 1);
+  hedges_out = M.opins.release();
+  hoff_out = M.ooff.release();
+  w_out = wf.release();
+}
+
+void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
+                     uint32_t n, uint32_t e, hgp_ip::Hypergraph& h) {
+  h = hgp_ip::Hypergraph();
+  h.n = n;
+  h.vw.resize(n);
+  if (n) {
+    std::vector<uint32_t> s(n);
+    copy_d2h(c, s.data(), sizes, (uint64_t)n * 4);
+    for (uint32_t v = 0; v < n; ++v) h.vw[v] = s[v];
+  }
+  h.off.assign(1, 0);
+  h.ioff.assign((uint64_t)n + 1, 0);
+  Merged M;
+  merge_hedges(c, hedges, hoff, w, n, e, M);
+  const uint32_t u = M.u;
+  if (u == 0) return;
+  const uint64_t op = M.op;
+  const int pb = std::max(1, bit_width(n - 1));
   const int hb = std::max(1, bit_width(u - 1));
   HGP_REQUIRE(pb + hb <= 64, HGP_E_CAPACITY, "initial partition input: hedge and node ids exceed 64 key bits");
-  HGP_LAUNCH(k_ipp_out_pins, ge, 256, 0, c, rep.get(), uid.get(), e, pins.get(), start.get(), len.get(), ooff.get(),
-             hb, opins.get(), ikeys.get());
   // 4. incidence: hedges of every node in ascending order
+  DBuf<uint64_t> ikeys(c, op);
+  HGP_LAUNCH(k_ipp_inc_keys, ceil_div(u, 256), 256, 0, c, M.ooff.get(), M.opins.get(), u, hb, ikeys.get());
   radix_sort(c, ikeys.get(), nullptr, op, pb + hb);
   DBuf<uint32_t> inc(c, op);
   DBuf<uint64_t> ioff(c, (uint64_t)n + 1);
@@ -225,9 +280,9 @@ void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const
   h.w.resize(u);
   h.pins.resize(op);
   h.inc.resize(op);
-  copy_d2h(c, h.off.data(), ooff.get(), ((uint64_t)u + 1) * 8);
-  copy_d2h(c, h.w.data(), ow.get(), (uint64_t)u * 8);
-  copy_d2h(c, h.pins.data(), opins.get(), op * 4);
+  copy_d2h(c, h.off.data(), M.ooff.get(), ((uint64_t)u + 1) * 8);
+  copy_d2h(c, h.w.data(), M.ow.get(), (uint64_t)u * 8);
+  copy_d2h(c, h.pins.data(), M.opins.get(), op * 4);
   copy_d2h(c, h.inc.data(), inc.get(), op * 4);
   copy_d2h(c, h.ioff.data(), ioff.get(), ((uint64_t)n + 1) * 8);
 }

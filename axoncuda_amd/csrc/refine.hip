@@ -30,6 +30,7 @@ constexpr uint32_t PP_HIST_CAP = 8192;  // partitions whose inbound counts aggre
 constexpr uint32_t PP_LDS_CAP = 4096;  // 3 x 16 KiB of dynamic LDS per wave
 constexpr uint32_t GAIN_CAP = 1024;
 constexpr uint32_t GAIN_LIGHT = 512;
+constexpr uint32_t GAIN_BLOCK_MIN = 4096;  // entries above which a node takes a workgroup even for small k
 constexpr uint32_t GAIN_TINY = 16;  // partition entries handled by one thread in registers
 constexpr uint32_t CAS_SMALL = 16;
 constexpr uint32_t CAS_LDS = 2048;
@@ -707,8 +708,8 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
 
 // Routes every node by bound = partition entries listed by its touching hedges: bound <=
 // GAIN_TINY is scored right here (thread per node, registers); the others are appended to
-// the light list (wave per node; all of them when k <= GAIN_LIGHT) or the block list
-// (workgroup per node), with the counts left on the device.  List order varies from run to
+// the light list (wave per node; up to GAIN_BLOCK_MIN entries when k <= GAIN_LIGHT) or the
+// block list (workgroup per node), with the counts left on the device.  List order varies from run to
 // run; each node's result does not depend on it.
 __global__ void __launch_bounds__(256) k_gains_route(GainArgs a, uint32_t n, uint64_t* __restrict__ bound,
                                                      uint32_t* __restrict__ light, uint32_t* __restrict__ block,
@@ -721,7 +722,7 @@ __global__ void __launch_bounds__(256) k_gains_route(GainArgs a, uint32_t n, uin
   if (tiny) gains_tiny_node(a, v);
   const bool rest = v < n && !tiny;
   if (rest) bound[v] = b;
-  const bool to_light = rest && (b <= GAIN_LIGHT || a.k <= GAIN_LIGHT);
+  const bool to_light = rest && (b <= GAIN_LIGHT || (a.k <= GAIN_LIGHT && b <= GAIN_BLOCK_MIN));
   const bool to_block = rest && !to_light;
   // list slots: one atomic per workgroup and list, then wave and lane offsets
   __shared__ uint32_t s_cnt[2][4], s_base[2];
@@ -744,6 +745,70 @@ __global__ void __launch_bounds__(256) k_gains_route(GainArgs a, uint32_t n, uin
   }
   if (to_light) light[base_l + lanes_below(ml)] = v;
   if (to_block) block[base_b + lanes_below(mb)] = v;
+}
+
+// ---- Jacobi label propagation (device initial partitioner) ------------------------------
+// movers: positive-gain proposals, keyed for a sort by gain descending (ties: node id)
+__global__ void k_jacobi_movers(const uint32_t* __restrict__ moves, const int64_t* __restrict__ gains, uint32_t n,
+                                uint32_t* __restrict__ flag) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) flag[v] = (moves[v] != NONE && gains[v] > 0) ? 1u : 0u;
+}
+__global__ void k_jacobi_keys(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos,
+                              const int64_t* __restrict__ gains, uint32_t n, uint64_t* __restrict__ keys,
+                              uint32_t* __restrict__ vals) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n || !flag[v]) return;
+  keys[pos[v]] = ~ordered_i64(gains[v]);
+  vals[pos[v]] = v;
+}
+__global__ void k_jacobi_dst(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ moves,
+                             const uint32_t* __restrict__ sizes, uint32_t m, uint32_t* __restrict__ dst,
+                             int64_t* __restrict__ sz) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  dst[i] = moves[vals[i]];
+  sz[i] = sizes[vals[i]];
+}
+__global__ void k_jacobi_sizes(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ sizes, uint32_t m,
+                               int64_t* __restrict__ sz) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) sz[i] = sizes[vals[i]];
+}
+// Sorted by (target asc, gain desc, node asc): a move applies when everything entering its
+// target up to it still fits next to the target's size before the round (moves leaving the
+// target are not counted on).
+__global__ void __launch_bounds__(256) k_jacobi_apply(const uint32_t* __restrict__ dst,
+                                                      const uint32_t* __restrict__ vals,
+                                                      const int64_t* __restrict__ incl,
+                                                      const uint64_t* __restrict__ doff, uint32_t m,
+                                                      const uint32_t* __restrict__ sizes,
+                                                      const uint32_t* __restrict__ before, uint32_t omega,
+                                                      uint32_t* __restrict__ part, uint32_t* __restrict__ psizes,
+                                                      uint32_t* __restrict__ applied) {
+  __shared__ uint32_t s_part[4];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t ok = 0;
+  if (i < m) {
+    const uint32_t q = dst[i];
+    const uint64_t s0 = doff[q];
+    const int64_t entering = incl[i] - (s0 ? incl[s0 - 1] : 0);
+    if ((int64_t)before[q] + entering <= (int64_t)omega) {
+      const uint32_t v = vals[i];
+      const uint32_t sv = sizes[v];
+      atomicSub(&psizes[part[v]], sv);
+      atomicAdd(&psizes[q], sv);
+      part[v] = q;
+      ok = 1;
+    }
+  }
+  const uint32_t c = wave_sum(ok);
+  if (lane_id() == 0) s_part[threadIdx.x / 64] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    if (t) atomicAdd(applied, t);
+  }
 }
 
 // partitions in descending mix32(q + repeat) order (mix32 is a bijection: no ties)
@@ -1615,14 +1680,13 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   // tiny nodes are scored by the router; the light and block lists and their counts stay on
   // the device, and the persistent light and block kernels stride over them
   DBuf<uint64_t> bound(c, L.n);
-  DBuf<uint32_t> ll(c, L.n), bl(c, L.k > GAIN_LIGHT ? L.n : 1), counts(c, 2);
+  DBuf<uint32_t> ll(c, L.n), bl(c, L.n), counts(c, 2);
   memset_async(c, counts.get(), 0, 8);
   HGP_LAUNCH(k_gains_route, ceil_div(L.n, 256), 256, 0, c, a, L.n, bound.get(), ll.get(), bl.get(), counts.get());
   const uint32_t capmax = L.k <= GAIN_CAP ? std::max<uint32_t>(L.k, 1) : GAIN_CAP;
   const uint32_t lgrid = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID));
   HGP_LAUNCH_B(gain_bytes(c, ll.get(), counts.get(), L.n, L.toff, bound.get()), k_gains_light, lgrid, 256,
                4 * capmax * 12, c, a, ll.get(), counts.get(), bound.get(), capmax);
-  if (L.k <= GAIN_LIGHT) return;
   DBuf<uint32_t> hl;
   uint32_t nh = 0;
   // workgroup per node with an LDS table; hashed tables that would overflow go to HBM
@@ -1926,6 +1990,38 @@ uint32_t refine_analyze(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, 
     copy_d2h(c, out.iviol, st.iv.get(), (uint64_t)V * 4);
   }
   return st.best.r;
+}
+
+uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t round, uint32_t* part, uint32_t* psizes) {
+  const uint32_t n = L.n;
+  if (n == 0) return 0;
+  PartitionLists pl;
+  build_partition_lists(c, L, part, pl, false);
+  DBuf<uint32_t> moves(c, n);
+  DBuf<int64_t> gains(c, n);
+  compute_gains(c, L, pl, HGP_MODE_KWAY, omega, round, NONE, part, psizes, moves.get(), gains.get());  // strict sizes
+  DBuf<uint32_t> flag(c, n);
+  DBuf<uint64_t> pos(c, (uint64_t)n + 1);
+  HGP_LAUNCH(k_jacobi_movers, ceil_div(n, 256), 256, 0, c, moves.get(), gains.get(), n, flag.get());
+  exclusive_scan(c, flag.get(), pos.get(), n);
+  const uint32_t m = (uint32_t)read_scalar(c, pos.get() + n);
+  if (m == 0) return 0;
+  DBuf<uint64_t> keys(c, m);
+  DBuf<uint32_t> vals(c, m), dst(c, m), before(c, std::max<uint32_t>(L.k, 1)), applied(c, 1);
+  DBuf<int64_t> sz(c, m);
+  DBuf<uint64_t> doff(c, (uint64_t)L.k + 1);
+  HGP_LAUNCH(k_jacobi_keys, ceil_div(n, 256), 256, 0, c, flag.get(), pos.get(), gains.get(), n, keys.get(), vals.get());
+  radix_sort_range(c, keys.get(), vals.get(), m);  // gain desc, node id asc (stable)
+  HGP_LAUNCH(k_jacobi_dst, ceil_div(m, 256), 256, 0, c, vals.get(), moves.get(), L.sizes, m, dst.get(), sz.get());
+  radix_sort(c, dst.get(), vals.get(), m, std::max(1, bit_width(L.k)));  // target asc (stable)
+  HGP_LAUNCH(k_jacobi_sizes, ceil_div(m, 256), 256, 0, c, vals.get(), L.sizes, m, sz.get());
+  inclusive_scan(c, sz.get(), sz.get(), m);
+  segment_offsets(c, dst.get(), 0, m, L.k, doff.get());
+  copy_d2d(c, before.get(), psizes, (uint64_t)L.k * 4);
+  memset_async(c, applied.get(), 0, 4);
+  HGP_LAUNCH(k_jacobi_apply, ceil_div(m, 256), 256, 0, c, dst.get(), vals.get(), sz.get(), doff.get(), m, L.sizes,
+             before.get(), omega, part, psizes, applied.get());
+  return read_scalar(c, applied.get());
 }
 
 void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* part,

@@ -56,6 +56,14 @@ void refine(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t del
 uint32_t refine_analyze(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t delta, uint32_t repeats,
                         uint32_t fm_repeat, bool chainup, const uint32_t* part, const uint32_t* psizes,
                         const AnalyzeOut& out);
+// Device k-way initial partition of a level (best-fit-decreasing starts, Jacobi rounds, the
+// start with the lowest lambda-1 kept): allocates part[n] and psizes[k].
+void initial_partition_device(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t starts, uint32_t rounds,
+                              uint32_t*& part_out, uint32_t*& psizes_out);
+// One Jacobi label-propagation round (k-way): every node proposes its best move, positive-
+// gain moves are sorted by (target, gain desc, node) and applied while they fit next to the
+// target's size before the round.  Returns the number of moves applied.
+uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t round, uint32_t* part, uint32_t* psizes);
 void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* part,
                    uint32_t e, uint32_t k, uint32_t* out);
 int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* part,
@@ -67,6 +75,10 @@ int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff,
 // device; equal to hgp_ip::from_level on the same level.
 void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
                      uint32_t n, uint32_t e, hgp_ip::Hypergraph& h);
+// The same merge left on the device as a level (sorted pins, no sources): e_out hedges with
+// float weights = summed fixed-point weights / 2^18.
+void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
+                        uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out);
 
 // ---- post-processing (postproc.hip) -----------------------------------------------------
 void greedy_merge_groups(Ctx& c, const uint32_t* sizes, const uint32_t* inb, uint32_t n, uint32_t ngroups,

@@ -64,7 +64,13 @@ typedef struct hgp_params {
   int32_t device_touching_construction; /* -dtc */
   int32_t initial_partitions_merge;     /* -ipm */
   int32_t verbose;                  /* -v 0..3 */
+  /* k-way initial partitioner when Mt-KaHyPar is not installed: HGP_IP_HOST (recursive
+   * bisection on the host, the default) or HGP_IP_DEVICE (best-fit-decreasing starts
+   * improved by label propagation on the device).  No counterpart in the reference. */
+  int32_t initial_partitioner;
 } hgp_params;
+
+enum { HGP_IP_HOST = 0, HGP_IP_DEVICE = 1 };
 
 /* Per-run statistics filled by hgp_partition. */
 typedef struct hgp_run_stats {

@@ -96,3 +96,5 @@ def test_cli_help_and_errors():
     assert r.returncode == 1
     r = subprocess.run([exe, "-bogus"], capture_output=True, text=True)
     assert r.returncode == 1 and "Unknown option" in r.stderr
+    r = subprocess.run([exe, "-ip", "gpu"], capture_output=True, text=True)
+    assert r.returncode == 1 and "'host' or 'device'" in r.stderr

@@ -59,6 +59,26 @@ def test_pipeline_deterministic(dev, rng):
     assert O.validity(orc, a, omega, 2**31 - 1, 8) == 1
 
 
+@pytest.mark.parametrize("k", [4, 32])
+def test_device_initial_partitioner(dev, k, rng):
+    """The opt-in device initial partitioner (best-fit-decreasing starts, label propagation,
+    a further coarsened hierarchy): valid partitions, the same on a second run, and a
+    lambda-1 in the range of the host recursive bisection's on the same input (measured:
+    within 1% at k=4, 16% above at k=32; on the 3.2M-node netlist at k=64, within 0.5%)."""
+    hg = H.Hypergraph.generate("netlist", n=120_000, e=128_000, seed=7)
+    n, e, p = hg.sizes()
+    omega = int(np.ceil(np.float32(1.03) * np.float32(n) / np.float32(k)))
+    prm = G.params(H.HGP_MODE_KWAY, omega, 2**31 - 1, k, kway=k, eps=0.03)
+    host, _ = G.partition(dev, hg, prm)
+    prm.initial_partitioner = H.HGP_IP_DEVICE
+    a, _ = G.partition(dev, hg, prm)
+    b, _ = G.partition(dev, hg, prm)
+    assert np.array_equal(a, b), "two runs differ"
+    assert hg.validity(a, omega, 2**31 - 1, k) == 1 and int(a.max()) < k
+    ka, kh = hg.metrics(a)[0], hg.metrics(host)[0]
+    assert ka <= 1.25 * kh, (ka, kh)
+
+
 def test_cli_roundtrip(dev, tmp_path, rng):
     n, off, pins, srcc, w = gen.random_hypergraph(5000, 5000, rng, max_size=8)
     f = str(tmp_path / "c0.hgr")
