@@ -1363,10 +1363,22 @@ __device__ __forceinline__ int64_t dst_count(const uint4* hinfo, const uint2* pp
     if (pp[s + i].x == q) return pp_dcnt[s + i];
   return 0;
 }
+// The first event of every (partition, hedge) run looks up the run's starting destination
+// count once (a scan of the hedge's partition list); the run's other events read it there.
+__global__ void k_inbound_group_init(const uint64_t* __restrict__ keys, uint64_t m, int eb, int rb,
+                                     const uint4* __restrict__ hinfo, const uint2* __restrict__ pp,
+                                     const uint32_t* __restrict__ pp_dcnt, int64_t* __restrict__ ginit) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t key = keys[i];
+  if (i > 0 && (keys[i - 1] >> rb) == (key >> rb)) return;
+  const uint32_t q = (uint32_t)(key >> (eb + rb));
+  const uint32_t e = (uint32_t)((key >> rb) & (((uint64_t)1 << eb) - 1));
+  ginit[i] = dst_count(hinfo, pp, pp_dcnt, e, q);
+}
 __global__ void k_inbound_transitions(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                      const int64_t* __restrict__ cum, uint64_t m, int eb, int rb,
-                                      const uint4* __restrict__ hinfo, const uint2* __restrict__ pp,
-                                      const uint32_t* __restrict__ pp_dcnt, uint32_t* __restrict__ tflag,
+                                      const int64_t* __restrict__ cum, uint64_t m, int rb,
+                                      const int64_t* __restrict__ ginit, uint32_t* __restrict__ tflag,
                                       int32_t* __restrict__ tdelta) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
@@ -1380,9 +1392,7 @@ __global__ void k_inbound_transitions(const uint64_t* __restrict__ keys, const u
     else hi = mid;
   }
   const int64_t before = lo == 0 ? 0 : cum[lo - 1];
-  const uint32_t q = (uint32_t)(key >> (eb + rb));
-  const uint32_t e = (uint32_t)((key >> rb) & (((uint64_t)1 << eb) - 1));
-  const int64_t init = dst_count(hinfo, pp, pp_dcnt, e, q);
+  const int64_t init = ginit[lo];
   const int64_t cur = init + (cum[i] - before);
   const int64_t prev = cur - (int64_t)(int32_t)vals[i];
   int32_t d = 0;
@@ -1900,8 +1910,11 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
       inclusive_scan(c, cum.get(), cum.get(), m);
       DBuf<uint32_t> tflag(c, m);
       DBuf<int32_t> tdelta(c, m);
-      HGP_LAUNCH(k_inbound_transitions, ceil_div(m, 256), 256, 0, c, keys.get(), vals.get(), cum.get(), m, eb, rb,
-                 pl.hinfo.get(), pl.pp.get(), pl.dcnt.get(), tflag.get(), tdelta.get());
+      DBuf<int64_t> ginit(c, m);
+      HGP_LAUNCH(k_inbound_group_init, ceil_div(m, 256), 256, 0, c, keys.get(), m, eb, rb, pl.hinfo.get(), pl.pp.get(),
+                 pl.dcnt.get(), ginit.get());
+      HGP_LAUNCH(k_inbound_transitions, ceil_div(m, 256), 256, 0, c, keys.get(), vals.get(), cum.get(), m, rb,
+                 ginit.get(), tflag.get(), tdelta.get());
       DBuf<uint64_t> tpos(c, m + 1);
       exclusive_scan(c, tflag.get(), tpos.get(), m);
       const uint64_t T = read_scalar(c, tpos.get() + m);
