@@ -59,41 +59,36 @@ __device__ __forceinline__ uint64_t cand_key(uint32_t u, uint32_t v, uint32_t sc
   return ((uint64_t)s << 32) | v;
 }
 
-__device__ __forceinline__ void table_insert(uint32_t* keys, uint32_t* score, uint32_t* inter, uint32_t mask,
-                                             uint32_t v, uint32_t contrib, bool shared_in) {
-  uint32_t h = slot_hash(v) & mask;
-  for (;;) {
-    uint32_t old = atomicCAS(&keys[h], NONE, v);
-    if (old == NONE || old == v) {
-      atomicAdd(&score[h], contrib);
-      if (shared_in) atomicAdd(&inter[h], 1u);
-      return;
-    }
-    h = (h + 1) & mask;
-  }
-}
 
+// One wave per node of visit bound b <= CAND_LIGHT.  The node's LDS table has the smallest
+// power of two >= 2b slots (at least 64), so the distinct neighbours (<= b) never fill it
+// and a node with few neighbours clears and ranks few slots.  INTER keeps the shared-inbound
+// counts (their array is not allocated otherwise).
+template <bool INTER>
 __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
                                                     const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ tch,
                                                     const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
                                                     const float* __restrict__ w, const uint32_t* __restrict__ sizes,
-                                                    const uint32_t* __restrict__ list, uint32_t nlist, uint32_t cnc,
-                                                    uint32_t omega, uint32_t delta, uint32_t* __restrict__ pairs,
+                                                    const uint32_t* __restrict__ list, uint32_t nlist,
+                                                    const uint64_t* __restrict__ bound, uint32_t cnc, uint32_t omega,
+                                                    uint32_t delta, uint32_t* __restrict__ pairs,
                                                     uint32_t* __restrict__ scores) {
   __shared__ uint32_t s_keys[4][CAND_CAP];
   __shared__ uint32_t s_score[4][CAND_CAP];
-  __shared__ uint32_t s_inter[4][CAND_CAP];
+  __shared__ uint32_t s_inter[INTER ? 4 : 1][INTER ? CAND_CAP : 1];
   const uint32_t wv = threadIdx.x / 64, l = lane_id();
   const uint32_t idx = blockIdx.x * 4 + wv;
   if (idx >= nlist) return;
   const uint32_t u = list[idx];
   uint32_t* keys = s_keys[wv];
   uint32_t* score = s_score[wv];
-  uint32_t* inter = s_inter[wv];
-  for (uint32_t s = l; s < CAND_CAP; s += 64) {
+  uint32_t* inter = INTER ? s_inter[wv] : nullptr;
+  uint32_t cap = 64;
+  while (cap < 2 * bound[u] && cap < CAND_CAP) cap <<= 1;
+  for (uint32_t s = l; s < cap; s += 64) {
     keys[s] = NONE;
     score[s] = 0;
-    inter[s] = 0;
+    if (INTER) inter[s] = 0;
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -102,8 +97,15 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
   wave_visit_pins(tch, t0, t1, hoff, hedges, [&](const PinVisit& pv) {
     if (!pv.act || pv.pin == u) return;
     const uint32_t contrib = fixed_weight(w[pv.e]) / pv.len;
-    const bool shared_in = (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
-    table_insert(keys, score, inter, CAND_CAP - 1, pv.pin, contrib, shared_in);
+    const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
+    uint32_t h = slot_hash(pv.pin) & (cap - 1);
+    for (;;) {
+      const uint32_t old = atomicCAS(&keys[h], NONE, pv.pin);
+      if (old == NONE || old == pv.pin) break;
+      h = (h + 1) & (cap - 1);
+    }
+    atomicAdd(&score[h], contrib);
+    if (INTER && shared_in) atomicAdd(&inter[h], 1u);
   });
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -112,19 +114,27 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
 #pragma unroll
   for (uint32_t s = 0; s < CAND_SLOTS; ++s) {
     const uint32_t slot = l + 64 * s;
-    const uint32_t v = keys[slot];
-    cand[s] = v == NONE ? 0ull : cand_key(u, v, score[slot], inter[slot], size_u, inb_u, sizes, inb, omega, delta);
+    const uint32_t v = slot < cap ? keys[slot] : NONE;
+    cand[s] = v == NONE ? 0ull
+                        : cand_key(u, v, score[slot], INTER ? inter[slot] : 0u, size_u, inb_u, sizes, inb, omega, delta);
   }
   for (uint32_t i = 0; i < cnc; ++i) {
     uint64_t m = 0;
 #pragma unroll
     for (uint32_t s = 0; s < CAND_SLOTS; ++s) m = cand[s] > m ? cand[s] : m;
     const uint64_t best = wave_max(m);
+    if (best == 0) {  // no candidates left: the remaining slots are empty
+      for (uint32_t r = i + l; r < cnc; r += 64) {
+        pairs[(uint64_t)u * cnc + r] = NONE;
+        scores[(uint64_t)u * cnc + r] = 0;
+      }
+      break;
+    }
 #pragma unroll
     for (uint32_t s = 0; s < CAND_SLOTS; ++s)
-      if (best != 0 && cand[s] == best) cand[s] = 0;
+      if (cand[s] == best) cand[s] = 0;
     if (l == 0) {
-      pairs[(uint64_t)u * cnc + i] = best ? (uint32_t)best : NONE;
+      pairs[(uint64_t)u * cnc + i] = (uint32_t)best;
       scores[(uint64_t)u * cnc + i] = (uint32_t)(best >> 32);
     }
   }
@@ -639,8 +649,12 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   DBuf<uint32_t> light(c, nl), med(c, nm), big(c, nb), heavy(c, nh);
   HGP_LAUNCH(k_split4, g, 256, 0, c, fl.get(), fm.get(), fb.get(), lpos.get(), mpos.get(), bpos.get(), n, light.get(),
              med.get(), big.get(), heavy.get());
-  HGP_LAUNCH(k_cand_light, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, light.get(), nl, cnc,
-             omega, delta, pairs, scores);
+  if (inter)
+    HGP_LAUNCH(k_cand_light<true>, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, light.get(),
+               nl, bound.get(), cnc, omega, delta, pairs, scores);
+  else
+    HGP_LAUNCH(k_cand_light<false>, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, light.get(),
+               nl, bound.get(), cnc, omega, delta, pairs, scores);
 #define CAND_LAUNCH(K, list, cnt, threads)                                                                          \
   HGP_LAUNCH(K, cnt, threads, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, list, caps.get(), cnc, omega, delta, \
              pairs, scores)
