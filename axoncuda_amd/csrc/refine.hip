@@ -892,7 +892,12 @@ struct ChainArrays {
   uint32_t* prev;
   uint32_t* choice;
   unsigned long long* claim;
+  uint32_t* progress;  // progress[it] != 0: iteration it committed a link
 };
+// An iteration after one that committed nothing would repeat it exactly: skip it.
+__device__ __forceinline__ bool chain_stalled(const ChainArrays& c, int it) {
+  return it > 0 && c.progress[it - 1] == 0;
+}
 
 __global__ void k_chain_sortkeys(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
                                  const int64_t* __restrict__ gains, uint32_t n, uint64_t* __restrict__ k1,
@@ -941,9 +946,9 @@ __device__ __forceinline__ int size_bucket(uint32_t s) { return s == 0 ? 0 : 31 
 // first 256 entries of that partition's list.  The list is sorted by gain (descending, valid
 // moves first), so once a choice exists the scan stops at the first lower gain or invalid
 // entry: nothing later can beat it.
-__global__ void k_chain_propose(ChainArrays c, uint32_t n) {
+__global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || chain_stalled(c, it)) return;
   uint32_t choice = NONE;
   c.claim[i] = 0ull;
   if (c.next[i] == NONE && c.ob[i] < c.oe[i]) {
@@ -975,7 +980,8 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n) {
 // table and then issues one global atomic per distinct target.
 constexpr uint32_t RESOLVE_BLOCK = 1024;
 constexpr uint32_t RESOLVE_SLOTS = 2048;  // > 2 x the workgroup's claims: the table never fills
-__global__ void __launch_bounds__(RESOLVE_BLOCK) k_chain_resolve(ChainArrays c, uint32_t n) {
+__global__ void __launch_bounds__(RESOLVE_BLOCK) k_chain_resolve(ChainArrays c, uint32_t n, int it) {
+  if (chain_stalled(c, it)) return;
   __shared__ uint32_t s_tgt[RESOLVE_SLOTS];
   __shared__ unsigned long long s_key[RESOLVE_SLOTS];
   for (uint32_t t = threadIdx.x; t < RESOLVE_SLOTS; t += RESOLVE_BLOCK) {
@@ -1002,14 +1008,15 @@ __global__ void __launch_bounds__(RESOLVE_BLOCK) k_chain_resolve(ChainArrays c, 
   for (uint32_t t = threadIdx.x; t < RESOLVE_SLOTS; t += RESOLVE_BLOCK)
     if (s_tgt[t] != NONE) atomicMax(&c.claim[s_tgt[t]], s_key[t]);
 }
-__global__ void k_chain_commit(ChainArrays c, uint32_t n) {
+__global__ void k_chain_commit(ChainArrays c, uint32_t n, int it) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || chain_stalled(c, it)) return;
   const unsigned long long k = c.claim[s];
   if (k == 0ull || c.prev[s] != NONE) return;
   const uint32_t p = 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFull);
   c.prev[s] = p;
   if (c.next[p] == NONE) c.next[p] = s;
+  c.progress[it] = 1u;
 }
 __global__ void k_chain_comp(ChainArrays c, uint32_t n, uint32_t* __restrict__ comp, uint32_t* __restrict__ pos,
                              uint32_t* __restrict__ cflag, long long* __restrict__ cw,
@@ -1757,17 +1764,19 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   DBuf<uint32_t> src(c, n), dst(c, n), sz(c, n), ok(c, n), ob(c, n), oe(c, n), next(c, n), prev(c, n), choice(c, n);
   DBuf<int64_t> w(c, n);
   DBuf<unsigned long long> claim(c, n);
+  DBuf<uint32_t> progress(c, 4);
+  fill_u32(c, progress.get(), 0u, 4);
   ChainArrays ca{orig.get(), src.get(), dst.get(), sz.get(), w.get(), ok.get(), ob.get(), oe.get(),
-                 next.get(), prev.get(), choice.get(), claim.get()};
+                 next.get(), prev.get(), choice.get(), claim.get(), progress.get()};
   HGP_LAUNCH(k_chain_gather, g, 256, 0, c, ca, part, moves, gains, sizes, n);
   DBuf<uint64_t> poff(c, (uint64_t)k + 1);
   // src partitions of sorted space are the k2 keys >> 1
   segment_offsets(c, k2.get(), 1, n, k, poff.get());
   HGP_LAUNCH(k_chain_ranges, g, 256, 0, c, ca, poff.get(), n);
   for (int it = 0; it < 4; ++it) {
-    HGP_LAUNCH(k_chain_propose, g, 256, 0, c, ca, n);
-    HGP_LAUNCH(k_chain_resolve, ceil_div(n, RESOLVE_BLOCK), RESOLVE_BLOCK, 0, c, ca, n);
-    HGP_LAUNCH(k_chain_commit, g, 256, 0, c, ca, n);
+    HGP_LAUNCH(k_chain_propose, g, 256, 0, c, ca, n, it);
+    HGP_LAUNCH(k_chain_resolve, ceil_div(n, RESOLVE_BLOCK), RESOLVE_BLOCK, 0, c, ca, n, it);
+    HGP_LAUNCH(k_chain_commit, g, 256, 0, c, ca, n, it);
   }
   DBuf<uint32_t> comp(c, n), pos(c, n), cflag(c, n), cinv(c, n);
   DBuf<long long> cw(c, n);
