@@ -492,11 +492,16 @@ __global__ void k_pair_count(const uint32_t* __restrict__ parent, uint32_t n, ui
 }
 
 // Finish `cur` and keep climbing while this thread is the last child to arrive.
+// S and best are only ever changed by atomic read-modify-writes, which are performed at the
+// device's coherence point, and read the same way (adding 0 / or-ing 0).  A child waits for
+// its two updates to complete before it decrements the parent's pending count, and the last
+// child reads the parent's values after that decrement returned, so it sees every child's
+// contribution without device-wide cache write-backs or invalidations.
 __device__ void sweep_up(uint32_t cur, const uint32_t* __restrict__ parent, const uint32_t* __restrict__ wgt,
                          uint32_t* pending, unsigned long long* S, unsigned long long* best, uint32_t* processed) {
   for (;;) {
-    const uint64_t Sv = __hip_atomic_load(&S[cur], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t bv = __hip_atomic_load(&best[cur], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t Sv = atomicAdd(&S[cur], 0ull);
+    const uint64_t bv = atomicOr(&best[cur], 0ull);
     const uint64_t bg = bv >> 32;
     const uint64_t wout = Sv + bg;
     __hip_atomic_store(&processed[cur], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -504,12 +509,14 @@ __device__ void sweep_up(uint32_t cur, const uint32_t* __restrict__ parent, cons
     if (p == NONE) return;
     const int64_t gain = (int64_t)wgt[cur] - (int64_t)bg;
     const uint64_t key = ((uint64_t)(gain > 0 ? gain : 0) << 32) | cur;
-    atomicAdd(&S[p], (unsigned long long)wout);
-    atomicMax(&best[p], (unsigned long long)key);
-    __threadfence();
+    const uint64_t r1 = atomicAdd(&S[p], (unsigned long long)wout);
+    const uint64_t r2 = atomicMax(&best[p], (unsigned long long)key);
+    // both updates have returned (so they are performed) before the count goes down: the
+    // empty asm consumes the returned values, which keeps them returning atomics
+    asm volatile("" ::"v"(r1), "v"(r2));
+    __builtin_amdgcn_s_waitcnt(0);
     const uint32_t old = atomicSub(&pending[p], 1u);
     if (old != 1u) return;
-    __threadfence();
     cur = p;
   }
 }
@@ -546,13 +553,9 @@ __global__ void k_pair_break(const uint32_t* __restrict__ cycmin, uint32_t* pare
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n || !cycmin[v]) return;
   const uint32_t p = parent[v];
-  parent[v] = NONE;
-  __threadfence();
+  parent[v] = NONE;  // read by no other thread of this launch
   const uint32_t old = atomicSub(&pending[p], 1u);
-  if (old == 1u) {
-    __threadfence();
-    sweep_up(p, parent, wgt, pending, S, best, processed);
-  }
+  if (old == 1u) sweep_up(p, parent, wgt, pending, S, best, processed);
 }
 
 __device__ __forceinline__ uint32_t holder_of(const unsigned long long* best, uint32_t v) {
