@@ -893,6 +893,7 @@ struct ChainArrays {
   uint32_t* choice;
   unsigned long long* claim;
   uint32_t* progress;  // progress[it] != 0: iteration it committed a link
+  uint4* pk;           // per entry: gain (lo, hi), size, valid -- what a proposal scan reads
 };
 // An iteration after one that committed nothing would repeat it exactly: skip it.
 __device__ __forceinline__ bool chain_stalled(const ChainArrays& c, int it) {
@@ -925,6 +926,8 @@ __global__ void k_chain_gather(ChainArrays c, const uint32_t* __restrict__ part,
   c.sz[j] = sizes[v];
   c.w[j] = gains[v];
   c.ok[j] = moves[v] != NONE;
+  c.pk[j] = make_uint4((uint32_t)(uint64_t)gains[v], (uint32_t)((uint64_t)gains[v] >> 32), sizes[v],
+                       moves[v] != NONE ? 1u : 0u);
   c.next[j] = NONE;
   c.prev[j] = NONE;
 }
@@ -958,11 +961,12 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
     int64_t bw = 0;
     uint32_t bd = 0;
     for (uint32_t j = c.ob[i]; j < lim; ++j) {
-      if (!c.ok[j]) break;
-      const int64_t wj = c.w[j];
+      const uint4 e = c.pk[j];
+      if (!e.w) break;
+      const int64_t wj = (int64_t)(((uint64_t)e.y << 32) | e.x);
       if (choice != NONE && wj < bw) break;
       if (j == i || c.prev[j] != NONE) continue;
-      const uint32_t sj = c.sz[j];
+      const uint32_t sj = e.z;
       if (abs(bi - size_bucket(sj)) > 1) continue;
       const uint32_t d = si > sj ? si - sj : sj - si;
       if (choice == NONE || wj > bw || (wj == bw && d < bd)) {
@@ -1765,9 +1769,10 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   DBuf<int64_t> w(c, n);
   DBuf<unsigned long long> claim(c, n);
   DBuf<uint32_t> progress(c, 4);
+  DBuf<uint4> pk(c, n);
   fill_u32(c, progress.get(), 0u, 4);
   ChainArrays ca{orig.get(), src.get(), dst.get(), sz.get(), w.get(), ok.get(), ob.get(), oe.get(),
-                 next.get(), prev.get(), choice.get(), claim.get(), progress.get()};
+                 next.get(), prev.get(), choice.get(), claim.get(), progress.get(), pk.get()};
   HGP_LAUNCH(k_chain_gather, g, 256, 0, c, ca, part, moves, gains, sizes, n);
   DBuf<uint64_t> poff(c, (uint64_t)k + 1);
   // src partitions of sorted space are the k2 keys >> 1
