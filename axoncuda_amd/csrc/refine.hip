@@ -37,6 +37,7 @@ constexpr uint32_t CAS_LDS = 2048;
 constexpr uint32_t CAS_SORT = 1024;               // pins of a hedge cascaded through sorted events
 constexpr uint32_t CAS_SORT_S = 128;              // ... by the small-LDS instance (4x event slots)
 constexpr uint32_t CAS_EV = 4096;                 // event slots: next power of two >= 3 * CAS_SORT
+constexpr uint32_t CAS_SORT_NT = 256;              // threads sorting the events of one such hedge
 constexpr uint64_t HEAVY_BATCH_SLOTS = 32ull << 20;
 
 struct Best {
@@ -960,19 +961,40 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
     const uint32_t lim = min(c.oe[i], c.ob[i] + 256u);
     int64_t bw = 0;
     uint32_t bd = 0;
-    for (uint32_t j = c.ob[i]; j < lim; ++j) {
-      const uint4 e = c.pk[j];
-      if (!e.w) break;
-      const int64_t wj = (int64_t)(((uint64_t)e.y << 32) | e.x);
-      if (choice != NONE && wj < bw) break;
-      if (j == i || c.prev[j] != NONE) continue;
-      const uint32_t sj = e.z;
-      if (abs(bi - size_bucket(sj)) > 1) continue;
-      const uint32_t d = si > sj ? si - sj : sj - si;
-      if (choice == NONE || wj > bw || (wj == bw && d < bd)) {
-        choice = j;
-        bw = wj;
-        bd = d;
+    bool done = false;
+    // 8 entries (one 128-byte line of records) are loaded before any is examined, so a scan
+    // waits on one memory round trip per 8 entries instead of one per entry
+    for (uint32_t jb = c.ob[i]; jb < lim && !done; jb += 8) {
+      uint4 e[8];
+      uint32_t pv[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {  // unconditional loads (clamped index): no branch per load
+        const uint32_t jj = min(jb + u, lim - 1);
+        e[u] = c.pk[jj];
+        pv[u] = c.prev[jj];
+        if (jb + u >= lim) e[u].w = 0;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t j = jb + u;
+        if (!e[u].w) {  // invalid entry, or past the scan limit
+          done = true;
+          break;
+        }
+        const int64_t wj = (int64_t)(((uint64_t)e[u].y << 32) | e[u].x);
+        if (choice != NONE && wj < bw) {
+          done = true;
+          break;
+        }
+        if (j == i || pv[u] != NONE) continue;
+        const uint32_t sj = e[u].z;
+        if (abs(bi - size_bucket(sj)) > 1) continue;
+        const uint32_t d = si > sj ? si - sj : sj - si;
+        if (choice == NONE || wj > bw || (wj == bw && d < bd)) {
+          choice = j;
+          bw = wj;
+          bd = d;
+        }
       }
     }
   }
@@ -1117,35 +1139,66 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
     f_m[e] = (sorted && hl > CAS_SORT_S) ? 1u : 0u;
     f_big[e] = (hl > CAS_SMALL && !sorted) ? 1u : 0u;
   }
-  if (hl > CAS_SMALL) return;
+  if (hl > CAS_SMALL || hl == 0) return;
   uint32_t pt[CAS_SMALL], mv[CAS_SMALL], rk[CAS_SMALL];
-  const uint32_t len = (uint32_t)(t - s);
-  bool any = false;
-  for (uint32_t i = 0; i < len; ++i) {
-    const uint32_t x = a.hedges[s + i];
-    const uint4 q4 = a.pm[x];
-    pt[i] = q4.x;
-    mv[i] = q4.y;
-    rk[i] = q4.z;
-    any |= mv[i] != NONE;
+#pragma unroll
+  for (uint32_t u = 0; u < CAS_SMALL; ++u) {  // entries no batch fills (u >= 4 when len <= 4)
+    pt[u] = NONE;
+    mv[u] = NONE;
+    rk[u] = 0;
   }
+  const uint32_t len = (uint32_t)(t - s);
+  // The pins and their records are fetched in batches of unconditional loads (indexes clamped
+  // to the last pin), so a hedge costs two memory round trips per batch instead of two per pin;
+  // entries past the end are marked as non-moving pins of no partition.
+  auto fetch = [&](uint32_t u) {
+    const uint32_t x = a.hedges[s + min(u, len - 1)];
+    return x;
+  };
+  auto put = [&](uint32_t u, uint4 q4) {
+    const bool in = u < len;
+    pt[u] = in ? q4.x : NONE;
+    mv[u] = in ? q4.y : NONE;
+    rk[u] = q4.z;
+  };
+  {
+    uint32_t x[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) x[u] = fetch(u);
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) put(u, a.pm[x[u]]);
+  }
+  if (len > 4) {
+    uint32_t x[CAS_SMALL - 4];
+#pragma unroll
+    for (uint32_t u = 4; u < CAS_SMALL; ++u) x[u - 4] = fetch(u);
+#pragma unroll
+    for (uint32_t u = 4; u < CAS_SMALL; ++u) put(u, a.pm[x[u - 4]]);
+  }
+  bool any = false;
+#pragma unroll
+  for (uint32_t i = 0; i < CAS_SMALL; ++i) any |= mv[i] != NONE;
   if (!any) return;
   const int64_t W = (int64_t)fixed_weight(a.w[e]);
-  for (uint32_t i = 0; i < len; ++i) {
-    if (mv[i] == NONE) continue;
-    const uint32_t p = pt[i], q = mv[i], r = rk[i];
-    int64_t cp = 0, cq = 0;
-    for (uint32_t j = 0; j < len; ++j) {
-      cp += pt[j] == p;
-      cq += pt[j] == q;
-      if (mv[j] != NONE && rk[j] < r) {
-        if (mv[j] == p) cp++;
-        else if (mv[j] == q) cq++;
-        if (pt[j] == p) cp--;
-        else if (pt[j] == q) cq--;
+  // Static indexes only (the arrays stay in registers); entries past a hedge's end take no
+  // part.  The loops stop at the wave's longest hedge, rounded up to a multiple of 4.
+  const uint32_t jmax = __ballot(len > 12) ? 16u : __ballot(len > 8) ? 12u : __ballot(len > 4) ? 8u : 4u;
+#pragma unroll
+  for (uint32_t i = 0; i < CAS_SMALL; ++i) {
+    if (i < jmax && mv[i] != NONE) {
+      const uint32_t p = pt[i], q = mv[i], r = rk[i];
+      int32_t cp = 0, cq = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < CAS_SMALL; ++j) {
+        if (j < jmax) {
+          const bool before = mv[j] != NONE && rk[j] < r;  // moved before pin i
+          const uint32_t now = before ? mv[j] : pt[j];     // its partition when pin i moves
+          cp += now == p;
+          cq += now == q;
+        }
       }
+      atomicAdd(&a.seq[r], (unsigned long long)cascade_term(a.incc, W, cp, cq));
     }
-    atomicAdd(&a.seq[r], (unsigned long long)cascade_term(a.incc, W, cp, cq));
   }
 }
 
@@ -1196,7 +1249,7 @@ __global__ void __launch_bounds__(64) k_cascade_large(CascadeArgs a, const uint3
 }
 
 // Cascade of one hedge through sorted events (hedges of CAS_SMALL < len <= CAS_SORT pins,
-// one wave per hedge).  Every pin contributes an "init" event (its partition, +1) and every
+// one workgroup per hedge).  Every pin contributes an "init" event (its partition, +1) and every
 // moving pin an "out" event (source, -1) and an "in" event (target, +1) at its rank.  Sorted
 // by (partition, rank), the exclusive prefix of a move's out event is the pin count of its
 // source just before the move (the pin itself included) and that of its in event is the
@@ -1218,52 +1271,45 @@ __device__ __forceinline__ int32_t wave_seg_scan(int32_t v, bool& flag) {
   return v;
 }
 
-template <uint32_t PINS, uint32_t EVS>
-__global__ void __launch_bounds__(64) k_cascade_sorted(CascadeArgs a, const uint32_t* __restrict__ list, int rb) {
+template <uint32_t PINS, uint32_t EVS, uint32_t NT>
+__global__ void __launch_bounds__(NT) k_cascade_sorted(CascadeArgs a, const uint32_t* __restrict__ list, int rb) {
   __shared__ uint64_t s_ev[EVS];
   __shared__ uint32_t s_rk[PINS];
   __shared__ int32_t s_cp[PINS], s_cq[PINS];
+  __shared__ uint32_t s_moving;
   const uint32_t e = list[blockIdx.x];
-  const uint32_t l = lane_id();
+  const uint32_t l = lane_id(), tid = threadIdx.x;
   const uint64_t s = a.hoff[e];
   const uint32_t len = (uint32_t)(a.hoff[e + 1] - s);
-  // init events at [0, len), move events appended after them
-  uint32_t moving = 0;
-  for (uint32_t i = l; i < len; i += 64) {
+  if (tid == 0) s_moving = 0;
+  __syncthreads();
+  // init events at [0, len); the two events of each moving pin at any free pair of slots
+  // after them (the sort orders them, and every key is unique through its pin index)
+  for (uint32_t i = tid; i < len; i += NT) {
     const uint32_t x = a.hedges[s + i];
     const uint4 q4 = a.pm[x];
     const uint32_t p = q4.x, q = q4.y;
     s_ev[i] = ((uint64_t)p << (rb + 11)) | i;
     s_rk[i] = q == NONE ? NONE : q4.z;
-    moving += q != NONE;
+    if (q != NONE) {
+      const uint32_t o = len + 2 * atomicAdd(&s_moving, 1u);
+      const uint64_t r1 = (uint64_t)q4.z + 1;
+      s_ev[o] = ((uint64_t)p << (rb + 11)) | (r1 << 11) | i;
+      s_ev[o + 1] = ((uint64_t)q << (rb + 11)) | (r1 << 11) | (1u << 10) | i;
+    }
   }
-  moving = wave_sum(moving);
+  __syncthreads();
+  const uint32_t moving = s_moving;
   if (moving == 0) return;
   const uint32_t nev = len + 2 * moving;
   uint32_t ev_n = 64;
   while (ev_n < nev) ev_n <<= 1;
-  __syncthreads();
-  uint32_t base = len;
-  for (uint32_t b = 0; b < len; b += 64) {
-    const uint32_t i = b + l;
-    const bool mv = i < len && s_rk[i] != NONE;
-    const uint64_t mk = __ballot(mv);
-    if (mv) {
-      const uint32_t o = base + 2 * lanes_below(mk);
-      const uint32_t x = a.hedges[s + i];
-      const uint64_t r1 = (uint64_t)s_rk[i] + 1;
-      const uint4 q4 = a.pm[x];
-      s_ev[o] = ((uint64_t)q4.x << (rb + 11)) | (r1 << 11) | i;
-      s_ev[o + 1] = ((uint64_t)q4.y << (rb + 11)) | (r1 << 11) | (1u << 10) | i;
-    }
-    base += 2 * (uint32_t)__popcll(mk);
-  }
-  for (uint32_t i = nev + l; i < ev_n; i += 64) s_ev[i] = ~0ull;
+  for (uint32_t i = nev + tid; i < ev_n; i += NT) s_ev[i] = ~0ull;
   __syncthreads();
   // bitonic sort, ascending
   for (uint32_t k = 2; k <= ev_n; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t t = l; t < ev_n / 2; t += 64) {
+      for (uint32_t t = tid; t < ev_n / 2; t += NT) {
         const uint32_t i = (t / j) * 2 * j + (t % j);
         const uint32_t ixj = i + j;
         const uint64_t x = s_ev[i], y = s_ev[ixj];
@@ -1276,29 +1322,31 @@ __global__ void __launch_bounds__(64) k_cascade_sorted(CascadeArgs a, const uint
       __syncthreads();
     }
   }
-  // segmented (by partition) exclusive prefix of the event deltas
-  const int shift = rb + 11;
-  int32_t carry = 0;
-  for (uint32_t b = 0; b < nev; b += 64) {
-    const uint32_t i = b + l;
-    const bool act = i < nev;
-    const uint64_t key = act ? s_ev[i] : ~0ull;
-    const uint32_t rank1 = (uint32_t)((key >> 11) & (((uint64_t)1 << rb) - 1));
-    const bool dir_in = (key >> 10) & 1u;
-    const int32_t d = !act ? 0 : (rank1 == 0 ? 1 : (dir_in ? 1 : -1));
-    bool flag = act && (i == 0 || (s_ev[i - 1] >> shift) != (key >> shift));
-    int32_t v = wave_seg_scan(d, flag);
-    if (!flag) v += carry;  // segment continues from the previous chunk
-    if (act && rank1 != 0) {
-      const uint32_t idx = (uint32_t)(key & 1023u);
-      if (dir_in) s_cq[idx] = v - d;
-      else s_cp[idx] = v - d;
+  // segmented (by partition) exclusive prefix of the event deltas, by the first wave
+  if (tid < 64) {
+    const int shift = rb + 11;
+    int32_t carry = 0;
+    for (uint32_t b = 0; b < nev; b += 64) {
+      const uint32_t i = b + l;
+      const bool act = i < nev;
+      const uint64_t key = act ? s_ev[i] : ~0ull;
+      const uint32_t rank1 = (uint32_t)((key >> 11) & (((uint64_t)1 << rb) - 1));
+      const bool dir_in = (key >> 10) & 1u;
+      const int32_t d = !act ? 0 : (rank1 == 0 ? 1 : (dir_in ? 1 : -1));
+      bool flag = act && (i == 0 || (s_ev[i - 1] >> shift) != (key >> shift));
+      int32_t v = wave_seg_scan(d, flag);
+      if (!flag) v += carry;  // segment continues from the previous chunk
+      if (act && rank1 != 0) {
+        const uint32_t idx = (uint32_t)(key & 1023u);
+        if (dir_in) s_cq[idx] = v - d;
+        else s_cp[idx] = v - d;
+      }
+      carry = __shfl(v, 63, 64);
     }
-    carry = __shfl(v, 63, 64);
   }
   __syncthreads();
   const int64_t W = (int64_t)fixed_weight(a.w[e]);
-  for (uint32_t i = l; i < len; i += 64) {
+  for (uint32_t i = tid; i < len; i += NT) {
     const uint32_t r = s_rk[i];
     if (r == NONE) continue;
     atomicAdd(&a.seq[r], (unsigned long long)cascade_term(a.incc, W, s_cp[i], s_cq[i]));
@@ -1881,8 +1929,8 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
       const uint32_t m = Y.cas_n[cls];
       if (m == 0) continue;
       const uint32_t* lst = Y.cas_list[cls].get();
-      if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 4 * CAS_SORT_S>), m, 64, 0, c, a, lst, crb);
-      else if (cls == 1) HGP_LAUNCH((k_cascade_sorted<CAS_SORT, CAS_EV>), m, 64, 0, c, a, lst, crb);
+      if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 4 * CAS_SORT_S, 64>), m, 64, 0, c, a, lst, crb);
+      else if (cls == 1) HGP_LAUNCH((k_cascade_sorted<CAS_SORT, CAS_EV, CAS_SORT_NT>), m, CAS_SORT_NT, 0, c, a, lst, crb);
       else HGP_LAUNCH(k_cascade_large, m, 64, 0, c, a, lst);
     }
     inclusive_scan(c, (const int64_t*)st.seq.get(), (int64_t*)st.seq.get(), V);
