@@ -622,7 +622,13 @@ hgp_status hgp_partition(hgp_ctx* ctx, const hgp_params* params, const hgp_hgrap
 }
 
 hgp_status hgp_profile_kernel(hgp_ctx* ctx, const char* kernel_name) {
-  return guard(ctx, [&](Ctx& c) { c.prof_kernel = kernel_name ? kernel_name : ""; });
+  return guard(ctx, [&](Ctx& c) {
+    c.prof_kernel = kernel_name ? kernel_name : "";
+    if (!c.prof_kernel.empty() && !c.prof_dev) {
+      c.prof_dev = (unsigned long long*)hgp::dmalloc(c, hgp::PROF_SLOTS * 8);
+      hgp::memset_async(c, c.prof_dev, 0, hgp::PROF_SLOTS * 8);
+    }
+  });
 }
 
 hgp_status hgp_profile_read(hgp_ctx* ctx, double* total_ms, uint64_t* launches, uint64_t* algorithmic_bytes) {
@@ -637,6 +643,13 @@ hgp_status hgp_profile_read(hgp_ctx* ctx, double* total_ms, uint64_t* launches, 
       bytes += r.bytes;
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
+    }
+    if (c.prof_dev) {  // bytes the kernel counted on the device
+      unsigned long long slots[hgp::PROF_SLOTS];
+      hgp::copy_d2h(c, slots, c.prof_dev, sizeof(slots));
+      for (auto v : slots) bytes += v;
+      hgp::memset_async(c, c.prof_dev, 0, sizeof(slots));
+      hgp::sync(c);
     }
     if (total_ms) *total_ms = ms;
     if (launches) *launches = c.prof.size();

@@ -74,7 +74,11 @@ struct Ctx {
   uint64_t scan_tickets_issued = 0;
   std::string prof_kernel;     // base name of the kernel whose launches are timed ("" = none)
   std::vector<ProfRec> prof;   // timed launches not yet read
+  // algorithmic bytes that the profiled kernel counts itself on the device (PROF_SLOTS
+  // counters, spread so that the atomics do not all meet on one address)
+  unsigned long long* prof_dev = nullptr;
 };
+constexpr uint32_t PROF_SLOTS = 64;
 
 // Device memory: a caching allocator on top of hipMallocAsync.  Freed blocks are parked
 // per size class (4 classes per power of two) and handed out again by later requests on

@@ -228,6 +228,7 @@ struct GainArgs {
   int incc;
   uint32_t* moves;
   int64_t* gains;
+  unsigned long long* prof_bytes;  // when profiled: PROF_SLOTS counters of algorithmic bytes
 };
 
 __device__ __forceinline__ bool target_ok(const GainArgs& a, uint32_t q, uint32_t p, uint32_t s) {
@@ -393,8 +394,10 @@ __device__ __forceinline__ void gains_tiny_node(const GainArgs& a, uint32_t v) {
 // One wave per node.  With at most GAIN_CAP partitions the LDS table is indexed by partition
 // (k slots); otherwise it is an open-addressing table sized to twice the node's possible
 // distinct partitions.
-// Dynamic LDS: 4 waves x capmax slots of (int64 connection, uint32 key).
-__device__ __forceinline__ void gains_light_node(const GainArgs& a, uint32_t v, const uint64_t* __restrict__ bound,
+// Dynamic LDS: 4 waves x capmax slots of (int64 connection, uint32 key).  Returns this lane's
+// share of the node's algorithmic bytes: 36 (node record, offsets, result) + 20 per touching
+// hedge (list entry and hedge record) + 8 per partition entry.
+__device__ __forceinline__ uint64_t gains_light_node(const GainArgs& a, uint32_t v, const uint64_t* __restrict__ bound,
                                                  uint32_t capmax, int64_t* s_conn_all, uint32_t wv, uint32_t l) {
   const bool direct = a.k <= GAIN_CAP;
   uint32_t cap = a.k;
@@ -414,8 +417,10 @@ __device__ __forceinline__ void gains_light_node(const GainArgs& a, uint32_t v, 
   const uint32_t p = a.part[v], s = a.sizes[v];
   const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
   int64_t saving = 0, Wv = 0;
+  uint64_t entries = 0;
   Wv += wave_visit_hedge_lists(a.tch, t0, t1, a.hinfo, [&](const EntryVisit& ev) {
         if (!ev.act) return;
+        ++entries;
         const uint2 pe = a.pp[ev.abs];
         const uint32_t q = pe.x;
         const int64_t We = (int64_t)ev.w;
@@ -453,6 +458,7 @@ __device__ __forceinline__ void gains_light_node(const GainArgs& a, uint32_t v, 
     a.moves[v] = bq;
     a.gains[v] = bq == NONE ? 0 : bg;
   }
+  return 8 * entries + (l == 0 ? 36 + 20 * (t1 - t0) : 0);
 }
 
 // Persistent: the grid strides over the `*count` listed nodes (the count stays on the device).
@@ -462,11 +468,16 @@ __global__ void __launch_bounds__(256) k_gains_light(GainArgs a, const uint32_t*
   extern __shared__ int64_t s_conn_all[];
   const uint32_t wv = threadIdx.x / 64, l = lane_id();
   const uint32_t nlist = *count;
+  uint64_t bytes = 0;
   for (uint32_t idx = blockIdx.x * 4 + wv; idx < nlist; idx += gridDim.x * 4) {
-    gains_light_node(a, list[idx], bound, capmax, s_conn_all, wv, l);
+    bytes += gains_light_node(a, list[idx], bound, capmax, s_conn_all, wv, l);
     // every lane has read the table before the next node clears it
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  if (a.prof_bytes) {
+    bytes = wave_sum(bytes);
+    if (l == 0 && bytes) atomicAdd(&a.prof_bytes[blockIdx.x % PROF_SLOTS], (unsigned long long)bytes);
   }
 }
 
@@ -563,6 +574,8 @@ __global__ void __launch_bounds__(256) k_gains_heavy(GainArgs a, const uint32_t*
 // for every probe.
 constexpr uint32_t LIGHT_GRID = 16384;  // workgroups of the persistent light kernel (4 nodes each at a time)
 constexpr uint32_t BLOCK_GRID = 2048;   // workgroups of the persistent block kernel
+constexpr uint64_t BOUND_UNKNOWN = ~0ull;  // a node routed by degree: its bound was not summed
+constexpr uint32_t WIDE_LIGHT_DEG = 256;   // ... and it takes a wave (else a workgroup) up to this degree
 constexpr uint32_t GB_DIRECT = 20160;  // 8 bytes per slot
 constexpr uint32_t GB_SLOTS = 13440;   // 12 bytes per slot
 constexpr uint32_t GB_LIMIT = GB_SLOTS / 4 * 3;
@@ -707,23 +720,84 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
   }
 }
 
+// Appends the lanes whose flag is set to a device list: one atomic per workgroup and list,
+// then wave and lane offsets.  Every thread of the 256-thread workgroup must call it.
+template <int NL>
+__device__ __forceinline__ void append_lists(const bool (&to)[NL], uint32_t v, uint32_t* const (&lists)[NL],
+                                             uint32_t* __restrict__ counts) {
+  __shared__ uint32_t s_cnt[NL][4], s_base[NL];
+  const uint32_t l = lane_id(), wv = threadIdx.x / 64;
+  uint64_t m[NL];
+#pragma unroll
+  for (int x = 0; x < NL; ++x) {
+    m[x] = __ballot(to[x]);
+    if (l == 0) s_cnt[x][wv] = (uint32_t)__popcll(m[x]);
+  }
+  __syncthreads();
+  if (threadIdx.x < NL) {
+    const uint32_t t = s_cnt[threadIdx.x][0] + s_cnt[threadIdx.x][1] + s_cnt[threadIdx.x][2] + s_cnt[threadIdx.x][3];
+    s_base[threadIdx.x] = t ? atomicAdd(&counts[threadIdx.x], t) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < NL; ++x) {
+    uint32_t base = s_base[x];
+    for (uint32_t w = 0; w < wv; ++w) base += s_cnt[x][w];
+    if (to[x]) lists[x][base + lanes_below(m[x])] = v;
+  }
+  __syncthreads();  // the shared counts are reused by the next call
+}
+
 // Routes every node by bound = partition entries listed by its touching hedges: bound <=
 // GAIN_TINY is scored right here (thread per node, registers); the others are appended to
 // the light list (wave per node; up to GAIN_BLOCK_MIN entries when k <= GAIN_LIGHT) or the
-// block list (workgroup per node), with the counts left on the device.  List order varies from run to
-// run; each node's result does not depend on it.
+// block list (workgroup per node), with the counts left on the device.  When k <= GAIN_LIGHT
+// the light tables are indexed by partition and the bound only balances the work, so a node
+// touching more than GAIN_TINY hedges (it cannot be tiny) is routed by its degree instead of
+// being summed by one thread, and its bound is left unknown (BOUND_UNKNOWN).  List order
+// varies from run to run; each node's result does not depend on it nor on the route.
 __global__ void __launch_bounds__(256) k_gains_route(GainArgs a, uint32_t n, uint64_t* __restrict__ bound,
                                                      uint32_t* __restrict__ light, uint32_t* __restrict__ block,
                                                      uint32_t* __restrict__ counts) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t t0 = 0, t1 = 0;
+  if (v < n) {
+    t0 = a.toff[v];
+    t1 = a.toff[v + 1];
+  }
+  const uint64_t deg = t1 - t0;
+  const bool by_degree = v < n && a.k <= GAIN_LIGHT && deg > GAIN_TINY;
   uint64_t b = 0;
-  if (v < n)
-    for (uint64_t j = a.toff[v]; j < a.toff[v + 1]; ++j) b += a.hinfo[a.tch[j]].z;
-  const bool tiny = v < n && b <= GAIN_TINY;
+  if (v < n && !by_degree && deg > GAIN_TINY) {
+    for (uint64_t j = t0; j < t1; ++j) b += a.hinfo[a.tch[j]].z;
+  } else if (v < n && !by_degree && deg > 0) {
+    // at most GAIN_TINY touching hedges: unconditional loads (indexes clamped to the last
+    // one), 4 and then the rest, so the sum costs two memory round trips per batch
+    const uint32_t d = (uint32_t)deg;
+    uint32_t e[GAIN_TINY];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) e[u] = a.tch[t0 + min(u, d - 1)];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t z = a.hinfo[e[u]].z;
+      b += u < d ? z : 0u;
+    }
+    if (d > 4) {
+#pragma unroll
+      for (uint32_t u = 4; u < GAIN_TINY; ++u) e[u] = a.tch[t0 + min(u, d - 1)];
+#pragma unroll
+      for (uint32_t u = 4; u < GAIN_TINY; ++u) {
+        const uint32_t z = a.hinfo[e[u]].z;
+        b += u < d ? z : 0u;
+      }
+    }
+  }
+  const bool tiny = v < n && !by_degree && b <= GAIN_TINY;
   if (tiny) gains_tiny_node(a, v);
   const bool rest = v < n && !tiny;
-  if (rest) bound[v] = b;
-  const bool to_light = rest && (b <= GAIN_LIGHT || (a.k <= GAIN_LIGHT && b <= GAIN_BLOCK_MIN));
+  if (rest) bound[v] = by_degree ? BOUND_UNKNOWN : b;
+  const bool to_light =
+      rest && (by_degree ? deg <= WIDE_LIGHT_DEG : (b <= GAIN_LIGHT || (a.k <= GAIN_LIGHT && b <= GAIN_BLOCK_MIN)));
   const bool to_block = rest && !to_light;
   // list slots: one atomic per workgroup and list, then wave and lane offsets
   __shared__ uint32_t s_cnt[2][4], s_base[2];
@@ -994,6 +1068,10 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
           choice = j;
           bw = wj;
           bd = d;
+          if (d == 0) {  // later entries gain no more and cannot have a smaller difference
+            done = true;
+            break;
+          }
         }
       }
     }
@@ -1701,28 +1779,6 @@ uint32_t discount_of(uint32_t rep, uint32_t R) { return rep < R / 3 ? 1u : (rep 
 // profiled): per node 36 (partition, size, two offsets, move and gain out) + 20 per touching
 // hedge (hedge id and its 16-byte record: list start, list length, weight) + 8 per listed
 // partition entry (partition and pin count).
-__global__ void k_gain_bytes(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-                             const uint64_t* __restrict__ toff, const uint64_t* __restrict__ bound,
-                             unsigned long long* out) {
-  uint64_t b = 0;
-  const uint32_t m = *count;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
-    const uint32_t v = list[i];
-    b += 36 + 20 * (toff[v + 1] - toff[v]) + 8 * bound[v];
-  }
-  b = wave_sum(b);
-  if (lane_id() == 0 && b) atomicAdd(out, (unsigned long long)b);
-}
-
-uint64_t gain_bytes(Ctx& c, const uint32_t* list, const uint32_t* count, uint32_t n, const uint64_t* toff,
-                    const uint64_t* bound) {
-  if (n == 0) return 0;
-  DBuf<unsigned long long> out(c, 1);
-  memset_async(c, out.get(), 0, 8);
-  HGP_LAUNCH(k_gain_bytes, std::min<uint32_t>(ceil_div(n, 256), 1024), 256, 0, c, list, count, toff, bound, out.get());
-  return read_scalar(c, out.get());
-}
-
 void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int mode, uint32_t omega, uint32_t repeat,
                    uint32_t discount, const uint32_t* part, const uint32_t* psizes, uint32_t* moves, int64_t* gains) {
   // hash order of partitions for the non-adjacent fallback, sorted on the device
@@ -1730,7 +1786,7 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   DBuf<uint32_t> okeys(c, kk), dorder(c, kk);
   HGP_LAUNCH(k_order_keys, ceil_div(L.k, 256), 256, 0, c, L.k, repeat, okeys.get(), dorder.get());
   radix_sort(c, okeys.get(), dorder.get(), L.k, 32);
-  GainArgs a;
+  GainArgs a{};
   a.tch = L.tch;
   a.toff = L.toff;
   a.hinfo = pl.hinfo.get();
@@ -1754,8 +1810,10 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   HGP_LAUNCH(k_gains_route, ceil_div(L.n, 256), 256, 0, c, a, L.n, bound.get(), ll.get(), bl.get(), counts.get());
   const uint32_t capmax = L.k <= GAIN_CAP ? std::max<uint32_t>(L.k, 1) : GAIN_CAP;
   const uint32_t lgrid = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID));
-  HGP_LAUNCH_B(gain_bytes(c, ll.get(), counts.get(), L.n, L.toff, bound.get()), k_gains_light, lgrid, 256,
-               4 * capmax * 12, c, a, ll.get(), counts.get(), bound.get(), capmax);
+  // when this kernel is profiled it counts its algorithmic bytes itself (no extra pass)
+  a.prof_bytes = prof_match(c, "k_gains_light") ? c.prof_dev : nullptr;
+  HGP_LAUNCH(k_gains_light, lgrid, 256, 4 * capmax * 12, c, a, ll.get(), counts.get(), bound.get(), capmax);
+  a.prof_bytes = nullptr;
   DBuf<uint32_t> hl;
   uint32_t nh = 0;
   // workgroup per node with an LDS table; hashed tables that would overflow go to HBM
