@@ -355,8 +355,8 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K* __restrict__ k
 }
 
 template <typename K>
-void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits) {
-  if (n <= 1 || bits <= 0) return;
+void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 0) {
+  if (n <= 1 || bits <= lo) return;
   const uint32_t nblocks = ceil_div(n, RS_TILE);
   DBuf<K> k2(c, n);
   DBuf<uint32_t> v2(c, vals ? n : 0);
@@ -366,7 +366,7 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits) {
   K* kb = k2.get();
   uint32_t* va = vals;
   uint32_t* vb = v2.get();
-  for (int shift = 0; shift < bits; shift += 8) {
+  for (int shift = lo; shift < bits; shift += 8) {
     // bytes: keys read once, one count per (digit, tile) written
     HGP_LAUNCH_B(n * sizeof(K) + 1024ull * nblocks, (k_rs_hist<K>), nblocks, RS_BLOCK, 0, c, ka, n, shift, nblocks,
                  counts.get());
@@ -453,6 +453,9 @@ void inclusive_scan(Ctx& c, const int32_t* in, int32_t* out, uint64_t n) { scan_
 
 void radix_sort(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int bits) { radix_impl<uint32_t>(c, keys, vals, n, bits); }
 void radix_sort(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int bits) { radix_impl<uint64_t>(c, keys, vals, n, bits); }
+void radix_sort_bits(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int lo, int hi) {
+  radix_impl<uint64_t>(c, keys, vals, n, hi, lo);
+}
 
 void segment_offsets(Ctx& c, const uint32_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off) {
   HGP_LAUNCH((k_segment_offsets<uint32_t>), ceil_div(n + 1, 256), 256, 0, c, keys, shift, n, nseg, off);

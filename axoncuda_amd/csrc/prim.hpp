@@ -17,6 +17,9 @@ void inclusive_scan(Ctx& c, const int32_t* in, int32_t* out, uint64_t n);
 // Stable LSD radix sorts over key bits [0, bits); values ride along (nullable).
 void radix_sort(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int bits);
 void radix_sort(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int bits);
+// Stable sort by key bits [lo, hi) only: when the input is already ordered by the bits below
+// lo, the result is ordered by bits [0, hi), in (hi - lo) / 8 passes instead of hi / 8.
+void radix_sort_bits(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int lo, int hi);
 // Stable sort over the keys' actual range: subtracts the minimum key (the keys are left
 // rebased) and sorts only the bits that the maximum difference needs.
 void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n);

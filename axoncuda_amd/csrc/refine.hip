@@ -2002,7 +2002,9 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
     DBuf<uint32_t> vals(c, m);
     DBuf<int64_t> cum(c, m);
     HGP_LAUNCH(k_size_events, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), L.sizes, n, rb, keys.get(), vals.get());
-    radix_sort(c, keys.get(), vals.get(), m, rb + kb);
+    // the events sit at 2 * rank (+1), so they are already in rank order: sorting by the
+    // partition bits alone orders them by (partition, rank)
+    radix_sort_bits(c, keys.get(), vals.get(), m, rb, rb + kb);
     HGP_LAUNCH(k_to_i64, ceil_div(m, 256), 256, 0, c, vals.get(), m, cum.get());
     inclusive_scan(c, cum.get(), cum.get(), m);
     segment_offsets(c, keys.get(), rb, m, L.k, poff.get());
