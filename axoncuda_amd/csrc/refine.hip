@@ -74,34 +74,79 @@ __global__ void __launch_bounds__(256) k_pp_small(const uint32_t* __restrict__ h
       if (large_flag) large_flag[e] = 1u;
     } else {
       if (large_flag) large_flag[e] = 0u;
-      uint32_t qs[PP_SMALL], cs[PP_SMALL], ds[PP_SMALL];
-      uint32_t m = 0;
+      const uint32_t len = (uint32_t)(t - s);
       const uint32_t sc = INB ? srcc[e] : 0u;
-      for (uint64_t i = s; i < t; ++i) {
-        const uint32_t q = part[hedges[i]];
-        uint32_t k = 0;
-        while (k < m && qs[k] != q) ++k;
-        if (k == m) {
-          qs[m] = q;
-          cs[m] = 0;
-          ds[m] = 0;
-          ++m;
-        }
-        cs[k] += 1;
-        if (INB) ds[k] += (i - s) >= sc ? 1u : 0u;
+      // Pins and their partitions are fetched in batches of unconditional loads (indexes
+      // clamped to the last pin): two memory round trips per batch instead of two per pin.
+      uint32_t pq[PP_SMALL];
+#pragma unroll
+      for (uint32_t u = 0; u < PP_SMALL; ++u) pq[u] = NONE;
+      if (len > 0) {  // an empty hedge has no pin to clamp to
+        uint32_t x[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) x[u] = hedges[s + min(u, len - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) pq[u] = part[x[u]];
       }
-      for (uint32_t k = 0; k < m; ++k) {
-        pp[s + k] = make_uint2(qs[k], cs[k]);
-        if (INB) {
-          pp_dcnt[s + k] = ds[k];
-          if (ds[k]) {
-            if (k_lds) atomicAdd(&hist[qs[k]], 1u);
-            else atomicAdd(&inbsz[qs[k]], 1u);
+      if (len > 4) {
+        uint32_t x[PP_SMALL - 4];
+#pragma unroll
+        for (uint32_t u = 4; u < PP_SMALL; ++u) x[u - 4] = hedges[s + min(u, len - 1)];
+#pragma unroll
+        for (uint32_t u = 4; u < PP_SMALL; ++u) pq[u] = part[x[u - 4]];
+      }
+      // (partition, count, destination count) in first-appearance order; static indexes
+      // only, so the tables stay in registers
+      uint32_t qs[PP_SMALL], cs[PP_SMALL], ds[PP_SMALL];
+#pragma unroll
+      for (uint32_t k = 0; k < PP_SMALL; ++k) {
+        qs[k] = NONE;
+        cs[k] = 0;
+        ds[k] = 0;
+      }
+      uint32_t m = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < PP_SMALL; ++u) {
+        if (u < len) {
+          const uint32_t q = pq[u];
+          const uint32_t d = (INB && u >= sc) ? 1u : 0u;
+          bool found = false;
+#pragma unroll
+          for (uint32_t k = 0; k < PP_SMALL; ++k) {
+            const bool hit = qs[k] == q;
+            cs[k] += hit ? 1u : 0u;
+            if (INB) ds[k] += hit ? d : 0u;
+            found |= hit;
+          }
+          if (!found) {
+#pragma unroll
+            for (uint32_t k = 0; k < PP_SMALL; ++k) {
+              const bool put = k == m;
+              qs[k] = put ? q : qs[k];
+              cs[k] = put ? 1u : cs[k];
+              if (INB) ds[k] = put ? d : ds[k];
+            }
+            ++m;
+          }
+        }
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < PP_SMALL; ++k) {
+        if (k < m) {
+          pp[s + k] = make_uint2(qs[k], cs[k]);
+          if (INB) {
+            pp_dcnt[s + k] = ds[k];
+            if (ds[k]) {
+              if (k_lds) atomicAdd(&hist[qs[k]], 1u);
+              else atomicAdd(&inbsz[qs[k]], 1u);
+            }
           }
         }
       }
       lambda[e] = m;
-      hinfo[e] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), m, w ? fixed_weight(w[e]) : 0u);
+      // a single entry rides in the record in place of the list start
+      const uint64_t x = m == 1 ? (((uint64_t)cs[0] << 32) | qs[0]) : s;
+      hinfo[e] = make_uint4((uint32_t)x, (uint32_t)(x >> 32), m, w ? fixed_weight(w[e]) : 0u);
     }
   }
   if (INB && k_lds) {
@@ -165,6 +210,8 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
   // HBM tables were written by atomics at the device cache: read them there too
   auto ld = [](uint32_t* x) { return GMEM ? atomic_load_relaxed(x) : *x; };
   uint32_t base = 0;
+  uint2 first = make_uint2(0u, 0u);  // entry 0, held by the lane that wrote it
+  bool wrote_first = false;
   for (uint32_t b = 0; b < cap; b += 64) {
     const uint32_t i = b + l;
     const uint32_t q = ld(&keys[i]);
@@ -172,7 +219,12 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
     const uint64_t m = __ballot(has);
     if (has) {
       const uint32_t o = base + lanes_below(m);
-      pp[s + o] = make_uint2(q, ld(&cnt[i]));
+      const uint2 ent = make_uint2(q, ld(&cnt[i]));
+      pp[s + o] = ent;
+      if (o == 0) {
+        first = ent;
+        wrote_first = true;
+      }
       if (INB) {
         const uint32_t dc = ld(&dcnt[i]);
         pp_dcnt[s + o] = dc;
@@ -181,9 +233,15 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
     }
     base += (uint32_t)__popcll(m);
   }
+  const uint64_t fm = __ballot(wrote_first);
+  const int src = fm ? __ffsll((long long)fm) - 1 : 0;
+  first.x = __shfl(first.x, src, 64);
+  first.y = __shfl(first.y, src, 64);
   if (l == 0) {
     lambda[e] = base;
-    hinfo[e] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), base, w ? fixed_weight(w[e]) : 0u);
+    // a single entry rides in the record in place of the list start
+    const uint64_t x = base == 1 ? (((uint64_t)first.y << 32) | first.x) : s;
+    hinfo[e] = make_uint4((uint32_t)x, (uint32_t)(x >> 32), base, w ? fixed_weight(w[e]) : 0u);
   }
 }
 
@@ -215,7 +273,7 @@ __global__ void k_pp_gcap(const uint32_t* __restrict__ list, uint32_t m, const u
 struct GainArgs {
   const uint32_t* tch;
   const uint64_t* toff;
-  const uint4* hinfo;  // per hedge: list start (lo, hi), list length (lambda), fixed-point weight
+  const uint4* hinfo;  // per hedge: list start (lo, hi) or, for one entry, the entry; length (lambda); weight
   const uint2* pp;     // per list entry: (partition, pin count)
   const uint32_t* part;
   const uint32_t* sizes;
@@ -332,7 +390,7 @@ __device__ __forceinline__ void gains_tiny_node(const GainArgs& a, uint32_t v) {
     const uint64_t b = ((uint64_t)hi.y << 32) | hi.x;
     const uint32_t lam = hi.z;
     for (uint32_t x = 0; x < lam; ++x) {
-      const uint2 pe = a.pp[b + x];
+      const uint2 pe = lam == 1 ? make_uint2(hi.x, hi.y) : a.pp[b + x];  // one entry: in the record
       const uint32_t q = pe.x;
       if (q == p) {
         const uint32_t c = pe.y;
@@ -418,10 +476,10 @@ __device__ __forceinline__ uint64_t gains_light_node(const GainArgs& a, uint32_t
   const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
   int64_t saving = 0, Wv = 0;
   uint64_t entries = 0;
-  Wv += wave_visit_hedge_lists(a.tch, t0, t1, a.hinfo, [&](const EntryVisit& ev) {
+  Wv += wave_visit_hedge_lists(a.tch, t0, t1, a.hinfo, a.pp, [&](const EntryVisit& ev) {
         if (!ev.act) return;
         ++entries;
-        const uint2 pe = a.pp[ev.abs];
+        const uint2 pe = ev.pe;
         const uint32_t q = pe.x;
         const int64_t We = (int64_t)ev.w;
         if (q == p) {
@@ -504,9 +562,9 @@ __global__ void __launch_bounds__(256) k_gains_heavy(GainArgs a, const uint32_t*
   const uint64_t cnt = t1 - t0;
   const uint64_t lo = t0 + cnt * wv / 4, hi = t0 + cnt * (wv + 1) / 4;
   int64_t saving = 0, Wv = 0;
-  Wv += wave_visit_hedge_lists(a.tch, lo, hi, a.hinfo, [&](const EntryVisit& ev) {
+  Wv += wave_visit_hedge_lists(a.tch, lo, hi, a.hinfo, a.pp, [&](const EntryVisit& ev) {
         if (!ev.act) return;
-        const uint2 pe = a.pp[ev.abs];
+        const uint2 pe = ev.pe;
         const uint32_t q = pe.x;
         const int64_t We = (int64_t)ev.w;
         if (q == p) {
@@ -623,9 +681,9 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
   const uint64_t cnt = t1 - t0;
   const uint64_t lo = t0 + cnt * wv / 16, hi = t0 + cnt * (wv + 1) / 16;
   int64_t saving = 0, Wv = 0;
-  Wv += wave_visit_hedge_lists(a.tch, lo, hi, a.hinfo, [&](const EntryVisit& ev) {
+  Wv += wave_visit_hedge_lists(a.tch, lo, hi, a.hinfo, a.pp, [&](const EntryVisit& ev) {
     if (!ev.act) return;
-    const uint2 pe = a.pp[ev.abs];
+    const uint2 pe = ev.pe;
     const uint32_t q = pe.x;
     const int64_t We = (int64_t)ev.w;
     if (q == p) {
@@ -1491,11 +1549,13 @@ __global__ void k_inbound_events(const uint32_t* __restrict__ part, const uint32
     o += 2;
   }
 }
-// count of destination pins of hedge e in partition q (0 when absent)
-__device__ __forceinline__ int64_t dst_count(const uint4* hinfo, const uint2* pp, const uint32_t* pp_dcnt, uint32_t e,
-                                             uint32_t q) {
+// count of destination pins of hedge e in partition q (0 when absent).  The lists start at
+// the hedge's pin offset; a list of one entry is read from the record.
+__device__ __forceinline__ int64_t dst_count(const uint4* hinfo, const uint64_t* hoff, const uint2* pp,
+                                             const uint32_t* pp_dcnt, uint32_t e, uint32_t q) {
   const uint4 hi = hinfo[e];
-  const uint64_t s = ((uint64_t)hi.y << 32) | hi.x;
+  const uint64_t s = hoff[e];
+  if (hi.z == 1) return hi.x == q ? pp_dcnt[s] : 0;
   for (uint32_t i = 0; i < hi.z; ++i)
     if (pp[s + i].x == q) return pp_dcnt[s + i];
   return 0;
@@ -1503,7 +1563,8 @@ __device__ __forceinline__ int64_t dst_count(const uint4* hinfo, const uint2* pp
 // The first event of every (partition, hedge) run looks up the run's starting destination
 // count once (a scan of the hedge's partition list); the run's other events read it there.
 __global__ void k_inbound_group_init(const uint64_t* __restrict__ keys, uint64_t m, int eb, int rb,
-                                     const uint4* __restrict__ hinfo, const uint2* __restrict__ pp,
+                                     const uint4* __restrict__ hinfo, const uint64_t* __restrict__ hoff,
+                                     const uint2* __restrict__ pp,
                                      const uint32_t* __restrict__ pp_dcnt, int64_t* __restrict__ ginit) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
@@ -1511,7 +1572,7 @@ __global__ void k_inbound_group_init(const uint64_t* __restrict__ keys, uint64_t
   if (i > 0 && (keys[i - 1] >> rb) == (key >> rb)) return;
   const uint32_t q = (uint32_t)(key >> (eb + rb));
   const uint32_t e = (uint32_t)((key >> rb) & (((uint64_t)1 << eb) - 1));
-  ginit[i] = dst_count(hinfo, pp, pp_dcnt, e, q);
+  ginit[i] = dst_count(hinfo, hoff, pp, pp_dcnt, e, q);
 }
 __global__ void k_inbound_transitions(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                       const int64_t* __restrict__ cum, uint64_t m, int rb,
@@ -1670,7 +1731,7 @@ namespace {
 
 struct PartitionLists {
   DBuf<uint2> pp;      // (partition, pin count) per list entry, at the hedge's own offsets
-  DBuf<uint4> hinfo;   // per hedge: list start (lo, hi), length, fixed-point weight
+  DBuf<uint4> hinfo;   // per hedge: list start (lo, hi) or its one entry, length, fixed-point weight
   DBuf<uint32_t> dcnt, lambda, inbsz;
 };
 
@@ -2033,8 +2094,8 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
       DBuf<uint32_t> tflag(c, m);
       DBuf<int32_t> tdelta(c, m);
       DBuf<int64_t> ginit(c, m);
-      HGP_LAUNCH(k_inbound_group_init, ceil_div(m, 256), 256, 0, c, keys.get(), m, eb, rb, pl.hinfo.get(), pl.pp.get(),
-                 pl.dcnt.get(), ginit.get());
+      HGP_LAUNCH(k_inbound_group_init, ceil_div(m, 256), 256, 0, c, keys.get(), m, eb, rb, pl.hinfo.get(), L.hoff,
+                 pl.pp.get(), pl.dcnt.get(), ginit.get());
       HGP_LAUNCH(k_inbound_transitions, ceil_div(m, 256), 256, 0, c, keys.get(), vals.get(), cum.get(), m, rb,
                  ginit.get(), tflag.get(), tdelta.get());
       DBuf<uint64_t> tpos(c, m + 1);

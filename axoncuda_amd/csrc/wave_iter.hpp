@@ -35,8 +35,9 @@ struct EntryVisit {
   uint64_t j;    // index into the touching list
   uint32_t e;    // hedge id
   uint32_t pos;  // position inside the hedge's range
-  uint64_t abs;  // absolute entry index
+  uint64_t abs;  // absolute entry index (wave_visit_entries only)
   uint32_t w;    // the hedge's fixed-point weight (wave_visit_hedge_lists only)
+  uint2 pe;      // the list entry itself (wave_visit_hedge_lists only)
 };
 
 template <typename StartF, typename LenF, typename F>
@@ -77,12 +78,15 @@ __device__ __forceinline__ void wave_visit_entries(const uint32_t* __restrict__ 
   }
 }
 
-// The same traversal over per-hedge lists described by one 16-byte record per hedge
-// (start lo, start hi, length, fixed-point weight), so each touching hedge costs one load.
-// Returns this lane's share of the summed weights of the touching hedges.
+// The same traversal over per-hedge lists of 8-byte entries described by one 16-byte record
+// per hedge (start lo, start hi, length, fixed-point weight), so each touching hedge costs one
+// load.  A list of one entry is held in the record itself, in place of the start, and is not
+// read from `lists`.  The visitor gets the entry in `pe`.  Returns this lane's share of the
+// summed weights of the touching hedges.
 template <typename F>
 __device__ __forceinline__ int64_t wave_visit_hedge_lists(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
-                                                          const uint4* __restrict__ hinfo, F&& visit) {
+                                                          const uint4* __restrict__ hinfo,
+                                                          const uint2* __restrict__ lists, F&& visit) {
   const uint32_t l = lane_id();
   int64_t wsum = 0;
   for (uint64_t jb = t0; jb < t1; jb += 64) {
@@ -115,9 +119,12 @@ __device__ __forceinline__ int64_t wave_visit_hedge_lists(const uint32_t* __rest
       ev.w = __shfl(wf, (int)k, 64);
       const uint64_t os = shfl_u64(s, k);
       const uint32_t oex = __shfl(excl, (int)k, 64);
+      const uint32_t olen = __shfl(len, (int)k, 64);
       ev.j = jb + k;
       ev.pos = fi - oex;
-      ev.abs = os + ev.pos;
+      ev.abs = 0;
+      ev.pe = olen == 1 ? make_uint2((uint32_t)os, (uint32_t)(os >> 32))
+                        : (ev.act ? lists[os + ev.pos] : make_uint2(0u, 0u));
       visit(ev);
     }
   }
