@@ -880,6 +880,279 @@ __global__ void __launch_bounds__(256) k_gains_route(GainArgs a, uint32_t n, uin
   if (to_block) block[base_b + lanes_below(mb)] = v;
 }
 
+// ---- k-way gains from partition masks (k <= MASK_K) ------------------------------------
+// With at most 64 partitions a hedge's partition list fits in two 64-bit masks: the
+// partitions among its pins and those holding exactly one of them.  For a node v in p:
+// Wv = sum of its hedge weights, saving = sum of the weights whose single-pin mask has p
+// (v is then the hedge's only pin in p), conn(q) = sum of the weights whose mask has q; the
+// gain of moving to q is saving - Wv + conn(q), exactly what the list kernels compute, at one
+// 16-byte record per touching hedge whatever the hedge's connectivity.
+constexpr uint32_t MASK_K = 64;
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int o) {
+  const uint32_t lo = __shfl_xor((uint32_t)v, o, 64), hi = __shfl_xor((uint32_t)(v >> 32), o, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ void mask_add(uint64_t& s1, uint64_t& s2, uint32_t q) {
+  const uint64_t bit = 1ull << q;
+  s2 |= s1 & bit;  // seen before: at least two pins
+  s1 |= bit;
+}
+// Hedges of at most PP_SMALL pins, one thread each (batched unconditional loads, indexes
+// clamped to the last pin); longer ones are flagged for k_hmask_large.
+__global__ void __launch_bounds__(256) k_hmask_small(const uint32_t* __restrict__ hedges,
+                                                     const uint64_t* __restrict__ hoff,
+                                                     const uint32_t* __restrict__ part, uint32_t E,
+                                                     uint4* __restrict__ hm, uint32_t* __restrict__ large_flag) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const uint64_t s = hoff[e], t = hoff[e + 1];
+  const uint64_t hl = t - s;
+  if (large_flag) large_flag[e] = hl > PP_SMALL ? 1u : 0u;
+  if (hl > PP_SMALL) return;
+  const uint32_t len = (uint32_t)hl;
+  uint64_t s1 = 0, s2 = 0;
+  if (len > 0) {
+    uint32_t x[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) x[u] = hedges[s + min(u, len - 1)];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t q = part[x[u]];
+      if (u < len) mask_add(s1, s2, q);
+    }
+    if (len > 4) {
+      uint32_t y[PP_SMALL - 4];
+#pragma unroll
+      for (uint32_t u = 4; u < PP_SMALL; ++u) y[u - 4] = hedges[s + min(u, len - 1)];
+#pragma unroll
+      for (uint32_t u = 4; u < PP_SMALL; ++u) {
+        const uint32_t q = part[y[u - 4]];
+        if (u < len) mask_add(s1, s2, q);
+      }
+    }
+  }
+  const uint64_t single = s1 & ~s2;
+  hm[e] = make_uint4((uint32_t)s1, (uint32_t)(s1 >> 32), (uint32_t)single, (uint32_t)(single >> 32));
+}
+// One wave per listed hedge: lanes stride over the pins, then the (seen, seen twice) masks
+// are combined across lanes.
+__global__ void __launch_bounds__(64) k_hmask_large(const uint32_t* __restrict__ hedges,
+                                                    const uint64_t* __restrict__ hoff,
+                                                    const uint32_t* __restrict__ part,
+                                                    const uint32_t* __restrict__ list, uint4* __restrict__ hm) {
+  const uint32_t e = list[blockIdx.x];
+  const uint32_t l = lane_id();
+  const uint64_t s = hoff[e], t = hoff[e + 1];
+  uint64_t s1 = 0, s2 = 0;
+  for (uint64_t i = s + l; i < t; i += 64) mask_add(s1, s2, part[hedges[i]]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t o1 = shfl_xor_u64(s1, o), o2 = shfl_xor_u64(s2, o);
+    s2 |= o2 | (s1 & o1);
+    s1 |= o1;
+  }
+  if (l == 0) {
+    const uint64_t single = s1 & ~s2;
+    hm[e] = make_uint4((uint32_t)s1, (uint32_t)(s1 >> 32), (uint32_t)single, (uint32_t)(single >> 32));
+  }
+}
+
+// Routing for the mask path: nodes touching at most GAIN_TINY hedges whose masks list at most
+// GAIN_TINY partitions in all are scored right here (one thread, a register table, as
+// gains_tiny_node); every other node goes to the wave kernel's list.
+__global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint4* __restrict__ hm,
+                                                          const float* __restrict__ w, uint32_t n,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t t0 = 0, t1 = 0;
+  if (v < n) {
+    t0 = a.toff[v];
+    t1 = a.toff[v + 1];
+  }
+  const uint64_t deg = t1 - t0;
+  uint32_t b = GAIN_TINY + 1;
+  if (v < n && deg <= GAIN_TINY) {
+    b = 0;
+    if (deg > 0) {  // the entry bound: partitions per touching hedge, summed (batched loads)
+      const uint32_t d = (uint32_t)deg;
+      uint32_t e[GAIN_TINY];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) e[u] = a.tch[t0 + min(u, d - 1)];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint2 m = *(const uint2*)&hm[e[u]];
+        b += u < d ? (uint32_t)(__popc(m.x) + __popc(m.y)) : 0u;
+      }
+      if (d > 4) {
+#pragma unroll
+        for (uint32_t u = 4; u < GAIN_TINY; ++u) e[u] = a.tch[t0 + min(u, d - 1)];
+#pragma unroll
+        for (uint32_t u = 4; u < GAIN_TINY; ++u) {
+          const uint2 m = *(const uint2*)&hm[e[u]];
+          b += u < d ? (uint32_t)(__popc(m.x) + __popc(m.y)) : 0u;
+        }
+      }
+    }
+  }
+  const bool tiny = v < n && b <= GAIN_TINY;
+  if (tiny) {
+    const uint32_t p = a.part[v], s = a.sizes[v];
+    uint32_t qs[GAIN_TINY];
+    int64_t cs[GAIN_TINY];
+#pragma unroll
+    for (uint32_t k = 0; k < GAIN_TINY; ++k) {
+      qs[k] = NONE;
+      cs[k] = 0;
+    }
+    uint32_t m = 0;
+    int64_t saving = 0, Wv = 0;
+    for (uint64_t j = t0; j < t1; ++j) {
+      const uint32_t e = a.tch[j];
+      const uint4 r = hm[e];
+      const int64_t We = (int64_t)fixed_weight(w[e]);
+      Wv += We;
+      uint64_t mk = ((uint64_t)r.y << 32) | r.x;
+      const uint64_t sg = ((uint64_t)r.w << 32) | r.z;
+      if ((sg >> p) & 1ull) saving += We;
+      mk &= ~(1ull << p);
+      while (mk) {
+        const uint32_t q = (uint32_t)__ffsll((long long)mk) - 1;
+        mk &= mk - 1;
+        bool found = false;
+#pragma unroll
+        for (uint32_t k = 0; k < GAIN_TINY; ++k) {
+          const bool hit = qs[k] == q;
+          cs[k] += hit ? We : 0;
+          found |= hit;
+        }
+        if (!found) {
+#pragma unroll
+          for (uint32_t k = 0; k < GAIN_TINY; ++k) {
+            const bool put = k == m;
+            qs[k] = put ? q : qs[k];
+            cs[k] = put ? We : cs[k];
+          }
+          ++m;
+        }
+      }
+    }
+    const int64_t base = saving - Wv;
+    int64_t bg = INT64_MIN;
+    uint32_t bh = 0, bq = NONE;
+#pragma unroll
+    for (uint32_t k = 0; k < GAIN_TINY; ++k) {
+      const uint32_t q = qs[k];
+      if (q == NONE || cs[k] <= 0 || !target_ok(a, q, p, s)) continue;
+      const int64_t g = base + cs[k];
+      const uint32_t h = mix32(q + a.repeat);
+      if (bq == NONE || g > bg || (g == bg && h > bh)) {
+        bg = g;
+        bh = h;
+        bq = q;
+      }
+    }
+    if (bq == NONE) {  // first valid partition in hash order without a positive connection
+      for (uint32_t x = 0; x < a.k; ++x) {
+        const uint32_t q = a.order[x];
+        if (!target_ok(a, q, p, s)) continue;
+        bool adjacent = false;
+#pragma unroll
+        for (uint32_t k = 0; k < GAIN_TINY; ++k) adjacent |= qs[k] == q && cs[k] > 0;
+        if (!adjacent) {
+          bq = q;
+          break;
+        }
+      }
+      bg = base;
+    }
+    a.moves[v] = bq;
+    a.gains[v] = bq == NONE ? 0 : bg;
+  }
+  // list slots: one atomic per workgroup, then wave and lane offsets
+  const bool to_list = v < n && !tiny;
+  __shared__ uint32_t s_cnt[4], s_base;
+  const uint32_t l = lane_id(), wv = threadIdx.x / 64;
+  const uint64_t ml = __ballot(to_list);
+  if (l == 0) s_cnt[wv] = (uint32_t)__popcll(ml);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    s_base = t ? atomicAdd(count, t) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = s_base;
+  for (uint32_t x = 0; x < wv; ++x) base += s_cnt[x];
+  if (to_list) list[base + lanes_below(ml)] = v;
+}
+
+// One wave per listed node, lane q holding partition q's connection.  The wave loads 64
+// touching hedges' records at a time; each record is then broadcast and every lane adds the
+// weight when its partition's bit is set.  psizes and the hash order are per-lane constants
+// of the launch.  Persistent: the grid strides over the `*count` listed nodes.
+__global__ void __launch_bounds__(256) k_gains_mask(GainArgs a, const uint4* __restrict__ hm,
+                                                    const float* __restrict__ w, const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count) {
+  const uint32_t l = lane_id(), wv = threadIdx.x / 64;
+  const bool qvalid = l < a.k;
+  const uint32_t psz = qvalid ? a.psizes[l] : 0u;
+  const uint32_t hq = mix32(l + a.repeat);
+  const uint32_t ord = qvalid ? a.order[l] : NONE;  // lane i: the i-th partition in hash order
+  const uint32_t nlist = *count;
+  uint64_t bytes = 0;
+  for (uint32_t idx = blockIdx.x * 4 + wv; idx < nlist; idx += gridDim.x * 4) {
+    const uint32_t v = list[idx];
+    const uint32_t p = a.part[v], s = a.sizes[v];
+    const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
+    int64_t conn = 0, wl = 0, sl = 0;
+    for (uint64_t jb = t0; jb < t1; jb += 64) {
+      const uint64_t j = jb + l;
+      uint32_t mlo = 0, mhi = 0, we = 0;
+      if (j < t1) {
+        const uint32_t e = a.tch[j];
+        const uint4 r = hm[e];
+        we = fixed_weight(w[e]);
+        mlo = r.x;
+        mhi = r.y;
+        const uint64_t sg = ((uint64_t)r.w << 32) | r.z;
+        wl += we;
+        if ((sg >> p) & 1ull) sl += we;
+      }
+      const uint32_t cnt = (uint32_t)min<uint64_t>(64, t1 - jb);
+      for (uint32_t i = 0; i < cnt; ++i) {
+        const uint32_t lo = __builtin_amdgcn_readlane(mlo, i), hi = __builtin_amdgcn_readlane(mhi, i);
+        const uint32_t wi = __builtin_amdgcn_readlane(we, i);
+        const uint32_t word = l < 32 ? lo : hi;
+        conn += ((word >> (l & 31)) & 1u) ? (int64_t)wi : 0;
+      }
+    }
+    const int64_t base = wave_sum(sl) - wave_sum(wl);
+    // best adjacent target: positive connection, room in the target; ties by the hash
+    const bool ok = qvalid && l != p && (uint32_t)(psz + s - s / a.discount) <= a.omega;
+    int64_t bg = base + conn;
+    uint32_t bh = hq, bq = (ok && conn > 0) ? l : NONE;
+    wave_best(bg, bh, bq);
+    if (bq == NONE) {  // first valid partition in hash order without a positive connection
+      const uint32_t qi = ord == NONE ? 0u : ord;
+      const uint32_t pq = __shfl(psz, (int)qi, 64);
+      const int64_t cq = __shfl(conn, (int)qi, 64);
+      const bool fit = ord != NONE && qi != p && (uint32_t)(pq + s - s / a.discount) <= a.omega && cq <= 0;
+      const uint64_t fm = __ballot(fit);
+      bq = fm ? __shfl(ord, __ffsll((long long)fm) - 1, 64) : NONE;
+      bg = base;
+    }
+    if (l == 0) {
+      a.moves[v] = bq;
+      a.gains[v] = bq == NONE ? 0 : bg;
+    }
+    bytes += (l == 0 ? 36 : 0) + 24 * (uint64_t)((t1 - t0 + 63 - l) / 64);
+  }
+  if (a.prof_bytes) {
+    bytes = wave_sum(bytes);
+    if (l == 0 && bytes) atomicAdd(&a.prof_bytes[blockIdx.x % PROF_SLOTS], (unsigned long long)bytes);
+  }
+}
+
 // ---- Jacobi label propagation (device initial partitioner) ------------------------------
 // movers: positive-gain proposals, keyed for a sort by gain descending (ties: node id)
 __global__ void k_jacobi_movers(const uint32_t* __restrict__ moves, const int64_t* __restrict__ gains, uint32_t n,
@@ -1733,6 +2006,7 @@ struct PartitionLists {
   DBuf<uint2> pp;      // (partition, pin count) per list entry, at the hedge's own offsets
   DBuf<uint4> hinfo;   // per hedge: list start (lo, hi) or its one entry, length, fixed-point weight
   DBuf<uint32_t> dcnt, lambda, inbsz;
+  DBuf<uint4> hm;      // instead of the lists (k-way, k <= MASK_K): per hedge, its partition masks
 };
 
 // Partition lists of every hedge.  want_inbound adds the destination counts and the
@@ -1754,12 +2028,32 @@ struct LevelLayout {
   uint32_t cas_n[3] = {0, 0, 0};
 };
 
+// masks: only the partition masks (k-way with k <= MASK_K: nothing else reads the lists).
 void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, PartitionLists& pl, bool want_inbound,
-                           LevelLayout* layout = nullptr) {
+                           LevelLayout* layout = nullptr, bool masks = false) {
   LevelLayout local;
   LevelLayout& Y = layout ? *layout : local;
   if (!Y.ready) Y.P = L.e ? read_scalar(c, L.hoff + L.e) : 0;
   const uint64_t P = Y.P;
+  if (masks) {
+    DBuf<uint32_t> lflag;
+    if (!Y.ready) lflag = DBuf<uint32_t>(c, L.e);
+    pl.hm = DBuf<uint4>(c, std::max<uint32_t>(L.e, 1));
+    HGP_LAUNCH(k_hmask_small, ceil_div(L.e, 256), 256, 0, c, L.hedges, L.hoff, part, L.e, pl.hm.get(), lflag.get());
+    if (!Y.ready) {  // the hedges above PP_SMALL pins (no HBM tables at k <= MASK_K)
+      Y.ready = true;
+      DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
+      exclusive_scan(c, lflag.get(), pos.get(), L.e);
+      Y.nl = (uint32_t)read_scalar(c, pos.get() + L.e);
+      if (Y.nl) {
+        Y.large = DBuf<uint32_t>(c, Y.nl);
+        DBuf<uint32_t> small(c, L.e - Y.nl);
+        HGP_LAUNCH(k_split, ceil_div(L.e, 256), 256, 0, c, lflag.get(), pos.get(), L.e, Y.large.get(), small.get());
+      }
+    }
+    if (Y.nl) HGP_LAUNCH(k_hmask_large, Y.nl, 64, 0, c, L.hedges, L.hoff, part, Y.large.get(), pl.hm.get());
+    return;
+  }
   pl.pp = DBuf<uint2>(c, P);
   pl.hinfo = DBuf<uint4>(c, L.e);
   pl.lambda = DBuf<uint32_t>(c, L.e);
@@ -1863,6 +2157,15 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   a.incc = mode == HGP_MODE_INCC;
   a.moves = moves;
   a.gains = gains;
+  if (pl.hm.get()) {  // k-way from partition masks: tiny nodes in the router, the rest a wave each
+    DBuf<uint32_t> ml(c, std::max<uint32_t>(L.n, 1)), mc(c, 1);
+    memset_async(c, mc.get(), 0, 4);
+    HGP_LAUNCH(k_gains_route_mask, ceil_div(L.n, 256), 256, 0, c, a, pl.hm.get(), L.w, L.n, ml.get(), mc.get());
+    a.prof_bytes = prof_match(c, "k_gains_mask") ? c.prof_dev : nullptr;
+    HGP_LAUNCH(k_gains_mask, std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID)), 256, 0, c, a,
+               pl.hm.get(), L.w, ml.get(), mc.get());
+    return;
+  }
   // tiny nodes are scored by the router; the light and block lists and their counts stay on
   // the device, and the persistent light and block kernels stride over them
   DBuf<uint64_t> bound(c, L.n);
@@ -1996,7 +2299,7 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
   const uint32_t g = ceil_div(n, 256);
   const bool incc = mode == HGP_MODE_INCC;
   PartitionLists pl;
-  build_partition_lists(c, L, part, pl, incc, layout);
+  build_partition_lists(c, L, part, pl, incc, layout, !incc && L.k <= MASK_K);
   st.moves = DBuf<uint32_t>(c, n);
   st.gains = DBuf<int64_t>(c, n);
   st.ranks = DBuf<uint32_t>(c, n);
@@ -2192,7 +2495,7 @@ uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rou
   const uint32_t n = L.n;
   if (n == 0) return 0;
   PartitionLists pl;
-  build_partition_lists(c, L, part, pl, false);
+  build_partition_lists(c, L, part, pl, false, nullptr, L.k <= MASK_K);
   DBuf<uint32_t> moves(c, n);
   DBuf<int64_t> gains(c, n);
   compute_gains(c, L, pl, HGP_MODE_KWAY, omega, round, NONE, part, psizes, moves.get(), gains.get());  // strict sizes
