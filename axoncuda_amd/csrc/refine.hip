@@ -16,6 +16,7 @@
 //   6. best prefix (best_move_functor, headers/data_types.cuh:54-77) and apply (:328-370).
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 
 #include "prim.hpp"
 #include "stages.hpp"
@@ -958,9 +959,10 @@ __global__ void __launch_bounds__(64) k_hmask_large(const uint32_t* __restrict__
   }
 }
 
-// Routing for the mask path: nodes touching at most GAIN_TINY hedges whose masks list at most
-// GAIN_TINY partitions in all are scored right here (one thread, a register table, as
-// gains_tiny_node); every other node goes to the wave kernel's list.
+// Routing for the mask path: a node touching at most GAIN_TINY hedges is scored right here,
+// one thread per node with its hedges' masks and weights in registers (batched loads): the
+// candidate targets are the bits of the masks' union, each scored by one pass over the
+// masks.  Every other node goes to the wave kernel's list.
 __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint4* __restrict__ hm,
                                                           const float* __restrict__ w, uint32_t n,
                                                           uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
@@ -971,80 +973,64 @@ __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint
     t1 = a.toff[v + 1];
   }
   const uint64_t deg = t1 - t0;
-  uint32_t b = GAIN_TINY + 1;
-  if (v < n && deg <= GAIN_TINY) {
-    b = 0;
-    if (deg > 0) {  // the entry bound: partitions per touching hedge, summed (batched loads)
-      const uint32_t d = (uint32_t)deg;
-      uint32_t e[GAIN_TINY];
-#pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) e[u] = a.tch[t0 + min(u, d - 1)];
-#pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) {
-        const uint2 m = *(const uint2*)&hm[e[u]];
-        b += u < d ? (uint32_t)(__popc(m.x) + __popc(m.y)) : 0u;
-      }
-      if (d > 4) {
-#pragma unroll
-        for (uint32_t u = 4; u < GAIN_TINY; ++u) e[u] = a.tch[t0 + min(u, d - 1)];
-#pragma unroll
-        for (uint32_t u = 4; u < GAIN_TINY; ++u) {
-          const uint2 m = *(const uint2*)&hm[e[u]];
-          b += u < d ? (uint32_t)(__popc(m.x) + __popc(m.y)) : 0u;
-        }
-      }
-    }
-  }
-  const bool tiny = v < n && b <= GAIN_TINY;
-  if (tiny) {
+  const bool small = v < n && deg <= GAIN_TINY;
+  if (small) {
     const uint32_t p = a.part[v], s = a.sizes[v];
-    uint32_t qs[GAIN_TINY];
-    int64_t cs[GAIN_TINY];
-#pragma unroll
-    for (uint32_t k = 0; k < GAIN_TINY; ++k) {
-      qs[k] = NONE;
-      cs[k] = 0;
-    }
-    uint32_t m = 0;
+    const uint32_t d = (uint32_t)deg;
+    uint64_t mk[GAIN_TINY];
+    uint32_t wt[GAIN_TINY];
     int64_t saving = 0, Wv = 0;
-    for (uint64_t j = t0; j < t1; ++j) {
-      const uint32_t e = a.tch[j];
-      const uint4 r = hm[e];
-      const int64_t We = (int64_t)fixed_weight(w[e]);
-      Wv += We;
-      uint64_t mk = ((uint64_t)r.y << 32) | r.x;
-      const uint64_t sg = ((uint64_t)r.w << 32) | r.z;
-      if ((sg >> p) & 1ull) saving += We;
-      mk &= ~(1ull << p);
-      while (mk) {
-        const uint32_t q = (uint32_t)__ffsll((long long)mk) - 1;
-        mk &= mk - 1;
-        bool found = false;
 #pragma unroll
-        for (uint32_t k = 0; k < GAIN_TINY; ++k) {
-          const bool hit = qs[k] == q;
-          cs[k] += hit ? We : 0;
-          found |= hit;
-        }
-        if (!found) {
-#pragma unroll
-          for (uint32_t k = 0; k < GAIN_TINY; ++k) {
-            const bool put = k == m;
-            qs[k] = put ? q : qs[k];
-            cs[k] = put ? We : cs[k];
-          }
-          ++m;
-        }
-      }
+    for (uint32_t u = 0; u < GAIN_TINY; ++u) {
+      mk[u] = 0;
+      wt[u] = 0;
     }
+    // records of hedges u0 .. u0 + B - 1: all loads issued first, then consumed
+    auto batch = [&](auto B_, uint32_t u0) {
+      constexpr uint32_t B = decltype(B_)::value;
+      uint32_t e[B];
+#pragma unroll
+      for (uint32_t u = 0; u < B; ++u) e[u] = a.tch[t0 + min(u0 + u, d - 1)];
+      uint4 r[B];
+      float wf[B];
+#pragma unroll
+      for (uint32_t u = 0; u < B; ++u) {
+        r[u] = hm[e[u]];
+        wf[u] = w[e[u]];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < B; ++u) {
+        const bool in = u0 + u < d;
+        const uint32_t we = in ? fixed_weight(wf[u]) : 0u;
+        const uint64_t m = in ? ((uint64_t)r[u].y << 32) | r[u].x : 0ull;
+        const uint64_t sg = ((uint64_t)r[u].w << 32) | r[u].z;
+        mk[u0 + u] = m;
+        wt[u0 + u] = we;
+        Wv += we;
+        saving += ((sg >> p) & 1ull) ? (int64_t)we : 0;  // v is the hedge's only pin in p
+      }
+    };
+    if (d > 0) batch(std::integral_constant<uint32_t, 4>{}, 0);
+    if (d > 4) batch(std::integral_constant<uint32_t, GAIN_TINY - 4>{}, 4);
+    auto conn_of = [&](uint32_t q) {
+      int64_t c = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < GAIN_TINY; ++u) c += ((mk[u] >> q) & 1ull) ? (int64_t)wt[u] : 0;
+      return c;
+    };
     const int64_t base = saving - Wv;
+    uint64_t uni = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < GAIN_TINY; ++u) uni |= mk[u];
+    uni &= ~(1ull << p);
     int64_t bg = INT64_MIN;
     uint32_t bh = 0, bq = NONE;
-#pragma unroll
-    for (uint32_t k = 0; k < GAIN_TINY; ++k) {
-      const uint32_t q = qs[k];
-      if (q == NONE || cs[k] <= 0 || !target_ok(a, q, p, s)) continue;
-      const int64_t g = base + cs[k];
+    while (uni) {  // ascending partitions; (gain, hash) decides, and the hash is a bijection
+      const uint32_t q = (uint32_t)__ffsll((long long)uni) - 1;
+      uni &= uni - 1;
+      const int64_t c = conn_of(q);
+      if (c <= 0 || !target_ok(a, q, p, s)) continue;
+      const int64_t g = base + c;
       const uint32_t h = mix32(q + a.repeat);
       if (bq == NONE || g > bg || (g == bg && h > bh)) {
         bg = g;
@@ -1055,14 +1041,9 @@ __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint
     if (bq == NONE) {  // first valid partition in hash order without a positive connection
       for (uint32_t x = 0; x < a.k; ++x) {
         const uint32_t q = a.order[x];
-        if (!target_ok(a, q, p, s)) continue;
-        bool adjacent = false;
-#pragma unroll
-        for (uint32_t k = 0; k < GAIN_TINY; ++k) adjacent |= qs[k] == q && cs[k] > 0;
-        if (!adjacent) {
-          bq = q;
-          break;
-        }
+        if (!target_ok(a, q, p, s) || conn_of(q) > 0) continue;
+        bq = q;
+        break;
       }
       bg = base;
     }
@@ -1070,7 +1051,7 @@ __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint
     a.gains[v] = bq == NONE ? 0 : bg;
   }
   // list slots: one atomic per workgroup, then wave and lane offsets
-  const bool to_list = v < n && !tiny;
+  const bool to_list = v < n && !small;
   __shared__ uint32_t s_cnt[4], s_base;
   const uint32_t l = lane_id(), wv = threadIdx.x / 64;
   const uint64_t ml = __ballot(to_list);
@@ -1548,7 +1529,9 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
     f_m[e] = (sorted && hl > CAS_SORT_S) ? 1u : 0u;
     f_big[e] = (hl > CAS_SMALL && !sorted) ? 1u : 0u;
   }
-  if (hl > CAS_SMALL || hl == 0) return;
+  // a hedge of one pin adds nothing: its pin finds itself alone in p and nobody in q, so the
+  // two terms of cascade_term cancel
+  if (hl > CAS_SMALL || hl <= 1) return;
   uint32_t pt[CAS_SMALL], mv[CAS_SMALL], rk[CAS_SMALL];
 #pragma unroll
   for (uint32_t u = 0; u < CAS_SMALL; ++u) {  // entries no batch fills (u >= 4 when len <= 4)
