@@ -1281,6 +1281,7 @@ struct ChainArrays {
   unsigned long long* claim;
   uint32_t* progress;  // progress[it] != 0: iteration it committed a link
   uint4* pk;           // per entry: gain (lo, hi), size, valid -- what a proposal scan reads
+  uint32_t* from;      // per entry: where its next proposal scan starts
 };
 // An iteration after one that committed nothing would repeat it exactly: skip it.
 __device__ __forceinline__ bool chain_stalled(const ChainArrays& c, int it) {
@@ -1329,13 +1330,16 @@ __global__ void k_chain_ranges(ChainArrays c, const uint64_t* __restrict__ poff,
   const uint32_t q = c.dst[j];
   c.ob[j] = (uint32_t)poff[q];
   c.oe[j] = (uint32_t)poff[q + 1];
+  c.from[j] = c.ob[j];
 }
 __device__ __forceinline__ int size_bucket(uint32_t s) { return s == 0 ? 0 : 31 - __clz(s); }
 
 // Each open move proposes to the best free move out of its destination partition among the
 // first 256 entries of that partition's list.  The list is sorted by gain (descending, valid
 // moves first), so once a choice exists the scan stops at the first lower gain or invalid
-// entry: nothing later can beat it.
+// entry: nothing later can beat it.  An entry that cannot be chosen (invalid, the proposer
+// itself, already linked, or of a size bucket too far) never can again, so the next scan
+// starts at the first entry this one could choose.
 __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || chain_stalled(c, it)) return;
@@ -1345,12 +1349,14 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
     const uint32_t si = c.sz[i];
     const int bi = size_bucket(si);
     const uint32_t lim = min(c.oe[i], c.ob[i] + 256u);
+    const uint32_t j0 = max(c.from[i], c.ob[i]);
+    uint32_t first = lim;  // first entry that could be chosen (scan end when none)
     int64_t bw = 0;
     uint32_t bd = 0;
     bool done = false;
     // 8 entries (one 128-byte line of records) are loaded before any is examined, so a scan
     // waits on one memory round trip per 8 entries instead of one per entry
-    for (uint32_t jb = c.ob[i]; jb < lim && !done; jb += 8) {
+    for (uint32_t jb = j0; jb < lim && !done; jb += 8) {
       uint4 e[8];
       uint32_t pv[8];
 #pragma unroll
@@ -1364,6 +1370,7 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
       for (uint32_t u = 0; u < 8; ++u) {
         const uint32_t j = jb + u;
         if (!e[u].w) {  // invalid entry, or past the scan limit
+          if (choice == NONE) first = min(j, lim);
           done = true;
           break;
         }
@@ -1376,6 +1383,7 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
         const uint32_t sj = e[u].z;
         if (abs(bi - size_bucket(sj)) > 1) continue;
         const uint32_t d = si > sj ? si - sj : sj - si;
+        if (choice == NONE) first = j;
         if (choice == NONE || wj > bw || (wj == bw && d < bd)) {
           choice = j;
           bw = wj;
@@ -1387,6 +1395,7 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
         }
       }
     }
+    c.from[i] = first;
   }
   c.choice[i] = choice;
 }
@@ -1498,6 +1507,12 @@ struct CascadeArgs {
   unsigned long long* seq;
 };
 
+// (partition, move target or NONE, rank) of node x
+__device__ __forceinline__ uint3 pm_at(const CascadeArgs& a, uint32_t x) {
+  const uint4 r = a.pm[x];
+  return make_uint3(r.x, r.y, r.z);
+}
+
 __global__ void k_pack_nodes(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
                              const uint32_t* __restrict__ ranks, uint32_t n, uint4* __restrict__ pm) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1547,7 +1562,7 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
     const uint32_t x = a.hedges[s + min(u, len - 1)];
     return x;
   };
-  auto put = [&](uint32_t u, uint4 q4) {
+  auto put = [&](uint32_t u, uint3 q4) {
     const bool in = u < len;
     pt[u] = in ? q4.x : NONE;
     mv[u] = in ? q4.y : NONE;
@@ -1558,14 +1573,14 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) x[u] = fetch(u);
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) put(u, a.pm[x[u]]);
+    for (uint32_t u = 0; u < 4; ++u) put(u, pm_at(a, x[u]));
   }
   if (len > 4) {
     uint32_t x[CAS_SMALL - 4];
 #pragma unroll
     for (uint32_t u = 4; u < CAS_SMALL; ++u) x[u - 4] = fetch(u);
 #pragma unroll
-    for (uint32_t u = 4; u < CAS_SMALL; ++u) put(u, a.pm[x[u - 4]]);
+    for (uint32_t u = 4; u < CAS_SMALL; ++u) put(u, pm_at(a, x[u - 4]));
   }
   bool any = false;
 #pragma unroll
@@ -1605,21 +1620,21 @@ __global__ void __launch_bounds__(64) k_cascade_large(CascadeArgs a, const uint3
   if (in_lds) {
     for (uint32_t i = l; i < len; i += 64) {
       const uint32_t x = a.hedges[s + i];
-      const uint4 q4 = a.pm[x];
+      const uint3 q4 = pm_at(a, x);
       s_pt[i] = q4.x;
       s_mv[i] = q4.y;
       s_rk[i] = q4.z;
       any |= s_mv[i] != NONE;
     }
   } else {
-    for (uint32_t i = l; i < len; i += 64) any |= a.pm[a.hedges[s + i]].y != NONE;
+    for (uint32_t i = l; i < len; i += 64) any |= pm_at(a, a.hedges[s + i]).y != NONE;
   }
   __syncthreads();
   if (!__ballot(any)) return;
   const int64_t W = (int64_t)fixed_weight(a.w[e]);
-  auto PT = [&](uint32_t j) { return in_lds ? s_pt[j] : a.pm[a.hedges[s + j]].x; };
-  auto MV = [&](uint32_t j) { return in_lds ? s_mv[j] : a.pm[a.hedges[s + j]].y; };
-  auto RK = [&](uint32_t j) { return in_lds ? s_rk[j] : a.pm[a.hedges[s + j]].z; };
+  auto PT = [&](uint32_t j) { return in_lds ? s_pt[j] : pm_at(a, a.hedges[s + j]).x; };
+  auto MV = [&](uint32_t j) { return in_lds ? s_mv[j] : pm_at(a, a.hedges[s + j]).y; };
+  auto RK = [&](uint32_t j) { return in_lds ? s_rk[j] : pm_at(a, a.hedges[s + j]).z; };
   for (uint32_t i = l; i < len; i += 64) {
     const uint32_t q = MV(i);
     if (q == NONE) continue;
@@ -1679,7 +1694,7 @@ __global__ void __launch_bounds__(NT) k_cascade_sorted(CascadeArgs a, const uint
   // after them (the sort orders them, and every key is unique through its pin index)
   for (uint32_t i = tid; i < len; i += NT) {
     const uint32_t x = a.hedges[s + i];
-    const uint4 q4 = a.pm[x];
+    const uint3 q4 = pm_at(a, x);
     const uint32_t p = q4.x, q = q4.y;
     s_ev[i] = ((uint64_t)p << (rb + 11)) | i;
     s_rk[i] = q == NONE ? NONE : q4.z;
@@ -2223,9 +2238,10 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   DBuf<unsigned long long> claim(c, n);
   DBuf<uint32_t> progress(c, 4);
   DBuf<uint4> pk(c, n);
+  DBuf<uint32_t> from(c, n);
   fill_u32(c, progress.get(), 0u, 4);
   ChainArrays ca{orig.get(), src.get(), dst.get(), sz.get(), w.get(), ok.get(), ob.get(), oe.get(),
-                 next.get(), prev.get(), choice.get(), claim.get(), progress.get(), pk.get()};
+                 next.get(), prev.get(), choice.get(), claim.get(), progress.get(), pk.get(), from.get()};
   HGP_LAUNCH(k_chain_gather, g, 256, 0, c, ca, part, moves, gains, sizes, n);
   DBuf<uint64_t> poff(c, (uint64_t)k + 1);
   // src partitions of sorted space are the k2 keys >> 1
