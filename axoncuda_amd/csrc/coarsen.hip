@@ -94,10 +94,12 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const uint64_t t0 = toff[u], t1 = toff[u + 1];
   const uint32_t inb_u = inb[u];
-  wave_visit_pins(tch, t0, t1, hoff, hedges, [&](const PinVisit& pv) {
+  // per touching hedge, once: its contribution floor(W / |e|) and its source-pin count
+  auto hvf = [&](uint32_t e, uint32_t len) { return make_uint2(fixed_weight(w[e]) / len, INTER ? srcc[e] : 0u); };
+  wave_visit_pins_hv_until(tch, t0, t1, hoff, hedges, hvf, [&](const PinVisitHV& pv) {
     if (!pv.act || pv.pin == u) return;
-    const uint32_t contrib = fixed_weight(w[pv.e]) / pv.len;
-    const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
+    const uint32_t contrib = pv.hv.x;
+    const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= pv.hv.y;
     uint32_t h = slot_hash(pv.pin) & (cap - 1);
     for (;;) {
       const uint32_t old = atomicCAS(&keys[h], NONE, pv.pin);
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
     }
     atomicAdd(&score[h], contrib);
     if (INTER && shared_in) atomicAdd(&inter[h], 1u);
-  });
+  }, [] { return false; });
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const uint32_t size_u = sizes[u];
@@ -297,11 +299,12 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
     }
     if (GMEM) __threadfence();
     __syncthreads();
-    wave_visit_pins_until(tch, a, b, hoff, hedges, [&](const PinVisit& pv) {
+    auto hvf = [&](uint32_t e, uint32_t len) { return make_uint2(fixed_weight(w[e]) / len, INTER ? srcc[e] : 0u); };
+    wave_visit_pins_hv_until(tch, a, b, hoff, hedges, hvf, [&](const PinVisitHV& pv) {
       if (!pv.act || pv.pin == u) return;
       if (CH > 1 && pv.pin % CH != (uint32_t)chunk) return;
-      const uint32_t contrib = fixed_weight(w[pv.e]) / pv.len;
-      const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= srcc[pv.e];
+      const uint32_t contrib = pv.hv.x;  // floor(W / |e|), computed once per touching hedge
+      const bool shared_in = INTER && (pv.j - t0) < inb_u && pv.pos >= pv.hv.y;
       if (direct) {
         const uint32_t bit = 1u << (pv.pin & 31u);
         if (!(atomic_load_relaxed(&keys[pv.pin >> 5]) & bit)) atomicOr(&keys[pv.pin >> 5], bit);

@@ -174,6 +174,58 @@ __device__ __forceinline__ void wave_visit_pins_until(const uint32_t* __restrict
   }
 }
 
+// The same traversal with two per-hedge values computed once per touching hedge by the lane
+// that loads it, `hvf(e, len)`, and handed to every visit of that hedge's pins in `hv`.
+struct PinVisitHV : PinVisit {
+  uint2 hv;
+};
+template <typename H, typename F, typename S>
+__device__ __forceinline__ void wave_visit_pins_hv_until(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
+                                                         const uint64_t* __restrict__ hoff,
+                                                         const uint32_t* __restrict__ hedges, H&& hvf, F&& visit,
+                                                         S&& stop) {
+  const uint32_t l = lane_id();
+  for (uint64_t jb = t0; jb < t1; jb += 64) {
+    if (stop()) return;
+    const uint64_t j = jb + l;
+    uint32_t e = 0, len = 0;
+    uint64_t s = 0;
+    uint2 hv = make_uint2(0u, 0u);
+    if (j < t1) {
+      e = tch[j];
+      s = hoff[e];
+      len = (uint32_t)(hoff[e + 1] - s);
+      hv = hvf(e, len);
+    }
+    const uint32_t incl = wave_scan_incl(len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - len;
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+      if (f0 && stop()) return;
+      const uint32_t fi = f0 + l;
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t step = 32; step > 0; step >>= 1) {
+        const uint32_t cand = k + step;
+        const uint32_t ex = __shfl(excl, (int)cand, 64);
+        if (ex <= fi) k = cand;
+      }
+      PinVisitHV pv;
+      pv.act = fi < total;
+      pv.e = __shfl(e, (int)k, 64);
+      const uint64_t os = shfl_u64(s, k);
+      const uint32_t oex = __shfl(excl, (int)k, 64);
+      pv.len = __shfl(len, (int)k, 64);
+      pv.hv.x = __shfl(hv.x, (int)k, 64);
+      pv.hv.y = __shfl(hv.y, (int)k, 64);
+      pv.j = jb + k;
+      pv.pos = fi - oex;
+      pv.pin = pv.act ? hedges[os + pv.pos] : 0u;
+      visit(pv);
+    }
+  }
+}
+
 template <typename F>
 __device__ __forceinline__ void wave_visit_pins(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
                                                 const uint64_t* __restrict__ hoff,
