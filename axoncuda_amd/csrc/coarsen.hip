@@ -271,7 +271,8 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
                                                 const uint32_t* __restrict__ sizes, uint32_t cnc, uint32_t omega,
                                                 uint32_t delta, uint32_t* __restrict__ pairs,
                                                 uint32_t* __restrict__ scores, uint64_t* s_w,
-                                                uint32_t* s_state = nullptr, uint32_t limit = 0) {
+                                                uint32_t* s_state = nullptr, uint32_t limit = 0,
+                                                uint64_t* bytes = nullptr) {
   const uint32_t nkeys = direct ? (cap + 31) / 32 : cap;
   const uint64_t t0 = toff[u], t1 = toff[u + 1];
   const uint32_t inb_u = inb[u];
@@ -300,7 +301,9 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
     if (GMEM) __threadfence();
     __syncthreads();
     auto hvf = [&](uint32_t e, uint32_t len) { return make_uint2(fixed_weight(w[e]) / len, INTER ? srcc[e] : 0u); };
+    uint64_t visits = 0;
     wave_visit_pins_hv_until(tch, a, b, hoff, hedges, hvf, [&](const PinVisitHV& pv) {
+      if (bytes && pv.act) ++visits;
       if (!pv.act || pv.pin == u) return;
       if (CH > 1 && pv.pin % CH != (uint32_t)chunk) return;
       const uint32_t contrib = pv.hv.x;  // floor(W / |e|), computed once per touching hedge
@@ -338,11 +341,15 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
         v = keys[s];
         if (v == NONE) continue;
       }
+      if (bytes) *bytes += 8;  // the neighbour's size and inbound count
       top_insert(top, cand_key(u, v, score[s], INTER ? inter[s] : 0u, size_u, inb_u, sizes, inb, omega, delta));
     }
+    // the pin ids visited; per touching hedge its list entry, offsets, weight (and source count)
+    if (bytes) *bytes += 4 * visits + (threadIdx.x == 0 && chunk == 0 ? (INTER ? 28 : 24) * (t1 - t0) : 0);
     if (CH > 1) __syncthreads();  // the next pass clears the table
   }
   block_top_write<NW>(top, cnc, s_w, u, pairs, scores);
+  if (bytes && threadIdx.x == 0) *bytes += 24 + 8 * (uint64_t)cnc;  // offsets, size, inbound count, results
   return true;
 }
 
@@ -353,15 +360,25 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
       const uint32_t *__restrict__ caps, uint32_t cnc, uint32_t omega, uint32_t delta, uint32_t *__restrict__ pairs, \
       uint32_t *__restrict__ scores
 
+// `prof` (null unless this kernel is profiled): PROF_SLOTS counters of algorithmic bytes.
 template <bool INTER>
-__global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS) {
+__global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS, unsigned long long* __restrict__ prof) {
   __shared__ uint32_t s_keys[CAND_MED_CAP];
   __shared__ uint32_t s_score[CAND_MED_CAP];
   __shared__ uint32_t s_inter[INTER ? CAND_MED_CAP : 1];
   __shared__ uint64_t s_w[64];
+  __shared__ uint64_t s_bytes[4];
   const uint32_t u = list[blockIdx.x];
+  uint64_t bytes = 0;
   cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
-                                   sizes, cnc, omega, delta, pairs, scores, s_w);
+                                   sizes, cnc, omega, delta, pairs, scores, s_w, nullptr, 0, prof ? &bytes : nullptr);
+  if (prof) {
+    bytes = wave_sum(bytes);
+    if (lane_id() == 0) s_bytes[threadIdx.x / 64] = bytes;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      atomicAdd(&prof[blockIdx.x % PROF_SLOTS], (unsigned long long)(s_bytes[0] + s_bytes[1] + s_bytes[2] + s_bytes[3]));
+  }
 }
 
 // One node per 1024-thread workgroup with a hashed LDS table of up to CAND_BIG_SLOTS.
@@ -664,8 +681,14 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
 #define CAND_LAUNCH(K, list, cnt, threads)                                                                          \
   HGP_LAUNCH(K, cnt, threads, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, list, caps.get(), cnc, omega, delta, \
              pairs, scores)
-  if (inter) CAND_LAUNCH(k_cand_medium<true>, med.get(), nm, 256);
-  else CAND_LAUNCH(k_cand_medium<false>, med.get(), nm, 256);
+  // profiled: the kernel counts its algorithmic bytes itself
+  unsigned long long* prof = prof_match(c, "k_cand_medium") ? c.prof_dev : nullptr;
+  if (inter)
+    HGP_LAUNCH(k_cand_medium<true>, nm, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, med.get(), caps.get(), cnc,
+               omega, delta, pairs, scores, prof);
+  else
+    HGP_LAUNCH(k_cand_medium<false>, nm, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, med.get(), caps.get(),
+               cnc, omega, delta, pairs, scores, prof);
   if (n <= direct_n) {
     if (inter) CAND_LAUNCH(k_cand_direct<true>, big.get(), nb, 1024);
     else CAND_LAUNCH(k_cand_direct<false>, big.get(), nb, 1024);

@@ -37,7 +37,7 @@ K, EPS = 64, 0.03
 HBM_PEAK_GBS = 8000.0
 # Kernel whose launches the roofline object covers: the largest share of device time on this
 # workload in the kernel-trace summary under profiles/.
-ROOFLINE_KERNEL = "k_gains_light"
+ROOFLINE_KERNEL = "k_cand_medium"
 # HBM bytes per launch of that kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
 # scaled by the same counters on a plain device copy of known size (profiles/traffic_<kernel>.json).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{ROOFLINE_KERNEL}.json")
