@@ -966,6 +966,13 @@ __global__ void __launch_bounds__(64) k_hmask_large(const uint32_t* __restrict__
 __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint4* __restrict__ hm,
                                                           const float* __restrict__ w, uint32_t n,
                                                           uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+  // partition sizes and the hash order, for the candidates' room checks and the fallback
+  __shared__ uint32_t s_ps[MASK_K], s_ord[MASK_K];
+  if (threadIdx.x < a.k) {
+    s_ps[threadIdx.x] = a.psizes[threadIdx.x];
+    s_ord[threadIdx.x] = a.order[threadIdx.x];
+  }
+  __syncthreads();
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t t0 = 0, t1 = 0;
   if (v < n) {
@@ -1019,17 +1026,19 @@ __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint
       return c;
     };
     const int64_t base = saving - Wv;
-    uint64_t uni = 0;
+    auto fits = [&](uint32_t q) { return q != p && (uint32_t)(s_ps[q] + s - s / a.discount) <= a.omega; };
+    // partitions with a positive connection: in a mask of positive weight (weights are >= 0)
+    uint64_t pos = 0;
 #pragma unroll
-    for (uint32_t u = 0; u < GAIN_TINY; ++u) uni |= mk[u];
-    uni &= ~(1ull << p);
+    for (uint32_t u = 0; u < GAIN_TINY; ++u) pos |= wt[u] ? mk[u] : 0ull;
+    pos &= ~(1ull << p);
     int64_t bg = INT64_MIN;
     uint32_t bh = 0, bq = NONE;
-    while (uni) {  // ascending partitions; (gain, hash) decides, and the hash is a bijection
-      const uint32_t q = (uint32_t)__ffsll((long long)uni) - 1;
-      uni &= uni - 1;
+    for (uint64_t cand = pos; cand;) {  // ascending; (gain, hash) decides, the hash is a bijection
+      const uint32_t q = (uint32_t)__ffsll((long long)cand) - 1;
+      cand &= cand - 1;
+      if (!fits(q)) continue;
       const int64_t c = conn_of(q);
-      if (c <= 0 || !target_ok(a, q, p, s)) continue;
       const int64_t g = base + c;
       const uint32_t h = mix32(q + a.repeat);
       if (bq == NONE || g > bg || (g == bg && h > bh)) {
@@ -1040,8 +1049,8 @@ __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint
     }
     if (bq == NONE) {  // first valid partition in hash order without a positive connection
       for (uint32_t x = 0; x < a.k; ++x) {
-        const uint32_t q = a.order[x];
-        if (!target_ok(a, q, p, s) || conn_of(q) > 0) continue;
+        const uint32_t q = s_ord[x];
+        if (!fits(q) || ((pos >> q) & 1ull)) continue;
         bq = q;
         break;
       }
