@@ -170,6 +170,7 @@ void hgp_ctx_destroy(hgp_ctx* ctx) {
   hgp::release_cached(ctx->c);
   (void)hipStreamSynchronize(ctx->c.stream);
   if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
+  if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
   delete ctx;
 }
 

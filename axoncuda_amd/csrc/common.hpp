@@ -11,6 +11,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -77,7 +78,9 @@ struct Ctx {
   // algorithmic bytes that the profiled kernel counts itself on the device (PROF_SLOTS
   // counters, spread so that the atomics do not all meet on one address)
   unsigned long long* prof_dev = nullptr;
+  void* pinned = nullptr;  // small pinned host buffer for scalar read-backs (allocated on first use)
 };
+constexpr uint64_t PINNED_BYTES = 256;
 constexpr uint32_t PROF_SLOTS = 64;
 
 // Device memory: a caching allocator on top of hipMallocAsync.  Freed blocks are parked
@@ -139,11 +142,19 @@ inline void copy_d2d(Ctx& c, void* dst, const void* src, uint64_t bytes) {
 inline void copy_h2d(Ctx& c, void* dst, const void* src, uint64_t bytes) {
   if (bytes) HGP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
 }
+// Small read-backs land in the context's pinned buffer (a direct copy, no staging through
+// pageable memory), larger ones straight in `dst`; either way the stream is synchronized.
 inline void copy_d2h(Ctx& c, void* dst, const void* src, uint64_t bytes) {
-  if (bytes) {
-    HGP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+  if (!bytes) return;
+  if (bytes <= PINNED_BYTES) {
+    if (!c.pinned) HGP_HIP(hipHostMalloc(&c.pinned, PINNED_BYTES, hipHostMallocDefault));
+    HGP_HIP(hipMemcpyAsync(c.pinned, src, bytes, hipMemcpyDeviceToHost, c.stream));
     HGP_HIP(hipStreamSynchronize(c.stream));
+    std::memcpy(dst, c.pinned, bytes);
+    return;
   }
+  HGP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+  HGP_HIP(hipStreamSynchronize(c.stream));
 }
 template <typename T>
 inline T read_scalar(Ctx& c, const T* d) {

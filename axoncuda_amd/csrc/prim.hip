@@ -412,6 +412,10 @@ __global__ void k_minmax_u64(const uint64_t* __restrict__ p, uint64_t n, unsigne
     atomicMax(&out[1], (unsigned long long)mx);
   }
 }
+__global__ void k_init_minmax(unsigned long long* mm) {
+  mm[0] = ~0ull;
+  mm[1] = 0ull;
+}
 __global__ void k_sub_u64(uint64_t* p, uint64_t n, const unsigned long long* mm) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] -= mm[0];
@@ -467,8 +471,7 @@ void segment_offsets(Ctx& c, const uint64_t* keys, int shift, uint64_t n, uint32
 void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n) {
   if (n <= 1) return;
   DBuf<unsigned long long> mm(c, 2);
-  const unsigned long long init[2] = {~0ull, 0ull};
-  copy_h2d(c, mm.get(), init, sizeof init);
+  HGP_LAUNCH(k_init_minmax, 1, 1, 0, c, mm.get());  // on the stream: no host-to-device copy
   HGP_LAUNCH(k_minmax_u64, std::min<uint32_t>(ceil_div(n, 256), 512), 256, 0, c, keys, n, mm.get());
   unsigned long long h[2];
   copy_d2h(c, h, mm.get(), sizeof h);
