@@ -354,8 +354,12 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K* __restrict__ k
   }
 }
 
+// LSD passes over bits [lo, bits).  With `kbuf` (and `vbuf`) the result ends up in those
+// buffers by exchanging them with the scratch ones after an odd pass count; otherwise it is
+// copied back into keys / vals.
 template <typename K>
-void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 0) {
+void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 0, DBuf<K>* kbuf = nullptr,
+                DBuf<uint32_t>* vbuf = nullptr) {
   if (n <= 1 || bits <= lo) return;
   const uint32_t nblocks = ceil_div(n, RS_TILE);
   DBuf<K> k2(c, n);
@@ -383,6 +387,11 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 
     std::swap(va, vb);
   }
   if (ka != keys) {
+    if (kbuf && (!vals || vbuf)) {  // hand the scratch buffers over instead of copying
+      std::swap(*kbuf, k2);
+      if (vals) std::swap(*vbuf, v2);
+      return;
+    }
     copy_d2d(c, keys, ka, n * sizeof(K));
     if (vals) copy_d2d(c, vals, va, n * sizeof(uint32_t));
   }
@@ -460,6 +469,12 @@ void radix_sort(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int bits) { 
 void radix_sort_bits(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int lo, int hi) {
   radix_impl<uint64_t>(c, keys, vals, n, hi, lo);
 }
+void radix_sort_buf(Ctx& c, DBuf<uint32_t>& keys, DBuf<uint32_t>& vals, uint64_t n, int lo, int hi) {
+  radix_impl<uint32_t>(c, keys.get(), vals.get(), n, hi, lo, &keys, &vals);
+}
+void radix_sort_buf(Ctx& c, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, uint64_t n, int lo, int hi) {
+  radix_impl<uint64_t>(c, keys.get(), vals.get(), n, hi, lo, &keys, &vals);
+}
 
 void segment_offsets(Ctx& c, const uint32_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off) {
   HGP_LAUNCH((k_segment_offsets<uint32_t>), ceil_div(n + 1, 256), 256, 0, c, keys, shift, n, nseg, off);
@@ -468,7 +483,8 @@ void segment_offsets(Ctx& c, const uint64_t* keys, int shift, uint64_t n, uint32
   HGP_LAUNCH((k_segment_offsets<uint64_t>), ceil_div(n + 1, 256), 256, 0, c, keys, shift, n, nseg, off);
 }
 
-void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n) {
+namespace {
+void range_impl(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, DBuf<uint64_t>* kbuf, DBuf<uint32_t>* vbuf) {
   if (n <= 1) return;
   DBuf<unsigned long long> mm(c, 2);
   HGP_LAUNCH(k_init_minmax, 1, 1, 0, c, mm.get());  // on the stream: no host-to-device copy
@@ -477,7 +493,13 @@ void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n) {
   copy_d2h(c, h, mm.get(), sizeof h);
   if (h[1] == h[0]) return;  // all keys equal: a stable sort is the identity
   HGP_LAUNCH(k_sub_u64, ceil_div(n, 256), 256, 0, c, keys, n, mm.get());
-  radix_impl<uint64_t>(c, keys, vals, n, bit_width(h[1] - h[0]));
+  radix_impl<uint64_t>(c, keys, vals, n, bit_width(h[1] - h[0]), 0, kbuf, vbuf);
+}
+}  // namespace
+
+void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n) { range_impl(c, keys, vals, n, nullptr, nullptr); }
+void radix_sort_range_buf(Ctx& c, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, uint64_t n) {
+  range_impl(c, keys.get(), vals.get(), n, &keys, &vals);
 }
 
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, uint64_t n) { HGP_LAUNCH(k_fill_u32, ceil_div(n, 256), 256, 0, c, p, v, n); }

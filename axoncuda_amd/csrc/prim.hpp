@@ -20,9 +20,17 @@ void radix_sort(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int bits);
 // Stable sort by key bits [lo, hi) only: when the input is already ordered by the bits below
 // lo, the result is ordered by bits [0, hi), in (hi - lo) / 8 passes instead of hi / 8.
 void radix_sort_bits(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int lo, int hi);
+// The same over bits [lo, hi) of buffers the caller owns: after an odd number of passes the
+// buffers are exchanged with the sort's scratch ones instead of copied back (the DBufs then
+// hold different allocations; any raw pointer taken before is stale).
+void radix_sort_buf(Ctx& c, DBuf<uint32_t>& keys, DBuf<uint32_t>& vals, uint64_t n, int lo, int hi);
+void radix_sort_buf(Ctx& c, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, uint64_t n, int lo, int hi);
 // Stable sort over the keys' actual range: subtracts the minimum key (the keys are left
 // rebased) and sorts only the bits that the maximum difference needs.
 void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n);
+// The same on buffers the caller owns, exchanged with the scratch ones instead of copied back
+// (see radix_sort_buf).
+void radix_sort_range_buf(Ctx& c, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, uint64_t n);
 
 // off[s] = first index i with (keys[i] >> shift) >= s, for s in [0, nseg]; keys sorted.
 void segment_offsets(Ctx& c, const uint32_t* keys, int shift, uint64_t n, uint32_t nseg, uint64_t* off);

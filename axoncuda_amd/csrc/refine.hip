@@ -1274,6 +1274,16 @@ __global__ void __launch_bounds__(256) k_count_valid(const uint32_t* __restrict_
   }
 }
 
+// zeroes three arrays of n entries in one launch (instead of three fills)
+__global__ void k_zero3(unsigned long long* __restrict__ a, uint32_t* __restrict__ b, uint32_t* __restrict__ c,
+                        uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  a[i] = 0ull;
+  b[i] = 0u;
+  c[i] = 0u;
+}
+
 // chaining: edges in sorted space j (by source partition asc, valid first, gain desc)
 struct ChainArrays {
   const uint32_t* orig;  // j -> node
@@ -2225,9 +2235,9 @@ void rank_by_gain(Ctx& c, uint32_t n, const uint32_t* moves, const int64_t* gain
   DBuf<uint64_t> keys(c, n);
   DBuf<uint32_t> vals(c, n), k2(c, n);
   HGP_LAUNCH(k_rank_keys_gain, ceil_div(n, 256), 256, 0, c, moves, gains, n, keys.get(), vals.get());
-  radix_sort_range(c, keys.get(), vals.get(), n);
+  radix_sort_range_buf(c, keys, vals, n);
   HGP_LAUNCH(k_invalid_keys, ceil_div(n, 256), 256, 0, c, moves, vals.get(), n, k2.get());
-  radix_sort(c, k2.get(), vals.get(), n, 1);
+  radix_sort_buf(c, k2, vals, n, 0, 1);
   HGP_LAUNCH(k_scatter_ranks, ceil_div(n, 256), 256, 0, c, vals.get(), n, ranks);
 }
 
@@ -2238,9 +2248,9 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   DBuf<uint64_t> k1(c, n);
   DBuf<uint32_t> orig(c, n), k2(c, n);
   HGP_LAUNCH(k_chain_sortkeys, g, 256, 0, c, part, moves, gains, n, k1.get(), orig.get());
-  radix_sort_range(c, k1.get(), orig.get(), n);
+  radix_sort_range_buf(c, k1, orig, n);
   HGP_LAUNCH(k_chain_sortkeys2, g, 256, 0, c, part, moves, orig.get(), n, k2.get());
-  radix_sort(c, k2.get(), orig.get(), n, bit_width(2ull * k + 1));
+  radix_sort_buf(c, k2, orig, n, 0, bit_width(2ull * k + 1));
   k1.reset();
   DBuf<uint32_t> src(c, n), dst(c, n), sz(c, n), ok(c, n), ob(c, n), oe(c, n), next(c, n), prev(c, n), choice(c, n);
   DBuf<int64_t> w(c, n);
@@ -2263,9 +2273,7 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   }
   DBuf<uint32_t> comp(c, n), pos(c, n), cflag(c, n), cinv(c, n);
   DBuf<long long> cw(c, n);
-  memset_async(c, cflag.get(), 0, (uint64_t)n * 4);
-  memset_async(c, cinv.get(), 0, (uint64_t)n * 4);
-  memset_async(c, cw.get(), 0, (uint64_t)n * 8);
+  HGP_LAUNCH(k_zero3, g, 256, 0, c, (unsigned long long*)cw.get(), cflag.get(), cinv.get(), (uint64_t)n);
   HGP_LAUNCH(k_chain_comp, g, 256, 0, c, ca, n, comp.get(), pos.get(), cflag.get(), cw.get(), cinv.get());
   DBuf<uint64_t> cpos(c, (uint64_t)n + 1);
   exclusive_scan(c, cflag.get(), cpos.get(), n);
@@ -2277,17 +2285,17 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   // components by (valid first, summed gain desc, head id asc): the list starts in ascending
   // head id; a stable sort on the summed gain, then a stable 1-bit pass that moves the
   // components made of an invalid move (summed gain 0) behind every valid one
-  radix_sort_range(c, ckey.get(), clist.get(), m);
+  radix_sort_range_buf(c, ckey, clist, m);
   DBuf<uint32_t> inv2(c, m);
   HGP_LAUNCH(k_gather_u32, ceil_div(m, 256), 256, 0, c, cinv.get(), clist.get(), m, inv2.get());
-  radix_sort(c, inv2.get(), clist.get(), m, 1);
+  radix_sort_buf(c, inv2, clist, m, 0, 1);
   HGP_LAUNCH(k_comp_rank, ceil_div(m, 256), 256, 0, c, clist.get(), m, crank.get());
   // final order: (component rank, position), ties in sorted-space index order: the keys are
   // laid out in index order and the sort is stable
   DBuf<uint64_t> ek(c, n);
   DBuf<uint32_t> ev(c, n);
   HGP_LAUNCH(k_edge_keys, g, 256, 0, c, comp.get(), pos.get(), crank.get(), n, ek.get(), ev.get());
-  radix_sort(c, ek.get(), ev.get(), n, bit_width(m) + 8);
+  radix_sort_buf(c, ek, ev, n, 0, bit_width(m) + 8);
   HGP_LAUNCH(k_edge_ranks, g, 256, 0, c, ev.get(), orig.get(), n, ranks);
 }
 
@@ -2326,9 +2334,8 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
   st.iv = DBuf<int32_t>(c, std::max<uint32_t>(V, 1));
   st.best.r = NONE;
   if (V == 0) return;
-  memset_async(c, st.seq.get(), 0, (uint64_t)V * 8);
-  memset_async(c, st.sv.get(), 0, (uint64_t)V * 4);
-  memset_async(c, st.iv.get(), 0, (uint64_t)V * 4);
+  HGP_LAUNCH(k_zero3, ceil_div(V, 256), 256, 0, c, st.seq.get(), (uint32_t*)st.sv.get(), (uint32_t*)st.iv.get(),
+             (uint64_t)V);
   // cascade
   {
     DBuf<uint4> pm(c, n);
@@ -2376,7 +2383,7 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
     HGP_LAUNCH(k_size_events, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), L.sizes, n, rb, keys.get(), vals.get());
     // the events sit at 2 * rank (+1), so they are already in rank order: sorting by the
     // partition bits alone orders them by (partition, rank)
-    radix_sort_bits(c, keys.get(), vals.get(), m, rb, rb + kb);
+    radix_sort_buf(c, keys, vals, m, rb, rb + kb);
     HGP_LAUNCH(k_to_i64, ceil_div(m, 256), 256, 0, c, vals.get(), m, cum.get());
     inclusive_scan(c, cum.get(), cum.get(), m);
     segment_offsets(c, keys.get(), rb, m, L.k, poff.get());
