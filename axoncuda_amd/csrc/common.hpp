@@ -212,31 +212,67 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Wave-wide reductions (all 64 lanes active; every lane gets the result).  The first four
+// exchanges are DPP lane moves inside 16-lane rows (quad xor 1, quad xor 2, half-row mirror,
+// row mirror: each pairs disjoint groups, so sums stay exact), the fifth a ds_swizzle that
+// swaps the 16-lane halves of each 32, and the two 32-lane results are read as scalars.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_move(T v) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)v;
+    return (T)(((uint64_t)dpp_u32<CTRL>((uint32_t)(u >> 32)) << 32) | dpp_u32<CTRL>((uint32_t)u));
+  } else {
+    return (T)dpp_u32<CTRL>((uint32_t)v);
+  }
+}
+template <typename T>
+__device__ __forceinline__ T swap16_move(T v) {  // lane i <- lane i ^ 16
+  auto sw = [](uint32_t x) { return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F); };
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)v;
+    return (T)(((uint64_t)sw((uint32_t)(u >> 32)) << 32) | sw((uint32_t)u));
+  } else {
+    return (T)sw((uint32_t)v);
+  }
+}
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int lane) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return (T)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (T)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  }
+}
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce(T v, Op op) {
+  v = op(v, dpp_move<0xB1>(v));   // quad_perm [1,0,3,2]: lane ^ 1
+  v = op(v, dpp_move<0x4E>(v));   // quad_perm [2,3,0,1]: lane ^ 2
+  v = op(v, dpp_move<0x141>(v));  // row_half_mirror: the two quads of each 8
+  v = op(v, dpp_move<0x140>(v));  // row_mirror: the two halves of each 16
+  v = op(v, swap16_move(v));      // the two rows of each 32
+  return op(lane_value(v, 0), lane_value(v, 32));
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  return wave_reduce(v, [](T a, T b) { return (T)(a + b); });
 }
 
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    T x = __shfl_xor(v, o, 64);
-    v = x > v ? x : v;
-  }
-  return v;
+  return wave_reduce(v, [](T a, T b) { return b > a ? b : a; });
 }
 
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    T x = __shfl_xor(v, o, 64);
-    v = x < v ? x : v;
-  }
-  return v;
+  return wave_reduce(v, [](T a, T b) { return b < a ? b : a; });
 }
 
 // inclusive prefix sum across the 64 lanes
