@@ -27,6 +27,19 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, uint32_t src) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Owner lane of flat position f0 + lane among the 64 hedges of a chunk whose exclusive start
+// offsets are `excl` (each hedge has at least one entry, so starts are distinct; lanes past the
+// list carry length 0 and start at the chunk total, beyond every active position): the number
+// of starts before the window (a ballot) plus those inside it at or before this position (an
+// OR-reduced bitmap of the window's starts), minus one.
+__device__ __forceinline__ uint32_t chunk_owner(uint32_t excl, uint32_t f0, uint64_t le_l) {
+  const uint32_t before = (uint32_t)__popcll(__ballot(excl < f0));
+  const uint64_t mine = (excl >= f0 && excl - f0 < 64u) ? (1ull << (excl - f0)) : 0ull;
+  const uint64_t heads = wave_reduce(mine, [](uint64_t a, uint64_t b) { return a | b; });
+  return before + (uint32_t)__popcll(heads & le_l) - 1u;
+}
+__device__ __forceinline__ uint64_t lanes_upto(uint32_t l) { return l == 63 ? ~0ull : ((2ull << l) - 1ull); }
+
 // Generic form: hedge e owns the entry range [start(e), start(e) + length(e)) of some
 // per-hedge array; the visitor gets the flattened entry index in `pin` replaced by the
 // absolute entry index (field `abs`).
@@ -104,15 +117,10 @@ __device__ __forceinline__ int64_t wave_visit_hedge_lists(const uint32_t* __rest
     const uint32_t incl = wave_scan_incl(len);
     const uint32_t total = __shfl(incl, 63, 64);
     const uint32_t excl = incl - len;
+    const uint64_t le_l = lanes_upto(l);
     for (uint32_t f0 = 0; f0 < total; f0 += 64) {
       const uint32_t fi = f0 + l;
-      uint32_t k = 0;
-#pragma unroll
-      for (uint32_t step = 32; step > 0; step >>= 1) {
-        const uint32_t cand = k + step;
-        const uint32_t ex = __shfl(excl, (int)cand, 64);
-        if (ex <= fi) k = cand;
-      }
+      const uint32_t k = chunk_owner(excl, f0, le_l);  // every touching hedge lists >= 1 entry
       EntryVisit ev;
       ev.act = fi < total;
       ev.e = __shfl(e, (int)k, 64);
@@ -150,16 +158,11 @@ __device__ __forceinline__ void wave_visit_pins_until(const uint32_t* __restrict
     const uint32_t incl = wave_scan_incl(len);
     const uint32_t total = __shfl(incl, 63, 64);
     const uint32_t excl = incl - len;
+    const uint64_t le_l = lanes_upto(l);
     for (uint32_t f0 = 0; f0 < total; f0 += 64) {
       if (f0 && stop()) return;
       const uint32_t fi = f0 + l;
-      uint32_t k = 0;
-#pragma unroll
-      for (uint32_t step = 32; step > 0; step >>= 1) {
-        const uint32_t cand = k + step;
-        const uint32_t ex = __shfl(excl, (int)cand, 64);
-        if (ex <= fi) k = cand;
-      }
+      const uint32_t k = chunk_owner(excl, f0, le_l);  // every touching hedge has a pin
       PinVisit pv;
       pv.act = fi < total;
       pv.e = __shfl(e, (int)k, 64);
@@ -200,16 +203,11 @@ __device__ __forceinline__ void wave_visit_pins_hv_until(const uint32_t* __restr
     const uint32_t incl = wave_scan_incl(len);
     const uint32_t total = __shfl(incl, 63, 64);
     const uint32_t excl = incl - len;
+    const uint64_t le_l = lanes_upto(l);
     for (uint32_t f0 = 0; f0 < total; f0 += 64) {
       if (f0 && stop()) return;
       const uint32_t fi = f0 + l;
-      uint32_t k = 0;
-#pragma unroll
-      for (uint32_t step = 32; step > 0; step >>= 1) {
-        const uint32_t cand = k + step;
-        const uint32_t ex = __shfl(excl, (int)cand, 64);
-        if (ex <= fi) k = cand;
-      }
+      const uint32_t k = chunk_owner(excl, f0, le_l);
       PinVisitHV pv;
       pv.act = fi < total;
       pv.e = __shfl(e, (int)k, 64);
