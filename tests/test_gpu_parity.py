@@ -235,6 +235,11 @@ REFINE_CASES = [
     ("incc_k12", 0, 12, "srcs"),
     ("incc_k40_float", 0, 40, "float_w"),
     ("kway_big", 1, 4, "big_hedges"),
+    # k-way up to 64 partitions scores from partition masks (both 32-bit words in use here),
+    # above 64 from partition lists
+    ("kway_k64_wide", 1, 64, "hgr_wide"),
+    ("kway_k48_float", 1, 48, "float_w"),
+    ("kway_k100_lists", 1, 100, "hgr_wide"),
 ]
 
 
