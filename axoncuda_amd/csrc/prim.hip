@@ -286,17 +286,23 @@ void scan_impl(Ctx& c, const TI* in, TO* out, uint64_t n, bool exclusive) {
 // ---------------------------------------------------------------------------------------
 // radix sort
 constexpr int RS_BLOCK = 256;
-constexpr int RS_ROUNDS = 16;
-constexpr int RS_TILE = RS_BLOCK * RS_ROUNDS;
+constexpr uint32_t RS_MAX_ROUNDS = 16;  // keys per thread and tile of a large sort
+constexpr uint32_t RS_TARGET_TILES = 1024;  // small sorts shrink their tiles to spread over the CUs
+
+// Rounds of RS_BLOCK keys per tile: one tile per workgroup, so a small sort runs many short
+// workgroups side by side instead of a few that walk 16 rounds one after another.
+inline uint32_t rs_rounds(uint64_t n) {
+  return (uint32_t)std::min<uint64_t>(RS_MAX_ROUNDS, std::max<uint64_t>(1, ceil_div(n, (uint64_t)RS_BLOCK * RS_TARGET_TILES)));
+}
 
 template <typename K>
-__global__ void k_rs_hist(const K* __restrict__ keys, uint64_t n, int shift, uint32_t nblocks,
+__global__ void k_rs_hist(const K* __restrict__ keys, uint64_t n, int shift, uint32_t nblocks, uint32_t rounds,
                           uint32_t* __restrict__ counts) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
-  uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
-  for (int r = 0; r < RS_ROUNDS; ++r) {
+  uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK * rounds;
+  for (uint32_t r = 0; r < rounds; ++r) {
     uint64_t i = base + (uint64_t)r * RS_BLOCK + threadIdx.x;
     if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
   }
@@ -307,14 +313,14 @@ __global__ void k_rs_hist(const K* __restrict__ keys, uint64_t n, int shift, uin
 template <typename K, bool HAS_VALS>
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                          K* __restrict__ okeys, uint32_t* __restrict__ ovals,
-                                                         uint64_t n, int shift, uint32_t nblocks,
+                                                         uint64_t n, int shift, uint32_t nblocks, uint32_t rounds,
                                                          const uint64_t* __restrict__ offsets) {
   __shared__ uint64_t run[256];
   __shared__ uint32_t wcnt[RS_BLOCK / 64][256];
   const uint32_t t = threadIdx.x, w = t / 64;
   run[t] = offsets[(uint64_t)t * nblocks + blockIdx.x];
-  uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
-  for (int r = 0; r < RS_ROUNDS; ++r) {
+  uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK * rounds;
+  for (uint32_t r = 0; r < rounds; ++r) {
     uint64_t i = base + (uint64_t)r * RS_BLOCK + t;
     if (base + (uint64_t)r * RS_BLOCK >= n) break;  // uniform across the block
 #pragma unroll
@@ -361,7 +367,8 @@ template <typename K>
 void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 0, DBuf<K>* kbuf = nullptr,
                 DBuf<uint32_t>* vbuf = nullptr) {
   if (n <= 1 || bits <= lo) return;
-  const uint32_t nblocks = ceil_div(n, RS_TILE);
+  const uint32_t rounds = rs_rounds(n);
+  const uint32_t nblocks = ceil_div(n, (uint64_t)RS_BLOCK * rounds);
   DBuf<K> k2(c, n);
   DBuf<uint32_t> v2(c, vals ? n : 0);
   DBuf<uint32_t> counts(c, (uint64_t)256 * nblocks);
@@ -372,17 +379,17 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 
   uint32_t* vb = v2.get();
   for (int shift = lo; shift < bits; shift += 8) {
     // bytes: keys read once, one count per (digit, tile) written
-    HGP_LAUNCH_B(n * sizeof(K) + 1024ull * nblocks, (k_rs_hist<K>), nblocks, RS_BLOCK, 0, c, ka, n, shift, nblocks,
+    HGP_LAUNCH_B(n * sizeof(K) + 1024ull * nblocks, (k_rs_hist<K>), nblocks, RS_BLOCK, 0, c, ka, n, shift, nblocks, rounds,
                  counts.get());
     exclusive_scan(c, counts.get(), offsets.get(), (uint64_t)256 * nblocks);
     // bytes: keys (and values) read once and written once, one offset per (digit, tile) read
     const uint64_t sb = n * (2 * sizeof(K) + (vals ? 8 : 0)) + 2048ull * nblocks;
     if (vals)
       HGP_LAUNCH_B(sb, (k_rs_scatter<K, true>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks,
-                   offsets.get());
+                   rounds, offsets.get());
     else
       HGP_LAUNCH_B(sb, (k_rs_scatter<K, false>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks,
-                   offsets.get());
+                   rounds, offsets.get());
     std::swap(ka, kb);
     std::swap(va, vb);
   }
