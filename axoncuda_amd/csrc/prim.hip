@@ -287,7 +287,7 @@ void scan_impl(Ctx& c, const TI* in, TO* out, uint64_t n, bool exclusive) {
 // radix sort
 constexpr int RS_BLOCK = 256;
 constexpr uint32_t RS_MAX_ROUNDS = 16;  // keys per thread and tile of a large sort
-constexpr uint32_t RS_TARGET_TILES = 1024;  // small sorts shrink their tiles to spread over the CUs
+constexpr uint32_t RS_TARGET_TILES = 512;  // small sorts shrink their tiles to spread over the CUs
 
 // Rounds of RS_BLOCK keys per tile: one tile per workgroup, so a small sort runs many short
 // workgroups side by side instead of a few that walk 16 rounds one after another.
