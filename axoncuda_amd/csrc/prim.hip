@@ -360,6 +360,103 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K* __restrict__ k
   }
 }
 
+// Whole LSD sort of a small array by one workgroup: each pass (digit histogram, digit
+// offsets, stable scatter in rounds of RS1_BLOCK keys) runs between barriers, so a small sort
+// costs one launch instead of three or four per pass.  Keys of the pass stay in registers
+// between the histogram and the scatter.  Ends in kb/vb after an odd pass count.
+constexpr int RS1_BLOCK = 1024;
+constexpr uint32_t RS1_MAX_ROUNDS = 16;
+constexpr uint64_t RS1_MAX_N = (uint64_t)RS1_BLOCK * RS1_MAX_ROUNDS;
+
+template <typename K, bool HAS_VALS>
+__global__ void __launch_bounds__(RS1_BLOCK) k_rs_small(K* ka, uint32_t* va, K* kb, uint32_t* vb, uint32_t n,
+                                                        int lo, int bits) {
+  __shared__ uint32_t h[256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t s_w[4];
+  __shared__ uint32_t wcnt[RS1_BLOCK / 64][256];
+  const uint32_t t = threadIdx.x, w = t / 64, l = lane_id();
+  const uint32_t rounds = (n + RS1_BLOCK - 1) / RS1_BLOCK;
+  for (int shift = lo; shift < bits; shift += 8) {
+    K key[RS1_MAX_ROUNDS];
+    uint32_t val[RS1_MAX_ROUNDS];
+    if (t < 256) h[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < RS1_MAX_ROUNDS; ++r) {
+      const uint32_t i = r * RS1_BLOCK + t;
+      key[r] = 0;
+      val[r] = 0;
+      if (i < n) {
+        key[r] = ka[i];
+        if (HAS_VALS) val[r] = va[i];
+        atomicAdd(&h[(uint32_t)(key[r] >> shift) & 0xFFu], 1u);
+      }
+    }
+    __syncthreads();
+    // run[d]: keys of digits below d (exclusive scan of the histogram by the first four waves)
+    uint32_t x = 0, inc = 0;
+    if (t < 256) {
+      x = h[t];
+      inc = wave_scan_incl(x);
+      if (l == 63) s_w[w] = inc;
+    }
+    __syncthreads();
+    if (t < 256) {
+      uint32_t b = 0;
+      for (uint32_t q = 0; q < w; ++q) b += s_w[q];
+      run[t] = b + inc - x;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < RS1_MAX_ROUNDS; ++r) {
+      if (r < rounds) {  // uniform across the block
+#pragma unroll
+        for (int q = 0; q < 4; ++q) (&wcnt[0][0])[q * RS1_BLOCK + t] = 0;
+        __syncthreads();
+        const uint32_t i = r * RS1_BLOCK + t;
+        const bool valid = i < n;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const bool bit = (d >> b) & 1u;
+          const uint64_t bb = __ballot(bit);
+          peers &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = lanes_below(peers);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        // per digit (thread t < 256 handles digit t): exclusive prefix over the waves
+        uint32_t acc = 0;
+        if (t < 256) {
+#pragma unroll
+          for (int q = 0; q < RS1_BLOCK / 64; ++q) {
+            const uint32_t cq = wcnt[q][t];
+            wcnt[q][t] = acc;
+            acc += cq;
+          }
+        }
+        __syncthreads();
+        if (valid) {
+          const uint32_t pos = run[d] + wcnt[w][d] + rank;
+          kb[pos] = key[r];
+          if (HAS_VALS) vb[pos] = val[r];
+        }
+        __syncthreads();
+        if (t < 256) run[t] += acc;
+      }
+    }
+    K* tk = ka;
+    ka = kb;
+    kb = tk;
+    uint32_t* tv = va;
+    va = vb;
+    vb = tv;
+    __threadfence();  // the next pass reads what this one wrote
+    __syncthreads();
+  }
+}
+
 // LSD passes over bits [lo, bits).  With `kbuf` (and `vbuf`) the result ends up in those
 // buffers by exchanging them with the scratch ones after an odd pass count; otherwise it is
 // copied back into keys / vals.
@@ -367,6 +464,29 @@ template <typename K>
 void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 0, DBuf<K>* kbuf = nullptr,
                 DBuf<uint32_t>* vbuf = nullptr) {
   if (n <= 1 || bits <= lo) return;
+  if (n <= RS1_MAX_N) {
+    DBuf<K> k2(c, n);
+    DBuf<uint32_t> v2(c, vals ? n : 0);
+    const int passes = (bits - lo + 7) / 8;
+    // bytes: keys (and values) read once and written once per pass
+    const uint64_t sb = (uint64_t)passes * n * (2 * sizeof(K) + (vals ? 8 : 0));
+    if (vals)
+      HGP_LAUNCH_B(sb, (k_rs_small<K, true>), 1, RS1_BLOCK, 0, c, keys, vals, k2.get(), v2.get(), (uint32_t)n, lo,
+                   bits);
+    else
+      HGP_LAUNCH_B(sb, (k_rs_small<K, false>), 1, RS1_BLOCK, 0, c, keys, vals, k2.get(), v2.get(), (uint32_t)n, lo,
+                   bits);
+    if (passes & 1) {
+      if (kbuf && (!vals || vbuf)) {  // hand the scratch buffers over instead of copying
+        std::swap(*kbuf, k2);
+        if (vals) std::swap(*vbuf, v2);
+        return;
+      }
+      copy_d2d(c, keys, k2.get(), n * sizeof(K));
+      if (vals) copy_d2d(c, vals, v2.get(), n * sizeof(uint32_t));
+    }
+    return;
+  }
   const uint32_t rounds = rs_rounds(n);
   const uint32_t nblocks = ceil_div(n, (uint64_t)RS_BLOCK * rounds);
   DBuf<K> k2(c, n);
