@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
   __shared__ uint32_t s_score[4][CAND_CAP];
   __shared__ uint32_t s_inter[INTER ? 4 : 1][INTER ? CAND_CAP : 1];
   const uint32_t wv = threadIdx.x / 64, l = lane_id();
-  const uint32_t idx = blockIdx.x * 4 + wv;
+  const uint32_t idx = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
   if (idx >= nlist) return;
   const uint32_t u = list[idx];
   uint32_t* keys = s_keys[wv];
@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS, unsigned long lo
   __shared__ uint32_t s_inter[INTER ? CAND_MED_CAP : 1];
   __shared__ uint64_t s_w[64];
   __shared__ uint64_t s_bytes[4];
-  const uint32_t u = list[blockIdx.x];
+  const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
   if (!prof) {
     cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb,
                                      w, sizes, cnc, omega, delta, pairs, scores, s_w);
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(1024) k_cand_big(CAND_ARGS) {
   __shared__ uint32_t s_score[SLOTS];
   __shared__ uint32_t s_inter[INTER ? SLOTS : 1];
   __shared__ uint64_t s_w[256];
-  const uint32_t u = list[blockIdx.x];
+  const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
   cand_block_node<false, INTER, 16>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb,
                                     w, sizes, cnc, omega, delta, pairs, scores, s_w);
 }
@@ -421,7 +421,7 @@ __global__ void __launch_bounds__(1024) k_cand_opt(CAND_ARGS, uint32_t* __restri
   __shared__ uint32_t s_inter[INTER ? SLOTS : 1];
   __shared__ uint64_t s_w[256];
   __shared__ uint32_t s_state[2];
-  const uint32_t u = list[blockIdx.x];
+  const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
   const bool done = cand_block_node<false, INTER, 16, true, false, CH>(u, SLOTS, false, s_keys, s_score, s_inter,
                                                                        hedges, hoff, srcc, tch, toff, inb, w, sizes,
                                                                        cnc, omega, delta, pairs, scores, s_w, s_state,
@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(1024) k_cand_direct(CAND_ARGS) {
   __shared__ uint32_t s_score[N];
   __shared__ uint32_t s_inter[INTER ? N : 1];
   __shared__ uint64_t s_w[256];
-  const uint32_t u = list[blockIdx.x];
+  const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
   cand_block_node<false, INTER, 16>(u, caps[u], true, s_bits, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
                                     sizes, cnc, omega, delta, pairs, scores, s_w);
 }

@@ -275,6 +275,16 @@ __device__ __forceinline__ T wave_min(T v) {
   return wave_reduce(v, [](T a, T b) { return b < a ? b : a; });
 }
 
+// Workgroup b of a grid of g is dispatched to XCD b % 8 (round-robin over the 8 XCDs).  The
+// returned logical index hands XCD x one contiguous range of [0, g), so work items of nearby
+// ids (nodes that share hedges) meet in the same XCD's L2 instead of being fetched into all
+// eight.  A bijection on [0, g): XCD x runs g / 8 workgroups, one more when x < g % 8.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t g) {
+  constexpr uint32_t XCDS = 8;
+  const uint32_t x = b % XCDS, i = b / XCDS, q = g / XCDS, r = g % XCDS;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // inclusive prefix sum across the 64 lanes
 template <typename T>
 __device__ __forceinline__ T wave_scan_incl(T v) {
