@@ -297,14 +297,14 @@ inline uint32_t rs_rounds(uint64_t n) {
 
 template <typename K>
 __global__ void k_rs_hist(const K* __restrict__ keys, uint64_t n, int shift, uint32_t nblocks, uint32_t rounds,
-                          uint32_t* __restrict__ counts) {
+                          K kmin, uint32_t* __restrict__ counts) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
   uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK * rounds;
   for (uint32_t r = 0; r < rounds; ++r) {
     uint64_t i = base + (uint64_t)r * RS_BLOCK + threadIdx.x;
-    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+    if (i < n) atomicAdd(&h[(uint32_t)((keys[i] - kmin) >> shift) & 0xFFu], 1u);
   }
   __syncthreads();
   counts[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
@@ -314,7 +314,7 @@ template <typename K, bool HAS_VALS>
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                          K* __restrict__ okeys, uint32_t* __restrict__ ovals,
                                                          uint64_t n, int shift, uint32_t nblocks, uint32_t rounds,
-                                                         const uint64_t* __restrict__ offsets) {
+                                                         K kmin, const uint64_t* __restrict__ offsets) {
   __shared__ uint64_t run[256];
   __shared__ uint32_t wcnt[RS_BLOCK / 64][256];
   const uint32_t t = threadIdx.x, w = t / 64;
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K* __restrict__ k
     K key = valid ? keys[i] : (K)0;
     uint32_t val = 0;
     if (HAS_VALS && valid) val = vals[i];
-    const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+    const uint32_t d = (uint32_t)((key - kmin) >> shift) & 0xFFu;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -370,7 +370,7 @@ constexpr uint64_t RS1_MAX_N = (uint64_t)RS1_BLOCK * RS1_MAX_ROUNDS;
 
 template <typename K, bool HAS_VALS>
 __global__ void __launch_bounds__(RS1_BLOCK) k_rs_small(K* ka, uint32_t* va, K* kb, uint32_t* vb, uint32_t n,
-                                                        int lo, int bits) {
+                                                        int lo, int bits, K kmin) {
   __shared__ uint32_t h[256];
   __shared__ uint32_t run[256];
   __shared__ uint32_t s_w[4];
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(RS1_BLOCK) k_rs_small(K* ka, uint32_t* va, K* 
       if (i < n) {
         key[r] = ka[i];
         if (HAS_VALS) val[r] = va[i];
-        atomicAdd(&h[(uint32_t)(key[r] >> shift) & 0xFFu], 1u);
+        atomicAdd(&h[(uint32_t)((key[r] - kmin) >> shift) & 0xFFu], 1u);
       }
     }
     __syncthreads();
@@ -415,7 +415,7 @@ __global__ void __launch_bounds__(RS1_BLOCK) k_rs_small(K* ka, uint32_t* va, K* 
         __syncthreads();
         const uint32_t i = r * RS1_BLOCK + t;
         const bool valid = i < n;
-        const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
+        const uint32_t d = (uint32_t)((key[r] - kmin) >> shift) & 0xFFu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -457,12 +457,12 @@ __global__ void __launch_bounds__(RS1_BLOCK) k_rs_small(K* ka, uint32_t* va, K* 
   }
 }
 
-// LSD passes over bits [lo, bits).  With `kbuf` (and `vbuf`) the result ends up in those
+// LSD passes over bits [lo, bits) of key - kmin (the keys themselves are moved unchanged).  With `kbuf` (and `vbuf`) the result ends up in those
 // buffers by exchanging them with the scratch ones after an odd pass count; otherwise it is
 // copied back into keys / vals.
 template <typename K>
 void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 0, DBuf<K>* kbuf = nullptr,
-                DBuf<uint32_t>* vbuf = nullptr) {
+                DBuf<uint32_t>* vbuf = nullptr, K kmin = 0) {
   if (n <= 1 || bits <= lo) return;
   if (n <= RS1_MAX_N) {
     DBuf<K> k2(c, n);
@@ -472,10 +472,10 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 
     const uint64_t sb = (uint64_t)passes * n * (2 * sizeof(K) + (vals ? 8 : 0));
     if (vals)
       HGP_LAUNCH_B(sb, (k_rs_small<K, true>), 1, RS1_BLOCK, 0, c, keys, vals, k2.get(), v2.get(), (uint32_t)n, lo,
-                   bits);
+                   bits, kmin);
     else
       HGP_LAUNCH_B(sb, (k_rs_small<K, false>), 1, RS1_BLOCK, 0, c, keys, vals, k2.get(), v2.get(), (uint32_t)n, lo,
-                   bits);
+                   bits, kmin);
     if (passes & 1) {
       if (kbuf && (!vals || vbuf)) {  // hand the scratch buffers over instead of copying
         std::swap(*kbuf, k2);
@@ -499,17 +499,17 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 
   uint32_t* vb = v2.get();
   for (int shift = lo; shift < bits; shift += 8) {
     // bytes: keys read once, one count per (digit, tile) written
-    HGP_LAUNCH_B(n * sizeof(K) + 1024ull * nblocks, (k_rs_hist<K>), nblocks, RS_BLOCK, 0, c, ka, n, shift, nblocks, rounds,
+    HGP_LAUNCH_B(n * sizeof(K) + 1024ull * nblocks, (k_rs_hist<K>), nblocks, RS_BLOCK, 0, c, ka, n, shift, nblocks, rounds, kmin,
                  counts.get());
     exclusive_scan(c, counts.get(), offsets.get(), (uint64_t)256 * nblocks);
     // bytes: keys (and values) read once and written once, one offset per (digit, tile) read
     const uint64_t sb = n * (2 * sizeof(K) + (vals ? 8 : 0)) + 2048ull * nblocks;
     if (vals)
       HGP_LAUNCH_B(sb, (k_rs_scatter<K, true>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks,
-                   rounds, offsets.get());
+                   rounds, kmin, offsets.get());
     else
       HGP_LAUNCH_B(sb, (k_rs_scatter<K, false>), nblocks, RS_BLOCK, 0, c, ka, va, kb, vb, n, shift, nblocks,
-                   rounds, offsets.get());
+                   rounds, kmin, offsets.get());
     std::swap(ka, kb);
     std::swap(va, vb);
   }
@@ -551,10 +551,6 @@ __global__ void k_minmax_u64(const uint64_t* __restrict__ p, uint64_t n, unsigne
 __global__ void k_init_minmax(unsigned long long* mm) {
   mm[0] = ~0ull;
   mm[1] = 0ull;
-}
-__global__ void k_sub_u64(uint64_t* p, uint64_t n, const unsigned long long* mm) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] -= mm[0];
 }
 
 template <typename K>
@@ -619,8 +615,8 @@ void range_impl(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, DBuf<uint64_
   unsigned long long h[2];
   copy_d2h(c, h, mm.get(), sizeof h);
   if (h[1] == h[0]) return;  // all keys equal: a stable sort is the identity
-  HGP_LAUNCH(k_sub_u64, ceil_div(n, 256), 256, 0, c, keys, n, mm.get());
-  radix_impl<uint64_t>(c, keys, vals, n, bit_width(h[1] - h[0]), 0, kbuf, vbuf);
+  // digits of key - min: the passes cover only the keys' range
+  radix_impl<uint64_t>(c, keys, vals, n, bit_width(h[1] - h[0]), 0, kbuf, vbuf, (uint64_t)h[0]);
 }
 }  // namespace
 

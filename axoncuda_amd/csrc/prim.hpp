@@ -25,8 +25,8 @@ void radix_sort_bits(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int lo,
 // hold different allocations; any raw pointer taken before is stale).
 void radix_sort_buf(Ctx& c, DBuf<uint32_t>& keys, DBuf<uint32_t>& vals, uint64_t n, int lo, int hi);
 void radix_sort_buf(Ctx& c, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, uint64_t n, int lo, int hi);
-// Stable sort over the keys' actual range: subtracts the minimum key (the keys are left
-// rebased) and sorts only the bits that the maximum difference needs.
+// Stable sort over the keys' actual range: digits are taken from key - min, so only the bits
+// that the maximum difference needs are sorted; the keys come out unchanged.
 void radix_sort_range(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n);
 // The same on buffers the caller owns, exchanged with the scratch ones instead of copied back
 // (see radix_sort_buf).
