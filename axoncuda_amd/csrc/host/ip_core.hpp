@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <stdexcept>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -779,6 +781,12 @@ inline void partition(const Hypergraph& h, uint32_t k, float eps, uint64_t omega
     bisect_recursive(h, gids, k, 0, eps_b, omega, opt, part, 0);
     repair_balance(h, k, omega, part);
   }
+  // constraints that no k-partition can meet (a size limit below the nodes' weights) leave nodes
+  // unplaced: report them instead of indexing past the k partition sizes
+  for (uint32_t v = 0; v < h.n; ++v)
+    if (part[v] >= k)
+      throw std::runtime_error("initial partitioning: node " + std::to_string(v) + " fits in none of the " +
+                               std::to_string(k) + " partitions of size limit " + std::to_string(omega));
   for (uint32_t p = 0; p < k; ++p) psizes[p] = 0;
   for (uint32_t v = 0; v < h.n; ++v) psizes[part[v]] += (uint32_t)h.vw[v];
 }
