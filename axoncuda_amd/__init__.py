@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("initial_partitions_merge", C.c_int32),
         ("verbose", C.c_int32),
         ("initial_partitioner", C.c_int32),
+        ("initial_prune", C.c_int32),
     ]
 
 
@@ -60,6 +61,7 @@ class RunStats(C.Structure):
         ("bytes_algorithmic", C.c_uint64),
         ("level_nodes", C.c_uint32 * 64),
         ("level_pins", C.c_uint64 * 64),
+        ("peak_device_bytes", C.c_uint64),
     ]
 
 
@@ -76,6 +78,9 @@ SIGNATURES = [
     ("hgp_memcpy_htod", C.c_int32, [_vp, _vp, _vp, C.c_uint64]),
     ("hgp_memcpy_dtoh", C.c_int32, [_vp, _vp, _vp, C.c_uint64]),
     ("hgp_memcpy_dtod", C.c_int32, [_vp, _vp, _vp, C.c_uint64]),
+    ("hgp_ctx_memory", C.c_int32, [_vp, _u64p, _u64p, _u64p]),
+    ("hgp_ctx_trim", C.c_int32, [_vp]),
+    ("hgp_job_footprint", C.c_uint64, [_vp, C.POINTER(Params)]),
     ("hgp_hgraph_load", C.c_int32, [C.c_char_p, C.c_int, C.POINTER(_vp)]),
     ("hgp_hgraph_from_csr", C.c_int32, [C.c_uint32, C.c_uint32, _u64p, _u32p, _u32p, C.POINTER(C.c_float), C.c_int,
                                         C.POINTER(_vp)]),
@@ -120,9 +125,9 @@ SIGNATURES = [
     ("hgp_inbound_sizes", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp]),
     ("hgp_connectivity", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, _i64p]),
     ("hgp_initial_partition", C.c_int32, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, _vp, C.c_uint32,
-                                          C.c_float, C.c_uint32, C.POINTER(_vp), C.POINTER(_vp)]),
-    ("hgp_ip_input_check", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
-                                       C.POINTER(C.c_uint64)]),
+                                          C.c_float, C.c_uint32, C.c_int32, C.POINTER(_vp), C.POINTER(_vp)]),
+    ("hgp_ip_input", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_int32, _u32p, _u64p, _u64p,
+                                 _u32p, _u64p]),
     ("hgp_partition", C.c_int32, [_vp, C.POINTER(Params), _vp, _u32p, C.POINTER(RunStats)]),
     ("hgp_profile_kernel", C.c_int32, [_vp, C.c_char_p]),
     ("hgp_profile_read", C.c_int32, [_vp, C.POINTER(C.c_double), _u64p, _u64p]),

@@ -127,6 +127,7 @@ void hgp_default_params(hgp_params* p) {
   p->initial_partitions_merge = 0;
   p->verbose = 0;
   p->initial_partitioner = HGP_IP_HOST;
+  p->initial_prune = 1;
 }
 
 hgp_status hgp_ctx_create(int device, uint64_t hbm_budget, hgp_ctx** out) {
@@ -215,6 +216,16 @@ hgp_status hgp_memcpy_dtod(hgp_ctx* ctx, void* d_dst, const void* d_src, uint64_
     hgp::sync(c);
   });
 }
+
+hgp_status hgp_ctx_memory(const hgp_ctx* ctx, uint64_t* in_use, uint64_t* cached, uint64_t* peak) {
+  if (!ctx) return HGP_E_INVALID;
+  if (in_use) *in_use = ctx->c.in_use;
+  if (cached) *cached = ctx->c.cached;
+  if (peak) *peak = ctx->c.peak;
+  return HGP_OK;
+}
+
+hgp_status hgp_ctx_trim(hgp_ctx* ctx) { return guard(ctx, [&](Ctx& c) { hgp::release_cached(c); }); }
 
 hgp_status hgp_memcpy_dtoh(hgp_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes) {
   return guard(ctx, [&](Ctx& c) { hgp::copy_d2h(c, h_dst, d_src, bytes); });
@@ -566,38 +577,30 @@ hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64
 hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
                                  const float* d_hedge_weights, uint32_t num_nodes, uint32_t num_hedges,
                                  uint64_t hedges_size, const uint32_t* d_nodes_sizes, uint32_t k, float epsilon,
-                                 uint32_t max_nodes_per_part, uint32_t** d_partitions, uint32_t** d_partitions_sizes) {
+                                 uint32_t max_nodes_per_part, int32_t prune, uint32_t** d_partitions,
+                                 uint32_t** d_partitions_sizes) {
   (void)hedges_size;
   if (!d_partitions || !d_partitions_sizes || k == 0) return HGP_E_INVALID;
   return guard(ctx, [&](Ctx& c) {
     hgp::initial_partition(c, d_hedges, d_hedges_offsets, d_hedge_weights, num_nodes, num_hedges, d_nodes_sizes, k,
-                           epsilon, max_nodes_per_part, *d_partitions, *d_partitions_sizes, nullptr);
+                           epsilon, max_nodes_per_part, prune != 0, *d_partitions, *d_partitions_sizes, nullptr);
   });
 }
 
-hgp_status hgp_ip_input_check(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
-                              const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
-                              uint32_t num_hedges, int32_t* equal, uint64_t* merged_hedges) {
-  if (!equal) return HGP_E_INVALID;
+hgp_status hgp_ip_input(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                        const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                        uint32_t num_hedges, int32_t prune, uint32_t* num_merged, uint64_t* merged_pins,
+                        uint64_t* h_offsets, uint32_t* h_pins, uint64_t* h_weights) {
+  if (!num_merged || !merged_pins) return HGP_E_INVALID;
   return guard(ctx, [&](Ctx& c) {
-    hgp_ip::Hypergraph dh;
-    hgp::ip_input_device(c, d_hedges, d_hedges_offsets, d_hedge_weights, d_nodes_sizes, num_nodes, num_hedges, dh);
-    const uint32_t e = num_hedges;
-    std::vector<uint64_t> off((size_t)e + 1, 0);
-    if (e) hgp::copy_d2h(c, off.data(), d_hedges_offsets, ((uint64_t)e + 1) * 8);
-    std::vector<uint32_t> pins(off[e]), sizes(num_nodes), wfx(e);
-    std::vector<float> w(e);
-    if (off[e]) hgp::copy_d2h(c, pins.data(), d_hedges, off[e] * 4);
-    if (num_nodes) hgp::copy_d2h(c, sizes.data(), d_nodes_sizes, (uint64_t)num_nodes * 4);
-    if (e) hgp::copy_d2h(c, w.data(), d_hedge_weights, (uint64_t)e * 4);
-    for (uint32_t i = 0; i < e; ++i) {
-      const float v = w[i] * 262144.0f;
-      wfx[i] = !(v > 0.0f) ? 0u : (v >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)v);
-    }
-    const hgp_ip::Hypergraph hh = hgp_ip::from_level(num_nodes, e, off.data(), pins.data(), wfx.data(), sizes.data());
-    *equal = dh.n == hh.n && dh.off == hh.off && dh.pins == hh.pins && dh.w == hh.w && dh.vw == hh.vw &&
-             dh.ioff == hh.ioff && dh.inc == hh.inc;
-    if (merged_hedges) *merged_hedges = dh.off.size() - 1;
+    hgp_ip::Hypergraph h;
+    hgp::ip_input_device(c, d_hedges, d_hedges_offsets, d_hedge_weights, d_nodes_sizes, num_nodes, num_hedges,
+                         prune != 0, h);
+    *num_merged = (uint32_t)(h.off.size() - 1);
+    *merged_pins = h.pins.size();
+    if (h_offsets) std::copy(h.off.begin(), h.off.end(), h_offsets);
+    if (h_pins) std::copy(h.pins.begin(), h.pins.end(), h_pins);
+    if (h_weights) std::copy(h.w.begin(), h.w.end(), h_weights);
   });
 }
 

@@ -83,20 +83,20 @@ bool run_mtkahypar(const std::string& bin, const hgp_ip::Hypergraph& h, uint32_t
 }  // namespace
 
 void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                       const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, uint32_t*& part_out,
+                       const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, bool prune, uint32_t*& part_out,
                        uint32_t*& psizes_out, double* ms, const RefineLevel* device_level) {
   auto t0 = std::chrono::high_resolution_clock::now();
   static const std::string mtk = find_mtkahypar();
   if (device_level && mtk.empty()) {  // device label-propagation partitioner (opt-in)
     RefineLevel RL = *device_level;
     RL.k = k;
-    initial_partition_device(c, RL, omega, DEVICE_IP_STARTS, DEVICE_IP_ROUNDS, part_out, psizes_out);
+    initial_partition_device(c, RL, omega, DEVICE_IP_STARTS, DEVICE_IP_ROUNDS, prune, part_out, psizes_out);
     if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
     return;
   }
   const auto t1 = t0;
   hgp_ip::Hypergraph h;
-  ip_input_device(c, hedges, hoff, w, sizes, n, e, h);
+  ip_input_device(c, hedges, hoff, w, sizes, n, e, prune, h);
   const auto t2 = std::chrono::high_resolution_clock::now();
   std::vector<uint32_t> part(n, 0), psizes(k, 0);
   bool done = false;
@@ -112,7 +112,13 @@ void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, con
       if (c.verbose) fprintf(stderr, "WARNING: mtkahypar failed, using the built-in initial partitioner\n");
     }
   }
-  if (!done) hgp_ip::partition(h, k, eps, omega, part.data(), psizes.data());
+  if (!done) {
+    try {
+      hgp_ip::partition(h, k, eps, omega, part.data(), psizes.data());
+    } catch (const hgp_ip::Infeasible& x) {
+      throw Error(HGP_E_INFEASIBLE, x.what());
+    }
+  }
   const auto t3 = std::chrono::high_resolution_clock::now();
   if (c.verbose >= 2) {
     using msd = std::chrono::duration<double, std::milli>;
@@ -164,10 +170,15 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
   HGP_REQUIRE(omega > 0 && delta > 0 && max_parts > 0, HGP_E_INVALID, "constraints must be positive");
   HGP_REQUIRE(prm.candidates_count >= 1 && prm.candidates_count <= 16, HGP_E_INVALID, "-cnc must be in 1..16");
   HGP_REQUIRE(!kw || prm.kway >= 1, HGP_E_INVALID, "-k must be positive");
+  // K-way runs size the partition tables by k; the reference sets max_parts = k in that mode
+  // (sources/runconfig.cpp:190-192), so any other value is a caller error.
+  HGP_REQUIRE(!kw || max_parts == prm.kway, HGP_E_INVALID, "k-way mode needs max_parts == kway");
   const uint32_t cnc = prm.candidates_count;
   const uint32_t target_parts = std::min(max_parts, (uint32_t)(((uint64_t)n0 + omega - 1) / omega));
   hgp_run_stats st{};
   st.total_pins = P0;
+  const uint64_t held_before = c.in_use;
+  c.peak = c.in_use;
 
   hipEvent_t ev_start, ev_core, ev_stop;
   HGP_HIP(hipEventCreate(&ev_start));
@@ -225,12 +236,12 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
     const RefineLevel* ip_dev = prm.initial_partitioner == HGP_IP_DEVICE ? &ipl : nullptr;
     if (kw && n < 8192) {  // KWAY_INIT_UPPER_THREASHOLD (headers/defines.cuh:35)
       uint32_t *pp, *ps;
-      initial_partition(c, L.hedges.get(), L.hoff.get(), w.get(), n, E, L.sizes.get(), prm.kway, prm.epsilon, omega, pp, ps,
-                        &st.ip_ms, ip_dev);
+      initial_partition(c, L.hedges.get(), L.hoff.get(), w.get(), n, E, L.sizes.get(), prm.kway, prm.epsilon, omega,
+                        prm.initial_prune != 0, pp, ps, &st.ip_ms, ip_dev);
       part = DBuf<uint32_t>();
       part.ctx = &c, part.ptr = pp, part.n = n;
       psizes.ctx = &c, psizes.ptr = ps, psizes.n = prm.kway;
-      K = max_parts;
+      K = prm.kway;
       break;
     }
     DBuf<uint32_t> pairs(c, (uint64_t)n * cnc), scores(c, (uint64_t)n * cnc);
@@ -249,11 +260,11 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
     if (c.verbose) fprintf(stderr, "  grouped %u -> %u nodes (ratio %.3f)\n", n, nn, ratio);
     if (kw && (ratio > 0.95f || nn < 1024)) {  // KWAY_INIT_SHRINK_RATIO_LIMIT / LOWER_THREASHOLD
       uint32_t *pp, *ps;
-      initial_partition(c, L.hedges.get(), L.hoff.get(), w.get(), n, E, L.sizes.get(), prm.kway, prm.epsilon, omega, pp, ps,
-                        &st.ip_ms, ip_dev);
+      initial_partition(c, L.hedges.get(), L.hoff.get(), w.get(), n, E, L.sizes.get(), prm.kway, prm.epsilon, omega,
+                        prm.initial_prune != 0, pp, ps, &st.ip_ms, ip_dev);
       part.ctx = &c, part.ptr = pp, part.n = n;
       psizes.ctx = &c, psizes.ptr = ps, psizes.n = prm.kway;
-      K = max_parts;
+      K = prm.kway;
       break;
     }
     if (!kw && (nn <= target_parts || nn == n || (depth >= 32 && ratio > 0.95f))) {
@@ -347,6 +358,7 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
   st.core_ms = core_ms;
   st.device_ms = dev_ms;
   st.bytes_algorithmic = (uint64_t)bytes;
+  st.peak_device_bytes = c.peak - held_before;
   (void)repeats_total;
   if (stats) *stats = st;
 }

@@ -21,7 +21,7 @@ constexpr uint32_t DEVICE_IP_ROUNDS = HGP_DIP_ROUNDS;
 // recursive bisection, or the device partitioner when `device_level` (the same level with
 // its touching lists) is given.
 void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                       const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, uint32_t*& part_out,
+                       const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, bool prune, uint32_t*& part_out,
                        uint32_t*& psizes_out, double* ms, const RefineLevel* device_level = nullptr);
 
 // Level-0 input: either a host model or device-resident arrays (copied into the level).

@@ -19,6 +19,11 @@
 
 namespace hgp_ip {
 
+// Raised when the size limit cannot hold the nodes in k partitions.
+struct Infeasible : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 struct Hypergraph {
   uint32_t n = 0;
   std::vector<uint64_t> off;   // hedge offsets
@@ -79,119 +84,6 @@ inline void build_incidence(Hypergraph& h, uint32_t threads = 1) {
     for (uint64_t e = lo; e < hi; ++e)
       for (uint64_t i = h.off[e]; i < h.off[e + 1]; ++i) h.inc[pos[h.pins[i]]++] = (uint32_t)e;
   });
-}
-
-// Coarse level -> IP input: hedges with >= 2 distinct pins, identical pin sets merged
-// (weights summed), in order of first appearance.  The per-hedge work (sort, dedup, hash)
-// runs on `threads` threads over contiguous hedge ranges; identical sets are then found in
-// hedge order through an open-addressing table keyed by the hash of the sorted pin set.
-inline Hypergraph from_level(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins,
-                             const uint32_t* wfx, const uint32_t* sizes, uint32_t threads = 8) {
-  Hypergraph h;
-  h.n = n;
-  h.vw.assign(sizes, sizes + n);
-  // 1. per hedge: sorted distinct pins (in place in a copy), length, hash
-  std::vector<uint32_t> sp(pins, pins + hoff[e]);
-  std::vector<uint32_t> len(e, 0);
-  std::vector<uint64_t> hs(e, 0);
-  auto work = [&](uint32_t lo, uint32_t hi) {
-    for (uint32_t i = lo; i < hi; ++i) {
-      uint32_t* b = sp.data() + hoff[i];
-      uint32_t* t = sp.data() + hoff[i + 1];
-      std::sort(b, t);
-      uint32_t l = (uint32_t)(std::unique(b, t) - b);
-      len[i] = l;
-      if (l < 2) continue;
-      uint64_t x = 1469598103934665603ull;
-      for (uint32_t j = 0; j < l; ++j) x = splitmix64(x ^ b[j]);
-      hs[i] = x;
-    }
-  };
-  threads = std::max<uint32_t>(1, std::min<uint32_t>(threads, e / 65536 + 1));
-  if (threads == 1) {
-    work(0, e);
-  } else {
-    std::vector<std::thread> th;
-    for (uint32_t t = 0; t < threads; ++t)
-      th.emplace_back(work, (uint32_t)((uint64_t)e * t / threads), (uint32_t)((uint64_t)e * (t + 1) / threads));
-    for (auto& x : th) x.join();
-  }
-  // 2. merge identical sets: every hedge finds the first hedge with the same pin set.  The
-  // hash space is split over threads; each thread walks its hedges in order with a table of
-  // its own, so rep[] does not depend on the thread count.
-  std::vector<uint32_t> rep(e, UINT32_MAX);
-  {
-    const uint32_t T = threads;
-    std::vector<std::thread> th;
-    auto part = [&](uint32_t t) {
-      uint64_t mine = 0;
-      for (uint32_t i = 0; i < e; ++i) mine += len[i] >= 2 && (hs[i] >> 48) % T == t;
-      uint64_t cap = 16;
-      while (cap < 2 * mine + 2) cap <<= 1;
-      std::vector<uint32_t> table(cap, UINT32_MAX);
-      for (uint32_t i = 0; i < e; ++i) {
-        const uint32_t l = len[i];
-        if (l < 2 || (hs[i] >> 48) % T != t) continue;
-        const uint64_t x = hs[i];
-        const uint32_t* b = sp.data() + hoff[i];
-        uint64_t slot = x & (cap - 1);
-        for (;;) {
-          const uint32_t c = table[slot];
-          if (c == UINT32_MAX) {
-            table[slot] = i;
-            rep[i] = i;
-            break;
-          }
-          if (hs[c] == x && len[c] == l && std::equal(b, b + l, sp.data() + hoff[c])) {
-            rep[i] = c;
-            break;
-          }
-          slot = (slot + 1) & (cap - 1);
-        }
-      }
-    };
-    for (uint32_t t = 1; t < T; ++t) th.emplace_back(part, t);
-    part(0);
-    for (auto& x : th) x.join();
-  }
-  // 3. unique hedges in order of first appearance; weights summed over their copies.  Per
-  // chunk counts of unique hedges and their pins give every chunk its output positions.
-  std::vector<uint32_t> uid(e, UINT32_MAX);
-  std::vector<uint64_t> cu(threads + 1, 0), cp(threads + 1, 0);
-  parallel_chunks(e, threads, [&](uint32_t t, uint64_t lo, uint64_t hi) {
-    uint64_t u = 0, q = 0;
-    for (uint64_t i = lo; i < hi; ++i)
-      if (rep[i] == i) {
-        u++;
-        q += len[i];
-      }
-    cu[t + 1] = u;
-    cp[t + 1] = q;
-  });
-  for (uint32_t t = 0; t < threads; ++t) {
-    cu[t + 1] += cu[t];
-    cp[t + 1] += cp[t];
-  }
-  const uint64_t m = cu[threads], np = cp[threads];
-  h.off.resize(m + 1);
-  h.w.resize(m);
-  h.pins.resize(np);
-  h.off[0] = 0;
-  parallel_chunks(e, threads, [&](uint32_t t, uint64_t lo, uint64_t hi) {
-    uint64_t u = cu[t], q = cp[t];
-    for (uint64_t i = lo; i < hi; ++i)
-      if (rep[i] == i) {
-        uid[i] = (uint32_t)u;
-        h.w[u] = wfx[i];
-        std::copy(sp.data() + hoff[i], sp.data() + hoff[i] + len[i], h.pins.data() + q);
-        q += len[i];
-        h.off[++u] = q;
-      }
-  });
-  for (uint32_t i = 0; i < e; ++i)  // the copies (few) add their weights to their first hedge
-    if (rep[i] != UINT32_MAX && rep[i] != i) h.w[uid[rep[i]]] += wfx[i];
-  build_incidence(h, threads);
-  return h;
 }
 
 // Indexed binary max-heap keyed by (gain, -id).
@@ -770,6 +662,17 @@ inline void repair_balance(const Hypergraph& h, uint32_t k, uint64_t omega, uint
 // Entry point: part[n] in [0,k), psizes[k] = summed node weights.
 inline void partition(const Hypergraph& h, uint32_t k, float eps, uint64_t omega, uint32_t* part, uint32_t* psizes,
                       const Options& opt = Options()) {
+  // Constraints that no k-partition meets: a node heavier than the size limit, or more
+  // weight than k parts of that limit hold.  Checked before any part is assigned.
+  uint64_t total = 0, heaviest = 0;
+  for (uint32_t v = 0; v < h.n; ++v) {
+    total += h.vw[v];
+    heaviest = std::max<uint64_t>(heaviest, h.vw[v]);
+  }
+  if (k == 0 || heaviest > omega || total > (uint64_t)k * omega)
+    throw Infeasible("initial partitioning: " + std::to_string(h.n) + " nodes of total weight " +
+                     std::to_string(total) + " (heaviest " + std::to_string(heaviest) + ") do not fit " +
+                     std::to_string(k) + " partitions of size limit " + std::to_string(omega));
   std::vector<uint32_t> gids(h.n);
   for (uint32_t v = 0; v < h.n; ++v) gids[v] = v;
   uint32_t levels = 0;
@@ -781,12 +684,6 @@ inline void partition(const Hypergraph& h, uint32_t k, float eps, uint64_t omega
     bisect_recursive(h, gids, k, 0, eps_b, omega, opt, part, 0);
     repair_balance(h, k, omega, part);
   }
-  // constraints that no k-partition can meet (a size limit below the nodes' weights) leave nodes
-  // unplaced: report them instead of indexing past the k partition sizes
-  for (uint32_t v = 0; v < h.n; ++v)
-    if (part[v] >= k)
-      throw std::runtime_error("initial partitioning: node " + std::to_string(v) + " fits in none of the " +
-                               std::to_string(k) + " partitions of size limit " + std::to_string(omega));
   for (uint32_t p = 0; p < k; ++p) psizes[p] = 0;
   for (uint32_t v = 0; v < h.n; ++v) psizes[part[v]] += (uint32_t)h.vw[v];
 }

@@ -90,6 +90,9 @@ void worker_loop(hgp_server* s, hgp_ctx* ctx) {
     job->part.assign(job->n, 0);
     job->status = hgp_partition(ctx, &job->prm, job->hg, job->part.data(), &job->st);
     if (job->status != HGP_OK) job->error = hgp_ctx_last_error(ctx);
+    // The worker's context keeps the job's freed blocks for reuse; admission counts only
+    // running jobs, so give them back before the job's reservation is released.
+    (void)hgp_ctx_trim(ctx);
     job->t_end = clock_type::now();
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -104,6 +107,15 @@ void worker_loop(hgp_server* s, hgp_ctx* ctx) {
 }  // namespace
 
 extern "C" {
+
+uint64_t hgp_job_footprint(const hgp_hgraph* hg, const hgp_params* params) {
+  (void)params;
+  if (!hg) return 0;
+  uint32_t n = 0, e = 0;
+  uint64_t p = 0;
+  hgp_hgraph_sizes(hg, &n, &e, &p);
+  return footprint(n, e, p);
+}
 
 hgp_status hgp_server_create(int device, uint64_t hbm_budget, uint32_t workers, hgp_server** out) {
   if (!out || workers == 0) return HGP_E_INVALID;
@@ -142,7 +154,7 @@ hgp_status hgp_server_submit(hgp_server* s, const hgp_hgraph* hg, const hgp_para
   job->hg = hg;
   job->prm = *params;
   job->n = n;
-  job->footprint = footprint(n, e, p);
+  job->footprint = hgp_job_footprint(hg, params);
   job->t_submit = clock_type::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

@@ -98,7 +98,7 @@ void multistart_lp(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t starts
 // projected back level by level with the pipeline's k-way refinement.  The given level
 // itself is left to the caller's refinement.
 void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uint32_t starts, uint32_t rounds,
-                              uint32_t*& part_out, uint32_t*& psizes_out) {
+                              bool prune, uint32_t*& part_out, uint32_t*& psizes_out) {
   struct Mini {
     RefineLevel L;
     DBuf<uint32_t> hedges, srcc, tch, inb, sizes, groups;
@@ -114,12 +114,12 @@ void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uin
   const uint32_t cnc = 16, delta = 0x7FFFFFFFu;
   // a merged level of n nodes from (hedges, hoff, w) with e hedges; takes ownership of sizes
   auto make_level = [&](const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                        DBuf<uint32_t>&& sizes) {
+                        bool prune_level, DBuf<uint32_t>&& sizes) {
     auto M = std::make_unique<Mini>();
     uint32_t me = 0, *mh;
     uint64_t* mo;
     float* mw;
-    merge_level_device(c, hedges, hoff, w, n, e, me, mh, mo, mw);
+    merge_level_device(c, hedges, hoff, w, n, e, prune_level, me, mh, mo, mw);
     M->hedges.ctx = &c, M->hedges.ptr = mh, M->hedges.n = 1;
     M->hoff.ctx = &c, M->hoff.ptr = mo, M->hoff.n = (uint64_t)me + 1;
     M->w.ctx = &c, M->w.ptr = mw, M->w.n = std::max<uint32_t>(me, 1);
@@ -141,7 +141,7 @@ void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uin
   {
     DBuf<uint32_t> s0(c, std::max<uint32_t>(L0.n, 1));
     copy_d2d(c, s0.get(), L0.sizes, (uint64_t)L0.n * 4);
-    lv.push_back(make_level(L0.hedges, L0.hoff, L0.w, L0.n, L0.e, std::move(s0)));
+    lv.push_back(make_level(L0.hedges, L0.hoff, L0.w, L0.n, L0.e, prune, std::move(s0)));
   }
   if (c.verbose >= 2)
     fprintf(stderr, "  device initial partition: merged level %u nodes, %u hedges, %llu pins (%.1f ms)\n", L0.n,
@@ -168,7 +168,7 @@ void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uin
     dch.ctx = &c, dch.ptr = ch, dch.n = std::max<uint64_t>(cs, 1);
     dco.ctx = &c, dco.ptr = co, dco.n = (uint64_t)L.e + 1;
     dcc.ctx = &c, dcc.ptr = cc, dcc.n = std::max<uint32_t>(L.e, 1);
-    auto N = make_level(ch, co, L.w, nn, L.e, std::move(gsizes));
+    auto N = make_level(ch, co, L.w, nn, L.e, false, std::move(gsizes));
     M.groups = std::move(groups);
     lv.push_back(std::move(N));
     if (c.verbose >= 2)

@@ -6,7 +6,21 @@
 // weights summed, unique hedges in order of first appearance, node sizes as vertex weights.
 // The host version of this step (hgp_ip::from_level) walks 6.7M pins on the c2 benchmark;
 // here it is a handful of sorts and scans, and the host receives the merged hypergraph and
-// its incidence lists ready to use.  The output equals from_level's exactly.
+// its incidence lists ready to use.
+//
+// Before the merge, hedges are pruned as the reference prunes them (armonic_degree_score_kernel
+// and prune_hedges_kernel, kernels/init_part.cu:221-281, launched at sources/init_part.cu:749-817):
+// a hedge's score is its weight over the sum of its pins' inverse degrees, the expected number
+// of kept hedges is N * floor(P / E), and kept weights are divided by the keep probability.
+// The reference draws with cuRAND Philox in float; this build uses exact integers, a
+// splitmix64 draw per hedge and three IEEE double operations, so that the device and the
+// oracle keep the same hedges with the same weights:
+//   inv_e   = sum_{v in e} floor(2^24 / deg(v))                       (u64)
+//   ratio_e = min(floor(W_e * 2^24 / inv_e), floor((2^64 - 1) / E))    (u64; 0 when inv_e = 0)
+//   S = sum_e ratio_e,  T = N * floor(P / E)
+//   p_e  = S == 0 ? 1 : min(1, (double)T * (double)ratio_e / (double)S)
+//   keep = (double)(splitmix64(86 ^ e * 0x9E3779B97F4A7C15) >> 11) < p_e * 2^53
+//   W'_e = keep ? min((uint64)((double)W_e / p_e), 2^40) : 0
 #include <vector>
 
 #include "host/ip_core.hpp"
@@ -43,9 +57,10 @@ __global__ void k_ipp_compact(const uint64_t* __restrict__ keys, const uint32_t*
 // per hedge: distinct-pin range, hash of the sorted set, fixed-point weight; hedges of fewer
 // than two distinct pins are flagged out
 __global__ void k_ipp_hash(const uint64_t* __restrict__ hoff, const uint64_t* __restrict__ kpos,
-                           const uint32_t* __restrict__ pins, const float* __restrict__ w, uint32_t e,
-                           uint64_t* __restrict__ hs, uint64_t* __restrict__ start, uint32_t* __restrict__ len,
-                           uint32_t* __restrict__ wfx, uint32_t* __restrict__ live) {
+                           const uint32_t* __restrict__ pins, const float* __restrict__ w,
+                           const unsigned long long* __restrict__ pw, uint32_t e, uint64_t* __restrict__ hs,
+                           uint64_t* __restrict__ start, uint32_t* __restrict__ len,
+                           unsigned long long* __restrict__ wfx, uint32_t* __restrict__ live) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= e) return;
   const uint64_t s = kpos[hoff[x]], t = kpos[hoff[x + 1]];
@@ -60,9 +75,66 @@ __global__ void k_ipp_hash(const uint64_t* __restrict__ hoff, const uint64_t* __
   hs[x] = h;
   start[x] = s;
   len[x] = l;
-  const float v = w[x] * 262144.0f;
-  wfx[x] = !(v > 0.0f) ? 0u : (v >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)v);
-  live[x] = l >= 2 ? 1u : 0u;
+  if (pw) {  // pruned: weight 0 marks a dropped hedge
+    wfx[x] = pw[x];
+    live[x] = l >= 2 && pw[x] != 0 ? 1u : 0u;
+  } else {
+    wfx[x] = fixed_weight(w[x]);
+    live[x] = l >= 2 ? 1u : 0u;
+  }
+}
+
+// deg[v] = number of pins equal to v (the touching degree)
+__global__ void k_ipp_deg(const uint32_t* __restrict__ hedges, uint64_t p, uint32_t* __restrict__ deg) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p) atomicAdd(&deg[hedges[i]], 1u);
+}
+
+// per hedge: ratio_e (see the header); the workgroup's sum goes to one of 64 spread counters
+__global__ __launch_bounds__(256) void k_ipp_ratio(const uint32_t* __restrict__ hedges,
+                                                   const uint64_t* __restrict__ hoff, const float* __restrict__ w,
+                                                   const uint32_t* __restrict__ deg, uint32_t e, uint64_t cap,
+                                                   unsigned long long* __restrict__ ratio,
+                                                   unsigned long long* __restrict__ sums) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t r = 0;
+  if (x < e) {
+    uint64_t inv = 0;
+    for (uint64_t i = hoff[x]; i < hoff[x + 1]; ++i) inv += (uint64_t)(16777216u / deg[hedges[i]]);
+    if (inv) {
+      r = ((uint64_t)fixed_weight(w[x]) << 24) / inv;
+      if (r > cap) r = cap;
+    }
+    ratio[x] = r;
+  }
+  __shared__ uint64_t part[4];
+  const uint64_t ws = wave_sum(r);
+  if (lane_id() == 0) part[threadIdx.x / 64] = ws;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&sums[blockIdx.x % 64], (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
+__device__ __forceinline__ uint64_t splitmix64_dev(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// keep draw and scaled weight per hedge (see the header)
+__global__ void k_ipp_prune(const float* __restrict__ w, const unsigned long long* __restrict__ ratio, uint32_t e,
+                            double t, double sum, unsigned long long* __restrict__ pw) {
+#pragma clang fp contract(off)
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= e) return;
+  const double p = sum == 0.0 ? 1.0 : fmin(1.0, t * (double)ratio[x] / sum);
+  const uint64_t u = splitmix64_dev(86ull ^ ((uint64_t)x * 0x9E3779B97F4A7C15ull)) >> 11;
+  uint64_t out = 0;
+  if ((double)u < p * 9007199254740992.0) {
+    const double sc = (double)fixed_weight(w[x]) / p;
+    out = sc >= 1099511627776.0 ? 1099511627776ull : (uint64_t)sc;
+  }
+  pw[x] = out;
 }
 
 __global__ void k_ipp_live_keys(const uint32_t* __restrict__ live, const uint64_t* __restrict__ lpos,
@@ -118,14 +190,14 @@ __global__ void k_ipp_first(const uint32_t* __restrict__ rep, uint32_t e, uint32
 
 // output length and weight per unique hedge (u = its rank among the first hedges)
 __global__ void k_ipp_out_meta(const uint32_t* __restrict__ rep, const uint64_t* __restrict__ uid, uint32_t e,
-                               const uint32_t* __restrict__ len, const uint32_t* __restrict__ wfx,
+                               const uint32_t* __restrict__ len, const unsigned long long* __restrict__ wfx,
                                uint64_t* __restrict__ olen, unsigned long long* __restrict__ ow) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= e || rep[x] == NONE) return;
   const uint32_t r = rep[x];
   const uint64_t u = uid[r];
   if (r == x) olen[u] = len[x];
-  atomicAdd(&ow[u], (unsigned long long)wfx[x]);
+  atomicAdd(&ow[u], wfx[x]);
 }
 
 // pins of the unique hedges at their output offsets
@@ -172,11 +244,33 @@ struct Merged {
   uint64_t op = 0;
 };
 
+// Keep draws and scaled weights of the reference's hedge pruning (see the header): pw[e],
+// 0 for a dropped hedge.
+void prune_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
+                  uint64_t p, DBuf<unsigned long long>& pw) {
+  DBuf<uint32_t> deg(c, n);
+  memset_async(c, deg.get(), 0, (uint64_t)n * 4);
+  HGP_LAUNCH(k_ipp_deg, ceil_div(p, 256), 256, 0, c, hedges, p, deg.get());
+  DBuf<unsigned long long> ratio(c, e), sums(c, 64);
+  memset_async(c, sums.get(), 0, 64 * 8);
+  const uint64_t cap = ~0ull / e;
+  HGP_LAUNCH(k_ipp_ratio, ceil_div(e, 256), 256, 0, c, hedges, hoff, w, deg.get(), e, cap, ratio.get(), sums.get());
+  unsigned long long part[64];
+  copy_d2h(c, part, sums.get(), sizeof part);
+  uint64_t sum = 0;
+  for (auto v : part) sum += v;
+  const uint64_t t = (uint64_t)n * (p / e);
+  pw = DBuf<unsigned long long>(c, e);
+  HGP_LAUNCH(k_ipp_prune, ceil_div(e, 256), 256, 0, c, w, ratio.get(), e, (double)t, (double)sum, pw.get());
+}
+
 void merge_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                  Merged& out) {
+                  bool prune, Merged& out) {
   out = Merged();
   if (e == 0 || n == 0) return;
   const uint64_t p = read_scalar(c, hoff + e);
+  DBuf<unsigned long long> pw;
+  if (prune) prune_hedges(c, hedges, hoff, w, n, e, p, pw);
   const int pb = std::max(1, bit_width(n - 1));
   const int eb = std::max(1, bit_width(e - 1));
   HGP_REQUIRE(pb + eb <= 64, HGP_E_CAPACITY, "hedge merge: hedge and node ids exceed 64 key bits");
@@ -196,9 +290,11 @@ void merge_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const fl
   keys.reset();
   // 2. per hedge: hash, range, weight; hedges of >= 2 pins sorted by (hash, id)
   DBuf<uint64_t> hs(c, e), start(c, e);
-  DBuf<uint32_t> len(c, e), wfx(c, e), live(c, e);
-  HGP_LAUNCH(k_ipp_hash, ge, 256, 0, c, hoff, kpos.get(), pins.get(), w, e, hs.get(), start.get(), len.get(),
+  DBuf<uint32_t> len(c, e), live(c, e);
+  DBuf<unsigned long long> wfx(c, e);
+  HGP_LAUNCH(k_ipp_hash, ge, 256, 0, c, hoff, kpos.get(), pins.get(), w, pw.get(), e, hs.get(), start.get(), len.get(),
              wfx.get(), live.get());
+  pw.reset();
   DBuf<uint64_t> lpos(c, (uint64_t)e + 1);
   exclusive_scan(c, live.get(), lpos.get(), e);
   const uint32_t m = (uint32_t)read_scalar(c, lpos.get() + e);
@@ -232,9 +328,9 @@ void merge_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const fl
 }  // namespace
 
 void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                        uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out) {
+                        bool prune, uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out) {
   Merged M;
-  merge_hedges(c, hedges, hoff, w, n, e, M);
+  merge_hedges(c, hedges, hoff, w, n, e, prune, M);
   e_out = M.u;
   if (M.u == 0) {
     M.ooff = DBuf<uint64_t>(c, 1);
@@ -249,7 +345,7 @@ void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, co
 }
 
 void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
-                     uint32_t n, uint32_t e, hgp_ip::Hypergraph& h) {
+                     uint32_t n, uint32_t e, bool prune, hgp_ip::Hypergraph& h) {
   h = hgp_ip::Hypergraph();
   h.n = n;
   h.vw.resize(n);
@@ -261,7 +357,7 @@ void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const
   h.off.assign(1, 0);
   h.ioff.assign((uint64_t)n + 1, 0);
   Merged M;
-  merge_hedges(c, hedges, hoff, w, n, e, M);
+  merge_hedges(c, hedges, hoff, w, n, e, prune, M);
   const uint32_t u = M.u;
   if (u == 0) return;
   const uint64_t op = M.op;

@@ -59,7 +59,7 @@ uint32_t refine_analyze(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, 
 // Device k-way initial partition of a level (best-fit-decreasing starts, Jacobi rounds, the
 // start with the lowest lambda-1 kept): allocates part[n] and psizes[k].
 void initial_partition_device(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t starts, uint32_t rounds,
-                              uint32_t*& part_out, uint32_t*& psizes_out);
+                              bool prune, uint32_t*& part_out, uint32_t*& psizes_out);
 // One Jacobi label-propagation round (k-way): every node proposes its best move, positive-
 // gain moves are sorted by (target, gain desc, node) and applied while they fit next to the
 // target's size before the round.  Returns the number of moves applied.
@@ -71,14 +71,15 @@ int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff,
 
 // ---- initial partitioner input (ipprep.hip) ------------------------------------------------
 // The coarsest level as the initial partitioner takes it (hedges of >= 2 distinct pins,
-// sorted, identical sets merged in order of first appearance, incidence lists), built on the
-// device; equal to hgp_ip::from_level on the same level.
+// sorted, identical sets merged in order of first appearance with summed 2^18 weights,
+// incidence lists), built on the device.  With `prune`, hedges are first dropped or kept
+// with rescaled weights by the reference's degree-score pruning (sources/init_part.cu:749-817).
 void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
-                     uint32_t n, uint32_t e, hgp_ip::Hypergraph& h);
+                     uint32_t n, uint32_t e, bool prune, hgp_ip::Hypergraph& h);
 // The same merge left on the device as a level (sorted pins, no sources): e_out hedges with
 // float weights = summed fixed-point weights / 2^18.
 void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                        uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out);
+                        bool prune, uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out);
 
 // ---- post-processing (postproc.hip) -----------------------------------------------------
 void greedy_merge_groups(Ctx& c, const uint32_t* sizes, const uint32_t* inb, uint32_t n, uint32_t ngroups,

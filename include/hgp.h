@@ -68,6 +68,10 @@ typedef struct hgp_params {
    * bisection on the host, the default) or HGP_IP_DEVICE (best-fit-decreasing starts
    * improved by label propagation on the device).  No counterpart in the reference. */
   int32_t initial_partitioner;
+  /* 1 (default): before the k-way initial partition, drop hedges by the reference's
+   * degree-score pruning and rescale the kept weights (sources/init_part.cu:749-817);
+   * 0: hand every hedge to the initial partitioner. */
+  int32_t initial_prune;
 } hgp_params;
 
 enum { HGP_IP_HOST = 0, HGP_IP_DEVICE = 1 };
@@ -83,6 +87,7 @@ typedef struct hgp_run_stats {
   uint64_t bytes_algorithmic; /* compulsory traffic model, summed over levels */
   uint32_t level_nodes[64];
   uint64_t level_pins[64];
+  uint64_t peak_device_bytes; /* most device memory the run held at once (size classes) */
 } hgp_run_stats;
 
 /* ---- context and device memory ------------------------------------------ */
@@ -98,6 +103,11 @@ hgp_status hgp_memcpy_htod(hgp_ctx* ctx, void* d_dst, const void* h_src, uint64_
 hgp_status hgp_memcpy_dtoh(hgp_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
 /* Device-to-device copy on the context stream (synchronous); no counterpart in the reference. */
 hgp_status hgp_memcpy_dtod(hgp_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
+/* Device memory held by the context: bytes handed out, bytes of freed blocks kept for reuse,
+ * and the most handed out at once.  hgp_ctx_trim returns the kept blocks to HIP.  No
+ * counterpart in the reference (it frees every buffer with cudaFree). */
+hgp_status hgp_ctx_memory(const hgp_ctx* ctx, uint64_t* in_use, uint64_t* cached, uint64_t* peak);
+hgp_status hgp_ctx_trim(hgp_ctx* ctx);
 
 /* ---- host model: loaders and savers (includes/hgraph.hpp) ---------------- */
 /* Loads .hgr (hMETIS, loadhMETIS :936-1013), .snn (loadSNN :707-743) or .axh
@@ -266,24 +276,27 @@ hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64
                             int64_t* km1_fixed);
 
 /* initial_partitioning_kahypar (sources/init_part.cu:733-953): k-way initial
- * partition of a coarse level.  Runs `mtkahypar` when it is on PATH (same
- * command line as the reference); otherwise the built-in recursive-bisection
+ * partition of a coarse level.  With `prune`, hedges are first pruned as the reference
+ * prunes them (armonic_degree_score_kernel + prune_hedges_kernel, kernels/init_part.cu:221-281)
+ * with a splitmix64 draw per hedge in place of cuRAND Philox.  Runs `mtkahypar` when it is
+ * on PATH (same command line as the reference); otherwise the built-in recursive-bisection
  * partitioner.  Allocates d_partitions[num_nodes] and d_partitions_sizes[k]. */
 hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
                                  const float* d_hedge_weights, uint32_t num_nodes, uint32_t num_hedges,
                                  uint64_t hedges_size, const uint32_t* d_nodes_sizes, uint32_t k, float epsilon,
-                                 uint32_t max_nodes_per_part, uint32_t** d_partitions,
+                                 uint32_t max_nodes_per_part, int32_t prune, uint32_t** d_partitions,
                                  uint32_t** d_partitions_sizes);
 
-/* The initial partitioner's input (hedges of >= 2 distinct pins, sorted, identical sets
- * merged with summed 2^18 fixed-point weights, incidence lists) is built on the device.
- * This entry point builds it both on the device and on the host (the restatement of
- * sources/init_part.cu:733-953 that the partitioner's own tests use) and sets *equal to 1
- * when the two agree; *merged_hedges receives the device result's hedge count.  No
- * counterpart in the reference: it exists for parity tests of that step. */
-hgp_status hgp_ip_input_check(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
-                              const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
-                              uint32_t num_hedges, int32_t* equal, uint64_t* merged_hedges);
+/* The initial partitioner's input, built on the device exactly as hgp_initial_partition
+ * builds it (sources/init_part.cu:749-888): optional pruning, then hedges of >= 2 distinct
+ * pins with sorted pins, identical pin sets merged into their first appearance with their
+ * 2^18 fixed-point weights summed.  Two-phase: with NULL arrays only *num_merged and
+ * *merged_pins are set; then h_offsets[num_merged + 1], h_pins[merged_pins] and
+ * h_weights[num_merged] receive the result.  For parity tests of that step. */
+hgp_status hgp_ip_input(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                        const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                        uint32_t num_hedges, int32_t prune, uint32_t* num_merged, uint64_t* merged_pins,
+                        uint64_t* h_offsets, uint32_t* h_pins, uint64_t* h_weights);
 
 /* ---- whole pipeline (sources/main.cu:38-940) ------------------------------ */
 /* Partitions the host model; writes h_partitions_out[num_nodes] (zero-based ids). */
@@ -313,6 +326,9 @@ hgp_status hgp_partition_device(hgp_ctx* ctx, const hgp_params* params, uint32_t
  * of hgp_partition on the same input.  latency_ms = submit -> done, queue_ms = submit ->
  * start. */
 typedef struct hgp_server hgp_server;
+/* Device bytes the service reserves for one job when admitting it: an upper bound of the
+ * run's peak_device_bytes for the pipeline's per-level buffers. */
+uint64_t hgp_job_footprint(const hgp_hgraph* hg, const hgp_params* params);
 hgp_status hgp_server_create(int device, uint64_t hbm_budget, uint32_t workers, hgp_server** out);
 hgp_status hgp_server_submit(hgp_server* server, const hgp_hgraph* hg, const hgp_params* params, uint64_t* job_id);
 hgp_status hgp_server_wait(hgp_server* server, uint64_t job_id, uint32_t* h_partitions_out, hgp_run_stats* stats,

@@ -407,8 +407,13 @@ void hedge_counts(const Level& L, const uint32_t* part, std::vector<std::unorder
 // successor score (gain, -|size difference|) replaces w - alpha*|ds|; claims are won by
 // (score, -predecessor); components are ranked by (summed gain desc, head id asc) and
 // moves ordered by (component rank, position, sorted index).
+struct ChainLinks {  // per node: predecessor node (NONE = head), component head node, position
+  std::vector<uint32_t> prev, comp, pos;
+};
+
 void chain_ranks(uint32_t n, const uint32_t* part, const std::vector<uint32_t>& moves,
-                 const std::vector<int64_t>& gains, const uint32_t* sizes, std::vector<uint32_t>& ranks) {
+                 const std::vector<int64_t>& gains, const uint32_t* sizes, std::vector<uint32_t>& ranks,
+                 ChainLinks* links = nullptr) {
   // sorted space: stable by (src partition asc, gain desc); invalid moves last within a src
   std::vector<uint32_t> orig(n);
   for (uint32_t i = 0; i < n; ++i) orig[i] = i;
@@ -531,6 +536,16 @@ void chain_ranks(uint32_t n, const uint32_t* part, const std::vector<uint32_t>& 
   });
   ranks.assign(n, 0);
   for (uint32_t r = 0; r < n; ++r) ranks[orig[order[r]]] = r;
+  if (links) {
+    links->prev.assign(n, NONE);
+    links->comp.assign(n, NONE);
+    links->pos.assign(n, 0);
+    for (uint32_t j = 0; j < n; ++j) {
+      links->prev[orig[j]] = prev[j] == NONE ? NONE : orig[prev[j]];
+      links->comp[orig[j]] = orig[comp[j]];
+      links->pos[orig[j]] = pos[j];
+    }
+  }
 }
 
 void analyze(const Level& L, int mode, uint32_t omega, uint32_t delta, uint32_t R, uint32_t k, uint32_t fm_repeat,
@@ -901,6 +916,82 @@ void build_level(Level& L, uint32_t n, uint32_t E, const uint64_t* hoff, const u
   for (uint32_t e = 0; e < E; ++e) L.W[e] = fixed_weight(w[e]);
 }
 
+// The initial partitioner's input (sources/init_part.cu:749-888).  Pruning
+// (armonic_degree_score_kernel and prune_hedges_kernel, kernels/init_part.cu:221-281): a
+// hedge's score is its weight over the sum of its pins' inverse degrees, about N * floor(P/E)
+// hedges are kept, kept weights are divided by the keep probability.  The reference draws
+// with cuRAND Philox in float; the rule restated here is the build's exact one:
+//   inv_e = sum_{v in e} floor(2^24 / deg(v)); ratio_e = min(floor(W_e 2^24 / inv_e), floor((2^64-1)/E))
+//   p_e = S ? min(1, T * ratio_e / S) : 1 in IEEE double, T = N floor(P/E), S = sum ratio_e
+//   keep iff (splitmix64(86 ^ e * 0x9E3779B97F4A7C15) >> 11) < p_e * 2^53; W'_e = min(floor(W_e / p_e), 2^40)
+// Then the file writer's filter (:822-847): hedges of >= 2 pins, pins sorted, identical
+// pin sets merged with their weights summed (kept here in order of first appearance).
+struct IpInput {
+  std::vector<uint64_t> off{0};
+  std::vector<uint32_t> pins;
+  std::vector<uint64_t> w;
+};
+
+uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+std::vector<uint64_t> prune_weights(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w) {
+  std::vector<uint64_t> out(e, 0);
+  if (e == 0) return out;
+  const uint64_t P = hoff[e] - hoff[0];
+  std::vector<uint32_t> deg(n, 0);
+  for (uint64_t i = hoff[0]; i < hoff[e]; ++i) deg[pins[i]]++;
+  std::vector<uint64_t> ratio(e, 0);
+  const uint64_t cap = ~0ull / e;
+  uint64_t S = 0;
+  for (uint32_t x = 0; x < e; ++x) {
+    uint64_t inv = 0;
+    for (uint64_t i = hoff[x]; i < hoff[x + 1]; ++i) inv += 16777216u / deg[pins[i]];
+    if (inv) ratio[x] = std::min<uint64_t>(((uint64_t)fixed_weight(w[x]) << 24) / inv, cap);
+    S += ratio[x];
+  }
+  const double T = (double)((uint64_t)n * (P / e)), Sd = (double)S;
+  for (uint32_t x = 0; x < e; ++x) {
+    volatile double prod = T * (double)ratio[x];  // rounded on its own, as on the device
+    const double p = S == 0 ? 1.0 : std::min(1.0, prod / Sd);
+    const uint64_t u = splitmix64(86ull ^ ((uint64_t)x * 0x9E3779B97F4A7C15ull)) >> 11;
+    if ((double)u < p * 9007199254740992.0) {
+      const double sc = (double)fixed_weight(w[x]) / p;
+      out[x] = sc >= 1099511627776.0 ? 1099511627776ull : (uint64_t)sc;
+    }
+  }
+  return out;
+}
+
+IpInput ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w, bool prune) {
+  IpInput in;
+  std::vector<uint64_t> pw;
+  if (prune) pw = prune_weights(n, e, hoff, pins, w);
+  std::map<std::vector<uint32_t>, uint32_t> first;
+  for (uint32_t x = 0; x < e; ++x) {
+    if (prune && pw[x] == 0) continue;
+    std::vector<uint32_t> set(pins + hoff[x], pins + hoff[x + 1]);
+    std::sort(set.begin(), set.end());
+    set.erase(std::unique(set.begin(), set.end()), set.end());
+    if (set.size() < 2) continue;
+    const uint64_t wx = prune ? pw[x] : fixed_weight(w[x]);
+    auto it = first.find(set);
+    if (it != first.end()) {
+      in.w[it->second] += wx;
+      continue;
+    }
+    first.emplace(set, (uint32_t)in.w.size());
+    in.pins.insert(in.pins.end(), set.begin(), set.end());
+    in.off.push_back(in.pins.size());
+    in.w.push_back(wx);
+  }
+  return in;
+}
+
 }  // namespace
 
 // =======================================================================================
@@ -1170,18 +1261,67 @@ int orc_merge_small_partitions(uint32_t n, uint32_t k, const uint32_t* psizes, c
   return 0;
 }
 
+// The chaining step alone on given moves and gains (invalid moves: moves[v] == NONE):
+// ranks[v] and the links behind them (prev[v], component head comp[v], position pos[v]).
+int orc_chain_links(uint32_t n, const uint32_t* part, const uint32_t* moves, const int64_t* gains,
+                    const uint32_t* sizes, uint32_t* ranks, uint32_t* prev, uint32_t* comp, uint32_t* pos) {
+  std::vector<uint32_t> mv(moves, moves + n), rk;
+  std::vector<int64_t> gn(gains, gains + n);
+  ChainLinks L;
+  chain_ranks(n, part, mv, gn, sizes, rk, &L);
+  std::copy(rk.begin(), rk.end(), ranks);
+  std::copy(L.prev.begin(), L.prev.end(), prev);
+  std::copy(L.comp.begin(), L.comp.end(), comp);
+  std::copy(L.pos.begin(), L.pos.end(), pos);
+  return 0;
+}
+
+int orc_ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
+                 const uint32_t* sizes, int prune, uint32_t* e_out, uint64_t* p_out, uint64_t* off_out,
+                 uint32_t* pins_out, uint64_t* w_out, uint64_t* vw_out) {
+  IpInput in = ip_input(n, e, hoff, pins, w, prune != 0);
+  *e_out = (uint32_t)(in.off.size() - 1);
+  *p_out = in.pins.size();
+  if (off_out) std::copy(in.off.begin(), in.off.end(), off_out);
+  if (pins_out) std::copy(in.pins.begin(), in.pins.end(), pins_out);
+  if (w_out) std::copy(in.w.begin(), in.w.end(), w_out);
+  if (vw_out)
+    for (uint32_t v = 0; v < n; ++v) vw_out[v] = sizes[v];
+  return 0;
+}
+
 int orc_relabel(uint32_t n, uint32_t* part, uint32_t* new_k) {
   *new_k = relabel(n, part);
   return 0;
 }
 
 int orc_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
-                          const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, uint32_t* part,
+                          const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, int prune, uint32_t* part,
                           uint32_t* psizes) {
-  std::vector<uint32_t> wfx(e);
-  for (uint32_t i = 0; i < e; ++i) wfx[i] = fixed_weight(w[i]);
-  hgp_ip::Hypergraph h = hgp_ip::from_level(n, e, hoff, pins, wfx.data(), sizes);
-  hgp_ip::partition(h, k, eps, omega, part, psizes);
+  IpInput in = ip_input(n, e, hoff, pins, w, prune != 0);
+  hgp_ip::Hypergraph h;
+  h.n = n;
+  h.off = std::move(in.off);
+  h.pins = std::move(in.pins);
+  h.w = std::move(in.w);
+  h.vw.assign(sizes, sizes + n);
+  // incidence: the hedges of every node in ascending order
+  h.ioff.assign((uint64_t)n + 1, 0);
+  for (uint32_t v : h.pins) h.ioff[v + 1]++;
+  for (uint32_t v = 0; v < n; ++v) h.ioff[v + 1] += h.ioff[v];
+  h.inc.resize(h.pins.size());
+  {
+    std::vector<uint64_t> at(h.ioff.begin(), h.ioff.end() - 1);
+    for (uint64_t x = 0; x + 1 < h.off.size(); ++x)
+      for (uint64_t i = h.off[x]; i < h.off[x + 1]; ++i) h.inc[at[h.pins[i]]++] = (uint32_t)x;
+  }
+  try {
+    hgp_ip::partition(h, k, eps, omega, part, psizes);
+  } catch (const hgp_ip::Infeasible&) {
+    return 6;  // HGP_E_INFEASIBLE
+  } catch (const std::exception&) {
+    return 8;  // HGP_E_INTERNAL
+  }
   return 0;
 }
 
@@ -1189,10 +1329,11 @@ int orc_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const ui
 // (KWAY :358-391 and :448-485, INCC :495-564), uncoarsening and refinement on the way
 // up, INCC small-partition merge and final zero-based relabel (:803-830).
 int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, uint32_t kway, float eps, uint32_t cnc,
-                  uint32_t refine_repeats, int ipm, uint32_t n0, uint32_t E, const uint64_t* hoff0,
+                  uint32_t refine_repeats, int ipm, int prune, uint32_t n0, uint32_t E, const uint64_t* hoff0,
                   const uint32_t* pins0, const uint32_t* srcc0, const float* w, uint32_t* part_out,
                   uint32_t* num_levels) {
   const bool kw = mode == 1;
+  if (kw && max_parts != kway) return 1;  // k-way sets max_parts = k (sources/runconfig.cpp:190-192)
   const uint32_t target_parts = std::min(max_parts, (uint32_t)((n0 + omega - 1) / omega));
   struct Lv {
     uint32_t n;
@@ -1219,9 +1360,10 @@ int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, 
     if (kw && n < 8192) {  // KWAY base case before coarsening
       part.assign(n, 0);
       psizes.assign(kway, 0);
-      orc_initial_partition(n, E, L.hoff.data(), L.pins.data(), w, L.sizes.data(), kway, eps, omega, part.data(),
-                            psizes.data());
-      K = max_parts;
+      if (int rc = orc_initial_partition(n, E, L.hoff.data(), L.pins.data(), w, L.sizes.data(), kway, eps, omega,
+                                         prune, part.data(), psizes.data()))
+        return rc;
+      K = kway;
       break;
     }
     std::vector<uint32_t> pairs((uint64_t)n * cnc), scores((uint64_t)n * cnc);
@@ -1237,9 +1379,10 @@ int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, 
     if (kw && (ratio > 0.95f || nn < 1024)) {
       part.assign(n, 0);
       psizes.assign(kway, 0);
-      orc_initial_partition(n, E, L.hoff.data(), L.pins.data(), w, L.sizes.data(), kway, eps, omega, part.data(),
-                            psizes.data());
-      K = max_parts;
+      if (int rc = orc_initial_partition(n, E, L.hoff.data(), L.pins.data(), w, L.sizes.data(), kway, eps, omega,
+                                         prune, part.data(), psizes.data()))
+        return rc;
+      K = kway;
       break;
     }
     if (!kw && (nn <= target_parts || nn == n || (depth >= 32 && ratio > 0.95f))) {

@@ -89,14 +89,27 @@ int orc_merge_small_partitions(uint32_t n, uint32_t k, const uint32_t* psizes, c
 
 int orc_relabel(uint32_t n, uint32_t* part, uint32_t* new_k);
 
-/* Built-in k-way initial partitioner on a level (same algorithm as the product's). */
+/* Chaining alone (sources/chaining.cu:19-254): ranks plus the links behind them, per node:
+ * predecessor (UINT32_MAX for a head), component head and position along the chain. */
+int orc_chain_links(uint32_t n, const uint32_t* part, const uint32_t* moves, const int64_t* gains,
+                    const uint32_t* sizes, uint32_t* ranks, uint32_t* prev, uint32_t* comp, uint32_t* pos);
+
+/* The initial partitioner's input (sources/init_part.cu:749-888), restated here: optional
+ * hedge pruning, then hedges of >= 2 pins, sorted, identical sets merged in order of first
+ * appearance with summed weights; two-phase (NULL arrays return the sizes). */
+int orc_ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
+                 const uint32_t* sizes, int prune, uint32_t* e_out, uint64_t* p_out, uint64_t* off_out,
+                 uint32_t* pins_out, uint64_t* w_out, uint64_t* vw_out);
+
+/* Built-in k-way initial partitioner on a level: the oracle's own input restatement (with
+ * the hedge pruning when `prune`), partitioned by the product's recursive bisection. */
 int orc_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins,
                           const float* w, const uint32_t* sizes, uint32_t k, float eps, uint32_t omega,
-                          uint32_t* part, uint32_t* psizes);
+                          int prune, uint32_t* part, uint32_t* psizes);
 
 /* Whole pipeline (sources/main.cu:38-940) on a host CSR; part[n] out.  Returns 0 on success. */
 int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, uint32_t kway, float eps,
-                  uint32_t cnc, uint32_t refine_repeats, int ipm, uint32_t n, uint32_t e,
+                  uint32_t cnc, uint32_t refine_repeats, int ipm, int prune, uint32_t n, uint32_t e,
                   const uint64_t* hoff, const uint32_t* pins, const uint32_t* srcc, const float* w,
                   uint32_t* part, uint32_t* num_levels);
 

@@ -25,6 +25,14 @@ _lib = None
 _ref = None
 
 
+class OracleError(RuntimeError):
+    """A failing oracle call: `status` is the hgp_status code it returned."""
+
+    def __init__(self, status, where):
+        super().__init__(f"{where} returned status {status}")
+        self.status = status
+
+
 def P(a, t):
     return a.ctypes.data_as(t)
 
@@ -46,9 +54,10 @@ def lib():
         _lib.orc_connectivity_fixed.restype = C.c_int64
         _lib.orc_connectivity_fixed.argtypes = [C.c_uint32, u64p, u32p, f32p, u32p]
         _lib.orc_initial_partition.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, f32p, u32p, C.c_uint32, C.c_float,
-                                               C.c_uint32, u32p, u32p]
+                                               C.c_uint32, C.c_int, u32p, u32p]
         _lib.orc_partition.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
-                                       C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p, u32p, u32p]
+                                       C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p,
+                                       u32p, u32p]
         _lib.orc_metrics.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p, u32p, f64p, f64p, f64p]
     return _lib
 
@@ -272,21 +281,26 @@ def relabel(part):
     return k.value, part
 
 
-def initial_partition(hg: HG, sizes, k, eps, omega):
+def initial_partition(hg: HG, sizes, k, eps, omega, prune=True):
     part = np.zeros(hg.n, np.uint32)
     ps = np.zeros(k, np.uint32)
     sizes = np.ascontiguousarray(sizes, np.uint32)
-    lib().orc_initial_partition(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(sizes, u32p), k, eps,
-                                omega, P(part, u32p), P(ps, u32p))
+    rc = lib().orc_initial_partition(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(sizes, u32p),
+                                     k, eps, omega, 1 if prune else 0, P(part, u32p), P(ps, u32p))
+    if rc:
+        raise OracleError(rc, "orc_initial_partition")
     return part, ps
 
 
-def partition(hg: HG, mode, omega, delta, max_parts, kway=0, eps=0.0, cnc=16, repeats=16, ipm=False):
+def partition(hg: HG, mode, omega, delta, max_parts, kway=0, eps=0.0, cnc=16, repeats=16, ipm=False, prune=True):
     part = np.zeros(hg.n, np.uint32)
     lv = C.c_uint32()
-    lib().orc_partition(mode, omega, delta, max_parts, kway, eps, cnc, repeats, 1 if ipm else 0, hg.n, hg.e,
-                        P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(hg.w, f32p), P(part, u32p),
-                        C.byref(lv))
+    rc = lib().orc_partition(mode, omega, delta, max_parts, kway, eps, cnc, repeats, 1 if ipm else 0,
+                             1 if prune else 0, hg.n, hg.e,
+                             P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(hg.w, f32p), P(part, u32p),
+                             C.byref(lv))
+    if rc:
+        raise OracleError(rc, "orc_partition")
     return part, lv.value
 
 
@@ -308,3 +322,32 @@ def kway_omega(n, k, eps):
     """(uint32)ceil((1 + eps) * (float)n / k) in float32, as sources/runconfig.cpp:191."""
     v = (np.float32(1) + np.float32(eps)) * np.float32(n) / np.float32(k)
     return int(np.ceil(np.float32(v)))
+
+
+def chain_links(part, moves, gains, sizes):
+    """Chaining on given moves/gains: (ranks, prev, comp, pos) per node (NONE = no predecessor)."""
+    n = len(part)
+    part = np.ascontiguousarray(part, np.uint32)
+    moves = np.ascontiguousarray(moves, np.uint32)
+    gains = np.ascontiguousarray(gains, np.int64)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    ranks, prev, comp, pos = (np.zeros(n, np.uint32) for _ in range(4))
+    lib().orc_chain_links(n, P(part, u32p), P(moves, u32p), P(gains, i64p), P(sizes, u32p), P(ranks, u32p),
+                          P(prev, u32p), P(comp, u32p), P(pos, u32p))
+    return ranks, prev, comp, pos
+
+
+def ip_input(hg: HG, sizes, prune=True):
+    """The initial partitioner's input restated by the oracle from a level (optional hedge
+    pruning, merge of identical sets): (offsets, sorted pins, summed 2^18 weights, node weights)."""
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    e_out, p_out = C.c_uint32(), C.c_uint64()
+    args = (hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(sizes, u32p), 1 if prune else 0,
+            C.byref(e_out), C.byref(p_out))
+    lib().orc_ip_input(*args, None, None, None, None)
+    off = np.zeros(e_out.value + 1, np.uint64)
+    pins = np.zeros(p_out.value, np.uint32)
+    w = np.zeros(e_out.value, np.uint64)
+    vw = np.zeros(hg.n, np.uint64)
+    lib().orc_ip_input(*args, P(off, u64p), P(pins, u32p), P(w, u64p), P(vw, u64p))
+    return off, pins, w, vw

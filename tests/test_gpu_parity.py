@@ -372,12 +372,9 @@ def test_inbound_and_connectivity(dev, case, rng):
     assert km1.value == O.connectivity_fixed(hg, part)
 
 
-@pytest.mark.parametrize("n,e,max_size,copies", [(300, 2000, 5, 0.3), (5000, 40000, 12, 0.2), (64, 3000, 4, 0.6)])
-def test_ip_input_device_equals_host(dev, n, e, max_size, copies, rng):
-    """The initial partitioner's input built on the device (sort, merge identical pin sets,
-    sum weights, incidence) equals the host restatement exactly.  The hedges include exact
-    copies, copies with their pins permuted, repeated pins, single-pin hedges and weights
-    that round to zero in fixed point."""
+def ip_input_hypergraph(n, e, max_size, copies, rng):
+    """Hedges with exact copies, copies with their pins permuted, repeated pins, single-pin
+    hedges and weights that round to zero in fixed point."""
     hedges = []
     for _ in range(e):
         if hedges and rng.random() < copies:
@@ -393,14 +390,35 @@ def test_ip_input_device_equals_host(dev, n, e, max_size, copies, rng):
     pins = np.array([x for h in hedges for x in h], np.uint32)
     w = (rng.random(e) * 3).astype(np.float32)
     w[rng.random(e) < 0.05] = 1e-7
-    hg = O.HG(n, off, pins, np.zeros(e, np.uint32), w)
+    return O.HG(n, off, pins, np.zeros(e, np.uint32), w)
+
+
+def device_ip_input(dev, L, d_sizes, n, e, prune):
+    m, p = C.c_uint32(), C.c_uint64()
+    args = (dev.ctx, G.V(L.pins), G.V(L.off), G.V(L.w), G.V(d_sizes), n, e, 1 if prune else 0, C.byref(m), C.byref(p))
+    dev.check(dev.lib.hgp_ip_input(*args, None, None, None), "ip input sizes")
+    off = np.zeros(m.value + 1, np.uint64)
+    pins = np.zeros(p.value, np.uint32)
+    w = np.zeros(m.value, np.uint64)
+    dev.check(dev.lib.hgp_ip_input(*args, off.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                   pins.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   w.ctypes.data_as(C.POINTER(C.c_uint64))), "ip input")
+    return off, pins, w
+
+
+@pytest.mark.parametrize("prune", [False, True])
+@pytest.mark.parametrize("n,e,max_size,copies", [(300, 2000, 5, 0.3), (5000, 40000, 12, 0.2), (64, 3000, 4, 0.6)])
+def test_ip_input_device_equals_oracle(dev, n, e, max_size, copies, prune, rng):
+    """The initial partitioner's input built on the device (hedge pruning, sort, merge of
+    identical pin sets, summed weights) equals the oracle's restatement exactly."""
+    hg = ip_input_hypergraph(n, e, max_size, copies, rng)
     L = G.Level(dev, hg)
-    d_sizes = dev.upload(rng.integers(1, 5, size=n).astype(np.uint32))
-    eq, m = C.c_int32(), C.c_uint64()
-    dev.check(dev.lib.hgp_ip_input_check(dev.ctx, G.V(L.pins), G.V(L.off), G.V(L.w), G.V(d_sizes), n, e, C.byref(eq),
-                                         C.byref(m)), "ip input check")
-    assert eq.value == 1
-    assert 0 < m.value < e
+    sizes = rng.integers(1, 5, size=n).astype(np.uint32)
+    d_sizes = dev.upload(sizes)
+    off, pins, w = device_ip_input(dev, L, d_sizes, n, e, prune)
+    eo, ep, ew, _ = O.ip_input(hg, sizes, prune=prune)
+    assert np.array_equal(off, eo) and np.array_equal(pins, ep) and np.array_equal(w, ew)
+    assert 0 < len(w) < e
 
 
 def test_initial_partition(dev, rng):
@@ -412,7 +430,7 @@ def test_initial_partition(dev, rng):
     d_sizes = dev.upload(sizes)
     pp, ps = G.V(), G.V()
     dev.check(dev.lib.hgp_initial_partition(dev.ctx, G.V(L.pins), G.V(L.off), G.V(L.w), hg.n, hg.e, L.p, G.V(d_sizes),
-                                            k, eps, omega, C.byref(pp), C.byref(ps)), "initial partition")
+                                            k, eps, omega, 1, C.byref(pp), C.byref(ps)), "initial partition")
     gp = dev.download(pp.value, hg.n, np.uint32)
     ep, eps_ = O.initial_partition(hg, sizes, k, eps, omega)
     assert np.array_equal(gp, ep)
