@@ -130,6 +130,7 @@ SIGNATURES = [
                                  _u32p, _u64p]),
     ("hgp_partition", C.c_int32, [_vp, C.POINTER(Params), _vp, _u32p, C.POINTER(RunStats)]),
     ("hgp_profile_kernel", C.c_int32, [_vp, C.c_char_p]),
+    ("hgp_profile_count", C.c_int32, [_vp, C.c_int32]),
     ("hgp_profile_read", C.c_int32, [_vp, C.POINTER(C.c_double), _u64p, _u64p]),
     ("hgp_server_create", C.c_int32, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(_vp)]),
     ("hgp_server_submit", C.c_int32, [_vp, _vp, C.POINTER(Params), _u64p]),
@@ -308,18 +309,21 @@ class Device:
 
 
 class KernelProfile:
-    """Times every launch of one kernel (HIP events on the library's stream) from now on."""
+    """Times every launch of one kernel (HIP events on the library's stream) from now on;
+    with count_bytes the launches also count their algorithmic bytes (extra device work)."""
 
-    def __init__(self, dev: Device, kernel: str):
+    def __init__(self, dev: Device, kernel: str, count_bytes: bool = False):
         self.dev = dev
         dev.check(dev.lib.hgp_profile_read(dev.ctx, None, None, None), "hgp_profile_read")
         dev.check(dev.lib.hgp_profile_kernel(dev.ctx, kernel.encode()), "hgp_profile_kernel")
+        dev.check(dev.lib.hgp_profile_count(dev.ctx, 1 if count_bytes else 0), "hgp_profile_count")
 
     def read(self) -> dict:
         ms, n, b = C.c_double(), C.c_uint64(), C.c_uint64()
         self.dev.check(self.dev.lib.hgp_profile_read(self.dev.ctx, C.byref(ms), C.byref(n), C.byref(b)),
                        "hgp_profile_read")
         self.dev.check(self.dev.lib.hgp_profile_kernel(self.dev.ctx, None), "hgp_profile_kernel")
+        self.dev.check(self.dev.lib.hgp_profile_count(self.dev.ctx, 0), "hgp_profile_count")
         return {"ms": ms.value, "launches": n.value, "bytes": b.value}
 
 

@@ -321,7 +321,18 @@ hgp_status hgp_save_partitioning(const char* path, const uint32_t* part, uint32_
 hgp_status hgp_save_partitioned(const hgp_hgraph* hg, const uint32_t* part, const char* path) {
   if (!hg || !part || !path) return HGP_E_INVALID;
   std::vector<uint32_t> p(part, part + hg->n);
+  // the quotient needs partition ids 0..N-1 with none missing (includes/hgraph.hpp:292-295)
+  uint32_t k = 0;
+  for (uint32_t x : p) k = std::max(k, x + 1);
+  std::vector<uint8_t> seen(k, 0);
+  for (uint32_t x : p) seen[x] = 1;
+  for (uint8_t s : seen)
+    if (!s) return HGP_E_INVALID;
   hgp_hgraph q = hgp::partitions_hypergraph(*hg, p);
+  // every quotient hedge needs a destination (the reference's HyperGraph constructor,
+  // includes/hgraph.hpp:145, refuses the quotient otherwise)
+  for (uint32_t i = 0; i < q.e(); ++i)
+    if (q.off[i + 1] - q.off[i] == q.srcc[i]) return HGP_E_INVALID;
   return hgp::save_hgraph(q, path).empty() ? HGP_OK : HGP_E_IO;
 }
 
@@ -635,6 +646,8 @@ hgp_status hgp_profile_kernel(hgp_ctx* ctx, const char* kernel_name) {
   });
 }
 
+hgp_status hgp_profile_count(hgp_ctx* ctx, int32_t on) { return guard(ctx, [&](Ctx& c) { c.prof_count = on != 0; }); }
+
 hgp_status hgp_profile_read(hgp_ctx* ctx, double* total_ms, uint64_t* launches, uint64_t* algorithmic_bytes) {
   return guard(ctx, [&](Ctx& c) {
     hgp::sync(c);
@@ -674,7 +687,14 @@ hgp_status hgp_partition_device(hgp_ctx* ctx, const hgp_params* params, uint32_t
       hgp::PipelineInput in;
       in.n = num_nodes;
       in.e = num_hedges;
+      // the offsets start at 0 and the pins name nodes below num_nodes (the host path
+      // checks the same when it builds its model)
+      HGP_REQUIRE(hgp::read_scalar(c, d_hedges_offsets) == 0, HGP_E_INVALID,
+                  "hgp_partition_device: d_hedges_offsets[0] must be 0");
       in.p = num_hedges ? hgp::read_scalar(c, d_hedges_offsets + num_hedges) : 0;
+      HGP_REQUIRE(in.p == 0 || d_hedges, HGP_E_INVALID, "hgp_partition_device: pins without a pin array");
+      HGP_REQUIRE(in.p == 0 || hgp::reduce_max_u32(c, d_hedges, in.p) < num_nodes, HGP_E_INVALID,
+                  "hgp_partition_device: a pin is not below num_nodes");
       in.d_hedges = d_hedges;
       in.d_hoff = d_hedges_offsets;
       in.d_srcc = d_srcs_count;

@@ -262,8 +262,7 @@ __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32
 // OPT (optimistic): the node's distinct neighbours may exceed the table.  New keys are
 // counted; past `limit` the workgroup stops and the caller hands the node to a larger table.  At most one insert per thread is in flight when
 // the limit trips, so the table never fills.  Returns false when the node overflowed.
-// COUNT: this thread's share of the algorithmic bytes is added to *bytes at the end.
-template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true, int CH = 1, bool COUNT = false>
+template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true, int CH = 1>
 __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
                                                 uint32_t* inter, const uint32_t* __restrict__ hedges,
                                                 const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ srcc,
@@ -272,8 +271,7 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
                                                 const uint32_t* __restrict__ sizes, uint32_t cnc, uint32_t omega,
                                                 uint32_t delta, uint32_t* __restrict__ pairs,
                                                 uint32_t* __restrict__ scores, uint64_t* s_w,
-                                                uint32_t* s_state = nullptr, uint32_t limit = 0,
-                                                uint64_t* bytes = nullptr) {
+                                                uint32_t* s_state = nullptr, uint32_t limit = 0) {
   const uint32_t nkeys = direct ? (cap + 31) / 32 : cap;
   const uint64_t t0 = toff[u], t1 = toff[u + 1];
   const uint32_t inb_u = inb[u];
@@ -287,7 +285,6 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
   uint64_t top[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) top[i] = 0;
-  uint64_t acc = 0;  // COUNT: algorithmic bytes
   // CH > 1: the neighbours are split by id modulo CH and scored one class per pass, so the
   // table holds a CH-th of them; the per-thread top lists carry over between passes.
   for (int chunk = 0; chunk < CH; ++chunk) {
@@ -303,9 +300,7 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
     if (GMEM) __threadfence();
     __syncthreads();
     auto hvf = [&](uint32_t e, uint32_t len) { return make_uint2(fixed_weight(w[e]) / len, INTER ? srcc[e] : 0u); };
-    uint64_t visits = 0;
     wave_visit_pins_hv_until(tch, a, b, hoff, hedges, hvf, [&](const PinVisitHV& pv) {
-      if (COUNT && pv.act) ++visits;
       if (!pv.act || pv.pin == u) return;
       if (CH > 1 && pv.pin % CH != (uint32_t)chunk) return;
       const uint32_t contrib = pv.hv.x;  // floor(W / |e|), computed once per touching hedge
@@ -343,15 +338,12 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
         v = keys[s];
         if (v == NONE) continue;
       }
-      if (COUNT) acc += 8;  // the neighbour's size and inbound count
       top_insert(top, cand_key(u, v, score[s], INTER ? inter[s] : 0u, size_u, inb_u, sizes, inb, omega, delta));
     }
     // the pin ids visited; per touching hedge its list entry, offsets, weight (and source count)
-    if (COUNT) acc += 4 * visits + (threadIdx.x == 0 && chunk == 0 ? (INTER ? 28 : 24) * (t1 - t0) : 0);
     if (CH > 1) __syncthreads();  // the next pass clears the table
   }
   block_top_write<NW>(top, cnc, s_w, u, pairs, scores);
-  if (COUNT) *bytes += acc + (threadIdx.x == 0 ? 24 + 8 * (uint64_t)cnc : 0);  // + offsets, size, inbound, results
   return true;
 }
 
@@ -362,29 +354,15 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
       const uint32_t *__restrict__ caps, uint32_t cnc, uint32_t omega, uint32_t delta, uint32_t *__restrict__ pairs, \
       uint32_t *__restrict__ scores
 
-// `prof` (null unless this kernel is profiled): PROF_SLOTS counters of algorithmic bytes.
 template <bool INTER>
-__global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS, unsigned long long* __restrict__ prof) {
+__global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS) {
   __shared__ uint32_t s_keys[CAND_MED_CAP];
   __shared__ uint32_t s_score[CAND_MED_CAP];
   __shared__ uint32_t s_inter[INTER ? CAND_MED_CAP : 1];
   __shared__ uint64_t s_w[64];
-  __shared__ uint64_t s_bytes[4];
   const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
-  if (!prof) {
-    cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb,
-                                     w, sizes, cnc, omega, delta, pairs, scores, s_w);
-  } else {
-    uint64_t bytes = 0;
-    cand_block_node<false, INTER, 4, false, true, 1, true>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff,
-                                                           srcc, tch, toff, inb, w, sizes, cnc, omega, delta, pairs,
-                                                           scores, s_w, nullptr, 0, &bytes);
-    bytes = wave_sum(bytes);
-    if (lane_id() == 0) s_bytes[threadIdx.x / 64] = bytes;
-    __syncthreads();
-    if (threadIdx.x == 0)
-      atomicAdd(&prof[blockIdx.x % PROF_SLOTS], (unsigned long long)(s_bytes[0] + s_bytes[1] + s_bytes[2] + s_bytes[3]));
-  }
+  cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
+                                   sizes, cnc, omega, delta, pairs, scores, s_w);
 }
 
 // One node per 1024-thread workgroup with a hashed LDS table of up to CAND_BIG_SLOTS.
@@ -687,14 +665,8 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
 #define CAND_LAUNCH(K, list, cnt, threads)                                                                          \
   HGP_LAUNCH(K, cnt, threads, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, list, caps.get(), cnc, omega, delta, \
              pairs, scores)
-  // profiled: the kernel counts its algorithmic bytes itself
-  unsigned long long* prof = prof_match(c, "k_cand_medium") ? c.prof_dev : nullptr;
-  if (inter)
-    HGP_LAUNCH(k_cand_medium<true>, nm, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, med.get(), caps.get(), cnc,
-               omega, delta, pairs, scores, prof);
-  else
-    HGP_LAUNCH(k_cand_medium<false>, nm, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, med.get(), caps.get(),
-               cnc, omega, delta, pairs, scores, prof);
+  if (inter) CAND_LAUNCH(k_cand_medium<true>, med.get(), nm, 256);
+  else CAND_LAUNCH(k_cand_medium<false>, med.get(), nm, 256);
   if (n <= direct_n) {
     if (inter) CAND_LAUNCH(k_cand_direct<true>, big.get(), nb, 1024);
     else CAND_LAUNCH(k_cand_direct<false>, big.get(), nb, 1024);

@@ -78,6 +78,7 @@ struct Ctx {
   // algorithmic bytes that the profiled kernel counts itself on the device (PROF_SLOTS
   // counters, spread so that the atomics do not all meet on one address)
   unsigned long long* prof_dev = nullptr;
+  bool prof_count = false;  // kernels with a byte model count their bytes (extra device work)
   void* pinned = nullptr;  // small pinned host buffer for scalar read-backs (allocated on first use)
 };
 constexpr uint64_t PINNED_BYTES = 256;

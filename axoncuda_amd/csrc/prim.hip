@@ -580,6 +580,14 @@ __global__ void k_max_u64(const uint64_t* __restrict__ p, uint64_t n, unsigned l
   if (lane_id() == 0) atomicMax(out, (unsigned long long)m);
 }
 
+__global__ void k_max_u32(const uint32_t* __restrict__ p, uint64_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    m = p[i] > m ? p[i] : m;
+  m = wave_max(m);
+  if (lane_id() == 0) atomicMax(out, m);
+}
+
 }  // namespace
 
 void exclusive_scan(Ctx& c, const uint32_t* in, uint64_t* out, uint64_t n) { scan_impl<uint32_t, uint64_t>(c, in, out, n, true); }
@@ -627,6 +635,13 @@ void radix_sort_range_buf(Ctx& c, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, ui
 
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, uint64_t n) { HGP_LAUNCH(k_fill_u32, ceil_div(n, 256), 256, 0, c, p, v, n); }
 void iota_u32(Ctx& c, uint32_t* p, uint64_t n) { HGP_LAUNCH(k_iota_u32, ceil_div(n, 256), 256, 0, c, p, n); }
+uint32_t reduce_max_u32(Ctx& c, const uint32_t* p, uint64_t n) {
+  DBuf<uint32_t> out(c, 1);
+  memset_async(c, out.get(), 0, sizeof(uint32_t));
+  if (n) HGP_LAUNCH(k_max_u32, std::min<uint32_t>(ceil_div(n, 256), 2048), 256, 0, c, p, n, out.get());
+  return read_scalar(c, out.get());
+}
+
 uint64_t reduce_max_u64(Ctx& c, const uint64_t* p, uint64_t n) {
   DBuf<unsigned long long> out(c, 1);
   memset_async(c, out.get(), 0, sizeof(unsigned long long));

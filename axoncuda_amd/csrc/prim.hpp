@@ -39,6 +39,7 @@ void segment_offsets(Ctx& c, const uint64_t* keys, int shift, uint64_t n, uint32
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, uint64_t n);
 void iota_u32(Ctx& c, uint32_t* p, uint64_t n);
 uint64_t reduce_max_u64(Ctx& c, const uint64_t* p, uint64_t n);
+uint32_t reduce_max_u32(Ctx& c, const uint32_t* p, uint64_t n);
 
 inline int bit_width(uint64_t x) {
   int b = 0;

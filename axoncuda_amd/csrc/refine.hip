@@ -1628,6 +1628,33 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
   }
 }
 
+// Algorithmic bytes of one k_cascade_small launch, counted when that kernel is profiled (a
+// separate launch, outside the timed one).  Per hedge: its two offsets (8 B, shared with its
+// neighbours); per hedge of 2..CAS_SMALL pins: every pin id (4 B) and its node's (partition,
+// target, rank) (12 B), the weight (4 B) when a pin moves, and one 8-byte cascade term per
+// moving pin.
+__global__ __launch_bounds__(256) void k_cascade_small_bytes(CascadeArgs a, uint32_t E,
+                                                             unsigned long long* __restrict__ prof) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t b = 0;
+  if (e < E) {
+    b = 8;
+    const uint64_t s = a.hoff[e], t = a.hoff[e + 1];
+    const uint64_t len = t - s;
+    if (len >= 2 && len <= CAS_SMALL) {
+      uint32_t moving = 0;
+      for (uint64_t i = s; i < t; ++i) moving += a.pm[a.hedges[i]].y != NONE;
+      b += 16 * len + (moving ? 4 + 8 * (uint64_t)moving : 0);
+    }
+  }
+  __shared__ uint64_t part[4];
+  const uint64_t ws = wave_sum(b);
+  if (lane_id() == 0) part[threadIdx.x / 64] = ws;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicAdd(&prof[blockIdx.x % PROF_SLOTS], (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
 __global__ void __launch_bounds__(64) k_cascade_large(CascadeArgs a, const uint32_t* __restrict__ list) {
   __shared__ uint32_t s_pt[CAS_LDS], s_mv[CAS_LDS], s_rk[CAS_LDS];
   const uint32_t e = list[blockIdx.x];
@@ -2178,7 +2205,7 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
     DBuf<uint32_t> ml(c, std::max<uint32_t>(L.n, 1)), mc(c, 1);
     memset_async(c, mc.get(), 0, 4);
     HGP_LAUNCH(k_gains_route_mask, ceil_div(L.n, 256), 256, 0, c, a, pl.hm.get(), L.w, L.n, ml.get(), mc.get());
-    a.prof_bytes = prof_match(c, "k_gains_mask") ? c.prof_dev : nullptr;
+    a.prof_bytes = prof_match(c, "k_gains_mask") && c.prof_count ? c.prof_dev : nullptr;
     HGP_LAUNCH(k_gains_mask, std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID)), 256, 0, c, a,
                pl.hm.get(), L.w, ml.get(), mc.get());
     return;
@@ -2192,7 +2219,7 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   const uint32_t capmax = L.k <= GAIN_CAP ? std::max<uint32_t>(L.k, 1) : GAIN_CAP;
   const uint32_t lgrid = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID));
   // when this kernel is profiled it counts its algorithmic bytes itself (no extra pass)
-  a.prof_bytes = prof_match(c, "k_gains_light") ? c.prof_dev : nullptr;
+  a.prof_bytes = prof_match(c, "k_gains_light") && c.prof_count ? c.prof_dev : nullptr;
   HGP_LAUNCH(k_gains_light, lgrid, 256, 4 * capmax * 12, c, a, ll.get(), counts.get(), bound.get(), capmax);
   a.prof_bytes = nullptr;
   DBuf<uint32_t> hl;
@@ -2351,6 +2378,8 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
     if (!known)
       for (int cls = 0; cls < 3; ++cls) fl[cls] = DBuf<uint32_t>(c, L.e);
     HGP_LAUNCH(k_cascade_small, ceil_div(L.e, 256), 256, 0, c, a, L.e, sort_max, fl[0].get(), fl[1].get(), fl[2].get());
+    if (prof_match(c, "k_cascade_small") && c.prof_count && c.prof_dev)
+      HGP_LAUNCH(k_cascade_small_bytes, ceil_div(L.e, 256), 256, 0, c, a, L.e, c.prof_dev);
     if (!known) {
       Y.cas_sort_max = sort_max;
       DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);

@@ -88,4 +88,5 @@ void merge_small_partitions(Ctx& c, const uint32_t* psizes, const uint32_t* pinb
                             uint32_t omega, uint32_t delta, uint32_t* part);
 void relabel(Ctx& c, uint32_t n, uint32_t* part, uint32_t& new_k);
 
+
 }  // namespace hgp

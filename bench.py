@@ -7,14 +7,22 @@ partitioning (coarsening, initial partition, uncoarsening + refinement) of that 
 starting from device-resident CSR arrays; the partition vector lands in host memory.
 
 value = post-dedup pins / second of one step (the reference's "device core" boundary,
-sources/main.cu:260-264,891-896), with inputs already in HBM.
+sources/main.cu:260-264,891-896), with inputs already in HBM.  Beside it, the line carries:
 
-The roofline object covers the dominant kernel: its launches are bracketed by HIP events on
-the library's stream during the timed steps, and its algorithmic bytes per launch come from
-the library's byte model of that kernel.  cpu_baseline is the sequential oracle run on a
-bounded sample of the same workload (100K nodes) on the host.
+* pipeline: the compulsory HBM traffic of a step (every stage's inputs and outputs once per
+  executed level and refinement round) over the step time, against the 8 TB/s peak;
+* roofline: the kernel with the largest share of device time (the small-hedge cascade).  Its
+  launches in the timed steps are bracketed by HIP events on the library's stream; its
+  algorithmic bytes per launch are counted on the device in one extra, untimed step (the
+  pipeline is deterministic, so every step launches the same work); traffic comes from
+  rocprofv3 FETCH_SIZE / WRITE_SIZE counters scaled by a plain copy of known size;
+* end_to_end: the hgraph_gpu command line on the same hypergraph written as .hgr, timed from
+  process start to the partition file written (parse, dedup, upload, partition, write);
+* cpu_baseline: the sequential oracle on one host thread, on a 100K-node sample of the same
+  generator, with its lambda-1 and the GPU's lambda-1 on that same sample;
+* serving: the continuous-batching service over jobs of 10^4 to 2 x 10^7 pins.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-e2e] [--no-serve]
 """
 from __future__ import annotations
 
@@ -22,7 +30,9 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -37,11 +47,16 @@ K, EPS = 64, 0.03
 HBM_PEAK_GBS = 8000.0
 # Kernel whose launches the roofline object covers: the largest share of device time on this
 # workload in the kernel-trace summary under profiles/.
-ROOFLINE_KERNEL = "k_cand_medium"
+ROOFLINE_KERNEL = "k_cascade_small"
 # HBM bytes per launch of that kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
 # scaled by the same counters on a plain device copy of known size (profiles/traffic_<kernel>.json).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{ROOFLINE_KERNEL}.json")
 COPY_CAL_BYTES = 1 << 30  # the plain copy that calibrates the counters
+CPU_SAMPLE_NODES = 100_000
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def kway_omega(n: int, k: int, eps: float) -> int:
@@ -59,22 +74,63 @@ def kway_params(n: int, k: int, eps: float) -> H.Params:
     return prm
 
 
-def cpu_baseline(budget_s: float = 30.0) -> dict:
-    """Sequential oracle on a 100K-node sample of the same workload (one host core)."""
+def netlist(n: int, seed: int) -> H.Hypergraph:
+    e = int(round(n * N_NETS / N_NODES))
+    hg = H.Hypergraph.generate("netlist", n=n, e=e, alpha=ALPHA, max_size=MAX_SIZE, radius=RADIUS,
+                               frac_local=FRAC_LOCAL, seed=seed)
+    n, off, pins, srcc, w = hg.arrays()
+    return H.Hypergraph.from_csr(n, off, pins, srcc, w, dedup=True)
+
+
+def end_to_end(hg: H.Hypergraph, tmpdir: str) -> dict:
+    """The drop-in command line, timed from process start to the partition file written."""
+    n, e, p = hg.sizes()
+    path = os.path.join(tmpdir, "netlist.hgr")
+    pfile = os.path.join(tmpdir, "netlist.part")
+    hg.save(path)
+    exe = os.path.join(H.BIN_DIR, "hgraph_gpu")
+    cmd = [exe, "-r", path, "-k", str(K), str(EPS), "-p", pfile]
+    t0 = time.perf_counter()
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    wall = time.perf_counter() - t0
+    if r.returncode != 0 or not os.path.exists(pfile):
+        raise RuntimeError(f"hgraph_gpu failed ({r.returncode}): {r.stderr[-2000:]}")
+    km1 = None
+    for line in r.stdout.splitlines():
+        if line.startswith("Connectivity:"):
+            km1 = float(line.split()[1])
+    log(f"# end to end: {wall:.2f} s, lambda-1 {km1}")
+    return {"value": p / wall, "unit": "pins/s", "wall_s": wall, "lambda_minus_1": km1,
+            "command": "hgraph_gpu -r netlist.hgr -k 64 0.03 -p netlist.part (reference defaults otherwise: -smh 2)",
+            "boundary": "process start -> partition file written (.hgr parse, dedup, upload, partition, write)"}
+
+
+def cpu_baseline(lib, dev) -> dict:
+    """Sequential oracle (one host thread) on a 100K-node sample of the same generator, and
+    the GPU pipeline on the same sample, both scored by lambda-1."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    n = 100_000
-    e = int(round(n * N_NETS / N_NODES))
-    hg = H.Hypergraph.generate("netlist", n=n, e=e, alpha=ALPHA, max_size=MAX_SIZE, radius=RADIUS,
-                               frac_local=FRAC_LOCAL, seed=1)
+    hg = netlist(CPU_SAMPLE_NODES, 1)
     g = O.HG(*hg.arrays())
+    O.set_ip_threads(1)
     t0 = time.perf_counter()
-    O.partition(g, 1, kway_omega(n, K, EPS), 2**31 - 1, K, kway=K, eps=EPS)
+    part, _ = O.partition(g, 1, kway_omega(g.n, K, EPS), 2**31 - 1, K, kway=K, eps=EPS)
     dt = time.perf_counter() - t0
+    O.set_ip_threads(0)
+    km1_cpu = O.metrics(g, part)[0]
+    prm = kway_params(g.n, K, EPS)
+    gpart = np.zeros(g.n, np.uint32)
+    st = H.RunStats()
+    dev.check(lib.hgp_partition(dev.ctx, C.byref(prm), hg.h, gpart.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                C.byref(st)), "hgp_partition")
+    km1_gpu = hg.metrics(gpart)[0]
+    log(f"# cpu baseline: {dt:.1f} s on one thread, lambda-1 cpu {km1_cpu:.0f} gpu {km1_gpu:.0f}")
     return {"value": len(g.pins) / dt, "unit": "pins/s", "cores": 1, "kind": "port",
-            "sample": f"netlist {n} nodes / {e} nets / {len(g.pins)} pins, k={K}, eps={EPS}, "
-                      f"one run of the oracle pipeline ({dt:.1f} s)"}
+            "sample": f"netlist {g.n} nodes / {g.e} nets / {len(g.pins)} pins, k={K}, eps={EPS}; one run of the "
+                      f"oracle pipeline on one host thread ({dt:.1f} s)",
+            "lambda_minus_1": km1_cpu, "gpu_lambda_minus_1_same_sample": km1_gpu,
+            "gpu_core_ms_same_sample": st.core_ms, "same_partition": bool(np.array_equal(part, gpart))}
 
 
 def serve_jobs(n_jobs: int, max_pins: float, seed: int):
@@ -86,7 +142,7 @@ def serve_jobs(n_jobs: int, max_pins: float, seed: int):
     jobs = []
     for i in range(n_jobs):
         if i % 32 == 0:
-            print(f"# generating job {i}", file=sys.stderr, flush=True)
+            log(f"# generating job {i}")
         pins = float(np.exp(rng.uniform(np.log(1e4), np.log(max_pins))))
         if i % 2 == 0:
             e = max(64, int(pins / 4))
@@ -113,32 +169,34 @@ def serve_jobs(n_jobs: int, max_pins: float, seed: int):
     return jobs
 
 
-def serve_bench(args) -> dict:
+def serve_bench(jobs_n: int, max_pins: float, workers: int, load: float, seed: int) -> dict:
     """Plain sequential runs of the job stream, then the same jobs through the continuous-
     batching service with Poisson arrivals at `load` times the plain service rate (the
     speedup over plain runs cannot exceed about `load`)."""
-    jobs = serve_jobs(args.jobs, args.max_pins, args.seed)
+    jobs = serve_jobs(jobs_n, max_pins, seed)
     total_pins = sum(p for _, _, p in jobs)
-    print(f"# serving stream: {len(jobs)} jobs, {total_pins} pins", file=sys.stderr, flush=True)
+    log(f"# serving stream: {len(jobs)} jobs, {total_pins} pins")
     lib = H.load_library()
     dev = H.Device(0)
-    plain_parts = []
+    plain_parts, plain_lat = [], []
     t0 = time.perf_counter()
     for i, (hg, prm, _) in enumerate(jobs):
         if i % 32 == 0:
-            print(f"# plain: job {i}", file=sys.stderr, flush=True)
+            log(f"# plain: job {i}")
         n, _, _ = hg.sizes()
         part = np.zeros(n, np.uint32)
         st = H.RunStats()
+        t1 = time.perf_counter()
         dev.check(lib.hgp_partition(dev.ctx, C.byref(prm), hg.h, part.ctypes.data_as(C.POINTER(C.c_uint32)),
                                     C.byref(st)), "hgp_partition")
+        plain_lat.append((time.perf_counter() - t1) * 1e3)
         plain_parts.append(part)
     t_plain = time.perf_counter() - t0
     dev.close()
-    print(f"# plain sequential: {t_plain:.2f} s", file=sys.stderr, flush=True)
-    rate = args.load * len(jobs) / t_plain  # offered load: a multiple of what plain runs sustain
-    arrivals = np.cumsum(np.random.default_rng(args.seed + 7).exponential(1.0 / rate, len(jobs)))
-    srv = H.Server(0, workers=args.workers)
+    log(f"# plain sequential: {t_plain:.2f} s")
+    rate = load * len(jobs) / t_plain  # offered load: a multiple of what plain runs sustain
+    arrivals = np.cumsum(np.random.default_rng(seed + 7).exponential(1.0 / rate, len(jobs)))
+    srv = H.Server(0, workers=workers)
     ids = []
     t0 = time.perf_counter()
     for (hg, prm, _), ta in zip(jobs, arrivals):
@@ -149,7 +207,7 @@ def serve_bench(args) -> dict:
     lat, same = [], 0
     for i, j in enumerate(ids):
         if i % 32 == 0:
-            print(f"# served: waiting for job {i}", file=sys.stderr, flush=True)
+            log(f"# served: waiting for job {i}")
         part, st, l, q = srv.wait(j)
         lat.append(l)
         same += int(np.array_equal(part, plain_parts[i]))
@@ -159,11 +217,12 @@ def serve_bench(args) -> dict:
     return {"metric": "aggregate pins/s of the serving stream; p50/p99 job latency", "value": total_pins / t_serve,
             "unit": "pins/s", "plain_value": total_pins / t_plain, "speedup_vs_plain": t_plain / t_serve,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "jobs": len(jobs), "workers": args.workers, "total_pins": total_pins,
+            "plain_p50_ms": float(np.percentile(plain_lat, 50)), "plain_p99_ms": float(np.percentile(plain_lat, 99)),
+            "jobs": len(jobs), "workers": workers, "total_pins": total_pins,
             "arrival_rate_per_s": rate, "results_equal_to_plain": same, "data": "synthetic",
-            "config": {"workload": f"{len(jobs)} jobs, pins log-uniform in [1e4, {args.max_pins:g}], half random "
+            "config": {"workload": f"{len(jobs)} jobs, pins log-uniform in [1e4, {max_pins:g}], half random "
                                    "(|e| = 2 + Geom(1/2)), half layered SNN (fan-out 64); modes k=2/8/32 eps 0.03 "
-                                   f"or -m 1024 4096 1048576; Poisson arrivals at {args.load:g}x the plain service rate"}}
+                                   f"or -m 1024 4096 1048576; Poisson arrivals at {load:g}x the plain service rate"}}
 
 
 def main() -> int:
@@ -172,12 +231,15 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end command-line leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--serve", action="store_true", help="run the serving-stream benchmark instead")
-    ap.add_argument("--jobs", type=int, default=256)
-    ap.add_argument("--max-pins", type=float, default=2e6)
-    ap.add_argument("--workers", type=int, default=8)
-    ap.add_argument("--load", type=float, default=4.0, help="offered load of --serve, in multiples of the plain rate")
+    ap.add_argument("--serve", action="store_true", help="run only the serving-stream benchmark")
+    ap.add_argument("--jobs", type=int, default=32, help="serving jobs (the full stream is 256)")
+    ap.add_argument("--max-pins", type=float, default=2e7)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--load", type=float, default=4.0, help="offered load of the serving leg, in multiples of the "
+                                                             "plain rate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain device copy of 1 GiB (for rocprofv3 --pmc counter calibration)")
     args = ap.parse_args()
@@ -186,19 +248,18 @@ def main() -> int:
               "(value is the one GPU's throughput)", file=sys.stderr)
         return 2
     if args.serve:
-        print(json.dumps(serve_bench(args)))
+        print(json.dumps(serve_bench(args.jobs, args.max_pins, args.workers, args.load, args.seed)))
         return 0
     lib = H.load_library()
 
     t0 = time.perf_counter()
-    hg = H.Hypergraph.generate("netlist", n=N_NODES, e=N_NETS, alpha=ALPHA, max_size=MAX_SIZE, radius=RADIUS,
-                               frac_local=FRAC_LOCAL, seed=args.seed)
-    n, off, pins, srcc, w = hg.arrays()
-    hg = H.Hypergraph.from_csr(n, off, pins, srcc, w, dedup=True)
+    hg = netlist(N_NODES, args.seed)
     n, off, pins, srcc, w = hg.arrays()
     e, p = len(srcc), len(pins)
-    print(f"# workload generated: n={n} e={e} pins={p} mean|e|={p / e:.3f} ({time.perf_counter() - t0:.1f} s)",
-          file=sys.stderr, flush=True)
+    log(f"# workload generated: n={n} e={e} pins={p} mean|e|={p / e:.3f} ({time.perf_counter() - t0:.1f} s)")
+
+    tmpdir = tempfile.mkdtemp(prefix="hgp_bench_")
+    e2e = None if args.no_e2e else end_to_end(hg, tmpdir)  # its own process, before this one uses the GPU
 
     dev = H.Device(0)
     d_pins, d_off, d_srcc, d_w = dev.upload(pins), dev.upload(off), dev.upload(srcc), dev.upload(w)
@@ -214,33 +275,42 @@ def main() -> int:
 
     for i in range(args.warmup):
         step()
-        print(f"# warmup {i} done: {st.core_ms:.1f} ms, {st.levels} levels", file=sys.stderr, flush=True)
+        log(f"# warmup {i} done: {st.core_ms:.1f} ms, {st.levels} levels, ip {st.ip_ms:.1f} ms")
 
     prof = H.KernelProfile(dev, ROOFLINE_KERNEL)
     dev.check(lib.hgp_synchronize(dev.ctx), "sync")
     t0 = time.perf_counter()
-    core_ms = []
+    core_ms, ip_ms = [], []
     for i in range(args.steps):
         step()
         core_ms.append(st.core_ms)
+        ip_ms.append(st.ip_ms)
     dev.check(lib.hgp_synchronize(dev.ctx), "sync")
     elapsed = time.perf_counter() - t0
     kp = prof.read()
-    print(f"# timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    log(f"# timed {args.steps} steps in {elapsed:.3f} s")
+    # one more step, untimed, in which the roofline kernel counts its algorithmic bytes
+    cnt = H.KernelProfile(dev, ROOFLINE_KERNEL, count_bytes=True)
+    step()
+    kc = cnt.read()
 
     km1, cut, soed = hg.metrics(part)
     valid = hg.validity(part, prm.max_nodes_per_part, prm.max_inbound_per_part, prm.max_parts)
     ms_per_step = elapsed * 1e3 / args.steps
     value = p * args.steps / elapsed
 
+    pipeline = {"bytes_algorithmic_per_step": int(st.bytes_algorithmic),
+                "achieved": st.bytes_algorithmic / (ms_per_step / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": st.bytes_algorithmic / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+                "model": "every stage's inputs and outputs once per executed level and refinement round"}
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = None
-    if kp["launches"]:
+    if kp["launches"] and kc["launches"]:
         avg_s = kp["ms"] / kp["launches"] / 1e3
-        bytes_per_launch = kp["bytes"] / kp["launches"]
+        bytes_per_launch = kc["bytes"] / kc["launches"]
         achieved = bytes_per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -253,9 +323,11 @@ def main() -> int:
         dev.check(lib.hgp_memcpy_dtod(dev.ctx, C.c_void_p(b), C.c_void_p(a), COPY_CAL_BYTES), "hgp_memcpy_dtod")
         dev.free(a)
         dev.free(b)
-        print(f"# copy calibration: {COPY_CAL_BYTES} bytes copied device to device", file=sys.stderr, flush=True)
+        log(f"# copy calibration: {COPY_CAL_BYTES} bytes copied device to device")
 
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None if args.no_cpu else cpu_baseline(lib, dev)
+    dev.close()
+    serving = None if args.no_serve else serve_bench(args.jobs, args.max_pins, args.workers, args.load, args.seed)
     out = {
         "metric": "pins/s end-to-end at equal λ−1 (vs plain & CPU); p50/p99 job latency batched",
         "value": value, "unit": "pins/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -263,15 +335,19 @@ def main() -> int:
         "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"netlist {n} nodes / {e} nets / {p} pins (power law alpha {ALPHA} on [2,{MAX_SIZE}], "
                                f"radius {RADIUS}, {int(FRAC_LOCAL * 100)}% local), -k {K} {EPS}",
-                   "boundary": "device-resident CSR -> host partition vector", "smh": 0,
-                   "candidates": prm.candidates_count, "refine_repeats": prm.refine_repeats,
+                   "variant": "core: device-resident CSR -> host partition vector (end_to_end holds the command "
+                              "line from process start)",
+                   "smh": 0, "candidates": prm.candidates_count, "refine_repeats": prm.refine_repeats,
                    "levels": st.levels, "lambda_minus_1": km1, "valid": bool(valid == 1),
-                   "core_ms_p50": float(np.median(core_ms)), "ip_ms": st.ip_ms},
+                   "core_ms_p50": float(np.median(core_ms)), "ip_ms_p50": float(np.median(ip_ms)),
+                   "initial_prune": prm.initial_prune},
+        "pipeline": pipeline,
         "roofline": roofline,
+        "end_to_end": e2e,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out))
-    dev.close()
     return 0
 
 

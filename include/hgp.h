@@ -54,7 +54,7 @@ typedef struct hgp_params {
   int32_t mode;                     /* HGP_MODE_* */
   uint32_t max_nodes_per_part;      /* Omega */
   uint32_t max_inbound_per_part;    /* Delta */
-  uint32_t max_parts;
+  uint32_t max_parts;               /* in KWAY mode it must equal kway (HGP_E_INVALID otherwise) */
   uint32_t kway;                    /* -k k   (KWAY only) */
   float epsilon;                    /* -k eps (KWAY only) */
   uint32_t candidates_count;        /* -cnc, 1..16 (default 16) */
@@ -309,10 +309,15 @@ hgp_status hgp_partition(hgp_ctx* ctx, const hgp_params* params, const hgp_hgrap
  * device time, the launch count and the launches' algorithmic bytes (0 for kernels without
  * a byte model), and clears the record. */
 hgp_status hgp_profile_kernel(hgp_ctx* ctx, const char* kernel_name);
+/* on != 0: the profiled kernel's launches also count their algorithmic bytes on the device,
+ * by a model that adds device work (for a separate, untimed run); 0 (default): timing only. */
+hgp_status hgp_profile_count(hgp_ctx* ctx, int32_t on);
 hgp_status hgp_profile_read(hgp_ctx* ctx, double* total_ms, uint64_t* launches, uint64_t* algorithmic_bytes);
 
 /* Same pipeline on a hypergraph already resident in HBM (sources first, rule-2
- * deduplicated as hgp_hgraph_load leaves it).  The inputs are only read. */
+ * deduplicated as hgp_hgraph_load leaves it).  The inputs are only read.  Contract:
+ * d_hedges_offsets[0] == 0, offsets non-decreasing, every pin < num_nodes; the first and
+ * the last are checked (HGP_E_INVALID), the order of the offsets is the caller's to keep. */
 hgp_status hgp_partition_device(hgp_ctx* ctx, const hgp_params* params, uint32_t num_nodes, uint32_t num_hedges,
                                 const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
                                 const uint32_t* d_srcs_count, const float* d_hedge_weights,

@@ -967,6 +967,8 @@ std::vector<uint64_t> prune_weights(uint32_t n, uint32_t e, const uint64_t* hoff
   return out;
 }
 
+int g_ip_threads = 0;  // 1: the initial partitioner runs on one thread; 0: its default
+
 IpInput ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w, bool prune) {
   IpInput in;
   std::vector<uint64_t> pw;
@@ -1290,6 +1292,8 @@ int orc_ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* p
   return 0;
 }
 
+void orc_set_ip_threads(int threads) { g_ip_threads = threads; }
+
 int orc_relabel(uint32_t n, uint32_t* part, uint32_t* new_k) {
   *new_k = relabel(n, part);
   return 0;
@@ -1316,7 +1320,9 @@ int orc_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const ui
       for (uint64_t i = h.off[x]; i < h.off[x + 1]; ++i) h.inc[at[h.pins[i]]++] = (uint32_t)x;
   }
   try {
-    hgp_ip::partition(h, k, eps, omega, part, psizes);
+    hgp_ip::Options opt;
+    if (g_ip_threads == 1) opt.parallel_depth = 0;  // every try and half on the calling thread
+    hgp_ip::partition(h, k, eps, omega, part, psizes, opt);
   } catch (const hgp_ip::Infeasible&) {
     return 6;  // HGP_E_INFEASIBLE
   } catch (const std::exception&) {

@@ -89,6 +89,10 @@ int orc_merge_small_partitions(uint32_t n, uint32_t k, const uint32_t* psizes, c
 
 int orc_relabel(uint32_t n, uint32_t* part, uint32_t* new_k);
 
+/* 1: the built-in initial partitioner runs every try and half on the calling thread;
+ * 0 (default): it uses up to 16 threads.  The partition is the same either way. */
+void orc_set_ip_threads(int threads);
+
 /* Chaining alone (sources/chaining.cu:19-254): ranks plus the links behind them, per node:
  * predecessor (UINT32_MAX for a head), component head and position along the chain. */
 int orc_chain_links(uint32_t n, const uint32_t* part, const uint32_t* moves, const int64_t* gains,

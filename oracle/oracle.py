@@ -77,6 +77,7 @@ def ref():
         _ref.ref_free.argtypes = [C.c_void_p]
         _ref.ref_metrics.argtypes = [C.c_void_p, u32p, f64p, f64p, f64p]
         _ref.ref_validity.argtypes = [C.c_void_p, u32p, C.c_uint32, C.c_uint32, C.c_uint32]
+        _ref.ref_save.argtypes = [C.c_void_p, u32p, C.c_char_p, C.c_int]
     return _ref
 
 
@@ -139,6 +140,19 @@ def ref_validity(path, part, omega, delta, max_parts):
     v = R.ref_validity(h, P(part, u32p), omega, delta, max_parts)
     R.ref_free(h)
     return v
+
+
+def ref_save(path_in, part, path_out, which=0):
+    """The reference's -s writer (which=0: quotient hypergraph in the format of path_out's
+    extension) or -p writer (which=1) on the hypergraph loaded from path_in and deduplicated,
+    as the reference's main saves it (sources/main.cu:125,919,933)."""
+    R = ref()
+    h = R.ref_load(path_in.encode(), 1)
+    assert h
+    part = np.ascontiguousarray(part, np.uint32)
+    rc = R.ref_save(h, P(part, u32p), path_out.encode(), which)
+    R.ref_free(h)
+    return rc
 
 
 def dedup(hg: HG) -> HG:
@@ -322,6 +336,11 @@ def kway_omega(n, k, eps):
     """(uint32)ceil((1 + eps) * (float)n / k) in float32, as sources/runconfig.cpp:191."""
     v = (np.float32(1) + np.float32(eps)) * np.float32(n) / np.float32(k)
     return int(np.ceil(np.float32(v)))
+
+
+def set_ip_threads(threads):
+    """1: the initial partitioner stays on one thread; 0: its default (up to 16 threads)."""
+    lib().orc_set_ip_threads(int(threads))
 
 
 def chain_links(part, moves, gains, sizes):

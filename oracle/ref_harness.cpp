@@ -6,6 +6,7 @@
 // reference loaders, rule-2 dedup, quotient-hypergraph metrics and validity check so that
 // tests can pin the oracle's restatement of those functions against the real thing.
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,31 @@ int ref_metrics(void* h, const uint32_t* part, double* km1, double* cutnet, doub
     *soed = hg->soedFromPart(p);
     return 0;
   } catch (...) {
+    return -1;
+  }
+}
+
+// The -s and -p outputs (sources/runconfig.cpp:219-247): getPartitionsHypergraph(part,2,true)
+// saved by savehMETIS / saveSNN / saveAXH as the path's extension says (which = 0), or the
+// partition file by savePartitioning (which = 1).  Returns 0 on success, -1 on a thrown error.
+int ref_save(void* h, const uint32_t* part, const char* path, int which) {
+  try {
+    HyperGraph* hg = (HyperGraph*)h;
+    std::vector<uint32_t> p(part, part + hg->nodes());
+    std::string s(path);
+    if (which == 1) {
+      HyperGraph::savePartitioning(s, p);
+      return 0;
+    }
+    HyperGraph q = hg->getPartitionsHypergraph(p, 2, true);
+    const std::string ext = s.size() > 4 ? s.substr(s.size() - 4) : "";
+    if (ext == ".hgr") q.savehMETIS(s);
+    else if (ext == ".snn") q.saveSNN(s);
+    else if (ext == ".axh") q.saveAXH(s);
+    else return -1;
+    return 0;
+  } catch (const std::exception& x) {
+    fprintf(stderr, "ref_save: %s\n", x.what());
     return -1;
   }
 }

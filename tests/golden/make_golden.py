@@ -126,6 +126,30 @@ def main():
     data["grader_hgr_fmt1"] = {"part_file": "hgr_fmt1.part", "args": ["-k-prt", "3", "0.5"],
                                "connectivity": float(lines["Connectivity"]), "cut_net": float(lines["Cut-net"]),
                                "soed": float(lines["SOED"])}
+    # the reference's -s writers (quotient hypergraph by getPartitionsHypergraph(part,2,true)) in
+    # all three formats, and its -p writer, for one seeded partition of every input format
+    saved = {}
+    for name in ("hgr_fmt1", "hgr_fmt10", "snn", "axh"):
+        entry = data[name]
+        # ids made sequential: the reference's quotient builder requires 0..N-1 (includes/hgraph.hpp:292-295)
+        part = np.unique(np.array(entry["partitions"][2]["part"], np.uint32), return_inverse=True)[1].astype(np.uint32)
+        src = os.path.join(HERE, entry["file"])
+        outs = {}
+        for ext in (".hgr", ".snn", ".axh"):
+            out = os.path.join(HERE, "saved", f"{name}.quotient{ext}")
+            os.makedirs(os.path.dirname(out), exist_ok=True)
+            if O.ref_save(src, part, out, 0) == 0:
+                outs[ext] = os.path.relpath(out, HERE)
+            else:  # the reference refuses this output (the CLI exits 1): saveAXH takes no hedge without
+                # sources (includes/hgraph.hpp:866-867), and no quotient hedge may lack destinations (:145)
+                if os.path.exists(out):
+                    os.remove(out)
+                outs[ext] = None
+        pout = os.path.join(HERE, "saved", f"{name}.part")
+        assert O.ref_save(src, part, pout, 1) == 0
+        outs["part"] = os.path.relpath(pout, HERE)
+        saved[name] = {"part": [int(x) for x in part], "files": outs}
+    data["saved"] = saved
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(data, f, indent=0)
     print("wrote", len(data), "fixture entries")

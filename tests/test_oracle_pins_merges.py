@@ -289,3 +289,18 @@ def test_kway_partition_rejects_max_parts_other_than_k():
     with pytest.raises(O.OracleError) as ex:
         O.partition(g, 1, O.kway_omega(g.n, 4, 0.03), 2**31 - 1, 8, kway=4, eps=0.03)
     assert ex.value.status == 1
+
+
+def test_initial_partitioner_is_independent_of_threads():
+    """The same partition on one thread and on up to 16 (tries and halves in parallel)."""
+    rng = seeded("ip-threads")
+    g = O.dedup(O.HG(*gen.random_hypergraph(20000, 24000, rng, max_size=12, weights="int")))
+    sizes = np.ones(g.n, np.uint32)
+    omega = O.kway_omega(g.n, 16, 0.03)
+    try:
+        O.set_ip_threads(1)
+        one, ps1 = O.initial_partition(g, sizes, 16, 0.03, omega, prune=False)
+    finally:
+        O.set_ip_threads(0)
+    many, ps2 = O.initial_partition(g, sizes, 16, 0.03, omega, prune=False)
+    assert np.array_equal(one, many) and np.array_equal(ps1, ps2)
