@@ -27,7 +27,7 @@ _i32p = C.POINTER(C.c_int32)
 _vp = C.c_void_p
 
 
-HGP_IP_HOST, HGP_IP_DEVICE = 0, 1
+HGP_IP_HOST, HGP_IP_DEVICE, HGP_IP_AUTO = 0, 1, 2
 
 
 class Params(C.Structure):
@@ -47,6 +47,11 @@ class Params(C.Structure):
         ("verbose", C.c_int32),
         ("initial_partitioner", C.c_int32),
         ("initial_prune", C.c_int32),
+        ("lp_rounds", C.c_uint32),
+        ("ip_starts", C.c_uint32),
+        ("ip_rounds", C.c_uint32),
+        ("ip_nodes_per_part", C.c_uint32),
+        ("ip_refine_repeats", C.c_uint32),
     ]
 
 
@@ -124,6 +129,8 @@ SIGNATURES = [
     ("hgp_relabel_partitions", C.c_int32, [_vp, C.c_uint32, _vp, _u32p]),
     ("hgp_inbound_sizes", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp]),
     ("hgp_connectivity", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, _i64p]),
+    ("hgp_refine_lp", C.c_int32, [_vp, C.POINTER(Params), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32,
+                                  C.c_uint32, _vp, _vp]),
     ("hgp_initial_partition", C.c_int32, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, _vp, C.c_uint32,
                                           C.c_float, C.c_uint32, C.c_int32, C.POINTER(_vp), C.POINTER(_vp)]),
     ("hgp_ip_input", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_int32, _u32p, _u64p, _u64p,

@@ -126,8 +126,9 @@ void hgp_default_params(hgp_params* p) {
   p->device_touching_construction = 0;
   p->initial_partitions_merge = 0;
   p->verbose = 0;
-  p->initial_partitioner = HGP_IP_HOST;
-  p->initial_prune = 1;
+  p->initial_partitioner = HGP_IP_AUTO;
+  p->initial_prune = -1;  // the reference's pruning when Mt-KaHyPar runs, none otherwise
+  p->lp_rounds = 0;
 }
 
 hgp_status hgp_ctx_create(int device, uint64_t hbm_budget, hgp_ctx** out) {
@@ -510,6 +511,22 @@ hgp_status hgp_refine(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_
   });
 }
 
+hgp_status hgp_refine_lp(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_hedges,
+                         const uint64_t* d_hedges_offsets, const uint32_t* d_srcs_count, const uint32_t* d_touching,
+                         const uint64_t* d_touching_offsets, const uint32_t* d_inbound_count,
+                         const float* d_hedge_weights, const uint32_t* d_nodes_sizes, uint32_t num_nodes,
+                         uint32_t num_hedges, uint32_t num_partitions, uint32_t* d_partitions,
+                         uint32_t* d_partitions_sizes) {
+  if (!params || !d_partitions || !d_partitions_sizes || num_partitions == 0) return HGP_E_INVALID;
+  if (params->mode != HGP_MODE_KWAY) return HGP_E_INVALID;  // sizes only: no inbound-set checks
+  return guard(ctx, [&](Ctx& c) {
+    hgp::RefineLevel L = make_level(num_nodes, num_hedges, num_partitions, d_hedges, d_hedges_offsets, d_srcs_count,
+                                    d_touching, d_touching_offsets, d_inbound_count, d_hedge_weights, d_nodes_sizes);
+    hgp::lp_refine(c, L, params->max_nodes_per_part, params->lp_rounds, d_partitions, d_partitions_sizes);
+    hgp::sync(c);
+  });
+}
+
 hgp_status hgp_refine_analyze(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_hedges,
                               const uint64_t* d_hedges_offsets, const uint32_t* d_srcs_count, const uint32_t* d_touching,
                               const uint64_t* d_touching_offsets, const uint32_t* d_inbound_count,
@@ -594,7 +611,7 @@ hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const u
   if (!d_partitions || !d_partitions_sizes || k == 0) return HGP_E_INVALID;
   return guard(ctx, [&](Ctx& c) {
     hgp::initial_partition(c, d_hedges, d_hedges_offsets, d_hedge_weights, num_nodes, num_hedges, d_nodes_sizes, k,
-                           epsilon, max_nodes_per_part, prune != 0, *d_partitions, *d_partitions_sizes, nullptr);
+                           epsilon, max_nodes_per_part, prune, *d_partitions, *d_partitions_sizes, nullptr);
   });
 }
 
@@ -606,7 +623,7 @@ hgp_status hgp_ip_input(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* 
   return guard(ctx, [&](Ctx& c) {
     hgp_ip::Hypergraph h;
     hgp::ip_input_device(c, d_hedges, d_hedges_offsets, d_hedge_weights, d_nodes_sizes, num_nodes, num_hedges,
-                         prune != 0, h);
+                         (uint32_t)std::max(prune, 0), h);
     *num_merged = (uint32_t)(h.off.size() - 1);
     *merged_pins = h.pins.size();
     if (h_offsets) std::copy(h.off.begin(), h.off.end(), h_offsets);

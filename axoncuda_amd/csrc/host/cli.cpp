@@ -35,7 +35,8 @@ void print_help() {
       "  -smh <lvl>  Recover device memory by migrating to the host the pre-coarsening hypergraph up the provided level\n"
       "  -dtc        When set, construct touching sets on the device, rather than on the host (faster - uses more device memory)\n"
       "  -ipm        When set, initial partitions are greedily merged to reduce their number (and save memory - minor quality loss)\n"
-      "  -ip <where> k-way initial partitioner without mtkahypar: 'host' (recursive bisection, default) or 'device' (label propagation)\n"
+      "  -ip <where> k-way initial partitioner without mtkahypar: 'auto' (default: 'device' on levels of 2^21 pins or more), 'device' (multistart label propagation) or 'host' (recursive bisection)\n"
+      "  -lp <num>   k-way: label-propagation rounds before each level's FM rounds (default 0)\n"
       "  -v <lvl>    Set the verbosity level: 0 metrics only, 1 steps and phases, 2 kernel launches, 3  \n"
       "  -h          Show this help message\n");
 }
@@ -142,11 +143,15 @@ int main(int argc, char** argv) {
     } else if (a == "-ipm") {
       prm.initial_partitions_merge = 1;
     } else if (a == "-ip") {
-      if (i + 1 >= argc) { std::fprintf(stderr, "Error: -ip requires 'host' or 'device'\n"); return 1; }
+      if (i + 1 >= argc) { std::fprintf(stderr, "Error: -ip requires 'auto', 'host' or 'device'\n"); return 1; }
       const std::string w = argv[++i];
       if (w == "host") prm.initial_partitioner = HGP_IP_HOST;
       else if (w == "device") prm.initial_partitioner = HGP_IP_DEVICE;
-      else { std::fprintf(stderr, "Error: -ip requires 'host' or 'device'\n"); return 1; }
+      else if (w == "auto") prm.initial_partitioner = HGP_IP_AUTO;
+      else { std::fprintf(stderr, "Error: -ip requires 'auto', 'host' or 'device'\n"); return 1; }
+    } else if (a == "-lp") {
+      if (i + 1 >= argc) { std::fprintf(stderr, "Error: -lp requires a positive integer value\n"); return 1; }
+      prm.lp_rounds = (uint32_t)stoul_or_die(argv[++i], "-lp");
     } else if (a == "-v") {
       if (i + 1 >= argc) { std::fprintf(stderr, "Error: -v requires a positive value between 0 and 3\n"); return 1; }
       verbosity = (int)stoul_or_die(argv[++i], "-v");

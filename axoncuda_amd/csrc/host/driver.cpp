@@ -83,14 +83,16 @@ bool run_mtkahypar(const std::string& bin, const hgp_ip::Hypergraph& h, uint32_t
 }  // namespace
 
 void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                       const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, bool prune, uint32_t*& part_out,
-                       uint32_t*& psizes_out, double* ms, const RefineLevel* device_level) {
+                       const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, int32_t prune_arg,
+                       uint32_t*& part_out, uint32_t*& psizes_out, double* ms, const RefineLevel* device_level,
+                       const DeviceIpOptions& dip) {
   auto t0 = std::chrono::high_resolution_clock::now();
   static const std::string mtk = find_mtkahypar();
-  if (device_level && mtk.empty()) {  // device label-propagation partitioner (opt-in)
+  const uint32_t prune = prune_arg >= 0 ? (uint32_t)prune_arg : (mtk.empty() ? 0u : 1u);
+  if (device_level && mtk.empty()) {  // device multistart partitioner
     RefineLevel RL = *device_level;
     RL.k = k;
-    initial_partition_device(c, RL, omega, DEVICE_IP_STARTS, DEVICE_IP_ROUNDS, prune, part_out, psizes_out);
+    initial_partition_device(c, RL, omega, dip, prune, part_out, psizes_out);
     if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
     return;
   }
@@ -174,6 +176,11 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
   // (sources/runconfig.cpp:190-192), so any other value is a caller error.
   HGP_REQUIRE(!kw || max_parts == prm.kway, HGP_E_INVALID, "k-way mode needs max_parts == kway");
   const uint32_t cnc = prm.candidates_count;
+  DeviceIpOptions dip;
+  if (prm.ip_starts) dip.starts = prm.ip_starts;
+  if (prm.ip_rounds) dip.rounds = prm.ip_rounds;
+  if (prm.ip_nodes_per_part) dip.nodes_per_part = prm.ip_nodes_per_part;
+  if (prm.ip_refine_repeats) dip.refine_repeats = prm.ip_refine_repeats;
   const uint32_t target_parts = std::min(max_parts, (uint32_t)(((uint64_t)n0 + omega - 1) / omega));
   hgp_run_stats st{};
   st.total_pins = P0;
@@ -233,11 +240,13 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
     if (c.verbose) fprintf(stderr, "Coarsening level %u, remaining nodes=%u, pins=%llu\n", depth, n, (unsigned long long)L.P);
     const RefineLevel ipl{L.n, E, prm.kway, L.hedges.get(), L.hoff.get(), L.srcc.get(), L.tch.get(), L.toff.get(),
                           L.inb.get(), w.get(), L.sizes.get()};
-    const RefineLevel* ip_dev = prm.initial_partitioner == HGP_IP_DEVICE ? &ipl : nullptr;
+    const bool dev_ip = prm.initial_partitioner == HGP_IP_DEVICE ||
+                        (prm.initial_partitioner == HGP_IP_AUTO && L.P >= HGP_IP_AUTO_DEVICE_PINS);
+    const RefineLevel* ip_dev = dev_ip ? &ipl : nullptr;
     if (kw && n < 8192) {  // KWAY_INIT_UPPER_THREASHOLD (headers/defines.cuh:35)
       uint32_t *pp, *ps;
       initial_partition(c, L.hedges.get(), L.hoff.get(), w.get(), n, E, L.sizes.get(), prm.kway, prm.epsilon, omega,
-                        prm.initial_prune != 0, pp, ps, &st.ip_ms, ip_dev);
+                        prm.initial_prune, pp, ps, &st.ip_ms, ip_dev, dip);
       part = DBuf<uint32_t>();
       part.ctx = &c, part.ptr = pp, part.n = n;
       psizes.ctx = &c, psizes.ptr = ps, psizes.n = prm.kway;
@@ -261,7 +270,7 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
     if (kw && (ratio > 0.95f || nn < 1024)) {  // KWAY_INIT_SHRINK_RATIO_LIMIT / LOWER_THREASHOLD
       uint32_t *pp, *ps;
       initial_partition(c, L.hedges.get(), L.hoff.get(), w.get(), n, E, L.sizes.get(), prm.kway, prm.epsilon, omega,
-                        prm.initial_prune != 0, pp, ps, &st.ip_ms, ip_dev);
+                        prm.initial_prune, pp, ps, &st.ip_ms, ip_dev, dip);
       part.ctx = &c, part.ptr = pp, part.n = n;
       psizes.ctx = &c, psizes.ptr = ps, psizes.n = prm.kway;
       K = prm.kway;
@@ -333,6 +342,7 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
     RefineLevel RL{L.n, E, K, L.hedges.get(), L.hoff.get(), L.srcc.get(), L.tch.get(), L.toff.get(), L.inb.get(),
                    w.get(), L.sizes.get()};
     if (c.verbose) fprintf(stderr, "Uncoarsening level %d, nodes=%u\n", d, L.n);
+    if (kw && prm.lp_rounds) lp_refine(c, RL, omega, prm.lp_rounds, part.get(), psizes.get());
     refine(c, RL, prm.mode, omega, delta, prm.refine_repeats, d == 0, part.get(), psizes.get(), pinb.get());
     repeats_total += prm.refine_repeats;
     bytes += prm.refine_repeats * (4.0 * L.P + 12.0 * E + 4.0 * L.P + 8.0 * L.n + 4.0 * L.n + 16.0 * L.n);

@@ -7,22 +7,14 @@
 
 namespace hgp {
 
-// Starts and Jacobi rounds of the device initial partitioner (HGP_IP_DEVICE).
-#ifndef HGP_DIP_STARTS
-#define HGP_DIP_STARTS 2
-#endif
-#ifndef HGP_DIP_ROUNDS
-#define HGP_DIP_ROUNDS 8
-#endif
-constexpr uint32_t DEVICE_IP_STARTS = HGP_DIP_STARTS;
-constexpr uint32_t DEVICE_IP_ROUNDS = HGP_DIP_ROUNDS;
-
 // k-way initial partition of a coarse level: Mt-KaHyPar when installed, else the host
 // recursive bisection, or the device partitioner when `device_level` (the same level with
-// its touching lists) is given.
+// its touching lists) is given.  prune: the keep-target multiplier of the hedge pruning,
+// or -1 for the reference's pruning before Mt-KaHyPar and none before the built-in ones.
 void initial_partition(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                       const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, bool prune, uint32_t*& part_out,
-                       uint32_t*& psizes_out, double* ms, const RefineLevel* device_level = nullptr);
+                       const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, int32_t prune, uint32_t*& part_out,
+                       uint32_t*& psizes_out, double* ms, const RefineLevel* device_level = nullptr,
+                       const DeviceIpOptions& dip = DeviceIpOptions());
 
 // Level-0 input: either a host model or device-resident arrays (copied into the level).
 struct PipelineInput {

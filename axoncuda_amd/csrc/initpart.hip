@@ -24,15 +24,6 @@ namespace hgp {
 
 namespace {
 
-#ifndef HGP_DIP_PER_PART
-#define HGP_DIP_PER_PART 16
-#endif
-#ifndef HGP_DIP_REPEATS
-#define HGP_DIP_REPEATS 4
-#endif
-constexpr uint32_t IP_COARSE_PER_PART = HGP_DIP_PER_PART;  // stop coarsening at about this many nodes per part
-constexpr uint32_t IP_REFINE_REPEATS = HGP_DIP_REPEATS;    // k-way refinement rounds per projected level
-
 // Best-fit decreasing: every node goes to the fullest bin it fits in (ties: lower bin); a
 // node that fits no bin goes to the lightest one.
 void bfd_pack(const std::vector<uint32_t>& sizes, uint32_t k, uint32_t omega, uint64_t seed, std::vector<uint32_t>& part,
@@ -93,12 +84,12 @@ void multistart_lp(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t starts
 
 // The level's hedges are merged first (identical pin sets summed, single pins dropped: the
 // cut is the same), then the level is coarsened further with the pipeline's own stages
-// (candidates, pairing, contraction, merging again) down to about IP_COARSE_PER_PART nodes
+// (candidates, pairing, contraction, merging again) down to about opt.nodes_per_part nodes
 // per part; the multistart packing and label propagation run there, and the result is
 // projected back level by level with the pipeline's k-way refinement.  The given level
 // itself is left to the caller's refinement.
-void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uint32_t starts, uint32_t rounds,
-                              bool prune, uint32_t*& part_out, uint32_t*& psizes_out) {
+void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, const DeviceIpOptions& opt,
+                              uint32_t prune, uint32_t*& part_out, uint32_t*& psizes_out) {
   struct Mini {
     RefineLevel L;
     DBuf<uint32_t> hedges, srcc, tch, inb, sizes, groups;
@@ -110,11 +101,11 @@ void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uin
   auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
   const auto t0 = clk::now();
   const uint32_t k = std::max<uint32_t>(L0.k, 1);
-  const uint32_t target = std::max<uint32_t>(IP_COARSE_PER_PART * k, 256);
+  const uint32_t target = std::max<uint32_t>(std::max<uint32_t>(opt.nodes_per_part, 1) * k, 256);
   const uint32_t cnc = 16, delta = 0x7FFFFFFFu;
   // a merged level of n nodes from (hedges, hoff, w) with e hedges; takes ownership of sizes
   auto make_level = [&](const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                        bool prune_level, DBuf<uint32_t>&& sizes) {
+                        uint32_t prune_level, DBuf<uint32_t>&& sizes) {
     auto M = std::make_unique<Mini>();
     uint32_t me = 0, *mh;
     uint64_t* mo;
@@ -168,7 +159,7 @@ void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uin
     dch.ctx = &c, dch.ptr = ch, dch.n = std::max<uint64_t>(cs, 1);
     dco.ctx = &c, dco.ptr = co, dco.n = (uint64_t)L.e + 1;
     dcc.ctx = &c, dcc.ptr = cc, dcc.n = std::max<uint32_t>(L.e, 1);
-    auto N = make_level(ch, co, L.w, nn, L.e, false, std::move(gsizes));
+    auto N = make_level(ch, co, L.w, nn, L.e, 0u, std::move(gsizes));
     M.groups = std::move(groups);
     lv.push_back(std::move(N));
     if (c.verbose >= 2)
@@ -176,7 +167,7 @@ void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uin
               lv.back()->L.e, (unsigned long long)lv.back()->P, ms_since(t0));
   }
   DBuf<uint32_t> part, ps;
-  multistart_lp(c, lv.back()->L, omega, starts, rounds, part, ps);
+  multistart_lp(c, lv.back()->L, omega, opt.starts, opt.rounds, part, ps);
   if (c.verbose >= 2) {
     sync(c);
     fprintf(stderr, "  device initial partition: label propagation done (%.1f ms)\n", ms_since(t0));
@@ -187,7 +178,7 @@ void initial_partition_device(Ctx& c, const RefineLevel& L0, uint32_t omega, uin
     uncoarsen(c, lv[d]->groups.get(), part.get(), L.n, fine.get());
     part = std::move(fine);
     lv[d + 1].reset();
-    refine(c, L, HGP_MODE_KWAY, omega, delta, IP_REFINE_REPEATS, false, part.get(), ps.get(), nullptr);
+    refine(c, L, HGP_MODE_KWAY, omega, delta, opt.refine_repeats, false, part.get(), ps.get(), nullptr);
     if (c.verbose >= 2) {
       sync(c);
       fprintf(stderr, "  device initial partition: projected to %u nodes (%.1f ms)\n", L.n, ms_since(t0));

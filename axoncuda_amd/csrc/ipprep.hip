@@ -11,13 +11,14 @@
 // Before the merge, hedges are pruned as the reference prunes them (armonic_degree_score_kernel
 // and prune_hedges_kernel, kernels/init_part.cu:221-281, launched at sources/init_part.cu:749-817):
 // a hedge's score is its weight over the sum of its pins' inverse degrees, the expected number
-// of kept hedges is N * floor(P / E), and kept weights are divided by the keep probability.
+// of kept hedges is f * N * floor(P / E) (f = 1 is the reference's target; 0 turns pruning
+// off), and kept weights are divided by the keep probability.
 // The reference draws with cuRAND Philox in float; this build uses exact integers, a
 // splitmix64 draw per hedge and three IEEE double operations, so that the device and the
 // oracle keep the same hedges with the same weights:
 //   inv_e   = sum_{v in e} floor(2^24 / deg(v))                       (u64)
 //   ratio_e = min(floor(W_e * 2^24 / inv_e), floor((2^64 - 1) / E))    (u64; 0 when inv_e = 0)
-//   S = sum_e ratio_e,  T = N * floor(P / E)
+//   S = sum_e ratio_e,  T = f * N * floor(P / E)
 //   p_e  = S == 0 ? 1 : min(1, (double)T * (double)ratio_e / (double)S)
 //   keep = (double)(splitmix64(86 ^ e * 0x9E3779B97F4A7C15) >> 11) < p_e * 2^53
 //   W'_e = keep ? min((uint64)((double)W_e / p_e), 2^40) : 0
@@ -247,7 +248,7 @@ struct Merged {
 // Keep draws and scaled weights of the reference's hedge pruning (see the header): pw[e],
 // 0 for a dropped hedge.
 void prune_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                  uint64_t p, DBuf<unsigned long long>& pw) {
+                  uint64_t p, uint32_t mult, DBuf<unsigned long long>& pw) {
   DBuf<uint32_t> deg(c, n);
   memset_async(c, deg.get(), 0, (uint64_t)n * 4);
   HGP_LAUNCH(k_ipp_deg, ceil_div(p, 256), 256, 0, c, hedges, p, deg.get());
@@ -259,18 +260,18 @@ void prune_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const fl
   copy_d2h(c, part, sums.get(), sizeof part);
   uint64_t sum = 0;
   for (auto v : part) sum += v;
-  const uint64_t t = (uint64_t)n * (p / e);
+  const uint64_t t = (uint64_t)mult * n * (p / e);
   pw = DBuf<unsigned long long>(c, e);
   HGP_LAUNCH(k_ipp_prune, ceil_div(e, 256), 256, 0, c, w, ratio.get(), e, (double)t, (double)sum, pw.get());
 }
 
 void merge_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                  bool prune, Merged& out) {
+                  uint32_t prune, Merged& out) {
   out = Merged();
   if (e == 0 || n == 0) return;
   const uint64_t p = read_scalar(c, hoff + e);
   DBuf<unsigned long long> pw;
-  if (prune) prune_hedges(c, hedges, hoff, w, n, e, p, pw);
+  if (prune) prune_hedges(c, hedges, hoff, w, n, e, p, prune, pw);
   const int pb = std::max(1, bit_width(n - 1));
   const int eb = std::max(1, bit_width(e - 1));
   HGP_REQUIRE(pb + eb <= 64, HGP_E_CAPACITY, "hedge merge: hedge and node ids exceed 64 key bits");
@@ -328,7 +329,7 @@ void merge_hedges(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const fl
 }  // namespace
 
 void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                        bool prune, uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out) {
+                        uint32_t prune, uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out) {
   Merged M;
   merge_hedges(c, hedges, hoff, w, n, e, prune, M);
   e_out = M.u;
@@ -345,7 +346,7 @@ void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, co
 }
 
 void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
-                     uint32_t n, uint32_t e, bool prune, hgp_ip::Hypergraph& h) {
+                     uint32_t n, uint32_t e, uint32_t prune, hgp_ip::Hypergraph& h) {
   h = hgp_ip::Hypergraph();
   h.n = n;
   h.vw.resize(n);

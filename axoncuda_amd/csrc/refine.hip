@@ -2567,6 +2567,25 @@ uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rou
   return read_scalar(c, applied.get());
 }
 
+void lp_refine(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rounds, uint32_t* part, uint32_t* psizes) {
+  if (rounds == 0 || L.n == 0) return;
+  DBuf<uint32_t> sp(c, L.n), ss(c, std::max<uint32_t>(L.k, 1));
+  int64_t cur = connectivity_fixed(c, L.hedges, L.hoff, L.w, part, L.e);
+  for (uint32_t r = 0; r < rounds; ++r) {
+    copy_d2d(c, sp.get(), part, (uint64_t)L.n * 4);
+    copy_d2d(c, ss.get(), psizes, (uint64_t)L.k * 4);
+    if (jacobi_round(c, L, omega, r, part, psizes) == 0) break;
+    const int64_t now = connectivity_fixed(c, L.hedges, L.hoff, L.w, part, L.e);
+    if (c.verbose >= 2) fprintf(stderr, "  lp n=%u round=%u km1 %lld -> %lld\n", L.n, r, (long long)cur, (long long)now);
+    if (now > cur) {  // undo the round
+      copy_d2d(c, part, sp.get(), (uint64_t)L.n * 4);
+      copy_d2d(c, psizes, ss.get(), (uint64_t)L.k * 4);
+      break;
+    }
+    cur = now;
+  }
+}
+
 void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* part,
                    uint32_t e, uint32_t k, uint32_t* out) {
   RefineLevel L{};

@@ -58,28 +58,39 @@ uint32_t refine_analyze(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, 
                         const AnalyzeOut& out);
 // Device k-way initial partition of a level (best-fit-decreasing starts, Jacobi rounds, the
 // start with the lowest lambda-1 kept): allocates part[n] and psizes[k].
-void initial_partition_device(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t starts, uint32_t rounds,
-                              bool prune, uint32_t*& part_out, uint32_t*& psizes_out);
+struct DeviceIpOptions {
+  uint32_t starts = 2;           // best-fit-decreasing starts
+  uint32_t rounds = 8;           // Jacobi rounds per start (fewer when a round applies nothing)
+  uint32_t nodes_per_part = 16;  // coarsen the level further to about this many nodes per part
+  uint32_t refine_repeats = 4;   // k-way refinement rounds per projected level
+};
+void initial_partition_device(Ctx& c, const RefineLevel& L, uint32_t omega, const DeviceIpOptions& opt,
+                              uint32_t prune, uint32_t*& part_out, uint32_t*& psizes_out);
 // One Jacobi label-propagation round (k-way): every node proposes its best move, positive-
 // gain moves are sorted by (target, gain desc, node) and applied while they fit next to the
 // target's size before the round.  Returns the number of moves applied.
 uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t round, uint32_t* part, uint32_t* psizes);
+// Label-propagation refinement (k-way): up to `rounds` Jacobi rounds; a round that applies
+// nothing ends them, a round that raises lambda-1 is undone and ends them.
+void lp_refine(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rounds, uint32_t* part, uint32_t* psizes);
 void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* part,
                    uint32_t e, uint32_t k, uint32_t* out);
 int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* part,
                            uint32_t e);
 
+
 // ---- initial partitioner input (ipprep.hip) ------------------------------------------------
 // The coarsest level as the initial partitioner takes it (hedges of >= 2 distinct pins,
 // sorted, identical sets merged in order of first appearance with summed 2^18 weights,
-// incidence lists), built on the device.  With `prune`, hedges are first dropped or kept
-// with rescaled weights by the reference's degree-score pruning (sources/init_part.cu:749-817).
+// incidence lists), built on the device.  With `prune` = f > 0, hedges are first dropped or
+// kept with rescaled weights by the reference's degree-score pruning (sources/init_part.cu:749-817)
+// aiming at f times its target of N * floor(P / E) kept hedges.
 void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* sizes,
-                     uint32_t n, uint32_t e, bool prune, hgp_ip::Hypergraph& h);
+                     uint32_t n, uint32_t e, uint32_t prune, hgp_ip::Hypergraph& h);
 // The same merge left on the device as a level (sorted pins, no sources): e_out hedges with
 // float weights = summed fixed-point weights / 2^18.
 void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
-                        bool prune, uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out);
+                        uint32_t prune, uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out);
 
 // ---- post-processing (postproc.hip) -----------------------------------------------------
 void greedy_merge_groups(Ctx& c, const uint32_t* sizes, const uint32_t* inb, uint32_t n, uint32_t ngroups,

@@ -97,7 +97,7 @@ def end_to_end(hg: H.Hypergraph, tmpdir: str) -> dict:
         raise RuntimeError(f"hgraph_gpu failed ({r.returncode}): {r.stderr[-2000:]}")
     km1 = None
     for line in r.stdout.splitlines():
-        if line.startswith("Connectivity:"):
+        if line.strip().startswith("Connectivity:"):
             km1 = float(line.split()[1])
     log(f"# end to end: {wall:.2f} s, lambda-1 {km1}")
     return {"value": p / wall, "unit": "pins/s", "wall_s": wall, "lambda_minus_1": km1,

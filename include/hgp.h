@@ -65,16 +65,34 @@ typedef struct hgp_params {
   int32_t initial_partitions_merge;     /* -ipm */
   int32_t verbose;                  /* -v 0..3 */
   /* k-way initial partitioner when Mt-KaHyPar is not installed: HGP_IP_HOST (recursive
-   * bisection on the host, the default) or HGP_IP_DEVICE (best-fit-decreasing starts
-   * improved by label propagation on the device).  No counterpart in the reference. */
+   * bisection on the host), HGP_IP_DEVICE (the level coarsened further on the device,
+   * best-fit-decreasing starts improved by label propagation, projected back with k-way
+   * refinement) or HGP_IP_AUTO (the default: the device partitioner when the level holds at
+   * least HGP_IP_AUTO_DEVICE_PINS pins, where the bisection's host time grows, the bisection
+   * below).  No counterpart in the reference. */
   int32_t initial_partitioner;
-  /* 1 (default): before the k-way initial partition, drop hedges by the reference's
-   * degree-score pruning and rescale the kept weights (sources/init_part.cu:749-817);
-   * 0: hand every hedge to the initial partitioner. */
+  /* f > 0: before the k-way initial partition, drop hedges by the reference's degree-score
+   * pruning and rescale the kept weights (sources/init_part.cu:749-817), aiming at f times
+   * the reference's target of N * floor(P / E) kept hedges (f = 1 is the reference's rule);
+   * 0: hand every hedge to the initial partitioner; -1 (default): 1 when Mt-KaHyPar runs
+   * (the reference's behaviour), 0 for the built-in partitioners. */
   int32_t initial_prune;
+  /* k-way only: up to this many Jacobi label-propagation rounds before each level's FM
+   * rounds (0, the default: none).  A round that applies nothing ends them; a round that
+   * raises lambda-1 is undone and ends them.  The reference's only label propagation is its
+   * legacy initial partitioner (kernels/init_part.cu:89-151). */
+  uint32_t lp_rounds;
+  /* The device initial partitioner (HGP_IP_DEVICE): BFD starts, Jacobi rounds per start,
+   * coarsening target in nodes per part, and k-way refinement rounds per projected level
+   * (0 in any of them: the built-in default). */
+  uint32_t ip_starts;
+  uint32_t ip_rounds;
+  uint32_t ip_nodes_per_part;
+  uint32_t ip_refine_repeats;
 } hgp_params;
 
-enum { HGP_IP_HOST = 0, HGP_IP_DEVICE = 1 };
+enum { HGP_IP_HOST = 0, HGP_IP_DEVICE = 1, HGP_IP_AUTO = 2 };
+#define HGP_IP_AUTO_DEVICE_PINS 2097152u
 
 /* Per-run statistics filled by hgp_partition. */
 typedef struct hgp_run_stats {
@@ -229,6 +247,20 @@ hgp_status hgp_refine(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_
                       uint32_t* d_partitions, uint32_t* d_partitions_sizes,
                       uint32_t* d_partitions_inbound_sizes);
 
+/* Label-propagation refinement, k-way (modelled on the reference's legacy Jacobi rounds,
+ * kernels/init_part.cu:89-151): up to params->lp_rounds rounds in which every node proposes
+ * its best strictly-fitting move (tie hash mix32(q + round)) and the positive proposals are
+ * applied by (target, gain desc, node) while each target's size before the round plus what
+ * entered it fits max_nodes_per_part; a round that applies nothing ends them, and a round
+ * that raises lambda-1 is undone and ends them.  Updates d_partitions and
+ * d_partitions_sizes in place. */
+hgp_status hgp_refine_lp(hgp_ctx* ctx, const hgp_params* params, const uint32_t* d_hedges,
+                         const uint64_t* d_hedges_offsets, const uint32_t* d_srcs_count,
+                         const uint32_t* d_touching, const uint64_t* d_touching_offsets,
+                         const uint32_t* d_inbound_count, const float* d_hedge_weights,
+                         const uint32_t* d_nodes_sizes, uint32_t num_nodes, uint32_t num_hedges,
+                         uint32_t num_partitions, uint32_t* d_partitions, uint32_t* d_partitions_sizes);
+
 /* One analysis round of refinementRepeats without applying it, exposing every
  * intermediate (host outputs, each num_nodes long):
  *   moves[v], gains[v]   : fm_refinement_gains_kernel (kernels/refinement.cu:82-209),
@@ -275,10 +307,12 @@ hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64
                             const float* d_hedge_weights, const uint32_t* d_partitions, uint32_t num_hedges,
                             int64_t* km1_fixed);
 
+
 /* initial_partitioning_kahypar (sources/init_part.cu:733-953): k-way initial
- * partition of a coarse level.  With `prune`, hedges are first pruned as the reference
- * prunes them (armonic_degree_score_kernel + prune_hedges_kernel, kernels/init_part.cu:221-281)
- * with a splitmix64 draw per hedge in place of cuRAND Philox.  Runs `mtkahypar` when it is
+ * partition of a coarse level.  With `prune` = f > 0, hedges are first pruned as the
+ * reference prunes them (armonic_degree_score_kernel + prune_hedges_kernel,
+ * kernels/init_part.cu:221-281) with a splitmix64 draw per hedge in place of cuRAND Philox,
+ * aiming at f * N * floor(P / E) kept hedges (f = 1: the reference's target).  Runs `mtkahypar` when it is
  * on PATH (same command line as the reference); otherwise the built-in recursive-bisection
  * partitioner.  Allocates d_partitions[num_nodes] and d_partitions_sizes[k]. */
 hgp_status hgp_initial_partition(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,

@@ -548,17 +548,16 @@ void chain_ranks(uint32_t n, const uint32_t* part, const std::vector<uint32_t>& 
   }
 }
 
-void analyze(const Level& L, int mode, uint32_t omega, uint32_t delta, uint32_t R, uint32_t k, uint32_t fm_repeat,
-             bool chainup, const uint32_t* part, const uint32_t* psizes, Analysis& A) {
+// gains (fm_refinement_gains_kernel, kernels/refinement.cu:82-209) in 2^18 fixed point:
+// gain(q) = saving - sum_{e: c_e(q)=0} W_e; best q != p with psizes[q] + s - floor(s /
+// discount) <= omega, by (gain, mix32(q + repeat)); moves[v] = NONE when no q fits.
+void best_moves(const Level& L, bool incc, uint32_t omega, uint32_t discount, uint32_t k, uint32_t fm_repeat,
+                const uint32_t* part, const uint32_t* psizes,
+                const std::vector<std::unordered_map<uint32_t, uint32_t>>& cnt, std::vector<uint32_t>& moves,
+                std::vector<int64_t>& gains) {
   const uint32_t n = L.n;
-  const bool incc = mode == 0;
-  const uint32_t discount = discount_of(fm_repeat, R);
-  std::vector<std::unordered_map<uint32_t, uint32_t>> cnt, dcnt;
-  hedge_counts(L, part, cnt, &dcnt);
-  A.moves.assign(n, NONE);
-  A.gains.assign(n, 0);
-  // gains (fm_refinement_gains_kernel, kernels/refinement.cu:82-209) in 2^18 fixed point:
-  // gain(q) = saving - sum_{e: c_e(q)=0} W_e; best valid q by (gain, mix32(q + repeat)).
+  moves.assign(n, NONE);
+  gains.assign(n, 0);
   std::vector<int64_t> conn(k, 0);
   std::vector<uint32_t> touched;
   for (uint32_t v = 0; v < n; ++v) {
@@ -596,9 +595,19 @@ void analyze(const Level& L, int mode, uint32_t omega, uint32_t delta, uint32_t 
       }
     }
     for (uint32_t q : touched) conn[q] = 0;
-    A.moves[v] = bq;
-    A.gains[v] = bq == NONE ? 0 : bg;
+    moves[v] = bq;
+    gains[v] = bq == NONE ? 0 : bg;
   }
+}
+
+void analyze(const Level& L, int mode, uint32_t omega, uint32_t delta, uint32_t R, uint32_t k, uint32_t fm_repeat,
+             bool chainup, const uint32_t* part, const uint32_t* psizes, Analysis& A) {
+  const uint32_t n = L.n;
+  const bool incc = mode == 0;
+  const uint32_t discount = discount_of(fm_repeat, R);
+  std::vector<std::unordered_map<uint32_t, uint32_t>> cnt, dcnt;
+  hedge_counts(L, part, cnt, &dcnt);
+  best_moves(L, incc, omega, discount, k, fm_repeat, part, psizes, cnt, A.moves, A.gains);
   // ranks
   if (chainup) {
     std::vector<uint32_t> order(n);
@@ -728,6 +737,76 @@ void analyze(const Level& L, int mode, uint32_t omega, uint32_t delta, uint32_t 
       continue;
     }
     if (A.seq[r] > A.seq[b]) A.best = r;
+  }
+}
+
+// One Jacobi label-propagation round, k-way (the reference's legacy "jacobi" step:
+// fm_refinement_gains_kernel then jacobi_apply_moves_kernel, sources/init_part.cu:380-474,
+// kernels/init_part.cu:89-151).  Every node proposes its best move with strict sizes
+// (discount none) and tie hash mix32(q + round); proposals of positive gain are taken in
+// (target asc, gain desc, node asc) order, and a move applies when the target's size before
+// the round plus everything proposed into it up to and including this move fits omega
+// (moves leaving the target are not counted on; once one fails, the rest of that target's
+// list fails too).  The reference applies every proposal and breaks ties with coin flips;
+// this rule is deterministic and keeps the sizes.  Returns the number of moves applied.
+uint32_t jacobi_round(const Level& L, uint32_t omega, uint32_t k, uint32_t round, uint32_t* part, uint32_t* psizes) {
+  std::vector<std::unordered_map<uint32_t, uint32_t>> cnt;
+  hedge_counts(L, part, cnt, nullptr);
+  std::vector<uint32_t> moves;
+  std::vector<int64_t> gains;
+  best_moves(L, false, omega, NONE, k, round, part, psizes, cnt, moves, gains);
+  std::vector<uint32_t> movers;
+  for (uint32_t v = 0; v < L.n; ++v)
+    if (moves[v] != NONE && gains[v] > 0) movers.push_back(v);
+  std::sort(movers.begin(), movers.end(), [&](uint32_t a, uint32_t b) {
+    if (moves[a] != moves[b]) return moves[a] < moves[b];
+    if (gains[a] != gains[b]) return gains[a] > gains[b];
+    return a < b;
+  });
+  const std::vector<uint32_t> before(psizes, psizes + k);
+  std::vector<uint64_t> entering(k, 0);
+  uint32_t applied = 0;
+  for (uint32_t v : movers) {
+    const uint32_t q = moves[v], s = L.sizes[v];
+    entering[q] += s;
+    if ((uint64_t)before[q] + entering[q] > omega) continue;
+    psizes[part[v]] -= s;
+    psizes[q] += s;
+    part[v] = q;
+    ++applied;
+  }
+  return applied;
+}
+
+int64_t km1_fixed(const Level& L, const uint32_t* part) {
+  int64_t total = 0;
+  std::unordered_set<uint32_t> seen;
+  for (uint32_t e = 0; e < L.E; ++e) {
+    seen.clear();
+    for (uint64_t i = L.hoff[e]; i < L.hoff[e + 1]; ++i) seen.insert(part[L.pins[i]]);
+    if (!seen.empty()) total += (int64_t)L.W[e] * (int64_t)(seen.size() - 1);
+  }
+  return total;
+}
+
+// Label-propagation refinement (k-way, optional, before a level's FM rounds): up to
+// `rounds` Jacobi rounds; a round that applies nothing ends it, and a round that raises
+// lambda-1 is undone and ends it.
+void lp_refine(const Level& L, uint32_t omega, uint32_t k, uint32_t rounds, uint32_t* part, uint32_t* psizes) {
+  if (rounds == 0) return;
+  int64_t cur = km1_fixed(L, part);
+  std::vector<uint32_t> sp, ss;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    sp.assign(part, part + L.n);
+    ss.assign(psizes, psizes + k);
+    if (jacobi_round(L, omega, k, r, part, psizes) == 0) break;
+    const int64_t now = km1_fixed(L, part);
+    if (now > cur) {
+      std::copy(sp.begin(), sp.end(), part);
+      std::copy(ss.begin(), ss.end(), psizes);
+      break;
+    }
+    cur = now;
   }
 }
 
@@ -918,11 +997,11 @@ void build_level(Level& L, uint32_t n, uint32_t E, const uint64_t* hoff, const u
 
 // The initial partitioner's input (sources/init_part.cu:749-888).  Pruning
 // (armonic_degree_score_kernel and prune_hedges_kernel, kernels/init_part.cu:221-281): a
-// hedge's score is its weight over the sum of its pins' inverse degrees, about N * floor(P/E)
-// hedges are kept, kept weights are divided by the keep probability.  The reference draws
+// hedge's score is its weight over the sum of its pins' inverse degrees, about f * N * floor(P/E)
+// hedges are kept (f = 1 is the reference's target), kept weights are divided by the keep probability.  The reference draws
 // with cuRAND Philox in float; the rule restated here is the build's exact one:
 //   inv_e = sum_{v in e} floor(2^24 / deg(v)); ratio_e = min(floor(W_e 2^24 / inv_e), floor((2^64-1)/E))
-//   p_e = S ? min(1, T * ratio_e / S) : 1 in IEEE double, T = N floor(P/E), S = sum ratio_e
+//   p_e = S ? min(1, T * ratio_e / S) : 1 in IEEE double, T = f N floor(P/E), S = sum ratio_e
 //   keep iff (splitmix64(86 ^ e * 0x9E3779B97F4A7C15) >> 11) < p_e * 2^53; W'_e = min(floor(W_e / p_e), 2^40)
 // Then the file writer's filter (:822-847): hedges of >= 2 pins, pins sorted, identical
 // pin sets merged with their weights summed (kept here in order of first appearance).
@@ -939,7 +1018,8 @@ uint64_t splitmix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
-std::vector<uint64_t> prune_weights(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w) {
+std::vector<uint64_t> prune_weights(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
+                                    uint32_t mult) {
   std::vector<uint64_t> out(e, 0);
   if (e == 0) return out;
   const uint64_t P = hoff[e] - hoff[0];
@@ -954,7 +1034,7 @@ std::vector<uint64_t> prune_weights(uint32_t n, uint32_t e, const uint64_t* hoff
     if (inv) ratio[x] = std::min<uint64_t>(((uint64_t)fixed_weight(w[x]) << 24) / inv, cap);
     S += ratio[x];
   }
-  const double T = (double)((uint64_t)n * (P / e)), Sd = (double)S;
+  const double T = (double)((uint64_t)mult * n * (P / e)), Sd = (double)S;
   for (uint32_t x = 0; x < e; ++x) {
     volatile double prod = T * (double)ratio[x];  // rounded on its own, as on the device
     const double p = S == 0 ? 1.0 : std::min(1.0, prod / Sd);
@@ -969,10 +1049,10 @@ std::vector<uint64_t> prune_weights(uint32_t n, uint32_t e, const uint64_t* hoff
 
 int g_ip_threads = 0;  // 1: the initial partitioner runs on one thread; 0: its default
 
-IpInput ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w, bool prune) {
+IpInput ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w, uint32_t prune) {
   IpInput in;
   std::vector<uint64_t> pw;
-  if (prune) pw = prune_weights(n, e, hoff, pins, w);
+  if (prune) pw = prune_weights(n, e, hoff, pins, w, prune);
   std::map<std::vector<uint32_t>, uint32_t> first;
   for (uint32_t x = 0; x < e; ++x) {
     if (prune && pw[x] == 0) continue;
@@ -992,6 +1072,132 @@ IpInput ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* p
     in.w.push_back(wx);
   }
   return in;
+}
+
+// Best-fit-decreasing start of the device initial partitioner (modelled on the reference's
+// legacy host BFD packing, sources/init_part.cu:37-180): the nodes in a seeded shuffle
+// (Fisher-Yates over a splitmix64 stream started at seed * 0x9E3779B97F4A7C15 + 1), stably
+// sorted by size descending; each goes to the fullest bin it fits in (ties: lower bin), or to
+// the lightest bin (ties: lower) when it fits none.
+void bfd_start(const std::vector<uint32_t>& sizes, uint32_t k, uint32_t omega, uint64_t seed, std::vector<uint32_t>& part,
+               std::vector<uint32_t>& loads) {
+  const uint32_t n = (uint32_t)sizes.size();
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+  for (uint32_t i = n; i >= 2; --i) {
+    st = splitmix64(st);
+    std::swap(order[i - 1], order[st % i]);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sizes[a] > sizes[b]; });
+  part.assign(n, 0);
+  loads.assign(k, 0);
+  for (uint32_t v : order) {
+    uint32_t pick = NONE;
+    for (uint32_t b = 0; b < k; ++b) {
+      if ((uint64_t)loads[b] + sizes[v] > omega) continue;
+      if (pick == NONE || loads[b] > loads[pick]) pick = b;
+    }
+    if (pick == NONE) {
+      pick = 0;
+      for (uint32_t b = 1; b < k; ++b)
+        if (loads[b] < loads[pick]) pick = b;
+    }
+    part[v] = pick;
+    loads[pick] += sizes[v];
+  }
+}
+
+// The device initial partitioner (k-way), restated: the level's hedges merged (identical
+// pin sets summed, fewer than two pins dropped, optional pruning), then coarsened further by
+// the pipeline's own stages (candidates with cnc 16 and no inbound limit, pairing, coarse
+// hedges, merged again) while it has more than max(16 k, 256) nodes, fewer than 32 levels
+// and the pairing shrinks it below 95%; on the coarsest merged level, `starts` BFD starts each
+// improved by up to `rounds` Jacobi rounds (until one applies nothing), the start of lowest
+// lambda-1 kept (the first on ties); then projected back level by level with `repeats` k-way
+// refinement rounds each.  The coarsening target is max(per_part * k, 256) nodes.  Merged weights are the summed 2^18 fixed-point weights over 2^18
+// as float.
+struct MiniLevel {
+  uint32_t n = 0, E = 0;
+  std::vector<uint64_t> hoff, toff;
+  std::vector<uint32_t> pins, srcc, tch, inb, sizes, groups;
+  std::vector<float> w;
+};
+
+void mini_level(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w, uint32_t prune,
+                std::vector<uint32_t> sizes, MiniLevel& M) {
+  IpInput in = ip_input(n, e, hoff, pins, w, prune);
+  M.n = n;
+  M.E = (uint32_t)in.w.size();
+  M.hoff = std::move(in.off);
+  M.pins = std::move(in.pins);
+  M.srcc.assign(M.E, 0);
+  M.w.resize(M.E);
+  for (uint32_t x = 0; x < M.E; ++x) M.w[x] = (float)((double)in.w[x] / 262144.0);
+  M.sizes = std::move(sizes);
+  touching(n, M.E, M.hoff.data(), M.pins.data(), M.srcc.data(), M.toff, M.tch, M.inb);
+}
+
+void device_ip(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
+               const uint32_t* sizes, uint32_t k, uint32_t omega, uint32_t prune, uint32_t starts, uint32_t rounds,
+               uint32_t per_part, uint32_t repeats, uint32_t* part_out, uint32_t* psizes_out) {
+  k = std::max<uint32_t>(k, 1);
+  const uint32_t target = std::max<uint32_t>(std::max<uint32_t>(per_part, 1) * k, 256), cnc = 16,
+                 delta = 0x7FFFFFFFu;
+  std::vector<MiniLevel> lv(1);
+  mini_level(n, e, hoff, pins, w, prune, std::vector<uint32_t>(sizes, sizes + n), lv[0]);
+  while (lv.back().n > target && lv.size() < 32) {
+    MiniLevel& M = lv.back();
+    std::vector<uint32_t> pairs((uint64_t)M.n * cnc), scores((uint64_t)M.n * cnc);
+    orc_candidates(M.n, M.E, M.hoff.data(), M.pins.data(), M.srcc.data(), M.toff.data(), M.tch.data(), M.inb.data(),
+                   M.w.data(), M.sizes.data(), cnc, omega, delta, pairs.data(), scores.data());
+    uint32_t nn = 0;
+    std::vector<uint32_t> groups(M.n), gs(M.n), ung(M.n);
+    std::vector<uint64_t> uoff(M.n + 1);
+    orc_group_nodes(M.n, M.inb.data(), pairs.data(), scores.data(), M.sizes.data(), cnc, omega, delta, &nn,
+                    groups.data(), gs.data(), ung.data(), uoff.data(), nullptr);
+    if ((float)nn / (float)M.n > 0.95f) break;
+    gs.resize(nn);
+    std::vector<uint64_t> co;
+    std::vector<uint32_t> cp, cs;
+    coarsen_hedges(M.E, M.hoff.data(), M.pins.data(), M.srcc.data(), groups.data(), co, cp, cs);
+    M.groups = std::move(groups);
+    MiniLevel N;
+    mini_level(nn, M.E, co.data(), cp.data(), M.w.data(), 0, std::move(gs), N);
+    lv.push_back(std::move(N));
+  }
+  std::vector<uint32_t> part, psizes;
+  {
+    const MiniLevel& M = lv.back();
+    Level L;
+    build_level(L, M.n, M.E, M.hoff.data(), M.pins.data(), M.srcc.data(), M.toff.data(), M.tch.data(), M.inb.data(),
+                M.w.data(), M.sizes.data());
+    int64_t best = INT64_MAX;
+    std::vector<uint32_t> sp, ss;
+    for (uint32_t s = 0; s < std::max<uint32_t>(starts, 1); ++s) {
+      bfd_start(M.sizes, k, omega, s, sp, ss);
+      for (uint32_t r = 0; r < rounds; ++r)
+        if (jacobi_round(L, omega, k, r, sp.data(), ss.data()) == 0) break;
+      const int64_t km1 = km1_fixed(L, sp.data());
+      if (km1 < best) {
+        best = km1;
+        part = sp;
+        psizes = ss;
+      }
+    }
+  }
+  for (size_t d = lv.size() - 1; d-- > 0;) {
+    const MiniLevel& M = lv[d];
+    std::vector<uint32_t> fine(M.n);
+    for (uint32_t v = 0; v < M.n; ++v) fine[v] = part[M.groups[v]];
+    part.swap(fine);
+    Level L;
+    build_level(L, M.n, M.E, M.hoff.data(), M.pins.data(), M.srcc.data(), M.toff.data(), M.tch.data(), M.inb.data(),
+                M.w.data(), M.sizes.data());
+    refine(L, 1, omega, delta, repeats, k, false, part.data(), psizes.data(), nullptr);
+  }
+  std::copy(part.begin(), part.end(), part_out);
+  std::copy(psizes.begin(), psizes.end(), psizes_out);
 }
 
 }  // namespace
@@ -1281,7 +1487,7 @@ int orc_chain_links(uint32_t n, const uint32_t* part, const uint32_t* moves, con
 int orc_ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
                  const uint32_t* sizes, int prune, uint32_t* e_out, uint64_t* p_out, uint64_t* off_out,
                  uint32_t* pins_out, uint64_t* w_out, uint64_t* vw_out) {
-  IpInput in = ip_input(n, e, hoff, pins, w, prune != 0);
+  IpInput in = ip_input(n, e, hoff, pins, w, (uint32_t)std::max(prune, 0));
   *e_out = (uint32_t)(in.off.size() - 1);
   *p_out = in.pins.size();
   if (off_out) std::copy(in.off.begin(), in.off.end(), off_out);
@@ -1294,6 +1500,44 @@ int orc_ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* p
 
 void orc_set_ip_threads(int threads) { g_ip_threads = threads; }
 
+int orc_jacobi_round(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const uint64_t* toff,
+                     const uint32_t* tch, const uint32_t* inb, const float* w, const uint32_t* sizes, uint32_t k,
+                     uint32_t omega, uint32_t round, uint32_t* part, uint32_t* psizes, uint32_t* applied) {
+  std::vector<uint32_t> srcc(e, 0);
+  Level L;
+  build_level(L, n, e, hoff, pins, srcc.data(), toff, tch, inb, w, sizes);
+  *applied = jacobi_round(L, omega, k, round, part, psizes);
+  return 0;
+}
+
+int orc_lp_refine(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const uint64_t* toff,
+                  const uint32_t* tch, const uint32_t* inb, const float* w, const uint32_t* sizes, uint32_t k,
+                  uint32_t omega, uint32_t rounds, uint32_t* part, uint32_t* psizes) {
+  std::vector<uint32_t> srcc(e, 0);
+  Level L;
+  build_level(L, n, e, hoff, pins, srcc.data(), toff, tch, inb, w, sizes);
+  lp_refine(L, omega, k, rounds, part, psizes);
+  return 0;
+}
+
+int orc_bfd_start(uint32_t n, const uint32_t* sizes, uint32_t k, uint32_t omega, uint64_t seed, uint32_t* part,
+                  uint32_t* loads) {
+  std::vector<uint32_t> s(sizes, sizes + n), p, l;
+  bfd_start(s, k, omega, seed, p, l);
+  std::copy(p.begin(), p.end(), part);
+  std::copy(l.begin(), l.end(), loads);
+  return 0;
+}
+
+int orc_device_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
+                                 const uint32_t* sizes, uint32_t k, uint32_t omega, int prune, uint32_t starts,
+                                 uint32_t rounds, uint32_t per_part, uint32_t repeats, uint32_t* part,
+                                 uint32_t* psizes) {
+  device_ip(n, e, hoff, pins, w, sizes, k, omega, (uint32_t)std::max(prune, 0), starts, rounds, per_part, repeats, part,
+            psizes);
+  return 0;
+}
+
 int orc_relabel(uint32_t n, uint32_t* part, uint32_t* new_k) {
   *new_k = relabel(n, part);
   return 0;
@@ -1302,7 +1546,7 @@ int orc_relabel(uint32_t n, uint32_t* part, uint32_t* new_k) {
 int orc_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
                           const uint32_t* sizes, uint32_t k, float eps, uint32_t omega, int prune, uint32_t* part,
                           uint32_t* psizes) {
-  IpInput in = ip_input(n, e, hoff, pins, w, prune != 0);
+  IpInput in = ip_input(n, e, hoff, pins, w, (uint32_t)std::max(prune, 0));
   hgp_ip::Hypergraph h;
   h.n = n;
   h.off = std::move(in.off);
@@ -1335,7 +1579,8 @@ int orc_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const ui
 // (KWAY :358-391 and :448-485, INCC :495-564), uncoarsening and refinement on the way
 // up, INCC small-partition merge and final zero-based relabel (:803-830).
 int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, uint32_t kway, float eps, uint32_t cnc,
-                  uint32_t refine_repeats, int ipm, int prune, uint32_t n0, uint32_t E, const uint64_t* hoff0,
+                  uint32_t refine_repeats, int ipm, int prune, int ip_device, const uint32_t* ip_options,
+                  uint32_t lp_rounds, uint32_t n0, uint32_t E, const uint64_t* hoff0,
                   const uint32_t* pins0, const uint32_t* srcc0, const float* w, uint32_t* part_out,
                   uint32_t* num_levels) {
   const bool kw = mode == 1;
@@ -1357,6 +1602,19 @@ int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, 
   lv[0].sizes.assign(n0, 1u);
   touching(n0, E, lv[0].hoff.data(), lv[0].pins.data(), lv[0].srcc.data(), lv[0].toff, lv[0].tch, lv[0].inb);
   std::vector<uint32_t> part, psizes, pinb;
+  uint32_t ip_opt[4] = {2, 8, 16, 4};
+  for (int i = 0; i < 4; ++i)
+    if (ip_options && ip_options[i]) ip_opt[i] = ip_options[i];
+  // the k-way base case: the device partitioner's restatement (with the product's default
+  // options: 2 starts, 8 rounds, 16 nodes per part, 4 refinement rounds) or the bisection
+  auto base_ip = [&](uint32_t n, const uint64_t* hoff, const uint32_t* pins, const uint32_t* sizes) -> int {
+    if (ip_device == 1 || (ip_device == 2 && hoff[E] >= 2097152u)) {  // device, or auto on big levels
+      device_ip(n, E, hoff, pins, w, sizes, kway, omega, (uint32_t)std::max(prune, 0), ip_opt[0], ip_opt[1],
+                ip_opt[2], ip_opt[3], part.data(), psizes.data());
+      return 0;
+    }
+    return orc_initial_partition(n, E, hoff, pins, w, sizes, kway, eps, omega, prune, part.data(), psizes.data());
+  };
   uint32_t K = 0;
   bool base_refined_level0 = false;
   uint32_t depth = 0;
@@ -1366,8 +1624,7 @@ int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, 
     if (kw && n < 8192) {  // KWAY base case before coarsening
       part.assign(n, 0);
       psizes.assign(kway, 0);
-      if (int rc = orc_initial_partition(n, E, L.hoff.data(), L.pins.data(), w, L.sizes.data(), kway, eps, omega,
-                                         prune, part.data(), psizes.data()))
+      if (int rc = base_ip(n, L.hoff.data(), L.pins.data(), L.sizes.data()))
         return rc;
       K = kway;
       break;
@@ -1385,8 +1642,7 @@ int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, 
     if (kw && (ratio > 0.95f || nn < 1024)) {
       part.assign(n, 0);
       psizes.assign(kway, 0);
-      if (int rc = orc_initial_partition(n, E, L.hoff.data(), L.pins.data(), w, L.sizes.data(), kway, eps, omega,
-                                         prune, part.data(), psizes.data()))
+      if (int rc = base_ip(n, L.hoff.data(), L.pins.data(), L.sizes.data()))
         return rc;
       K = kway;
       break;
@@ -1418,6 +1674,7 @@ int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, 
     std::vector<float> wv(w, w + E);
     build_level(LL, L.n, E, L.hoff.data(), L.pins.data(), L.srcc.data(), L.toff.data(), L.tch.data(), L.inb.data(),
                 wv.data(), L.sizes.data());
+    if (kw) lp_refine(LL, omega, K, lp_rounds, part.data(), psizes.data());
     refine(LL, mode, omega, delta, refine_repeats, K, d == 0, part.data(), psizes.data(), pinb.data());
     if (d == 0) base_refined_level0 = true;
   }

@@ -106,14 +106,38 @@ int orc_ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* p
                  uint32_t* pins_out, uint64_t* w_out, uint64_t* vw_out);
 
 /* Built-in k-way initial partitioner on a level: the oracle's own input restatement (with
- * the hedge pruning when `prune`), partitioned by the product's recursive bisection. */
+ * the hedge pruning when `prune` = f > 0, aiming at f * N * floor(P / E) hedges), partitioned by the product's recursive bisection. */
 int orc_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins,
                           const float* w, const uint32_t* sizes, uint32_t k, float eps, uint32_t omega,
                           int prune, uint32_t* part, uint32_t* psizes);
 
-/* Whole pipeline (sources/main.cu:38-940) on a host CSR; part[n] out.  Returns 0 on success. */
+/* One Jacobi label-propagation round, k-way, on a level whose hedges are all destinations
+ * (the device initial partitioner's step and the optional LP refinement); *applied = moves. */
+int orc_jacobi_round(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const uint64_t* toff,
+                     const uint32_t* tch, const uint32_t* inb, const float* w, const uint32_t* sizes, uint32_t k,
+                     uint32_t omega, uint32_t round, uint32_t* part, uint32_t* psizes, uint32_t* applied);
+/* LP refinement: up to `rounds` Jacobi rounds, ending at a round that applies nothing or
+ * raises lambda-1 (undone). */
+int orc_lp_refine(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const uint64_t* toff,
+                  const uint32_t* tch, const uint32_t* inb, const float* w, const uint32_t* sizes, uint32_t k,
+                  uint32_t omega, uint32_t rounds, uint32_t* part, uint32_t* psizes);
+/* Best-fit-decreasing start of the device initial partitioner (seeded shuffle, sizes desc). */
+int orc_bfd_start(uint32_t n, const uint32_t* sizes, uint32_t k, uint32_t omega, uint64_t seed, uint32_t* part,
+                  uint32_t* loads);
+/* The device initial partitioner restated (merged level, further coarsening, multistart
+ * BFD + Jacobi, projection with k-way refinement). */
+int orc_device_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const float* w,
+                                 const uint32_t* sizes, uint32_t k, uint32_t omega, int prune, uint32_t starts,
+                                 uint32_t rounds, uint32_t per_part, uint32_t repeats, uint32_t* part,
+                                 uint32_t* psizes);
+
+/* Whole pipeline (sources/main.cu:38-940).  ip_device: 0 the bisection, 1 the device initial
+ * partitioner's restatement, 2 the device one on levels of at least 2^21 pins; ip_options: its (starts, rounds, nodes per part,
+ * refinement rounds), NULL or 0 entries for the defaults (2, 8, 16, 4); lp_rounds: Jacobi
+ * rounds before each level's FM rounds (k-way). on a host CSR; part[n] out.  Returns 0 on success. */
 int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, uint32_t kway, float eps,
-                  uint32_t cnc, uint32_t refine_repeats, int ipm, int prune, uint32_t n, uint32_t e,
+                  uint32_t cnc, uint32_t refine_repeats, int ipm, int prune, int ip_device,
+                  const uint32_t* ip_options, uint32_t lp_rounds, uint32_t n, uint32_t e,
                   const uint64_t* hoff, const uint32_t* pins, const uint32_t* srcc, const float* w,
                   uint32_t* part, uint32_t* num_levels);
 

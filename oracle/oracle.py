@@ -56,8 +56,16 @@ def lib():
         _lib.orc_initial_partition.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, f32p, u32p, C.c_uint32, C.c_float,
                                                C.c_uint32, C.c_int, u32p, u32p]
         _lib.orc_partition.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
-                                       C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p,
-                                       u32p, u32p]
+                                       C.c_uint32, C.c_int, C.c_int, C.c_int, u32p, C.c_uint32, C.c_uint32,
+                                       C.c_uint32, u64p, u32p, u32p, f32p, u32p, u32p]
+        _lib.orc_jacobi_round.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, u64p, u32p, u32p, f32p, u32p, C.c_uint32,
+                                          C.c_uint32, C.c_uint32, u32p, u32p, u32p]
+        _lib.orc_lp_refine.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, u64p, u32p, u32p, f32p, u32p, C.c_uint32,
+                                       C.c_uint32, C.c_uint32, u32p, u32p]
+        _lib.orc_bfd_start.argtypes = [C.c_uint32, u32p, C.c_uint32, C.c_uint32, C.c_uint64, u32p, u32p]
+        _lib.orc_device_initial_partition.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, f32p, u32p, C.c_uint32,
+                                                      C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                      C.c_uint32, u32p, u32p]
         _lib.orc_metrics.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p, u32p, f64p, f64p, f64p]
     return _lib
 
@@ -300,22 +308,71 @@ def initial_partition(hg: HG, sizes, k, eps, omega, prune=True):
     ps = np.zeros(k, np.uint32)
     sizes = np.ascontiguousarray(sizes, np.uint32)
     rc = lib().orc_initial_partition(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(sizes, u32p),
-                                     k, eps, omega, 1 if prune else 0, P(part, u32p), P(ps, u32p))
+                                     k, eps, omega, int(prune), P(part, u32p), P(ps, u32p))
     if rc:
         raise OracleError(rc, "orc_initial_partition")
     return part, ps
 
 
-def partition(hg: HG, mode, omega, delta, max_parts, kway=0, eps=0.0, cnc=16, repeats=16, ipm=False, prune=True):
+def partition(hg: HG, mode, omega, delta, max_parts, kway=0, eps=0.0, cnc=16, repeats=16, ipm=False, prune=0,
+              ip="auto", lp_rounds=0, ip_options=(0, 0, 0, 0)):
+    """Whole pipeline.  ip: "auto" (the product's default: the device multistart partitioner's
+    restatement on levels of at least 2^21 pins, the recursive bisection below), "device" or
+    "host"; prune: the keep-target
+    multiplier of the hedge pruning before a k-way initial partition (0: none); lp_rounds:
+    label-propagation rounds before each level's FM rounds (k-way); ip_options: the device
+    partitioner's (starts, rounds, nodes per part, refinement rounds), 0 = default."""
+    opts = np.ascontiguousarray(ip_options, np.uint32)
     part = np.zeros(hg.n, np.uint32)
     lv = C.c_uint32()
     rc = lib().orc_partition(mode, omega, delta, max_parts, kway, eps, cnc, repeats, 1 if ipm else 0,
-                             1 if prune else 0, hg.n, hg.e,
+                             int(prune), {"host": 0, "device": 1, "auto": 2}[ip], P(opts, u32p), lp_rounds, hg.n, hg.e,
                              P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(hg.w, f32p), P(part, u32p),
                              C.byref(lv))
     if rc:
         raise OracleError(rc, "orc_partition")
     return part, lv.value
+
+
+def jacobi_round(hg: HG, sizes, k, omega, rnd, part, psizes):
+    """One Jacobi round (all pins destinations): (part, psizes, applied) after it."""
+    toff, tch, inb = touching(hg)
+    part = np.ascontiguousarray(part, np.uint32).copy()
+    psizes = np.ascontiguousarray(psizes, np.uint32).copy()
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    applied = C.c_uint32()
+    lib().orc_jacobi_round(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(toff, u64p), P(tch, u32p), P(inb, u32p),
+                           P(hg.w, f32p), P(sizes, u32p), k, omega, rnd, P(part, u32p), P(psizes, u32p),
+                           C.byref(applied))
+    return part, psizes, applied.value
+
+
+def lp_refine(hg: HG, sizes, k, omega, rounds, part, psizes):
+    toff, tch, inb = touching(hg)
+    part = np.ascontiguousarray(part, np.uint32).copy()
+    psizes = np.ascontiguousarray(psizes, np.uint32).copy()
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    lib().orc_lp_refine(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(toff, u64p), P(tch, u32p), P(inb, u32p),
+                        P(hg.w, f32p), P(sizes, u32p), k, omega, rounds, P(part, u32p), P(psizes, u32p))
+    return part, psizes
+
+
+def bfd_start(sizes, k, omega, seed):
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    part = np.zeros(len(sizes), np.uint32)
+    loads = np.zeros(k, np.uint32)
+    lib().orc_bfd_start(len(sizes), P(sizes, u32p), k, omega, seed, P(part, u32p), P(loads, u32p))
+    return part, loads
+
+
+def device_initial_partition(hg: HG, sizes, k, omega, prune=0, starts=2, rounds=8, per_part=16, repeats=4):
+    part = np.zeros(hg.n, np.uint32)
+    ps = np.zeros(k, np.uint32)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    lib().orc_device_initial_partition(hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(sizes, u32p),
+                                       k, omega, int(prune), starts, rounds, per_part, repeats, P(part, u32p),
+                                       P(ps, u32p))
+    return part, ps
 
 
 def validity(hg: HG, part, omega, delta, max_parts):
@@ -361,7 +418,7 @@ def ip_input(hg: HG, sizes, prune=True):
     pruning, merge of identical sets): (offsets, sorted pins, summed 2^18 weights, node weights)."""
     sizes = np.ascontiguousarray(sizes, np.uint32)
     e_out, p_out = C.c_uint32(), C.c_uint64()
-    args = (hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(sizes, u32p), 1 if prune else 0,
+    args = (hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.w, f32p), P(sizes, u32p), int(prune),
             C.byref(e_out), C.byref(p_out))
     lib().orc_ip_input(*args, None, None, None, None)
     off = np.zeros(e_out.value + 1, np.uint64)

@@ -143,6 +143,20 @@ def refine(dev, L: Level, sizes, k, mode, omega, delta, repeats, final, part, ps
     return res
 
 
+def refine_lp(dev, L: Level, sizes, k, omega, rounds, part, psizes):
+    prm = params(H.HGP_MODE_KWAY, omega, 2**31 - 1, k, kway=k)
+    prm.lp_rounds = rounds
+    d_sizes = dev.upload(np.asarray(sizes, np.uint32))
+    d_part = dev.upload(np.asarray(part, np.uint32))
+    d_ps = dev.upload(np.asarray(psizes, np.uint32))
+    dev.check(dev.lib.hgp_refine_lp(dev.ctx, C.byref(prm), V(L.pins), V(L.off), V(L.srcc), V(L.tch), V(L.toff),
+                                    V(L.inb), V(L.w), V(d_sizes), L.n, L.e, k, V(d_part), V(d_ps)), "hgp_refine_lp")
+    res = dev.download(d_part, L.n, np.uint32), dev.download(d_ps, k, np.uint32)
+    for x in (d_sizes, d_part, d_ps):
+        dev.free(x)
+    return res
+
+
 def partition(dev, hgraph: H.Hypergraph, prm):
     n, _, _ = hgraph.sizes()
     part = np.zeros(n, np.uint32)

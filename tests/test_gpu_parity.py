@@ -395,7 +395,7 @@ def ip_input_hypergraph(n, e, max_size, copies, rng):
 
 def device_ip_input(dev, L, d_sizes, n, e, prune):
     m, p = C.c_uint32(), C.c_uint64()
-    args = (dev.ctx, G.V(L.pins), G.V(L.off), G.V(L.w), G.V(d_sizes), n, e, 1 if prune else 0, C.byref(m), C.byref(p))
+    args = (dev.ctx, G.V(L.pins), G.V(L.off), G.V(L.w), G.V(d_sizes), n, e, int(prune), C.byref(m), C.byref(p))
     dev.check(dev.lib.hgp_ip_input(*args, None, None, None), "ip input sizes")
     off = np.zeros(m.value + 1, np.uint64)
     pins = np.zeros(p.value, np.uint32)
@@ -406,7 +406,7 @@ def device_ip_input(dev, L, d_sizes, n, e, prune):
     return off, pins, w
 
 
-@pytest.mark.parametrize("prune", [False, True])
+@pytest.mark.parametrize("prune", [0, 1, 4])
 @pytest.mark.parametrize("n,e,max_size,copies", [(300, 2000, 5, 0.3), (5000, 40000, 12, 0.2), (64, 3000, 4, 0.6)])
 def test_ip_input_device_equals_oracle(dev, n, e, max_size, copies, prune, rng):
     """The initial partitioner's input built on the device (hedge pruning, sort, merge of

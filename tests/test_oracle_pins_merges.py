@@ -154,8 +154,9 @@ def splitmix64(x):
     return x ^ (x >> 31)
 
 
-def prune_rule(hs, w, n):
-    """Keep draw and scaled weight per hedge: 0 = dropped."""
+def prune_rule(hs, w, n, mult=1):
+    """Keep draw and scaled weight per hedge: 0 = dropped.  The keep target is mult times the
+    reference's N * floor(P / E)."""
     e = len(hs)
     P = sum(len(h) for h in hs)
     deg = [0] * n
@@ -169,7 +170,7 @@ def prune_rule(hs, w, n):
         ratio.append(min((fixed(w[x]) << 24) // inv, cap) if inv else 0)
     S = sum(ratio)
     assert S <= M64
-    T = n * (P // e)
+    T = mult * n * (P // e)
     out, probs = [], []
     for x in range(e):
         p = 1.0 if S == 0 else min(1.0, float(T) * float(ratio[x]) / float(S))
@@ -183,7 +184,7 @@ def prune_rule(hs, w, n):
 
 
 def ip_input_sets(hs, w, n, prune):
-    weights = prune_rule(hs, w, n)[0] if prune else [fixed(x) for x in w]
+    weights = prune_rule(hs, w, n, int(prune))[0] if prune else [fixed(x) for x in w]
     order, total = [], {}
     for x, h in enumerate(hs):
         if prune and weights[x] == 0:
@@ -202,7 +203,7 @@ def hedge_lists(g):
     return [[int(x) for x in g.pins[g.off[i]:g.off[i + 1]]] for i in range(g.e)]
 
 
-@pytest.mark.parametrize("prune", [False, True])
+@pytest.mark.parametrize("prune", [0, 1, 3])
 @pytest.mark.parametrize("seed,n,e", [(0, 40, 300), (1, 200, 2000), (2, 12, 400)])
 def test_ip_input_is_the_merged_set_family(seed, n, e, prune):
     rng = seeded(f"ipin{seed}")
@@ -229,18 +230,19 @@ def test_ip_input_is_the_merged_set_family(seed, n, e, prune):
     assert np.array_equal(gvw, sizes.astype(np.uint64))
 
 
-def test_pruning_keeps_about_the_target_and_preserves_total_weight():
-    """The reference's intent (kernels/init_part.cu:221-281): about N * floor(P / E) hedges are
-    kept, hedges whose probability reaches one always stay with their weight, and the scaled
-    weights estimate the total weight without bias."""
+@pytest.mark.parametrize("mult", [1, 4])
+def test_pruning_keeps_about_the_target_and_preserves_total_weight(mult):
+    """The reference's intent (kernels/init_part.cu:221-281): about mult * N * floor(P / E)
+    hedges are kept, hedges whose probability reaches one always stay with their weight, and
+    the scaled weights estimate the total weight without bias."""
     rng = seeded("prune-stats")
     n, e = 2000, 60000
     g = O.HG(*gen.random_hypergraph(n, e, rng, max_size=8, weights="float"))
     hs = hedge_lists(g)
-    out, probs, T = prune_rule(hs, g.w, n)
+    out, probs, T = prune_rule(hs, g.w, n, mult)
     kept = sum(1 for x in out if x)
     expect = sum(probs)
-    assert T == n * (len(g.pins) // e)
+    assert T == mult * n * (len(g.pins) // e)
     assert abs(kept - expect) < 5 * np.sqrt(expect)
     assert expect <= T * 1.0001
     W = [fixed(x) for x in g.w]
@@ -251,8 +253,8 @@ def test_pruning_keeps_about_the_target_and_preserves_total_weight():
     assert abs(est - tot) / tot < 0.05
     # the oracle applies the same rule
     sizes = np.ones(n, np.uint32)
-    goff, gpins, gw, _ = O.ip_input(g, sizes, prune=True)
-    sets, weights = ip_input_sets(hs, g.w, n, True)
+    goff, gpins, gw, _ = O.ip_input(g, sizes, prune=mult)
+    sets, weights = ip_input_sets(hs, g.w, n, mult)
     assert gw.tolist() == weights and len(goff) - 1 == len(sets)
 
 
