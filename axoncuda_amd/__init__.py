@@ -129,6 +129,7 @@ SIGNATURES = [
     ("hgp_relabel_partitions", C.c_int32, [_vp, C.c_uint32, _vp, _u32p]),
     ("hgp_inbound_sizes", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp]),
     ("hgp_connectivity", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, _i64p]),
+    ("hgp_connectivity_noalloc", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp]),
     ("hgp_refine_lp", C.c_int32, [_vp, C.POINTER(Params), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32,
                                   C.c_uint32, _vp, _vp]),
     ("hgp_initial_partition", C.c_int32, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, _vp, C.c_uint32,

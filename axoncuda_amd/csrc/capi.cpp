@@ -152,6 +152,13 @@ hgp_status hgp_ctx_create(int device, uint64_t hbm_budget, hgp_ctx** out) {
     return HGP_E_HIP;
   }
   ctx->c.own_stream = true;
+  try {
+    hgp::noalloc_setup(device);  // this device's no-allocation scratch (device is current)
+  } catch (const hgp::Error& e) {
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+    return e.status;
+  }
   hipMemPool_t pool;
   if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
     uint64_t thr = UINT64_MAX;
@@ -599,6 +606,17 @@ hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64
   if (!km1_fixed) return HGP_E_INVALID;
   return guard(ctx, [&](Ctx& c) {
     *km1_fixed = hgp::connectivity_fixed(c, d_hedges, d_hedges_offsets, d_hedge_weights, d_partitions, num_hedges);
+  });
+}
+
+hgp_status hgp_connectivity_noalloc(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                                    const float* d_hedge_weights, const uint32_t* d_partitions, uint32_t num_hedges,
+                                    uint32_t num_partitions, int64_t* d_km1_fixed) {
+  if (!d_km1_fixed || (num_hedges && (!d_hedges || !d_hedges_offsets || !d_hedge_weights || !d_partitions)))
+    return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    hgp::connectivity_noalloc(c, d_hedges, d_hedges_offsets, d_hedge_weights, d_partitions, num_hedges, num_partitions,
+                              d_km1_fixed);
   });
 }
 

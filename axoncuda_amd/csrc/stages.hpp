@@ -78,6 +78,15 @@ void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const u
 int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* part,
                            uint32_t e);
 
+// ---- lambda-1 without device allocation (noalloc.hip) -----------------------------------
+constexpr uint32_t NOALLOC_MAX_PARTS = 65536;  // one LDS bitmap of this many bits per wave
+constexpr uint32_t NOALLOC_MAX_DEVICES = 64;
+// Looks up this device's copy of the file-scope scratch array (the device must be current).
+void noalloc_setup(int device);
+// *d_out = sum_e W_e (lambda(e) - 1) in 2^18 fixed point, or -1 when a pin's partition id is
+// >= k; launches two kernels on the context stream and allocates nothing.
+void connectivity_noalloc(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* part,
+                          uint32_t e, uint32_t k, int64_t* d_out);
 
 // ---- initial partitioner input (ipprep.hip) ------------------------------------------------
 // The coarsest level as the initial partitioner takes it (hedges of >= 2 distinct pins,

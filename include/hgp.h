@@ -307,6 +307,18 @@ hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64
                             const float* d_hedge_weights, const uint32_t* d_partitions, uint32_t num_hedges,
                             int64_t* km1_fixed);
 
+/* hgp_connectivity without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The result goes to device memory:
+ * *d_km1_fixed = the same 2^18 fixed-point lambda-1, or -1 when a pin's partition id is
+ * >= num_partitions.  Launches two kernels on the context stream (the call returns before
+ * they finish) and calls no hipMalloc, hipFree or memset.  Its partial sums, ticket and
+ * error flag live in a fixed library array of 16.5 KiB per device (2048 workgroup
+ * partials); num_partitions above 65536 is rejected with HGP_E_CAPACITY.  Calls that
+ * overlap on one device share that array and race: serialise them (one stream, or events). */
+hgp_status hgp_connectivity_noalloc(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                                    const float* d_hedge_weights, const uint32_t* d_partitions, uint32_t num_hedges,
+                                    uint32_t num_partitions, int64_t* d_km1_fixed);
 
 /* initial_partitioning_kahypar (sources/init_part.cu:733-953): k-way initial
  * partition of a coarse level.  With `prune` = f > 0, hedges are first pruned as the
