@@ -70,6 +70,24 @@ class RunStats(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """hgp_tuning: launch parameters chosen by the autotuner (speed only, never results)."""
+    _fields_ = [
+        ("sort_rounds", C.c_uint32 * 40),
+        ("heavy_table_slots", C.c_uint32),
+        ("heavy_table_fill16", C.c_uint32),
+        ("cascade_block", C.c_uint32),
+        ("scan_lookback_tiles", C.c_uint32),
+        ("tuned", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {"sort_rounds": {i: int(r) for i, r in enumerate(self.sort_rounds) if r},
+                "heavy_table_slots": int(self.heavy_table_slots), "heavy_table_fill16": int(self.heavy_table_fill16),
+                "cascade_block": int(self.cascade_block), "scan_lookback_tiles": int(self.scan_lookback_tiles),
+                "tuned": int(self.tuned)}
+
+
 # (name, restype, argtypes) of every exported entry point of include/hgp.h
 SIGNATURES = [
     ("hgp_default_params", None, [C.POINTER(Params)]),
@@ -129,6 +147,15 @@ SIGNATURES = [
     ("hgp_relabel_partitions", C.c_int32, [_vp, C.c_uint32, _vp, _u32p]),
     ("hgp_inbound_sizes", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp]),
     ("hgp_connectivity", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, _i64p]),
+    ("hgp_quotient", C.c_int32, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _u32p, _u32p, _u64p,
+                                 C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    ("hgp_autotune", C.c_int32, [_vp, C.c_uint32, C.POINTER(Tuning)]),
+    ("hgp_tuning_get", C.c_int32, [_vp, C.POINTER(Tuning)]),
+    ("hgp_tuning_set", C.c_int32, [_vp, C.POINTER(Tuning)]),
+    ("hgp_tuning_defaults", None, [C.POINTER(Tuning)]),
+    ("hgp_tuning_set_process_default", C.c_int32, [C.POINTER(Tuning)]),
+    ("hgp_tuning_save", C.c_int32, [C.POINTER(Tuning), C.c_char_p]),
+    ("hgp_tuning_load", C.c_int32, [C.c_char_p, C.POINTER(Tuning)]),
     ("hgp_connectivity_noalloc", C.c_int32, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp]),
     ("hgp_refine_lp", C.c_int32, [_vp, C.POINTER(Params), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32,
                                   C.c_uint32, _vp, _vp]),

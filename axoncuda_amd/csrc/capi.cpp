@@ -6,6 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
 #include <unordered_map>
 
 #include "common.hpp"
@@ -87,6 +90,36 @@ using hgp::Ctx;
 
 namespace {
 
+// launch parameters that new contexts start with (hgp_tuning_set_process_default)
+std::mutex g_tune_mu;
+hgp::Tuning g_default_tuning;
+
+void to_public(const hgp::Tuning& t, hgp_tuning* o) {
+  std::memset(o, 0, sizeof *o);
+  for (int i = 0; i < 40; ++i) o->sort_rounds[i] = t.sort_rounds[i];
+  o->heavy_table_slots = t.heavy_slots;
+  o->heavy_table_fill16 = t.heavy_fill16;
+  o->cascade_block = t.cascade_block;
+  o->scan_lookback_tiles = t.scan_lookback_tiles;
+  o->tuned = t.tuned ? 1u : 0u;
+}
+
+bool from_public(const hgp_tuning* o, hgp::Tuning& t) {
+  for (int i = 0; i < 40; ++i)
+    if (o->sort_rounds[i] > 16) return false;
+  const uint32_t s = o->heavy_table_slots, b = o->cascade_block;
+  if (s < (1u << 13) || s > (1u << 24) || (s & (s - 1))) return false;
+  if (o->heavy_table_fill16 < 4 || o->heavy_table_fill16 > 14) return false;
+  if (b < 64 || b > 1024 || (b & (b - 1))) return false;
+  for (int i = 0; i < 40; ++i) t.sort_rounds[i] = o->sort_rounds[i];
+  t.heavy_slots = s;
+  t.heavy_fill16 = o->heavy_table_fill16;
+  t.cascade_block = b;
+  t.scan_lookback_tiles = o->scan_lookback_tiles;
+  t.tuned = o->tuned != 0;
+  return true;
+}
+
 template <typename F>
 hgp_status guard(hgp_ctx* ctx, F&& f) {
   if (!ctx) return HGP_E_INVALID;
@@ -144,6 +177,10 @@ hgp_status hgp_ctx_create(int device, uint64_t hbm_budget, hgp_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HGP_E_NODEVICE;
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return HGP_E_NODEVICE;
   hgp_ctx* ctx = new hgp_ctx();
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    ctx->c.tune = g_default_tuning;
+  }
   ctx->c.device = device;
   ctx->c.budget = hbm_budget;
   ctx->c.num_cus = (uint32_t)prop.multiProcessorCount;
@@ -607,6 +644,107 @@ hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64
   return guard(ctx, [&](Ctx& c) {
     *km1_fixed = hgp::connectivity_fixed(c, d_hedges, d_hedges_offsets, d_hedge_weights, d_partitions, num_hedges);
   });
+}
+
+hgp_status hgp_quotient(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                        const uint32_t* d_srcs_count, const float* d_hedge_weights, const uint32_t* d_partitions,
+                        uint32_t num_nodes, uint32_t num_hedges, uint32_t* num_partitions, uint32_t* num_merged,
+                        uint64_t* num_pins, uint32_t** d_pins, uint64_t** d_offsets, uint32_t** d_srcs_count_out,
+                        float** d_weights) {
+  if (!num_partitions || !num_merged || !num_pins || !d_pins || !d_offsets || !d_srcs_count_out || !d_weights)
+    return HGP_E_INVALID;
+  if (num_hedges && (!d_hedges || !d_hedges_offsets || !d_srcs_count || !d_hedge_weights)) return HGP_E_INVALID;
+  if (num_nodes && !d_partitions) return HGP_E_INVALID;
+  return guard(ctx, [&](Ctx& c) {
+    uint32_t k = 0, u = 0;
+    uint64_t p = 0;
+    hgp::quotient(c, d_hedges, d_hedges_offsets, d_srcs_count, d_hedge_weights, d_partitions, num_nodes, num_hedges, k,
+                  u, p, *d_pins, *d_offsets, *d_srcs_count_out, *d_weights);
+    hgp::sync(c);
+    *num_partitions = k;
+    *num_merged = u;
+    *num_pins = p;
+  });
+}
+
+hgp_status hgp_autotune(hgp_ctx* ctx, uint32_t max_log2, hgp_tuning* out) {
+  return guard(ctx, [&](Ctx& c) {
+    hgp::autotune(c, max_log2);
+    if (out) to_public(c.tune, out);
+  });
+}
+
+hgp_status hgp_tuning_get(const hgp_ctx* ctx, hgp_tuning* out) {
+  if (!ctx || !out) return HGP_E_INVALID;
+  to_public(ctx->c.tune, out);
+  return HGP_OK;
+}
+
+hgp_status hgp_tuning_set(hgp_ctx* ctx, const hgp_tuning* in) {
+  if (!ctx || !in) return HGP_E_INVALID;
+  hgp::Tuning t;
+  if (!from_public(in, t)) return HGP_E_INVALID;
+  ctx->c.tune = t;
+  return HGP_OK;
+}
+
+void hgp_tuning_defaults(hgp_tuning* out) {
+  if (out) to_public(hgp::Tuning(), out);
+}
+
+hgp_status hgp_tuning_set_process_default(const hgp_tuning* in) {
+  hgp::Tuning t;
+  if (in && !from_public(in, t)) return HGP_E_INVALID;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_default_tuning = t;
+  return HGP_OK;
+}
+
+hgp_status hgp_tuning_save(const hgp_tuning* t, const char* path) {
+  if (!t || !path) return HGP_E_INVALID;
+  std::ofstream f(path);
+  if (!f) return HGP_E_IO;
+  f << "# hgp_tuning (launch parameters; results do not depend on them)\n";
+  for (int i = 0; i < 40; ++i)
+    if (t->sort_rounds[i]) f << "sort_rounds " << i << " " << t->sort_rounds[i] << "\n";
+  f << "heavy_table_slots " << t->heavy_table_slots << "\n";
+  f << "heavy_table_fill16 " << t->heavy_table_fill16 << "\n";
+  f << "cascade_block " << t->cascade_block << "\n";
+  f << "scan_lookback_tiles " << t->scan_lookback_tiles << "\n";
+  f << "tuned " << t->tuned << "\n";
+  return f.good() ? HGP_OK : HGP_E_IO;
+}
+
+hgp_status hgp_tuning_load(const char* path, hgp_tuning* out) {
+  if (!path || !out) return HGP_E_INVALID;
+  std::ifstream f(path);
+  if (!f) return HGP_E_IO;
+  hgp_tuning t;
+  hgp_tuning_defaults(&t);
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream ss(line);
+    std::string key;
+    ss >> key;
+    uint64_t a = 0, b = 0;
+    if (key == "sort_rounds") {
+      if (!(ss >> a >> b) || a >= 40 || b > 16) return HGP_E_INVALID;
+      t.sort_rounds[a] = (uint32_t)b;
+      continue;
+    }
+    if (!(ss >> a) || a > 0xFFFFFFFFull) return HGP_E_INVALID;
+    if (key == "heavy_table_slots") t.heavy_table_slots = (uint32_t)a;
+    else if (key == "heavy_table_fill16") t.heavy_table_fill16 = (uint32_t)a;
+    else if (key == "cascade_block") t.cascade_block = (uint32_t)a;
+    else if (key == "scan_lookback_tiles") t.scan_lookback_tiles = (uint32_t)a;
+    else if (key == "tuned") t.tuned = (uint32_t)a;
+    else return HGP_E_INVALID;
+  }
+  hgp::Tuning chk;
+  if (!from_public(&t, chk)) return HGP_E_INVALID;
+  *out = t;
+  return HGP_OK;
 }
 
 hgp_status hgp_connectivity_noalloc(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,

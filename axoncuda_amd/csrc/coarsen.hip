@@ -33,7 +33,6 @@ constexpr uint32_t CAND_BIG_SLOTS = 16384;  // LDS hash slots of a big node (128
 constexpr uint32_t CAND_DIRECT_N = 39168;   // levels up to this size index LDS by node id (<= 160 KB;
                                             // half that with inbound counts)
 constexpr uint64_t HEAVY_TABLE_SLOTS = 64ull << 20;  // HBM table slots shared by the heavy grid
-constexpr uint32_t HEAVY_TRY_SLOTS = 65536;          // first-attempt HBM table of a heavy node
 
 // Pins visited when scanning a node's touching hedges (itself excluded once per hedge).
 __global__ void k_visit_bound(const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ tch,
@@ -425,12 +424,12 @@ __global__ void __launch_bounds__(1024) k_cand_direct(CAND_ARGS) {
 // taking nodes blockIdx.x, blockIdx.x + gridDim.x, ...  A table costs its size to clear and
 // to rank, and the worst-case size (from the visit bound) can be far above the distinct
 // neighbours, so nodes whose worst case exceeds `try_cap` first try a table of try_cap
-// slots (optimistically, stopping at 3/4 load) and only the ones that overflow it go to a
+// slots (optimistically, stopping at `try_fill` entries) and only the ones that overflow it go to a
 // second launch (try_cap = 0) with their worst-case table.  Nodes indexed by node id
 // (cap == n) also wait for the second launch, so that they do not size every region.
 __global__ void __launch_bounds__(256) k_cand_heavy(CAND_ARGS, uint32_t nlist, uint32_t n, uint64_t region,
                                                     uint32_t* __restrict__ gkeys, uint32_t* __restrict__ gscore,
-                                                    uint32_t* __restrict__ ginter, uint32_t try_cap,
+                                                    uint32_t* __restrict__ ginter, uint32_t try_cap, uint32_t try_fill,
                                                     uint32_t* __restrict__ ovf, uint32_t* __restrict__ ovf_count) {
   __shared__ uint64_t s_w[64];
   __shared__ uint32_t s_state[2];
@@ -448,7 +447,7 @@ __global__ void __launch_bounds__(256) k_cand_heavy(CAND_ARGS, uint32_t nlist, u
     } else {
       const bool done = cand_block_node<true, true, 4, true>(u, try_cap, false, keys, score, inter, hedges, hoff, srcc,
                                                              tch, toff, inb, w, sizes, cnc, omega, delta, pairs,
-                                                             scores, s_w, s_state, try_cap / 4 * 3);
+                                                             scores, s_w, s_state, try_fill);
       if (!done && threadIdx.x == 0) ovf[atomicAdd(ovf_count, 1u)] = u;
       __syncthreads();  // s_state is reused by the next node
     }
@@ -704,12 +703,14 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     fprintf(stderr, "  candidates: n=%u light %u medium %u big %u optimistic %u (overflow %u, then %u)\n", n, nl, nm,
             nb, nh, no1, no);
   if (no == 0) return;
-  // HBM tables: first with tables capped at HEAVY_TRY_SLOTS, then the overflow with full ones
+  // HBM tables: first with tables capped at c.tune.heavy_slots (filled to heavy_fill16 / 16, the
+  // autotuner's choices; 65,536 and 3/4 by default), then the overflow with full ones
   DBuf<uint32_t> ovf2(c, no), novf2(c, 1), mc(c, 1);
   const uint32_t* lst = ovf.get();
   uint32_t cntl = no;
   for (int pass = 0; pass < 2 && cntl; ++pass) {
-    const uint32_t try_cap = pass == 0 ? HEAVY_TRY_SLOTS : 0u;
+    const uint32_t try_cap = pass == 0 ? c.tune.heavy_slots : 0u;
+    const uint32_t try_fill = (uint32_t)((uint64_t)try_cap * c.tune.heavy_fill16 / 16);
     memset_async(c, mc.get(), 0, 4);
     memset_async(c, novf2.get(), 0, 4);
     HGP_LAUNCH(k_heavy_max_cap, std::min<uint32_t>(ceil_div(cntl, 256), 1024), 256, 0, c, lst, caps.get(), cntl,
@@ -718,7 +719,7 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)cntl, HEAVY_TABLE_SLOTS / region, 4096}));
     DBuf<uint32_t> gk(c, grid * region), gs(c, grid * region), gi(c, grid * region);
     HGP_LAUNCH(k_cand_heavy, (uint32_t)grid, 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, lst, caps.get(),
-               cnc, omega, delta, pairs, scores, cntl, n, region, gk.get(), gs.get(), gi.get(), try_cap, ovf2.get(),
+               cnc, omega, delta, pairs, scores, cntl, n, region, gk.get(), gs.get(), gi.get(), try_cap, try_fill, ovf2.get(),
                novf2.get());
     if (c.verbose >= 2)
       fprintf(stderr, "  candidates: %u nodes in HBM tables (pass %d, region %llu, grid %llu)\n", cntl, pass,

@@ -51,6 +51,17 @@ struct ProfRec {
   uint64_t bytes = 0;
 };
 
+// Launch parameters chosen by the autotuner (hgp_tuning in the C ABI).  They change speed
+// only: every stage returns the same result under any valid setting.
+struct Tuning {
+  uint32_t sort_rounds[40] = {};     // radix-sort keys per thread and tile, by floor(log2 n); 0 = rule
+  uint32_t heavy_slots = 65536;      // first-attempt HBM hash table of a heavy candidate node
+  uint32_t heavy_fill16 = 12;        // its fill limit in sixteenths
+  uint32_t cascade_block = 256;      // threads per workgroup of the thread-per-hedge cascade
+  uint32_t scan_lookback_tiles = 64;  // largest scan (in tiles) that takes the single-pass look-back
+  bool tuned = false;
+};
+
 // Execution context: device, stream, HBM budget accounting, last error text.
 struct Ctx {
   int device = 0;
@@ -80,6 +91,7 @@ struct Ctx {
   unsigned long long* prof_dev = nullptr;
   bool prof_count = false;  // kernels with a byte model count their bytes (extra device work)
   void* pinned = nullptr;  // small pinned host buffer for scalar read-backs (allocated on first use)
+  Tuning tune;
 };
 constexpr uint64_t PINNED_BYTES = 256;
 constexpr uint32_t PROF_SLOTS = 64;

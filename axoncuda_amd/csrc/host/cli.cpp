@@ -37,6 +37,7 @@ void print_help() {
       "  -ipm        When set, initial partitions are greedily merged to reduce their number (and save memory - minor quality loss)\n"
       "  -ip <where> k-way initial partitioner without mtkahypar: 'auto' (default: 'device' on levels of 2^21 pins or more), 'device' (multistart label propagation) or 'host' (recursive bisection)\n"
       "  -lp <num>   k-way: label-propagation rounds before each level's FM rounds (default 0)\n"
+      "  -tune <file> Launch parameters: read them from the file, or autotune this device and write them there\n"
       "  -v <lvl>    Set the verbosity level: 0 metrics only, 1 steps and phases, 2 kernel launches, 3  \n"
       "  -h          Show this help message\n");
 }
@@ -87,7 +88,7 @@ int main(int argc, char** argv) {
     return 0;
   }
   enum { C_NONE, C_NAME, C_MANL, C_KWAY } ctype = C_NONE;
-  std::string load_path, save_path, part_path, cname;
+  std::string load_path, save_path, part_path, cname, tune_path;
   hgp_params prm;
   hgp_default_params(&prm);
   uint32_t m_omega = 0, m_delta = 0, m_maxp = 0;
@@ -149,6 +150,9 @@ int main(int argc, char** argv) {
       else if (w == "device") prm.initial_partitioner = HGP_IP_DEVICE;
       else if (w == "auto") prm.initial_partitioner = HGP_IP_AUTO;
       else { std::fprintf(stderr, "Error: -ip requires 'auto', 'host' or 'device'\n"); return 1; }
+    } else if (a == "-tune") {
+      if (i + 1 >= argc) { std::fprintf(stderr, "Error: -tune requires a file path\n"); return 1; }
+      tune_path = argv[++i];
     } else if (a == "-lp") {
       if (i + 1 >= argc) { std::fprintf(stderr, "Error: -lp requires a positive integer value\n"); return 1; }
       prm.lp_rounds = (uint32_t)stoul_or_die(argv[++i], "-lp");
@@ -258,6 +262,18 @@ int main(int argc, char** argv) {
     return s;
   }
   std::printf("HIP device:\n  Using device 0 (gfx950)\n");
+  if (!tune_path.empty()) {  // launch parameters only: the partition does not depend on them
+    hgp_tuning t;
+    if (hgp_tuning_load(tune_path.c_str(), &t) == HGP_OK) {
+      hgp_tuning_set(ctx, &t);
+      std::printf("Launch parameters read from %s\n", tune_path.c_str());
+    } else if (hgp_autotune(ctx, 0, &t) == HGP_OK && hgp_tuning_save(&t, tune_path.c_str()) == HGP_OK) {
+      std::printf("Launch parameters autotuned and written to %s\n", tune_path.c_str());
+    } else {
+      std::fprintf(stderr, "WARNING: could not autotune or write %s, using the built-in launch parameters\n",
+                   tune_path.c_str());
+    }
+  }
 
   // rule-2 dedup before the device section (sources/main.cu:125)
   hgp_hgraph* dd = nullptr;

@@ -58,10 +58,14 @@ struct hgp_server {
 
 namespace {
 
-// Peak device bytes of one pipeline run, from the level-0 sizes (measured peaks of the
-// benchmark workloads stay below this bound: ~100 B per pin and ~60 B per node).
+// Peak device bytes of one pipeline run, from the level-0 sizes.  Measured peaks with every
+// level resident (-smh 0), in bytes per level-0 pin: random hypergraphs (|e| = 2 + Geom(1/2))
+// 71 at 10^4 pins, 118 at 2 x 10^5, 192 at 3 x 10^6 and 232 at 2 x 10^7 (k = 8: twelve levels
+// that shrink slowly, then the device initial partitioner on a coarsest level of millions of
+// pins); layered SNNs 61-139; the 3.2M-node netlist 166.  The bound keeps about 10% above the
+// largest of them.
 uint64_t footprint(uint32_t n, uint32_t e, uint64_t p) {
-  return 160ull * p + 96ull * n + 48ull * e + (64ull << 20);
+  return 256ull * p + 128ull * n + 64ull * e + (64ull << 20);
 }
 
 void worker_loop(hgp_server* s, hgp_ctx* ctx) {

@@ -24,8 +24,11 @@ namespace hgp {
 
 namespace {
 
-// Best-fit decreasing: every node goes to the fullest bin it fits in (ties: lower bin); a
-// node that fits no bin goes to the lightest one.
+// Best-fit decreasing in two levels: every node goes to the fullest bin it fits in below the
+// average load min(Omega, ceil(W / k)) (W = total size), else to the fullest bin it fits in
+// below Omega, else to the lightest bin (ties: lower bin).  Packing against the average load
+// spreads the nodes over all k bins; packing against Omega alone fills bins to capacity and,
+// with a few percent of slack, leaves the last ones empty.
 void bfd_pack(const std::vector<uint32_t>& sizes, uint32_t k, uint32_t omega, uint64_t seed, std::vector<uint32_t>& part,
               std::vector<uint32_t>& loads) {
   const uint32_t n = (uint32_t)sizes.size();
@@ -39,12 +42,18 @@ void bfd_pack(const std::vector<uint32_t>& sizes, uint32_t k, uint32_t omega, ui
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sizes[a] > sizes[b]; });
   part.assign(n, 0);
   loads.assign(k, 0);
+  uint64_t total = 0;
+  for (uint32_t s : sizes) total += s;
+  const uint64_t avg = std::min<uint64_t>(omega, (total + k - 1) / k);
   for (uint32_t v : order) {
     const uint32_t s = sizes[v];
     uint32_t best = k;
-    for (uint32_t b = 0; b < k; ++b)
-      if ((uint64_t)loads[b] + s <= omega && (best == k || loads[b] > loads[best])) best = b;
-    if (best == k) {
+    for (uint32_t b = 0; b < k; ++b)  // fullest bin below the average load
+      if ((uint64_t)loads[b] + s <= avg && (best == k || loads[b] > loads[best])) best = b;
+    if (best == k)
+      for (uint32_t b = 0; b < k; ++b)  // fullest bin below Omega
+        if ((uint64_t)loads[b] + s <= omega && (best == k || loads[b] > loads[best])) best = b;
+    if (best == k) {  // lightest bin
       best = 0;
       for (uint32_t b = 1; b < k; ++b)
         if (loads[b] < loads[best]) best = b;

@@ -118,7 +118,6 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_lookback(const TI* in, TO* 
 // of tiles the decoupled look-back's chain of publications is slower than two extra passes.
 constexpr int RS3_ITEMS = 8;
 constexpr int RS3_TILE = SCAN_BLOCK * RS3_ITEMS;
-constexpr uint32_t LOOKBACK_MAX_TILES = 64;
 
 template <typename T>
 __device__ __forceinline__ T block_excl_scan(T v, T* lds_waves, T* total) {
@@ -257,7 +256,7 @@ void scan_impl(Ctx& c, const TI* in, TO* out, uint64_t n, bool exclusive) {
     return;
   }
   const uint32_t tiles = ceil_div(n, SCAN_TILE);
-  if (tiles > LOOKBACK_MAX_TILES) {
+  if (tiles > c.tune.scan_lookback_tiles) {  // default 64 (the autotuner may move it)
     const uint32_t t3 = ceil_div(n, RS3_TILE);
     DBuf<TO> partial(c, (uint64_t)t3 + 1);
     HGP_LAUNCH((k_tile_reduce<TI, TO>), t3, SCAN_BLOCK, 0, c, in, n, partial.get());
@@ -290,8 +289,11 @@ constexpr uint32_t RS_MAX_ROUNDS = 16;  // keys per thread and tile of a large s
 constexpr uint32_t RS_TARGET_TILES = 512;  // small sorts shrink their tiles to spread over the CUs
 
 // Rounds of RS_BLOCK keys per tile: one tile per workgroup, so a small sort runs many short
-// workgroups side by side instead of a few that walk 16 rounds one after another.
-inline uint32_t rs_rounds(uint64_t n) {
+// workgroups side by side instead of a few that walk 16 rounds one after another.  The
+// autotuner may set the rounds per size class (floor(log2 n)); the result is the same.
+inline uint32_t rs_rounds(const Ctx& c, uint64_t n) {
+  const int cls = bit_width(n) - 1;
+  if (cls >= 0 && cls < 40 && c.tune.sort_rounds[cls]) return std::min(c.tune.sort_rounds[cls], RS_MAX_ROUNDS);
   return (uint32_t)std::min<uint64_t>(RS_MAX_ROUNDS, std::max<uint64_t>(1, ceil_div(n, (uint64_t)RS_BLOCK * RS_TARGET_TILES)));
 }
 
@@ -487,7 +489,7 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 
     }
     return;
   }
-  const uint32_t rounds = rs_rounds(n);
+  const uint32_t rounds = rs_rounds(c, n);
   const uint32_t nblocks = ceil_div(n, (uint64_t)RS_BLOCK * rounds);
   DBuf<K> k2(c, n);
   DBuf<uint32_t> v2(c, vals ? n : 0);

@@ -2377,7 +2377,8 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
     DBuf<uint32_t> fl[3];
     if (!known)
       for (int cls = 0; cls < 3; ++cls) fl[cls] = DBuf<uint32_t>(c, L.e);
-    HGP_LAUNCH(k_cascade_small, ceil_div(L.e, 256), 256, 0, c, a, L.e, sort_max, fl[0].get(), fl[1].get(), fl[2].get());
+    const uint32_t cb = c.tune.cascade_block;  // 256 unless the autotuner chose otherwise
+    HGP_LAUNCH(k_cascade_small, ceil_div(L.e, cb), cb, 0, c, a, L.e, sort_max, fl[0].get(), fl[1].get(), fl[2].get());
     if (prof_match(c, "k_cascade_small") && c.prof_count && c.prof_dev)
       HGP_LAUNCH(k_cascade_small_bytes, ceil_div(L.e, 256), 256, 0, c, a, L.e, c.prof_dev);
     if (!known) {
@@ -2565,6 +2566,24 @@ uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rou
   HGP_LAUNCH(k_jacobi_apply, ceil_div(m, 256), 256, 0, c, dst.get(), vals.get(), sz.get(), doff.get(), m, L.sizes,
              before.get(), omega, part, psizes, applied.get());
   return read_scalar(c, applied.get());
+}
+
+float time_cascade_small(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint4* pm,
+                         uint32_t e, uint32_t block, unsigned long long* seq) {
+  CascadeArgs a{hedges, hoff, w, pm, 0, seq};
+  hipEvent_t t0, t1;
+  HGP_HIP(hipEventCreate(&t0));
+  HGP_HIP(hipEventCreate(&t1));
+  HGP_HIP(hipEventRecord(t0, c.stream));
+  HGP_LAUNCH(k_cascade_small, ceil_div(e, block), block, 0, c, a, e, 0u, (uint32_t*)nullptr, (uint32_t*)nullptr,
+             (uint32_t*)nullptr);
+  HGP_HIP(hipEventRecord(t1, c.stream));
+  HGP_HIP(hipEventSynchronize(t1));
+  float ms = 0;
+  HGP_HIP(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return ms;
 }
 
 void lp_refine(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rounds, uint32_t* part, uint32_t* psizes) {

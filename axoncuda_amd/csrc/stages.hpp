@@ -73,6 +73,14 @@ uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rou
 // Label-propagation refinement (k-way): up to `rounds` Jacobi rounds; a round that applies
 // nothing ends them, a round that raises lambda-1 is undone and ends them.
 void lp_refine(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rounds, uint32_t* part, uint32_t* psizes);
+// Milliseconds of one thread-per-hedge cascade launch with `block` threads per workgroup
+// (pm: per node (partition, target or NONE, rank, 0); seq: one counter per rank).  For the
+// autotuner.
+float time_cascade_small(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint4* pm,
+                         uint32_t e, uint32_t block, unsigned long long* seq);
+// The autotuner (tune.hip): times the launch parameters of Ctx::tune on synthetic inputs of
+// up to 2^max_log2 keys / hedges and keeps the fastest of each in c.tune.
+void autotune(Ctx& c, uint32_t max_log2);
 void inbound_sizes(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const uint32_t* part,
                    uint32_t e, uint32_t k, uint32_t* out);
 int64_t connectivity_fixed(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint32_t* part,
@@ -100,6 +108,14 @@ void ip_input_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const
 // float weights = summed fixed-point weights / 2^18.
 void merge_level_device(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, uint32_t n, uint32_t e,
                         uint32_t prune, uint32_t& e_out, uint32_t*& hedges_out, uint64_t*& hoff_out, float*& w_out);
+
+// ---- quotient hypergraph (quotient.hip) ---------------------------------------------------
+// getPartitionsHypergraph(part, 2, true) on the device: k_out partitions (ids must be 0..k-1,
+// all used), e_out merged hedges of p_out pins in order of first appearance (sources then
+// destinations, each ascending) with their float weights summed in hedge order.
+void quotient(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint32_t* srcc, const float* w,
+              const uint32_t* part, uint32_t n, uint32_t e, uint32_t& k_out, uint32_t& e_out, uint64_t& p_out,
+              uint32_t*& qpins, uint64_t*& qoff, uint32_t*& qsrcc, float*& qw);
 
 // ---- post-processing (postproc.hip) -----------------------------------------------------
 void greedy_merge_groups(Ctx& c, const uint32_t* sizes, const uint32_t* inb, uint32_t n, uint32_t ngroups,

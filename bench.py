@@ -7,7 +7,9 @@ partitioning (coarsening, initial partition, uncoarsening + refinement) of that 
 starting from device-resident CSR arrays; the partition vector lands in host memory.
 
 value = post-dedup pins / second of one step (the reference's "device core" boundary,
-sources/main.cu:260-264,891-896), with inputs already in HBM.  Beside it, the line carries:
+sources/main.cu:260-264,891-896), with inputs already in HBM.  Before the warm-up the launch
+parameters are autotuned once (hgp_autotune, untimed; results do not depend on them) and
+reported under config.tuning.  Beside it, the line carries:
 
 * pipeline: the compulsory HBM traffic of a step (every stage's inputs and outputs once per
   executed level and refinement round) over the step time, against the 8 TB/s peak;
@@ -22,7 +24,7 @@ sources/main.cu:260-264,891-896), with inputs already in HBM.  Beside it, the li
   generator, with its lambda-1 and the GPU's lambda-1 on that same sample;
 * serving: the continuous-batching service over jobs of 10^4 to 2 x 10^7 pins.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-e2e] [--no-serve]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-e2e] [--no-serve] [--no-tune]
 """
 from __future__ import annotations
 
@@ -233,6 +235,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end command-line leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch parameters (no autotuning)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--serve", action="store_true", help="run only the serving-stream benchmark")
     ap.add_argument("--jobs", type=int, default=32, help="serving jobs (the full stream is 256)")
@@ -262,6 +265,14 @@ def main() -> int:
     e2e = None if args.no_e2e else end_to_end(hg, tmpdir)  # its own process, before this one uses the GPU
 
     dev = H.Device(0)
+    tuning = None
+    if not args.no_tune:  # launch parameters only (untimed; results do not depend on them)
+        t = H.Tuning()
+        t0 = time.perf_counter()
+        dev.check(lib.hgp_autotune(dev.ctx, 26, C.byref(t)), "hgp_autotune")
+        dev.check(lib.hgp_tuning_set_process_default(C.byref(t)), "tuning default")  # serving workers too
+        tuning = dict(t.as_dict(), seconds=time.perf_counter() - t0)
+        log(f"# autotuned in {tuning['seconds']:.1f} s: {t.as_dict()}")
     d_pins, d_off, d_srcc, d_w = dev.upload(pins), dev.upload(off), dev.upload(srcc), dev.upload(w)
     prm = kway_params(n, K, EPS)
     part = np.zeros(n, np.uint32)
@@ -340,7 +351,8 @@ def main() -> int:
                    "smh": 0, "candidates": prm.candidates_count, "refine_repeats": prm.refine_repeats,
                    "levels": st.levels, "lambda_minus_1": km1, "valid": bool(valid == 1),
                    "core_ms_p50": float(np.median(core_ms)), "ip_ms_p50": float(np.median(ip_ms)),
-                   "initial_prune": prm.initial_prune},
+                   "initial_prune": prm.initial_prune, "initial_partitioner": prm.initial_partitioner,
+                   "tuning": tuning},
         "pipeline": pipeline,
         "roofline": roofline,
         "end_to_end": e2e,

@@ -94,6 +94,28 @@ typedef struct hgp_params {
 enum { HGP_IP_HOST = 0, HGP_IP_DEVICE = 1, HGP_IP_AUTO = 2 };
 #define HGP_IP_AUTO_DEVICE_PINS 2097152u
 
+/* Launch parameters chosen by the autotuner.  No counterpart in the reference, whose block
+ * sizes, hash-set sizes and LDS budgets are compile-time constants (headers/construction.cuh:
+ * 22-29, headers/coarsening.cuh:21-31).  They change speed only: every entry point returns
+ * the same result under any valid setting. */
+typedef struct hgp_tuning {
+  /* radix sorts of 2^c .. 2^(c+1) - 1 keys: keys per thread in a workgroup's tile (the tile is
+   * 256 x this many keys), 1..16; 0 = the built-in rule (about 512 tiles per sort) */
+  uint32_t sort_rounds[40];
+  /* candidate scoring of heavy nodes: slots of the first-attempt HBM hash table (a power of
+   * two, 2^13..2^24) and its fill limit in sixteenths (4..14), i.e. the table occupancy that
+   * the reference's -om multiplier sets for its spill hash sets */
+  uint32_t heavy_table_slots;
+  uint32_t heavy_table_fill16;
+  /* threads per workgroup of the thread-per-hedge cascade: 64, 128, 256, 512 or 1024 */
+  uint32_t cascade_block;
+  /* scans of more than this many 4096-element tiles take three passes instead of the
+   * single-pass look-back */
+  uint32_t scan_lookback_tiles;
+  /* 1 when set by hgp_autotune */
+  uint32_t tuned;
+} hgp_tuning;
+
 /* Per-run statistics filled by hgp_partition. */
 typedef struct hgp_run_stats {
   double core_ms;       /* device-resident input -> partition on host */
@@ -301,11 +323,46 @@ hgp_status hgp_inbound_sizes(hgp_ctx* ctx, const uint32_t* d_hedges, const uint6
                              const uint32_t* d_srcs_count, const uint32_t* d_partitions, uint32_t num_hedges,
                              uint32_t num_partitions, uint32_t* d_out);
 
+/* getPartitionsHypergraph(part, 2, true) (includes/hgraph.hpp:286-376) on the device: the
+ * partitioned hypergraph, one node per partition.  Every hedge becomes (its distinct source
+ * partitions, its distinct destination partitions that are not source ones); hedges with at
+ * most one source partition and no destination partition are dropped; identical hedges are
+ * merged with their weights summed.  Unlike the reference (a 64-bit hash of the sets as a map
+ * key, unordered output), equal sets are compared exactly and the merged hedges come in order
+ * of first appearance, sources then destinations, each ascending, each weight summed in float
+ * in hedge order: the same hypergraph hgp_save_partitioned writes.  Partition ids must be
+ * 0..k-1 with every id used (HGP_E_INVALID otherwise, as the reference throws).  Allocates
+ * *d_pins[*num_pins], *d_offsets[*num_merged + 1] (u64), *d_srcs_count[*num_merged] and
+ * *d_weights[*num_merged]; *num_partitions = k. */
+hgp_status hgp_quotient(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
+                        const uint32_t* d_srcs_count, const float* d_hedge_weights, const uint32_t* d_partitions,
+                        uint32_t num_nodes, uint32_t num_hedges, uint32_t* num_partitions, uint32_t* num_merged,
+                        uint64_t* num_pins, uint32_t** d_pins, uint64_t** d_offsets, uint32_t** d_srcs_count_out,
+                        float** d_weights);
+
 /* lambda-1 on the device: sum_e w_e * (lambda(e) - 1), in 2^18 fixed point
  * (compute_connectivity_kernel, kernels/init_part.cu:287-314). */
 hgp_status hgp_connectivity(hgp_ctx* ctx, const uint32_t* d_hedges, const uint64_t* d_hedges_offsets,
                             const float* d_hedge_weights, const uint32_t* d_partitions, uint32_t num_hedges,
                             int64_t* km1_fixed);
+
+/* The autotuner (no counterpart in the reference): times every hgp_tuning knob's candidates
+ * on synthetic inputs built on this context's device (sorts of 2^15 .. 2^max_log2 keys,
+ * max_log2 = 0 for 26; about a few seconds) and keeps the fastest in the context; *out (may
+ * be NULL) receives the result.  Results of every entry point are unchanged by it. */
+hgp_status hgp_autotune(hgp_ctx* ctx, uint32_t max_log2, hgp_tuning* out);
+/* The context's current launch parameters, and replacing them (HGP_E_INVALID when a value is
+ * out of its range; nothing is changed then). */
+hgp_status hgp_tuning_get(const hgp_ctx* ctx, hgp_tuning* out);
+hgp_status hgp_tuning_set(hgp_ctx* ctx, const hgp_tuning* in);
+/* The built-in defaults, and the launch parameters that contexts created from now on start
+ * with (serving workers included); NULL restores the built-in defaults. */
+void hgp_tuning_defaults(hgp_tuning* out);
+hgp_status hgp_tuning_set_process_default(const hgp_tuning* in);
+/* A tuning as a text file ("key value" lines), and back (HGP_E_IO / HGP_E_INVALID on a
+ * missing or malformed file). */
+hgp_status hgp_tuning_save(const hgp_tuning* t, const char* path);
+hgp_status hgp_tuning_load(const char* path, hgp_tuning* out);
 
 /* hgp_connectivity without any device allocation.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph capture, tight

@@ -1077,8 +1077,9 @@ IpInput ip_input(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* p
 // Best-fit-decreasing start of the device initial partitioner (modelled on the reference's
 // legacy host BFD packing, sources/init_part.cu:37-180): the nodes in a seeded shuffle
 // (Fisher-Yates over a splitmix64 stream started at seed * 0x9E3779B97F4A7C15 + 1), stably
-// sorted by size descending; each goes to the fullest bin it fits in (ties: lower bin), or to
-// the lightest bin (ties: lower) when it fits none.
+// sorted by size descending; each goes to the fullest bin it fits in below the average load
+// A = min(omega, ceil(total / k)), else to the fullest bin it fits in below omega, else to
+// the lightest bin (ties: lower bin in all three).
 void bfd_start(const std::vector<uint32_t>& sizes, uint32_t k, uint32_t omega, uint64_t seed, std::vector<uint32_t>& part,
                std::vector<uint32_t>& loads) {
   const uint32_t n = (uint32_t)sizes.size();
@@ -1092,11 +1093,17 @@ void bfd_start(const std::vector<uint32_t>& sizes, uint32_t k, uint32_t omega, u
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sizes[a] > sizes[b]; });
   part.assign(n, 0);
   loads.assign(k, 0);
+  uint64_t total = 0;
+  for (uint32_t s : sizes) total += s;
+  const uint64_t avg = std::min<uint64_t>(omega, (total + k - 1) / k);
   for (uint32_t v : order) {
     uint32_t pick = NONE;
-    for (uint32_t b = 0; b < k; ++b) {
-      if ((uint64_t)loads[b] + sizes[v] > omega) continue;
-      if (pick == NONE || loads[b] > loads[pick]) pick = b;
+    for (uint64_t cap : {avg, (uint64_t)omega}) {
+      for (uint32_t b = 0; b < k; ++b) {
+        if ((uint64_t)loads[b] + sizes[v] > cap) continue;
+        if (pick == NONE || loads[b] > loads[pick]) pick = b;
+      }
+      if (pick != NONE) break;
     }
     if (pick == NONE) {
       pick = 0;
@@ -1685,6 +1692,61 @@ int orc_partition(int mode, uint32_t omega, uint32_t delta, uint32_t max_parts, 
   relabel(n0, part.data());
   std::copy(part.begin(), part.end(), part_out);
   if (num_levels) *num_levels = depth + 1;
+  return 0;
+}
+
+// getPartitionsHypergraph(part, 2, true) (includes/hgraph.hpp:286-376): per hedge the set of
+// source partitions and the set of destination partitions minus the source ones; dropped
+// when at most one source partition and no destination remain; equal (sources, destinations)
+// merged with float weights summed in hedge order.  Exact set keys (the reference keys a map
+// by a 64-bit hash of the sets) and merged hedges in order of first appearance.  Two-phase:
+// with null arrays only the sizes are returned.  Returns 1 when the partition ids are not
+// 0..k-1 with every id used (the reference throws).
+int orc_quotient(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const uint32_t* srcc,
+                 const float* w, const uint32_t* part, uint32_t* k_out, uint32_t* e_out, uint64_t* p_out,
+                 uint64_t* qoff, uint32_t* qpins, uint32_t* qsrcc, float* qw) {
+  uint32_t k = 0;
+  for (uint32_t v = 0; v < n; ++v) k = std::max(k, part[v] + 1);
+  std::vector<uint8_t> used(k, 0);
+  for (uint32_t v = 0; v < n; ++v) used[part[v]] = 1;
+  for (uint32_t q = 0; q < k; ++q)
+    if (!used[q]) return 1;
+  std::map<std::pair<std::vector<uint32_t>, std::vector<uint32_t>>, uint32_t> index;
+  std::vector<std::pair<std::vector<uint32_t>, std::vector<uint32_t>>> sets;
+  std::vector<float> ws;
+  for (uint32_t x = 0; x < e; ++x) {
+    std::set<uint32_t> sp, dp;
+    for (uint64_t i = hoff[x]; i < hoff[x + 1]; ++i) {
+      if (i - hoff[x] < srcc[x]) sp.insert(part[pins[i]]);
+      else dp.insert(part[pins[i]]);
+    }
+    for (uint32_t s : sp) dp.erase(s);
+    if (sp.size() <= 1 && dp.empty()) continue;
+    std::pair<std::vector<uint32_t>, std::vector<uint32_t>> key{{sp.begin(), sp.end()}, {dp.begin(), dp.end()}};
+    auto it = index.find(key);
+    if (it != index.end()) {
+      ws[it->second] += w[x];
+      continue;
+    }
+    index.emplace(key, (uint32_t)sets.size());
+    sets.push_back(key);
+    ws.push_back(w[x]);
+  }
+  uint64_t total = 0;
+  for (auto& s : sets) total += s.first.size() + s.second.size();
+  *k_out = k;
+  *e_out = (uint32_t)sets.size();
+  *p_out = total;
+  if (!qoff) return 0;
+  qoff[0] = 0;
+  uint64_t at = 0;
+  for (size_t g = 0; g < sets.size(); ++g) {
+    for (uint32_t s : sets[g].first) qpins[at++] = s;
+    for (uint32_t d : sets[g].second) qpins[at++] = d;
+    qoff[g + 1] = at;
+    qsrcc[g] = (uint32_t)sets[g].first.size();
+    qw[g] = ws[g];
+  }
   return 0;
 }
 

@@ -131,6 +131,13 @@ int orc_device_initial_partition(uint32_t n, uint32_t e, const uint64_t* hoff, c
                                  uint32_t rounds, uint32_t per_part, uint32_t repeats, uint32_t* part,
                                  uint32_t* psizes);
 
+/* getPartitionsHypergraph(part, 2, true) (includes/hgraph.hpp:286-376) with exact set keys,
+ * merged hedges in order of first appearance; two-phase (null arrays: sizes only); 1 when
+ * the partition ids are not 0..k-1, all used. */
+int orc_quotient(uint32_t n, uint32_t e, const uint64_t* hoff, const uint32_t* pins, const uint32_t* srcc,
+                 const float* w, const uint32_t* part, uint32_t* k_out, uint32_t* e_out, uint64_t* p_out,
+                 uint64_t* qoff, uint32_t* qpins, uint32_t* qsrcc, float* qw);
+
 /* Whole pipeline (sources/main.cu:38-940).  ip_device: 0 the bisection, 1 the device initial
  * partitioner's restatement, 2 the device one on levels of at least 2^21 pins; ip_options: its (starts, rounds, nodes per part,
  * refinement rounds), NULL or 0 entries for the defaults (2, 8, 16, 4); lp_rounds: Jacobi

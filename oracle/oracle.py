@@ -67,6 +67,8 @@ def lib():
                                                       C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                                       C.c_uint32, u32p, u32p]
         _lib.orc_metrics.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p, u32p, f64p, f64p, f64p]
+        _lib.orc_quotient.argtypes = [C.c_uint32, C.c_uint32, u64p, u32p, u32p, f32p, u32p, u32p, u32p, u64p, u64p,
+                                      u32p, u32p, f32p]
     return _lib
 
 
@@ -373,6 +375,23 @@ def device_initial_partition(hg: HG, sizes, k, omega, prune=0, starts=2, rounds=
                                        k, omega, int(prune), starts, rounds, per_part, repeats, P(part, u32p),
                                        P(ps, u32p))
     return part, ps
+
+
+def quotient(hg: HG, part):
+    """The partitioned hypergraph (k, HG of the merged hedges); raises on non-sequential ids."""
+    part = np.ascontiguousarray(part, np.uint32)
+    k, e, p = C.c_uint32(), C.c_uint32(), C.c_uint64()
+    args = (hg.n, hg.e, P(hg.off, u64p), P(hg.pins, u32p), P(hg.srcc, u32p), P(hg.w, f32p), P(part, u32p),
+            C.byref(k), C.byref(e), C.byref(p))
+    rc = lib().orc_quotient(*args, None, None, None, None)
+    if rc:
+        raise OracleError(rc, "orc_quotient")
+    off = np.zeros(e.value + 1, np.uint64)
+    pins = np.zeros(p.value, np.uint32)
+    srcc = np.zeros(e.value, np.uint32)
+    w = np.zeros(e.value, np.float32)
+    lib().orc_quotient(*args, P(off, u64p), P(pins, u32p), P(srcc, u32p), P(w, f32p))
+    return k.value, HG(k.value, off, pins, srcc, w)
 
 
 def validity(hg: HG, part, omega, delta, max_parts):

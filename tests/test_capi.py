@@ -98,3 +98,34 @@ def test_cli_help_and_errors():
     assert r.returncode == 1 and "Unknown option" in r.stderr
     r = subprocess.run([exe, "-ip", "gpu"], capture_output=True, text=True)
     assert r.returncode == 1 and "'host' or 'device'" in r.stderr
+
+
+def test_tuning_file_round_trip_and_validation(tmp_path):
+    """hgp_tuning as a text file: defaults, save/load round trip, malformed files refused."""
+    lib = H.load_library()
+    t = H.Tuning()
+    lib.hgp_tuning_defaults(C.byref(t))
+    assert t.heavy_table_slots == 65536 and t.heavy_table_fill16 == 12 and t.cascade_block == 256
+    assert t.scan_lookback_tiles == 64 and t.tuned == 0 and not any(t.sort_rounds)
+    t.sort_rounds[20] = 4
+    t.sort_rounds[26] = 16
+    t.cascade_block = 512
+    t.heavy_table_slots = 1 << 17
+    t.heavy_table_fill16 = 10
+    t.tuned = 1
+    p = str(tmp_path / "tune.txt")
+    assert lib.hgp_tuning_save(C.byref(t), p.encode()) == 0
+    u = H.Tuning()
+    assert lib.hgp_tuning_load(p.encode(), C.byref(u)) == 0
+    assert u.as_dict() == t.as_dict()
+    assert lib.hgp_tuning_load(str(tmp_path / "missing.txt").encode(), C.byref(u)) == 5  # HGP_E_IO
+    for bad in ("cascade_block 96\n", "heavy_table_slots 5000\n", "heavy_table_fill16 15\n", "sort_rounds 3 17\n",
+                "sort_rounds 40 1\n", "unknown_key 1\n", "cascade_block\n"):
+        q = tmp_path / "bad.txt"
+        q.write_text(bad)
+        assert lib.hgp_tuning_load(str(q).encode(), C.byref(u)) == 1, bad  # HGP_E_INVALID
+    bad = H.Tuning()
+    lib.hgp_tuning_defaults(C.byref(bad))
+    bad.cascade_block = 100
+    assert lib.hgp_tuning_set_process_default(C.byref(bad)) == 1
+    assert lib.hgp_tuning_set_process_default(None) == 0

@@ -162,3 +162,43 @@ def test_save_refuses_partition_ids_with_gaps(tmp_path):
     part[::2] = 2  # ids 0 and 2, no 1 (includes/hgraph.hpp:292-295 throws)
     rc = H.load_library().hgp_save_partitioned(hg.h, part.ctypes.data_as(H._u32p), str(tmp_path / "q.hgr").encode())
     assert rc == 1
+
+
+def oracle_quotient_records(q, ext):
+    """Records of the oracle's quotient as the reference's savers lay them out (savehMETIS:
+    u32(w * 1024) weights when any weight is not 1, 1-based pins, sources first; saveAXH:
+    pin count, source count, pins, float weight bits)."""
+    k, g = q
+    recs = []
+    weighted = any(np.float32(x) != np.float32(1.0) for x in g.w)
+    for i in range(g.e):
+        pins = [int(x) for x in g.pins[g.off[i]:g.off[i + 1]]]
+        if ext == ".hgr":
+            wt = str(int(np.float32(g.w[i]) * np.float32(1024.0))) if weighted else None
+            recs.append((wt, tuple(str(p + 1) for p in pins)))
+        else:
+            recs.append((len(pins), int(g.srcc[i]), tuple(pins), int(np.float32(g.w[i]).view(np.uint32))))
+    head = f"{g.e} {k}" + (" 01" if weighted else "") if ext == ".hgr" else (k, g.e)
+    return head, sorted(recs)
+
+
+@pytest.mark.parametrize("case", SAVED)
+@pytest.mark.parametrize("ext", [".hgr", ".axh"])
+def test_oracle_quotient_matches_reference_saver(case, ext):
+    """The oracle's getPartitionsHypergraph restatement against the reference's own saved
+    quotient files (hedge order compared as a multiset, as the reference's is unordered)."""
+    c = DATA["saved"][case]
+    ref = c["files"][ext]
+    if ref is None:
+        pytest.skip("the reference refuses this output")
+    part = np.array(c["part"], np.uint32)
+    hg = O.HG(*H.Hypergraph.load(os.path.join(GOLD, DATA[case]["file"]), dedup=True).arrays())
+    assert oracle_quotient_records(O.quotient(hg, part), ext) == records(os.path.join(GOLD, ref))
+
+
+def test_oracle_quotient_refuses_partition_ids_with_gaps():
+    hg = O.HG(*H.Hypergraph.load(os.path.join(GOLD, "hgr_fmt1.hgr"), dedup=True).arrays())
+    part = np.zeros(hg.n, np.uint32)
+    part[::2] = 2
+    with pytest.raises(O.OracleError):
+        O.quotient(hg, part)

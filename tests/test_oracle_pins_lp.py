@@ -8,8 +8,8 @@ blocks by brute force on tiny inputs (CPU only).
   round plus everything proposed into it so far fits omega.
 * Label-propagation refinement never raises lambda-1 and keeps sizes consistent.
 * Best-fit decreasing with distinct sizes (where the seeded shuffle cannot matter) equals
-  the textbook rule: sizes descending, fullest bin that fits (ties: lower), otherwise the
-  lightest bin (ties: lower).
+  the textbook rule: sizes descending, the fullest bin that fits below the average load,
+  else below omega (ties: lower), otherwise the lightest bin (ties: lower).
 * The device initial partitioner's restatement places every node, reports its part
   weights, and is a pure function of its input.
 """
@@ -112,12 +112,14 @@ def test_lp_refine_never_raises_lambda(seed):
 
 
 def textbook_bfd(sizes, k, omega):
+    """Best fit decreasing against the average load first, then against omega."""
     order = sorted(range(len(sizes)), key=lambda v: -int(sizes[v]))
     loads = [0] * k
     part = [0] * len(sizes)
+    avg = min(omega, -(-int(np.sum(sizes)) // k))
     for v in order:
         s = int(sizes[v])
-        fit = [b for b in range(k) if loads[b] + s <= omega]
+        fit = [b for b in range(k) if loads[b] + s <= avg] or [b for b in range(k) if loads[b] + s <= omega]
         b = min(fit, key=lambda b: (-loads[b], b)) if fit else min(range(k), key=lambda b: (loads[b], b))
         part[v] = b
         loads[b] += s
@@ -141,8 +143,8 @@ def test_bfd_shuffle_orders_equal_sizes_only():
     sizes = np.array([2] * 10 + [5, 9], np.uint32)
     a, la = O.bfd_start(sizes, 3, 12, 0)
     b, lb = O.bfd_start(sizes, 3, 12, 5)
-    # the two largest items go first whatever the shuffle: 9 into bin 0, 5 into bin 1 (bin 0
-    # is fuller but 9 + 5 > 12)
+    # the two largest items go first whatever the shuffle: 9 into bin 0, 5 into bin 1 (the
+    # average load is ceil(34 / 3) = 12 and 9 + 5 > 12)
     assert a[11] == b[11] == 0 and a[10] == b[10] == 1
     for p, l in ((a, la), (b, lb)):
         assert np.bincount(p, weights=sizes, minlength=3).tolist() == l.tolist()
@@ -161,3 +163,13 @@ def test_device_initial_partitioner_invariants(k):
     assert np.array_equal(part, part2) and np.array_equal(ps, ps2)
     # small node weights and 5% slack: the packing fits, and the rounds after it keep sizes
     assert int(ps.max()) <= omega
+
+
+def test_bfd_uses_every_bin_when_slack_is_small():
+    """With 3% slack, packing against omega alone fills 63 of 64 bins; against the average
+    load every bin gets nodes."""
+    sizes = np.full(6400, 5, np.uint32)
+    k = 64
+    omega = int(np.ceil(1.03 * sizes.sum() / k))
+    part, loads = O.bfd_start(sizes, k, omega, 0)
+    assert (loads > 0).all() and int(loads.max()) <= omega
