@@ -1215,6 +1215,30 @@ __global__ void k_order_keys(uint32_t k, uint32_t repeat, uint32_t* __restrict__
   vals[q] = q;
 }
 
+__global__ void k_compact_u64(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint32_t n,
+                              const uint64_t* __restrict__ in, uint64_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) out[pos[i]] = in[i];
+}
+// the small-hedge CSR in first-pin order: sorted keys (first pin << 32 | hedge) -> lengths,
+// then pins and weights at the scanned offsets
+__global__ void k_small_lens(const uint64_t* __restrict__ keys, uint32_t m, const uint64_t* __restrict__ hoff,
+                             uint64_t* __restrict__ len) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t e = (uint32_t)keys[i];
+  len[i] = hoff[e + 1] - hoff[e];
+}
+__global__ void k_small_fill(const uint64_t* __restrict__ keys, uint32_t m, const uint64_t* __restrict__ hoff,
+                             const uint32_t* __restrict__ hedges, const float* __restrict__ w,
+                             const uint64_t* __restrict__ off, uint32_t* __restrict__ pins, float* __restrict__ sw) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t e = (uint32_t)keys[i];
+  const uint64_t s = hoff[e], len = hoff[e + 1] - s, o = off[i];
+  for (uint64_t j = 0; j < len; ++j) pins[o + j] = hedges[s + j];
+  sw[i] = w[e];
+}
 __global__ void k_compact(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint32_t n,
                           uint32_t* __restrict__ out) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1521,13 +1545,23 @@ struct CascadeArgs {
   const uint32_t* hedges;
   const uint64_t* hoff;
   const float* w;
-  const uint4* pm;  // per node: (partition, move target or NONE, rank, 0), one 16-byte gather per pin
+  // per node (partition, move target or NONE, rank): one 8-byte gather per pin when every
+  // partition id fits 16 bits (pm2: partition | target << 16, target NONE as 0xFFFF; rank),
+  // otherwise one 16-byte gather (pm: partition, target, rank, 0)
+  const uint4* pm;
+  const uint2* pm2;
   int incc;
   unsigned long long* seq;
 };
+constexpr uint32_t PM2_MAX_K = 0xFFFFu;  // partition ids < 0xFFFF leave 0xFFFF for "no move"
 
 // (partition, move target or NONE, rank) of node x
 __device__ __forceinline__ uint3 pm_at(const CascadeArgs& a, uint32_t x) {
+  if (a.pm2) {
+    const uint2 r = a.pm2[x];
+    const uint32_t q = r.x >> 16;
+    return make_uint3(r.x & 0xFFFFu, q == 0xFFFFu ? NONE : q, r.y);
+  }
   const uint4 r = a.pm[x];
   return make_uint3(r.x, r.y, r.z);
 }
@@ -1536,6 +1570,13 @@ __global__ void k_pack_nodes(const uint32_t* __restrict__ part, const uint32_t* 
                              const uint32_t* __restrict__ ranks, uint32_t n, uint4* __restrict__ pm) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) pm[v] = make_uint4(part[v], moves[v], ranks[v], 0u);
+}
+__global__ void k_pack_nodes2(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
+                              const uint32_t* __restrict__ ranks, uint32_t n, uint2* __restrict__ pm2) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t q = moves[v] == NONE ? 0xFFFFu : moves[v];
+  pm2[v] = make_uint2(part[v] | (q << 16), ranks[v]);
 }
 
 __device__ __forceinline__ int64_t cascade_term(int incc, int64_t W, int64_t cp, int64_t cq) {
@@ -1549,23 +1590,41 @@ __device__ __forceinline__ int64_t cascade_term(int incc, int64_t W, int64_t cp,
   return t;
 }
 
-// Hedges of at most CAS_SMALL pins, one thread each; larger ones are flagged for the sorted
-// kernels (up to CAS_SORT_S or sort_max pins) or the direct LDS/HBM kernel.
-__global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, uint32_t* __restrict__ f_s,
-                                uint32_t* __restrict__ f_m, uint32_t* __restrict__ f_big) {
+// Cascade classes of a level's hedges: 2..CAS_SMALL pins (the thread-per-hedge kernel; the
+// key orders them by their first pin), the sorted kernels (up to CAS_SORT_S or sort_max
+// pins) and the direct LDS/HBM kernel.  A hedge of one pin adds nothing: its pin finds itself
+// alone in p and nobody in q, so the two terms of cascade_term cancel.
+__global__ void k_cascade_classes(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff, uint32_t E,
+                                  uint32_t sort_max, uint32_t* __restrict__ f_small, uint64_t* __restrict__ k_small,
+                                  uint32_t* __restrict__ f_s, uint32_t* __restrict__ f_m, uint32_t* __restrict__ f_big) {
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
-  const uint64_t s = a.hoff[e], t = a.hoff[e + 1];
-  const uint64_t hl = t - s;
+  const uint64_t s = hoff[e], hl = hoff[e + 1] - s;
+  const bool small = hl >= 2 && hl <= CAS_SMALL;
   const bool sorted = hl > CAS_SMALL && hl <= sort_max;
-  if (f_s) {  // class flags (null once the level's classes are known)
-    f_s[e] = (sorted && hl <= CAS_SORT_S) ? 1u : 0u;
-    f_m[e] = (sorted && hl > CAS_SORT_S) ? 1u : 0u;
-    f_big[e] = (hl > CAS_SMALL && !sorted) ? 1u : 0u;
-  }
-  // a hedge of one pin adds nothing: its pin finds itself alone in p and nobody in q, so the
-  // two terms of cascade_term cancel
-  if (hl > CAS_SMALL || hl <= 1) return;
+  f_small[e] = small ? 1u : 0u;
+  k_small[e] = small ? ((uint64_t)hedges[s] << 32) | e : 0ull;
+  f_s[e] = (sorted && hl <= CAS_SORT_S) ? 1u : 0u;
+  f_m[e] = (sorted && hl > CAS_SORT_S) ? 1u : 0u;
+  f_big[e] = (hl > CAS_SMALL && !sorted) ? 1u : 0u;
+}
+
+// The level's hedges of 2..CAS_SMALL pins as their own CSR, in order of their first pin (built
+// once per level): offsets, pins and weights are read coalesced, and nearby threads gather the
+// records of nearby nodes.
+struct SmallHedges {
+  const uint64_t* off;
+  const uint32_t* pins;
+  const float* w;
+  uint32_t m;
+};
+
+// Hedges of 2..CAS_SMALL pins, one thread each, each XCD given a contiguous range of the
+// first-pin order so that the records its threads gather meet in one L2.
+__global__ void k_cascade_small(CascadeArgs a, SmallHedges h) {
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= h.m) return;
+  const uint64_t s = h.off[i], t = h.off[i + 1];
   uint32_t pt[CAS_SMALL], mv[CAS_SMALL], rk[CAS_SMALL];
 #pragma unroll
   for (uint32_t u = 0; u < CAS_SMALL; ++u) {  // entries no batch fills (u >= 4 when len <= 4)
@@ -1578,7 +1637,7 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
   // to the last pin), so a hedge costs two memory round trips per batch instead of two per pin;
   // entries past the end are marked as non-moving pins of no partition.
   auto fetch = [&](uint32_t u) {
-    const uint32_t x = a.hedges[s + min(u, len - 1)];
+    const uint32_t x = h.pins[s + min(u, len - 1)];
     return x;
   };
   auto put = [&](uint32_t u, uint3 q4) {
@@ -1605,7 +1664,7 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
 #pragma unroll
   for (uint32_t i = 0; i < CAS_SMALL; ++i) any |= mv[i] != NONE;
   if (!any) return;
-  const int64_t W = (int64_t)fixed_weight(a.w[e]);
+  const int64_t W = (int64_t)fixed_weight(h.w[i]);
   // Static indexes only (the arrays stay in registers); entries past a hedge's end take no
   // part.  The loops stop at the wave's longest hedge, rounded up to a multiple of 4.
   const uint32_t jmax = __ballot(len > 12) ? 16u : __ballot(len > 8) ? 12u : __ballot(len > 4) ? 8u : 4u;
@@ -1623,29 +1682,26 @@ __global__ void k_cascade_small(CascadeArgs a, uint32_t E, uint32_t sort_max, ui
           cq += now == q;
         }
       }
-      atomicAdd(&a.seq[r], (unsigned long long)cascade_term(a.incc, W, cp, cq));
+      const int64_t term = cascade_term(a.incc, W, cp, cq);
+      if (term) atomicAdd(&a.seq[r], (unsigned long long)term);
     }
   }
 }
 
 // Algorithmic bytes of one k_cascade_small launch, counted when that kernel is profiled (a
-// separate launch, outside the timed one).  Per hedge: its two offsets (8 B, shared with its
-// neighbours); per hedge of 2..CAS_SMALL pins: every pin id (4 B) and its node's (partition,
-// target, rank) (12 B), the weight (4 B) when a pin moves, and one 8-byte cascade term per
-// moving pin.
-__global__ __launch_bounds__(256) void k_cascade_small_bytes(CascadeArgs a, uint32_t E,
+// separate launch, outside the timed one).  Per hedge of 2..CAS_SMALL pins: its two offsets
+// (8 B, shared with its neighbours); every pin id (4 B) and its node's record (8 B packed, 16 B
+// otherwise); the weight (4 B) when a pin moves, and one 8-byte cascade term per moving pin.
+__global__ __launch_bounds__(256) void k_cascade_small_bytes(CascadeArgs a, SmallHedges h,
                                                              unsigned long long* __restrict__ prof) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t b = 0;
-  if (e < E) {
-    b = 8;
-    const uint64_t s = a.hoff[e], t = a.hoff[e + 1];
-    const uint64_t len = t - s;
-    if (len >= 2 && len <= CAS_SMALL) {
-      uint32_t moving = 0;
-      for (uint64_t i = s; i < t; ++i) moving += a.pm[a.hedges[i]].y != NONE;
-      b += 16 * len + (moving ? 4 + 8 * (uint64_t)moving : 0);
-    }
+  if (i0 < h.m) {
+    const uint64_t s = h.off[i0], t = h.off[i0 + 1];
+    const uint64_t len = t - s, rec = a.pm2 ? 8 : 16;
+    uint32_t moving = 0;
+    for (uint64_t i = s; i < t; ++i) moving += pm_at(a, h.pins[i]).y != NONE;
+    b = 8 + (4 + rec) * len + (moving ? 4 + 8 * (uint64_t)moving : 0);
   }
   __shared__ uint64_t part[4];
   const uint64_t ws = wave_sum(b);
@@ -2070,6 +2126,11 @@ struct LevelLayout {
   uint32_t cas_sort_max = 0xFFFFFFFFu;
   DBuf<uint32_t> cas_list[3];
   uint32_t cas_n[3] = {0, 0, 0};
+  // hedges of 2..CAS_SMALL pins as their own CSR, in order of their first pin
+  DBuf<uint64_t> cs_off;
+  DBuf<uint32_t> cs_pins;
+  DBuf<float> cs_w;
+  uint32_t cas_small_n = 0;
 };
 
 // masks: only the partition masks (k-way with k <= MASK_K: nothing else reads the lists).
@@ -2365,24 +2426,29 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
              (uint64_t)V);
   // cascade
   {
-    DBuf<uint4> pm(c, n);
-    HGP_LAUNCH(k_pack_nodes, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), n, pm.get());
-    CascadeArgs a{L.hedges, L.hoff, L.w, pm.get(), incc ? 1 : 0, st.seq.get()};
+    const bool packed = L.k < PM2_MAX_K;
+    DBuf<uint4> pm;
+    DBuf<uint2> pm2;
+    if (packed) {
+      pm2 = DBuf<uint2>(c, n);
+      HGP_LAUNCH(k_pack_nodes2, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), n, pm2.get());
+    } else {
+      pm = DBuf<uint4>(c, n);
+      HGP_LAUNCH(k_pack_nodes, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), n, pm.get());
+    }
+    CascadeArgs a{L.hedges, L.hoff, L.w, pm.get(), pm2.get(), incc ? 1 : 0, st.seq.get()};
     const int crb = std::max(1, bit_width(V));  // rank + 1 <= V
     const int cpb = std::max(1, bit_width(L.k));
     const uint32_t sort_max = cpb + crb + 11 < 64 ? CAS_SORT : 0u;
     LevelLayout local;
     LevelLayout& Y = layout ? *layout : local;
-    const bool known = Y.cas_sort_max == sort_max;
-    DBuf<uint32_t> fl[3];
-    if (!known)
-      for (int cls = 0; cls < 3; ++cls) fl[cls] = DBuf<uint32_t>(c, L.e);
-    const uint32_t cb = c.tune.cascade_block;  // 256 unless the autotuner chose otherwise
-    HGP_LAUNCH(k_cascade_small, ceil_div(L.e, cb), cb, 0, c, a, L.e, sort_max, fl[0].get(), fl[1].get(), fl[2].get());
-    if (prof_match(c, "k_cascade_small") && c.prof_count && c.prof_dev)
-      HGP_LAUNCH(k_cascade_small_bytes, ceil_div(L.e, 256), 256, 0, c, a, L.e, c.prof_dev);
-    if (!known) {
+    if (Y.cas_sort_max != sort_max) {  // the level's cascade classes (its first round)
       Y.cas_sort_max = sort_max;
+      DBuf<uint32_t> fl[4];
+      for (int cls = 0; cls < 4; ++cls) fl[cls] = DBuf<uint32_t>(c, L.e);
+      DBuf<uint64_t> ks(c, std::max<uint32_t>(L.e, 1));
+      HGP_LAUNCH(k_cascade_classes, ceil_div(L.e, 256), 256, 0, c, L.hedges, L.hoff, L.e, sort_max, fl[3].get(),
+                 ks.get(), fl[0].get(), fl[1].get(), fl[2].get());
       DBuf<uint64_t> pos(c, (uint64_t)L.e + 1);
       for (int cls = 0; cls < 3; ++cls) {
         exclusive_scan(c, fl[cls].get(), pos.get(), L.e);
@@ -2391,6 +2457,30 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
         if (Y.cas_n[cls])
           HGP_LAUNCH(k_compact, ceil_div(L.e, 256), 256, 0, c, fl[cls].get(), pos.get(), L.e, Y.cas_list[cls].get());
       }
+      // hedges of 2..CAS_SMALL pins as their own CSR, in order of their first pin
+      exclusive_scan(c, fl[3].get(), pos.get(), L.e);
+      const uint32_t m = (uint32_t)read_scalar(c, pos.get() + L.e);
+      Y.cas_small_n = m;
+      Y.cs_off = DBuf<uint64_t>(c, (uint64_t)m + 1);
+      if (m) {
+        DBuf<uint64_t> kc(c, m);
+        HGP_LAUNCH(k_compact_u64, ceil_div(L.e, 256), 256, 0, c, fl[3].get(), pos.get(), L.e, ks.get(), kc.get());
+        radix_sort(c, kc.get(), nullptr, m, 32 + std::max(1, bit_width(L.n)));
+        HGP_LAUNCH(k_small_lens, ceil_div(m, 256), 256, 0, c, kc.get(), m, L.hoff, Y.cs_off.get());
+        exclusive_scan(c, Y.cs_off.get(), Y.cs_off.get(), m);
+        const uint64_t sp = read_scalar(c, Y.cs_off.get() + m);
+        Y.cs_pins = DBuf<uint32_t>(c, sp);
+        Y.cs_w = DBuf<float>(c, m);
+        HGP_LAUNCH(k_small_fill, ceil_div(m, 256), 256, 0, c, kc.get(), m, L.hoff, L.hedges, L.w, Y.cs_off.get(),
+                   Y.cs_pins.get(), Y.cs_w.get());
+      }
+    }
+    if (Y.cas_small_n) {
+      const SmallHedges sh{Y.cs_off.get(), Y.cs_pins.get(), Y.cs_w.get(), Y.cas_small_n};
+      const uint32_t cb = c.tune.cascade_block;  // 256 unless the autotuner chose otherwise
+      HGP_LAUNCH(k_cascade_small, ceil_div(Y.cas_small_n, cb), cb, 0, c, a, sh);
+      if (prof_match(c, "k_cascade_small") && c.prof_count && c.prof_dev)
+        HGP_LAUNCH(k_cascade_small_bytes, ceil_div(Y.cas_small_n, 256), 256, 0, c, a, sh, c.prof_dev);
     }
     for (int cls = 0; cls < 3; ++cls) {
       const uint32_t m = Y.cas_n[cls];
@@ -2568,15 +2658,15 @@ uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rou
   return read_scalar(c, applied.get());
 }
 
-float time_cascade_small(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint4* pm,
-                         uint32_t e, uint32_t block, unsigned long long* seq) {
-  CascadeArgs a{hedges, hoff, w, pm, 0, seq};
+float time_cascade_small(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint2* pm2,
+                         uint32_t m, uint32_t block, unsigned long long* seq) {
+  CascadeArgs a{hedges, hoff, w, nullptr, pm2, 0, seq};
+  const SmallHedges sh{hoff, hedges, w, m};
   hipEvent_t t0, t1;
   HGP_HIP(hipEventCreate(&t0));
   HGP_HIP(hipEventCreate(&t1));
   HGP_HIP(hipEventRecord(t0, c.stream));
-  HGP_LAUNCH(k_cascade_small, ceil_div(e, block), block, 0, c, a, e, 0u, (uint32_t*)nullptr, (uint32_t*)nullptr,
-             (uint32_t*)nullptr);
+  HGP_LAUNCH(k_cascade_small, ceil_div(m, block), block, 0, c, a, sh);
   HGP_HIP(hipEventRecord(t1, c.stream));
   HGP_HIP(hipEventSynchronize(t1));
   float ms = 0;

@@ -52,13 +52,13 @@ __global__ void k_tune_pins(const uint64_t* __restrict__ off, uint32_t e, uint32
   for (uint64_t i = off[x]; i < off[x + 1]; ++i)
     pins[i] = (uint32_t)((first + tune_mix(seed + i) % 4096) % n);
 }
-__global__ void k_tune_pm(uint32_t n, uint4* __restrict__ pm) {
+__global__ void k_tune_pm(uint32_t n, uint2* __restrict__ pm2) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const uint64_t h = tune_mix(v);
   const uint32_t p = (uint32_t)(h & 63), q = (uint32_t)((h >> 8) & 63);
   const bool moves = ((h >> 16) % 10) < 4 && q != p;
-  pm[v] = make_uint4(p, moves ? q : NONE, v, 0u);  // rank = node id (a permutation)
+  pm2[v] = make_uint2(p | ((moves ? q : 0xFFFFu) << 16), v);  // rank = node id (a permutation)
 }
 // a heavy level: hub hedges of `len` pins drawn from all n nodes; every hedge also holds
 // hub node x % hubs, so the hubs see far more distinct neighbours than LDS tables hold
@@ -165,7 +165,7 @@ void tune_cascade(Ctx& c, uint32_t max_log2) {
   DBuf<uint32_t> ones(c, e);
   fill_u32(c, ones.get(), 0x3F800000u, e);  // 1.0f
   copy_d2d(c, w.get(), ones.get(), (uint64_t)e * 4);
-  DBuf<uint4> pm(c, n);
+  DBuf<uint2> pm(c, n);
   HGP_LAUNCH(k_tune_pm, ceil_div(n, 256), 256, 0, c, n, pm.get());
   DBuf<unsigned long long> seq(c, n);
   uint32_t best_b = 256;
