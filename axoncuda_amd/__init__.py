@@ -52,6 +52,7 @@ class Params(C.Structure):
         ("ip_rounds", C.c_uint32),
         ("ip_nodes_per_part", C.c_uint32),
         ("ip_refine_repeats", C.c_uint32),
+        ("refine_event_budget", C.c_uint64),
     ]
 
 
@@ -78,6 +79,8 @@ class Tuning(C.Structure):
         ("heavy_table_fill16", C.c_uint32),
         ("cascade_block", C.c_uint32),
         ("scan_lookback_tiles", C.c_uint32),
+        ("mask_wide4_degree", C.c_uint32),
+        ("mask_wide16_degree", C.c_uint32),
         ("tuned", C.c_uint32),
     ]
 
@@ -85,6 +88,7 @@ class Tuning(C.Structure):
         return {"sort_rounds": {i: int(r) for i, r in enumerate(self.sort_rounds) if r},
                 "heavy_table_slots": int(self.heavy_table_slots), "heavy_table_fill16": int(self.heavy_table_fill16),
                 "cascade_block": int(self.cascade_block), "scan_lookback_tiles": int(self.scan_lookback_tiles),
+                "mask_wide4_degree": int(self.mask_wide4_degree), "mask_wide16_degree": int(self.mask_wide16_degree),
                 "tuned": int(self.tuned)}
 
 

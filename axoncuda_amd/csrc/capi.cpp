@@ -101,6 +101,8 @@ void to_public(const hgp::Tuning& t, hgp_tuning* o) {
   o->heavy_table_fill16 = t.heavy_fill16;
   o->cascade_block = t.cascade_block;
   o->scan_lookback_tiles = t.scan_lookback_tiles;
+  o->mask_wide4_degree = t.mask_wide4_degree;
+  o->mask_wide16_degree = t.mask_wide16_degree;
   o->tuned = t.tuned ? 1u : 0u;
 }
 
@@ -116,6 +118,8 @@ bool from_public(const hgp_tuning* o, hgp::Tuning& t) {
   t.heavy_fill16 = o->heavy_table_fill16;
   t.cascade_block = b;
   t.scan_lookback_tiles = o->scan_lookback_tiles;
+  t.mask_wide4_degree = o->mask_wide4_degree;
+  t.mask_wide16_degree = o->mask_wide16_degree;
   t.tuned = o->tuned != 0;
   return true;
 }
@@ -711,6 +715,8 @@ hgp_status hgp_tuning_save(const hgp_tuning* t, const char* path) {
   f << "heavy_table_fill16 " << t->heavy_table_fill16 << "\n";
   f << "cascade_block " << t->cascade_block << "\n";
   f << "scan_lookback_tiles " << t->scan_lookback_tiles << "\n";
+  f << "mask_wide4_degree " << t->mask_wide4_degree << "\n";
+  f << "mask_wide16_degree " << t->mask_wide16_degree << "\n";
   f << "tuned " << t->tuned << "\n";
   return f.good() ? HGP_OK : HGP_E_IO;
 }
@@ -738,6 +744,8 @@ hgp_status hgp_tuning_load(const char* path, hgp_tuning* out) {
     else if (key == "heavy_table_fill16") t.heavy_table_fill16 = (uint32_t)a;
     else if (key == "cascade_block") t.cascade_block = (uint32_t)a;
     else if (key == "scan_lookback_tiles") t.scan_lookback_tiles = (uint32_t)a;
+    else if (key == "mask_wide4_degree") t.mask_wide4_degree = (uint32_t)a;
+    else if (key == "mask_wide16_degree") t.mask_wide16_degree = (uint32_t)a;
     else if (key == "tuned") t.tuned = (uint32_t)a;
     else return HGP_E_INVALID;
   }

@@ -59,6 +59,8 @@ struct Tuning {
   uint32_t heavy_fill16 = 12;        // its fill limit in sixteenths
   uint32_t cascade_block = 256;      // threads per workgroup of the thread-per-hedge cascade
   uint32_t scan_lookback_tiles = 64;  // largest scan (in tiles) that takes the single-pass look-back
+  uint32_t mask_wide4_degree = 128;   // mean touching hedges per node from which k-way gains take
+  uint32_t mask_wide16_degree = 512;  // 4 (16) waves per node
   bool tuned = false;
 };
 
@@ -92,6 +94,9 @@ struct Ctx {
   bool prof_count = false;  // kernels with a byte model count their bytes (extra device work)
   void* pinned = nullptr;  // small pinned host buffer for scalar read-backs (allocated on first use)
   Tuning tune;
+  // device bytes the refinement's inbound-set event stage may take at once (0: half of what
+  // the device has free, cached blocks included)
+  uint64_t event_budget = 0;
 };
 constexpr uint64_t PINNED_BYTES = 256;
 constexpr uint32_t PROF_SLOTS = 64;

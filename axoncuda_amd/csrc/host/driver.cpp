@@ -176,6 +176,12 @@ void partition(Ctx& c, const hgp_params& prm, const PipelineInput& in, uint32_t*
   // (sources/runconfig.cpp:190-192), so any other value is a caller error.
   HGP_REQUIRE(!kw || max_parts == prm.kway, HGP_E_INVALID, "k-way mode needs max_parts == kway");
   const uint32_t cnc = prm.candidates_count;
+  struct BudgetScope {  // the budget holds for this call only
+    Ctx& c;
+    uint64_t saved;
+    ~BudgetScope() { c.event_budget = saved; }
+  } budget_scope{c, c.event_budget};
+  c.event_budget = prm.refine_event_budget;
   DeviceIpOptions dip;
   if (prm.ip_starts) dip.starts = prm.ip_starts;
   if (prm.ip_rounds) dip.rounds = prm.ip_rounds;

@@ -959,6 +959,29 @@ __global__ void __launch_bounds__(64) k_hmask_large(const uint32_t* __restrict__
   }
 }
 
+// The mask path's node list is split by XCD: the router's workgroups take node ranges through
+// xcd_block, and the nodes one XCD routes go to that XCD's own sub-list, starting at the
+// first node of its range (a range holds at most that many listed nodes) with a counter of its
+// own.  The wave kernels' workgroups on XCD x consume sub-list x, so the hedge records of
+// neighbouring nodes are fetched into one L2.
+// Below XCD_SPLIT_GRID router workgroups the split would leave XCDs without nodes, so there is
+// one list (number 0) that every workgroup consumes.
+constexpr uint32_t XCDS = 8, XCD_SPLIT_GRID = 64;
+__device__ __forceinline__ uint32_t xcd_list_base(uint32_t x, uint32_t route_grid) {
+  const uint32_t q = route_grid / XCDS, r = route_grid % XCDS;
+  return (x * q + (x < r ? x : r)) * 256u;  // first node of XCD x's range (256 per workgroup)
+}
+// (list, first workgroup index, workgroups) a mask-kernel workgroup consumes
+struct SubList {
+  const uint32_t* list;
+  uint32_t count, first, step;
+};
+__device__ __forceinline__ SubList sub_list(const uint32_t* list, const uint32_t* count, uint32_t route_grid) {
+  if (route_grid < XCD_SPLIT_GRID) return SubList{list, count[0], blockIdx.x, gridDim.x};
+  const uint32_t x = blockIdx.x % XCDS;
+  return SubList{list + xcd_list_base(x, route_grid), count[x], blockIdx.x / XCDS, (gridDim.x - x + XCDS - 1) / XCDS};
+}
+
 // Routing for the mask path: a node touching at most GAIN_TINY hedges is scored right here,
 // one thread per node with its hedges' masks and weights in registers (batched loads): the
 // candidate targets are the bits of the masks' union, each scored by one pass over the
@@ -973,7 +996,8 @@ __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint
     s_ord[threadIdx.x] = a.order[threadIdx.x];
   }
   __syncthreads();
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool split = gridDim.x >= XCD_SPLIT_GRID;
+  const uint32_t v = (split ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x) * blockDim.x + threadIdx.x;
   uint64_t t0 = 0, t1 = 0;
   if (v < n) {
     t0 = a.toff[v];
@@ -1059,83 +1083,183 @@ __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint
     a.moves[v] = bq;
     a.gains[v] = bq == NONE ? 0 : bg;
   }
-  // list slots: one atomic per workgroup, then wave and lane offsets
+  // list slots in this XCD's sub-list: one atomic per workgroup, then wave and lane offsets
   const bool to_list = v < n && !small;
   __shared__ uint32_t s_cnt[4], s_base;
-  const uint32_t l = lane_id(), wv = threadIdx.x / 64;
+  const uint32_t l = lane_id(), wv = threadIdx.x / 64, xcd = split ? blockIdx.x % XCDS : 0u;
   const uint64_t ml = __ballot(to_list);
   if (l == 0) s_cnt[wv] = (uint32_t)__popcll(ml);
   __syncthreads();
   if (threadIdx.x == 0) {
     const uint32_t t = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    s_base = t ? atomicAdd(count, t) : 0u;
+    s_base = t ? atomicAdd(&count[xcd], t) : 0u;
   }
   __syncthreads();
-  uint32_t base = s_base;
+  uint32_t base = (split ? xcd_list_base(xcd, gridDim.x) : 0u) + s_base;
   for (uint32_t x = 0; x < wv; ++x) base += s_cnt[x];
   if (to_list) list[base + lanes_below(ml)] = v;
+}
+
+// Lane q holds partition q's connection; the wave picks the move (its lanes' per-launch
+// constants as in the mask kernels below) and lane 0 writes it.
+__device__ __forceinline__ void mask_pick(const GainArgs& a, uint32_t v, uint32_t p, uint32_t s, uint32_t l, bool qvalid,
+                                          uint32_t psz, uint32_t hq, uint32_t ord, int64_t base, int64_t conn) {
+  // best adjacent target: positive connection, room in the target; ties by the hash
+  const bool ok = qvalid && l != p && (uint32_t)(psz + s - s / a.discount) <= a.omega;
+  int64_t bg = base + conn;
+  uint32_t bh = hq, bq = (ok && conn > 0) ? l : NONE;
+  wave_best(bg, bh, bq);
+  if (bq == NONE) {  // first valid partition in hash order without a positive connection
+    const uint32_t qi = ord == NONE ? 0u : ord;
+    const uint32_t pq = __shfl(psz, (int)qi, 64);
+    const int64_t cq = __shfl(conn, (int)qi, 64);
+    const bool fit = ord != NONE && qi != p && (uint32_t)(pq + s - s / a.discount) <= a.omega && cq <= 0;
+    const uint64_t fm = __ballot(fit);
+    bq = fm ? __shfl(ord, __ffsll((long long)fm) - 1, 64) : NONE;
+    bg = base;
+  }
+  if (l == 0) {
+    a.moves[v] = bq;
+    a.gains[v] = bq == NONE ? 0 : bg;
+  }
+}
+// Touching hedges jb0, jb0 + stride, ... (64 at a time) of a node in partition p: lane q's
+// connection, and the lane's shares of the node's hedge weight and of its saving.
+__device__ __forceinline__ void mask_accumulate(const GainArgs& a, const uint4* __restrict__ hm,
+                                                const float* __restrict__ w, uint64_t jb0, uint64_t t1,
+                                                uint64_t stride, uint32_t p, uint32_t l, int64_t& conn, int64_t& wl,
+                                                int64_t& sl) {
+  for (uint64_t jb = jb0; jb < t1; jb += stride) {
+    const uint64_t j = jb + l;
+    uint32_t mlo = 0, mhi = 0, we = 0;
+    if (j < t1) {
+      const uint32_t e = a.tch[j];
+      const uint4 r = hm[e];
+      we = fixed_weight(w[e]);
+      mlo = r.x;
+      mhi = r.y;
+      const uint64_t sg = ((uint64_t)r.w << 32) | r.z;
+      wl += we;
+      if ((sg >> p) & 1ull) sl += we;
+    }
+    const uint32_t cnt = (uint32_t)min<uint64_t>(64, t1 - jb);
+    for (uint32_t i = 0; i < cnt; ++i) {
+      const uint32_t lo = __builtin_amdgcn_readlane(mlo, i), hi = __builtin_amdgcn_readlane(mhi, i);
+      const uint32_t wi = __builtin_amdgcn_readlane(we, i);
+      const uint32_t word = l < 32 ? lo : hi;
+      conn += ((word >> (l & 31)) & 1u) ? (int64_t)wi : 0;
+    }
+  }
 }
 
 // One wave per listed node, lane q holding partition q's connection.  The wave loads 64
 // touching hedges' records at a time; each record is then broadcast and every lane adds the
 // weight when its partition's bit is set.  psizes and the hash order are per-lane constants
-// of the launch.  Persistent: the grid strides over the `*count` listed nodes.
+// of the launch.  Persistent: the workgroups of XCD x (at least one: the grid is at least
+// XCDS) stride over sub-list x, or all of them over the one list.
 __global__ void __launch_bounds__(256) k_gains_mask(GainArgs a, const uint4* __restrict__ hm,
                                                     const float* __restrict__ w, const uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ count) {
+                                                    const uint32_t* __restrict__ count, uint32_t route_grid) {
   const uint32_t l = lane_id(), wv = threadIdx.x / 64;
   const bool qvalid = l < a.k;
   const uint32_t psz = qvalid ? a.psizes[l] : 0u;
   const uint32_t hq = mix32(l + a.repeat);
   const uint32_t ord = qvalid ? a.order[l] : NONE;  // lane i: the i-th partition in hash order
-  const uint32_t nlist = *count;
+  const SubList sl_ = sub_list(list, count, route_grid);
   uint64_t bytes = 0;
-  for (uint32_t idx = blockIdx.x * 4 + wv; idx < nlist; idx += gridDim.x * 4) {
-    const uint32_t v = list[idx];
+  for (uint32_t idx = sl_.first * 4 + wv; idx < sl_.count; idx += sl_.step * 4) {
+    const uint32_t v = sl_.list[idx];
     const uint32_t p = a.part[v], s = a.sizes[v];
     const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
     int64_t conn = 0, wl = 0, sl = 0;
-    for (uint64_t jb = t0; jb < t1; jb += 64) {
-      const uint64_t j = jb + l;
-      uint32_t mlo = 0, mhi = 0, we = 0;
-      if (j < t1) {
-        const uint32_t e = a.tch[j];
-        const uint4 r = hm[e];
-        we = fixed_weight(w[e]);
-        mlo = r.x;
-        mhi = r.y;
-        const uint64_t sg = ((uint64_t)r.w << 32) | r.z;
-        wl += we;
-        if ((sg >> p) & 1ull) sl += we;
-      }
-      const uint32_t cnt = (uint32_t)min<uint64_t>(64, t1 - jb);
-      for (uint32_t i = 0; i < cnt; ++i) {
-        const uint32_t lo = __builtin_amdgcn_readlane(mlo, i), hi = __builtin_amdgcn_readlane(mhi, i);
-        const uint32_t wi = __builtin_amdgcn_readlane(we, i);
-        const uint32_t word = l < 32 ? lo : hi;
-        conn += ((word >> (l & 31)) & 1u) ? (int64_t)wi : 0;
-      }
-    }
-    const int64_t base = wave_sum(sl) - wave_sum(wl);
-    // best adjacent target: positive connection, room in the target; ties by the hash
-    const bool ok = qvalid && l != p && (uint32_t)(psz + s - s / a.discount) <= a.omega;
-    int64_t bg = base + conn;
-    uint32_t bh = hq, bq = (ok && conn > 0) ? l : NONE;
-    wave_best(bg, bh, bq);
-    if (bq == NONE) {  // first valid partition in hash order without a positive connection
-      const uint32_t qi = ord == NONE ? 0u : ord;
-      const uint32_t pq = __shfl(psz, (int)qi, 64);
-      const int64_t cq = __shfl(conn, (int)qi, 64);
-      const bool fit = ord != NONE && qi != p && (uint32_t)(pq + s - s / a.discount) <= a.omega && cq <= 0;
-      const uint64_t fm = __ballot(fit);
-      bq = fm ? __shfl(ord, __ffsll((long long)fm) - 1, 64) : NONE;
-      bg = base;
-    }
-    if (l == 0) {
-      a.moves[v] = bq;
-      a.gains[v] = bq == NONE ? 0 : bg;
-    }
+    mask_accumulate(a, hm, w, t0, t1, 64, p, l, conn, wl, sl);
+    mask_pick(a, v, p, s, l, qvalid, psz, hq, ord, wave_sum(sl) - wave_sum(wl), conn);
     bytes += (l == 0 ? 36 : 0) + 24 * (uint64_t)((t1 - t0 + 63 - l) / 64);
+  }
+  if (a.prof_bytes) {
+    bytes = wave_sum(bytes);
+    if (l == 0 && bytes) atomicAdd(&a.prof_bytes[blockIdx.x % PROF_SLOTS], (unsigned long long)bytes);
+  }
+}
+// Nodes of thousands of touching hedges (the coarse levels): a wave per node would leave most
+// of the chip idle, and the heaviest node would set the launch's time.  Here W waves share a
+// node, wave u taking chunks u, u + W, ... of 64 hedges, and each lane adds its hedge's weight
+// to the LDS connection of every partition in its mask other than the node's own (most hedges
+// of a coarse node lie inside its partition and add nothing), instead of broadcasting every
+// record to all lanes.  The chunk after the current one is loaded while the current one is
+// added.  The sums are integers, so their order does not matter.
+template <uint32_t W>
+__global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const uint4* __restrict__ hm,
+                                                            const float* __restrict__ w,
+                                                            const uint32_t* __restrict__ list,
+                                                            const uint32_t* __restrict__ count, uint32_t route_grid) {
+  __shared__ unsigned long long s_conn[MASK_K];
+  __shared__ int64_t s_base[W];
+  const uint32_t l = lane_id(), wv = threadIdx.x / 64;
+  const bool qvalid = l < a.k;
+  const uint32_t psz = qvalid ? a.psizes[l] : 0u;
+  const uint32_t hq = mix32(l + a.repeat);
+  const uint32_t ord = qvalid ? a.order[l] : NONE;
+  const SubList sl_ = sub_list(list, count, route_grid);
+  if (threadIdx.x < MASK_K) s_conn[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t bytes = 0;
+  for (uint32_t idx = sl_.first; idx < sl_.count; idx += sl_.step) {  // uniform over the workgroup
+    const uint32_t v = sl_.list[idx];
+    const uint32_t p = a.part[v], s = a.sizes[v];
+    const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
+    const uint64_t stride = 64 * W;
+    const uint64_t notp = ~(1ull << p);
+    int64_t wl = 0, sl = 0;
+    // lane l walks entries j, j + stride, ...: the list entry two chunks ahead and the record
+    // one chunk ahead are in flight while the current record is added
+    uint64_t j = t0 + 64 * wv + l;
+    uint32_t e0 = j < t1 ? a.tch[j] : NONE, e1 = j + stride < t1 ? a.tch[j + stride] : NONE;
+    uint4 r0 = make_uint4(0, 0, 0, 0);
+    float w0 = 0.0f;
+    if (e0 != NONE) {
+      r0 = hm[e0];
+      w0 = w[e0];
+    }
+    while (e0 != NONE) {
+      const uint32_t e2 = j + 2 * stride < t1 ? a.tch[j + 2 * stride] : NONE;
+      uint4 r1 = make_uint4(0, 0, 0, 0);
+      float w1 = 0.0f;
+      if (e1 != NONE) {
+        r1 = hm[e1];
+        w1 = w[e1];
+      }
+      const uint32_t we = fixed_weight(w0);
+      const uint64_t sg = ((uint64_t)r0.w << 32) | r0.z;
+      wl += we;
+      if ((sg >> p) & 1ull) sl += we;
+      uint64_t m = (((uint64_t)r0.y << 32) | r0.x) & notp;
+      if (we)
+        while (m) {
+          const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
+          m &= m - 1;
+          atomicAdd(&s_conn[q], (unsigned long long)we);
+        }
+      e0 = e1;
+      e1 = e2;
+      r0 = r1;
+      w0 = w1;
+      j += stride;
+    }
+    const int64_t b = wave_sum(sl) - wave_sum(wl);
+    if (l == 0) s_base[wv] = b;
+    __syncthreads();
+    if (wv == 0) {
+      int64_t bs = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < W; ++u) bs += s_base[u];
+      const int64_t cs = l == p ? 0 : (int64_t)s_conn[l];  // the own partition's sum is not kept
+      s_conn[l] = 0;
+      mask_pick(a, v, p, s, l, qvalid, psz, hq, ord, bs, cs);
+    }
+    __syncthreads();  // the sums are cleared before the next node adds to them
+    const uint64_t d = t1 - t0, mine = d > 64 * wv + l ? (d - 64 * wv - l + 64 * W - 1) / (64 * W) : 0;
+    bytes += (threadIdx.x == 0 ? 36 : 0) + 24 * mine;
   }
   if (a.prof_bytes) {
     bytes = wave_sum(bytes);
@@ -1897,29 +2021,39 @@ __global__ void k_size_flags(const uint64_t* __restrict__ keys, const uint32_t* 
 }
 
 // inbound events: (partition, hedge, rank) +-1 for every inbound hedge of every move
-__global__ void k_inbound_event_counts(const uint32_t* __restrict__ moves, const uint32_t* __restrict__ inb, uint32_t n,
+// events of partitions in [plo, phi) only (the stage runs in chunks of partitions when all
+// events at once would not fit the device)
+__global__ void k_inbound_event_counts(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
+                                       const uint32_t* __restrict__ inb, uint32_t n, uint32_t plo, uint32_t phi,
                                        uint32_t* __restrict__ cnt) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < n) cnt[v] = moves[v] == NONE ? 0u : 2u * inb[v];
+  if (v >= n) return;
+  const uint32_t q = moves[v], p = part[v];
+  const uint32_t in = q == NONE ? 0u : (uint32_t)(p >= plo && p < phi) + (uint32_t)(q >= plo && q < phi);
+  cnt[v] = in * inb[v];
 }
 __global__ void k_inbound_events(const uint32_t* __restrict__ part, const uint32_t* __restrict__ moves,
                                  const uint32_t* __restrict__ ranks, const uint32_t* __restrict__ tch,
                                  const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
-                                 const uint64_t* __restrict__ eoff, uint32_t n, int eb, int rb,
-                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                 const uint64_t* __restrict__ eoff, uint32_t n, int eb, int rb, uint32_t plo,
+                                 uint32_t phi, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const uint32_t q = moves[v];
   if (q == NONE) return;
   const uint32_t p = part[v], r = ranks[v];
+  const bool pin = p >= plo && p < phi, qin = q >= plo && q < phi;
   uint64_t o = eoff[v];
   for (uint64_t j = toff[v]; j < toff[v] + inb[v]; ++j) {
     const uint64_t e = tch[j];
-    keys[o] = ((uint64_t)p << (eb + rb)) | (e << rb) | r;
-    vals[o] = 0xFFFFFFFFu;  // -1
-    keys[o + 1] = ((uint64_t)q << (eb + rb)) | (e << rb) | r;
-    vals[o + 1] = 1u;
-    o += 2;
+    if (pin) {
+      keys[o] = ((uint64_t)p << (eb + rb)) | (e << rb) | r;
+      vals[o++] = 0xFFFFFFFFu;  // -1
+    }
+    if (qin) {
+      keys[o] = ((uint64_t)q << (eb + rb)) | (e << rb) | r;
+      vals[o++] = 1u;
+    }
   }
 }
 // count of destination pins of hedge e in partition q (0 when absent).  The lists start at
@@ -2107,6 +2241,7 @@ struct PartitionLists {
   DBuf<uint4> hinfo;   // per hedge: list start (lo, hi) or its one entry, length, fixed-point weight
   DBuf<uint32_t> dcnt, lambda, inbsz;
   DBuf<uint4> hm;      // instead of the lists (k-way, k <= MASK_K): per hedge, its partition masks
+  uint64_t P = 0;      // pins of the level
 };
 
 // Partition lists of every hedge.  want_inbound adds the destination counts and the
@@ -2140,6 +2275,7 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
   LevelLayout& Y = layout ? *layout : local;
   if (!Y.ready) Y.P = L.e ? read_scalar(c, L.hoff + L.e) : 0;
   const uint64_t P = Y.P;
+  pl.P = P;
   if (masks) {
     DBuf<uint32_t> lflag;
     if (!Y.ready) lflag = DBuf<uint32_t>(c, L.e);
@@ -2234,6 +2370,20 @@ void build_partition_lists(Ctx& c, const RefineLevel& L, const uint32_t* part, P
 
 uint32_t discount_of(uint32_t rep, uint32_t R) { return rep < R / 3 ? 1u : (rep < 2 * R / 3 ? 2u : NONE); }
 
+// peak device bytes per inbound-set event: keys and values with their sort buffers and tile
+// counts, running sums, transition flags, deltas, group starts and positions
+constexpr uint64_t INBOUND_EVENT_BYTES = 64;
+
+uint64_t event_budget(Ctx& c) {
+  if (c.event_budget) return c.event_budget;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 16ull << 30;
+  }
+  return std::max<uint64_t>(1ull << 26, ((uint64_t)free_b + c.cached) / 2);
+}
+
 
 // Algorithmic bytes of the gain kernels over `list` (evaluated only while the kernel is
 // profiled): per node 36 (partition, size, two offsets, move and gain out) + 20 per touching
@@ -2263,12 +2413,26 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
   a.moves = moves;
   a.gains = gains;
   if (pl.hm.get()) {  // k-way from partition masks: tiny nodes in the router, the rest a wave each
-    DBuf<uint32_t> ml(c, std::max<uint32_t>(L.n, 1)), mc(c, 1);
-    memset_async(c, mc.get(), 0, 4);
-    HGP_LAUNCH(k_gains_route_mask, ceil_div(L.n, 256), 256, 0, c, a, pl.hm.get(), L.w, L.n, ml.get(), mc.get());
+    // the sub-lists start at their XCD's first node: room for the route grid's 256 nodes per workgroup
+    const uint32_t rg = std::max<uint32_t>(1, ceil_div(L.n, 256));
+    DBuf<uint32_t> ml(c, (uint64_t)rg * 256), mc(c, XCDS);
+    memset_async(c, mc.get(), 0, 4 * XCDS);
+    HGP_LAUNCH(k_gains_route_mask, rg, 256, 0, c, a, pl.hm.get(), L.w, L.n, ml.get(), mc.get());
     a.prof_bytes = prof_match(c, "k_gains_mask") && c.prof_count ? c.prof_dev : nullptr;
-    HGP_LAUNCH(k_gains_mask, std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID)), 256, 0, c, a,
-               pl.hm.get(), L.w, ml.get(), mc.get());
+    // waves per node from the level's mean degree: a wave per node while the nodes are many
+    // and light, 4 or 16 when a wave would walk thousands of hedges alone
+    const uint64_t mean = L.n ? pl.P / L.n : 0;
+    const uint32_t W = mean >= c.tune.mask_wide16_degree ? 16 : (mean >= c.tune.mask_wide4_degree ? 4 : 1);
+    // every XCD needs a workgroup for its sub-list: grids of at least XCDS
+    if (W == 16)
+      HGP_LAUNCH(k_gains_mask_wide<16>, std::max<uint32_t>(XCDS, std::min<uint32_t>(L.n, LIGHT_GRID)), 1024, 0, c, a,
+                 pl.hm.get(), L.w, ml.get(), mc.get(), rg);
+    else if (W == 4)
+      HGP_LAUNCH(k_gains_mask_wide<4>, std::max<uint32_t>(XCDS, std::min<uint32_t>(L.n, LIGHT_GRID)), 256, 0, c, a,
+                 pl.hm.get(), L.w, ml.get(), mc.get(), rg);
+    else
+      HGP_LAUNCH(k_gains_mask, std::max<uint32_t>(XCDS, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID)), 256, 0, c,
+                 a, pl.hm.get(), L.w, ml.get(), mc.get(), rg);
     return;
   }
   // tiny nodes are scored by the router; the light and block lists and their counts stay on
@@ -2517,14 +2681,31 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
     HGP_REQUIRE(kb + eb + rb <= 64, HGP_E_CAPACITY, "inbound events: key exceeds 64 bits");
     DBuf<uint32_t> ecnt(c, n);
     DBuf<uint64_t> eoff(c, (uint64_t)n + 1);
-    HGP_LAUNCH(k_inbound_event_counts, g, 256, 0, c, st.moves.get(), L.inb, n, ecnt.get());
+    // All events at once need about INBOUND_EVENT_BYTES each (keys and values with their
+    // sort buffers, running sums, transition flags, deltas, group starts, positions); when that
+    // exceeds the event budget the partitions are split into ranges whose events are processed
+    // one range after the other.  Every event stays with its partition, so the violations
+    // summed over the ranges are exact.
+    HGP_LAUNCH(k_inbound_event_counts, g, 256, 0, c, part, st.moves.get(), L.inb, n, 0u, L.k, ecnt.get());
     exclusive_scan(c, ecnt.get(), eoff.get(), n);
-    const uint64_t m = read_scalar(c, eoff.get() + n);
-    if (m) {
+    const uint64_t m_all = read_scalar(c, eoff.get() + n);
+    const uint64_t budget = event_budget(c);
+    const uint32_t chunks = (uint32_t)std::min<uint64_t>(
+        std::max<uint32_t>(L.k, 1), std::max<uint64_t>(1, (m_all * INBOUND_EVENT_BYTES + budget - 1) / budget));
+    if (c.verbose >= 2 && chunks > 1)
+      fprintf(stderr, "  inbound events: %llu in %u partition ranges\n", (unsigned long long)m_all, chunks);
+    for (uint32_t ch = 0; ch < chunks && m_all; ++ch) {
+      const uint32_t plo = (uint32_t)((uint64_t)L.k * ch / chunks), phi = (uint32_t)((uint64_t)L.k * (ch + 1) / chunks);
+      if (chunks > 1) {
+        HGP_LAUNCH(k_inbound_event_counts, g, 256, 0, c, part, st.moves.get(), L.inb, n, plo, phi, ecnt.get());
+        exclusive_scan(c, ecnt.get(), eoff.get(), n);
+      }
+      const uint64_t m = chunks > 1 ? read_scalar(c, eoff.get() + n) : m_all;
+      if (!m) continue;
       DBuf<uint64_t> keys(c, m);
       DBuf<uint32_t> vals(c, m);
       HGP_LAUNCH(k_inbound_events, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), L.tch, L.toff, L.inb, eoff.get(), n,
-                 eb, rb, keys.get(), vals.get());
+                 eb, rb, plo, phi, keys.get(), vals.get());
       radix_sort(c, keys.get(), vals.get(), m, kb + eb + rb);
       DBuf<int64_t> cum(c, m);
       HGP_LAUNCH(k_to_i64, ceil_div(m, 256), 256, 0, c, vals.get(), m, cum.get());

@@ -89,6 +89,11 @@ typedef struct hgp_params {
   uint32_t ip_rounds;
   uint32_t ip_nodes_per_part;
   uint32_t ip_refine_repeats;
+  /* INCC refinement: device bytes its inbound-set event stage may take at once; the events of
+   * a round that need more are processed in ranges of partitions, one after the other, with
+   * the same result (0, the default: half of the device memory free at that moment).  The
+   * reference sizes this stage without a bound (sources/refinement.cu:307-491). */
+  uint64_t refine_event_budget;
 } hgp_params;
 
 enum { HGP_IP_HOST = 0, HGP_IP_DEVICE = 1, HGP_IP_AUTO = 2 };
@@ -112,6 +117,10 @@ typedef struct hgp_tuning {
   /* scans of more than this many 4096-element tiles take three passes instead of the
    * single-pass look-back */
   uint32_t scan_lookback_tiles;
+  /* k-way gains from partition masks (k <= 64): the mean touching hedges per node of a level
+   * from which each node takes 4 waves, and from which it takes 16, instead of one */
+  uint32_t mask_wide4_degree;
+  uint32_t mask_wide16_degree;
   /* 1 when set by hgp_autotune */
   uint32_t tuned;
 } hgp_tuning;

@@ -49,6 +49,22 @@ def test_incc_pipeline_matches_oracle(dev, ipm, rng):
     assert O.validity(orc, part, omega, delta, maxp) == 1
 
 
+@pytest.mark.parametrize("budget", [1, 40000])
+def test_incc_inbound_events_in_partition_ranges(dev, budget, rng):
+    """An event budget far below what a round's inbound-set events need splits them into
+    ranges of partitions (one partition per range at a budget of one byte) processed one
+    after the other: the partition is still the oracle's."""
+    n, off, pins, srcc, w = gen.snn_like(6, 800, 12, rng)
+    prod, orc = hg_pair(n, off, pins, srcc, w)
+    omega, delta, maxp = 64, 600, 4800
+    prm = G.params(H.HGP_MODE_INCC, omega, delta, maxp)
+    prm.refine_event_budget = budget
+    print(f"case: INCC pipeline event budget {budget}", flush=True)
+    part, _ = G.partition(dev, prod, prm)
+    exp, _ = O.partition(orc, 0, omega, delta, maxp)
+    assert np.array_equal(part, exp)
+
+
 def test_pipeline_deterministic(dev, rng):
     n, off, pins, srcc, w = gen.random_hypergraph(30000, 32000, rng, max_size=12, weights="int")
     prod, orc = hg_pair(n, off, pins, srcc, w)

@@ -29,6 +29,8 @@ def extremes():
         t.heavy_table_slots = 1 << 13
         t.heavy_table_fill16 = 4
         t.scan_lookback_tiles = 0
+        t.mask_wide4_degree = 0  # every level's mask gains 16 waves per node
+        t.mask_wide16_degree = 0
 
     def large(t):
         for i in range(40):
@@ -37,8 +39,14 @@ def extremes():
         t.heavy_table_slots = 1 << 20
         t.heavy_table_fill16 = 14
         t.scan_lookback_tiles = 1 << 20
+        t.mask_wide4_degree = 1 << 31  # a wave per node on every level
+        t.mask_wide16_degree = 1 << 31
 
-    return [("small", small), ("large", large)]
+    def four_waves(t):
+        t.mask_wide4_degree = 0  # 4 waves per node on every level
+        t.mask_wide16_degree = 1 << 31
+
+    return [("small", small), ("large", large), ("four_waves", four_waves)]
 
 
 def with_tuning(dev, setter, fn):
