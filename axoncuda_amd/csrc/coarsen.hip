@@ -35,16 +35,63 @@ constexpr uint32_t CAND_DIRECT_N = 39168;   // levels up to this size index LDS 
 constexpr uint64_t HEAVY_TABLE_SLOTS = 64ull << 20;  // HBM table slots shared by the heavy grid
 
 // Pins visited when scanning a node's touching hedges (itself excluded once per hedge).
-__global__ void k_visit_bound(const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ tch,
-                              const uint64_t* __restrict__ toff, uint32_t n, uint64_t* __restrict__ bound) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  uint64_t b = 0;
-  for (uint64_t j = toff[v]; j < toff[v + 1]; ++j) {
-    uint32_t e = tch[j];
-    b += hoff[e + 1] - hoff[e] - 1;
+// A node touching at most VISIT_LIGHT hedges is bounded by its own thread; the others (the
+// coarse levels' nodes touch thousands) are listed for k_visit_bound_heavy.
+constexpr uint32_t VISIT_LIGHT = 32, VISIT_HEAVY_GRID = 4096;
+__global__ void __launch_bounds__(256) k_visit_bound(const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ tch,
+                                                     const uint64_t* __restrict__ toff, uint32_t n,
+                                                     uint64_t* __restrict__ bound, uint32_t* __restrict__ heavy,
+                                                     uint32_t* __restrict__ nheavy) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  bool to_list = false;
+  if (v < n) {
+    const uint64_t t0 = toff[v], t1 = toff[v + 1];
+    if (t1 - t0 <= VISIT_LIGHT) {
+      uint64_t b = 0;
+      for (uint64_t j = t0; j < t1; ++j) {
+        const uint32_t e = tch[j];
+        b += hoff[e + 1] - hoff[e] - 1;
+      }
+      bound[v] = b;
+    } else {
+      to_list = true;
+    }
   }
-  bound[v] = b;
+  const uint64_t m = __ballot(to_list);
+  if (!m) return;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(nheavy, (uint32_t)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (to_list) heavy[base + lanes_below(m)] = v;
+}
+// One 256-thread workgroup per listed node (persistent over the list): each thread takes every
+// 256th touching hedge, four loads in flight at a time.
+__global__ void __launch_bounds__(256) k_visit_bound_heavy(const uint64_t* __restrict__ hoff,
+                                                           const uint32_t* __restrict__ tch,
+                                                           const uint64_t* __restrict__ toff,
+                                                           const uint32_t* __restrict__ heavy,
+                                                           const uint32_t* __restrict__ nheavy,
+                                                           uint64_t* __restrict__ bound) {
+  __shared__ uint64_t s_part[4];
+  const uint32_t cnt = *nheavy, wv = threadIdx.x / 64;
+  for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {  // uniform over the workgroup
+    const uint32_t v = heavy[i];
+    const uint64_t t0 = toff[v], t1 = toff[v + 1];
+    uint64_t b = 0;
+    for (uint64_t j = t0 + threadIdx.x; j < t1; j += 4 * 256) {
+      uint32_t e[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) e[u] = j + u * 256 < t1 ? tch[j + u * 256] : NONE;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+        if (e[u] != NONE) b += hoff[e[u] + 1] - hoff[e[u]] - 1;
+    }
+    b = wave_sum(b);
+    if (lane_id() == 0) s_part[wv] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) bound[v] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    __syncthreads();
+  }
 }
 
 // candidate key = (final score << 32) | v, or 0 when v is filtered out
@@ -642,7 +689,13 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   const uint32_t g = ceil_div(n, 256);
   DBuf<uint64_t> bound(c, n);
   DBuf<uint32_t> fl(c, n), fm(c, n), fb(c, n), caps(c, n);
-  HGP_LAUNCH(k_visit_bound, g, 256, 0, c, hoff, tch, toff, n, bound.get());
+  {
+    DBuf<uint32_t> heavy(c, n), nheavy(c, 1);
+    memset_async(c, nheavy.get(), 0, 4);
+    HGP_LAUNCH(k_visit_bound, g, 256, 0, c, hoff, tch, toff, n, bound.get(), heavy.get(), nheavy.get());
+    HGP_LAUNCH(k_visit_bound_heavy, std::min<uint32_t>(n, VISIT_HEAVY_GRID), 256, 0, c, hoff, tch, toff, heavy.get(),
+               nheavy.get(), bound.get());
+  }
   HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, direct_n, big_cap, fl.get(), fm.get(), fb.get(), caps.get());
   DBuf<uint64_t> lpos(c, (uint64_t)n + 1), mpos(c, (uint64_t)n + 1), bpos(c, (uint64_t)n + 1);
   exclusive_scan(c, fl.get(), lpos.get(), n);
