@@ -81,6 +81,7 @@ class Tuning(C.Structure):
         ("scan_lookback_tiles", C.c_uint32),
         ("mask_wide4_degree", C.c_uint32),
         ("mask_wide16_degree", C.c_uint32),
+        ("side_streams", C.c_uint32),
         ("tuned", C.c_uint32),
     ]
 
@@ -89,7 +90,7 @@ class Tuning(C.Structure):
                 "heavy_table_slots": int(self.heavy_table_slots), "heavy_table_fill16": int(self.heavy_table_fill16),
                 "cascade_block": int(self.cascade_block), "scan_lookback_tiles": int(self.scan_lookback_tiles),
                 "mask_wide4_degree": int(self.mask_wide4_degree), "mask_wide16_degree": int(self.mask_wide16_degree),
-                "tuned": int(self.tuned)}
+                "side_streams": int(self.side_streams), "tuned": int(self.tuned)}
 
 
 # (name, restype, argtypes) of every exported entry point of include/hgp.h

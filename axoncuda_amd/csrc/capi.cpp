@@ -103,6 +103,7 @@ void to_public(const hgp::Tuning& t, hgp_tuning* o) {
   o->scan_lookback_tiles = t.scan_lookback_tiles;
   o->mask_wide4_degree = t.mask_wide4_degree;
   o->mask_wide16_degree = t.mask_wide16_degree;
+  o->side_streams = t.side_streams;
   o->tuned = t.tuned ? 1u : 0u;
 }
 
@@ -120,6 +121,7 @@ bool from_public(const hgp_tuning* o, hgp::Tuning& t) {
   t.scan_lookback_tiles = o->scan_lookback_tiles;
   t.mask_wide4_degree = o->mask_wide4_degree;
   t.mask_wide16_degree = o->mask_wide16_degree;
+  t.side_streams = o->side_streams & 3u;
   t.tuned = o->tuned != 0;
   return true;
 }
@@ -219,6 +221,7 @@ void hgp_ctx_destroy(hgp_ctx* ctx) {
   for (auto& kv : ctx->c.live) (void)hipFreeAsync(kv.first, ctx->c.stream);
   hgp::release_cached(ctx->c);
   (void)hipStreamSynchronize(ctx->c.stream);
+  hgp::destroy_sides(ctx->c);
   if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
   if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
   delete ctx;
@@ -717,6 +720,7 @@ hgp_status hgp_tuning_save(const hgp_tuning* t, const char* path) {
   f << "scan_lookback_tiles " << t->scan_lookback_tiles << "\n";
   f << "mask_wide4_degree " << t->mask_wide4_degree << "\n";
   f << "mask_wide16_degree " << t->mask_wide16_degree << "\n";
+  f << "side_streams " << t->side_streams << "\n";
   f << "tuned " << t->tuned << "\n";
   return f.good() ? HGP_OK : HGP_E_IO;
 }
@@ -746,6 +750,7 @@ hgp_status hgp_tuning_load(const char* path, hgp_tuning* out) {
     else if (key == "scan_lookback_tiles") t.scan_lookback_tiles = (uint32_t)a;
     else if (key == "mask_wide4_degree") t.mask_wide4_degree = (uint32_t)a;
     else if (key == "mask_wide16_degree") t.mask_wide16_degree = (uint32_t)a;
+    else if (key == "side_streams") t.side_streams = (uint32_t)a;
     else if (key == "tuned") t.tuned = (uint32_t)a;
     else return HGP_E_INVALID;
   }

@@ -708,17 +708,24 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   DBuf<uint32_t> light(c, nl), med(c, nm), big(c, nb), heavy(c, nh);
   HGP_LAUNCH(k_split4, g, 256, 0, c, fl.get(), fm.get(), fb.get(), lpos.get(), mpos.get(), bpos.get(), n, light.get(),
              med.get(), big.get(), heavy.get());
-  if (inter)
-    HGP_LAUNCH(k_cand_light<true>, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, light.get(),
-               nl, bound.get(), cnc, omega, delta, pairs, scores);
-  else
-    HGP_LAUNCH(k_cand_light<false>, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, light.get(),
-               nl, bound.get(), cnc, omega, delta, pairs, scores);
+  // the tiers write disjoint rows of pairs and scores: light and medium nodes on side streams
+  // beside the big and heavy ones (which allocate, so they stay on the main stream)
+  Fork fk(c, c.tune.side_streams & 1u);
+  fk.on(0, [&] {
+    if (inter)
+      HGP_LAUNCH(k_cand_light<true>, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes,
+                 light.get(), nl, bound.get(), cnc, omega, delta, pairs, scores);
+    else
+      HGP_LAUNCH(k_cand_light<false>, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes,
+                 light.get(), nl, bound.get(), cnc, omega, delta, pairs, scores);
+  });
 #define CAND_LAUNCH(K, list, cnt, threads)                                                                          \
   HGP_LAUNCH(K, cnt, threads, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, list, caps.get(), cnc, omega, delta, \
              pairs, scores)
-  if (inter) CAND_LAUNCH(k_cand_medium<true>, med.get(), nm, 256);
-  else CAND_LAUNCH(k_cand_medium<false>, med.get(), nm, 256);
+  fk.on(1, [&] {
+    if (inter) CAND_LAUNCH(k_cand_medium<true>, med.get(), nm, 256);
+    else CAND_LAUNCH(k_cand_medium<false>, med.get(), nm, 256);
+  });
   if (n <= direct_n) {
     if (inter) CAND_LAUNCH(k_cand_direct<true>, big.get(), nb, 1024);
     else CAND_LAUNCH(k_cand_direct<false>, big.get(), nb, 1024);

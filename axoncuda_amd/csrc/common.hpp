@@ -61,6 +61,7 @@ struct Tuning {
   uint32_t scan_lookback_tiles = 64;  // largest scan (in tiles) that takes the single-pass look-back
   uint32_t mask_wide4_degree = 128;   // mean touching hedges per node from which k-way gains take
   uint32_t mask_wide16_degree = 512;  // 4 (16) waves per node
+  uint32_t side_streams = 2;          // bit 0: candidate tiers, bit 1: cascade classes beside events
   bool tuned = false;
 };
 
@@ -97,7 +98,47 @@ struct Ctx {
   // device bytes the refinement's inbound-set event stage may take at once (0: half of what
   // the device has free, cached blocks included)
   uint64_t event_budget = 0;
+  // side streams for independent kernels of one stage (created on first use, default priority)
+  static constexpr int SIDES = 3;
+  hipStream_t side[SIDES] = {};
+  hipEvent_t fork_ev = nullptr, side_ev[SIDES] = {};
 };
+
+// Independent kernels of one stage on the context's side streams, beside the main stream's.
+// The side streams start after everything queued on the main stream so far; when the Fork
+// ends, the main stream waits for all work given to them.  Work given to a side stream must
+// not allocate or free device memory (the caching allocator is ordered by the main stream) nor
+// use the context's scans or sorts (their status arrays are shared), and the buffers it reads
+// or writes must outlive the Fork.
+class Fork {
+ public:
+  explicit Fork(Ctx& c, bool enabled = true);
+  ~Fork();
+  template <typename F>
+  void on(int i, F&& f) {  // f's launches go to side stream i (the main stream when disabled)
+    if (!enabled_) {
+      f();
+      return;
+    }
+    start(i);
+    struct Restore {
+      Ctx& c;
+      hipStream_t s;
+      ~Restore() { c.stream = s; }
+    } restore{c_, c_.stream};
+    c_.stream = c_.side[i];
+    f();
+  }
+  Fork(const Fork&) = delete;
+  Fork& operator=(const Fork&) = delete;
+
+ private:
+  void start(int i);
+  Ctx& c_;
+  bool enabled_;
+  uint32_t used_ = 0;
+};
+void destroy_sides(Ctx& c);
 constexpr uint64_t PINNED_BYTES = 256;
 constexpr uint32_t PROF_SLOTS = 64;
 

@@ -635,6 +635,44 @@ void radix_sort_range_buf(Ctx& c, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, ui
   range_impl(c, keys.get(), vals.get(), n, &keys, &vals);
 }
 
+Fork::Fork(Ctx& c, bool enabled) : c_(c), enabled_(enabled) {
+  if (!enabled_) return;
+  if (!c_.fork_ev) HGP_HIP(hipEventCreateWithFlags(&c_.fork_ev, hipEventDisableTiming));
+  HGP_HIP(hipEventRecord(c_.fork_ev, c_.stream));
+}
+void Fork::start(int i) {
+  HGP_REQUIRE(i >= 0 && i < Ctx::SIDES, HGP_E_INVALID, "Fork: no such side stream");
+  if (used_ & (1u << i)) return;
+  if (!c_.side[i]) {
+    HGP_HIP(hipStreamCreateWithFlags(&c_.side[i], hipStreamNonBlocking));
+    HGP_HIP(hipEventCreateWithFlags(&c_.side_ev[i], hipEventDisableTiming));
+  }
+  HGP_HIP(hipStreamWaitEvent(c_.side[i], c_.fork_ev, 0));
+  used_ |= 1u << i;
+}
+Fork::~Fork() {
+  // errors here have no caller to reach: a failed join leaves the stream unsynchronised,
+  // which the context's next synchronisation reports
+  for (int i = 0; i < Ctx::SIDES; ++i)
+    if (used_ & (1u << i)) {
+      (void)hipEventRecord(c_.side_ev[i], c_.side[i]);
+      (void)hipStreamWaitEvent(c_.stream, c_.side_ev[i], 0);
+    }
+}
+void destroy_sides(Ctx& c) {
+  for (int i = 0; i < Ctx::SIDES; ++i) {
+    if (c.side[i]) {
+      (void)hipStreamSynchronize(c.side[i]);
+      (void)hipStreamDestroy(c.side[i]);
+      (void)hipEventDestroy(c.side_ev[i]);
+      c.side[i] = nullptr;
+      c.side_ev[i] = nullptr;
+    }
+  }
+  if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
+  c.fork_ev = nullptr;
+}
+
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, uint64_t n) { HGP_LAUNCH(k_fill_u32, ceil_div(n, 256), 256, 0, c, p, v, n); }
 void iota_u32(Ctx& c, uint32_t* p, uint64_t n) { HGP_LAUNCH(k_iota_u32, ceil_div(n, 256), 256, 0, c, p, n); }
 uint32_t reduce_max_u32(Ctx& c, const uint32_t* p, uint64_t n) {

@@ -16,6 +16,7 @@
 //   6. best prefix (best_move_functor, headers/data_types.cuh:54-77) and apply (:328-370).
 #include <algorithm>
 #include <numeric>
+#include <optional>
 #include <type_traits>
 
 #include "prim.hpp"
@@ -2588,11 +2589,15 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
   if (V == 0) return;
   HGP_LAUNCH(k_zero3, ceil_div(V, 256), 256, 0, c, st.seq.get(), (uint32_t*)st.sv.get(), (uint32_t*)st.iv.get(),
              (uint64_t)V);
-  // cascade
+  // cascade.  The hedges of 2..CAS_SMALL pins first, alone on the stream; then, with bit 1 of
+  // the side_streams tuning, the sorted and direct classes on side streams beside the size and
+  // inbound events (all classes add their terms to seq with atomics; the node records stay
+  // alive until the join before seq's prefix sum).
+  DBuf<uint4> pm;
+  DBuf<uint2> pm2;
+  std::optional<Fork> cas_fork;
   {
     const bool packed = L.k < PM2_MAX_K;
-    DBuf<uint4> pm;
-    DBuf<uint2> pm2;
     if (packed) {
       pm2 = DBuf<uint2>(c, n);
       HGP_LAUNCH(k_pack_nodes2, g, 256, 0, c, part, st.moves.get(), st.ranks.get(), n, pm2.get());
@@ -2646,15 +2651,18 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
       if (prof_match(c, "k_cascade_small") && c.prof_count && c.prof_dev)
         HGP_LAUNCH(k_cascade_small_bytes, ceil_div(Y.cas_small_n, 256), 256, 0, c, a, sh, c.prof_dev);
     }
+    cas_fork.emplace(c, (c.tune.side_streams & 2u) != 0);
     for (int cls = 0; cls < 3; ++cls) {
       const uint32_t m = Y.cas_n[cls];
       if (m == 0) continue;
       const uint32_t* lst = Y.cas_list[cls].get();
-      if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 4 * CAS_SORT_S, 64>), m, 64, 0, c, a, lst, crb);
-      else if (cls == 1) HGP_LAUNCH((k_cascade_sorted<CAS_SORT, CAS_EV, CAS_SORT_NT>), m, CAS_SORT_NT, 0, c, a, lst, crb);
-      else HGP_LAUNCH(k_cascade_large, m, 64, 0, c, a, lst);
+      cas_fork->on(cls, [&] {
+        if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 4 * CAS_SORT_S, 64>), m, 64, 0, c, a, lst, crb);
+        else if (cls == 1)
+          HGP_LAUNCH((k_cascade_sorted<CAS_SORT, CAS_EV, CAS_SORT_NT>), m, CAS_SORT_NT, 0, c, a, lst, crb);
+        else HGP_LAUNCH(k_cascade_large, m, 64, 0, c, a, lst);
+      });
     }
-    inclusive_scan(c, (const int64_t*)st.seq.get(), (int64_t*)st.seq.get(), V);
   }
   const int rb = std::max(1, bit_width(V - 1));
   const int kb = std::max(1, bit_width(L.k));
@@ -2737,6 +2745,8 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
     }
     inclusive_scan(c, st.iv.get(), st.iv.get(), V);
   }
+  cas_fork.reset();  // the main stream waits for the cascade's side streams
+  inclusive_scan(c, (const int64_t*)st.seq.get(), (int64_t*)st.seq.get(), V);
   // best prefix
   {
     const uint32_t blocks = std::min<uint32_t>(ceil_div(V, 256), 1024);

@@ -121,6 +121,9 @@ typedef struct hgp_tuning {
    * from which each node takes 4 waves, and from which it takes 16, instead of one */
   uint32_t mask_wide4_degree;
   uint32_t mask_wide16_degree;
+  /* independent kernels of one stage on side streams (default priority) beside the context's
+   * stream: bit 0 the candidate tiers, bit 1 the cascade's hedge classes */
+  uint32_t side_streams;
   /* 1 when set by hgp_autotune */
   uint32_t tuned;
 } hgp_tuning;

@@ -31,6 +31,7 @@ def extremes():
         t.scan_lookback_tiles = 0
         t.mask_wide4_degree = 0  # every level's mask gains 16 waves per node
         t.mask_wide16_degree = 0
+        t.side_streams = 0
 
     def large(t):
         for i in range(40):
@@ -41,6 +42,7 @@ def extremes():
         t.scan_lookback_tiles = 1 << 20
         t.mask_wide4_degree = 1 << 31  # a wave per node on every level
         t.mask_wide16_degree = 1 << 31
+        t.side_streams = 3
 
     def four_waves(t):
         t.mask_wide4_degree = 0  # 4 waves per node on every level
