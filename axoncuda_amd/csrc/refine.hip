@@ -37,8 +37,8 @@ constexpr uint32_t GAIN_TINY = 16;  // partition entries handled by one thread i
 constexpr uint32_t CAS_SMALL = 16;
 constexpr uint32_t CAS_LDS = 2048;
 constexpr uint32_t CAS_SORT = 1024;               // pins of a hedge cascaded through sorted events
-constexpr uint32_t CAS_SORT_S = 128;              // ... by the small-LDS instance (4x event slots)
-constexpr uint32_t CAS_EV = 4096;                 // event slots: next power of two >= 3 * CAS_SORT
+constexpr uint32_t CAS_SORT_S = 128;              // ... by the small-LDS instance
+constexpr uint32_t CAS_EV = 2048;                 // event slots: next power of two >= 2 * CAS_SORT
 constexpr uint32_t CAS_SORT_NT = 256;              // threads sorting the events of one such hedge
 constexpr uint64_t HEAVY_BATCH_SLOTS = 32ull << 20;
 
@@ -1883,14 +1883,14 @@ __global__ void __launch_bounds__(64) k_cascade_large(CascadeArgs a, const uint3
 }
 
 // Cascade of one hedge through sorted events (hedges of CAS_SMALL < len <= CAS_SORT pins,
-// one workgroup per hedge).  Every pin contributes an "init" event (its partition, +1) and every
-// moving pin an "out" event (source, -1) and an "in" event (target, +1) at its rank.  Sorted
-// by (partition, rank), the exclusive prefix of a move's out event is the pin count of its
-// source just before the move (the pin itself included) and that of its in event is the
-// count of its target: exactly c_p and c_q of the cascade term, in O(len log^2 len) instead
-// of O(len * moving pins).
-// key = partition << (rb + 11) | rank' << 11 | dir << 10 | local index (rank' = 0 for init,
-// rank + 1 for moves; dir 1 = in event).
+// one workgroup per hedge).  The pins' partitions are counted in an LDS hash table; every
+// moving pin contributes an "out" event (source, -1) and an "in" event (target, +1) at its
+// rank.  Sorted by (partition, rank), a move event's exclusive prefix within its partition,
+// plus that partition's pin count, is the pin count of its source just before the move (the
+// pin itself included) for the out event and that of its target for the in event: exactly
+// c_p and c_q of the cascade term, in O(m log^2 m) for m moving pins instead of
+// O(len * m).
+// key = partition << (rb + 11) | (rank + 1) << 11 | dir << 10 | local index (dir 1 = in).
 __device__ __forceinline__ int32_t wave_seg_scan(int32_t v, bool& flag) {
   const uint32_t l = lane_id();
 #pragma unroll
@@ -1907,7 +1907,10 @@ __device__ __forceinline__ int32_t wave_seg_scan(int32_t v, bool& flag) {
 
 template <uint32_t PINS, uint32_t EVS, uint32_t NT>
 __global__ void __launch_bounds__(NT) k_cascade_sorted(CascadeArgs a, const uint32_t* __restrict__ list, int rb) {
+  static_assert(EVS >= 2 * PINS && (EVS & (EVS - 1)) == 0, "two event slots per pin, a power of two");
+  constexpr uint32_t HS = EVS;  // partition table: at least twice the pins, so it never fills
   __shared__ uint64_t s_ev[EVS];
+  __shared__ uint32_t s_hk[HS], s_hc[HS];
   __shared__ uint32_t s_rk[PINS];
   __shared__ int32_t s_cp[PINS], s_cq[PINS];
   __shared__ uint32_t s_moving;
@@ -1916,17 +1919,26 @@ __global__ void __launch_bounds__(NT) k_cascade_sorted(CascadeArgs a, const uint
   const uint64_t s = a.hoff[e];
   const uint32_t len = (uint32_t)(a.hoff[e + 1] - s);
   if (tid == 0) s_moving = 0;
+  for (uint32_t h = tid; h < HS; h += NT) {
+    s_hk[h] = NONE;
+    s_hc[h] = 0;
+  }
   __syncthreads();
-  // init events at [0, len); the two events of each moving pin at any free pair of slots
-  // after them (the sort orders them, and every key is unique through its pin index)
+  // pin counts per partition, and the two events of each moving pin
   for (uint32_t i = tid; i < len; i += NT) {
     const uint32_t x = a.hedges[s + i];
     const uint3 q4 = pm_at(a, x);
     const uint32_t p = q4.x, q = q4.y;
-    s_ev[i] = ((uint64_t)p << (rb + 11)) | i;
+    uint32_t h = slot_hash(p) & (HS - 1);
+    for (;;) {
+      const uint32_t old = atomicCAS(&s_hk[h], NONE, p);
+      if (old == NONE || old == p) break;
+      h = (h + 1) & (HS - 1);
+    }
+    atomicAdd(&s_hc[h], 1u);
     s_rk[i] = q == NONE ? NONE : q4.z;
     if (q != NONE) {
-      const uint32_t o = len + 2 * atomicAdd(&s_moving, 1u);
+      const uint32_t o = 2 * atomicAdd(&s_moving, 1u);
       const uint64_t r1 = (uint64_t)q4.z + 1;
       s_ev[o] = ((uint64_t)p << (rb + 11)) | (r1 << 11) | i;
       s_ev[o + 1] = ((uint64_t)q << (rb + 11)) | (r1 << 11) | (1u << 10) | i;
@@ -1935,7 +1947,7 @@ __global__ void __launch_bounds__(NT) k_cascade_sorted(CascadeArgs a, const uint
   __syncthreads();
   const uint32_t moving = s_moving;
   if (moving == 0) return;
-  const uint32_t nev = len + 2 * moving;
+  const uint32_t nev = 2 * moving;
   uint32_t ev_n = 64;
   while (ev_n < nev) ev_n <<= 1;
   for (uint32_t i = nev + tid; i < ev_n; i += NT) s_ev[i] = ~0ull;
@@ -1956,7 +1968,8 @@ __global__ void __launch_bounds__(NT) k_cascade_sorted(CascadeArgs a, const uint
       __syncthreads();
     }
   }
-  // segmented (by partition) exclusive prefix of the event deltas, by the first wave
+  // segmented (by partition) exclusive prefix of the move deltas, by the first wave; each
+  // event adds its partition's pin count (0 for a target partition with no pin here)
   if (tid < 64) {
     const int shift = rb + 11;
     int32_t carry = 0;
@@ -1964,16 +1977,26 @@ __global__ void __launch_bounds__(NT) k_cascade_sorted(CascadeArgs a, const uint
       const uint32_t i = b + l;
       const bool act = i < nev;
       const uint64_t key = act ? s_ev[i] : ~0ull;
-      const uint32_t rank1 = (uint32_t)((key >> 11) & (((uint64_t)1 << rb) - 1));
       const bool dir_in = (key >> 10) & 1u;
-      const int32_t d = !act ? 0 : (rank1 == 0 ? 1 : (dir_in ? 1 : -1));
+      const int32_t d = !act ? 0 : (dir_in ? 1 : -1);
       bool flag = act && (i == 0 || (s_ev[i - 1] >> shift) != (key >> shift));
       int32_t v = wave_seg_scan(d, flag);
       if (!flag) v += carry;  // segment continues from the previous chunk
-      if (act && rank1 != 0) {
+      if (act) {
+        const uint32_t x = (uint32_t)(key >> shift);
+        uint32_t h = slot_hash(x) & (HS - 1), cnt = 0;
+        for (;;) {
+          const uint32_t kx = s_hk[h];
+          if (kx == x) {
+            cnt = s_hc[h];
+            break;
+          }
+          if (kx == NONE) break;
+          h = (h + 1) & (HS - 1);
+        }
         const uint32_t idx = (uint32_t)(key & 1023u);
-        if (dir_in) s_cq[idx] = v - d;
-        else s_cp[idx] = v - d;
+        if (dir_in) s_cq[idx] = (int32_t)cnt + v - d;
+        else s_cp[idx] = (int32_t)cnt + v - d;
       }
       carry = __shfl(v, 63, 64);
     }
@@ -2657,7 +2680,7 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
       if (m == 0) continue;
       const uint32_t* lst = Y.cas_list[cls].get();
       cas_fork->on(cls, [&] {
-        if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 4 * CAS_SORT_S, 64>), m, 64, 0, c, a, lst, crb);
+        if (cls == 0) HGP_LAUNCH((k_cascade_sorted<CAS_SORT_S, 2 * CAS_SORT_S, 64>), m, 64, 0, c, a, lst, crb);
         else if (cls == 1)
           HGP_LAUNCH((k_cascade_sorted<CAS_SORT, CAS_EV, CAS_SORT_NT>), m, CAS_SORT_NT, 0, c, a, lst, crb);
         else HGP_LAUNCH(k_cascade_large, m, 64, 0, c, a, lst);
