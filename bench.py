@@ -240,7 +240,7 @@ def main() -> int:
     ap.add_argument("--serve", action="store_true", help="run only the serving-stream benchmark")
     ap.add_argument("--jobs", type=int, default=32, help="serving jobs (the full stream is 256)")
     ap.add_argument("--max-pins", type=float, default=2e7)
-    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--workers", type=int, default=6)
     ap.add_argument("--load", type=float, default=4.0, help="offered load of the serving leg, in multiples of the "
                                                              "plain rate")
     ap.add_argument("--copy-calibration", action="store_true",
