@@ -82,6 +82,8 @@ class Tuning(C.Structure):
         ("mask_wide4_degree", C.c_uint32),
         ("mask_wide16_degree", C.c_uint32),
         ("side_streams", C.c_uint32),
+        ("coop_max_n", C.c_uint32),
+        ("coop_tile", C.c_uint32),
         ("tuned", C.c_uint32),
     ]
 
@@ -90,7 +92,8 @@ class Tuning(C.Structure):
                 "heavy_table_slots": int(self.heavy_table_slots), "heavy_table_fill16": int(self.heavy_table_fill16),
                 "cascade_block": int(self.cascade_block), "scan_lookback_tiles": int(self.scan_lookback_tiles),
                 "mask_wide4_degree": int(self.mask_wide4_degree), "mask_wide16_degree": int(self.mask_wide16_degree),
-                "side_streams": int(self.side_streams), "tuned": int(self.tuned)}
+                "side_streams": int(self.side_streams), "coop_max_n": int(self.coop_max_n),
+                "coop_tile": int(self.coop_tile), "tuned": int(self.tuned)}
 
 
 # (name, restype, argtypes) of every exported entry point of include/hgp.h

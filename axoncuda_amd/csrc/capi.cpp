@@ -104,6 +104,8 @@ void to_public(const hgp::Tuning& t, hgp_tuning* o) {
   o->mask_wide4_degree = t.mask_wide4_degree;
   o->mask_wide16_degree = t.mask_wide16_degree;
   o->side_streams = t.side_streams;
+  o->coop_max_n = t.coop_max_n;
+  o->coop_tile = t.coop_tile;
   o->tuned = t.tuned ? 1u : 0u;
 }
 
@@ -114,6 +116,7 @@ bool from_public(const hgp_tuning* o, hgp::Tuning& t) {
   if (s < (1u << 13) || s > (1u << 24) || (s & (s - 1))) return false;
   if (o->heavy_table_fill16 < 4 || o->heavy_table_fill16 > 14) return false;
   if (b < 64 || b > 1024 || (b & (b - 1))) return false;
+  if (o->coop_tile < 256 || o->coop_tile > (1u << 20)) return false;
   for (int i = 0; i < 40; ++i) t.sort_rounds[i] = o->sort_rounds[i];
   t.heavy_slots = s;
   t.heavy_fill16 = o->heavy_table_fill16;
@@ -122,6 +125,8 @@ bool from_public(const hgp_tuning* o, hgp::Tuning& t) {
   t.mask_wide4_degree = o->mask_wide4_degree;
   t.mask_wide16_degree = o->mask_wide16_degree;
   t.side_streams = o->side_streams & 3u;
+  t.coop_max_n = o->coop_max_n;
+  t.coop_tile = o->coop_tile;
   t.tuned = o->tuned != 0;
   return true;
 }
@@ -721,6 +726,8 @@ hgp_status hgp_tuning_save(const hgp_tuning* t, const char* path) {
   f << "mask_wide4_degree " << t->mask_wide4_degree << "\n";
   f << "mask_wide16_degree " << t->mask_wide16_degree << "\n";
   f << "side_streams " << t->side_streams << "\n";
+  f << "coop_max_n " << t->coop_max_n << "\n";
+  f << "coop_tile " << t->coop_tile << "\n";
   f << "tuned " << t->tuned << "\n";
   return f.good() ? HGP_OK : HGP_E_IO;
 }
@@ -751,6 +758,8 @@ hgp_status hgp_tuning_load(const char* path, hgp_tuning* out) {
     else if (key == "mask_wide4_degree") t.mask_wide4_degree = (uint32_t)a;
     else if (key == "mask_wide16_degree") t.mask_wide16_degree = (uint32_t)a;
     else if (key == "side_streams") t.side_streams = (uint32_t)a;
+    else if (key == "coop_max_n") t.coop_max_n = (uint32_t)a;
+    else if (key == "coop_tile") t.coop_tile = (uint32_t)a;
     else if (key == "tuned") t.tuned = (uint32_t)a;
     else return HGP_E_INVALID;
   }

@@ -62,6 +62,8 @@ struct Tuning {
   uint32_t mask_wide4_degree = 128;   // mean touching hedges per node from which k-way gains take
   uint32_t mask_wide16_degree = 512;  // 4 (16) waves per node
   uint32_t side_streams = 2;          // bit 0: candidate tiers, bit 1: cascade classes beside events
+  uint32_t coop_max_n = 65536;        // sorts of RS1 .. this many keys run every pass in one launch
+  uint32_t coop_tile = 1024;          // keys per workgroup of those launches (at least)
   bool tuned = false;
 };
 
@@ -84,6 +86,8 @@ struct Ctx {
   void* scan_agg = nullptr;
   void* scan_incl = nullptr;
   unsigned long long* scan_ticket = nullptr;
+  unsigned int* coop_bar = nullptr;  // grid-barrier counter of the cooperative sorts (prim.hip)
+  uint64_t coop_issued = 0;
   uint32_t scan_cap_tiles = 0;
   uint32_t scan_gen = 0;
   uint64_t scan_tickets_issued = 0;

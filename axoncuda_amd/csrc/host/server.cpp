@@ -134,10 +134,13 @@ hgp_status hgp_server_create(int device, uint64_t hbm_budget, uint32_t workers, 
       return st;
     }
     // the workers already run side by side: more streams than the process's hardware queues
-    // would queue one worker's kernels behind another's, so the workers keep to one stream each
+    // would queue one worker's kernels behind another's, so the workers keep to one stream
+    // each; and they sort without grid barriers, so no number of workers can hold the device's
+    // slots with workgroups that wait for each other
     hgp_tuning t;
     if (hgp_tuning_get(c, &t) == HGP_OK) {
       t.side_streams = 0;
+      t.coop_max_n = 0;
       (void)hgp_tuning_set(c, &t);
     }
     s->ctxs.push_back(c);

@@ -248,6 +248,22 @@ void scan_status(Ctx& c, uint32_t tiles) {
   c.scan_cap_tiles = cap;
 }
 
+// Grid-barrier counter of the context's cooperative sorts: returns the count that a launch
+// making `arrivals` arrivals starts from, resetting the counter (on the stream) before it
+// could wrap.  The caller adds the arrivals to c.coop_issued once the launch is queued, so a
+// launch that fails to queue leaves the count as the device has it.
+unsigned int coop_barrier_base(Ctx& c, uint32_t arrivals) {
+  if (!c.coop_bar) {
+    c.coop_bar = (unsigned int*)dmalloc(c, 4);
+    c.coop_issued = 1u << 31;  // forces the reset below
+  }
+  if (c.coop_issued + (uint64_t)arrivals >= (1ull << 31)) {
+    fill_u32(c, c.coop_bar, 0u, 1);
+    c.coop_issued = 0;
+  }
+  return (unsigned int)c.coop_issued;
+}
+
 // in == out is safe: each workgroup reads only its own tile before writing it.
 template <typename TI, typename TO>
 void scan_impl(Ctx& c, const TI* in, TO* out, uint64_t n, bool exclusive) {
@@ -459,6 +475,108 @@ __global__ void __launch_bounds__(RS1_BLOCK) k_rs_small(K* ka, uint32_t* va, K* 
   }
 }
 
+// Whole LSD sort of a mid-size array in one launch: B workgroups each own a contiguous tile and
+// meet at grid barriers.  B <= COOP_MAX_BLOCKS = 64 is far below what the device holds at once
+// (at least 4 such workgroups per CU, 256 CUs), and every other kernel that may run beside one
+// finishes on its own (look-back scans wait only on tiles already running; serving workers do
+// not use these sorts), so all B workgroups become resident and every barrier is reached.  A pass counts its tile's digits, publishes the counts, and after a
+// barrier every workgroup derives its tile's digit offsets from all counts (digit totals
+// scanned over digits, plus the counts of the tiles before its own), scatters its keys stably
+// in rounds of 256, and waits at a second barrier before the next pass reads the output.
+// The barrier counter only grows; arrival k of a launch waits for base + B * k.
+constexpr uint32_t COOP_MAX_BLOCKS = 64;
+
+__device__ __forceinline__ void grid_barrier(unsigned int* ctr, unsigned int target) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this thread's writes before the arrival
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(ctr, 1u);
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other workgroups' writes after it
+}
+
+template <typename K, bool HAS_VALS>
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_coop(K* ka, uint32_t* va, K* kb, uint32_t* vb, uint32_t n,
+                                                      uint32_t tile, int lo, int bits, K kmin,
+                                                      uint32_t* __restrict__ counts, unsigned int* bar,
+                                                      unsigned int bar_base) {
+  __shared__ uint32_t h[256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t s_w[RS_BLOCK / 64];
+  __shared__ uint32_t wcnt[RS_BLOCK / 64][256];
+  const uint32_t B = gridDim.x, b = blockIdx.x, t = threadIdx.x, w = t / 64, l = lane_id();
+  const uint32_t beg = min(n, b * tile), end = min(n, beg + tile);
+  unsigned int target = bar_base;
+  for (int shift = lo; shift < bits; shift += 8) {
+    h[t] = 0;
+    __syncthreads();
+    for (uint32_t i = beg + t; i < end; i += RS_BLOCK) atomicAdd(&h[(uint32_t)((ka[i] - kmin) >> shift) & 0xFFu], 1u);
+    __syncthreads();
+    counts[(uint64_t)b * 256 + t] = h[t];
+    target += B;
+    grid_barrier(bar, target);
+    // digit t: its total over all tiles and its count in the tiles before this one
+    uint32_t tot = 0, pre = 0;
+    for (uint32_t x = 0; x < B; ++x) {
+      const uint32_t cx = counts[(uint64_t)x * 256 + t];
+      tot += cx;
+      pre += x < b ? cx : 0u;
+    }
+    const uint32_t inc = wave_scan_incl(tot);
+    if (l == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t q = 0; q < w; ++q) base += s_w[q];
+    run[t] = base + inc - tot + pre;
+    for (uint32_t r0 = beg; r0 < end; r0 += RS_BLOCK) {  // uniform across the workgroup
+#pragma unroll
+      for (int q = 0; q < RS_BLOCK / 64; ++q) wcnt[q][t] = 0;
+      __syncthreads();
+      const uint32_t i = r0 + t;
+      const bool valid = i < end;
+      const K key = valid ? ka[i] : (K)0;
+      uint32_t val = 0;
+      if (HAS_VALS && valid) val = va[i];
+      const uint32_t d = (uint32_t)((key - kmin) >> shift) & 0xFFu;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int bt = 0; bt < 8; ++bt) {
+        const bool bit = (d >> bt) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      const uint32_t rank = lanes_below(peers);
+      if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      uint32_t acc = 0;
+#pragma unroll
+      for (int q = 0; q < RS_BLOCK / 64; ++q) {
+        const uint32_t cq = wcnt[q][t];
+        wcnt[q][t] = acc;
+        acc += cq;
+      }
+      __syncthreads();
+      if (valid) {
+        const uint32_t pos = run[d] + wcnt[w][d] + rank;
+        kb[pos] = key;
+        if (HAS_VALS) vb[pos] = val;
+      }
+      __syncthreads();
+      run[t] += acc;
+    }
+    target += B;
+    grid_barrier(bar, target);  // the output is whole (and the counts free) before the next pass
+    K* tk = ka;
+    ka = kb;
+    kb = tk;
+    uint32_t* tv = va;
+    va = vb;
+    vb = tv;
+  }
+}
+
 // LSD passes over bits [lo, bits) of key - kmin (the keys themselves are moved unchanged).  With `kbuf` (and `vbuf`) the result ends up in those
 // buffers by exchanging them with the scratch ones after an odd pass count; otherwise it is
 // copied back into keys / vals.
@@ -478,6 +596,35 @@ void radix_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int bits, int lo = 
     else
       HGP_LAUNCH_B(sb, (k_rs_small<K, false>), 1, RS1_BLOCK, 0, c, keys, vals, k2.get(), v2.get(), (uint32_t)n, lo,
                    bits, kmin);
+    if (passes & 1) {
+      if (kbuf && (!vals || vbuf)) {  // hand the scratch buffers over instead of copying
+        std::swap(*kbuf, k2);
+        if (vals) std::swap(*vbuf, v2);
+        return;
+      }
+      copy_d2d(c, keys, k2.get(), n * sizeof(K));
+      if (vals) copy_d2d(c, vals, v2.get(), n * sizeof(uint32_t));
+    }
+    return;
+  }
+  if (n <= c.tune.coop_max_n) {
+    // one launch for every pass: ceil(n / coop_tile) workgroups (at most COOP_MAX_BLOCKS)
+    const uint32_t tile0 = std::max<uint32_t>(c.tune.coop_tile, (uint32_t)ceil_div(n, COOP_MAX_BLOCKS));
+    const uint32_t tile = (tile0 + RS_BLOCK - 1) / RS_BLOCK * RS_BLOCK;
+    const uint32_t B = ceil_div(n, tile);
+    const int passes = (bits - lo + 7) / 8;
+    const unsigned int base = coop_barrier_base(c, 2u * passes * B);
+    DBuf<K> k2(c, n);
+    DBuf<uint32_t> v2(c, vals ? n : 0);
+    DBuf<uint32_t> counts(c, (uint64_t)256 * B);
+    const uint64_t sb = (uint64_t)passes * n * (2 * sizeof(K) + (vals ? 8 : 0));
+    if (vals)
+      HGP_LAUNCH_B(sb, (k_rs_coop<K, true>), B, RS_BLOCK, 0, c, keys, vals, k2.get(), v2.get(), (uint32_t)n, tile, lo,
+                   bits, kmin, counts.get(), c.coop_bar, base);
+    else
+      HGP_LAUNCH_B(sb, (k_rs_coop<K, false>), B, RS_BLOCK, 0, c, keys, vals, k2.get(), v2.get(), (uint32_t)n, tile,
+                   lo, bits, kmin, counts.get(), c.coop_bar, base);
+    c.coop_issued += 2u * passes * B;
     if (passes & 1) {
       if (kbuf && (!vals || vbuf)) {  // hand the scratch buffers over instead of copying
         std::swap(*kbuf, k2);

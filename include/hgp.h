@@ -124,6 +124,11 @@ typedef struct hgp_tuning {
   /* independent kernels of one stage on side streams (default priority) beside the context's
    * stream: bit 0 the candidate tiers, bit 1 the cascade's hedge classes */
   uint32_t side_streams;
+  /* radix sorts of more than 16,384 and at most coop_max_n keys run all their passes in one
+   * launch of at most 64 workgroups that meet at grid barriers, each taking at least
+   * coop_tile keys (0 disables them; serving workers never use them) */
+  uint32_t coop_max_n;
+  uint32_t coop_tile;
   /* 1 when set by hgp_autotune */
   uint32_t tuned;
 } hgp_tuning;

@@ -32,6 +32,8 @@ def extremes():
         t.mask_wide4_degree = 0  # every level's mask gains 16 waves per node
         t.mask_wide16_degree = 0
         t.side_streams = 0
+        t.coop_max_n = 1 << 22  # every sort above 16 K keys in one launch of 64 workgroups
+        t.coop_tile = 256
 
     def large(t):
         for i in range(40):
@@ -43,6 +45,7 @@ def extremes():
         t.mask_wide4_degree = 1 << 31  # a wave per node on every level
         t.mask_wide16_degree = 1 << 31
         t.side_streams = 3
+        t.coop_max_n = 0  # no sort in one launch
 
     def four_waves(t):
         t.mask_wide4_degree = 0  # 4 waves per node on every level
