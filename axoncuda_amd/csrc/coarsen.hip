@@ -308,7 +308,7 @@ __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32
 // OPT (optimistic): the node's distinct neighbours may exceed the table.  New keys are
 // counted; past `limit` the workgroup stops and the caller hands the node to a larger table.  At most one insert per thread is in flight when
 // the limit trips, so the table never fills.  Returns false when the node overflowed.
-template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true, int CH = 1>
+template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true, int CH = 1, bool PIPE = true>
 __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
                                                 uint32_t* inter, const uint32_t* __restrict__ hedges,
                                                 const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ srcc,
@@ -346,7 +346,7 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
     if (GMEM) __threadfence();
     __syncthreads();
     auto hvf = [&](uint32_t e, uint32_t len) { return make_uint2(fixed_weight(w[e]) / len, INTER ? srcc[e] : 0u); };
-    wave_visit_pins_hv_until(tch, a, b, hoff, hedges, hvf, [&](const PinVisitHV& pv) {
+    wave_visit_pins_hv_until<PIPE>(tch, a, b, hoff, hedges, hvf, [&](const PinVisitHV& pv) {
       if (!pv.act || pv.pin == u) return;
       if (CH > 1 && pv.pin % CH != (uint32_t)chunk) return;
       const uint32_t contrib = pv.hv.x;  // floor(W / |e|), computed once per touching hedge
@@ -463,8 +463,10 @@ __global__ void __launch_bounds__(1024) k_cand_direct(CAND_ARGS) {
   __shared__ uint32_t s_inter[INTER ? N : 1];
   __shared__ uint64_t s_w[256];
   const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
-  cand_block_node<false, INTER, 16>(u, caps[u], true, s_bits, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
-                                    sizes, cnc, omega, delta, pairs, scores, s_w);
+  // without the pipelined traversal: its extra live registers spill in this kernel
+  cand_block_node<false, INTER, 16, false, true, 1, false>(u, caps[u], true, s_bits, s_score, s_inter, hedges, hoff,
+                                                           srcc, tch, toff, inb, w, sizes, cnc, omega, delta, pairs,
+                                                           scores, s_w);
 }
 
 // Heavy nodes: a persistent grid, each workgroup owning `region` slots of HBM tables and

@@ -178,11 +178,13 @@ __device__ __forceinline__ void wave_visit_pins_until(const uint32_t* __restrict
 }
 
 // The same traversal with two per-hedge values computed once per touching hedge by the lane
-// that loads it, `hvf(e, len)`, and handed to every visit of that hedge's pins in `hv`.
+// that loads it, `hvf(e, len)`, and handed to every visit of that hedge's pins in `hv`.  With
+// PIPE the next step's pin is loading while the current step is visited (one more step's
+// registers live across the visit).
 struct PinVisitHV : PinVisit {
   uint2 hv;
 };
-template <typename H, typename F, typename S>
+template <bool PIPE = true, typename H, typename F, typename S>
 __device__ __forceinline__ void wave_visit_pins_hv_until(const uint32_t* __restrict__ tch, uint64_t t0, uint64_t t1,
                                                          const uint64_t* __restrict__ hoff,
                                                          const uint32_t* __restrict__ hedges, H&& hvf, F&& visit,
@@ -204,8 +206,8 @@ __device__ __forceinline__ void wave_visit_pins_hv_until(const uint32_t* __restr
     const uint32_t total = __shfl(incl, 63, 64);
     const uint32_t excl = incl - len;
     const uint64_t le_l = lanes_upto(l);
-    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
-      if (f0 && stop()) return;
+    // step f0's owner, fields and pin load
+    auto step = [&](uint32_t f0) {
       const uint32_t fi = f0 + l;
       const uint32_t k = chunk_owner(excl, f0, le_l);
       PinVisitHV pv;
@@ -219,7 +221,22 @@ __device__ __forceinline__ void wave_visit_pins_hv_until(const uint32_t* __restr
       pv.j = jb + k;
       pv.pos = fi - oex;
       pv.pin = pv.act ? hedges[os + pv.pos] : 0u;
-      visit(pv);
+      return pv;
+    };
+    if constexpr (PIPE) {
+      PinVisitHV cur = step(0);
+      for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+        if (f0 && stop()) return;
+        PinVisitHV nxt = cur;
+        if (f0 + 64 < total) nxt = step(f0 + 64);  // uniform: total is wave-wide
+        visit(cur);
+        cur = nxt;
+      }
+    } else {
+      for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+        if (f0 && stop()) return;
+        visit(step(f0));
+      }
     }
   }
 }
