@@ -364,13 +364,20 @@ hgp_status hgp_save_partitioning(const char* path, const uint32_t* part, uint32_
   FILE* f = std::fopen(path, "w");
   if (!f) return HGP_E_IO;
   std::string buf;
-  buf.reserve((size_t)n * 4);
-  char tmp[16];
+  buf.resize((size_t)n * 11);  // at most 10 digits and a newline per node
+  char* o = buf.data();
   for (uint32_t v = 0; v < n; ++v) {
-    int l = std::snprintf(tmp, sizeof tmp, "%u\n", part[v]);
-    buf.append(tmp, l);
+    char tmp[10];
+    uint32_t x = part[v];
+    int l = 0;
+    do {
+      tmp[l++] = (char)('0' + x % 10);
+      x /= 10;
+    } while (x);
+    while (l) *o++ = tmp[--l];
+    *o++ = '\n';
   }
-  std::fwrite(buf.data(), 1, buf.size(), f);
+  std::fwrite(buf.data(), 1, (size_t)(o - buf.data()), f);
   std::fclose(f);
   return HGP_OK;
 }

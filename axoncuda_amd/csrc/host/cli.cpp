@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <sys/stat.h>
+#include <thread>
 #include <vector>
 
 #include "hgp.h"
@@ -170,6 +171,25 @@ int main(int argc, char** argv) {
   prm.verbose = verbosity > 1 ? verbosity - 1 : 0;
   if (prm.candidates_count == 0) { std::fprintf(stderr, "Error: -cnc requires a positive integer value\n"); return 1; }
 
+  // The device context (the HIP runtime's start-up) is created on a thread of its own while
+  // the file is read and deduplicated; the device is first used after that.
+  struct CtxStart {
+    hgp_ctx* ctx = nullptr;
+    hgp_status st = HGP_E_NODEVICE;
+    std::thread th;
+    CtxStart() : th([this] { st = hgp_ctx_create(0, 0, &ctx); }) {}
+    hgp_status take(hgp_ctx** out) {
+      if (th.joinable()) th.join();
+      *out = ctx;
+      ctx = nullptr;
+      return st;
+    }
+    ~CtxStart() {
+      if (th.joinable()) th.join();
+      if (ctx) hgp_ctx_destroy(ctx);
+    }
+  } ctx_start;
+
   // load (sources/runconfig.cpp:153-184)
   hgp_hgraph* hg = nullptr;
   if (!load_path.empty()) {
@@ -256,7 +276,7 @@ int main(int argc, char** argv) {
   std::printf("  Flags: %s%s\n", prm.device_touching_construction ? "dtc " : "", prm.initial_partitions_merge ? "ipm " : "");
 
   hgp_ctx* ctx = nullptr;
-  hgp_status s = hgp_ctx_create(0, 0, &ctx);
+  hgp_status s = ctx_start.take(&ctx);
   if (s != HGP_OK) {
     std::fprintf(stderr, "ERROR: no usable gfx950 device (status %d)\n", s);
     return s;

@@ -84,6 +84,55 @@ inline bool parse_uint(const char*& q, const char* e, uint64_t& out, bool& negat
   return true;
 }
 
+// Host threads for the loaders and graders: at most 16, the process's share of a GPU box.
+unsigned host_threads() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
+// fn(t, lo, hi) on T contiguous ranges of [0, count), one thread each (the caller's thread
+// takes range 0).
+template <typename F>
+void parallel_ranges(uint64_t count, unsigned T, F&& fn) {
+  if (T <= 1 || count < 2) {
+    fn(0u, (uint64_t)0, count);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < T; ++t) th.emplace_back([&, t] { fn(t, count * t / T, count * (t + 1) / T); });
+  fn(0u, (uint64_t)0, count / T);
+  for (auto& x : th) x.join();
+}
+
+// Parse of one hyperedge line [b, e): its pins (0-based) appended to `pins` and its weight;
+// returns "" or the error text of the line.
+const char* parse_hedge_line(const char* b, const char* e, bool ew, uint64_t V, std::vector<uint32_t>& pins,
+                             float& w, size_t& cnt) {
+  const char* q = b;
+  bool neg;
+  w = 1.0f;
+  if (ew) {
+    while (q < e && is_blank(*q)) ++q;
+    char* endp = nullptr;
+    w = std::strtof(q, &endp);
+    if (endp == q) w = 1.0f;
+    q = endp > e ? e : endp;
+  }
+  const size_t start = pins.size();
+  uint64_t v;
+  while (parse_uint(q, e, v, neg)) {
+    if (neg || v == 0 || v > V) return "Invalid node encountered";
+    pins.push_back((uint32_t)(v - 1));
+    if (q < e && !is_blank(*q)) break;  // token continues with non-digits: extraction stops
+  }
+  cnt = pins.size() - start;
+  if (cnt < 1) return "Hyperedge must contain at least one node";
+  return "";
+}
+
+// Files above this size are parsed by host_threads() threads: the body after the header is cut
+// into ranges at line ends, each range's data lines are counted, and then every range parses
+// its lines knowing their global index (hyperedge i, or vertex weight i - E).  The first
+// error in file order is reported, as by a sequential read.
+constexpr size_t HGR_PARALLEL_BYTES = 4u << 20;
+
 std::string load_hgr(const std::string& path, hgp_hgraph& hg) {
   std::vector<char> buf;
   if (!read_file(path, buf)) return "Cannot open .hgr file";
@@ -100,30 +149,117 @@ std::string load_hgr(const std::string& path, hgp_hgraph& hg) {
   const bool ew = fmt == 1 || fmt == 11, vw = fmt == 10 || fmt == 11;
   hg = hgp_hgraph();
   hg.n = (uint32_t)V;
+  const unsigned T = host_threads();
+  if (buf.size() >= HGR_PARALLEL_BYTES && T > 1) {
+    const char* body = lc.p;
+    const char* bend = lc.end;
+    // range starts at line starts
+    std::vector<const char*> cut(T + 1);
+    cut[0] = body;
+    cut[T] = bend;
+    for (unsigned t = 1; t < T; ++t) {
+      const char* x = body + (size_t)(bend - body) * t / T;
+      if (x < cut[t - 1]) x = cut[t - 1];
+      const char* nl = (const char*)std::memchr(x, '\n', (size_t)(bend - x));
+      cut[t] = nl ? nl + 1 : bend;
+    }
+    std::vector<uint64_t> lines(T + 1, 0);
+    parallel_ranges(T, T, [&](unsigned, uint64_t lo, uint64_t hi) {
+      for (uint64_t t = lo; t < hi; ++t) {
+        LineCursor c{cut[t], cut[t + 1]};
+        const char *xb, *xe;
+        uint64_t k = 0;
+        while (c.next(xb, xe)) ++k;
+        lines[t + 1] = k;
+      }
+    });
+    for (unsigned t = 0; t < T; ++t) lines[t + 1] += lines[t];
+    if (vw) hg.vweights.assign(hg.n, 1);
+    struct Part {
+      std::vector<uint32_t> pins;
+      std::vector<uint64_t> ends;  // end offset (local) of each kept hyperedge
+      std::vector<float> w;
+      uint64_t err_line = UINT64_MAX;
+      const char* err = "";
+    };
+    std::vector<Part> parts(T);
+    parallel_ranges(T, T, [&](unsigned, uint64_t lo, uint64_t hi) {
+      for (uint64_t t = lo; t < hi; ++t) {
+        Part& P = parts[t];
+        LineCursor c{cut[t], cut[t + 1]};
+        const char *xb, *xe;
+        uint64_t g = lines[t];
+        if (g < E) P.pins.reserve((size_t)(cut[t + 1] - cut[t]) / 4);
+        while (c.next(xb, xe)) {
+          if (g < E) {
+            float w;
+            size_t cnt;
+            const size_t start = P.pins.size();
+            const char* err = parse_hedge_line(xb, xe, ew, V, P.pins, w, cnt);
+            if (*err) {
+              P.err_line = g;
+              P.err = err;
+              break;
+            }
+            if (cnt == 1) {
+              P.pins.resize(start);  // singleton hedges are skipped
+            } else {
+              P.ends.push_back(P.pins.size());
+              P.w.push_back(w);
+            }
+          } else if (vw && g - E < V) {
+            const char* q = xb;
+            uint64_t x;
+            bool ng;
+            if (parse_uint(q, xe, x, ng)) hg.vweights[g - E] = (uint32_t)x;
+          } else {
+            break;
+          }
+          ++g;
+        }
+      }
+    });
+    for (unsigned t = 0; t < T; ++t)
+      if (parts[t].err_line != UINT64_MAX) return parts[t].err;  // ranges are in file order
+    if (lines[T] < E) return "Unexpected end of file while reading hyperedges";
+    uint64_t total = 0, edges = 0;
+    for (auto& P : parts) {
+      total += P.pins.size();
+      edges += P.ends.size();
+    }
+    hg.pins.resize(total);
+    hg.off.resize(edges + 1);
+    hg.srcc.assign(edges, 0);
+    hg.w.resize(edges);
+    std::vector<uint64_t> pbase(T + 1, 0), ebase(T + 1, 0);
+    for (unsigned t = 0; t < T; ++t) {
+      pbase[t + 1] = pbase[t] + parts[t].pins.size();
+      ebase[t + 1] = ebase[t] + parts[t].ends.size();
+    }
+    hg.off[0] = 0;
+    parallel_ranges(T, T, [&](unsigned, uint64_t lo, uint64_t hi) {
+      for (uint64_t t = lo; t < hi; ++t) {
+        const Part& P = parts[t];
+        std::memcpy(hg.pins.data() + pbase[t], P.pins.data(), P.pins.size() * 4);
+        for (size_t i = 0; i < P.ends.size(); ++i) {
+          hg.off[ebase[t] + i + 1] = pbase[t] + P.ends[i];
+          hg.w[ebase[t] + i] = P.w[i];
+        }
+      }
+    });
+    return "";
+  }
   hg.pins.reserve(buf.size() / 4);
   hg.srcc.reserve(E);
   hg.w.reserve(E);
   hg.off.reserve(E + 1);
   for (uint64_t i = 0; i < E; ++i) {
     if (!lc.next(b, e)) return "Unexpected end of file while reading hyperedges";
-    q = b;
-    float w = 1.0f;
-    if (ew) {
-      while (q < e && is_blank(*q)) ++q;
-      char* endp = nullptr;
-      w = std::strtof(q, &endp);
-      if (endp == q) w = 1.0f;
-      q = endp > e ? e : endp;
-    }
+    float w;
+    size_t cnt;
     const size_t start = hg.pins.size();
-    uint64_t v;
-    while (parse_uint(q, e, v, neg)) {
-      if (neg || v == 0 || v > V) return "Invalid node encountered";
-      hg.pins.push_back((uint32_t)(v - 1));
-      if (q < e && !is_blank(*q)) break;  // token continues with non-digits: extraction stops
-    }
-    const size_t cnt = hg.pins.size() - start;
-    if (cnt < 1) return "Hyperedge must contain at least one node";
+    const char* err = parse_hedge_line(b, e, ew, V, hg.pins, w, cnt);
+    if (*err) return err;
     if (cnt == 1) {
       hg.pins.resize(start);  // singleton hedges are skipped
       continue;
@@ -215,42 +351,81 @@ std::string load_hgraph(const std::string& path, hgp_hgraph& hg) {
   return "Failed to load hypergraph, unsupported file format (supported: '.hgr', '.snn', '.axh').";
 }
 
-// deduplicateHyperedges(2,false) (includes/hgraph.hpp:518-581): sources first, first
-// occurrence kept, a destination equal to a source dropped.  Stamped seen-array: O(pins).
-void dedup_rule2(hgp_hgraph& hg) {
-  std::vector<uint32_t> stamp(hg.n, 0);
+namespace {
+// Rule-2 dedup of hyperedges [h0, h1) into `out` (sources first, each pin's first occurrence
+// kept in order): `ends` gets each hedge's end in `out`, `nsrc` its source count.  Hedges of
+// up to 32 pins compare against the pins already kept; longer ones use `stamp` (one entry per
+// node, allocated on first use).
+void dedup_range(const hgp_hgraph& hg, uint32_t h0, uint32_t h1, std::vector<uint32_t>& out,
+                 std::vector<uint64_t>& ends, std::vector<uint32_t>& nsrc) {
+  std::vector<uint32_t> stamp;
   uint32_t gen = 0;
-  uint64_t wpos = 0;
-  uint64_t start = hg.off[0];
-  for (uint32_t i = 0; i < hg.e(); ++i) {
-    const uint64_t s = start, t = hg.off[i + 1];
-    start = t;
-    if (++gen == 0) {
-      std::fill(stamp.begin(), stamp.end(), 0);
-      gen = 1;
-    }
-    const uint64_t ws = wpos;
+  out.reserve((size_t)(hg.off[h1] - hg.off[h0]));
+  ends.reserve(h1 - h0);
+  nsrc.reserve(h1 - h0);
+  for (uint32_t i = h0; i < h1; ++i) {
+    const uint64_t s = hg.off[i], t = hg.off[i + 1];
+    const size_t ws = out.size();
     const uint32_t sc = hg.srcc[i];
-    for (uint64_t k = s; k < s + sc; ++k) {
-      const uint32_t v = hg.pins[k];
-      if (stamp[v] != gen) {
-        stamp[v] = gen;
-        hg.pins[wpos++] = v;
+    uint32_t ns = 0;
+    if (t - s <= 32) {
+      for (uint64_t k = s; k < t; ++k) {
+        const uint32_t v = hg.pins[k];
+        bool seen = false;
+        for (size_t x = ws; x < out.size() && !seen; ++x) seen = out[x] == v;
+        if (!seen) {
+          out.push_back(v);
+          if (k < s + sc) ++ns;
+        }
+      }
+    } else {
+      if (stamp.empty()) stamp.assign(hg.n, 0);
+      if (++gen == 0) {
+        std::fill(stamp.begin(), stamp.end(), 0);
+        gen = 1;
+      }
+      for (uint64_t k = s; k < t; ++k) {
+        const uint32_t v = hg.pins[k];
+        if (stamp[v] != gen) {
+          stamp[v] = gen;
+          out.push_back(v);
+          if (k < s + sc) ++ns;
+        }
       }
     }
-    const uint32_t nsrc = (uint32_t)(wpos - ws);
-    for (uint64_t k = s + sc; k < t; ++k) {
-      const uint32_t v = hg.pins[k];
-      if (stamp[v] != gen) {
-        stamp[v] = gen;
-        hg.pins[wpos++] = v;
-      }
-    }
-    hg.off[i] = ws;
-    hg.srcc[i] = nsrc;
+    ends.push_back(out.size());
+    nsrc.push_back(ns);
   }
-  hg.off[hg.e()] = wpos;
-  hg.pins.resize(wpos);
+}
+}  // namespace
+
+// Rule 2 (includes/hgraph.hpp:518-581): within each hedge a pin repeated among the sources, or
+// among the destinations, or appearing as both, keeps only its first occurrence (sources are
+// stored first, so a pin that is a source stays one).  Hedge ranges are deduplicated by
+// host_threads() threads and concatenated in order.
+void dedup_rule2(hgp_hgraph& hg) {
+  const uint32_t E = hg.e();
+  const unsigned T = hg.pins.size() >= (1u << 20) ? host_threads() : 1u;
+  std::vector<std::vector<uint32_t>> outs(T), nsrcs(T);
+  std::vector<std::vector<uint64_t>> endss(T);
+  std::vector<uint32_t> bounds(T + 1);
+  for (unsigned t = 0; t <= T; ++t) bounds[t] = (uint32_t)((uint64_t)E * t / T);
+  parallel_ranges(T, T, [&](unsigned, uint64_t lo, uint64_t hi) {
+    for (uint64_t t = lo; t < hi; ++t) dedup_range(hg, bounds[t], bounds[t + 1], outs[t], endss[t], nsrcs[t]);
+  });
+  std::vector<uint64_t> base(T + 1, 0);
+  for (unsigned t = 0; t < T; ++t) base[t + 1] = base[t] + outs[t].size();
+  hg.pins.resize(base[T]);
+  hg.off[0] = 0;
+  parallel_ranges(T, T, [&](unsigned, uint64_t lo, uint64_t hi) {
+    for (uint64_t t = lo; t < hi; ++t) {
+      std::memcpy(hg.pins.data() + base[t], outs[t].data(), outs[t].size() * 4);
+      for (uint32_t i = bounds[t]; i < bounds[t + 1]; ++i) {
+        hg.off[i + 1] = base[t] + endss[t][i - bounds[t]];
+        hg.srcc[i] = nsrcs[t][i - bounds[t]];
+      }
+    }
+  });
 }
 
 std::string save_hgraph(const hgp_hgraph& hg, const std::string& path) {
@@ -358,42 +533,101 @@ hgp_hgraph partitions_hypergraph(const hgp_hgraph& hg, const std::vector<uint32_
 // lambda-1 and cut-net of the quotient hypergraph, SOED on the input (grading/grade.cpp:300-308)
 Metrics metrics(const hgp_hgraph& hg, const std::vector<uint32_t>& part) {
   Metrics m;
-  std::vector<uint32_t> sp, dp, all;
-  for (uint32_t i = 0; i < hg.e(); ++i) {
-    sp.clear();
-    dp.clear();
-    all.clear();
-    for (uint64_t j = hg.off[i]; j < hg.off[i + 1]; ++j) {
-      const uint32_t q = part[hg.pins[j]];
-      all.push_back(q);
-      if (j - hg.off[i] < hg.srcc[i]) sp.push_back(q);
-      else dp.push_back(q);
+  const uint32_t E = hg.e();
+  // Per hedge: lam = distinct partitions of all its pins, ns = distinct source partitions,
+  // nd = distinct destination partitions that hold no source.  Counted by host_threads()
+  // threads with per-thread stamps indexed by partition id (sorted copies when the ids are
+  // too large to index); the sums then run in hedge order, as a sequential count would.
+  uint32_t maxq = 0;
+  for (uint32_t q : part) maxq = std::max(maxq, q);
+  std::vector<uint32_t> lam(E), nsv(E), ndv(E);
+  const bool stamped = (uint64_t)maxq < (1ull << 22);  // 16 MB of stamps per thread at most
+  const unsigned T = hg.pins.size() >= (1u << 20) ? host_threads() : 1u;
+  parallel_ranges(E, T, [&](unsigned, uint64_t lo, uint64_t hi) {
+    std::vector<uint32_t> s_all, sp, dp, all;
+    if (stamped) s_all.assign((size_t)maxq + 1, UINT32_MAX);
+    for (uint64_t i = lo; i < hi; ++i) {
+      const uint64_t s = hg.off[i], t = hg.off[i + 1], sc = s + hg.srcc[i];
+      if (stamped) {
+        uint32_t l = 0, ns = 0, nd = 0;
+        const uint32_t g = (uint32_t)i;
+        for (uint64_t j = s; j < sc; ++j) {  // sources first
+          const uint32_t q = part[hg.pins[j]];
+          if (s_all[q] != g) {
+            s_all[q] = g;
+            ++l;
+            ++ns;
+          }
+        }
+        for (uint64_t j = sc; j < t; ++j) {  // a destination partition seen first here holds no source
+          const uint32_t q = part[hg.pins[j]];
+          if (s_all[q] != g) {
+            s_all[q] = g;
+            ++l;
+            ++nd;
+          }
+        }
+        lam[i] = l;
+        nsv[i] = ns;
+        ndv[i] = nd;
+      } else {
+        sp.clear();
+        dp.clear();
+        all.clear();
+        for (uint64_t j = s; j < t; ++j) {
+          const uint32_t q = part[hg.pins[j]];
+          all.push_back(q);
+          (j < sc ? sp : dp).push_back(q);
+        }
+        std::sort(all.begin(), all.end());
+        lam[i] = (uint32_t)(std::unique(all.begin(), all.end()) - all.begin());
+        std::sort(sp.begin(), sp.end());
+        sp.erase(std::unique(sp.begin(), sp.end()), sp.end());
+        std::sort(dp.begin(), dp.end());
+        dp.erase(std::unique(dp.begin(), dp.end()), dp.end());
+        uint32_t nd = 0;
+        for (uint32_t x : dp)
+          if (!std::binary_search(sp.begin(), sp.end(), x)) ++nd;
+        nsv[i] = (uint32_t)sp.size();
+        ndv[i] = nd;
+      }
     }
-    std::sort(all.begin(), all.end());
-    const size_t lam = std::unique(all.begin(), all.end()) - all.begin();
-    std::sort(sp.begin(), sp.end());
-    sp.erase(std::unique(sp.begin(), sp.end()), sp.end());
-    std::sort(dp.begin(), dp.end());
-    dp.erase(std::unique(dp.begin(), dp.end()), dp.end());
-    size_t nd = 0;
-    for (uint32_t x : dp)
-      if (!std::binary_search(sp.begin(), sp.end(), x)) ++nd;
-    const size_t len = sp.size() + nd;
-    if (!(sp.size() <= 1 && nd == 0)) {
+  });
+  for (uint32_t i = 0; i < E; ++i) {
+    const size_t len = (size_t)nsv[i] + ndv[i];
+    if (!(nsv[i] <= 1 && ndv[i] == 0)) {
       m.connectivity += (double)hg.w[i] * (double)(len - 1);
       if (len > 1) m.cutnet += hg.w[i];
       m.quotient_pins += len;
     }
-    m.soed += (double)hg.w[i] * (double)(lam - 1) * (double)(hg.off[i + 1] - hg.off[i]);
+    m.soed += (double)hg.w[i] * (double)(lam[i] - 1) * (double)(hg.off[i + 1] - hg.off[i]);
   }
   return m;
 }
 
 int validity(const hgp_hgraph& hg, const std::vector<uint32_t>& part, uint32_t omega, uint32_t delta, uint32_t max_parts,
              std::string* why) {
+  // Partition sizes in an array indexed by id when the ids are small enough.  The map of the
+  // sequential check (built in node order) is built only to name a partition over the size
+  // limit, so that the message is the one that check gives.
+  uint32_t maxq = 0;
+  for (uint32_t p : part) maxq = std::max(maxq, p);
+  const bool arr = !part.empty() && (uint64_t)maxq < (1ull << 24);
+  std::vector<uint32_t> cnt;
   std::unordered_map<uint32_t, uint32_t> counter;
-  for (uint32_t p : part) counter[p]++;
-  const uint32_t count = (uint32_t)counter.size();
+  uint32_t count = 0;
+  bool over = false;
+  if (arr) {
+    cnt.assign((size_t)maxq + 1, 0);
+    for (uint32_t p : part) cnt[p]++;
+    for (uint32_t c : cnt) {
+      count += c != 0;
+      over |= c > omega;
+    }
+  } else {
+    for (uint32_t p : part) counter[p]++;
+    count = (uint32_t)counter.size();
+  }
   char buf[256];
   if (count > max_parts) {
     if (why) {
@@ -402,6 +636,8 @@ int validity(const hgp_hgraph& hg, const std::vector<uint32_t>& part, uint32_t o
     }
     return 0;
   }
+  if (arr && over)
+    for (uint32_t p : part) counter[p]++;
   for (auto& kv : counter)
     if (kv.second > omega) {
       if (why) {
@@ -411,20 +647,30 @@ int validity(const hgp_hgraph& hg, const std::vector<uint32_t>& part, uint32_t o
       return 0;
     }
   for (uint32_t i = 0; i < count; ++i)
-    if (!counter.count(i)) {
+    if (arr ? (i > maxq || cnt[i] == 0) : !counter.count(i)) {
       if (why) *why = "Partitions must be incrementally indexed from 0 onward.";
       return -1;
     }
-  std::vector<uint32_t> syn(count, 0);
-  std::vector<uint32_t> stamp(count, UINT32_MAX);
-  for (uint32_t i = 0; i < hg.e(); ++i)
-    for (uint64_t j = hg.off[i] + hg.srcc[i]; j < hg.off[i + 1]; ++j) {
-      const uint32_t q = part[hg.pins[j]];
-      if (stamp[q] != i) {
-        stamp[q] = i;
-        syn[q]++;
+  // inbound hedges per partition (a hedge counts once for each partition among its
+  // destinations): per-thread counts over hedge ranges, summed (integers, so exact)
+  const unsigned T = hg.pins.size() >= (1u << 20) ? host_threads() : 1u;
+  std::vector<std::vector<uint32_t>> syns(T);
+  parallel_ranges(hg.e(), T, [&](unsigned t, uint64_t lo, uint64_t hi) {
+    std::vector<uint32_t>& syn = syns[t];
+    syn.assign(count, 0);
+    std::vector<uint32_t> stamp(count, UINT32_MAX);
+    for (uint64_t i = lo; i < hi; ++i)
+      for (uint64_t j = hg.off[i] + hg.srcc[i]; j < hg.off[i + 1]; ++j) {
+        const uint32_t q = part[hg.pins[j]];
+        if (stamp[q] != (uint32_t)i) {
+          stamp[q] = (uint32_t)i;
+          syn[q]++;
+        }
       }
-    }
+  });
+  std::vector<uint32_t>& syn = syns[0];
+  for (unsigned t = 1; t < T; ++t)
+    for (uint32_t q = 0; q < count; ++q) syn[q] += syns[t][q];
   for (uint32_t i = 0; i < count; ++i)
     if (syn[i] > delta) {
       if (why) {

@@ -64,6 +64,73 @@ def test_product_loader_matches_oracle(tmp_path, fmt, rng):
             assert np.array_equal(a, b)
 
 
+def write_big_hgr(path, rng, n=60000, e=450000, fmt=11, bad_at=None):
+    """A .hgr above the threaded parser's 4 MB threshold: weights and vertex weights (fmt 11),
+    comment and blank lines, CRLF endings on some lines, repeated pins and singleton hedges.
+    With bad_at, hyperedge lines at those indexes name node n + 1."""
+    sizes = np.minimum(rng.geometric(0.3, e) + 1, 40)
+    sizes[rng.random(e) < 0.05] = 1  # singletons, skipped by the loader
+    lines = ["% threaded-parser test", f"{e} {n} {fmt}"]
+    bad = set(bad_at or [])
+    for i in range(e):
+        ps = rng.integers(1, n + 1, sizes[i])
+        if i in bad:
+            ps[-1] = n + 1
+        body = " ".join(map(str, ps))
+        if fmt in (1, 11):
+            body = f"{int(rng.integers(1, 9))} {body}"
+        lines.append(body + ("\r" if i % 7 == 3 else ""))
+        if i % 1000 == 17:
+            lines.append("   % comment inside the hyperedges")
+        if i % 5003 == 11:
+            lines.append("")
+    if fmt in (10, 11):
+        lines += [str(int(x)) for x in rng.integers(1, 5, n)]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("fmt", [0, 11])
+def test_threaded_hgr_parser_matches_oracle(tmp_path, fmt, rng):
+    p = str(tmp_path / "big.hgr")
+    write_big_hgr(p, rng, fmt=fmt)
+    assert os.path.getsize(p) > 4 << 20
+    for dedup in (False, True):
+        got = H.Hypergraph.load(p, dedup=dedup).arrays()
+        exp = O.load(p, dedup)
+        assert got[0] == exp.n
+        for a, b in zip(got[1:], (exp.off, exp.pins, exp.srcc, exp.w)):
+            assert np.array_equal(a, b)
+
+
+def test_threaded_hgr_parser_reports_errors(tmp_path, rng):
+    p = str(tmp_path / "bad.hgr")
+    write_big_hgr(p, rng, bad_at=[400000, 9])
+    hg = C.c_void_p()
+    assert H.load_library().hgp_hgraph_load(p.encode(), 0, C.byref(hg)) != 0
+    short = str(tmp_path / "short.hgr")
+    write_big_hgr(short, rng, fmt=0)
+    text = open(short).read().split("\n")
+    text[1] = text[1].replace(" 0", "").split()[0]
+    head = f"{int(text[1]) + 10} 60000"  # ten more hyperedges than the file holds
+    open(short, "w").write("\n".join([text[0], head] + text[2:]))
+    assert H.load_library().hgp_hgraph_load(short.encode(), 0, C.byref(hg)) != 0
+
+
+def test_threaded_metrics_and_validity_match_oracle(rng):
+    """Above 2^20 pins the graders count per hedge on several threads and sum in hedge order."""
+    n, off, pins, srcc, w = gen.random_hypergraph(150000, 300000, rng, max_size=12, src_prob=0.6, max_src=3,
+                                                  weights="float")
+    hg = H.Hypergraph.from_csr(n, off, pins, srcc, w, dedup=True)
+    assert hg.sizes()[2] > 1 << 20
+    orc = O.HG(*hg.arrays())
+    for k in (2, 64, 5000):
+        part = O.relabel(rng.integers(0, k, size=n).astype(np.uint32))[1]
+        assert hg.metrics(part) == O.metrics(orc, part)  # the same sums in the same order
+        for cap in ((n, 10**9, k), (n // k + 40, 400, k), (5, 10**9, k)):
+            assert hg.validity(part, *cap) == O.validity(orc, part, *cap)
+
+
 def test_product_metrics_and_validity_match_oracle(rng):
     n, off, pins, srcc, w = gen.random_hypergraph(200, 300, rng, max_size=8, src_prob=0.7, max_src=2, weights="int")
     hg = H.Hypergraph.from_csr(n, off, pins, srcc, w, dedup=True)
