@@ -27,8 +27,10 @@ namespace {
 constexpr uint32_t CAND_CAP = 1024;    // LDS slots per wave
 constexpr uint32_t CAND_LOG = 10;
 constexpr uint32_t CAND_LIGHT = 512;   // pin-visit bound served from LDS
-constexpr uint32_t CAND_SLOTS = CAND_CAP / 64;
+constexpr uint32_t CAND_LIGHT_S = 64;   // ... by the small-table instance (128 slots, 8 waves per workgroup)
+constexpr uint32_t CAND_LIGHT_M = 256;  // ... by the mid-table instance (512 slots, 8 waves per workgroup)
 constexpr uint32_t CAND_MED_CAP = 4096;   // LDS table slots of a medium node (48 KB)
+constexpr uint32_t CAND_MED_S = 2048;     // ... of the small-table medium instance
 constexpr uint32_t CAND_BIG_SLOTS = 16384;  // LDS hash slots of a big node (128 KB; half with inbound counts)
 constexpr uint32_t CAND_DIRECT_N = 39168;   // levels up to this size index LDS by node id (<= 160 KB;
                                             // half that with inbound counts)
@@ -106,12 +108,14 @@ __device__ __forceinline__ uint64_t cand_key(uint32_t u, uint32_t v, uint32_t sc
 }
 
 
-// One wave per node of visit bound b <= CAND_LIGHT.  The node's LDS table has the smallest
-// power of two >= 2b slots (at least 64), so the distinct neighbours (<= b) never fill it
-// and a node with few neighbours clears and ranks few slots.  INTER keeps the shared-inbound
-// counts (their array is not allocated otherwise).
-template <bool INTER>
-__global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
+// One wave per node of visit bound b <= CAND_LIGHT (CAP = CAND_CAP, 4 waves per workgroup),
+// b <= CAND_LIGHT_M (CAP = 512, 8 waves) or b <= CAND_LIGHT_S (CAP = 128, 8 waves): smaller
+// tables leave room for more waves per CU.  The
+// node's LDS table has the smallest power of two >= 2b slots (at least 64), so the distinct
+// neighbours (<= b) never fill it and a node with few neighbours clears and ranks few slots.
+// INTER keeps the shared-inbound counts (their array is not allocated otherwise).
+template <bool INTER, uint32_t CAP = CAND_CAP, uint32_t NW = 4>
+__global__ void __launch_bounds__(64 * NW) k_cand_light(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
                                                     const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ tch,
                                                     const uint64_t* __restrict__ toff, const uint32_t* __restrict__ inb,
                                                     const float* __restrict__ w, const uint32_t* __restrict__ sizes,
@@ -119,18 +123,19 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
                                                     const uint64_t* __restrict__ bound, uint32_t cnc, uint32_t omega,
                                                     uint32_t delta, uint32_t* __restrict__ pairs,
                                                     uint32_t* __restrict__ scores) {
-  __shared__ uint32_t s_keys[4][CAND_CAP];
-  __shared__ uint32_t s_score[4][CAND_CAP];
-  __shared__ uint32_t s_inter[INTER ? 4 : 1][INTER ? CAND_CAP : 1];
+  constexpr uint32_t SLOTS = CAP / 64;
+  __shared__ uint32_t s_keys[NW][CAP];
+  __shared__ uint32_t s_score[NW][CAP];
+  __shared__ uint32_t s_inter[INTER ? NW : 1][INTER ? CAP : 1];
   const uint32_t wv = threadIdx.x / 64, l = lane_id();
-  const uint32_t idx = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
+  const uint32_t idx = xcd_block(blockIdx.x, gridDim.x) * NW + wv;
   if (idx >= nlist) return;
   const uint32_t u = list[idx];
   uint32_t* keys = s_keys[wv];
   uint32_t* score = s_score[wv];
   uint32_t* inter = INTER ? s_inter[wv] : nullptr;
   uint32_t cap = 64;
-  while (cap < 2 * bound[u] && cap < CAND_CAP) cap <<= 1;
+  while (cap < 2 * bound[u] && cap < CAP) cap <<= 1;
   for (uint32_t s = l; s < cap; s += 64) {
     keys[s] = NONE;
     score[s] = 0;
@@ -158,9 +163,9 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const uint32_t size_u = sizes[u];
-  uint64_t cand[CAND_SLOTS];
+  uint64_t cand[SLOTS];
 #pragma unroll
-  for (uint32_t s = 0; s < CAND_SLOTS; ++s) {
+  for (uint32_t s = 0; s < SLOTS; ++s) {
     const uint32_t slot = l + 64 * s;
     const uint32_t v = slot < cap ? keys[slot] : NONE;
     cand[s] = v == NONE ? 0ull
@@ -169,7 +174,7 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
   for (uint32_t i = 0; i < cnc; ++i) {
     uint64_t m = 0;
 #pragma unroll
-    for (uint32_t s = 0; s < CAND_SLOTS; ++s) m = cand[s] > m ? cand[s] : m;
+    for (uint32_t s = 0; s < SLOTS; ++s) m = cand[s] > m ? cand[s] : m;
     const uint64_t best = wave_max(m);
     if (best == 0) {  // no candidates left: the remaining slots are empty
       for (uint32_t r = i + l; r < cnc; r += 64) {
@@ -179,7 +184,7 @@ __global__ void __launch_bounds__(256) k_cand_light(const uint32_t* __restrict__
       break;
     }
 #pragma unroll
-    for (uint32_t s = 0; s < CAND_SLOTS; ++s)
+    for (uint32_t s = 0; s < SLOTS; ++s)
       if (cand[s] == best) cand[s] = 0;
     if (l == 0) {
       pairs[(uint64_t)u * cnc + i] = (uint32_t)best;
@@ -258,8 +263,9 @@ __device__ __forceinline__ uint32_t hash_cap(uint64_t bound, uint32_t n) {
 // at most direct_n nodes, else hashed with <= big_cap slots), 3 heavy (HBM tables).
 // cap[v] = table slots; a heavy node whose table would not be smaller than n indexes directly.
 __global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uint32_t direct_n, uint32_t big_cap,
-                             uint32_t* __restrict__ f_light, uint32_t* __restrict__ f_med, uint32_t* __restrict__ f_big,
-                             uint32_t* __restrict__ cap) {
+                             uint32_t* __restrict__ f_small, uint32_t* __restrict__ f_mid,
+                             uint32_t* __restrict__ f_light, uint32_t* __restrict__ f_med_s,
+                             uint32_t* __restrict__ f_med, uint32_t* __restrict__ f_big, uint32_t* __restrict__ cap) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const uint64_t b = bound[v];
@@ -267,23 +273,36 @@ __global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uin
   const bool light = b <= CAND_LIGHT;
   const bool med = !light && c <= CAND_MED_CAP;
   const bool big = !light && !med && (n <= direct_n || c <= big_cap);
-  f_light[v] = light ? 1u : 0u;
-  f_med[v] = med ? 1u : 0u;
+  f_small[v] = b <= CAND_LIGHT_S ? 1u : 0u;
+  f_mid[v] = b > CAND_LIGHT_S && b <= CAND_LIGHT_M ? 1u : 0u;
+  f_light[v] = light && b > CAND_LIGHT_M ? 1u : 0u;
+  f_med_s[v] = med && c <= CAND_MED_S ? 1u : 0u;
+  f_med[v] = med && c > CAND_MED_S ? 1u : 0u;
   f_big[v] = big ? 1u : 0u;
   cap[v] = (big && n <= direct_n) || (!light && !med && !big && (uint64_t)n <= c) ? n : c;
 }
 
-__global__ void k_split4(const uint32_t* __restrict__ fa, const uint32_t* __restrict__ fb,
-                         const uint32_t* __restrict__ fc, const uint64_t* __restrict__ pa,
-                         const uint64_t* __restrict__ pb, const uint64_t* __restrict__ pc, uint32_t n,
-                         uint32_t* __restrict__ la, uint32_t* __restrict__ lb, uint32_t* __restrict__ lc,
-                         uint32_t* __restrict__ ld) {
+// node lists of the tiers flagged in f[0..T-2] (exclusive scans p[0..T-2]); the rest go to
+// the last list
+constexpr int TIERS = 7;
+struct TierLists {
+  const uint32_t* f[TIERS - 1];
+  const uint64_t* p[TIERS - 1];
+  uint32_t* l[TIERS];
+};
+__global__ void k_split_tiers(TierLists t, uint32_t n) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  if (fa[v]) la[pa[v]] = v;
-  else if (fb[v]) lb[pb[v]] = v;
-  else if (fc[v]) lc[pc[v]] = v;
-  else ld[v - pa[v] - pb[v] - pc[v]] = v;
+  uint64_t before = 0;
+#pragma unroll
+  for (int i = 0; i < TIERS - 1; ++i) {
+    if (t.f[i][v]) {
+      t.l[i][t.p[i][v]] = v;
+      return;
+    }
+    before += t.p[i][v];
+  }
+  t.l[TIERS - 1][v - before] = v;
 }
 
 // Largest table among the listed nodes, each capped at `try_cap`.
@@ -400,11 +419,13 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
       const uint32_t *__restrict__ caps, uint32_t cnc, uint32_t omega, uint32_t delta, uint32_t *__restrict__ pairs, \
       uint32_t *__restrict__ scores
 
-template <bool INTER>
+// CAP: the LDS table (CAND_MED_CAP, or CAND_MED_S for nodes whose table fits it: half the LDS,
+// so twice the workgroups per CU)
+template <bool INTER, uint32_t CAP = CAND_MED_CAP>
 __global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS) {
-  __shared__ uint32_t s_keys[CAND_MED_CAP];
-  __shared__ uint32_t s_score[CAND_MED_CAP];
-  __shared__ uint32_t s_inter[INTER ? CAND_MED_CAP : 1];
+  __shared__ uint32_t s_keys[CAP];
+  __shared__ uint32_t s_score[CAP];
+  __shared__ uint32_t s_inter[INTER ? CAP : 1];
   __shared__ uint64_t s_w[64];
   const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
   cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
@@ -690,7 +711,7 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   const uint32_t big_cap = inter ? CAND_BIG_SLOTS / 2 : CAND_BIG_SLOTS;
   const uint32_t g = ceil_div(n, 256);
   DBuf<uint64_t> bound(c, n);
-  DBuf<uint32_t> fl(c, n), fm(c, n), fb(c, n), caps(c, n);
+  DBuf<uint32_t> fs(c, n), fmid(c, n), fl(c, n), fms(c, n), fm(c, n), fb(c, n), caps(c, n);
   {
     DBuf<uint32_t> heavy(c, n), nheavy(c, 1);
     memset_async(c, nheavy.get(), 0, 4);
@@ -698,22 +719,53 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     HGP_LAUNCH(k_visit_bound_heavy, std::min<uint32_t>(n, VISIT_HEAVY_GRID), 256, 0, c, hoff, tch, toff, heavy.get(),
                nheavy.get(), bound.get());
   }
-  HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, direct_n, big_cap, fl.get(), fm.get(), fb.get(), caps.get());
-  DBuf<uint64_t> lpos(c, (uint64_t)n + 1), mpos(c, (uint64_t)n + 1), bpos(c, (uint64_t)n + 1);
-  exclusive_scan(c, fl.get(), lpos.get(), n);
-  exclusive_scan(c, fm.get(), mpos.get(), n);
-  exclusive_scan(c, fb.get(), bpos.get(), n);
-  const uint32_t nl = (uint32_t)read_scalar(c, lpos.get() + n);
-  const uint32_t nm = (uint32_t)read_scalar(c, mpos.get() + n);
-  const uint32_t nb = (uint32_t)read_scalar(c, bpos.get() + n);
-  const uint32_t nh = n - nl - nm - nb;
-  DBuf<uint32_t> light(c, nl), med(c, nm), big(c, nb), heavy(c, nh);
-  HGP_LAUNCH(k_split4, g, 256, 0, c, fl.get(), fm.get(), fb.get(), lpos.get(), mpos.get(), bpos.get(), n, light.get(),
-             med.get(), big.get(), heavy.get());
+  HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, direct_n, big_cap, fs.get(), fmid.get(), fl.get(), fms.get(),
+             fm.get(), fb.get(), caps.get());
+  // tier lists: small, mid and light tables, small and large medium tables, big or direct, heavy
+  const uint32_t* flags[TIERS - 1] = {fs.get(), fmid.get(), fl.get(), fms.get(), fm.get(), fb.get()};
+  DBuf<uint64_t> pos[TIERS - 1];
+  uint32_t cnt[TIERS];
+  uint32_t rest = n;
+  for (int i = 0; i < TIERS - 1; ++i) {
+    pos[i] = DBuf<uint64_t>(c, (uint64_t)n + 1);
+    exclusive_scan(c, flags[i], pos[i].get(), n);
+  }
+  for (int i = 0; i < TIERS - 1; ++i) {
+    cnt[i] = (uint32_t)read_scalar(c, pos[i].get() + n);
+    rest -= cnt[i];
+  }
+  cnt[TIERS - 1] = rest;
+  DBuf<uint32_t> lists[TIERS];
+  TierLists tl{};
+  for (int i = 0; i < TIERS; ++i) {
+    lists[i] = DBuf<uint32_t>(c, cnt[i]);
+    tl.l[i] = lists[i].get();
+    if (i < TIERS - 1) {
+      tl.f[i] = flags[i];
+      tl.p[i] = pos[i].get();
+    }
+  }
+  HGP_LAUNCH(k_split_tiers, g, 256, 0, c, tl, n);
+  const uint32_t ns = cnt[0], nmid = cnt[1], nl = cnt[2], nms = cnt[3], nm = cnt[4], nb = cnt[5], nh = cnt[6];
+  DBuf<uint32_t>&small = lists[0], &mid = lists[1], &light = lists[2], &med_s = lists[3], &med = lists[4],
+                 &big = lists[5], &heavy = lists[6];
   // the tiers write disjoint rows of pairs and scores: light and medium nodes on side streams
   // beside the big and heavy ones (which allocate, so they stay on the main stream)
   Fork fk(c, c.tune.side_streams & 1u);
   fk.on(0, [&] {
+    // visit bounds up to CAND_LIGHT_S: 128-slot tables, 8 waves per workgroup
+    if (inter)
+      HGP_LAUNCH((k_cand_light<true, 2 * CAND_LIGHT_S, 8>), ceil_div(ns, 8), 512, 0, c, hedges, hoff, srcc, tch, toff,
+                 inb, w, sizes, small.get(), ns, bound.get(), cnc, omega, delta, pairs, scores);
+    else
+      HGP_LAUNCH((k_cand_light<false, 2 * CAND_LIGHT_S, 8>), ceil_div(ns, 8), 512, 0, c, hedges, hoff, srcc, tch, toff,
+                 inb, w, sizes, small.get(), ns, bound.get(), cnc, omega, delta, pairs, scores);
+    if (inter)
+      HGP_LAUNCH((k_cand_light<true, 2 * CAND_LIGHT_M, 8>), ceil_div(nmid, 8), 512, 0, c, hedges, hoff, srcc, tch,
+                 toff, inb, w, sizes, mid.get(), nmid, bound.get(), cnc, omega, delta, pairs, scores);
+    else
+      HGP_LAUNCH((k_cand_light<false, 2 * CAND_LIGHT_M, 8>), ceil_div(nmid, 8), 512, 0, c, hedges, hoff, srcc, tch,
+                 toff, inb, w, sizes, mid.get(), nmid, bound.get(), cnc, omega, delta, pairs, scores);
     if (inter)
       HGP_LAUNCH(k_cand_light<true>, ceil_div(nl, 4), 256, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes,
                  light.get(), nl, bound.get(), cnc, omega, delta, pairs, scores);
@@ -725,6 +777,8 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   HGP_LAUNCH(K, cnt, threads, 0, c, hedges, hoff, srcc, tch, toff, inb, w, sizes, list, caps.get(), cnc, omega, delta, \
              pairs, scores)
   fk.on(1, [&] {
+    if (inter) CAND_LAUNCH((k_cand_medium<true, CAND_MED_S>), med_s.get(), nms, 256);
+    else CAND_LAUNCH((k_cand_medium<false, CAND_MED_S>), med_s.get(), nms, 256);
     if (inter) CAND_LAUNCH(k_cand_medium<true>, med.get(), nm, 256);
     else CAND_LAUNCH(k_cand_medium<false>, med.get(), nm, 256);
   });
