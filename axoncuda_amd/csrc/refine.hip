@@ -169,7 +169,9 @@ __host__ __device__ __forceinline__ uint32_t pp_table_cap(uint64_t len, uint32_t
 
 // One wave per hedge with more than 16 pins.  GMEM = false: the (partition, count, dst
 // count) table lives in LDS with `cap` slots; GMEM = true (hedges whose table would not fit
-// in LDS): in HBM at gtab + 3 * goff[i], sized by pp_table_cap.
+// in LDS): in HBM at gtab + 3 * goff[i], sized by pp_table_cap.  The list is written sorted
+// by partition id (the table, compacted into its own first slots and padded to a power of
+// two, is bitonic-sorted by the wave), so a partition's entry is found by binary search.
 template <bool INB, bool GMEM>
 __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ hedges, const uint64_t* __restrict__ hoff,
                                                  const uint32_t* __restrict__ srcc, const uint32_t* __restrict__ part,
@@ -211,38 +213,63 @@ __global__ void __launch_bounds__(64) k_pp_large(const uint32_t* __restrict__ he
   __syncthreads();
   // HBM tables were written by atomics at the device cache: read them there too
   auto ld = [](uint32_t* x) { return GMEM ? atomic_load_relaxed(x) : *x; };
+  // compaction: the occupied slots, in slot order, into registers of the lanes ahead of the
+  // writes (a chunk of 64 slots is read before any of it is overwritten)
   uint32_t base = 0;
-  uint2 first = make_uint2(0u, 0u);  // entry 0, held by the lane that wrote it
-  bool wrote_first = false;
   for (uint32_t b = 0; b < cap; b += 64) {
     const uint32_t i = b + l;
     const uint32_t q = ld(&keys[i]);
+    const uint32_t c = ld(&cnt[i]), d = ld(&dcnt[i]);
     const bool has = q != NONE;
     const uint64_t m = __ballot(has);
-    if (has) {
+    if (GMEM) __threadfence_block();  // this chunk's reads are done before any lane writes
+    __builtin_amdgcn_wave_barrier();
+    if (has) {  // base + rank <= b + l: every write lands on a slot this or an earlier chunk read
       const uint32_t o = base + lanes_below(m);
-      const uint2 ent = make_uint2(q, ld(&cnt[i]));
-      pp[s + o] = ent;
-      if (o == 0) {
-        first = ent;
-        wrote_first = true;
-      }
-      if (INB) {
-        const uint32_t dc = ld(&dcnt[i]);
-        pp_dcnt[s + o] = dc;
-        if (dc) atomicAdd(&inbsz[q], 1u);
-      }
+      keys[o] = q;
+      cnt[o] = c;
+      dcnt[o] = d;
+      if (INB && d) atomicAdd(&inbsz[q], 1u);
     }
+    if (GMEM) __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
     base += (uint32_t)__popcll(m);
   }
-  const uint64_t fm = __ballot(wrote_first);
-  const int src = fm ? __ffsll((long long)fm) - 1 : 0;
-  first.x = __shfl(first.x, src, 64);
-  first.y = __shfl(first.y, src, 64);
+  // pad to a power of two and sort the entries by partition id (NONE, the padding, last)
+  uint32_t np = 1;
+  while (np < base) np <<= 1;
+  for (uint32_t i = base + l; i < np; i += 64) keys[i] = NONE;
+  if (GMEM) __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t kk = 2; kk <= np; kk <<= 1) {
+    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+      for (uint32_t tt = l; tt < np / 2; tt += 64) {
+        const uint32_t i = (tt / j) * 2 * j + (tt % j), ixj = i + j;
+        const uint32_t a = ld(&keys[i]), bq = ld(&keys[ixj]);
+        const bool up = (i & kk) == 0;
+        if ((a > bq) == up) {
+          const uint32_t ca = ld(&cnt[i]), da = ld(&dcnt[i]);
+          keys[i] = bq;
+          cnt[i] = ld(&cnt[ixj]);
+          dcnt[i] = ld(&dcnt[ixj]);
+          keys[ixj] = a;
+          cnt[ixj] = ca;
+          dcnt[ixj] = da;
+        }
+      }
+      if (GMEM) __threadfence_block();
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  const uint64_t s0 = s;
+  for (uint32_t o = l; o < base; o += 64) {
+    pp[s0 + o] = make_uint2(ld(&keys[o]), ld(&cnt[o]));
+    if (INB) pp_dcnt[s0 + o] = ld(&dcnt[o]);
+  }
   if (l == 0) {
     lambda[e] = base;
     // a single entry rides in the record in place of the list start
-    const uint64_t x = base == 1 ? (((uint64_t)first.y << 32) | first.x) : s;
+    const uint64_t x = base == 1 ? (((uint64_t)ld(&cnt[0]) << 32) | ld(&keys[0])) : s0;
     hinfo[e] = make_uint4((uint32_t)x, (uint32_t)(x >> 32), base, w ? fixed_weight(w[e]) : 0u);
   }
 }
@@ -2081,15 +2108,30 @@ __global__ void k_inbound_events(const uint32_t* __restrict__ part, const uint32
   }
 }
 // count of destination pins of hedge e in partition q (0 when absent).  The lists start at
-// the hedge's pin offset; a list of one entry is read from the record.
+// the hedge's pin offset; a list of one entry is read from the record.  Lists of more than
+// PP_SMALL entries (only hedges of more than PP_SMALL pins have them) are sorted by
+// partition id: binary search; shorter ones are scanned.
 __device__ __forceinline__ int64_t dst_count(const uint4* hinfo, const uint64_t* hoff, const uint2* pp,
                                              const uint32_t* pp_dcnt, uint32_t e, uint32_t q) {
   const uint4 hi = hinfo[e];
   const uint64_t s = hoff[e];
   if (hi.z == 1) return hi.x == q ? pp_dcnt[s] : 0;
-  for (uint32_t i = 0; i < hi.z; ++i)
-    if (pp[s + i].x == q) return pp_dcnt[s + i];
-  return 0;
+  if (hi.z <= PP_SMALL) {
+    for (uint32_t i = 0; i < hi.z; ++i)
+      if (pp[s + i].x == q) return pp_dcnt[s + i];
+    return 0;
+  }
+  uint32_t lo = 0, n = hi.z;  // first entry with partition >= q
+  while (n > 0) {
+    const uint32_t h = n / 2;
+    if (pp[s + lo + h].x < q) {
+      lo += h + 1;
+      n -= h + 1;
+    } else {
+      n = h;
+    }
+  }
+  return lo < hi.z && pp[s + lo].x == q ? pp_dcnt[s + lo] : 0;
 }
 // The first event of every (partition, hedge) run looks up the run's starting destination
 // count once (a scan of the hedge's partition list); the run's other events read it there.
