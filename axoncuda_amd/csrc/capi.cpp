@@ -124,7 +124,7 @@ bool from_public(const hgp_tuning* o, hgp::Tuning& t) {
   t.scan_lookback_tiles = o->scan_lookback_tiles;
   t.mask_wide4_degree = o->mask_wide4_degree;
   t.mask_wide16_degree = o->mask_wide16_degree;
-  t.side_streams = o->side_streams & 3u;
+  t.side_streams = o->side_streams & 7u;
   t.coop_max_n = o->coop_max_n;
   t.coop_tile = o->coop_tile;
   t.tuned = o->tuned != 0;

@@ -61,7 +61,8 @@ struct Tuning {
   uint32_t scan_lookback_tiles = 64;  // largest scan (in tiles) that takes the single-pass look-back
   uint32_t mask_wide4_degree = 128;   // mean touching hedges per node from which k-way gains take
   uint32_t mask_wide16_degree = 512;  // 4 (16) waves per node
-  uint32_t side_streams = 2;          // bit 0: candidate tiers, bit 1: cascade classes beside events
+  uint32_t side_streams = 2;          // bit 0: candidate tiers, bit 1: cascade classes beside events,
+                                      // bit 2: ... and beside the small-hedge cascade
   uint32_t coop_max_n = 65536;        // sorts of RS1 .. this many keys run every pass in one launch
   uint32_t coop_tile = 1024;          // keys per workgroup of those launches (at least)
   bool tuned = false;

@@ -2709,13 +2709,18 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
                    Y.cs_pins.get(), Y.cs_w.get());
       }
     }
-    if (Y.cas_small_n) {
+    auto small_cascade = [&] {
+      if (!Y.cas_small_n) return;
       const SmallHedges sh{Y.cs_off.get(), Y.cs_pins.get(), Y.cs_w.get(), Y.cas_small_n};
       const uint32_t cb = c.tune.cascade_block;  // 256 unless the autotuner chose otherwise
       HGP_LAUNCH(k_cascade_small, ceil_div(Y.cas_small_n, cb), cb, 0, c, a, sh);
       if (prof_match(c, "k_cascade_small") && c.prof_count && c.prof_dev)
         HGP_LAUNCH(k_cascade_small_bytes, ceil_div(Y.cas_small_n, 256), 256, 0, c, a, sh, c.prof_dev);
-    }
+    };
+    // bit 2 of side_streams: the side streams start before the small-hedge cascade and run
+    // beside it too (otherwise after it, beside the events only)
+    const bool early = (c.tune.side_streams & 6u) == 6u;
+    if (!early) small_cascade();
     cas_fork.emplace(c, (c.tune.side_streams & 2u) != 0);
     for (int cls = 0; cls < 3; ++cls) {
       const uint32_t m = Y.cas_n[cls];
@@ -2728,6 +2733,7 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
         else HGP_LAUNCH(k_cascade_large, m, 64, 0, c, a, lst);
       });
     }
+    if (early) small_cascade();
   }
   const int rb = std::max(1, bit_width(V - 1));
   const int kb = std::max(1, bit_width(L.k));

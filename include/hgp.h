@@ -122,7 +122,8 @@ typedef struct hgp_tuning {
   uint32_t mask_wide4_degree;
   uint32_t mask_wide16_degree;
   /* independent kernels of one stage on side streams (default priority) beside the context's
-   * stream: bit 0 the candidate tiers, bit 1 the cascade's hedge classes */
+   * stream: bit 0 the candidate tiers, bit 1 the cascade's sorted hedge classes beside the
+   * size and inbound events, bit 2 (with bit 1) beside the small-hedge cascade too */
   uint32_t side_streams;
   /* radix sorts of more than 16,384 and at most coop_max_n keys run all their passes in one
    * launch of at most 64 workgroups that meet at grid barriers, each taking at least
