@@ -1638,9 +1638,12 @@ __global__ void k_chain_commit(ChainArrays c, uint32_t n, int it) {
   if (c.next[p] == NONE) c.next[p] = s;
   c.progress[it] = 1u;
 }
+// comp / pos: each move's component (its chain's head) and its distance from the head; a
+// chain whose head is not found within 256 steps (a cycle, or a longer chain) is keyed by
+// its lowest index at position 0 and raises *odd.  csize counts each component's moves.
 __global__ void k_chain_comp(ChainArrays c, uint32_t n, uint32_t* __restrict__ comp, uint32_t* __restrict__ pos,
                              uint32_t* __restrict__ cflag, long long* __restrict__ cw,
-                             uint32_t* __restrict__ cinv) {
+                             uint32_t* __restrict__ cinv, uint32_t* __restrict__ csize, uint32_t* __restrict__ odd) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t cur = i, mn = i, cc = NONE, ps = 0;
@@ -1657,10 +1660,12 @@ __global__ void k_chain_comp(ChainArrays c, uint32_t n, uint32_t* __restrict__ c
   if (cc == NONE) {
     cc = mn;
     ps = 0;
+    *odd = 1u;
   }
   comp[i] = cc;
   pos[i] = ps;
   cflag[cc] = 1u;
+  atomicAdd(&csize[cc], 1u);
   if (c.ok[i]) atomicAdd((unsigned long long*)&cw[cc], (unsigned long long)c.w[i]);  // two's complement sum
   else atomicOr(&cinv[cc], 1u);
 }
@@ -1677,6 +1682,25 @@ __global__ void k_comp_list(const uint32_t* __restrict__ cflag, const uint64_t* 
 __global__ void k_comp_rank(const uint32_t* __restrict__ clist, uint32_t m, uint32_t* __restrict__ crank) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < m) crank[clist[r]] = r;
+}
+// sizes of the components in rank order
+__global__ void k_comp_sizes(const uint32_t* __restrict__ clist, const uint32_t* __restrict__ csize, uint32_t m,
+                             uint32_t* __restrict__ out) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < m) out[r] = csize[clist[r]];
+}
+// Every chain has its head: a move's rank is its component's start (the sizes of the
+// components ranked before it) plus its distance from the head -- the position the stable
+// sort by (component rank, distance) would give it, distances being unique within a chain.
+__global__ void k_chain_direct_ranks(const uint32_t* __restrict__ comp, const uint32_t* __restrict__ pos,
+                                     const uint32_t* __restrict__ crank, const uint64_t* __restrict__ start,
+                                     const uint32_t* __restrict__ orig, uint32_t n, uint32_t* __restrict__ ranks) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) ranks[orig[j]] = (uint32_t)(start[crank[comp[j]]] + pos[j]);
+}
+__global__ void k_pack2(const uint64_t* __restrict__ a, const uint32_t* __restrict__ b, uint64_t* __restrict__ out) {
+  out[0] = a[0];
+  out[1] = b[0];
 }
 __global__ void k_edge_keys(const uint32_t* __restrict__ comp, const uint32_t* __restrict__ pos,
                             const uint32_t* __restrict__ crank, uint32_t n, uint64_t* __restrict__ keys,
@@ -2589,13 +2613,20 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
     HGP_LAUNCH(k_chain_resolve, ceil_div(n, RESOLVE_BLOCK), RESOLVE_BLOCK, 0, c, ca, n, it);
     HGP_LAUNCH(k_chain_commit, g, 256, 0, c, ca, n, it);
   }
-  DBuf<uint32_t> comp(c, n), pos(c, n), cflag(c, n), cinv(c, n);
+  DBuf<uint32_t> comp(c, n), pos(c, n), cflag(c, n), cinv(c, n), csize(c, n), odd(c, 1);
   DBuf<long long> cw(c, n);
   HGP_LAUNCH(k_zero3, g, 256, 0, c, (unsigned long long*)cw.get(), cflag.get(), cinv.get(), (uint64_t)n);
-  HGP_LAUNCH(k_chain_comp, g, 256, 0, c, ca, n, comp.get(), pos.get(), cflag.get(), cw.get(), cinv.get());
-  DBuf<uint64_t> cpos(c, (uint64_t)n + 1);
+  fill_u32(c, csize.get(), 0u, n);
+  fill_u32(c, odd.get(), 0u, 1);
+  HGP_LAUNCH(k_chain_comp, g, 256, 0, c, ca, n, comp.get(), pos.get(), cflag.get(), cw.get(), cinv.get(), csize.get(),
+             odd.get());
+  DBuf<uint64_t> cpos(c, (uint64_t)n + 1), mo(c, 2);
   exclusive_scan(c, cflag.get(), cpos.get(), n);
-  const uint32_t m = (uint32_t)read_scalar(c, cpos.get() + n);
+  HGP_LAUNCH(k_pack2, 1, 1, 0, c, cpos.get() + n, odd.get(), mo.get());  // one read-back for both
+  uint64_t mo_h[2];
+  copy_d2h(c, mo_h, mo.get(), sizeof mo_h);
+  const uint32_t m = (uint32_t)mo_h[0];
+  const bool direct = mo_h[1] == 0;
   DBuf<uint32_t> clist(c, m), cinvk(c, m), crank(c, n);
   DBuf<uint64_t> ckey(c, m);
   HGP_LAUNCH(k_comp_list, g, 256, 0, c, cflag.get(), cpos.get(), n, cw.get(), cinv.get(), clist.get(), ckey.get(),
@@ -2608,6 +2639,15 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   HGP_LAUNCH(k_gather_u32, ceil_div(m, 256), 256, 0, c, cinv.get(), clist.get(), m, inv2.get());
   radix_sort_buf(c, inv2, clist, m, 0, 1);
   HGP_LAUNCH(k_comp_rank, ceil_div(m, 256), 256, 0, c, clist.get(), m, crank.get());
+  if (direct) {  // every chain has its head: ranks from component starts and distances
+    DBuf<uint32_t> rsz(c, m);
+    DBuf<uint64_t> start(c, (uint64_t)m + 1);
+    HGP_LAUNCH(k_comp_sizes, ceil_div(m, 256), 256, 0, c, clist.get(), csize.get(), m, rsz.get());
+    exclusive_scan(c, rsz.get(), start.get(), m);
+    HGP_LAUNCH(k_chain_direct_ranks, g, 256, 0, c, comp.get(), pos.get(), crank.get(), start.get(), orig.get(), n,
+               ranks);
+    return;
+  }
   // final order: (component rank, position), ties in sorted-space index order: the keys are
   // laid out in index order and the sort is stable
   DBuf<uint64_t> ek(c, n);
