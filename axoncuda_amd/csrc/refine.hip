@@ -658,13 +658,14 @@ __global__ void __launch_bounds__(256) k_gains_heavy(GainArgs a, const uint32_t*
 // inserted only while fewer than GB_LIMIT are present; a node that meets one more sets the
 // overflow flag and is listed in `ovf` for the HBM-table kernel.  At most one insert per
 // thread can pass the check concurrently, so GB_LIMIT + 1024 < GB_SLOTS keeps a free slot
-// for every probe.
+// for every probe.  The slots a node inserts are also listed (GB_LIMIT + 1024 entries), so it
+// ranks and clears only those instead of the whole table.
 constexpr uint32_t LIGHT_GRID = 16384;  // workgroups of the persistent light kernel (4 nodes each at a time)
 constexpr uint32_t BLOCK_GRID = 2048;   // workgroups of the persistent block kernel
 constexpr uint64_t BOUND_UNKNOWN = ~0ull;  // a node routed by degree: its bound was not summed
 constexpr uint32_t WIDE_LIGHT_DEG = 256;   // ... and it takes a wave (else a workgroup) up to this degree
 constexpr uint32_t GB_DIRECT = 20160;  // 8 bytes per slot
-constexpr uint32_t GB_SLOTS = 13440;   // 12 bytes per slot
+constexpr uint32_t GB_SLOTS = 10496;   // 12 bytes per slot, and 3 for the list of inserted slots
 constexpr uint32_t GB_LIMIT = GB_SLOTS / 4 * 3;
 static_assert(GB_LIMIT + 1024 < GB_SLOTS, "the table must keep a free slot for every in-flight insert");
 
@@ -685,6 +686,7 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
   constexpr uint32_t SLOTS = DIRECT ? GB_DIRECT : GB_SLOTS;
   __shared__ int64_t s_conn[SLOTS];
   __shared__ uint32_t s_keys[DIRECT ? 1 : SLOTS];
+  __shared__ uint32_t s_ins[DIRECT ? 1 : GB_LIMIT + 1024];  // slots inserted for this node
   __shared__ int64_t s_sum[2][16];
   __shared__ int64_t s_bg[16];
   __shared__ uint32_t s_bh[16], s_bq[16];
@@ -692,12 +694,27 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
   __shared__ uint32_t s_first;
   const uint32_t cap = DIRECT ? a.k : SLOTS;
   const uint32_t nlist = *count;  // persistent: the grid strides over the listed nodes
+  if (!DIRECT) {  // the hashed table is cleared once; each node then clears the slots it used
+    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
+      s_conn[i] = 0;
+      s_keys[i] = NONE;
+    }
+  }
+  // the hashed table's slots used by the node: cleared (with the node's state reset after a barrier)
+  auto clear_used = [&] {
+    __syncthreads();  // every thread is done with the table and s_state
+    const uint32_t used = min(s_state[0], GB_LIMIT + 1024);
+    for (uint32_t i = threadIdx.x; i < used; i += blockDim.x) {
+      const uint32_t h = s_ins[i];
+      s_keys[h] = NONE;
+      s_conn[h] = 0;
+    }
+  };
   for (uint32_t bi = blockIdx.x; bi < nlist; bi += gridDim.x) {
   const uint32_t v = list[bi];
-  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
-    s_conn[i] = 0;
-    if (!DIRECT) s_keys[i] = NONE;
-  }
+  if (DIRECT)
+    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) s_conn[i] = 0;
+  __syncthreads();  // the previous node's clearing is done before its state is reset
   if (threadIdx.x == 0) {
     s_state[0] = 0;
     s_state[1] = 0;
@@ -734,7 +751,7 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
           }
           const uint32_t old = atomicCAS(&s_keys[h], NONE, q);
           if (old == NONE) {
-            atomicAdd(&s_state[0], 1u);
+            s_ins[atomicAdd(&s_state[0], 1u)] = h;  // below GB_LIMIT + 1024: one in flight per thread
             break;
           }
           if (old == q) break;
@@ -754,14 +771,34 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
   const bool overflow = !DIRECT && s_state[1];
   if (overflow) {
     if (threadIdx.x == 0) ovf[atomicAdd(novf, 1u)] = v;
-    __syncthreads();  // s_state is read by every thread before the next node resets it
+    clear_used();
     continue;
   }
   int64_t base = 0;
   for (int i = 0; i < 16; ++i) base += s_sum[0][i] - s_sum[1][i];
   int64_t bg;
   uint32_t bh, bq;
-  best_adjacent(a, DIRECT ? nullptr : s_keys, s_conn, cap, threadIdx.x, blockDim.x, p, s, base, bg, bh, bq);
+  if (DIRECT) {
+    best_adjacent(a, nullptr, s_conn, cap, threadIdx.x, blockDim.x, p, s, base, bg, bh, bq);
+  } else {  // the inserted slots only
+    bg = INT64_MIN;
+    bh = 0;
+    bq = NONE;
+    const uint32_t used = s_state[0];
+    for (uint32_t i = threadIdx.x; i < used; i += blockDim.x) {
+      const uint32_t hs = s_ins[i];
+      const uint32_t q = s_keys[hs];
+      const int64_t c = s_conn[hs];
+      if (c <= 0 || !target_ok(a, q, p, s)) continue;
+      const int64_t g = base + c;
+      const uint32_t h = mix32(q + a.repeat);
+      if (bq == NONE || g > bg || (g == bg && h > bh)) {
+        bg = g;
+        bh = h;
+        bq = q;
+      }
+    }
+  }
   wave_best(bg, bh, bq);
   if (l == 0) {
     s_bg[wv] = bg;
@@ -803,7 +840,7 @@ __global__ void __launch_bounds__(1024) k_gains_block(GainArgs a, const uint32_t
     a.moves[v] = q;
     a.gains[v] = q == NONE ? 0 : g;
   }
-  __syncthreads();  // the shared state is read before the next node resets it
+  if (!DIRECT) clear_used();
   }
 }
 
