@@ -2321,17 +2321,39 @@ __global__ void __launch_bounds__(256) k_best_final(const Best* __restrict__ par
   Best b = block_best(x);
   if (threadIdx.x == 0) *out = b;
 }
-__global__ void k_apply(const uint32_t* __restrict__ moves, const uint32_t* __restrict__ ranks,
-                        const uint32_t* __restrict__ sizes, uint32_t n, uint32_t best, uint32_t* __restrict__ part,
-                        uint32_t* psizes) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const uint32_t q = moves[v];
-  if (q == NONE || ranks[v] > best) return;
-  const uint32_t s = sizes[v];
-  atomicSub(&psizes[part[v]], s);
-  atomicAdd(&psizes[q], s);
-  part[v] = q;
+// Moves of rank <= best.  With k <= APPLY_LDS_K the size changes are summed per partition in
+// LDS and added once per workgroup (with few partitions, one atomic per moved node would
+// queue every node of the grid on the same few addresses).
+constexpr uint32_t APPLY_LDS_K = 1024;
+__global__ void __launch_bounds__(256) k_apply(const uint32_t* __restrict__ moves, const uint32_t* __restrict__ ranks,
+                                               const uint32_t* __restrict__ sizes, uint32_t n, uint32_t best,
+                                               uint32_t* __restrict__ part, uint32_t* psizes, uint32_t k) {
+  __shared__ long long s_d[APPLY_LDS_K];
+  const bool lds = k <= APPLY_LDS_K;
+  if (lds) {
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) s_d[i] = 0;
+    __syncthreads();
+  }
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) {
+    const uint32_t q = moves[v];
+    if (q != NONE && ranks[v] <= best) {
+      const uint32_t s = sizes[v], p = part[v];
+      if (lds) {
+        atomicAdd((unsigned long long*)&s_d[p], (unsigned long long)(-(long long)s));
+        atomicAdd((unsigned long long*)&s_d[q], (unsigned long long)s);
+      } else {
+        atomicSub(&psizes[p], s);
+        atomicAdd(&psizes[q], s);
+      }
+      part[v] = q;
+    }
+  }
+  if (lds) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x)
+      if (s_d[i]) atomicAdd(&psizes[i], (uint32_t)s_d[i]);  // two's complement: a net decrease wraps back
+  }
 }
 
 __global__ void k_connectivity(const uint32_t* __restrict__ lambda, const float* __restrict__ w, uint32_t E,
@@ -2929,7 +2951,7 @@ void refine(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint32_t del
               iv, (long long)gain, apply ? "apply" : "skip");
     if (apply) {
       HGP_LAUNCH(k_apply, ceil_div(L.n, 256), 256, 0, c, st.moves.get(), st.ranks.get(), L.sizes, L.n, st.best.r, part,
-                 psizes);
+                 psizes, L.k);
     } else {
       // schedule of sources/refinement.cu:539-542
       if (sv > 0 && !chainup) chainup = true;
