@@ -954,6 +954,7 @@ __global__ void __launch_bounds__(256) k_gains_route(GainArgs a, uint32_t n, uin
 // gain of moving to q is saving - Wv + conn(q), exactly what the list kernels compute, at one
 // 16-byte record per touching hedge whatever the hedge's connectivity.
 constexpr uint32_t MASK_K = 64;
+constexpr uint32_t MASK_SMALL_K = 8;  // partitions up to which the wide kernels sum in registers
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int o) {
   const uint32_t lo = __shfl_xor((uint32_t)v, o, 64), hi = __shfl_xor((uint32_t)(v >> 32), o, 64);
@@ -1276,6 +1277,12 @@ __global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const ui
     const uint64_t stride = 64 * W;
     const uint64_t notp = ~(1ull << p);
     int64_t wl = 0, sl = 0;
+    // with at most MASK_SMALL_K partitions every lane sums its hedges' connections in registers
+    // (LDS atomics would all meet on the same few slots) and the wave sums them at the end
+    const bool small_k = a.k <= MASK_SMALL_K;
+    int64_t creg[MASK_SMALL_K];
+#pragma unroll
+    for (uint32_t q = 0; q < MASK_SMALL_K; ++q) creg[q] = 0;
     // lane l walks entries j, j + stride, ...: the list entry two chunks ahead and the record
     // one chunk ahead are in flight while the current record is added
     uint64_t j = t0 + 64 * wv + l;
@@ -1299,12 +1306,16 @@ __global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const ui
       wl += we;
       if ((sg >> p) & 1ull) sl += we;
       uint64_t m = (((uint64_t)r0.y << 32) | r0.x) & notp;
-      if (we)
+      if (small_k) {
+#pragma unroll
+        for (uint32_t q = 0; q < MASK_SMALL_K; ++q) creg[q] += ((m >> q) & 1ull) ? (int64_t)we : 0;
+      } else if (we) {
         while (m) {
           const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
           m &= m - 1;
           atomicAdd(&s_conn[q], (unsigned long long)we);
         }
+      }
       e0 = e1;
       e1 = e2;
       r0 = r1;
@@ -1313,6 +1324,11 @@ __global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const ui
     }
     const int64_t b = wave_sum(sl) - wave_sum(wl);
     if (l == 0) s_base[wv] = b;
+    if (small_k)  // uniform: k is a launch constant
+      for (uint32_t q = 0; q < a.k; ++q) {
+        const int64_t cq = wave_sum(creg[q]);
+        if (l == 0 && cq) atomicAdd(&s_conn[q], (unsigned long long)cq);
+      }
     __syncthreads();
     if (wv == 0) {
       int64_t bs = 0;
