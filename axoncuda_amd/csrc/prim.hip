@@ -177,9 +177,11 @@ __global__ void k_scan_partials(TO* partial, uint32_t m) {
   if (threadIdx.x == 0) partial[m] = carry_s;
 }
 
+// Also stores the total at out[n] (block 0): the partials' scan left it at partial[gridDim.x].
 template <typename TI, typename TO>
 __global__ void k_tile_scan_excl(const TI* in, TO* out, uint64_t n, const TO* __restrict__ partial) {
   __shared__ TO sw[SCAN_BLOCK / 64];
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[gridDim.x];
   uint64_t base = (uint64_t)blockIdx.x * RS3_TILE + (uint64_t)threadIdx.x * RS3_ITEMS;
   TO v[RS3_ITEMS];
   TO s = 0;
@@ -221,10 +223,6 @@ __global__ void k_tile_scan_incl(const TI* in, TO* out, uint64_t n, const TO* __
   }
 }
 
-template <typename TO>
-__global__ void k_store_total(TO* out, uint64_t n, const TO* partial, uint32_t m) {
-  out[n] = partial[m];
-}
 
 // Status arrays of the context's scans (one stream: scans never overlap).
 void scan_status(Ctx& c, uint32_t tiles) {
@@ -279,7 +277,6 @@ void scan_impl(Ctx& c, const TI* in, TO* out, uint64_t n, bool exclusive) {
     HGP_LAUNCH((k_scan_partials<TO>), 1, 1024, 0, c, partial.get(), t3);
     if (exclusive) {
       HGP_LAUNCH((k_tile_scan_excl<TI, TO>), t3, SCAN_BLOCK, 0, c, in, out, n, partial.get());
-      HGP_LAUNCH((k_store_total<TO>), 1, 1, 0, c, out, n, partial.get(), t3);
     } else {
       HGP_LAUNCH((k_tile_scan_incl<TI, TO>), t3, SCAN_BLOCK, 0, c, in, out, n, partial.get());
     }
