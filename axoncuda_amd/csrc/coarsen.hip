@@ -193,11 +193,14 @@ __global__ void __launch_bounds__(64 * NW) k_cand_light(const uint32_t* __restri
   }
 }
 
-// Sorted (descending) per-thread list of the 16 best candidate keys seen so far.
-__device__ __forceinline__ void top_insert(uint64_t (&top)[16], uint64_t k) {
-  if (k <= top[15]) return;
+// Sorted (descending) per-thread list of the TOPN best candidate keys seen so far.  A thread
+// that owns at most TOPN < 16 table slots keeps all of them in a list of that length (fewer
+// registers during the ranking, so more waves per SIMD during the traversal).
+template <int TOPN>
+__device__ __forceinline__ void top_insert(uint64_t (&top)[TOPN], uint64_t k) {
+  if (k <= top[TOPN - 1]) return;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
+  for (int i = 0; i < TOPN; ++i) {
     const bool sw = k > top[i];
     const uint64_t t = top[i];
     top[i] = sw ? k : t;
@@ -208,8 +211,8 @@ __device__ __forceinline__ void top_insert(uint64_t (&top)[16], uint64_t k) {
 // Block-wide top-cnc of the per-thread lists (keys are distinct: they embed the node id).
 // Each of the NW waves extracts its cnc best by repeated wave maxima; wave 0 then merges the
 // NW x 16 survivors (NW / 4 per lane).
-template <int NW>
-__device__ __forceinline__ void block_top_write(uint64_t (&top)[16], uint32_t cnc, uint64_t* s_w, uint32_t u,
+template <int NW, int TOPN>
+__device__ __forceinline__ void block_top_write(uint64_t (&top)[TOPN], uint32_t cnc, uint64_t* s_w, uint32_t u,
                                                 uint32_t* __restrict__ pairs, uint32_t* __restrict__ scores) {
   static_assert(NW % 4 == 0, "NW must be a multiple of 4");
   const uint32_t wv = threadIdx.x / 64, l = lane_id();
@@ -219,8 +222,8 @@ __device__ __forceinline__ void block_top_write(uint64_t (&top)[16], uint32_t cn
       m = wave_max(top[0]);
       if (m != 0 && top[0] == m) {
 #pragma unroll
-        for (int j = 0; j < 15; ++j) top[j] = top[j + 1];
-        top[15] = 0;
+        for (int j = 0; j < TOPN - 1; ++j) top[j] = top[j + 1];
+        top[TOPN - 1] = 0;
       }
     }
     if (l == 0) s_w[wv * 16 + i] = m;
@@ -327,7 +330,10 @@ __global__ void k_heavy_max_cap(const uint32_t* __restrict__ heavy, const uint32
 // OPT (optimistic): the node's distinct neighbours may exceed the table.  New keys are
 // counted; past `limit` the workgroup stops and the caller hands the node to a larger table.  At most one insert per thread is in flight when
 // the limit trips, so the table never fills.  Returns false when the node overflowed.
-template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true, int CH = 1, bool PIPE = true>
+// TOPN: length of the per-thread candidate lists, 16 unless every thread owns at most TOPN
+// table slots (then its list holds all of them).
+template <bool GMEM, bool INTER, int NW, bool OPT = false, bool P2 = true, int CH = 1, bool PIPE = true,
+          int TOPN = 16>
 __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool direct, uint32_t* keys, uint32_t* score,
                                                 uint32_t* inter, const uint32_t* __restrict__ hedges,
                                                 const uint64_t* __restrict__ hoff, const uint32_t* __restrict__ srcc,
@@ -347,9 +353,9 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
   auto overflowed = [&] {
     return OPT && __hip_atomic_load(&s_state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
   };
-  uint64_t top[16];
+  uint64_t top[TOPN];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) top[i] = 0;
+  for (int i = 0; i < TOPN; ++i) top[i] = 0;
   // CH > 1: the neighbours are split by id modulo CH and scored one class per pass, so the
   // table holds a CH-th of them; the per-thread top lists carry over between passes.
   for (int chunk = 0; chunk < CH; ++chunk) {
@@ -408,7 +414,7 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
     // the pin ids visited; per touching hedge its list entry, offsets, weight (and source count)
     if (CH > 1) __syncthreads();  // the next pass clears the table
   }
-  block_top_write<NW>(top, cnc, s_w, u, pairs, scores);
+  block_top_write<NW, TOPN>(top, cnc, s_w, u, pairs, scores);
   return true;
 }
 
@@ -427,9 +433,13 @@ __global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS) {
   __shared__ uint32_t s_score[CAP];
   __shared__ uint32_t s_inter[INTER ? CAP : 1];
   __shared__ uint64_t s_w[64];
+  // a thread owns at most CAP / 256 slots of the node's table (its capacity is <= CAP)
+  constexpr int TOPN = CAP / 256 < 16 ? (int)(CAP / 256) : 16;
+  static_assert(CAP % 256 == 0, "table slots must split evenly over the workgroup");
   const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
-  cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
-                                   sizes, cnc, omega, delta, pairs, scores, s_w);
+  cand_block_node<false, INTER, 4, false, true, 1, true, TOPN>(u, caps[u], false, s_keys, s_score, s_inter, hedges,
+                                                               hoff, srcc, tch, toff, inb, w, sizes, cnc, omega, delta,
+                                                               pairs, scores, s_w);
 }
 
 // One node per 1024-thread workgroup with a hashed LDS table of up to CAND_BIG_SLOTS.
@@ -484,7 +494,8 @@ __global__ void __launch_bounds__(1024) k_cand_direct(CAND_ARGS) {
   __shared__ uint32_t s_inter[INTER ? N : 1];
   __shared__ uint64_t s_w[256];
   const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
-  // without the pipelined traversal: its extra live registers spill in this kernel
+  // without the pipelined traversal: with it this kernel measured no faster (its occupancy is
+  // set by the LDS score array)
   cand_block_node<false, INTER, 16, false, true, 1, false>(u, caps[u], true, s_bits, s_score, s_inter, hedges, hoff,
                                                            srcc, tch, toff, inb, w, sizes, cnc, omega, delta, pairs,
                                                            scores, s_w);

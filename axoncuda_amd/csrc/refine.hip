@@ -1850,15 +1850,34 @@ struct SmallHedges {
 
 // Hedges of 2..CAS_SMALL pins, one thread each, each XCD given a contiguous range of the
 // first-pin order so that the records its threads gather meet in one L2.
+// P16 (packed records, k < PM2_MAX_K): a pin's partition and target stay packed in one
+// register (partition | target << 16, 0xFFFF = none), so the pins take 32 registers instead
+// of 48 and more waves fit on a SIMD.
+template <bool P16>
 __global__ void k_cascade_small(CascadeArgs a, SmallHedges h) {
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= h.m) return;
   const uint64_t s = h.off[i], t = h.off[i + 1];
-  uint32_t pt[CAS_SMALL], mv[CAS_SMALL], rk[CAS_SMALL];
+  // per pin: partition (with P16 also the target), target, rank
+  uint32_t pt[CAS_SMALL], mv[P16 ? 1 : CAS_SMALL], rk[CAS_SMALL];
+  // a pin's partition, its target (NONE when it does not move) and whether it moves; entries
+  // past the hedge's end are non-moving pins of no partition (0xFFFF or NONE: no real id)
+  auto part_of = [&](uint32_t u) {
+    if constexpr (P16) return pt[u] & 0xFFFFu;
+    else return pt[u];
+  };
+  auto move_of = [&](uint32_t u) {
+    if constexpr (P16) return pt[u] >> 16;
+    else return mv[u];
+  };
+  auto moves = [&](uint32_t u) {
+    if constexpr (P16) return (pt[u] >> 16) != 0xFFFFu;
+    else return mv[u] != NONE;
+  };
 #pragma unroll
   for (uint32_t u = 0; u < CAS_SMALL; ++u) {  // entries no batch fills (u >= 4 when len <= 4)
-    pt[u] = NONE;
-    mv[u] = NONE;
+    pt[u] = P16 ? 0xFFFFFFFFu : NONE;
+    if constexpr (!P16) mv[u] = NONE;
     rk[u] = 0;
   }
   const uint32_t len = (uint32_t)(t - s);
@@ -1869,29 +1888,36 @@ __global__ void k_cascade_small(CascadeArgs a, SmallHedges h) {
     const uint32_t x = h.pins[s + min(u, len - 1)];
     return x;
   };
-  auto put = [&](uint32_t u, uint3 q4) {
+  auto put = [&](uint32_t u, uint32_t x) {
     const bool in = u < len;
-    pt[u] = in ? q4.x : NONE;
-    mv[u] = in ? q4.y : NONE;
-    rk[u] = q4.z;
+    if constexpr (P16) {
+      const uint2 r = a.pm2[x];
+      pt[u] = in ? r.x : 0xFFFFFFFFu;
+      rk[u] = r.y;
+    } else {
+      const uint3 q4 = pm_at(a, x);
+      pt[u] = in ? q4.x : NONE;
+      mv[u] = in ? q4.y : NONE;
+      rk[u] = q4.z;
+    }
   };
   {
     uint32_t x[4];
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) x[u] = fetch(u);
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) put(u, pm_at(a, x[u]));
+    for (uint32_t u = 0; u < 4; ++u) put(u, x[u]);
   }
   if (len > 4) {
     uint32_t x[CAS_SMALL - 4];
 #pragma unroll
     for (uint32_t u = 4; u < CAS_SMALL; ++u) x[u - 4] = fetch(u);
 #pragma unroll
-    for (uint32_t u = 4; u < CAS_SMALL; ++u) put(u, pm_at(a, x[u - 4]));
+    for (uint32_t u = 4; u < CAS_SMALL; ++u) put(u, x[u - 4]);
   }
   bool any = false;
 #pragma unroll
-  for (uint32_t i = 0; i < CAS_SMALL; ++i) any |= mv[i] != NONE;
+  for (uint32_t i = 0; i < CAS_SMALL; ++i) any |= moves(i);
   if (!any) return;
   const int64_t W = (int64_t)fixed_weight(h.w[i]);
   // Static indexes only (the arrays stay in registers); entries past a hedge's end take no
@@ -1899,14 +1925,14 @@ __global__ void k_cascade_small(CascadeArgs a, SmallHedges h) {
   const uint32_t jmax = __ballot(len > 12) ? 16u : __ballot(len > 8) ? 12u : __ballot(len > 4) ? 8u : 4u;
 #pragma unroll
   for (uint32_t i = 0; i < CAS_SMALL; ++i) {
-    if (i < jmax && mv[i] != NONE) {
-      const uint32_t p = pt[i], q = mv[i], r = rk[i];
+    if (i < jmax && moves(i)) {
+      const uint32_t p = part_of(i), q = move_of(i), r = rk[i];
       int32_t cp = 0, cq = 0;
 #pragma unroll
       for (uint32_t j = 0; j < CAS_SMALL; ++j) {
         if (j < jmax) {
-          const bool before = mv[j] != NONE && rk[j] < r;  // moved before pin i
-          const uint32_t now = before ? mv[j] : pt[j];     // its partition when pin i moves
+          const bool before = moves(j) && rk[j] < r;         // moved before pin i
+          const uint32_t now = before ? move_of(j) : part_of(j);  // its partition when pin i moves
           cp += now == p;
           cq += now == q;
         }
@@ -2828,7 +2854,8 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
       if (!Y.cas_small_n) return;
       const SmallHedges sh{Y.cs_off.get(), Y.cs_pins.get(), Y.cs_w.get(), Y.cas_small_n};
       const uint32_t cb = c.tune.cascade_block;  // 256 unless the autotuner chose otherwise
-      HGP_LAUNCH(k_cascade_small, ceil_div(Y.cas_small_n, cb), cb, 0, c, a, sh);
+      if (a.pm2) HGP_LAUNCH(k_cascade_small<true>, ceil_div(Y.cas_small_n, cb), cb, 0, c, a, sh);
+      else HGP_LAUNCH(k_cascade_small<false>, ceil_div(Y.cas_small_n, cb), cb, 0, c, a, sh);
       if (prof_match(c, "k_cascade_small") && c.prof_count && c.prof_dev)
         HGP_LAUNCH(k_cascade_small_bytes, ceil_div(Y.cas_small_n, 256), 256, 0, c, a, sh, c.prof_dev);
     };
@@ -3043,7 +3070,7 @@ float time_cascade_small(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, c
   HGP_HIP(hipEventCreate(&t0));
   HGP_HIP(hipEventCreate(&t1));
   HGP_HIP(hipEventRecord(t0, c.stream));
-  HGP_LAUNCH(k_cascade_small, ceil_div(m, block), block, 0, c, a, sh);
+  HGP_LAUNCH(k_cascade_small<true>, ceil_div(m, block), block, 0, c, a, sh);
   HGP_HIP(hipEventRecord(t1, c.stream));
   HGP_HIP(hipEventSynchronize(t1));
   float ms = 0;
