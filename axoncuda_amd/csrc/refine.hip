@@ -1254,7 +1254,9 @@ __global__ void __launch_bounds__(256) k_gains_mask(GainArgs a, const uint4* __r
 // of a coarse node lie inside its partition and add nothing), instead of broadcasting every
 // record to all lanes.  The chunk after the current one is loaded while the current one is
 // added.  The sums are integers, so their order does not matter.
-template <uint32_t W>
+// SMALLK: k <= MASK_SMALL_K (connections summed in registers); the other instance holds no
+// such registers, which leaves room for more waves per SIMD.
+template <uint32_t W, bool SMALLK>
 __global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const uint4* __restrict__ hm,
                                                             const float* __restrict__ w,
                                                             const uint32_t* __restrict__ list,
@@ -1279,10 +1281,11 @@ __global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const ui
     int64_t wl = 0, sl = 0;
     // with at most MASK_SMALL_K partitions every lane sums its hedges' connections in registers
     // (LDS atomics would all meet on the same few slots) and the wave sums them at the end
-    const bool small_k = a.k <= MASK_SMALL_K;
-    int64_t creg[MASK_SMALL_K];
+    constexpr bool small_k = SMALLK;
+    constexpr uint32_t NREG = SMALLK ? MASK_SMALL_K : 1;
+    int64_t creg[NREG];
 #pragma unroll
-    for (uint32_t q = 0; q < MASK_SMALL_K; ++q) creg[q] = 0;
+    for (uint32_t q = 0; q < NREG; ++q) creg[q] = 0;
     // lane l walks entries j, j + stride, ...: the list entry two chunks ahead and the record
     // one chunk ahead are in flight while the current record is added
     uint64_t j = t0 + 64 * wv + l;
@@ -1306,9 +1309,9 @@ __global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const ui
       wl += we;
       if ((sg >> p) & 1ull) sl += we;
       uint64_t m = (((uint64_t)r0.y << 32) | r0.x) & notp;
-      if (small_k) {
+      if constexpr (small_k) {
 #pragma unroll
-        for (uint32_t q = 0; q < MASK_SMALL_K; ++q) creg[q] += ((m >> q) & 1ull) ? (int64_t)we : 0;
+        for (uint32_t q = 0; q < NREG; ++q) creg[q] += ((m >> q) & 1ull) ? (int64_t)we : 0;
       } else if (we) {
         while (m) {
           const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
@@ -1324,8 +1327,8 @@ __global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const ui
     }
     const int64_t b = wave_sum(sl) - wave_sum(wl);
     if (l == 0) s_base[wv] = b;
-    if (small_k)  // uniform: k is a launch constant
-      for (uint32_t q = 0; q < a.k; ++q) {
+    if constexpr (small_k)
+      for (uint32_t q = 0; q < a.k; ++q) {  // a.k <= NREG
         const int64_t cq = wave_sum(creg[q]);
         if (l == 0 && cq) atomicAdd(&s_conn[q], (unsigned long long)cq);
       }
@@ -2615,12 +2618,16 @@ void compute_gains(Ctx& c, const RefineLevel& L, const PartitionLists& pl, int m
     const uint64_t mean = L.n ? pl.P / L.n : 0;
     const uint32_t W = mean >= c.tune.mask_wide16_degree ? 16 : (mean >= c.tune.mask_wide4_degree ? 4 : 1);
     // every XCD needs a workgroup for its sub-list: grids of at least XCDS
-    if (W == 16)
-      HGP_LAUNCH(k_gains_mask_wide<16>, std::max<uint32_t>(XCDS, std::min<uint32_t>(L.n, LIGHT_GRID)), 1024, 0, c, a,
-                 pl.hm.get(), L.w, ml.get(), mc.get(), rg);
+    const uint32_t wg = std::max<uint32_t>(XCDS, std::min<uint32_t>(L.n, LIGHT_GRID));
+    const bool sk = L.k <= MASK_SMALL_K;
+    if (W == 16 && sk)
+      HGP_LAUNCH((k_gains_mask_wide<16, true>), wg, 1024, 0, c, a, pl.hm.get(), L.w, ml.get(), mc.get(), rg);
+    else if (W == 16)
+      HGP_LAUNCH((k_gains_mask_wide<16, false>), wg, 1024, 0, c, a, pl.hm.get(), L.w, ml.get(), mc.get(), rg);
+    else if (W == 4 && sk)
+      HGP_LAUNCH((k_gains_mask_wide<4, true>), wg, 256, 0, c, a, pl.hm.get(), L.w, ml.get(), mc.get(), rg);
     else if (W == 4)
-      HGP_LAUNCH(k_gains_mask_wide<4>, std::max<uint32_t>(XCDS, std::min<uint32_t>(L.n, LIGHT_GRID)), 256, 0, c, a,
-                 pl.hm.get(), L.w, ml.get(), mc.get(), rg);
+      HGP_LAUNCH((k_gains_mask_wide<4, false>), wg, 256, 0, c, a, pl.hm.get(), L.w, ml.get(), mc.get(), rg);
     else
       HGP_LAUNCH(k_gains_mask, std::max<uint32_t>(XCDS, std::min<uint32_t>(ceil_div(L.n, 4), LIGHT_GRID)), 256, 0, c,
                  a, pl.hm.get(), L.w, ml.get(), mc.get(), rg);
