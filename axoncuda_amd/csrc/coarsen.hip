@@ -251,6 +251,127 @@ __device__ __forceinline__ void block_top_write(uint64_t (&top)[TOPN], uint32_t 
   __syncthreads();
 }
 
+// Exact top-cnc (cnc <= 16) of the candidate keys of a node's LDS table, written to the
+// node's pairs/scores in descending key order (NONE / 0 past the candidates).  Slot s holds
+// its key's high word in hi[s] (0: no candidate) and its low word (the neighbour id) in
+// keys[s], or s itself when the table is indexed by node id.  Keys are distinct (they embed
+// the neighbour id).  A radix select: each pass histograms the next 8 bits below those fixed
+// so far, over the keys that share the fixed bits, and fixes the bin that holds the cnc-th
+// largest key; it stops once every key of that bin is taken.  The first pass starts below
+// the largest key's top bit.  The passes sweep the table in parallel, where selecting by
+// repeated wave maxima would be a chain of dependent reductions.
+struct SelectScratch {
+  uint32_t hist[256];
+  uint64_t sel[16];    // the selected keys; first the per-wave maxima
+  uint64_t prefix;     // bits of the cnc-th key fixed so far (above bit `upper`)
+  uint32_t need;       // keys still to take among those sharing the prefix
+  uint32_t done;
+  uint32_t nsel;
+  uint32_t pad;
+};
+constexpr uint32_t SELECT_WORDS = (sizeof(SelectScratch) + 7) / 8;  // 8-byte words of the scratch
+static_assert(SELECT_WORDS <= 256, "the 1024-thread kernels' 256-word scratch holds a SelectScratch");
+__device__ __forceinline__ uint64_t shr64(uint64_t k, uint32_t b) { return b >= 64 ? 0ull : k >> b; }
+template <int NW>
+__device__ __forceinline__ void block_select_write(uint32_t cap, bool direct, const uint32_t* keys, const uint32_t* hi,
+                                                   uint64_t kmax, uint32_t cnc, uint32_t u,
+                                                   uint32_t* __restrict__ pairs, uint32_t* __restrict__ scores,
+                                                   SelectScratch* ss) {
+  static_assert(NW <= 16 && 64 * NW >= 256, "one histogram bin per thread at most, at most 16 wave maxima");
+  constexpr uint32_t B = 64 * NW;
+  const uint32_t tid = threadIdx.x, l = lane_id(), wv = tid / 64;
+  auto key_at = [&](uint32_t s) -> uint64_t {
+    const uint32_t h = hi[s];
+    return h ? ((uint64_t)h << 32) | (direct ? s : keys[s]) : 0ull;
+  };
+  kmax = wave_max(kmax);
+  if (l == 0) ss->sel[wv] = kmax;
+  if (tid == 0) {
+    ss->prefix = 0;
+    ss->need = cnc;
+    ss->done = 0;
+    ss->nsel = 0;
+  }
+  __syncthreads();
+  uint64_t bmax = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) bmax = ss->sel[w] > bmax ? ss->sel[w] : bmax;
+  uint64_t thr = ~0ull;  // the selected keys are the non-zero ones >= thr
+  if (bmax != 0) {       // uniform
+    uint32_t upper = 64 - (uint32_t)__clzll((long long)bmax);  // every key is below 2^upper
+    for (;;) {
+      const uint32_t lower = upper > 8 ? upper - 8 : 0u;
+      __syncthreads();  // the previous pass's reads of the histogram and state are done
+      if (tid < 256) ss->hist[tid] = 0;
+      __syncthreads();
+      const uint64_t pre = ss->prefix;
+      const uint32_t dmask = (1u << (upper - lower)) - 1u;
+      for (uint32_t s = tid; s < cap; s += B) {
+        const uint64_t k = key_at(s);
+        if (k && shr64(k, upper) == pre) atomicAdd(&ss->hist[(uint32_t)(k >> lower) & dmask], 1u);
+      }
+      __syncthreads();
+      if (wv == 0) {  // lane l holds bins 4l .. 4l + 3
+        const uint32_t c0 = ss->hist[4 * l], c1 = ss->hist[4 * l + 1], c2 = ss->hist[4 * l + 2],
+                       c3 = ss->hist[4 * l + 3];
+        const uint32_t c4 = c0 + c1 + c2 + c3;
+        const uint32_t incl = wave_scan_incl(c4);
+        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t need = ss->need;
+        if (total <= need) {  // only on the first pass: every candidate is taken
+          if (l == 0) {
+            ss->prefix = 0;
+            ss->done = 1;
+          }
+        } else {
+          const uint32_t at_or_above = total - (incl - c4);  // keys in bins >= 4l
+          const uint64_t ok = __ballot(at_or_above >= need);  // lanes 0 .. L (lane 0: total > need)
+          const uint32_t L = 63u - (uint32_t)__clzll((long long)ok);
+          if (l == L) {
+            uint32_t cum = at_or_above - c4, b, cb;  // cum: keys in the bins above bin 4L + b
+            if (cum + c3 >= need) b = 3, cb = c3;
+            else if (cum += c3, cum + c2 >= need) b = 2, cb = c2;
+            else if (cum += c2, cum + c1 >= need) b = 1, cb = c1;
+            else cum += c1, b = 0, cb = c0;
+            const uint32_t nn = need - cum;
+            ss->prefix = (pre << (upper - lower)) | (4 * L + b);
+            ss->need = nn;
+            ss->done = cb == nn ? 1u : 0u;
+          }
+        }
+      }
+      __syncthreads();
+      if (ss->done || lower == 0) {  // uniform; with lower == 0 a bin is one key: done
+        thr = ss->prefix << lower;
+        break;
+      }
+      upper = lower;
+    }
+  }
+  for (uint32_t s = tid; s < cap; s += B) {
+    const uint64_t k = key_at(s);
+    if (k && k >= thr) {
+      const uint32_t i = atomicAdd(&ss->nsel, 1u);
+      if (i < 16) ss->sel[i] = k;
+    }
+  }
+  __syncthreads();
+  const uint32_t n = ss->nsel < 16 ? ss->nsel : 16u;
+  if (tid < cnc) {
+    if (tid < n) {  // rank among the selected keys
+      const uint64_t k = ss->sel[tid];
+      uint32_t r = 0;
+      for (uint32_t j = 0; j < n; ++j) r += ss->sel[j] > k ? 1u : 0u;
+      pairs[(uint64_t)u * cnc + r] = (uint32_t)k;
+      scores[(uint64_t)u * cnc + r] = (uint32_t)(k >> 32);
+    } else {
+      pairs[(uint64_t)u * cnc + tid] = NONE;
+      scores[(uint64_t)u * cnc + tid] = 0;
+    }
+  }
+  __syncthreads();  // the table and the scratch are reused by the next node
+}
+
 // Table capacity of a node whose touching hedges hold `bound` other pins: distinct
 // neighbours are at most min(bound, n - 1); a power-of-two open-addressing table of twice
 // that (plus slack) never fills.
@@ -353,6 +474,9 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
   auto overflowed = [&] {
     return OPT && __hip_atomic_load(&s_state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
   };
+  // LDS tables scored in one pass are ranked by a radix select (s_w is a SelectScratch);
+  // tables in HBM, or scored in several passes, by per-thread candidate lists
+  constexpr bool SELECT = !GMEM && CH == 1;
   uint64_t top[TOPN];
 #pragma unroll
   for (int i = 0; i < TOPN; ++i) top[i] = 0;
@@ -400,6 +524,25 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
     if (GMEM) __threadfence();
     __syncthreads();
     if (OPT && s_state[1]) return false;
+    if constexpr (SELECT) {
+      // each slot's key, its high word written over the score (0: no candidate)
+      uint64_t kmax = 0;
+      for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
+        uint32_t v;
+        if (direct) {
+          if (!((keys[s >> 5] >> (s & 31u)) & 1u)) continue;
+          v = s;
+        } else {
+          v = keys[s];
+          if (v == NONE) continue;
+        }
+        const uint64_t k = cand_key(u, v, score[s], INTER ? inter[s] : 0u, size_u, inb_u, sizes, inb, omega, delta);
+        score[s] = (uint32_t)(k >> 32);
+        kmax = k > kmax ? k : kmax;
+      }
+      block_select_write<NW>(cap, direct, keys, score, kmax, cnc, u, pairs, scores, (SelectScratch*)s_w);
+      return true;
+    }
     for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
       uint32_t v;
       if (direct) {
@@ -432,14 +575,10 @@ __global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS) {
   __shared__ uint32_t s_keys[CAP];
   __shared__ uint32_t s_score[CAP];
   __shared__ uint32_t s_inter[INTER ? CAP : 1];
-  __shared__ uint64_t s_w[64];
-  // a thread owns at most CAP / 256 slots of the node's table (its capacity is <= CAP)
-  constexpr int TOPN = CAP / 256 < 16 ? (int)(CAP / 256) : 16;
-  static_assert(CAP % 256 == 0, "table slots must split evenly over the workgroup");
+  __shared__ uint64_t s_w[SELECT_WORDS];
   const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
-  cand_block_node<false, INTER, 4, false, true, 1, true, TOPN>(u, caps[u], false, s_keys, s_score, s_inter, hedges,
-                                                               hoff, srcc, tch, toff, inb, w, sizes, cnc, omega, delta,
-                                                               pairs, scores, s_w);
+  cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
+                                   sizes, cnc, omega, delta, pairs, scores, s_w);
 }
 
 // One node per 1024-thread workgroup with a hashed LDS table of up to CAND_BIG_SLOTS.

@@ -30,10 +30,22 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, uint32_t src) {
 // Owner lane of flat position f0 + lane among the 64 hedges of a chunk whose exclusive start
 // offsets are `excl` (each hedge has at least one entry, so starts are distinct; lanes past the
 // list carry length 0 and start at the chunk total, beyond every active position): the number
-// of starts before the window (a ballot) plus those inside it at or before this position (an
-// OR-reduced bitmap of the window's starts), minus one.
+// of starts before the window (a ballot) plus those inside it at or before this position,
+// minus one.  The starts inside the window belong to lanes [before, upto), a run because the
+// starts ascend with the lane.  A window of long hedges holds few of them, and each lane
+// compares its position with each one (read as a scalar); a window of many short hedges
+// counts them from an OR-reduced bitmap of the window's starts.  (Positions past the total
+// may get another owner lane than the bitmap would give; they are inactive.)
+constexpr uint32_t OWNER_FEW_STARTS = 12;
 __device__ __forceinline__ uint32_t chunk_owner(uint32_t excl, uint32_t f0, uint64_t le_l) {
   const uint32_t before = (uint32_t)__popcll(__ballot(excl < f0));
+  const uint32_t upto = (uint32_t)__popcll(__ballot(excl < f0 + 64u));
+  if (upto - before <= OWNER_FEW_STARTS) {  // uniform over the wave
+    const uint32_t fi = f0 + lane_id();
+    uint32_t k = before - 1u;  // before >= 1 unless f0 == 0, where lane 0 starts at 0 <= fi
+    for (uint32_t x = before; x < upto; ++x) k += (uint32_t)__builtin_amdgcn_readlane((int)excl, (int)x) <= fi ? 1u : 0u;
+    return k;
+  }
   const uint64_t mine = (excl >= f0 && excl - f0 < 64u) ? (1ull << (excl - f0)) : 0ull;
   const uint64_t heads = wave_reduce(mine, [](uint64_t a, uint64_t b) { return a | b; });
   return before + (uint32_t)__popcll(heads & le_l) - 1u;
