@@ -108,6 +108,9 @@ __device__ __forceinline__ uint64_t cand_key(uint32_t u, uint32_t v, uint32_t sc
 }
 
 
+// k >> b, 0 for b >= 64
+__device__ __forceinline__ uint64_t shr64(uint64_t k, uint32_t b) { return b >= 64 ? 0ull : k >> b; }
+
 // One wave per node of visit bound b <= CAND_LIGHT (CAP = CAND_CAP, 4 waves per workgroup),
 // b <= CAND_LIGHT_M (CAP = 512, 8 waves) or b <= CAND_LIGHT_S (CAP = 128, 8 waves): smaller
 // tables leave room for more waves per CU.  The
@@ -171,24 +174,105 @@ __global__ void __launch_bounds__(64 * NW) k_cand_light(const uint32_t* __restri
     cand[s] = v == NONE ? 0ull
                         : cand_key(u, v, score[slot], INTER ? inter[slot] : 0u, size_u, inb_u, sizes, inb, omega, delta);
   }
-  for (uint32_t i = 0; i < cnc; ++i) {
-    uint64_t m = 0;
+  // The top cnc keys by a radix select (as block_select_write, for one wave): the keys are in
+  // registers now, so the table's score array holds the histogram (CAP >= 128 bins) and its
+  // key array the selected keys.
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  constexpr uint32_t DB = CAP >= 256 ? 8 : 7, BINS = 1u << DB, BPL = BINS / 64;  // digit bits, bins per lane
+  static_assert(CAP >= BINS, "the histogram fits the score array");
+  uint32_t* hist = score;
+  uint64_t m = 0;
 #pragma unroll
-    for (uint32_t s = 0; s < SLOTS; ++s) m = cand[s] > m ? cand[s] : m;
-    const uint64_t best = wave_max(m);
-    if (best == 0) {  // no candidates left: the remaining slots are empty
-      for (uint32_t r = i + l; r < cnc; r += 64) {
-        pairs[(uint64_t)u * cnc + r] = NONE;
-        scores[(uint64_t)u * cnc + r] = 0;
+  for (uint32_t s = 0; s < SLOTS; ++s) m = cand[s] > m ? cand[s] : m;
+  const uint64_t kmax = wave_max(m);
+  uint64_t thr = ~0ull;  // the selected keys are the non-zero ones >= thr
+  if (kmax != 0) {
+    uint32_t upper = 64 - (uint32_t)__clzll((long long)kmax), need = cnc;
+    uint64_t prefix = 0;
+    for (;;) {
+      const uint32_t lower = upper > DB ? upper - DB : 0u;
+      const uint32_t dmask = (1u << (upper - lower)) - 1u;
+#pragma unroll
+      for (uint32_t i = 0; i < BPL; ++i) hist[l + 64 * i] = 0;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+      for (uint32_t s = 0; s < SLOTS; ++s) {
+        const uint64_t k = cand[s];
+        if (k && shr64(k, upper) == prefix) atomicAdd(&hist[(uint32_t)(k >> lower) & dmask], 1u);
       }
-      break;
-    }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      uint32_t c[BPL], cl = 0;  // lane l: bins BPL * l ..
 #pragma unroll
-    for (uint32_t s = 0; s < SLOTS; ++s)
-      if (cand[s] == best) cand[s] = 0;
-    if (l == 0) {
-      pairs[(uint64_t)u * cnc + i] = (uint32_t)best;
-      scores[(uint64_t)u * cnc + i] = (uint32_t)(best >> 32);
+      for (uint32_t i = 0; i < BPL; ++i) {
+        c[i] = hist[BPL * l + i];
+        cl += c[i];
+      }
+      const uint32_t incl = wave_scan_incl(cl);
+      const uint32_t total = __shfl(incl, 63, 64);
+      if (total <= need) {  // only on the first pass: every candidate is taken
+        thr = 0;
+        break;
+      }
+      const uint32_t at_or_above = total - (incl - cl);
+      const uint64_t ok = __ballot(at_or_above >= need);  // lanes 0 .. L
+      const uint32_t L = 63u - (uint32_t)__clzll((long long)ok);
+      uint32_t cum = at_or_above - cl, b = 0, cb = c[0];  // lane L's bins, from the top
+#pragma unroll
+      for (int i = (int)BPL - 1; i >= 0; --i) {
+        if (cum + c[i] >= need) {
+          b = (uint32_t)i;
+          cb = c[i];
+          break;
+        }
+        cum += c[i];
+      }
+      const uint32_t bL = (uint32_t)__builtin_amdgcn_readlane((int)b, (int)L);
+      const uint32_t cumL = (uint32_t)__builtin_amdgcn_readlane((int)cum, (int)L);
+      const uint32_t cbL = (uint32_t)__builtin_amdgcn_readlane((int)cb, (int)L);
+      prefix = (prefix << (upper - lower)) | (BPL * L + bL);
+      need -= cumL;
+      if (cbL == need || lower == 0) {  // every key of the bin is taken (a bin is one key at bit 0)
+        thr = prefix << lower;
+        break;
+      }
+      upper = lower;
+      __builtin_amdgcn_wave_barrier();  // the bins are read before the next pass clears them
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+  }
+  // the selected keys (at most cnc) into the key array as (lo, hi) pairs, then ranked
+  uint32_t mine = 0;
+#pragma unroll
+  for (uint32_t s = 0; s < SLOTS; ++s) mine += (cand[s] && cand[s] >= thr) ? 1u : 0u;
+  const uint32_t pincl = wave_scan_incl(mine);
+  const uint32_t n = min(__shfl(pincl, 63, 64), cnc);
+  uint32_t at = pincl - mine;
+#pragma unroll
+  for (uint32_t s = 0; s < SLOTS; ++s) {
+    const uint64_t k = cand[s];
+    if (k && k >= thr) {
+      if (at < 16) {
+        keys[2 * at] = (uint32_t)k;
+        keys[2 * at + 1] = (uint32_t)(k >> 32);
+      }
+      ++at;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (l < cnc) {
+    if (l < n) {
+      const uint64_t k = ((uint64_t)keys[2 * l + 1] << 32) | keys[2 * l];
+      uint32_t r = 0;
+      for (uint32_t j = 0; j < n; ++j) r += (((uint64_t)keys[2 * j + 1] << 32) | keys[2 * j]) > k ? 1u : 0u;
+      pairs[(uint64_t)u * cnc + r] = (uint32_t)k;
+      scores[(uint64_t)u * cnc + r] = (uint32_t)(k >> 32);
+    } else {
+      pairs[(uint64_t)u * cnc + l] = NONE;
+      scores[(uint64_t)u * cnc + l] = 0;
     }
   }
 }
@@ -271,7 +355,6 @@ struct SelectScratch {
 };
 constexpr uint32_t SELECT_WORDS = (sizeof(SelectScratch) + 7) / 8;  // 8-byte words of the scratch
 static_assert(SELECT_WORDS <= 256, "the 1024-thread kernels' 256-word scratch holds a SelectScratch");
-__device__ __forceinline__ uint64_t shr64(uint64_t k, uint32_t b) { return b >= 64 ? 0ull : k >> b; }
 template <int NW>
 __device__ __forceinline__ void block_select_write(uint32_t cap, bool direct, const uint32_t* keys, const uint32_t* hi,
                                                    uint64_t kmax, uint32_t cnc, uint32_t u,
