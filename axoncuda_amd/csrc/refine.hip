@@ -1109,10 +1109,17 @@ __global__ void __launch_bounds__(256) k_gains_route_mask(GainArgs a, const uint
     };
     if (d > 0) batch(std::integral_constant<uint32_t, 4>{}, 0);
     if (d > 4) batch(std::integral_constant<uint32_t, GAIN_TINY - 4>{}, 4);
+    // entries past a node's degree are zero: the sums stop at the largest degree among the wave's
+    // tiny nodes, rounded up to 4 (uniform over the active lanes)
+    const uint32_t dmax = __ballot(d > 12) ? 16u : __ballot(d > 8) ? 12u : __ballot(d > 4) ? 8u : 4u;
     auto conn_of = [&](uint32_t q) {
       int64_t c = 0;
 #pragma unroll
-      for (uint32_t u = 0; u < GAIN_TINY; ++u) c += ((mk[u] >> q) & 1ull) ? (int64_t)wt[u] : 0;
+      for (uint32_t g = 0; g < GAIN_TINY / 4; ++g) {
+        if (4 * g >= dmax) break;
+#pragma unroll
+        for (uint32_t u = 4 * g; u < 4 * g + 4; ++u) c += ((mk[u] >> q) & 1ull) ? (int64_t)wt[u] : 0;
+      }
       return c;
     };
     const int64_t base = saving - Wv;
@@ -1190,11 +1197,16 @@ __device__ __forceinline__ void mask_pick(const GainArgs& a, uint32_t v, uint32_
   }
 }
 // Touching hedges jb0, jb0 + stride, ... (64 at a time) of a node in partition p: lane q's
-// connection, and the lane's shares of the node's hedge weight and of its saving.
+// connection, and the lane's shares of the node's hedge weight and of its saving.  Each chunk's
+// masks and weights are staged in the wave's LDS rows (s_lo, s_hi, s_wt: 64 entries each) and
+// read back as broadcasts: lane q reads only the mask word that holds bit q.  While the chunk's
+// weights are below 2^26 their sum fits 32 bits, and the lane adds in 32 bits.
 __device__ __forceinline__ void mask_accumulate(const GainArgs& a, const uint4* __restrict__ hm,
                                                 const float* __restrict__ w, uint64_t jb0, uint64_t t1,
                                                 uint64_t stride, uint32_t p, uint32_t l, int64_t& conn, int64_t& wl,
-                                                int64_t& sl) {
+                                                int64_t& sl, uint32_t* s_lo, uint32_t* s_hi, uint32_t* s_wt) {
+  const uint32_t* col = l < 32 ? s_lo : s_hi;  // the mask word with this lane's partition bit
+  const uint32_t sh = l & 31u;
   for (uint64_t jb = jb0; jb < t1; jb += stride) {
     const uint64_t j = jb + l;
     uint32_t mlo = 0, mhi = 0, we = 0;
@@ -1208,13 +1220,21 @@ __device__ __forceinline__ void mask_accumulate(const GainArgs& a, const uint4* 
       wl += we;
       if ((sg >> p) & 1ull) sl += we;
     }
+    s_lo[l] = mlo;
+    s_hi[l] = mhi;
+    s_wt[l] = we;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const uint32_t cnt = (uint32_t)min<uint64_t>(64, t1 - jb);
-    for (uint32_t i = 0; i < cnt; ++i) {
-      const uint32_t lo = __builtin_amdgcn_readlane(mlo, i), hi = __builtin_amdgcn_readlane(mhi, i);
-      const uint32_t wi = __builtin_amdgcn_readlane(we, i);
-      const uint32_t word = l < 32 ? lo : hi;
-      conn += ((word >> (l & 31)) & 1u) ? (int64_t)wi : 0;
+    if (__ballot(we >= (1u << 26)) == 0) {  // uniform
+      uint32_t c32 = 0;
+      for (uint32_t i = 0; i < cnt; ++i) c32 += s_wt[i] & (0u - ((col[i] >> sh) & 1u));
+      conn += c32;
+    } else {
+      for (uint32_t i = 0; i < cnt; ++i) conn += ((col[i] >> sh) & 1u) ? (int64_t)s_wt[i] : 0;
     }
+    __builtin_amdgcn_wave_barrier();  // the rows are read before the next chunk is staged
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -1226,6 +1246,7 @@ __device__ __forceinline__ void mask_accumulate(const GainArgs& a, const uint4* 
 __global__ void __launch_bounds__(256) k_gains_mask(GainArgs a, const uint4* __restrict__ hm,
                                                     const float* __restrict__ w, const uint32_t* __restrict__ list,
                                                     const uint32_t* __restrict__ count, uint32_t route_grid) {
+  __shared__ uint32_t s_rec[4][3][64];  // per wave: a chunk's mask words and weights
   const uint32_t l = lane_id(), wv = threadIdx.x / 64;
   const bool qvalid = l < a.k;
   const uint32_t psz = qvalid ? a.psizes[l] : 0u;
@@ -1238,8 +1259,8 @@ __global__ void __launch_bounds__(256) k_gains_mask(GainArgs a, const uint4* __r
     const uint32_t p = a.part[v], s = a.sizes[v];
     const uint64_t t0 = a.toff[v], t1 = a.toff[v + 1];
     int64_t conn = 0, wl = 0, sl = 0;
-    mask_accumulate(a, hm, w, t0, t1, 64, p, l, conn, wl, sl);
-    mask_pick(a, v, p, s, l, qvalid, psz, hq, ord, wave_sum(sl) - wave_sum(wl), conn);
+    mask_accumulate(a, hm, w, t0, t1, 64, p, l, conn, wl, sl, s_rec[wv][0], s_rec[wv][1], s_rec[wv][2]);
+    mask_pick(a, v, p, s, l, qvalid, psz, hq, ord, wave_sum(sl - wl), conn);  // one reduction: the sums are exact
     bytes += (l == 0 ? 36 : 0) + 24 * (uint64_t)((t1 - t0 + 63 - l) / 64);
   }
   if (a.prof_bytes) {
@@ -1325,7 +1346,7 @@ __global__ void __launch_bounds__(64 * W) k_gains_mask_wide(GainArgs a, const ui
       w0 = w1;
       j += stride;
     }
-    const int64_t b = wave_sum(sl) - wave_sum(wl);
+    const int64_t b = wave_sum(sl - wl);
     if (l == 0) s_base[wv] = b;
     if constexpr (small_k)
       for (uint32_t q = 0; q < a.k; ++q) {  // a.k <= NREG
