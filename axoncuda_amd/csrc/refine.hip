@@ -1549,7 +1549,6 @@ struct ChainArrays {
   uint32_t* oe;
   uint32_t* next;
   uint32_t* prev;
-  uint32_t* choice;
   unsigned long long* claim;
   uint32_t* progress;  // progress[it] != 0: iteration it committed a link
   uint4* pk;           // per entry: gain (lo, hi), size, valid -- what a proposal scan reads
@@ -1590,6 +1589,7 @@ __global__ void k_chain_gather(ChainArrays c, const uint32_t* __restrict__ part,
                        moves[v] != NONE ? 1u : 0u);
   c.next[j] = NONE;
   c.prev[j] = NONE;
+  c.claim[j] = 0ull;
 }
 __global__ void k_chain_ranges(ChainArrays c, const uint64_t* __restrict__ poff, uint32_t n) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1612,12 +1612,25 @@ __device__ __forceinline__ int size_bucket(uint32_t s) { return s == 0 ? 0 : 31 
 // entry: nothing later can beat it.  An entry that cannot be chosen (invalid, the proposer
 // itself, already linked, or of a size bucket too far) never can again, so the next scan
 // starts at the first entry this one could choose.
-__global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || chain_stalled(c, it)) return;
+// Claims, in the same launch: the proposer with the smallest size difference, then the lowest
+// index, wins.  Proposals pile onto few targets (the heads of the partition lists) but the
+// lanes of a wave rarely share one, so the workgroup first merges its claims per target in an
+// LDS table and then issues one global atomic per distinct target.  Claims start at zero: the
+// buffer is cleared when it is filled, and the commit clears every claim it reads.
+constexpr uint32_t RESOLVE_BLOCK = 1024;
+constexpr uint32_t RESOLVE_SLOTS = 2048;  // > 2 x the workgroup's claims: the table never fills
+constexpr uint32_t PROPOSE_BATCH = 6;     // list entries per load round (60 VGPRs: two workgroups per CU)
+__global__ void __launch_bounds__(RESOLVE_BLOCK) k_chain_propose(ChainArrays c, uint32_t n, int it) {
+  if (chain_stalled(c, it)) return;  // uniform
+  __shared__ uint32_t s_tgt[RESOLVE_SLOTS];
+  __shared__ unsigned long long s_key[RESOLVE_SLOTS];
+  for (uint32_t t = threadIdx.x; t < RESOLVE_SLOTS; t += RESOLVE_BLOCK) {
+    s_tgt[t] = NONE;
+    s_key[t] = 0ull;
+  }
+  const uint32_t i = blockIdx.x * RESOLVE_BLOCK + threadIdx.x;
   uint32_t choice = NONE;
-  c.claim[i] = 0ull;
-  if (c.next[i] == NONE && c.ob[i] < c.oe[i]) {
+  if (i < n && c.next[i] == NONE && c.ob[i] < c.oe[i]) {
     const uint32_t si = c.sz[i];
     const int bi = size_bucket(si);
     const uint32_t lim = min(c.oe[i], c.ob[i] + 256u);
@@ -1626,20 +1639,20 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
     int64_t bw = 0;
     uint32_t bd = 0;
     bool done = false;
-    // 8 entries (one 128-byte line of records) are loaded before any is examined, so a scan
-    // waits on one memory round trip per 8 entries instead of one per entry
-    for (uint32_t jb = j0; jb < lim && !done; jb += 8) {
-      uint4 e[8];
-      uint32_t pv[8];
+    // PROPOSE_BATCH entries are loaded before any is examined, so a scan waits on one memory
+    // round trip per batch instead of one per entry
+    for (uint32_t jb = j0; jb < lim && !done; jb += PROPOSE_BATCH) {
+      uint4 e[PROPOSE_BATCH];
+      uint32_t pv[PROPOSE_BATCH];
 #pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) {  // unconditional loads (clamped index): no branch per load
+      for (uint32_t u = 0; u < PROPOSE_BATCH; ++u) {  // unconditional loads (clamped index): no branch per load
         const uint32_t jj = min(jb + u, lim - 1);
         e[u] = c.pk[jj];
         pv[u] = c.prev[jj];
         if (jb + u >= lim) e[u].w = 0;
       }
 #pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) {
+      for (uint32_t u = 0; u < PROPOSE_BATCH; ++u) {
         const uint32_t j = jb + u;
         if (!e[u].w) {  // invalid entry, or past the scan limit
           if (choice == NONE) first = min(j, lim);
@@ -1669,37 +1682,17 @@ __global__ void k_chain_propose(ChainArrays c, uint32_t n, int it) {
     }
     c.from[i] = first;
   }
-  c.choice[i] = choice;
-}
-// Claims: the proposer with the smallest size difference, then the lowest index, wins.
-// Proposals pile onto few targets (the heads of the partition lists) but the lanes of a wave
-// rarely share one, so a 1024-thread workgroup first merges its claims per target in an LDS
-// table and then issues one global atomic per distinct target.
-constexpr uint32_t RESOLVE_BLOCK = 1024;
-constexpr uint32_t RESOLVE_SLOTS = 2048;  // > 2 x the workgroup's claims: the table never fills
-__global__ void __launch_bounds__(RESOLVE_BLOCK) k_chain_resolve(ChainArrays c, uint32_t n, int it) {
-  if (chain_stalled(c, it)) return;
-  __shared__ uint32_t s_tgt[RESOLVE_SLOTS];
-  __shared__ unsigned long long s_key[RESOLVE_SLOTS];
-  for (uint32_t t = threadIdx.x; t < RESOLVE_SLOTS; t += RESOLVE_BLOCK) {
-    s_tgt[t] = NONE;
-    s_key[t] = 0ull;
-  }
-  __syncthreads();
-  const uint32_t i = blockIdx.x * RESOLVE_BLOCK + threadIdx.x;
-  if (i < n) {
-    const uint32_t s = c.choice[i];
-    if (s != NONE && c.prev[s] == NONE) {
-      const uint32_t d = c.sz[i] > c.sz[s] ? c.sz[i] - c.sz[s] : c.sz[s] - c.sz[i];
-      const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - d) << 32) | (0xFFFFFFFFu - i);
-      uint32_t h = slot_hash(s) & (RESOLVE_SLOTS - 1);
-      for (;;) {
-        const uint32_t old = atomicCAS(&s_tgt[h], NONE, s);
-        if (old == NONE || old == s) break;
-        h = (h + 1) & (RESOLVE_SLOTS - 1);
-      }
-      atomicMax(&s_key[h], key);
+  __syncthreads();  // the table is cleared
+  if (choice != NONE && c.prev[choice] == NONE) {
+    const uint32_t d = c.sz[i] > c.sz[choice] ? c.sz[i] - c.sz[choice] : c.sz[choice] - c.sz[i];
+    const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - d) << 32) | (0xFFFFFFFFu - i);
+    uint32_t h = slot_hash(choice) & (RESOLVE_SLOTS - 1);
+    for (;;) {
+      const uint32_t old = atomicCAS(&s_tgt[h], NONE, choice);
+      if (old == NONE || old == choice) break;
+      h = (h + 1) & (RESOLVE_SLOTS - 1);
     }
+    atomicMax(&s_key[h], key);
   }
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < RESOLVE_SLOTS; t += RESOLVE_BLOCK)
@@ -1709,7 +1702,9 @@ __global__ void k_chain_commit(ChainArrays c, uint32_t n, int it) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n || chain_stalled(c, it)) return;
   const unsigned long long k = c.claim[s];
-  if (k == 0ull || c.prev[s] != NONE) return;
+  if (k == 0ull) return;
+  c.claim[s] = 0ull;  // the next iteration's claims start at zero
+  if (c.prev[s] != NONE) return;
   const uint32_t p = 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFull);
   c.prev[s] = p;
   if (c.next[p] == NONE) c.next[p] = s;
@@ -2723,7 +2718,7 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   HGP_LAUNCH(k_chain_sortkeys2, g, 256, 0, c, part, moves, orig.get(), n, k2.get());
   radix_sort_buf(c, k2, orig, n, 0, bit_width(2ull * k + 1));
   k1.reset();
-  DBuf<uint32_t> src(c, n), dst(c, n), sz(c, n), ok(c, n), ob(c, n), oe(c, n), next(c, n), prev(c, n), choice(c, n);
+  DBuf<uint32_t> src(c, n), dst(c, n), sz(c, n), ok(c, n), ob(c, n), oe(c, n), next(c, n), prev(c, n);
   DBuf<int64_t> w(c, n);
   DBuf<unsigned long long> claim(c, n);
   DBuf<uint32_t> progress(c, 4);
@@ -2731,15 +2726,14 @@ void rank_by_chaining(Ctx& c, uint32_t n, uint32_t k, const uint32_t* part, cons
   DBuf<uint32_t> from(c, n);
   fill_u32(c, progress.get(), 0u, 4);
   ChainArrays ca{orig.get(), src.get(), dst.get(), sz.get(), w.get(), ok.get(), ob.get(), oe.get(),
-                 next.get(), prev.get(), choice.get(), claim.get(), progress.get(), pk.get(), from.get()};
+                 next.get(), prev.get(), claim.get(), progress.get(), pk.get(), from.get()};
   HGP_LAUNCH(k_chain_gather, g, 256, 0, c, ca, part, moves, gains, sizes, n);
   DBuf<uint64_t> poff(c, (uint64_t)k + 1);
   // src partitions of sorted space are the k2 keys >> 1
   segment_offsets(c, k2.get(), 1, n, k, poff.get());
   HGP_LAUNCH(k_chain_ranges, g, 256, 0, c, ca, poff.get(), n);
   for (int it = 0; it < 4; ++it) {
-    HGP_LAUNCH(k_chain_propose, g, 256, 0, c, ca, n, it);
-    HGP_LAUNCH(k_chain_resolve, ceil_div(n, RESOLVE_BLOCK), RESOLVE_BLOCK, 0, c, ca, n, it);
+    HGP_LAUNCH(k_chain_propose, ceil_div(n, RESOLVE_BLOCK), RESOLVE_BLOCK, 0, c, ca, n, it);
     HGP_LAUNCH(k_chain_commit, g, 256, 0, c, ca, n, it);
   }
   DBuf<uint32_t> comp(c, n), pos(c, n), cflag(c, n), cinv(c, n), csize(c, n), odd(c, 1);
