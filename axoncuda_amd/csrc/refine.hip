@@ -2086,8 +2086,14 @@ __global__ void __launch_bounds__(NT) k_cascade_sorted(CascadeArgs a, const uint
     }
     atomicAdd(&s_hc[h], 1u);
     s_rk[i] = q == NONE ? NONE : q4.z;
+    // event slots: one LDS atomic per wave (the lowest moving lane's), not one per moving pin
+    const uint64_t mv = __ballot(q != NONE);
+    uint32_t o0 = 0;
+    const uint32_t lead = mv ? (uint32_t)__ffsll((long long)mv) - 1u : 0u;
+    if (mv && l == lead) o0 = atomicAdd(&s_moving, (uint32_t)__popcll(mv));
+    o0 = (uint32_t)__builtin_amdgcn_readlane((int)o0, (int)lead);
     if (q != NONE) {
-      const uint32_t o = 2 * atomicAdd(&s_moving, 1u);
+      const uint32_t o = 2 * (o0 + (uint32_t)__popcll(mv & ((1ull << l) - 1ull)));
       const uint64_t r1 = (uint64_t)q4.z + 1;
       s_ev[o] = ((uint64_t)p << (rb + 11)) | (r1 << 11) | i;
       s_ev[o + 1] = ((uint64_t)q << (rb + 11)) | (r1 << 11) | (1u << 10) | i;
