@@ -1451,22 +1451,30 @@ __global__ void k_compact_u64(const uint32_t* __restrict__ flag, const uint64_t*
 }
 // the small-hedge CSR in first-pin order: sorted keys (first pin << 32 | hedge) -> lengths,
 // then pins and weights at the scanned offsets
+// The small-hedge CSR: hedge i (hedge keys[i], or i without keys) keeps its pins from a
+// multiple of 4, padded to a multiple of 4 with its last pin, so a thread loads them as 16-byte
+// vectors; its record is start << 5 | length (lengths 2..16).
+__device__ __forceinline__ uint32_t small_hedge_id(const uint64_t* keys, uint32_t i) {
+  return keys ? (uint32_t)keys[i] : i;
+}
 __global__ void k_small_lens(const uint64_t* __restrict__ keys, uint32_t m, const uint64_t* __restrict__ hoff,
                              uint64_t* __restrict__ len) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const uint32_t e = (uint32_t)keys[i];
-  len[i] = hoff[e + 1] - hoff[e];
+  const uint32_t e = small_hedge_id(keys, i);
+  len[i] = (hoff[e + 1] - hoff[e] + 3) & ~3ull;
 }
 __global__ void k_small_fill(const uint64_t* __restrict__ keys, uint32_t m, const uint64_t* __restrict__ hoff,
                              const uint32_t* __restrict__ hedges, const float* __restrict__ w,
-                             const uint64_t* __restrict__ off, uint32_t* __restrict__ pins, float* __restrict__ sw) {
+                             uint64_t* __restrict__ rec, uint32_t* __restrict__ pins, float* __restrict__ sw) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const uint32_t e = (uint32_t)keys[i];
-  const uint64_t s = hoff[e], len = hoff[e + 1] - s, o = off[i];
+  const uint32_t e = small_hedge_id(keys, i);
+  const uint64_t s = hoff[e], len = hoff[e + 1] - s, o = rec[i];  // rec holds the start until now
   for (uint64_t j = 0; j < len; ++j) pins[o + j] = hedges[s + j];
+  for (uint64_t j = len; j < ((len + 3) & ~3ull); ++j) pins[o + j] = hedges[s + len - 1];
   sw[i] = w[e];
+  rec[i] = (o << 5) | len;
 }
 __global__ void k_compact(const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint32_t n,
                           uint32_t* __restrict__ out) {
@@ -1861,7 +1869,7 @@ __global__ void k_cascade_classes(const uint32_t* __restrict__ hedges, const uin
 // once per level): offsets, pins and weights are read coalesced, and nearby threads gather the
 // records of nearby nodes.
 struct SmallHedges {
-  const uint64_t* off;
+  const uint64_t* rec;  // start << 5 | length; starts are multiples of 4
   const uint32_t* pins;
   const float* w;
   uint32_t m;
@@ -1876,7 +1884,10 @@ template <bool P16>
 __global__ void k_cascade_small(CascadeArgs a, SmallHedges h) {
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= h.m) return;
-  const uint64_t s = h.off[i], t = h.off[i + 1];
+  const uint64_t rec = h.rec[i];
+  const uint64_t s = rec >> 5;
+  const uint32_t len = (uint32_t)(rec & 31u);
+  const uint4* p4 = reinterpret_cast<const uint4*>(h.pins + s);  // 16-byte aligned: s % 4 == 0
   // per pin: partition (with P16 also the target), target, rank
   uint32_t pt[CAS_SMALL], mv[P16 ? 1 : CAS_SMALL], rk[CAS_SMALL];
   // a pin's partition, its target (NONE when it does not move) and whether it moves; entries
@@ -1899,14 +1910,9 @@ __global__ void k_cascade_small(CascadeArgs a, SmallHedges h) {
     if constexpr (!P16) mv[u] = NONE;
     rk[u] = 0;
   }
-  const uint32_t len = (uint32_t)(t - s);
-  // The pins and their records are fetched in batches of unconditional loads (indexes clamped
-  // to the last pin), so a hedge costs two memory round trips per batch instead of two per pin;
-  // entries past the end are marked as non-moving pins of no partition.
-  auto fetch = [&](uint32_t u) {
-    const uint32_t x = h.pins[s + min(u, len - 1)];
-    return x;
-  };
+  // The pins come as 16-byte vectors (the padding repeats the last pin) and their records in
+  // batches of unconditional loads, so a hedge costs two memory round trips per batch instead
+  // of two per pin; entries past the end are marked as non-moving pins of no partition.
   auto put = [&](uint32_t u, uint32_t x) {
     const bool in = u < len;
     if constexpr (P16) {
@@ -1920,17 +1926,19 @@ __global__ void k_cascade_small(CascadeArgs a, SmallHedges h) {
       rk[u] = q4.z;
     }
   };
+  static_assert(CAS_SMALL == 16, "four vectors of four pins");
+  const uint4 v0 = p4[0];
   {
-    uint32_t x[4];
-#pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) x[u] = fetch(u);
+    const uint32_t x[4] = {v0.x, v0.y, v0.z, v0.w};
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) put(u, x[u]);
   }
   if (len > 4) {
-    uint32_t x[CAS_SMALL - 4];
-#pragma unroll
-    for (uint32_t u = 4; u < CAS_SMALL; ++u) x[u - 4] = fetch(u);
+    // vectors past the hedge's padded pins are not loaded: their entries repeat pin 0
+    const uint4 v1 = p4[1];
+    const uint4 v2 = len > 8 ? p4[2] : make_uint4(v0.x, v0.x, v0.x, v0.x);
+    const uint4 v3 = len > 12 ? p4[3] : make_uint4(v0.x, v0.x, v0.x, v0.x);
+    const uint32_t x[CAS_SMALL - 4] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w, v3.x, v3.y, v3.z, v3.w};
 #pragma unroll
     for (uint32_t u = 4; u < CAS_SMALL; ++u) put(u, x[u - 4]);
   }
@@ -1971,10 +1979,9 @@ __global__ __launch_bounds__(256) void k_cascade_small_bytes(CascadeArgs a, Smal
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t b = 0;
   if (i0 < h.m) {
-    const uint64_t s = h.off[i0], t = h.off[i0 + 1];
-    const uint64_t len = t - s, rec = a.pm2 ? 8 : 16;
+    const uint64_t s = h.rec[i0] >> 5, len = h.rec[i0] & 31u, rec = a.pm2 ? 8 : 16;
     uint32_t moving = 0;
-    for (uint64_t i = s; i < t; ++i) moving += pm_at(a, h.pins[i]).y != NONE;
+    for (uint64_t i = s; i < s + len; ++i) moving += pm_at(a, h.pins[i]).y != NONE;
     b = 8 + (4 + rec) * len + (moving ? 4 + 8 * (uint64_t)moving : 0);
   }
   __shared__ uint64_t part[4];
@@ -2478,10 +2485,7 @@ struct LevelLayout {
   DBuf<uint32_t> cas_list[3];
   uint32_t cas_n[3] = {0, 0, 0};
   // hedges of 2..CAS_SMALL pins as their own CSR, in order of their first pin
-  DBuf<uint64_t> cs_off;
-  DBuf<uint32_t> cs_pins;
-  DBuf<float> cs_w;
-  uint32_t cas_small_n = 0;
+  SmallCsr cs;
 };
 
 // masks: only the partition masks (k-way with k <= MASK_K: nothing else reads the lists).
@@ -2863,29 +2867,23 @@ void analyze_round(Ctx& c, const RefineLevel& L, int mode, uint32_t omega, uint3
       // hedges of 2..CAS_SMALL pins as their own CSR, in order of their first pin
       exclusive_scan(c, fl[3].get(), pos.get(), L.e);
       const uint32_t m = (uint32_t)read_scalar(c, pos.get() + L.e);
-      Y.cas_small_n = m;
-      Y.cs_off = DBuf<uint64_t>(c, (uint64_t)m + 1);
+      Y.cs = SmallCsr();
       if (m) {
         DBuf<uint64_t> kc(c, m);
         HGP_LAUNCH(k_compact_u64, ceil_div(L.e, 256), 256, 0, c, fl[3].get(), pos.get(), L.e, ks.get(), kc.get());
         radix_sort(c, kc.get(), nullptr, m, 32 + std::max(1, bit_width(L.n)));
-        HGP_LAUNCH(k_small_lens, ceil_div(m, 256), 256, 0, c, kc.get(), m, L.hoff, Y.cs_off.get());
-        exclusive_scan(c, Y.cs_off.get(), Y.cs_off.get(), m);
-        const uint64_t sp = read_scalar(c, Y.cs_off.get() + m);
-        Y.cs_pins = DBuf<uint32_t>(c, sp);
-        Y.cs_w = DBuf<float>(c, m);
-        HGP_LAUNCH(k_small_fill, ceil_div(m, 256), 256, 0, c, kc.get(), m, L.hoff, L.hedges, L.w, Y.cs_off.get(),
-                   Y.cs_pins.get(), Y.cs_w.get());
+        Y.cs = small_csr(c, kc.get(), m, L.hoff, L.hedges, L.w);
       }
     }
     auto small_cascade = [&] {
-      if (!Y.cas_small_n) return;
-      const SmallHedges sh{Y.cs_off.get(), Y.cs_pins.get(), Y.cs_w.get(), Y.cas_small_n};
+      const uint32_t m = Y.cs.m;
+      if (!m) return;
+      const SmallHedges sh{Y.cs.rec.get(), Y.cs.pins.get(), Y.cs.w.get(), m};
       const uint32_t cb = c.tune.cascade_block;  // 256 unless the autotuner chose otherwise
-      if (a.pm2) HGP_LAUNCH(k_cascade_small<true>, ceil_div(Y.cas_small_n, cb), cb, 0, c, a, sh);
-      else HGP_LAUNCH(k_cascade_small<false>, ceil_div(Y.cas_small_n, cb), cb, 0, c, a, sh);
+      if (a.pm2) HGP_LAUNCH(k_cascade_small<true>, ceil_div(m, cb), cb, 0, c, a, sh);
+      else HGP_LAUNCH(k_cascade_small<false>, ceil_div(m, cb), cb, 0, c, a, sh);
       if (prof_match(c, "k_cascade_small") && c.prof_count && c.prof_dev)
-        HGP_LAUNCH(k_cascade_small_bytes, ceil_div(Y.cas_small_n, 256), 256, 0, c, a, sh, c.prof_dev);
+        HGP_LAUNCH(k_cascade_small_bytes, ceil_div(m, 256), 256, 0, c, a, sh, c.prof_dev);
     };
     // bit 2 of side_streams: the side streams start before the small-hedge cascade and run
     // beside it too (otherwise after it, beside the events only)
@@ -3090,10 +3088,26 @@ uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rou
   return read_scalar(c, applied.get());
 }
 
-float time_cascade_small(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint2* pm2,
-                         uint32_t m, uint32_t block, unsigned long long* seq) {
-  CascadeArgs a{hedges, hoff, w, nullptr, pm2, 0, seq};
-  const SmallHedges sh{hoff, hedges, w, m};
+SmallCsr small_csr(Ctx& c, const uint64_t* keys, uint32_t m, const uint64_t* hoff, const uint32_t* hedges,
+                   const float* w) {
+  SmallCsr r;
+  r.m = m;
+  if (!m) return r;
+  r.rec = DBuf<uint64_t>(c, (uint64_t)m + 1);
+  HGP_LAUNCH(k_small_lens, ceil_div(m, 256), 256, 0, c, keys, m, hoff, r.rec.get());
+  exclusive_scan(c, r.rec.get(), r.rec.get(), m);
+  const uint64_t sp = read_scalar(c, r.rec.get() + m);
+  r.pins = DBuf<uint32_t>(c, sp);
+  r.w = DBuf<float>(c, m);
+  HGP_LAUNCH(k_small_fill, ceil_div(m, 256), 256, 0, c, keys, m, hoff, hedges, w, r.rec.get(), r.pins.get(),
+             r.w.get());
+  return r;
+}
+
+float time_cascade_small(Ctx& c, const SmallCsr& cs, const uint2* pm2, uint32_t block, unsigned long long* seq) {
+  const uint32_t m = cs.m;
+  CascadeArgs a{nullptr, nullptr, nullptr, nullptr, pm2, 0, seq};
+  const SmallHedges sh{cs.rec.get(), cs.pins.get(), cs.w.get(), m};
   hipEvent_t t0, t1;
   HGP_HIP(hipEventCreate(&t0));
   HGP_HIP(hipEventCreate(&t1));

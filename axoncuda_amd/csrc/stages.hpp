@@ -73,11 +73,22 @@ uint32_t jacobi_round(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rou
 // Label-propagation refinement (k-way): up to `rounds` Jacobi rounds; a round that applies
 // nothing ends them, a round that raises lambda-1 is undone and ends them.
 void lp_refine(Ctx& c, const RefineLevel& L, uint32_t omega, uint32_t rounds, uint32_t* part, uint32_t* psizes);
-// Milliseconds of one thread-per-hedge cascade launch over m hedges of 2..16 pins each (a CSR
-// in processing order) with `block` threads per workgroup (pm2: per node partition | target
-// << 16 (0xFFFF: no move), rank; seq: one counter per rank).  For the autotuner.
-float time_cascade_small(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const float* w, const uint2* pm2,
-                         uint32_t m, uint32_t block, unsigned long long* seq);
+// Hedges of 2..16 pins as the thread-per-hedge cascade reads them: each hedge's pins start at a
+// multiple of 4 and are padded to one with its last pin; rec = start << 5 | length.
+struct SmallCsr {
+  DBuf<uint64_t> rec;
+  DBuf<uint32_t> pins;
+  DBuf<float> w;
+  uint32_t m = 0;
+};
+// The small-hedge CSR of hedges keys[i] (low 32 bits; i itself without keys), i < m, of the CSR
+// (hedges, hoff) with weights w.
+SmallCsr small_csr(Ctx& c, const uint64_t* keys, uint32_t m, const uint64_t* hoff, const uint32_t* hedges,
+                   const float* w);
+// Milliseconds of one thread-per-hedge cascade launch over a small-hedge CSR with `block`
+// threads per workgroup (pm2: per node partition | target << 16 (0xFFFF: no move), rank; seq:
+// one counter per rank).  For the autotuner.
+float time_cascade_small(Ctx& c, const SmallCsr& cs, const uint2* pm2, uint32_t block, unsigned long long* seq);
 // The autotuner (tune.hip): times the launch parameters of Ctx::tune on synthetic inputs of
 // up to 2^max_log2 keys / hedges and keeps the fastest of each in c.tune.
 void autotune(Ctx& c, uint32_t max_log2);

@@ -165,6 +165,7 @@ void tune_cascade(Ctx& c, uint32_t max_log2) {
   DBuf<uint32_t> ones(c, e);
   fill_u32(c, ones.get(), 0x3F800000u, e);  // 1.0f
   copy_d2d(c, w.get(), ones.get(), (uint64_t)e * 4);
+  const SmallCsr cs = small_csr(c, nullptr, e, off.get(), pins.get(), w.get());
   DBuf<uint2> pm(c, n);
   HGP_LAUNCH(k_tune_pm, ceil_div(n, 256), 256, 0, c, n, pm.get());
   DBuf<unsigned long long> seq(c, n);
@@ -172,7 +173,7 @@ void tune_cascade(Ctx& c, uint32_t max_log2) {
   float best = 1e30f;
   for (uint32_t b = 64; b <= 1024; b *= 2) {
     float ms = 1e30f;
-    for (int r = 0; r < 4; ++r) ms = std::min(ms, time_cascade_small(c, pins.get(), off.get(), w.get(), pm.get(), e, b, seq.get()));
+    for (int r = 0; r < 4; ++r) ms = std::min(ms, time_cascade_small(c, cs, pm.get(), b, seq.get()));
     if (c.verbose >= 2) fprintf(stderr, "  tune: cascade with %u threads per workgroup: %.3f ms\n", b, ms);
     if (ms < best) best = ms, best_b = b;
   }
