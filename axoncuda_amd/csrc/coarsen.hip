@@ -32,6 +32,7 @@ constexpr uint32_t CAND_LIGHT_M = 256;  // ... by the mid-table instance (512 sl
 constexpr uint32_t CAND_MED_CAP = 4096;   // LDS table slots of a medium node (48 KB)
 constexpr uint32_t CAND_MED_S = 2048;     // ... of the small-table medium instance
 constexpr uint32_t CAND_BIG_SLOTS = 16384;  // LDS hash slots of a big node (128 KB; half with inbound counts)
+constexpr uint32_t CAND_BIG8_SLOTS = 8192;  // LDS hash slots of the 8192-slot tier (64 KB)
 constexpr uint32_t CAND_DIRECT_N = 39168;   // levels up to this size index LDS by node id (<= 160 KB;
                                             // half that with inbound counts)
 constexpr uint64_t HEAVY_TABLE_SLOTS = 64ull << 20;  // HBM table slots shared by the heavy grid
@@ -469,29 +470,35 @@ __device__ __forceinline__ uint32_t hash_cap(uint64_t bound, uint32_t n) {
 // <= 4096 slots), 2 big (1024 threads, one LDS table: indexed by node id when the level has
 // at most direct_n nodes, else hashed with <= big_cap slots), 3 heavy (HBM tables).
 // cap[v] = table slots; a heavy node whose table would not be smaller than n indexes directly.
-__global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uint32_t direct_n, uint32_t big_cap,
-                             uint32_t* __restrict__ f_small, uint32_t* __restrict__ f_mid,
+__global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uint32_t direct_n, uint32_t big8_cap,
+                             uint32_t big_cap, uint32_t* __restrict__ f_small, uint32_t* __restrict__ f_mid,
                              uint32_t* __restrict__ f_light, uint32_t* __restrict__ f_med_s,
-                             uint32_t* __restrict__ f_med, uint32_t* __restrict__ f_big, uint32_t* __restrict__ cap) {
+                             uint32_t* __restrict__ f_med, uint32_t* __restrict__ f_big8, uint32_t* __restrict__ f_big,
+                             uint32_t* __restrict__ cap) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const uint64_t b = bound[v];
   const uint32_t c = hash_cap(b, n);
   const bool light = b <= CAND_LIGHT;
   const bool med = !light && c <= CAND_MED_CAP;
-  const bool big = !light && !med && (n <= direct_n || c <= big_cap);
+  const bool big8 = !light && !med && c <= big8_cap;
+  const bool big = !light && !med && !big8 && (n <= direct_n || c <= big_cap);
   f_small[v] = b <= CAND_LIGHT_S ? 1u : 0u;
   f_mid[v] = b > CAND_LIGHT_S && b <= CAND_LIGHT_M ? 1u : 0u;
   f_light[v] = light && b > CAND_LIGHT_M ? 1u : 0u;
   f_med_s[v] = med && c <= CAND_MED_S ? 1u : 0u;
   f_med[v] = med && c > CAND_MED_S ? 1u : 0u;
+  f_big8[v] = big8 ? 1u : 0u;
   f_big[v] = big ? 1u : 0u;
-  cap[v] = (big && n <= direct_n) || (!light && !med && !big && (uint64_t)n <= c) ? n : c;
+  // indexed by node id: the big tier on small levels, and heavy nodes whose table would not be
+  // smaller than n; every other tier hashes into a power-of-two table of c slots
+  const bool heavy = !light && !med && !big8 && !big;
+  cap[v] = (big && n <= direct_n) || (heavy && (uint64_t)n <= c) ? n : c;
 }
 
 // node lists of the tiers flagged in f[0..T-2] (exclusive scans p[0..T-2]); the rest go to
 // the last list
-constexpr int TIERS = 7;
+constexpr int TIERS = 8;
 struct TierLists {
   const uint32_t* f[TIERS - 1];
   const uint64_t* p[TIERS - 1];
@@ -675,6 +682,18 @@ __global__ void __launch_bounds__(1024) k_cand_big(CAND_ARGS) {
   const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
   cand_block_node<false, INTER, 16>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb,
                                     w, sizes, cnc, omega, delta, pairs, scores, s_w);
+}
+
+// One node per 1024-thread workgroup with a hashed LDS table of CAND_BIG8_SLOTS (64 KB, no
+// shared-inbound counts): two workgroups per CU where the big tier's table leaves room for one.
+__global__ void __launch_bounds__(1024) k_cand_big8(CAND_ARGS) {
+  __shared__ uint32_t s_keys[CAND_BIG8_SLOTS];
+  __shared__ uint32_t s_score[CAND_BIG8_SLOTS];
+  __shared__ uint32_t s_inter[1];
+  __shared__ uint64_t s_w[256];
+  const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
+  cand_block_node<false, false, 16>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
+                                    sizes, cnc, omega, delta, pairs, scores, s_w);
 }
 
 // Nodes whose worst-case table exceeds LDS: try the largest LDS table that fits (their
@@ -944,7 +963,7 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   const uint32_t big_cap = inter ? CAND_BIG_SLOTS / 2 : CAND_BIG_SLOTS;
   const uint32_t g = ceil_div(n, 256);
   DBuf<uint64_t> bound(c, n);
-  DBuf<uint32_t> fs(c, n), fmid(c, n), fl(c, n), fms(c, n), fm(c, n), fb(c, n), caps(c, n);
+  DBuf<uint32_t> fs(c, n), fmid(c, n), fl(c, n), fms(c, n), fm(c, n), fb8(c, n), fb(c, n), caps(c, n);
   {
     DBuf<uint32_t> heavy(c, n), nheavy(c, 1);
     memset_async(c, nheavy.get(), 0, 4);
@@ -952,10 +971,13 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     HGP_LAUNCH(k_visit_bound_heavy, std::min<uint32_t>(n, VISIT_HEAVY_GRID), 256, 0, c, hoff, tch, toff, heavy.get(),
                nheavy.get(), bound.get());
   }
-  HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, direct_n, big_cap, fs.get(), fmid.get(), fl.get(), fms.get(),
-             fm.get(), fb.get(), caps.get());
-  // tier lists: small, mid and light tables, small and large medium tables, big or direct, heavy
-  const uint32_t* flags[TIERS - 1] = {fs.get(), fmid.get(), fl.get(), fms.get(), fm.get(), fb.get()};
+  // the 8192-slot tier without shared-inbound counts (with them the big table already has 8192 slots)
+  const uint32_t big8_cap = inter ? 0u : CAND_BIG8_SLOTS;
+  HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, direct_n, big8_cap, big_cap, fs.get(), fmid.get(), fl.get(),
+             fms.get(), fm.get(), fb8.get(), fb.get(), caps.get());
+  // tier lists: small, mid and light tables, small and large medium tables, 8192-slot tables, big or
+  // direct, heavy
+  const uint32_t* flags[TIERS - 1] = {fs.get(), fmid.get(), fl.get(), fms.get(), fm.get(), fb8.get(), fb.get()};
   DBuf<uint64_t> pos[TIERS - 1];
   uint32_t cnt[TIERS];
   uint32_t rest = n;
@@ -979,9 +1001,10 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     }
   }
   HGP_LAUNCH(k_split_tiers, g, 256, 0, c, tl, n);
-  const uint32_t ns = cnt[0], nmid = cnt[1], nl = cnt[2], nms = cnt[3], nm = cnt[4], nb = cnt[5], nh = cnt[6];
+  const uint32_t ns = cnt[0], nmid = cnt[1], nl = cnt[2], nms = cnt[3], nm = cnt[4], nb8 = cnt[5], nb = cnt[6],
+                 nh = cnt[7];
   DBuf<uint32_t>&small = lists[0], &mid = lists[1], &light = lists[2], &med_s = lists[3], &med = lists[4],
-                 &big = lists[5], &heavy = lists[6];
+                 &big8 = lists[5], &big = lists[6], &heavy = lists[7];
   // the tiers write disjoint rows of pairs and scores: light and medium nodes on side streams
   // beside the big and heavy ones (which allocate, so they stay on the main stream)
   Fork fk(c, c.tune.side_streams & 1u);
@@ -1015,6 +1038,7 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     if (inter) CAND_LAUNCH(k_cand_medium<true>, med.get(), nm, 256);
     else CAND_LAUNCH(k_cand_medium<false>, med.get(), nm, 256);
   });
+  if (nb8) CAND_LAUNCH(k_cand_big8, big8.get(), nb8, 1024);
   if (n <= direct_n) {
     if (inter) CAND_LAUNCH(k_cand_direct<true>, big.get(), nb, 1024);
     else CAND_LAUNCH(k_cand_direct<false>, big.get(), nb, 1024);
