@@ -660,15 +660,16 @@ __device__ __forceinline__ bool cand_block_node(uint32_t u, uint32_t cap, bool d
 
 // CAP: the LDS table (CAND_MED_CAP, or CAND_MED_S for nodes whose table fits it: half the LDS,
 // so twice the workgroups per CU)
-template <bool INTER, uint32_t CAP = CAND_MED_CAP>
-__global__ void __launch_bounds__(256) k_cand_medium(CAND_ARGS) {
+// NW waves per node: 8 for the larger table, whose LDS allows only four workgroups per CU
+template <bool INTER, uint32_t CAP = CAND_MED_CAP, int NW = 4>
+__global__ void __launch_bounds__(64 * NW) k_cand_medium(CAND_ARGS) {
   __shared__ uint32_t s_keys[CAP];
   __shared__ uint32_t s_score[CAP];
   __shared__ uint32_t s_inter[INTER ? CAP : 1];
   __shared__ uint64_t s_w[SELECT_WORDS];
   const uint32_t u = list[xcd_block(blockIdx.x, gridDim.x)];
-  cand_block_node<false, INTER, 4>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
-                                   sizes, cnc, omega, delta, pairs, scores, s_w);
+  cand_block_node<false, INTER, NW>(u, caps[u], false, s_keys, s_score, s_inter, hedges, hoff, srcc, tch, toff, inb, w,
+                                    sizes, cnc, omega, delta, pairs, scores, s_w);
 }
 
 // One node per 1024-thread workgroup with a hashed LDS table of up to CAND_BIG_SLOTS.
@@ -1035,8 +1036,8 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   fk.on(1, [&] {
     if (inter) CAND_LAUNCH((k_cand_medium<true, CAND_MED_S>), med_s.get(), nms, 256);
     else CAND_LAUNCH((k_cand_medium<false, CAND_MED_S>), med_s.get(), nms, 256);
-    if (inter) CAND_LAUNCH(k_cand_medium<true>, med.get(), nm, 256);
-    else CAND_LAUNCH(k_cand_medium<false>, med.get(), nm, 256);
+    if (inter) CAND_LAUNCH((k_cand_medium<true, CAND_MED_CAP, 8>), med.get(), nm, 512);
+    else CAND_LAUNCH((k_cand_medium<false, CAND_MED_CAP, 8>), med.get(), nm, 512);
   });
   if (nb8) CAND_LAUNCH(k_cand_big8, big8.get(), nb8, 1024);
   if (n <= direct_n) {
