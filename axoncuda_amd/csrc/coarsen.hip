@@ -474,7 +474,7 @@ __global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uin
                              uint32_t big_cap, uint32_t* __restrict__ f_small, uint32_t* __restrict__ f_mid,
                              uint32_t* __restrict__ f_light, uint32_t* __restrict__ f_med_s,
                              uint32_t* __restrict__ f_med, uint32_t* __restrict__ f_big8, uint32_t* __restrict__ f_big,
-                             uint32_t* __restrict__ cap) {
+                             uint32_t* __restrict__ f_direct, uint32_t* __restrict__ cap) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const uint64_t b = bound[v];
@@ -482,7 +482,8 @@ __global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uin
   const bool light = b <= CAND_LIGHT;
   const bool med = !light && c <= CAND_MED_CAP;
   const bool big8 = !light && !med && c <= big8_cap;
-  const bool big = !light && !med && !big8 && (n <= direct_n || c <= big_cap);
+  const bool big = !light && !med && !big8 && c <= big_cap;                // hashed, on any level
+  const bool direct = !light && !med && !big8 && !big && n <= direct_n;  // indexed by node id
   f_small[v] = b <= CAND_LIGHT_S ? 1u : 0u;
   f_mid[v] = b > CAND_LIGHT_S && b <= CAND_LIGHT_M ? 1u : 0u;
   f_light[v] = light && b > CAND_LIGHT_M ? 1u : 0u;
@@ -490,15 +491,16 @@ __global__ void k_cand_tiers(const uint64_t* __restrict__ bound, uint32_t n, uin
   f_med[v] = med && c > CAND_MED_S ? 1u : 0u;
   f_big8[v] = big8 ? 1u : 0u;
   f_big[v] = big ? 1u : 0u;
-  // indexed by node id: the big tier on small levels, and heavy nodes whose table would not be
-  // smaller than n; every other tier hashes into a power-of-two table of c slots
-  const bool heavy = !light && !med && !big8 && !big;
-  cap[v] = (big && n <= direct_n) || (heavy && (uint64_t)n <= c) ? n : c;
+  f_direct[v] = direct ? 1u : 0u;
+  // indexed by node id: the direct tier, and heavy nodes whose table would not be smaller than n;
+  // every other tier hashes into a power-of-two table of c slots
+  const bool heavy = !light && !med && !big8 && !big && !direct;
+  cap[v] = direct || (heavy && (uint64_t)n <= c) ? n : c;
 }
 
 // node lists of the tiers flagged in f[0..T-2] (exclusive scans p[0..T-2]); the rest go to
 // the last list
-constexpr int TIERS = 8;
+constexpr int TIERS = 9;
 struct TierLists {
   const uint32_t* f[TIERS - 1];
   const uint64_t* p[TIERS - 1];
@@ -964,7 +966,7 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   const uint32_t big_cap = inter ? CAND_BIG_SLOTS / 2 : CAND_BIG_SLOTS;
   const uint32_t g = ceil_div(n, 256);
   DBuf<uint64_t> bound(c, n);
-  DBuf<uint32_t> fs(c, n), fmid(c, n), fl(c, n), fms(c, n), fm(c, n), fb8(c, n), fb(c, n), caps(c, n);
+  DBuf<uint32_t> fs(c, n), fmid(c, n), fl(c, n), fms(c, n), fm(c, n), fb8(c, n), fb(c, n), fd(c, n), caps(c, n);
   {
     DBuf<uint32_t> heavy(c, n), nheavy(c, 1);
     memset_async(c, nheavy.get(), 0, 4);
@@ -975,10 +977,11 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   // the 8192-slot tier without shared-inbound counts (with them the big table already has 8192 slots)
   const uint32_t big8_cap = inter ? 0u : CAND_BIG8_SLOTS;
   HGP_LAUNCH(k_cand_tiers, g, 256, 0, c, bound.get(), n, direct_n, big8_cap, big_cap, fs.get(), fmid.get(), fl.get(),
-             fms.get(), fm.get(), fb8.get(), fb.get(), caps.get());
+             fms.get(), fm.get(), fb8.get(), fb.get(), fd.get(), caps.get());
   // tier lists: small, mid and light tables, small and large medium tables, 8192-slot tables, big or
   // direct, heavy
-  const uint32_t* flags[TIERS - 1] = {fs.get(), fmid.get(), fl.get(), fms.get(), fm.get(), fb8.get(), fb.get()};
+  const uint32_t* flags[TIERS - 1] = {fs.get(),  fmid.get(), fl.get(), fms.get(),
+                                      fm.get(),  fb8.get(),  fb.get(), fd.get()};
   DBuf<uint64_t> pos[TIERS - 1];
   uint32_t cnt[TIERS];
   uint32_t rest = n;
@@ -1003,9 +1006,9 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
   }
   HGP_LAUNCH(k_split_tiers, g, 256, 0, c, tl, n);
   const uint32_t ns = cnt[0], nmid = cnt[1], nl = cnt[2], nms = cnt[3], nm = cnt[4], nb8 = cnt[5], nb = cnt[6],
-                 nh = cnt[7];
+                 nd = cnt[7], nh = cnt[8];
   DBuf<uint32_t>&small = lists[0], &mid = lists[1], &light = lists[2], &med_s = lists[3], &med = lists[4],
-                 &big8 = lists[5], &big = lists[6], &heavy = lists[7];
+                 &big8 = lists[5], &big = lists[6], &direct = lists[7], &heavy = lists[8];
   // the tiers write disjoint rows of pairs and scores: light and medium nodes on side streams
   // beside the big and heavy ones (which allocate, so they stay on the main stream)
   Fork fk(c, c.tune.side_streams & 1u);
@@ -1040,12 +1043,13 @@ void candidates(Ctx& c, const uint32_t* hedges, const uint64_t* hoff, const uint
     else CAND_LAUNCH((k_cand_medium<false, CAND_MED_CAP, 8>), med.get(), nm, 512);
   });
   if (nb8) CAND_LAUNCH(k_cand_big8, big8.get(), nb8, 1024);
-  if (n <= direct_n) {
-    if (inter) CAND_LAUNCH(k_cand_direct<true>, big.get(), nb, 1024);
-    else CAND_LAUNCH(k_cand_direct<false>, big.get(), nb, 1024);
-  } else {
+  if (nb) {
     if (inter) CAND_LAUNCH(k_cand_big<true>, big.get(), nb, 1024);
     else CAND_LAUNCH(k_cand_big<false>, big.get(), nb, 1024);
+  }
+  if (nd) {  // only on levels of at most direct_n nodes
+    if (inter) CAND_LAUNCH(k_cand_direct<true>, direct.get(), nd, 1024);
+    else CAND_LAUNCH(k_cand_direct<false>, direct.get(), nd, 1024);
   }
 #undef CAND_LAUNCH
   if (nh == 0) return;
